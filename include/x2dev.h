@@ -1,0 +1,188 @@
+/* x2dev.h — C-ABI of the MI355X (gfx950) path-tracing device for XPU2's shared integrator.
+ *
+ * This header is the drop-in boundary: the entry points below replace the reference's GPU lane
+ * (the C++ class `GpuLane`, xpu2/interactive/async_gpu.h:82) that the broker (`Renderer::render`,
+ * xpu2/interactive/async_render.h:318) drives from its feeder thread. Every entry point returns a status code
+ * instead of throwing; no C++ exception and no torch type crosses it. The caller owns every array it passes in;
+ * arrays are copied before the call returns. The device owns its film and the host copies of it.
+ *
+ * Record layouts (x2_chunk, the film) are bit-identical to the reference's so a broker written against the
+ * reference's types can hand them over unchanged.
+ */
+#ifndef X2DEV_H
+#define X2DEV_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (negative: failure; x2dev_last_error() says what failed). */
+#define X2_OK 0
+#define X2_ERR_ARGUMENT (-1)   /* a null pointer, a size out of range, a malformed record */
+#define X2_ERR_HIP (-2)        /* the HIP runtime reported an error */
+#define X2_ERR_STATE (-3)      /* a call out of order (launch before begin, download with launches in flight ...) */
+#define X2_ERR_BACKPRESSURE (-4) /* launch() while not ready(), or a launch larger than the mailbox ring allows */
+#define X2_ERR_MEMORY (-5)     /* device or pinned allocation failed */
+
+/* The unit of work: one 8x8 tile of the image and a range of sample indices (async_integrator.h:3057 Chunk). */
+typedef struct x2_chunk {
+    uint32_t tile;  /* tile index, row-major over the image's 8x8 tiles */
+    uint32_t first; /* first sample index */
+    uint32_t count; /* number of sample indices */
+    uint32_t cell;  /* owning cell of the work grid, for per-cell accounting */
+} x2_chunk;
+
+/* A sample handed back to the host (async_gpu_abi.h:24 Parked): re-rendered elsewhere from its identity. */
+typedef struct x2_parked {
+    uint32_t pixel, index, tries, pass;
+} x2_parked;
+
+/* Light types and sidedness (light_core.h:19-20). Only the environment and rectangle are implemented. */
+#define X2_LIGHT_ENV 0u
+#define X2_LIGHT_RECT 1u
+#define X2_SIDE_REGULAR 0u
+#define X2_SIDE_REVERSE 1u
+#define X2_SIDE_TWO 2u
+
+/* One light in render space, the subset of `lights::Light` (light_core.h:59-117) that a rectangle or a constant
+ * environment reads. Linear parts are rows (a local vector v maps to v.x*row0 + v.y*row1 + v.z*row2). */
+typedef struct x2_light {
+    uint32_t type, sidedness, hemispherical, visibility_mask;
+    float radiance[3];
+    float inv_area;            /* rect: 1 / render-space area; env: 1 / (4 pi) (or 1 / (2 pi) hemispherical) */
+    float position[3];
+    float plane_offset;        /* -dot(direction, position) */
+    float direction[3];        /* local +Z, unit: the side a one-sided light lights */
+    float scale;               /* local to render length scale */
+    float local_to_render[9];  /* rows, with scale */
+    float half_width;          /* local units */
+    float render_to_local[9];  /* rows, its inverse */
+    float half_height;         /* local units */
+    float render_to_local_rot[9]; /* rotation only */
+    float pad;
+} x2_light;
+
+/* A material: a Lambert lobe and a GGX lobe (no Fresnel, no energy compensation), plus self-emission.
+ * Lambert albedo = albedo * diffuse_weight; GGX lobe scale = specular_weight (white). A weight of zero omits the
+ * lobe. Lobes are recorded in that order (Lambert first), which fixes their branch addresses. */
+typedef struct x2_material {
+    float albedo[3];
+    float roughness;       /* MoonRay's input roughness; the lobe uses max(0.001, r^2) (shade_lobes.h:536) */
+    float diffuse_weight;
+    float specular_weight;
+    float emission[3];
+    uint32_t albedo_map;   /* reserved (0: none) */
+} x2_material;
+
+/* A triangle scene. Triangles are ordered mesh by mesh; mesh g owns triangles [geom_first[g], geom_first[g+1]),
+ * and a triangle's primitive id is its index less geom_first of its mesh. Meshes are two-sided. */
+typedef struct x2_scene {
+    const float* positions;        /* vertex_count x 3 */
+    const uint32_t* indices;       /* triangle_count x 3 */
+    const uint32_t* geom_first;    /* geom_count + 1 */
+    const uint32_t* geom_material; /* geom_count */
+    const x2_material* materials;  /* material_count */
+    const x2_light* lights;        /* light_count */
+    uint32_t vertex_count, triangle_count, geom_count, material_count, light_count, pad;
+} x2_scene;
+
+/* The integrator's settings: the used subset of `mixed::Config` (gpu_jobs.h:10-44), plus rr_min_depth (an
+ * extension: Russian roulette applies from that non-mirror depth on; 1 is the reference's behaviour). */
+typedef struct x2_config {
+    uint32_t width, height, spp, seed;
+    uint32_t sampling;          /* sampling policy bits (sample_stream.h:58-144): 1 camera, 2 first BSDF, 4 first light */
+    uint32_t filter;            /* 0: box (the only filter implemented) */
+    float filter_width;
+    float near_clip, far_clip;
+    float raster_to_camera[16]; /* m[row][col], row-vector convention */
+    float camera_to_render[16];
+    uint32_t light_count;
+    uint32_t bsdf_samples, light_samples;
+    uint32_t max_depth, max_diffuse_depth, max_glossy_depth, max_mirror_depth;
+    uint32_t clamp_depth;
+    float clamp_value;
+    float rr_threshold;
+    uint32_t rr_min_depth;
+    uint32_t shadow_reuse;      /* bounce-ray reuse: ray visibility bits whose bounces stand for shadow rays */
+    int32_t viewport[4];        /* min x, min y, max x, max y, half open */
+    uint32_t pad[2];
+} x2_config;
+
+/* What the host learns when a launch has finished (async_gpu.h:65 Completion), plus traversal counters. */
+typedef struct x2_completion {
+    uint64_t chunks, items, samples;
+    uint32_t errors, pad;
+    double device_seconds;     /* the launch's own duration, from HIP events around it */
+    uint64_t rays_trace;       /* closest-hit rays traced */
+    uint64_t rays_shadow;      /* any-hit shadow rays traced */
+    uint64_t nodes_visited;    /* BVH8 nodes fetched (256 B each) */
+    uint64_t triangles_tested; /* triangle records fetched (48 B each) */
+} x2_completion;
+
+typedef struct x2dev_options {
+    int32_t device;                 /* HIP device ordinal */
+    uint32_t max_chunks_per_launch; /* 16384 in the reference (async_gpu.h:43) */
+    uint32_t ring_chunks;           /* mailbox ring capacity, in chunks (2^17 in the reference) */
+    uint32_t counters;              /* nonzero: count rays, nodes and triangles per launch */
+} x2dev_options;
+
+typedef struct x2dev x2dev;
+
+/* Replaces GpuLane(ptx, ..., GpuOptions) (async_gpu.h:95-175): stream, mailbox ring, pinned staging, events. */
+int x2dev_create(const x2dev_options* options, x2dev** out);
+/* Replaces GpuLane::prepare(const cpu::PlanView&) (async_gpu.h:677): builds the BVH8 on the host and uploads the
+ * scene. */
+int x2dev_prepare(x2dev* d, const x2_scene* scene);
+/* Replaces GpuLane::begin(samplesPerChunk, adaptive, partial) (async_gpu.h:1372-1422): zeroes film and state;
+ * items per chunk = samples_per_chunk * 64. */
+int x2dev_begin(x2dev* d, const x2_config* config, uint32_t samples_per_chunk);
+/* Replaces GpuLane::ready() (async_gpu.h:1425): 1 when another launch may be queued (at most two in flight). */
+int x2dev_ready(const x2dev* d);
+/* Replaces GpuLane::launch(chunks, count, ..., samples) (async_gpu.h:1434-1500): copies the chunks into the ring and
+ * queues the render. Returns 1 when queued, 0 when there was nothing to run, negative on failure. */
+int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samples);
+/* Replaces GpuLane::poll(Completion&) (async_gpu.h:1503-1564): never waits. 1: the oldest launch finished and
+ * *out holds it; 0: nothing finished; negative on failure. */
+int x2dev_poll(x2dev* d, x2_completion* out);
+/* Replaces GpuLane::takeDeferred (async_gpu.h:297-300): samples handed to the CPU lane (none in this device). */
+int x2dev_take_deferred(x2dev* d, x2_parked* out, uint32_t capacity, uint32_t* taken);
+/* Replaces GpuLane::downloadFilm() (async_gpu.h:1567-1581): queues the film's copy to pinned host memory. */
+int x2dev_download_film(x2dev* d);
+/* Replaces GpuLane::filmReady() (async_gpu.h:1583-1590): 1 once the download has landed. */
+int x2dev_film_ready(x2dev* d);
+/* Replaces GpuLane::sums()/counts() (async_gpu.h:1705-1707): sums[pixel*4 + c] (rgb, alpha) in FP64 and the
+ * per-pixel sample counts, valid until the next download. */
+int x2dev_film(const x2dev* d, const double** sums, const uint32_t** counts);
+/* Replaces GpuLane::inFlight(), maxChunksPerLaunch() and launchThreads() (async_gpu.h:1426-1428,1725-1731). */
+uint32_t x2dev_in_flight(const x2dev* d);
+uint32_t x2dev_max_chunks_per_launch(const x2dev* d);
+uint32_t x2dev_launch_threads(const x2dev* d);
+/* Device pointers of the film (sums, counts) for callers that keep the film on the card (torch interop). */
+int x2dev_device_film(const x2dev* d, void** sums, void** counts);
+/* Frees everything the device holds. */
+void x2dev_destroy(x2dev* d);
+/* The last failure's text on the calling thread ("" when none). */
+const char* x2dev_last_error(void);
+
+/* Algorithmic byte sizes the roofline counts: one BVH8 node and one triangle record. */
+#define X2_NODE_BYTES 256u
+#define X2_TRIANGLE_BYTES 48u
+#define X2_FILM_BYTES_PER_PATH 72u
+
+/* The broker counterpart (async_render.h:318 Renderer::render, GPU lane only): one uniform frame, chunks made as
+ * the work grid makes them (async_grid.h:129-175, progressive ranges 1,1,2,spc over 64-px cells), launches sized
+ * to the measured rate with at most two in flight, completions polled, the film downloaded and every pixel's count
+ * audited against spp (async_render.h:1035). stats (may be null) receives totals over the frame's launches. */
+typedef struct x2_frame_stats {
+    uint64_t launches, chunks, samples;
+    uint32_t errors, audit_failures;
+    double device_seconds, wall_seconds;
+    uint64_t rays_trace, rays_shadow, nodes_visited, triangles_tested;
+} x2_frame_stats;
+int x2dev_render_frame(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, x2_frame_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,111 @@
+"""TEST INFRASTRUCTURE: ctypes loader for the CPU oracle (liboracle.so, built from x2_oracle.cpp by the Makefile
+beside it). Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline import this module."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+P = ctypes.POINTER
+u32, u64, f32, f64, c_int = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_double, ctypes.c_int
+vp = ctypes.c_void_p
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle not built: {LIB_PATH} (make -C oracle)")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "oracle_philox": (None, [P(u32), P(u32), P(u32)]),
+        "oracle_block": (None, [u32, u64, u64, u64, u32, P(u32)]),
+        "oracle_word": (u32, [u32, u64, u64, u64, u32]),
+        "oracle_unit": (f32, [u32]),
+        "oracle_sobol": (u32, [u32, u32]),
+        "oracle_directions": (None, [P(u32)]),
+        "oracle_owen_scramble": (u32, [u32, u32]),
+        "oracle_sampling_word": (u32, [u32, u64, u64, u64, u32, u32]),
+        "oracle_child_branch": (u64, [u64, u32, u32]),
+        "oracle_stream": (None, [u32, u64, u64, u64, u32, u32, P(f32)]),
+        "oracle_sincos": (None, [f32, P(f32), P(f32)]),
+        "oracle_lobe_eval": (None, [u32, P(f32), P(f32), P(f32), P(f32), f32, P(f32), P(f32)]),
+        "oracle_lobe_sample": (None, [u32, P(f32), P(f32), P(f32), f32, f32, f32, P(f32), P(f32), P(f32)]),
+        "oracle_light_sample": (c_int, [vp, P(f32), P(f32), P(f32), P(f32), P(f32), P(f32), P(f32)]),
+        "oracle_light_intersect": (c_int, [vp, P(f32), P(f32), P(f32), P(f32), P(f32)]),
+        "oracle_light_can_illuminate": (c_int, [vp, P(f32), P(f32)]),
+        "oracle_triangle": (c_int, [P(f32), P(f32), f32, f32, P(f32), P(f32), P(f32), P(f32)]),
+        "oracle_camera_ray": (c_int, [vp, f32, f32, P(f32), P(f32), P(f32), P(f32)]),
+        "oracle_geometry_hit": (c_int, [P(f32), P(f32), P(f32), P(f32), P(f32), P(f32)]),
+        "oracle_scene_create": (vp, [vp]),
+        "oracle_scene_destroy": (None, [vp]),
+        "oracle_closest": (None, [vp, P(f32), P(f32), f32, f32, c_int, P(u32), P(u32), P(f32)]),
+        "oracle_occluded": (c_int, [vp, P(f32), P(f32), f32, f32, c_int]),
+        "oracle_sample": (c_int, [vp, vp, u32, u32, P(f64), P(u64)]),
+        "oracle_render": (c_int, [vp, vp, u32, P(f64), P(u32), P(u64)]),
+        "oracle_render_chunks": (c_int, [vp, vp, vp, u32, u32, P(f64), P(u32), P(u64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def fa(values):
+    """A float32 ctypes array from a sequence (or None)."""
+    if values is None:
+        return None
+    a = (f32 * len(values))(*[float(v) for v in values])
+    return a
+
+
+class OracleScene:
+    """The oracle's copy of a scene (with its own BVH)."""
+
+    def __init__(self, scene_data):
+        self.lib = load()
+        self.h = self.lib.oracle_scene_create(ctypes.addressof(scene_data.c))
+        if not self.h:
+            raise RuntimeError("oracle_scene_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_scene_destroy(self.h)
+            self.h = None
+
+    def render(self, config, threads=None, chunks=None):
+        """The film (sums [pixels,4] f64, counts [pixels] u32) and counters (traces, shadows, errors)."""
+        n = config.width * config.height
+        sums = np.zeros((n, 4), np.float64)
+        counts = np.zeros(n, np.uint32)
+        counters = (u64 * 3)()
+        threads = threads or os.cpu_count() or 1
+        if chunks is None:
+            self.lib.oracle_render(self.h, ctypes.addressof(config), threads,
+                                   sums.ctypes.data_as(P(f64)), counts.ctypes.data_as(P(u32)), counters)
+        else:
+            self.lib.oracle_render_chunks(self.h, ctypes.addressof(config), ctypes.addressof(chunks), len(chunks),
+                                          threads, sums.ctypes.data_as(P(f64)), counts.ctypes.data_as(P(u32)),
+                                          counters)
+        return sums, counts, tuple(counters)
+
+    def sample(self, config, pixel, index):
+        rgba = (f64 * 4)()
+        counters = (u64 * 2)()
+        st = self.lib.oracle_sample(self.h, ctypes.addressof(config), pixel, index, rgba, counters)
+        return st, tuple(rgba), tuple(counters)
+
+    def closest(self, o, d, near=0.0, far=3.4e38, brute=False):
+        g, p, t = u32(), u32(), f32()
+        self.lib.oracle_closest(self.h, fa(o), fa(d), near, far, 1 if brute else 0, ctypes.byref(g),
+                                ctypes.byref(p), ctypes.byref(t))
+        return g.value, p.value, t.value
+
+    def occluded(self, o, d, near, far, brute=False):
+        return bool(self.lib.oracle_occluded(self.h, fa(o), fa(d), near, far, 1 if brute else 0))

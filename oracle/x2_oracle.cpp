@@ -1,0 +1,1362 @@
+// x2_oracle.cpp — TEST INFRASTRUCTURE ONLY (the oracle). A CPU restatement of the reference's shared path-tracing
+// integrator subset that the HIP device implements: the sample-identity RNG, the pinhole camera with a box filter,
+// the watertight triangle test with FP64 hit reconstruction, rectangle and constant-environment lights, Lambert and
+// GGX lobes, and MoonRay's multi-sampler estimator with its pending-ray stack, Russian roulette, MIS and bounce-ray
+// reuse. Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline load it; the product never does.
+//
+// Built with -O3 -ffp-contract=off -fno-fast-math (the reference's CPU parity flags, native_build.py:40), so every
+// float expression rounds as written. Each function cites the reference file:line it restates; citations are into
+// the reference tree (xpu2/interactive/ unless a path says otherwise).
+#include "x2_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+using U32 = uint32_t;
+using U64 = uint64_t;
+
+// ---------------------------------------------------------------------------------------------------------------
+// RNG: sample_stream.h
+struct Words { U32 v[4]; };
+// Philox4x32-10 (sample_stream.h:31-38, Random123's constants).
+Words philox(Words c, U32 ka, U32 kb) {
+    for (int round = 0; round < 10; ++round) {
+        const U64 a = U64(c.v[0]) * 0xd2511f53u, b = U64(c.v[2]) * 0xcd9e8d57u;
+        Words n;
+        n.v[0] = U32(b >> 32) ^ c.v[1] ^ ka;
+        n.v[1] = U32(b);
+        n.v[2] = U32(a >> 32) ^ c.v[3] ^ kb;
+        n.v[3] = U32(a);
+        c = n;
+        ka += 0x9e3779b9u;
+        kb += 0xbb67ae85u;
+    }
+    return c;
+}
+struct Identity { U32 pixel; U64 index, stream, branch; };
+// key (sample_stream.h:43-48) and block (:49-51).
+Words block(const Identity& id, U32 group) {
+    const U64 k = id.stream ^ (id.branch * 0x9e3779b97f4a7c15ull);
+    Words c{{id.pixel, U32(id.index), U32(id.index >> 32), group}};
+    return philox(c, U32(k), U32(k >> 32));
+}
+U32 word(const Identity& id, U32 dimension) { return block(id, dimension >> 2).v[dimension & 3]; }  // :52-54
+float unit(U32 bits) { return float(bits >> 8) * 0x1p-24f; }                                      // :56
+
+// Sobol direction numbers (sample_stream.h:82-100), regenerated here from the Joe-Kuo primitive polynomials
+// (S. Joe and F. Y. Kuo, new-joe-kuo-6.21201: degree s, coefficient bits a, initial m_1..m_s) by the standard
+// recurrence; row 0 is the van der Corput sequence, rows 1..16 are Joe-Kuo dimensions 2..17.
+struct JoeKuo { U32 s, a, m[6]; };
+const JoeKuo kJoeKuo[16] = {
+    {1, 0, {1}},           {2, 1, {1, 3}},          {3, 1, {1, 3, 1}},        {3, 2, {1, 1, 1}},
+    {4, 1, {1, 1, 3, 3}},  {4, 4, {1, 3, 5, 13}},   {5, 2, {1, 1, 5, 5, 17}}, {5, 4, {1, 1, 5, 5, 5}},
+    {5, 7, {1, 1, 7, 11, 19}}, {5, 11, {1, 1, 5, 1, 1}}, {5, 13, {1, 1, 1, 3, 11}}, {5, 14, {1, 3, 5, 5, 31}},
+    {6, 1, {1, 3, 3, 9, 7, 49}}, {6, 13, {1, 1, 1, 15, 21, 21}}, {6, 16, {1, 3, 1, 13, 27, 49}},
+    {6, 19, {1, 1, 1, 15, 7, 5}},
+};
+struct Directions {
+    U32 v[17][32];
+    Directions() {
+        for (U32 k = 0; k < 32; ++k) v[0][k] = 1u << (31 - k);
+        for (U32 d = 1; d < 17; ++d) {
+            const JoeKuo& j = kJoeKuo[d - 1];
+            U32 m[33] = {};
+            for (U32 i = 1; i <= j.s; ++i) m[i] = j.m[i - 1];
+            for (U32 i = j.s + 1; i <= 32; ++i) {
+                U32 x = m[i - j.s] ^ (m[i - j.s] << j.s);
+                for (U32 k = 1; k < j.s; ++k)
+                    if ((j.a >> (j.s - 1 - k)) & 1u) x ^= m[i - k] << k;
+                m[i] = x;
+            }
+            for (U32 k = 1; k <= 32; ++k) v[d][k - 1] = m[k] << (32 - k);
+        }
+    }
+};
+const Directions kDirections;
+U32 sobol(U32 index, U32 dimension) {  // sample_stream.h:102-107
+    U32 gray = index ^ (index >> 1), result = 0, bit = 0;
+    while (gray) {
+        if (gray & 1) result ^= kDirections.v[dimension][bit];
+        gray >>= 1;
+        ++bit;
+    }
+    return result;
+}
+U32 reverseBits(U32 x) {  // sample_stream.h:108-116
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+    x = ((x >> 4) & 0x0f0f0f0fu) | ((x & 0x0f0f0f0fu) << 4);
+    x = ((x >> 8) & 0x00ff00ffu) | ((x & 0x00ff00ffu) << 8);
+    return (x >> 16) | (x << 16);
+}
+// Burley 2020 hash-based Owen scrambling (sample_stream.h:122-126).
+U32 owenScramble(U32 x, U32 seed) {
+    x = reverseBits(x);
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return reverseBits(x);
+}
+// Policy dimensions (sample_stream.h:127-144).
+U32 samplingWord(const Identity& id, U32 dimension, U32 policy) {
+    const bool camera = dimension < 5, diffuse = dimension == 16 || dimension == 17,
+               light = dimension == 18 || dimension == 19, subsurface = dimension >= 20 && dimension <= 27;
+    if ((!camera || !(policy & 1)) && (!diffuse || !(policy & 2)) && (!light || !(policy & 4)) &&
+        (!subsurface || !(policy & 8)))
+        return word(id, dimension);
+    const U32 d = camera ? (dimension < 2 ? dimension : dimension < 4 ? dimension + 12 : 16) : dimension - 14;
+    Identity shift = id;
+    shift.index = id.index >> 32;
+    if (camera && dimension >= 2) {
+        const Words seeds = block(shift, dimension < 4 ? 0x50414444u : 0x50414445u);
+        return owenScramble(sobol(owenScramble(U32(id.index), seeds.v[0]), dimension < 4 ? dimension - 2 : 0u),
+                            seeds.v[dimension < 4 ? dimension - 1 : 1]);
+    }
+    return owenScramble(sobol(U32(id.index), d), block(shift, 0x534f424fu + (d >> 2)).v[d & 3]);
+}
+float draw(const Identity& id, U32 dimension, U32 policy) { return unit(samplingWord(id, dimension, policy)); }
+
+constexpr U32 CameraDimensions = 16, DimensionsPerBounce = 256;  // sample_stream.h:149
+// paths::Stream (async_integrator.h:151-176): one Philox block per four draws along a vertex's dimensions.
+struct Stream {
+    Identity id;
+    U32 dimension, group = ~0u;
+    U32 w[4] = {0, 0, 0, 0};
+    Stream(const Identity& i, U32 bounce, U32 offset)
+        : id(i), dimension(CameraDimensions + bounce * DimensionsPerBounce + offset) {}
+    float next() {
+        if ((dimension >> 2) != group) {
+            group = dimension >> 2;
+            const Words b = block(id, group);
+            for (int k = 0; k < 4; ++k) w[k] = b.v[k];
+        }
+        return unit(w[dimension++ & 3]);
+    }
+};
+// paths::childBranch (async_integrator.h:281-286).
+U64 childBranch(U64 parent, U32 a, U32 b) {
+    U64 z = parent + 0x9e3779b97f4a7c15ull * (U64(a) * 0x10001u + U64(b) + 1);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Vectors (surface_core.h:10-18, shade_math.h:15-26,65-69,113-120): every operation in the reference's order.
+struct V3 { float x, y, z; };
+V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+V3 operator*(V3 a, float b) { return {a.x * b, a.y * b, a.z * b}; }
+V3 operator*(float b, V3 a) { return {a.x * b, a.y * b, a.z * b}; }
+V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+V3 operator/(V3 a, float b) { return {a.x / b, a.y / b, a.z / b}; }
+float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+bool finite(float a) { return a == a && a <= 3.402823466e38f && a >= -3.402823466e38f; }
+bool finite3(V3 a) { return finite(a.x) && finite(a.y) && finite(a.z); }
+bool normalizeTo(V3 a, V3& b) {  // surface::normalize
+    const float l = dot(a, a);
+    if (!finite(l) || l <= 0) return false;
+    b = a * (1.0f / sqrtf(l));
+    return true;
+}
+float length(V3 a) { return sqrtf(dot(a, a)); }
+float luminance(V3 c) { return 0.299f * c.x + 0.587f * c.y + 0.114f * c.z; }
+bool isExactlyZero(V3 c) { return c.x == 0 && c.y == 0 && c.z == 0; }
+bool isEqual0(float x) { return fabsf(x - 0.0f) <= fmaxf(fabsf(x), 1.0f) * 1e-6f; }
+bool isBlack(V3 c) { return isEqual0(c.x) && isEqual0(c.y) && isEqual0(c.z); }
+V3 load3(const float* p) { return {p[0], p[1], p[2]}; }
+void store3(float* p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+constexpr float Pi = 3.14159265358979323846f, TwoPi = 6.28318530717958647692f;
+constexpr float OneOverPi = 0.318309886183790671538f, Epsilon = 1e-6f;
+
+// The shared sine/cosine polynomial (surface_core.h:116-126; ISPC 1.31's coefficients), used for every sin/cos on
+// the path so that both devices round alike. The reference's lobes call libm (CPU) or sincosf (CUDA) instead
+// (shade_math.h:123-130); this build uses the polynomial on both.
+void sincosPoly(float angle, float& sn, float& cs) {
+    const float kf = floorf(angle * .6366197466850281f), x = angle - kf * 1.5707963705062866f, x2 = x * x;
+    const int k = int(kf) & 3;
+    float s = -2.5029327943570934e-8f;
+    s = s * x2 + .000002760012648650445f;
+    s = s * x2 - .00019842604524455965f;
+    s = s * x2 + .008333347737789154f;
+    s = s * x2 - .1666666716337204f;
+    s = (s * x2 + 1) * x;
+    float c = -.0000002596301840185333f;
+    c = c * x2 + .000024756234779488295f;
+    c = c * x2 - .001388833043165505f;
+    c = c * x2 + .04166664183139801f;
+    c = c * x2 - .5f;
+    c = c * x2 + 1;
+    sn = k == 0 ? s : k == 1 ? c : k == 2 ? -s : -c;
+    cs = k == 0 ? c : k == 1 ? -s : k == 2 ? -c : s;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Camera (camera_core.h:53-67 transforms, :217-250 perspectiveRay without depth of field; surface_core.h:51-52 box
+// filter). The raster-to-camera and camera-to-render matrices are row-vector (scene_rdl2) matrices.
+struct Ray { V3 origin, direction; float nearT, farT; };
+V3 vectorM(const float* m, V3 a) {  // surface::vector
+    return {(a.x * m[0] + a.y * m[4]) + a.z * m[8], (a.x * m[1] + a.y * m[5]) + a.z * m[9],
+            (a.x * m[2] + a.y * m[6]) + a.z * m[10]};
+}
+V3 transformH(const float* m, V3 p) {
+    const V3 v = vectorM(m, p) + V3{m[12], m[13], m[14]};
+    const float w = ((p.x * m[3] + p.y * m[7]) + p.z * m[11]) + m[15];
+    if (w == 0.0f) return {0, 0, 0};
+    return {v.x / w, v.y / w, v.z / w};
+}
+V3 transformPoint(const float* m, V3 p) { return vectorM(m, p) + V3{m[12], m[13], m[14]}; }
+bool boxFilter(float width, float r, float& out) {
+    if (!finite(width) || width <= 0 || !finite(r) || r < 0 || r > 1) return false;
+    out = (r - .5f) * width + .5f;
+    return finite(out);
+}
+bool cameraRay(const x2_config& c, float x, float y, Ray& out) {
+    const V3 dir = transformH(c.raster_to_camera, {x, y, -1.0f});  // minus the focal point, 0
+    V3 n;
+    if (!normalizeTo(dir, n) || n.z >= 0) return false;
+    Ray r;
+    r.origin = transformPoint(c.camera_to_render, {0, 0, 0});
+    r.nearT = (-c.near_clip - 0.0f) / n.z;
+    r.farT = (-c.far_clip - 0.0f) / n.z;
+    if (!normalizeTo(vectorM(c.camera_to_render, dir), r.direction) || !finite3(r.origin) || !finite(r.nearT) ||
+        !finite(r.farT))
+        return false;
+    out = r;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The watertight ray/triangle test (S. Woop, C. Benthin, I. Wald, "Watertight Ray/Triangle Intersection", JCGT 2013):
+// the membership decision the reference leaves to OptiX/Embree. Exact distance ties go to the lower (geom, prim),
+// as triangle_hit.h:26 orders them, so the closest hit is a function of the scene alone, not of the BVH.
+struct WoopRay {
+    V3 org;
+    int kx, ky, kz;
+    float sx, sy, sz;
+    float nearT, farT;
+};
+float comp(V3 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : v.z; }
+WoopRay woopSetup(V3 org, V3 dir, float nearT, float farT) {
+    WoopRay w;
+    w.org = org;
+    const float ax = fabsf(dir.x), ay = fabsf(dir.y), az = fabsf(dir.z);
+    w.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    w.kx = w.kz == 2 ? 0 : w.kz + 1;
+    w.ky = w.kx == 2 ? 0 : w.kx + 1;
+    if (comp(dir, w.kz) < 0.0f) std::swap(w.kx, w.ky);
+    const float dz = comp(dir, w.kz);
+    w.sx = comp(dir, w.kx) / dz;
+    w.sy = comp(dir, w.ky) / dz;
+    w.sz = 1.0f / dz;
+    w.nearT = nearT;
+    w.farT = farT;
+    return w;
+}
+float edgeFunction(float a, float b, float c, float d) {  // a*b - c*d, again in double when it rounds to zero
+    const float e = a * b - c * d;
+    if (e != 0.0f) return e;
+    return float(double(a) * double(b) - double(c) * double(d));
+}
+bool woopHit(const WoopRay& w, V3 a, V3 b, V3 c, float& t) {
+    const V3 A = a - w.org, B = b - w.org, C = c - w.org;
+    const float Az = comp(A, w.kz), Bz = comp(B, w.kz), Cz = comp(C, w.kz);
+    const float Ax = comp(A, w.kx) - w.sx * Az, Ay = comp(A, w.ky) - w.sy * Az;
+    const float Bx = comp(B, w.kx) - w.sx * Bz, By = comp(B, w.ky) - w.sy * Bz;
+    const float Cx = comp(C, w.kx) - w.sx * Cz, Cy = comp(C, w.ky) - w.sy * Cz;
+    const float U = edgeFunction(Cx, By, Cy, Bx), V = edgeFunction(Ax, Cy, Ay, Cx), W = edgeFunction(Bx, Ay, By, Ax);
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = (U * (w.sz * Az) + V * (w.sz * Bz)) + W * (w.sz * Cz);
+    t = T / det;
+    return t >= w.nearT && t <= w.farT;
+}
+
+// Hit reconstruction (resume_core.h:13-47 geometryHit, triangle_hit.h:6-21 coordinates): t, u, v in FP64 from the
+// triangle's float vertices and the ray, the point rounded once, the conservative offset origins.
+struct Hit {
+    U32 geom = ~0u, prim = 0;
+    float t = 0, u = 0, v = 0;
+    V3 normal{0, 0, 0}, origin{0, 0, 0}, below{0, 0, 0}, point{0, 0, 0};
+};
+bool coordinates(const Ray& ray, V3 a, V3 b, V3 c, double& t, double& u, double& v) {
+    const double ax = a.x, ay = a.y, az = a.z;
+    const double ex = double(b.x) - ax, ey = double(b.y) - ay, ez = double(b.z) - az;
+    const double fx = double(c.x) - ax, fy = double(c.y) - ay, fz = double(c.z) - az;
+    const double dx = ray.direction.x, dy = ray.direction.y, dz = ray.direction.z;
+    const double px = dy * fz - dz * fy, py = dz * fx - dx * fz, pz = dx * fy - dy * fx;
+    const double determinant = (ex * px + ey * py) + ez * pz;
+    if (determinant == 0) return false;
+    const double tx = double(ray.origin.x) - ax, ty = double(ray.origin.y) - ay, tz = double(ray.origin.z) - az;
+    const double qx = ty * ez - tz * ey, qy = tz * ex - tx * ez, qz = tx * ey - ty * ex;
+    u = ((tx * px + ty * py) + tz * pz) / determinant;
+    v = ((dx * qx + dy * qy) + dz * qz) / determinant;
+    t = ((fx * qx + fy * qy) + fz * qz) / determinant;
+    return finite(float(t)) && finite(float(u)) && finite(float(v));
+}
+bool geometryHit(Hit& h, const Ray& ray, V3 a, V3 b, V3 c) {
+    const V3 e1 = b + (a * -1.0f), e2 = c + (a * -1.0f);
+    if (!normalizeTo(cross(e1, e2), h.normal)) return false;
+    if (dot(h.normal, ray.direction) > 0) h.normal = h.normal * -1.0f;
+    double pt, pu, pv;
+    if (!coordinates(ray, a, b, c, pt, pu, pv)) return false;
+    h.t = float(pt);
+    h.u = float(pu);
+    h.v = float(pv);
+    const V3 p{float(double(a.x) + ((double(b.x) - a.x) * pu + (double(c.x) - a.x) * pv)),
+               float(double(a.y) + ((double(b.y) - a.y) * pu + (double(c.y) - a.y) * pv)),
+               float(double(a.z) + ((double(b.z) - a.z) * pu + (double(c.z) - a.z) * pv))};
+    h.point = p;
+    const V3 error{fabsf(a.x) + fabsf(e1.x) + fabsf(e2.x), fabsf(a.y) + fabsf(e1.y) + fabsf(e2.y),
+                   fabsf(a.z) + fabsf(e1.z) + fabsf(e2.z)};
+    const float extent = fmaxf(fmaxf(fabsf(e1.x), fabsf(e1.y)),
+                               fmaxf(fabsf(e1.z), fmaxf(fabsf(e2.x), fmaxf(fabsf(e2.y), fabsf(e2.z)))));
+    const float d = 16 * 0x1p-23f *
+                    (fabsf(h.normal.x) * error.x + fabsf(h.normal.y) * error.y + fabsf(h.normal.z) * error.z + extent);
+    h.origin = p + h.normal * d;
+    if (h.normal.x != 0) h.origin.x = nextafterf(h.origin.x, h.normal.x > 0 ? FLT_MAX : -FLT_MAX);
+    if (h.normal.y != 0) h.origin.y = nextafterf(h.origin.y, h.normal.y > 0 ? FLT_MAX : -FLT_MAX);
+    if (h.normal.z != 0) h.origin.z = nextafterf(h.origin.z, h.normal.z > 0 ? FLT_MAX : -FLT_MAX);
+    h.below = p + h.normal * -d;
+    if (h.normal.x != 0) h.below.x = nextafterf(h.below.x, h.normal.x > 0 ? -FLT_MAX : FLT_MAX);
+    if (h.normal.y != 0) h.below.y = nextafterf(h.below.y, h.normal.y > 0 ? -FLT_MAX : FLT_MAX);
+    if (h.normal.z != 0) h.below.z = nextafterf(h.below.z, h.normal.z > 0 ? -FLT_MAX : FLT_MAX);
+    return finite3(h.origin) && finite3(h.below);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The oracle's own acceleration structure: a binary BVH, binned SAH, at most 4 triangles a leaf. It shares nothing
+// with the device's BVH8; the closest hit both find is the same only because the membership test and tie-break are.
+struct BNode {
+    float lo[3], hi[3];
+    U32 a, n;  // n > 0: leaf of triangles order[a, a+n); n == 0: children a and a + 1
+};
+struct Scene {
+    std::vector<V3> tri;  // 3 vertices per triangle, scene order
+    std::vector<U32> geomOf, primOf, material;
+    std::vector<x2_material> materials;
+    std::vector<x2_light> lights;
+    std::vector<BNode> nodes;
+    std::vector<U32> order;
+    U32 triangles = 0;
+};
+struct Box {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void grow(const float* l, const float* h) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], l[k]);
+            hi[k] = std::max(hi[k], h[k]);
+        }
+    }
+    void grow(V3 p) {
+        const float q[3] = {p.x, p.y, p.z};
+        grow(q, q);
+    }
+    float area() const {
+        if (lo[0] > hi[0]) return 0.0f;
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+void buildBvh(Scene& s) {
+    const U32 n = s.triangles;
+    std::vector<Box> boxes(n);
+    std::vector<float> centroid(size_t(n) * 3);
+    for (U32 i = 0; i < n; ++i) {
+        for (int v = 0; v < 3; ++v) boxes[i].grow(s.tri[size_t(i) * 3 + v]);
+        for (int k = 0; k < 3; ++k) centroid[size_t(i) * 3 + k] = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+    }
+    s.order.resize(n);
+    for (U32 i = 0; i < n; ++i) s.order[i] = i;
+    s.nodes.clear();
+    s.nodes.reserve(size_t(n) * 2 + 1);
+    s.nodes.push_back(BNode{});
+    struct Task { U32 node, first, count; };
+    std::vector<Task> tasks{{0, 0, n}};
+    constexpr int Bins = 16;
+    while (!tasks.empty()) {
+        const Task t = tasks.back();
+        tasks.pop_back();
+        Box bounds, cb;
+        for (U32 i = t.first; i < t.first + t.count; ++i) {
+            const U32 p = s.order[i];
+            bounds.grow(boxes[p].lo, boxes[p].hi);
+            cb.grow(&centroid[size_t(p) * 3], &centroid[size_t(p) * 3]);
+        }
+        BNode& node = s.nodes[t.node];
+        for (int k = 0; k < 3; ++k) {
+            node.lo[k] = t.count ? bounds.lo[k] : 0.0f;
+            node.hi[k] = t.count ? bounds.hi[k] : 0.0f;
+        }
+        if (t.count <= 4) {
+            s.nodes[t.node].a = t.count ? t.first : ~0u;  // only an empty scene's root holds nothing
+            s.nodes[t.node].n = t.count;
+            continue;
+        }
+        int axis = 0;
+        float best = FLT_MAX;
+        int bestSplit = -1;
+        for (int k = 0; k < 3; ++k) {
+            const float extent = cb.hi[k] - cb.lo[k];
+            if (!(extent > 0.0f)) continue;
+            Box bin[Bins];
+            U32 cnt[Bins] = {};
+            const float scale = float(Bins) / extent;
+            for (U32 i = t.first; i < t.first + t.count; ++i) {
+                const U32 p = s.order[i];
+                int b = int((centroid[size_t(p) * 3 + k] - cb.lo[k]) * scale);
+                b = std::min(std::max(b, 0), Bins - 1);
+                ++cnt[b];
+                bin[b].grow(boxes[p].lo, boxes[p].hi);
+            }
+            float leftArea[Bins];
+            U32 leftCount[Bins];
+            Box acc;
+            U32 c = 0;
+            for (int b = 0; b < Bins - 1; ++b) {
+                acc.grow(bin[b].lo, bin[b].hi);
+                c += cnt[b];
+                leftArea[b] = acc.area();
+                leftCount[b] = c;
+            }
+            acc = Box{};
+            c = 0;
+            for (int b = Bins - 1; b > 0; --b) {
+                acc.grow(bin[b].lo, bin[b].hi);
+                c += cnt[b];
+                const U32 l = leftCount[b - 1];
+                if (l == 0 || c == 0) continue;
+                const float cost = leftArea[b - 1] * float(l) + acc.area() * float(c);
+                if (cost < best) {
+                    best = cost;
+                    axis = k;
+                    bestSplit = b;
+                }
+            }
+        }
+        U32 mid;
+        if (bestSplit < 0) {
+            mid = t.first + t.count / 2;  // all centroids alike: split the list in half
+        } else {
+            const float extent = cb.hi[axis] - cb.lo[axis], scale = float(Bins) / extent;
+            auto it = std::partition(s.order.begin() + t.first, s.order.begin() + t.first + t.count, [&](U32 p) {
+                int b = int((centroid[size_t(p) * 3 + axis] - cb.lo[axis]) * scale);
+                b = std::min(std::max(b, 0), Bins - 1);
+                return b < bestSplit;
+            });
+            mid = U32(it - s.order.begin());
+            if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
+        }
+        const U32 left = U32(s.nodes.size());
+        s.nodes.push_back(BNode{});
+        s.nodes.push_back(BNode{});
+        s.nodes[t.node].a = left;
+        s.nodes[t.node].n = 0;
+        tasks.push_back({left, t.first, mid - t.first});
+        tasks.push_back({left + 1, mid, t.first + t.count - mid});
+    }
+}
+// A slab test that never culls a box holding a triangle the exact ray meets (Ize 2013: the far distance scaled by
+// 1 + 2 gamma(3)), with a further relative slack on the distances to compare against, so that triangles whose
+// computed distance ties or nearly ties the best are still tested.
+constexpr float FarScale = 1.0f + 2.0f * (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
+constexpr float Slack = 1.0f + 0x1p-12f;
+bool boxVisit(const BNode& b, V3 org, V3 inv, const bool zero[3], float lo, float hi) {
+    float tn = -FLT_MAX, tf = FLT_MAX;
+    const float o[3] = {org.x, org.y, org.z}, iv[3] = {inv.x, inv.y, inv.z};
+    for (int k = 0; k < 3; ++k) {
+        if (zero[k]) {
+            if (o[k] < b.lo[k] || o[k] > b.hi[k]) return false;
+            continue;
+        }
+        float t0 = (b.lo[k] - o[k]) * iv[k], t1 = (b.hi[k] - o[k]) * iv[k];
+        if (t0 > t1) std::swap(t0, t1);
+        tn = std::max(tn, t0);
+        tf = std::min(tf, t1);
+    }
+    tf = tf * (tf > 0 ? FarScale : 1.0f / FarScale);
+    const float loS = lo > 0 ? lo / Slack : lo * Slack, hiS = hi > 0 ? hi * Slack : hi / Slack;
+    return tn <= tf && tn <= hiS && tf >= loS;
+}
+bool keyLess(float t, U32 g, U32 p, float bt, U32 bg, U32 bp) {
+    return t < bt || (t == bt && (g < bg || (g == bg && p < bp)));
+}
+// Closest hit (or any hit with anyHit) among triangles with t in [near, far]. Returns the scene triangle index.
+U32 traverse(const Scene& s, V3 org, V3 dir, float nearT, float farT, bool anyHit, bool brute, float& tOut) {
+    const WoopRay w = woopSetup(org, dir, nearT, farT);
+    U32 bestTri = ~0u;
+    float bestT = farT;
+    U32 bg = ~0u, bp = ~0u;
+    auto test = [&](U32 tri) {
+        float t;
+        if (!woopHit(w, s.tri[size_t(tri) * 3], s.tri[size_t(tri) * 3 + 1], s.tri[size_t(tri) * 3 + 2], t)) return false;
+        if (bestTri == ~0u || keyLess(t, s.geomOf[tri], s.primOf[tri], bestT, bg, bp)) {
+            bestTri = tri;
+            bestT = t;
+            bg = s.geomOf[tri];
+            bp = s.primOf[tri];
+        }
+        return true;
+    };
+    if (brute || s.nodes.empty()) {
+        for (U32 i = 0; i < s.triangles; ++i)
+            if (test(i) && anyHit) break;
+        tOut = bestT;
+        return bestTri;
+    }
+    const bool zero[3] = {dir.x == 0.0f, dir.y == 0.0f, dir.z == 0.0f};
+    const V3 inv{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
+    U32 stack[128];
+    int top = 0;
+    stack[top++] = 0;
+    while (top) {
+        const BNode& b = s.nodes[stack[--top]];
+        if (b.n == 0 && b.a == ~0u) continue;
+        if (!boxVisit(b, org, inv, zero, nearT, bestTri == ~0u ? farT : bestT)) continue;
+        if (b.n) {
+            for (U32 i = b.a; i < b.a + b.n; ++i)
+                if (test(s.order[i]) && anyHit) {
+                    tOut = bestT;
+                    return bestTri;
+                }
+        } else {
+            if (top + 2 > 128) return ~0u;  // cannot happen for the depths the builder makes
+            stack[top++] = b.a + 1;
+            stack[top++] = b.a;
+        }
+    }
+    tOut = bestT;
+    return bestTri;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Lights (light_core.h): the rectangle and the constant environment, no texture, no spread, no clear radius.
+constexpr float InfiniteDistance = 3.4028234663852886e+38f - 1e32f;
+constexpr float EnvDistance = InfiniteDistance * 0.9f;
+constexpr float HitEpsilonEnd = 1.0f - 1e-3f;
+struct LightHit { V3 N; float distance; };
+V3 rows(const float* m, V3 v) {  // light_core.h:197-199
+    return ((load3(m) * v.x) + (load3(m + 3) * v.y)) + (load3(m + 6) * v.z);
+}
+V3 polar(float cosTheta, float sinTheta, float phi) {  // light_core.h:221-225
+    float s, c;
+    sincosPoly(phi, s, c);
+    return {sinTheta * c, sinTheta * s, cosTheta};
+}
+float planeDistance(const x2_light& l, V3 p) {  // light_core.h:255-260
+    float d = dot(load3(l.direction), p) + l.plane_offset;
+    if (l.sidedness == X2_SIDE_REVERSE) d = -d;
+    else if (l.sidedness == X2_SIDE_TWO) d = fabsf(d);
+    return d;
+}
+float areaToSolidAngleScale(V3 wi, V3 n, float dist) {  // light_core.h:215-219
+    const float denom = -dot(wi, n);
+    return denom < 0.00001f ? 0.0f : (dist * dist) / denom;
+}
+// Light::sample (light_core.h:476-500 env, :523-574 rect).
+bool lightSample(const x2_light& l, V3 p, const V3* n, const float r[3], V3& wi, LightHit& h) {
+    if (l.type == X2_LIGHT_ENV) {
+        if (l.hemispherical) {
+            const float cosTheta = r[0] > 1.0f ? 1.0f : r[0];
+            wi = rows(l.local_to_render, polar(cosTheta, sqrtf(1.0f - cosTheta * cosTheta), r[1] * TwoPi));
+        } else {
+            const float cosTheta = 1.0f - 2.0f * r[0];
+            wi = polar(cosTheta, sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta)), r[1] * TwoPi);
+        }
+        if (n && dot(*n, wi) < Epsilon) return false;
+        h.N = wi * -1.0f;
+        h.distance = EnvDistance;
+        return true;
+    }
+    if (l.type != X2_LIGHT_RECT) return false;
+    const float distance = planeDistance(l, p);
+    if (l.sidedness != X2_SIDE_TWO && distance < Epsilon) return false;
+    const V3 local{(0.5f - r[0]) * (2 * l.half_width), (0.5f - r[1]) * (2 * l.half_height), 0.0f};
+    wi = (rows(l.local_to_render, local) + load3(l.position)) - p;
+    h.distance = sqrtf(dot(wi, wi));
+    if (h.distance < Epsilon) return false;
+    wi = wi * (1.0f / h.distance);
+    if (n && dot(*n, wi) < Epsilon) return false;
+    const V3 dir = load3(l.direction);
+    h.N = dot(dir, wi) < 0.0f ? dir : dir * -1.0f;
+    return true;
+}
+// Light::eval (light_core.h:740-752 env, :789-799 rect).
+V3 lightEval(const x2_light& l, V3 wi, const LightHit& h, float& pdf) {
+    const V3 radiance = load3(l.radiance) * V3{1, 1, 1};
+    if (l.type == X2_LIGHT_ENV) {
+        pdf = l.inv_area;
+        return radiance;
+    }
+    pdf = l.inv_area;
+    pdf *= areaToSolidAngleScale(wi, h.N, h.distance);
+    return radiance;
+}
+// Light::intersect (light_core.h:813-847).
+bool lightIntersect(const x2_light& l, V3 p, V3 wi, float maxDistance, LightHit& h) {
+    if (l.type == X2_LIGHT_ENV) {
+        if (l.hemispherical && dot(wi, load3(l.direction)) < 0.0f) return false;
+        if (EnvDistance > maxDistance) return false;
+        h.N = wi * -1.0f;
+        h.distance = EnvDistance;
+        return true;
+    }
+    if (l.type != X2_LIGHT_RECT) return false;
+    const V3 localWi = rows(l.render_to_local_rot, wi), localP = rows(l.render_to_local, p - load3(l.position));
+    // rejectPosDirSidedness (light_core.h:261-264)
+    if ((localP.z < 0.0f) == (localWi.z < 0.0f)) return false;
+    if (localP.z < 0.0f ? l.sidedness == X2_SIDE_REGULAR : l.sidedness == X2_SIDE_REVERSE) return false;
+    const float localDistance = -localP.z / localWi.z, renderDistance = l.scale * localDistance;
+    if (renderDistance > maxDistance) return false;
+    const V3 hit = localP + localWi * localDistance;
+    if (fabsf(hit.x) >= l.half_width || fabsf(hit.y) >= l.half_height) return false;
+    const V3 dir = load3(l.direction);
+    h.N = localWi.z < 0.0f ? dir : dir * -1.0f;
+    h.distance = renderDistance;
+    return true;
+}
+// Light::canIlluminate (light_core.h:1024-1060), no subsurface radius.
+bool canIlluminate(const x2_light& l, V3 p, const V3* n) {
+    const float threshold = Epsilon - 0.0f;
+    if (l.type == X2_LIGHT_ENV) return true;
+    if (l.sidedness != X2_SIDE_TWO && planeDistance(l, p) < threshold) return false;
+    if (!n) return true;
+    const float c[4][2] = {{l.half_width, l.half_height}, {l.half_width, -l.half_height},
+                           {-l.half_width, l.half_height}, {-l.half_width, -l.half_height}};
+    for (int i = 0; i < 4; ++i)
+        if (dot(*n, (rows(l.local_to_render, V3{c[i][0], c[i][1], 0.0f}) + load3(l.position)) - p) > threshold)
+            return true;
+    return false;
+}
+// lightShadowSpan (async_integrator.h:305-311), no clear radius, min or max shadow distance.
+void lightShadowSpan(float distance, float& nearT, float& farT) {
+    nearT = fmaxf(nearT, 0.0f);
+    farT = fmaxf((distance - 0.0f) * HitEpsilonEnd, 2.0f * nearT);
+}
+float powerHeuristic(float f, float g) { return (f * f) / (f * f + g * g); }  // light_core.h:211
+
+// ---------------------------------------------------------------------------------------------------------------
+// Lobes (shade_lobes.h:205-282, 481-556; shade_lobes_eval.h:11, 93-146, 609-668; shade_math.h:135-262).
+enum : U32 { Reflection = 1u, Transmission = 2u, Diffuse = 4u, Glossy = 16u, Mirror = 64u };
+constexpr U32 AllSurfaceSides = Reflection | Transmission, AllLobes = Diffuse | Glossy | Mirror;
+constexpr U32 AllLobeTypes = AllSurfaceSides | AllLobes;
+bool matchesFlags(U32 type, U32 flags) { return (type & AllSurfaceSides & flags) && (type & AllLobes & flags); }
+U32 lobeRayMask(U32 t) {  // light_core.h:189-194
+    U32 mask = 0;
+    if (t & 1u) mask |= t & 0x54u;
+    if (t & 2u) mask |= (t & 0x54u) << 1;
+    return mask;
+}
+struct Frame { V3 x, y, z; };
+Frame frameOf(V3 N) {  // shade_math.h:138-141 (Duff et al.)
+    const float sign = copysignf(1.0f, N.z), a = -1.0f / (sign + N.z), b = N.x * N.y * a;
+    return {{1.0f + sign * N.x * N.x * a, sign * b, -sign * N.x}, {b, sign + N.y * N.y * a, -N.y}, N};
+}
+V3 localToGlobal(const Frame& f, V3 v) { return (f.x * v.x + f.y * v.y) + f.z * v.z; }
+V3 localSpherical(float cosTheta, float sinTheta, float phi) {
+    float s, c;
+    sincosPoly(phi, s, c);
+    return {sinTheta * c, sinTheta * s, cosTheta};
+}
+enum LobeKind : U32 { KindLambert = 0, KindGGX = 1 };
+struct Lobe {
+    U32 kind, type;
+    V3 scale, albedo;
+    Frame frame;
+    float roughness;  // GGX: max(0.001, r^2)
+};
+struct Slice { V3 Ng, wo; U32 flags; };
+struct Bsdf {
+    Lobe lobes[2];
+    U32 count = 0, type = 0;
+    V3 selfEmission{0, 0, 0};
+};
+V3 evalLambert(const Lobe& l, V3 wi, float& pdf) {  // shade_lobes_eval.h:611-620
+    const V3 N = l.frame.z;
+    const float cosWi = fmaxf(dot(N, wi), 0.0f);
+    pdf = cosWi * OneOverPi;
+    return (((1.0f * l.albedo) * l.scale) * OneOverPi) * cosWi;
+}
+float widenRoughness(float roughness, float cosNO) { return (1.2f - 0.2f * sqrtf(cosNO)) * roughness; }
+bool halfVector(V3 wo, V3 wi, V3& H) {  // shade_math.h:222-230
+    H = wo + wi;
+    const float l = length(H);
+    if (l > 0.001f) {
+        H = H * (1.0f / l);
+        return true;
+    }
+    return false;
+}
+V3 reflect(V3 N, V3 wo) {  // shade_math.h:190-194
+    const float NdotWo = dot(N, wo);
+    return -1.0f * wo + 2.0f * NdotWo * N;
+}
+V3 evalGGX(const Lobe& l, const Slice& s, V3 wi, float& pdf) {  // shade_lobes_eval.h:93-121, 131-135
+    const V3 N = l.frame.z;
+    const float cosNO = dot(N, s.wo), cosNI = dot(N, wi), roughness = l.roughness;
+    pdf = 0.0f;
+    if (cosNO <= 0.0f || cosNI <= 0.0f) return {0, 0, 0};
+    V3 m;
+    if (!halfVector(s.wo, wi, m)) return {0, 0, 0};
+    const float cosMI = dot(m, wi);
+    if (cosMI <= 0.0f) return {0, 0, 0};
+    const float cosThetaM = dot(m, l.frame.z);
+    if (cosThetaM <= 0.0f) return {0, 0, 0};
+    float alpha2 = roughness * roughness;
+    const float cosThetaM2 = cosThetaM * cosThetaM, piCosThetaM4 = Pi * cosThetaM2 * cosThetaM2;
+    const float tanThetaM2 = (1.0f - cosThetaM2) / cosThetaM2;
+    float tmp = alpha2 + tanThetaM2, D4 = 0.25f * alpha2 / (piCosThetaM4 * tmp * tmp);
+    const float invCosNO = 1.0f / cosNO, invCosNI = 1.0f / cosNI;
+    const float G1o = 2.0f / (1.0f + sqrtf(1.0f + alpha2 * (1.0f - cosNO * cosNO) * (invCosNO * invCosNO)));
+    const float G1i = 2.0f / (1.0f + sqrtf(1.0f + alpha2 * (1.0f - cosNI * cosNI) * (invCosNI * invCosNI)));
+    (void)invCosNI;
+    const V3 fr = l.scale * G1o * G1i * D4 * invCosNO * 1.0f;
+    const float w2 = 0.0f;  // no energy compensation: favg black (shade_lobes_eval.h:23-26)
+    const float alpha = widenRoughness(roughness, cosNO);
+    alpha2 = alpha * alpha;
+    tmp = alpha2 + tanThetaM2;
+    D4 = 0.25f * alpha2 / (piCosThetaM4 * tmp * tmp);
+    const float p2 = 0.0f, p1 = D4 * cosThetaM / cosMI;
+    pdf = (1.0f - w2) * p1 + w2 * p2;
+    return 1.0f * fr;  // shadowTerminatorFix off
+}
+V3 sampleGGXDir(const Slice& s, float r1, float r2, float roughness, float cosNO, const Frame& f) {  // :122-130
+    const float alpha = widenRoughness(roughness, cosNO), alpha2 = alpha * alpha;
+    const float tanThetaM2 = alpha2 * r1 / (1.0f - r1), cosThetaM = 1.0f / sqrtf(1.0f + tanThetaM2);
+    const float sinThetaM = cosThetaM * sqrtf(tanThetaM2);
+    const V3 m = localToGlobal(f, localSpherical(cosThetaM, sinThetaM, 2.0f * Pi * r2));
+    return reflect(m, s.wo);
+}
+V3 lobeEval(const Lobe& l, const Slice& s, V3 wi, float& pdf) {
+    return l.kind == KindLambert ? evalLambert(l, wi, pdf) : evalGGX(l, s, wi, pdf);
+}
+V3 lobeSample(const Lobe& l, const Slice& s, float r1, float r2, V3& wi, float& pdf) {  // :657-690
+    if (l.kind == KindLambert) {
+        const float cosTheta = r1 > 1.0f ? 1.0f : sqrtf(r1), sinTheta = sqrtf(1.0f - r1);
+        wi = localToGlobal(l.frame, localSpherical(cosTheta, sinTheta, r2 * TwoPi));
+        return evalLambert(l, wi, pdf);
+    }
+    const float cosNO = dot(l.frame.z, s.wo);
+    if (cosNO <= 0.0f) {
+        wi = {0, 0, 0};
+        pdf = 0.0f;
+        return {0, 0, 0};
+    }
+    const float w2 = 0.0f;
+    wi = sampleGGXDir(s, (r1 - w2) / (1.0f - w2), r2, l.roughness, cosNO, l.frame);
+    return evalGGX(l, s, wi, pdf);
+}
+U32 surfaceFlags(const Slice& s, V3 wi) {  // shade_lobes.h:323-326
+    return dot(s.Ng, s.wo) * dot(s.Ng, wi) > 0.0f ? (s.flags & ~U32(Transmission)) : (s.flags & ~U32(Reflection));
+}
+// The material's Bsdf at a hit: a Lambert lobe and a GGX lobe about the shading normal (lambertLobe,
+// cookTorranceLobe with KindGGX), in that order.
+void buildBsdf(const x2_material& m, V3 N, Bsdf& b) {
+    b.count = 0;
+    b.type = 0;
+    const V3 albedo = load3(m.albedo);
+    const Frame f = frameOf(N);
+    if (m.diffuse_weight > 0.0f) {
+        Lobe& l = b.lobes[b.count++];
+        l.kind = KindLambert;
+        l.type = Reflection | Diffuse;
+        l.frame = f;
+        l.albedo = albedo * m.diffuse_weight;
+        l.scale = {1, 1, 1};
+        l.roughness = 1.0f;
+        b.type |= l.type;
+    }
+    if (m.specular_weight > 0.0f) {
+        Lobe& l = b.lobes[b.count++];
+        l.kind = KindGGX;
+        l.type = Reflection | Glossy;
+        l.frame = f;
+        l.albedo = {0, 0, 0};
+        l.scale = {m.specular_weight, m.specular_weight, m.specular_weight};
+        l.roughness = fmaxf(0.001f, m.roughness * m.roughness);
+        b.type |= l.type;
+    }
+    b.selfEmission = load3(m.emission);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The estimator (async_integrator.h:1986-3042): the subset without volumes, presence, subsurface, AOVs, light
+// filters, light trees, mirrors or hair. rr_min_depth extends the roulette's start (1: the reference's).
+struct Pending {
+    V3 origin, dir, T;
+    U64 branch;
+    U32 depth, nonMirrorDepth, diffuseDepth, glossyDepth, mirrorDepth;
+    float nearT, farT;
+    U32 heldLight;
+    float heldFar;
+    V3 held;
+};
+constexpr U32 MaxPending = 32, PendingReserve = 16;  // async_integrator.h:250-267
+float bounceThinning(U32 top, U32 planned) {
+    const U32 room = top < PendingReserve ? PendingReserve - top : 1u;
+    return planned > room ? float(room) / float(planned) : 1.0f;
+}
+bool roomForChild(U32 top, float u, float& weight) {
+    if (top >= MaxPending) return false;
+    const float chance = float(MaxPending - top) / float(PendingReserve + 1);
+    if (u >= chance) return false;
+    weight = 1.0f / chance;
+    return true;
+}
+bool roulette(float lum, float threshold, float u, float& scale) {  // :141-147
+    scale = 1;
+    if (lum >= threshold) return true;
+    if (lum < u * threshold) return false;
+    scale = threshold / lum;
+    return true;
+}
+V3 smartClamp(V3 a, float clamp) {  // :134-138
+    if (clamp <= 0) return a;
+    const float m = fmaxf(a.x, fmaxf(a.y, a.z));
+    return m > clamp ? a * (clamp / m) : a;
+}
+
+struct Counters { U64 traces = 0, shadows = 0; };
+
+Hit trace(const Scene& s, const Ray& ray, Counters& k) {
+    ++k.traces;
+    Hit h;
+    float t;
+    const U32 tri = traverse(s, ray.origin, ray.direction, ray.nearT, ray.farT, false, false, t);
+    if (tri == ~0u) return h;
+    h.geom = s.geomOf[tri];
+    h.prim = s.primOf[tri];
+    if (!geometryHit(h, ray, s.tri[size_t(tri) * 3], s.tri[size_t(tri) * 3 + 1], s.tri[size_t(tri) * 3 + 2]))
+        h.normal = {0, 0, 0};
+    h.prim = tri;  // the scene triangle, for the shading lookups below
+    return h;
+}
+bool visible(const Scene& s, V3 o, V3 d, float nearT, float farT, Counters& k) {
+    ++k.shadows;
+    float t;
+    return traverse(s, o, d, nearT, farT, true, false, t) == ~0u;
+}
+
+int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double out[4], Counters& k) {
+    if (pixel >= c.width * c.height || c.bsdf_samples == 0 || c.bsdf_samples > 64 || c.light_samples == 0 ||
+        c.light_samples > 64)
+        return 2;
+    const Identity root{pixel, index, c.seed, 0};
+    float x, y;
+    if (!boxFilter(c.filter_width, draw(root, 0, c.sampling), x) || !boxFilter(c.filter_width, draw(root, 1, c.sampling), y))
+        return 2;
+    Ray camera;
+    if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, camera)) return 2;
+    Pending stack[MaxPending];
+    U32 top = 0;
+    {
+        Pending p{};
+        p.origin = camera.origin;
+        p.dir = camera.direction;
+        p.T = {1, 1, 1};
+        p.nearT = camera.nearT;
+        p.farT = camera.farT;
+        p.heldLight = ~0u;
+        stack[top++] = p;
+    }
+    V3 radiance{0, 0, 0};
+    float alpha = 0.0f;
+    const U32 rrMin = c.rr_min_depth > 1 ? c.rr_min_depth : 1u;
+    while (top) {
+        Pending pv = stack[--top];
+        const bool fromCamera = pv.depth == 0;
+        const Ray ray{pv.origin, pv.dir, fromCamera ? camera.nearT : pv.nearT, fromCamera ? camera.farT : pv.farT};
+        const Hit hit = trace(S, ray, k);
+        if (pv.heldLight != ~0u) {  // bounce-ray reuse (:2095-2106)
+            const float kept = hit.geom != ~0u && hit.t < pv.heldFar ? 0.0f : 1.0f;
+            if (kept > 0.0f) radiance = radiance + (kept < 1.0f ? pv.held * kept : pv.held);
+        }
+        const float hitAlpha = 1.0f - (0.0f + 0.0f + 0.0f) / 3.0f;
+        if (hit.geom == ~0u) {
+            if (pv.depth == 0) alpha += 1.0f * (1.0f - (1.0f + 1.0f + 1.0f) / 3.0f);  // camera pixel weight 1
+            continue;
+        }
+        // shade::hitState (shade_material.h:121-289) for a flat triangle without normals: Ng = (b-a) x (c-a),
+        // turned to face the ray.
+        const U32 tri = hit.prim;
+        const V3 a = S.tri[size_t(tri) * 3], b = S.tri[size_t(tri) * 3 + 1], cc = S.tri[size_t(tri) * 3 + 2];
+        if (hit.normal.x == 0 && hit.normal.y == 0 && hit.normal.z == 0) return 2;
+        V3 ng = cross(b - a, cc - a);
+        float l = length(ng);
+        if (!(l > 0)) {
+            const double ux = double(b.x) - a.x, uy = double(b.y) - a.y, uz = double(b.z) - a.z;
+            const double vx = double(cc.x) - a.x, vy = double(cc.y) - a.y, vz = double(cc.z) - a.z;
+            const double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+            const double dl = sqrt(nx * nx + ny * ny + nz * nz);
+            ng = dl > 0 ? V3{float(nx / dl), float(ny / dl), float(nz / dl)} : -ray.direction;
+            l = length(ng);
+            if (!(l > 0)) return 2;
+        }
+        V3 Ng = ng / l, N = Ng;
+        const V3 P = hit.point, wo = -ray.direction;
+        if (!(dot(Ng, ray.direction) < 0.0f)) {
+            N = -N;
+            Ng = -Ng;
+        }
+        if (!finite3(N) || !finite3(Ng)) return 2;
+        Bsdf bsdf;
+        buildBsdf(S.materials[S.material[tri]], N, bsdf);
+        if (pv.depth == 0) alpha += 1.0f * hitAlpha;
+        radiance = radiance + pv.T * bsdf.selfEmission;
+        if (bsdf.count == 0) continue;
+        U32 indirectFlags = 0;
+        if (c.bsdf_samples > 0 && c.max_depth > 0 && pv.depth < c.max_depth) {
+            if (pv.diffuseDepth < c.max_diffuse_depth) indirectFlags |= AllSurfaceSides | Diffuse;
+            if (pv.glossyDepth < c.max_glossy_depth) indirectFlags |= AllSurfaceSides | Glossy;
+            if (pv.mirrorDepth < c.max_mirror_depth) indirectFlags |= AllSurfaceSides | Mirror;
+        }
+        const bool doIndirect = indirectFlags != 0;
+        const V3 cull = (bsdf.type & Reflection) ? Ng : -Ng;
+        const V3* cullN = (bsdf.type & AllSurfaceSides) != AllSurfaceSides ? &cull : nullptr;
+        const Slice slice{Ng, wo, AllLobeTypes};
+        const bool split = pv.nonMirrorDepth == 0;
+        const U32 nl = split ? c.light_samples : 1;
+        const bool clampHere = pv.nonMirrorDepth >= c.clamp_depth;
+        const bool rr = pv.nonMirrorDepth >= rrMin && c.rr_threshold > 0;
+        U32 lobeSamples[2];
+        U32 planned = 0;
+        for (U32 q = 0; q < bsdf.count; ++q) {
+            lobeSamples[q] = !split ? 1 : c.bsdf_samples;
+            planned += lobeSamples[q];
+        }
+        const float thinning = bounceThinning(top, planned);
+        const V3 above = hit.origin, below = hit.below;
+
+        // Light samples (:2615-2785).
+        const U64 lightBranch = childBranch(pv.branch, 0xfffe, 0);
+        for (U32 i = 0; i < c.light_count; ++i) {
+            const x2_light& L = S.lights[i];
+            if (!canIlluminate(L, P, cullN)) continue;
+            const U64 lightAddress = childBranch(lightBranch, i, 0);
+            Identity lid = root;
+            lid.branch = lightAddress;
+            Stream ls(lid, pv.depth, 0);
+            for (U32 s = 0; s < nl; ++s) {
+                float r[3] = {ls.next(), ls.next(), ls.next()};
+                const float u = ls.next();
+                if (split) {
+                    const Identity strat{pixel, U64(index) * nl + s, c.seed, lightAddress};
+                    r[0] = draw(strat, CameraDimensions + 2, c.sampling);
+                    r[1] = draw(strat, CameraDimensions + 3, c.sampling);
+                }
+                V3 wi;
+                LightHit lh;
+                if (!lightSample(L, P, cullN, r, wi, lh)) continue;
+                float pdfL;
+                const V3 Li = lightEval(L, wi, lh, pdfL);
+                if (pdfL == 0.0f || isExactlyZero(Li) || !(fabsf(pdfL) < 3.4e38f) || !finite3(Li)) continue;
+                const V3 factor = pv.T * Li * (1.0f / (float(nl) * pdfL * 1.0f));
+                const U32 flags = surfaceFlags(slice, wi);
+                V3 t{0, 0, 0};
+                bool any = false;
+                for (U32 q = 0; q < bsdf.count; ++q) {
+                    const Lobe& lobe = bsdf.lobes[q];
+                    if (!matchesFlags(lobe.type, flags)) continue;
+                    if (!(lobeRayMask(lobe.type) & L.visibility_mask)) continue;
+                    float pdf;
+                    const V3 f = lobeEval(lobe, slice, wi, pdf);
+                    if (pdf == 0.0f || isExactlyZero(f)) continue;
+                    V3 tk = factor * f * powerHeuristic(float(nl) * pdfL, float(lobeSamples[q]) * pdf);
+                    if (clampHere) tk = smartClamp(tk, c.clamp_value);
+                    t = t + tk;
+                    any = true;
+                }
+                if (!any) continue;
+                if (rr) {
+                    float scale;
+                    if (!roulette(luminance(t), c.rr_threshold, u, scale)) continue;
+                    t = t * scale;
+                }
+                if (!finite3(t) || isExactlyZero(t)) continue;
+                const V3 shadowOrigin = dot(wi, Ng) >= 0.0f ? above : below;
+                float shadowNear = 0.0f, shadowFar;
+                lightShadowSpan(lh.distance, shadowNear, shadowFar);
+                const float kept = visible(S, shadowOrigin, wi, shadowNear, shadowFar, k) ? 1.0f : 0.0f;
+                if (kept > 0.0f) {
+                    if (kept < 1.0f) t = t * kept;
+                    radiance = radiance + t;
+                }
+            }
+        }
+
+        // BSDF samples (:2787-3031).
+        for (U32 q = 0; q < bsdf.count; ++q) {
+            const Lobe& lobe = bsdf.lobes[q];
+            const U32 nk = lobeSamples[q];
+            const bool mirror = (lobe.type & Mirror) != 0;
+            const U32 lobeMask = lobeRayMask(lobe.type);
+            const bool lobeIndirect = matchesFlags(lobe.type, indirectFlags);
+            const U64 lobeBranch = childBranch(pv.branch, q, 0xffff);
+            for (U32 i = 0; i < nk; ++i) {
+                const U64 sampleBranch = childBranch(lobeBranch, i, 0);
+                Identity bid = root;
+                bid.branch = sampleBranch;
+                Stream bs(bid, pv.depth, 0);
+                float r1 = bs.next(), r2 = bs.next();
+                const float choice = bs.next(), u = bs.next();
+                if (split && !mirror) {
+                    const Identity strat{pixel, U64(index) * nk + i, c.seed, lobeBranch};
+                    r1 = draw(strat, CameraDimensions, c.sampling);
+                    r2 = draw(strat, CameraDimensions + 1, c.sampling);
+                }
+                V3 wi;
+                float pdf;
+                const V3 f = lobeSample(lobe, slice, r1, r2, wi, pdf);
+                if (!(pdf != 0.0f && !isExactlyZero(f)) || !finite3(f) || !finite3(wi)) continue;
+                if (!matchesFlags(lobe.type, surfaceFlags(slice, wi))) continue;
+                const V3 pt = pv.T * f * (1.0f / (float(nk) * pdf));
+                const bool terminators = (lobe.type & ~indirectFlags & AllLobes) != 0;
+                (void)terminators;
+                U32 hits = 0;
+                for (U32 j = 0; j < c.light_count; ++j) {
+                    LightHit lh;
+                    if ((lobeMask & S.lights[j].visibility_mask) && lightIntersect(S.lights[j], P, wi, 3.4028234663852886e+38f, lh))
+                        ++hits;
+                }
+                V3 direct{0, 0, 0};
+                float lightDistance = 0;
+                bool lightHit = false;
+                U32 lightIndex = 0;
+                if (hits) {
+                    const U32 chosen = U32(choice * float(hits)) < hits ? U32(choice * float(hits)) : hits - 1;
+                    for (U32 j = 0, seen = 0; j < c.light_count; ++j) {
+                        LightHit lh;
+                        if (!(lobeMask & S.lights[j].visibility_mask) || !lightIntersect(S.lights[j], P, wi, 3.4028234663852886e+38f, lh))
+                            continue;
+                        if (seen++ != chosen) continue;
+                        float pdfL;
+                        const V3 Li = lightEval(S.lights[j], wi, lh, pdfL) * float(hits);
+                        if (!(pdfL == 0.0f || isExactlyZero(Li)) && fabsf(pdfL) < 3.4e38f && finite3(Li)) {
+                            direct = mirror ? Li * pt : Li * pt * powerHeuristic(float(nk) * pdf, float(nl) * pdfL);
+                            lightDistance = lh.distance;
+                            lightHit = true;
+                            lightIndex = j;
+                        }
+                        break;
+                    }
+                }
+                if (!lightHit && !doIndirect) continue;
+                if (clampHere) direct = smartClamp(direct, c.clamp_value);
+                V3 indirect = doIndirect ? pt : V3{0, 0, 0};
+                if (rr) {
+                    float scale;
+                    if (!roulette(fmaxf(luminance(direct), luminance(indirect)), c.rr_threshold, u, scale)) continue;
+                    direct = direct * scale;
+                    indirect = indirect * scale;
+                }
+                const V3 origin = dot(wi, Ng) >= 0.0f ? above : below;
+                float shadowNear = 0.0f, shadowFar = 0.0f, kept = 0.0f;
+                const bool bounces = lobeIndirect && !isExactlyZero(indirect) && finite3(indirect);
+                bool held = false;
+                if (lightHit && !isBlack(direct) && finite3(direct)) {
+                    lightShadowSpan(lightDistance, shadowNear, shadowFar);
+                    held = bounces && thinning >= 1.0f && top + PendingReserve < MaxPending && lobeMask &&
+                           (c.shadow_reuse & lobeMask) == lobeMask && shadowNear == 0.0f;
+                    if (!held) kept = visible(S, origin, wi, shadowNear, shadowFar, k) ? 1.0f : 0.0f;
+                }
+                const V3 directBefore = direct;
+                if (kept > 0.0f && !held) {
+                    if (kept < 1.0f) direct = direct * kept;
+                    radiance = radiance + direct;
+                }
+                if (!bounces) continue;
+                float keptShare = 1.0f;
+                if (thinning < 1.0f || top + PendingReserve >= MaxPending) {
+                    Identity kid = bid;
+                    kid.branch = childBranch(sampleBranch, 0xfff7, 0);
+                    Stream chance(kid, pv.depth, 0);
+                    if (thinning < 1.0f) {
+                        if (chance.next() >= thinning) continue;
+                        keptShare = 1.0f / thinning;
+                    }
+                    if (top + PendingReserve >= MaxPending) {
+                        float tight = 1.0f;
+                        if (!roomForChild(top, chance.next(), tight)) continue;
+                        keptShare *= tight;
+                    }
+                }
+                Pending child{};
+                child.origin = origin;
+                child.dir = wi;
+                child.T = indirect * keptShare;
+                child.branch = sampleBranch;
+                child.depth = pv.depth + 1;
+                child.diffuseDepth = pv.diffuseDepth + ((lobe.type & Diffuse) ? 1 : 0);
+                child.glossyDepth = pv.glossyDepth + ((lobe.type & Glossy) ? 1 : 0);
+                child.mirrorDepth = pv.mirrorDepth + (mirror ? 1 : 0);
+                child.nonMirrorDepth = pv.nonMirrorDepth + (mirror ? 0 : 1);
+                child.farT = 3.402823466e+38f;
+                child.nearT = 0.0f;
+                child.heldLight = ~0u;
+                if (held) {
+                    child.heldLight = lightIndex;
+                    child.heldFar = shadowFar;
+                    child.held = directBefore;
+                }
+                stack[top++] = child;
+            }
+        }
+    }
+    out[0] = radiance.x;
+    out[1] = radiance.y;
+    out[2] = radiance.z;
+    out[3] = alpha;
+    return 0;
+}
+
+// The pixel of a tile's i-th position (async_integrator.h:3128-3140), ~0u outside the image or viewport.
+U32 tilePixel(U32 tile, U32 position, const x2_config& c) {
+    const U32 tilesX = (c.width + 7) / 8;
+    const U32 x = (tile % tilesX) * 8 + position % 8, y = (tile / tilesX) * 8 + position / 8;
+    if (!(x < c.width && y < c.height)) return ~0u;
+    if (int(x) < c.viewport[0] || int(y) < c.viewport[1] || int(x) >= c.viewport[2] || int(y) >= c.viewport[3])
+        return ~0u;
+    return y * c.width + x;
+}
+
+// Chunk rendering over worker threads, each claiming chunks by an atomic counter (async_cpu.h:1584-1673
+// renderChunk: sample-major over the tile's 64 pixels into tile-local FP64, then added to the film). Chunks of the
+// same tile go to the same worker in list order so each pixel's samples add in index order.
+void renderChunks(const Scene& S, const x2_config& c, const std::vector<x2_chunk>& chunks, U32 threads, double* sums,
+                  U32* counts, U64 counters[3]) {
+    std::vector<std::vector<U32>> byTile;
+    std::vector<U32> tiles;
+    {
+        std::vector<int> slot(((c.width + 7) / 8) * ((c.height + 7) / 8), -1);
+        for (U32 i = 0; i < chunks.size(); ++i) {
+            const U32 t = chunks[i].tile;
+            if (t >= slot.size()) continue;
+            if (slot[t] < 0) {
+                slot[t] = int(byTile.size());
+                byTile.emplace_back();
+                tiles.push_back(t);
+            }
+            byTile[slot[t]].push_back(i);
+        }
+    }
+    std::atomic<U32> next{0};
+    std::atomic<U64> traces{0}, shadows{0}, errors{0};
+    auto worker = [&]() {
+        Counters k;
+        U64 errs = 0;
+        for (;;) {
+            const U32 w = next.fetch_add(1);
+            if (w >= byTile.size()) break;
+            double local[64][4] = {};
+            U32 n[64] = {};
+            for (const U32 ci : byTile[w]) {
+                const x2_chunk& ch = chunks[ci];
+                for (U32 s = 0; s < ch.count; ++s)
+                    for (U32 pos = 0; pos < 64; ++pos) {
+                        const U32 pixel = tilePixel(ch.tile, pos, c);
+                        if (pixel == ~0u) continue;
+                        double rgba[4];
+                        const int st = integrate(S, c, pixel, ch.first + s, rgba, k);
+                        if (st == 0) {
+                            for (int q = 0; q < 4; ++q) local[pos][q] += rgba[q];
+                            ++n[pos];
+                        } else if (st != 3) {  // a skipped sample adds nothing, not even its count (async_gpu.cu:347)
+                            ++errs;
+                        }
+                    }
+            }
+            for (U32 pos = 0; pos < 64; ++pos) {
+                const U32 pixel = tilePixel(tiles[w], pos, c);
+                if (pixel == ~0u) continue;
+                for (int q = 0; q < 4; ++q) sums[size_t(pixel) * 4 + q] += local[pos][q];
+                counts[pixel] += n[pos];
+            }
+        }
+        traces += k.traces;
+        shadows += k.shadows;
+        errors += errs;
+    };
+    threads = std::max<U32>(1, threads);
+    std::vector<std::thread> pool;
+    for (U32 t = 1; t < threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    counters[0] += traces;
+    counters[1] += shadows;
+    counters[2] += errors;
+}
+}  // namespace
+
+struct oracle_scene {
+    Scene s;
+};
+
+extern "C" {
+void oracle_philox(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
+    const Words w = philox(Words{{counter[0], counter[1], counter[2], counter[3]}}, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = w.v[i];
+}
+void oracle_block(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t group, uint32_t out[4]) {
+    const Words w = block(Identity{pixel, index, stream, branch}, group);
+    for (int i = 0; i < 4; ++i) out[i] = w.v[i];
+}
+uint32_t oracle_word(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t dimension) {
+    return word(Identity{pixel, index, stream, branch}, dimension);
+}
+float oracle_unit(uint32_t bits) { return unit(bits); }
+uint32_t oracle_sobol(uint32_t index, uint32_t dimension) { return dimension < 17 ? sobol(index, dimension) : 0u; }
+void oracle_directions(uint32_t out[17 * 32]) { std::memcpy(out, kDirections.v, sizeof(kDirections.v)); }
+uint32_t oracle_owen_scramble(uint32_t x, uint32_t seed) { return owenScramble(x, seed); }
+uint32_t oracle_sampling_word(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t dimension,
+                              uint32_t policy) {
+    return samplingWord(Identity{pixel, index, stream, branch}, dimension, policy);
+}
+uint64_t oracle_child_branch(uint64_t parent, uint32_t a, uint32_t b) { return childBranch(parent, a, b); }
+void oracle_stream(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t bounce, uint32_t n,
+                   float* out) {
+    Stream s(Identity{pixel, index, stream, branch}, bounce, 0);
+    for (uint32_t i = 0; i < n; ++i) out[i] = s.next();
+}
+void oracle_sincos(float angle, float* s, float* c) { sincosPoly(angle, *s, *c); }
+
+static Lobe testLobe(uint32_t kind, const float N[3], float roughness) {
+    x2_material m{};
+    m.albedo[0] = m.albedo[1] = m.albedo[2] = 1.0f;
+    m.roughness = roughness;
+    m.diffuse_weight = kind == 0 ? 1.0f : 0.0f;
+    m.specular_weight = kind == 1 ? 1.0f : 0.0f;
+    Bsdf b;
+    buildBsdf(m, load3(N), b);
+    return b.lobes[0];
+}
+void oracle_lobe_eval(uint32_t kind, const float N[3], const float Ng[3], const float wo[3], const float wi[3],
+                      float roughness, float f[3], float* pdf) {
+    const Lobe l = testLobe(kind, N, roughness);
+    const Slice s{load3(Ng), load3(wo), AllLobeTypes};
+    store3(f, lobeEval(l, s, load3(wi), *pdf));
+}
+void oracle_lobe_sample(uint32_t kind, const float N[3], const float Ng[3], const float wo[3], float roughness,
+                        float r1, float r2, float wi[3], float f[3], float* pdf) {
+    const Lobe l = testLobe(kind, N, roughness);
+    const Slice s{load3(Ng), load3(wo), AllLobeTypes};
+    V3 w;
+    store3(f, lobeSample(l, s, r1, r2, w, *pdf));
+    store3(wi, w);
+}
+int oracle_light_sample(const x2_light* l, const float p[3], const float* n, const float r[3], float wi[3],
+                        float* distance, float Li[3], float* pdf) {
+    V3 nn = n ? load3(n) : V3{0, 0, 0}, w;
+    LightHit h;
+    if (!lightSample(*l, load3(p), n ? &nn : nullptr, r, w, h)) return 0;
+    store3(Li, lightEval(*l, w, h, *pdf));
+    store3(wi, w);
+    *distance = h.distance;
+    return 1;
+}
+int oracle_light_intersect(const x2_light* l, const float p[3], const float wi[3], float Li[3], float* pdf,
+                           float* distance) {
+    LightHit h;
+    if (!lightIntersect(*l, load3(p), load3(wi), 3.4028234663852886e+38f, h)) return 0;
+    store3(Li, lightEval(*l, load3(wi), h, *pdf));
+    *distance = h.distance;
+    return 1;
+}
+int oracle_light_can_illuminate(const x2_light* l, const float p[3], const float* n) {
+    V3 nn = n ? load3(n) : V3{0, 0, 0};
+    return canIlluminate(*l, load3(p), n ? &nn : nullptr) ? 1 : 0;
+}
+int oracle_triangle(const float o[3], const float d[3], float near_t, float far_t, const float a[3], const float b[3],
+                    const float c[3], float* t) {
+    const WoopRay w = woopSetup(load3(o), load3(d), near_t, far_t);
+    return woopHit(w, load3(a), load3(b), load3(c), *t) ? 1 : 0;
+}
+
+int oracle_camera_ray(const x2_config* c, float x, float y, float origin[3], float dir[3], float* nearT, float* farT) {
+    Ray r;
+    if (!cameraRay(*c, x, y, r)) return 0;
+    store3(origin, r.origin);
+    store3(dir, r.direction);
+    *nearT = r.nearT;
+    *farT = r.farT;
+    return 1;
+}
+int oracle_geometry_hit(const float o[3], const float d[3], const float a[3], const float b[3], const float c[3],
+                        float out[13]) {
+    Hit h;
+    const Ray ray{load3(o), load3(d), 0.0f, 3.4e38f};
+    if (!geometryHit(h, ray, load3(a), load3(b), load3(c))) return 0;
+    out[0] = h.t;
+    store3(out + 1, h.point);
+    store3(out + 4, h.normal);
+    store3(out + 7, h.origin);
+    store3(out + 10, h.below);
+    return 1;
+}
+
+oracle_scene* oracle_scene_create(const x2_scene* in) {
+    if (!in || !in->positions || !in->indices || !in->geom_first || !in->geom_material || !in->materials) return nullptr;
+    auto* o = new oracle_scene;
+    Scene& s = o->s;
+    s.triangles = in->triangle_count;
+    s.tri.resize(size_t(in->triangle_count) * 3);
+    s.geomOf.resize(in->triangle_count);
+    s.primOf.resize(in->triangle_count);
+    s.material.resize(in->triangle_count);
+    for (U32 g = 0; g < in->geom_count; ++g)
+        for (U32 t = in->geom_first[g]; t < in->geom_first[g + 1] && t < in->triangle_count; ++t) {
+            s.geomOf[t] = g;
+            s.primOf[t] = t - in->geom_first[g];
+            s.material[t] = in->geom_material[g];
+        }
+    for (U32 t = 0; t < in->triangle_count; ++t)
+        for (int v = 0; v < 3; ++v) s.tri[size_t(t) * 3 + v] = load3(in->positions + size_t(in->indices[size_t(t) * 3 + v]) * 3);
+    s.materials.assign(in->materials, in->materials + in->material_count);
+    if (in->light_count) s.lights.assign(in->lights, in->lights + in->light_count);
+    buildBvh(s);
+    return o;
+}
+void oracle_scene_destroy(oracle_scene* s) { delete s; }
+void oracle_closest(const oracle_scene* o, const float org[3], const float d[3], float near_t, float far_t, int brute,
+                    uint32_t* geom, uint32_t* prim, float* t) {
+    const U32 tri = traverse(o->s, load3(org), load3(d), near_t, far_t, false, brute != 0, *t);
+    *geom = tri == ~0u ? ~0u : o->s.geomOf[tri];
+    *prim = tri == ~0u ? 0u : o->s.primOf[tri];
+}
+int oracle_occluded(const oracle_scene* o, const float org[3], const float d[3], float near_t, float far_t, int brute) {
+    float t;
+    return traverse(o->s, load3(org), load3(d), near_t, far_t, true, brute != 0, t) != ~0u ? 1 : 0;
+}
+int oracle_sample(const oracle_scene* o, const x2_config* c, uint32_t pixel, uint32_t index, double rgba[4],
+                  uint64_t counters[2]) {
+    Counters k;
+    const int st = integrate(o->s, *c, pixel, index, rgba, k);
+    counters[0] += k.traces;
+    counters[1] += k.shadows;
+    return st;
+}
+int oracle_render(const oracle_scene* o, const x2_config* c, uint32_t threads, double* sums, uint32_t* counts,
+                  uint64_t counters[3]) {
+    std::vector<x2_chunk> chunks;
+    const U32 tiles = ((c->width + 7) / 8) * ((c->height + 7) / 8);
+    for (U32 t = 0; t < tiles; ++t) chunks.push_back({t, 0, c->spp, 0});
+    renderChunks(o->s, *c, chunks, threads, sums, counts, counters);
+    return 0;
+}
+int oracle_render_chunks(const oracle_scene* o, const x2_config* c, const x2_chunk* chunks, uint32_t n,
+                         uint32_t threads, double* sums, uint32_t* counts, uint64_t counters[3]) {
+    renderChunks(o->s, *c, std::vector<x2_chunk>(chunks, chunks + n), threads, sums, counts, counters);
+    return 0;
+}
+}

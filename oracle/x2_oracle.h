@@ -1,0 +1,79 @@
+/* x2_oracle.h — TEST INFRASTRUCTURE. The CPU oracle: a plain C++ restatement of the reference's shared
+ * integrator subset (xpu2/interactive/async_integrator.h:1986-3042 and the headers it calls), used only by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg to check and time the HIP device. Nothing in the product
+ * links, loads or calls this library.
+ *
+ * Inputs use the product's boundary records (include/x2dev.h) so both sides read the same bytes. */
+#ifndef X2_ORACLE_H
+#define X2_ORACLE_H
+#include <stdint.h>
+#include "../include/x2dev.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* RNG (sample_stream.h). */
+void oracle_philox(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+void oracle_block(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t group, uint32_t out[4]);
+uint32_t oracle_word(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t dimension);
+float oracle_unit(uint32_t bits);
+uint32_t oracle_sobol(uint32_t index, uint32_t dimension);
+void oracle_directions(uint32_t out[17 * 32]);
+uint32_t oracle_owen_scramble(uint32_t x, uint32_t seed);
+uint32_t oracle_sampling_word(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t dimension,
+                              uint32_t policy);
+uint64_t oracle_child_branch(uint64_t parent, uint32_t a, uint32_t b);
+/* n consecutive draws of a vertex stream (async_integrator.h:151-176 paths::Stream). */
+void oracle_stream(uint32_t pixel, uint64_t index, uint64_t stream, uint64_t branch, uint32_t bounce, uint32_t n,
+                   float* out);
+
+/* Lobes (shade_lobes_eval.h). kind 0 Lambert (albedo 1), 1 GGX (scale 1). Returns f (with cosine) and pdf. */
+void oracle_lobe_eval(uint32_t kind, const float N[3], const float Ng[3], const float wo[3], const float wi[3],
+                      float roughness, float f[3], float* pdf);
+void oracle_lobe_sample(uint32_t kind, const float N[3], const float Ng[3], const float wo[3], float roughness,
+                        float r1, float r2, float wi[3], float f[3], float* pdf);
+void oracle_sincos(float angle, float* s, float* c);
+
+/* Lights (light_core.h). n may be null (no culling). Returns 1 when the sample/intersection exists. */
+int oracle_light_sample(const x2_light* l, const float p[3], const float* n, const float r[3], float wi[3],
+                        float* distance, float Li[3], float* pdf);
+int oracle_light_intersect(const x2_light* l, const float p[3], const float wi[3], float Li[3], float* pdf,
+                           float* distance);
+int oracle_light_can_illuminate(const x2_light* l, const float p[3], const float* n);
+
+/* The watertight triangle test. Returns 1 with *t on a hit within [near, far]. */
+int oracle_triangle(const float o[3], const float d[3], float near_t, float far_t, const float a[3],
+                    const float b[3], const float c[3], float* t);
+
+/* The pinhole camera's ray through raster (x, y) (camera_core.h:217-250). */
+int oracle_camera_ray(const x2_config* c, float x, float y, float origin[3], float dir[3], float* near_t, float* far_t);
+
+/* FP64 hit reconstruction (resume_core.h:17 geometryHit): out = t, point[3], normal[3], origin[3], below[3]. */
+int oracle_geometry_hit(const float o[3], const float d[3], const float a[3], const float b[3], const float c[3],
+                        float out[13]);
+
+/* A scene: copied, with the oracle's own binary SAH BVH built over it. */
+typedef struct oracle_scene oracle_scene;
+oracle_scene* oracle_scene_create(const x2_scene* scene);
+void oracle_scene_destroy(oracle_scene* s);
+/* Closest hit (geom, prim, t; geom = ~0u on a miss); brute != 0 tests every triangle instead of the BVH. */
+void oracle_closest(const oracle_scene* s, const float o[3], const float d[3], float near_t, float far_t, int brute,
+                    uint32_t* geom, uint32_t* prim, float* t);
+int oracle_occluded(const oracle_scene* s, const float o[3], const float d[3], float near_t, float far_t, int brute);
+
+/* One sample (async_integrator.h:1986 integratePaths). rgba receives radiance and alpha; counters[0] += traces,
+ * counters[1] += shadow rays. Returns the SampleStatus (0 Done, 2 Error, 3 Skip). */
+int oracle_sample(const oracle_scene* s, const x2_config* c, uint32_t pixel, uint32_t index, double rgba[4],
+                  uint64_t counters[2]);
+/* A whole uniform frame: every pixel's samples [0, spp) accumulated in sample-index order into FP64 sums (rgba) and
+ * counts, over `threads` worker threads claiming 8x8 tiles. counters: traces, shadow rays, errors. */
+int oracle_render(const oracle_scene* s, const x2_config* c, uint32_t threads, double* sums, uint32_t* counts,
+                  uint64_t counters[3]);
+/* The same over chunks (tile, first, count), each sample of each chunk once: the film of a partial frame. */
+int oracle_render_chunks(const oracle_scene* s, const x2_config* c, const x2_chunk* chunks, uint32_t n,
+                         uint32_t threads, double* sums, uint32_t* counts, uint64_t counters[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,15 @@
+// x2_bvh.h — the host BVH8 builder's interface (x2_bvh.cpp).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "x2_layout.h"
+
+namespace x2 {
+struct Bvh8Result {
+    std::vector<Node8> nodes;
+    std::vector<TriRecord> tris;
+    uint32_t maxStack = 0, depth = 0;  // traversal stack entries a ray can need; wide-tree depth
+};
+bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err);
+}  // namespace x2

@@ -1,0 +1,1004 @@
+// x2_kernel.hip — the render megakernel for gfx950: one lane = one sample (pixel, index), a wave64 = one 8x8 tile
+// at one sample index (the reference's item order, async_gpu.cu:429-436), so the wave's camera rays are coherent.
+//
+// A sample runs the reference's estimator whole (async_integrator.h:1986-3042, the Lambert/GGX/rect/env subset):
+// camera ray, closest-hit trace through the BVH8, hit reconstruction in FP64, next-event estimation over every
+// light with per-lobe MIS, a BSDF sample per lobe with light MIS and bounce-ray reuse, threshold Russian roulette,
+// and the pending-ray stack with thinning. Its radiance goes to the FP64 film by atomics with an exact count
+// (async_gpu.cu:331-336).
+//
+// Built with -ffp-contract=off and correctly rounded division and square root, so every float expression of the
+// estimator rounds as written, as the host oracle's does. Only the box tests use fused multiply-adds: they are
+// conservative (Ize 2013) and decide what is tested, never the result.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "x2_layout.h"
+#include "x2_rng.h"
+
+namespace x2 {
+
+__constant__ uint32_t kSobolDirections[17 * 32];
+
+// ---------------------------------------------------------------------------------------------------------------
+// Vectors in the reference's operation order (surface_core.h:10-18, shade_math.h:15-26, 65-69, 113-120).
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float b) { return {a.x * b, a.y * b, a.z * b}; }
+__device__ __forceinline__ V3 operator*(float b, V3 a) { return {a.x * b, a.y * b, a.z * b}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 operator/(V3 a, float b) { return {a.x / b, a.y / b, a.z / b}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ bool finite1(float a) { return a == a && a <= 3.402823466e38f && a >= -3.402823466e38f; }
+__device__ __forceinline__ bool finite3(V3 a) { return finite1(a.x) && finite1(a.y) && finite1(a.z); }
+__device__ __forceinline__ bool normalizeTo(V3 a, V3& b) {
+    const float l = dot(a, a);
+    if (!finite1(l) || l <= 0) return false;
+    b = a * (1.0f / sqrtf(l));
+    return true;
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ float luminance(V3 c) { return 0.299f * c.x + 0.587f * c.y + 0.114f * c.z; }
+__device__ __forceinline__ bool isExactlyZero(V3 c) { return c.x == 0 && c.y == 0 && c.z == 0; }
+__device__ __forceinline__ bool isEqual0(float x) { return fabsf(x - 0.0f) <= fmaxf(fabsf(x), 1.0f) * 1e-6f; }
+__device__ __forceinline__ bool isBlack(V3 c) { return isEqual0(c.x) && isEqual0(c.y) && isEqual0(c.z); }
+__device__ __forceinline__ V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+
+constexpr float Pi = 3.14159265358979323846f, TwoPi = 6.28318530717958647692f;
+constexpr float OneOverPi = 0.318309886183790671538f, Epsilon = 1e-6f;
+constexpr float MaxFloat = 3.402823466e+38f;
+
+// The shared sine/cosine polynomial (surface_core.h:116-126, ISPC 1.31's coefficients) for every sin/cos on the
+// path, so the device and the host round alike.
+__device__ __forceinline__ void sincosPoly(float angle, float& sn, float& cs) {
+    const float kf = floorf(angle * .6366197466850281f), x = angle - kf * 1.5707963705062866f, x2 = x * x;
+    const int k = int(kf) & 3;
+    float s = -2.5029327943570934e-8f;
+    s = s * x2 + .000002760012648650445f;
+    s = s * x2 - .00019842604524455965f;
+    s = s * x2 + .008333347737789154f;
+    s = s * x2 - .1666666716337204f;
+    s = (s * x2 + 1) * x;
+    float c = -.0000002596301840185333f;
+    c = c * x2 + .000024756234779488295f;
+    c = c * x2 - .001388833043165505f;
+    c = c * x2 + .04166664183139801f;
+    c = c * x2 - .5f;
+    c = c * x2 + 1;
+    sn = k == 0 ? s : k == 1 ? c : k == 2 ? -s : -c;
+    cs = k == 0 ? c : k == 1 ? -s : k == 2 ? -c : s;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Camera (camera_core.h:53-67, :217-250 without depth of field; surface_core.h:51-52 box filter).
+struct Ray {
+    V3 origin, direction;
+    float nearT, farT;
+};
+__device__ __forceinline__ V3 vectorM(const float* m, V3 a) {
+    return {(a.x * m[0] + a.y * m[4]) + a.z * m[8], (a.x * m[1] + a.y * m[5]) + a.z * m[9],
+            (a.x * m[2] + a.y * m[6]) + a.z * m[10]};
+}
+__device__ __forceinline__ bool cameraRay(const x2_config& c, float x, float y, Ray& out) {
+    const float* m = c.raster_to_camera;
+    const V3 p{x, y, -1.0f};
+    const V3 v = vectorM(m, p) + V3{m[12], m[13], m[14]};
+    const float w = ((p.x * m[3] + p.y * m[7]) + p.z * m[11]) + m[15];
+    const V3 dir = w == 0.0f ? V3{0, 0, 0} : V3{v.x / w, v.y / w, v.z / w};
+    V3 n;
+    if (!normalizeTo(dir, n) || n.z >= 0) return false;
+    const float* q = c.camera_to_render;
+    Ray r;
+    r.origin = vectorM(q, V3{0, 0, 0}) + V3{q[12], q[13], q[14]};
+    r.nearT = (-c.near_clip - 0.0f) / n.z;
+    r.farT = (-c.far_clip - 0.0f) / n.z;
+    if (!normalizeTo(vectorM(q, dir), r.direction) || !finite3(r.origin) || !finite1(r.nearT) || !finite1(r.farT))
+        return false;
+    out = r;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013). The permutation and shear are per ray; the edge
+// functions fall back to double only when they round to zero.
+struct WoopRay {
+    V3 org;
+    int kx, ky, kz;
+    float sx, sy, sz;
+    float nearT, farT;
+};
+__device__ __forceinline__ float comp(V3 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : v.z; }
+__device__ __forceinline__ WoopRay woopSetup(V3 org, V3 dir, float nearT, float farT) {
+    WoopRay w;
+    w.org = org;
+    const float ax = fabsf(dir.x), ay = fabsf(dir.y), az = fabsf(dir.z);
+    w.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    w.kx = w.kz == 2 ? 0 : w.kz + 1;
+    w.ky = w.kx == 2 ? 0 : w.kx + 1;
+    if (comp(dir, w.kz) < 0.0f) {
+        const int t = w.kx;
+        w.kx = w.ky;
+        w.ky = t;
+    }
+    const float dz = comp(dir, w.kz);
+    w.sx = comp(dir, w.kx) / dz;
+    w.sy = comp(dir, w.ky) / dz;
+    w.sz = 1.0f / dz;
+    w.nearT = nearT;
+    w.farT = farT;
+    return w;
+}
+__device__ __forceinline__ float edgeFunction(float a, float b, float c, float d) {
+    const float e = a * b - c * d;
+    if (e != 0.0f) return e;
+    return float(double(a) * double(b) - double(c) * double(d));
+}
+__device__ __forceinline__ bool woopHit(const WoopRay& w, V3 a, V3 b, V3 c, float& t) {
+    const V3 A = a - w.org, B = b - w.org, C = c - w.org;
+    const float Az = comp(A, w.kz), Bz = comp(B, w.kz), Cz = comp(C, w.kz);
+    const float Ax = comp(A, w.kx) - w.sx * Az, Ay = comp(A, w.ky) - w.sy * Az;
+    const float Bx = comp(B, w.kx) - w.sx * Bz, By = comp(B, w.ky) - w.sy * Bz;
+    const float Cx = comp(C, w.kx) - w.sx * Cz, Cy = comp(C, w.ky) - w.sy * Cz;
+    const float U = edgeFunction(Cx, By, Cy, Bx), V = edgeFunction(Ax, Cy, Ay, Cx), W = edgeFunction(Bx, Ay, By, Ax);
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = (U * (w.sz * Az) + V * (w.sz * Bz)) + W * (w.sz * Cz);
+    t = T / det;
+    return t >= w.nearT && t <= w.farT;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// BVH8 traversal. Closest hit keeps the least (t, geom, prim) key (triangle_hit.h:26 order), so the result does not
+// depend on the order nodes are visited in. Boxes are culled conservatively: the far distance is scaled by
+// 1 + 2 gamma(3) (Ize 2013) and compared against the best distance with a relative slack of 2^-12.
+constexpr float FarScale = 1.0f + 2.0f * (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
+constexpr float Slack = 1.0f + 0x1p-12f;
+
+struct Counters {
+    uint32_t traces = 0, shadows = 0, nodes = 0, triangles = 0;
+};
+
+struct BoxRay {
+    float ix, iy, iz, ox, oy, oz;  // 1/d and -o/d
+    bool sx, sy, sz;               // d >= 0 per axis: which plane is near
+};
+__device__ __forceinline__ BoxRay boxRay(V3 o, V3 d) {
+    BoxRay b;
+    const float dx = d.x != 0.0f ? d.x : copysignf(1e-30f, d.x), dy = d.y != 0.0f ? d.y : copysignf(1e-30f, d.y),
+                dz = d.z != 0.0f ? d.z : copysignf(1e-30f, d.z);
+    b.ix = 1.0f / dx;
+    b.iy = 1.0f / dy;
+    b.iz = 1.0f / dz;
+    b.ox = -o.x * b.ix;
+    b.oy = -o.y * b.iy;
+    b.oz = -o.z * b.iz;
+    b.sx = dx >= 0.0f;
+    b.sy = dy >= 0.0f;
+    b.sz = dz >= 0.0f;
+    return b;
+}
+
+__device__ __forceinline__ void cswap(unsigned long long& a, unsigned long long& b) {
+    const unsigned long long lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+// Tests a node's 8 children; returns the entry distance (>= 0) packed over the child reference, ~0 for a miss.
+__device__ __forceinline__ void nodeChildren(const Node8& n, const BoxRay& r, float lo, float hi,
+                                             unsigned long long key[8]) {
+    const float4* q = reinterpret_cast<const float4*>(&n);
+    float lox[8], hix[8], loy[8], hiy[8], loz[8], hiz[8];
+    uint32_t ch[8];
+    {
+        float4 v;
+#define X2_LOAD8(dst, off)            \
+    v = q[off];                       \
+    dst[0] = v.x;                     \
+    dst[1] = v.y;                     \
+    dst[2] = v.z;                     \
+    dst[3] = v.w;                     \
+    v = q[off + 1];                   \
+    dst[4] = v.x;                     \
+    dst[5] = v.y;                     \
+    dst[6] = v.z;                     \
+    dst[7] = v.w;
+        X2_LOAD8(lox, 0)
+        X2_LOAD8(hix, 2)
+        X2_LOAD8(loy, 4)
+        X2_LOAD8(hiy, 6)
+        X2_LOAD8(loz, 8)
+        X2_LOAD8(hiz, 10)
+#undef X2_LOAD8
+        const uint4* u = reinterpret_cast<const uint4*>(&n.child[0]);
+        const uint4 c0 = u[0], c1 = u[1];
+        ch[0] = c0.x, ch[1] = c0.y, ch[2] = c0.z, ch[3] = c0.w, ch[4] = c1.x, ch[5] = c1.y, ch[6] = c1.z, ch[7] = c1.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float nx = __builtin_fmaf(r.sx ? lox[i] : hix[i], r.ix, r.ox);
+        const float fx = __builtin_fmaf(r.sx ? hix[i] : lox[i], r.ix, r.ox);
+        const float ny = __builtin_fmaf(r.sy ? loy[i] : hiy[i], r.iy, r.oy);
+        const float fy = __builtin_fmaf(r.sy ? hiy[i] : loy[i], r.iy, r.oy);
+        const float nz = __builtin_fmaf(r.sz ? loz[i] : hiz[i], r.iz, r.oz);
+        const float fz = __builtin_fmaf(r.sz ? hiz[i] : loz[i], r.iz, r.oz);
+        const float tn = fmaxf(fmaxf(nx, ny), nz);
+        float tf = fminf(fminf(fx, fy), fz);
+        tf = tf * (tf > 0.0f ? FarScale : 1.0f / FarScale);
+        const bool hit = ch[i] != EmptyChild && tn <= tf && tn <= hi && tf >= lo;
+        key[i] = hit ? ((unsigned long long)__float_as_uint(fmaxf(tn, 0.0f)) << 32) | ch[i] : ~0ull;
+    }
+}
+
+__device__ __forceinline__ bool keyLess(float t, uint32_t g, uint32_t p, float bt, uint32_t bg, uint32_t bp) {
+    return t < bt || (t == bt && (g < bg || (g == bg && p < bp)));
+}
+
+// Closest hit among triangles with t in [near, far]; returns the triangle record index (~0u: none).
+__device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, float& tOut,
+                               Counters& k) {
+    const WoopRay w = woopSetup(org, dir, nearT, farT);
+    const BoxRay br = boxRay(org, dir);
+    const float lo = nearT > 0 ? nearT / Slack : nearT * Slack;
+    uint32_t best = ~0u, bg = ~0u, bp = ~0u;
+    float bestT = farT;
+    uint32_t stackRef[TraversalStack];
+    float stackT[TraversalStack];
+    int sp = 0;
+    uint32_t ref = 0;
+    for (;;) {
+        if (ref & LeafBit) {
+            const uint32_t first = (ref & 0x7fffffffu) >> 3, count = (ref & 7u) + 1;
+            for (uint32_t i = 0; i < count; ++i) {
+                const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
+                const float4 a = t4[0], b = t4[1], c = t4[2];
+                ++k.triangles;
+                float t;
+                if (!woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) continue;
+                const uint32_t g = __float_as_uint(a.w), p = __float_as_uint(b.w);
+                if (best == ~0u || keyLess(t, g, p, bestT, bg, bp)) {
+                    best = first + i;
+                    bestT = t;
+                    bg = g;
+                    bp = p;
+                }
+            }
+        } else {
+            ++k.nodes;
+            const float hi = best == ~0u ? (farT > 0 ? farT * Slack : farT / Slack) : (bestT > 0 ? bestT * Slack : bestT / Slack);
+            unsigned long long key[8];
+            nodeChildren(P.nodes[ref], br, lo, hi, key);
+            // Sort the 8 keys (19-comparator network): nearest first.
+            cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[4], key[5]); cswap(key[6], key[7]);
+            cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[4], key[6]); cswap(key[5], key[7]);
+            cswap(key[1], key[2]); cswap(key[5], key[6]); cswap(key[0], key[4]); cswap(key[3], key[7]);
+            cswap(key[1], key[5]); cswap(key[2], key[6]);
+            cswap(key[1], key[4]); cswap(key[3], key[6]);
+            cswap(key[2], key[4]); cswap(key[3], key[5]);
+            cswap(key[3], key[4]);
+#pragma unroll
+            for (int i = 7; i >= 1; --i)
+                if (key[i] != ~0ull) {
+                    stackRef[sp] = uint32_t(key[i]);
+                    stackT[sp] = __uint_as_float(uint32_t(key[i] >> 32));
+                    ++sp;
+                }
+            if (key[0] != ~0ull) {
+                ref = uint32_t(key[0]);
+                continue;
+            }
+        }
+        // Pop the nearest entry still worth visiting.
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            const float bound = best == ~0u ? farT : bestT;
+            if (stackT[sp] <= (bound > 0 ? bound * Slack : bound / Slack)) {
+                ref = stackRef[sp];
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    tOut = bestT;
+    return best;
+}
+
+// Any hit with t in [near, far] (a shadow ray; TERMINATE_ON_FIRST_HIT, async_gpu_device.h:521-533).
+__device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, Counters& k) {
+    const WoopRay w = woopSetup(org, dir, nearT, farT);
+    const BoxRay br = boxRay(org, dir);
+    const float lo = nearT > 0 ? nearT / Slack : nearT * Slack, hi = farT > 0 ? farT * Slack : farT / Slack;
+    uint32_t stackRef[TraversalStack];
+    int sp = 0;
+    uint32_t ref = 0;
+    for (;;) {
+        if (ref & LeafBit) {
+            const uint32_t first = (ref & 0x7fffffffu) >> 3, count = (ref & 7u) + 1;
+            for (uint32_t i = 0; i < count; ++i) {
+                const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
+                const float4 a = t4[0], b = t4[1], c = t4[2];
+                ++k.triangles;
+                float t;
+                if (woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) return true;
+            }
+        } else {
+            ++k.nodes;
+            unsigned long long key[8];
+            nodeChildren(P.nodes[ref], br, lo, hi, key);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (key[i] != ~0ull) stackRef[sp++] = uint32_t(key[i]);
+        }
+        if (sp == 0) return false;
+        ref = stackRef[--sp];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Hit reconstruction (resume_core.h:17-47, triangle_hit.h:6-21): FP64 t, u, v from the float vertices and the ray,
+// the point rounded once, conservative offset origins above and below the surface.
+struct Hit {
+    uint32_t geom, tri;
+    float t;
+    V3 normal, origin, below, point;
+};
+__device__ bool geometryHit(Hit& h, const Ray& ray, V3 a, V3 b, V3 c) {
+    const V3 e1 = b + (a * -1.0f), e2 = c + (a * -1.0f);
+    if (!normalizeTo(cross(e1, e2), h.normal)) return false;
+    if (dot(h.normal, ray.direction) > 0) h.normal = h.normal * -1.0f;
+    const double ax = a.x, ay = a.y, az = a.z;
+    const double ex = double(b.x) - ax, ey = double(b.y) - ay, ez = double(b.z) - az;
+    const double fx = double(c.x) - ax, fy = double(c.y) - ay, fz = double(c.z) - az;
+    const double dx = ray.direction.x, dy = ray.direction.y, dz = ray.direction.z;
+    const double px = dy * fz - dz * fy, py = dz * fx - dx * fz, pz = dx * fy - dy * fx;
+    const double determinant = (ex * px + ey * py) + ez * pz;
+    if (determinant == 0) return false;
+    const double tx = double(ray.origin.x) - ax, ty = double(ray.origin.y) - ay, tz = double(ray.origin.z) - az;
+    const double qx = ty * ez - tz * ey, qy = tz * ex - tx * ez, qz = tx * ey - ty * ex;
+    const double pu = ((tx * px + ty * py) + tz * pz) / determinant;
+    const double pv = ((dx * qx + dy * qy) + dz * qz) / determinant;
+    const double pt = ((fx * qx + fy * qy) + fz * qz) / determinant;
+    if (!finite1(float(pt)) || !finite1(float(pu)) || !finite1(float(pv))) return false;
+    h.t = float(pt);
+    const V3 p{float(ax + ((double(b.x) - ax) * pu + (double(c.x) - ax) * pv)),
+               float(ay + ((double(b.y) - ay) * pu + (double(c.y) - ay) * pv)),
+               float(az + ((double(b.z) - az) * pu + (double(c.z) - az) * pv))};
+    h.point = p;
+    const V3 error{fabsf(a.x) + fabsf(e1.x) + fabsf(e2.x), fabsf(a.y) + fabsf(e1.y) + fabsf(e2.y),
+                   fabsf(a.z) + fabsf(e1.z) + fabsf(e2.z)};
+    const float extent = fmaxf(fmaxf(fabsf(e1.x), fabsf(e1.y)),
+                               fmaxf(fabsf(e1.z), fmaxf(fabsf(e2.x), fmaxf(fabsf(e2.y), fabsf(e2.z)))));
+    const float d = 16 * 0x1p-23f *
+                    (fabsf(h.normal.x) * error.x + fabsf(h.normal.y) * error.y + fabsf(h.normal.z) * error.z + extent);
+    h.origin = p + h.normal * d;
+    if (h.normal.x != 0) h.origin.x = nextafterf(h.origin.x, h.normal.x > 0 ? MaxFloat : -MaxFloat);
+    if (h.normal.y != 0) h.origin.y = nextafterf(h.origin.y, h.normal.y > 0 ? MaxFloat : -MaxFloat);
+    if (h.normal.z != 0) h.origin.z = nextafterf(h.origin.z, h.normal.z > 0 ? MaxFloat : -MaxFloat);
+    h.below = p + h.normal * -d;
+    if (h.normal.x != 0) h.below.x = nextafterf(h.below.x, h.normal.x > 0 ? -MaxFloat : MaxFloat);
+    if (h.normal.y != 0) h.below.y = nextafterf(h.below.y, h.normal.y > 0 ? -MaxFloat : MaxFloat);
+    if (h.normal.z != 0) h.below.z = nextafterf(h.below.z, h.normal.z > 0 ? -MaxFloat : MaxFloat);
+    return finite3(h.origin) && finite3(h.below);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Lights (light_core.h): rectangle and constant environment.
+constexpr float InfiniteDistance = 3.4028234663852886e+38f - 1e32f;
+constexpr float EnvDistance = InfiniteDistance * 0.9f;
+constexpr float HitEpsilonEnd = 1.0f - 1e-3f;
+struct LightHit {
+    V3 N;
+    float distance;
+};
+__device__ __forceinline__ V3 rows(const float* m, V3 v) {
+    return ((ld3(m) * v.x) + (ld3(m + 3) * v.y)) + (ld3(m + 6) * v.z);
+}
+__device__ __forceinline__ V3 polar(float cosTheta, float sinTheta, float phi) {
+    float s, c;
+    sincosPoly(phi, s, c);
+    return {sinTheta * c, sinTheta * s, cosTheta};
+}
+__device__ __forceinline__ float planeDistance(const x2_light& l, V3 p) {
+    float d = dot(ld3(l.direction), p) + l.plane_offset;
+    if (l.sidedness == X2_SIDE_REVERSE) d = -d;
+    else if (l.sidedness == X2_SIDE_TWO) d = fabsf(d);
+    return d;
+}
+__device__ __forceinline__ float areaToSolidAngleScale(V3 wi, V3 n, float dist) {
+    const float denom = -dot(wi, n);
+    return denom < 0.00001f ? 0.0f : (dist * dist) / denom;
+}
+__device__ bool lightSample(const x2_light& l, V3 p, bool cull, V3 n, const float r[3], V3& wi, LightHit& h) {
+    if (l.type == X2_LIGHT_ENV) {
+        if (l.hemispherical) {
+            const float cosTheta = r[0] > 1.0f ? 1.0f : r[0];
+            wi = rows(l.local_to_render, polar(cosTheta, sqrtf(1.0f - cosTheta * cosTheta), r[1] * TwoPi));
+        } else {
+            const float cosTheta = 1.0f - 2.0f * r[0];
+            wi = polar(cosTheta, sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta)), r[1] * TwoPi);
+        }
+        if (cull && dot(n, wi) < Epsilon) return false;
+        h.N = wi * -1.0f;
+        h.distance = EnvDistance;
+        return true;
+    }
+    if (l.type != X2_LIGHT_RECT) return false;
+    const float distance = planeDistance(l, p);
+    if (l.sidedness != X2_SIDE_TWO && distance < Epsilon) return false;
+    const V3 local{(0.5f - r[0]) * (2 * l.half_width), (0.5f - r[1]) * (2 * l.half_height), 0.0f};
+    wi = (rows(l.local_to_render, local) + ld3(l.position)) - p;
+    h.distance = sqrtf(dot(wi, wi));
+    if (h.distance < Epsilon) return false;
+    wi = wi * (1.0f / h.distance);
+    if (cull && dot(n, wi) < Epsilon) return false;
+    const V3 dir = ld3(l.direction);
+    h.N = dot(dir, wi) < 0.0f ? dir : dir * -1.0f;
+    return true;
+}
+__device__ __forceinline__ V3 lightEval(const x2_light& l, V3 wi, const LightHit& h, float& pdf) {
+    const V3 radiance = ld3(l.radiance) * V3{1, 1, 1};
+    pdf = l.inv_area;
+    if (l.type != X2_LIGHT_ENV) pdf *= areaToSolidAngleScale(wi, h.N, h.distance);
+    return radiance;
+}
+__device__ bool lightIntersect(const x2_light& l, V3 p, V3 wi, float maxDistance, LightHit& h) {
+    if (l.type == X2_LIGHT_ENV) {
+        if (l.hemispherical && dot(wi, ld3(l.direction)) < 0.0f) return false;
+        if (EnvDistance > maxDistance) return false;
+        h.N = wi * -1.0f;
+        h.distance = EnvDistance;
+        return true;
+    }
+    if (l.type != X2_LIGHT_RECT) return false;
+    const V3 localWi = rows(l.render_to_local_rot, wi), localP = rows(l.render_to_local, p - ld3(l.position));
+    if ((localP.z < 0.0f) == (localWi.z < 0.0f)) return false;
+    if (localP.z < 0.0f ? l.sidedness == X2_SIDE_REGULAR : l.sidedness == X2_SIDE_REVERSE) return false;
+    const float localDistance = -localP.z / localWi.z, renderDistance = l.scale * localDistance;
+    if (renderDistance > maxDistance) return false;
+    const V3 hit = localP + localWi * localDistance;
+    if (fabsf(hit.x) >= l.half_width || fabsf(hit.y) >= l.half_height) return false;
+    const V3 dir = ld3(l.direction);
+    h.N = localWi.z < 0.0f ? dir : dir * -1.0f;
+    h.distance = renderDistance;
+    return true;
+}
+__device__ bool canIlluminate(const x2_light& l, V3 p, bool cull, V3 n) {
+    const float threshold = Epsilon - 0.0f;
+    if (l.type == X2_LIGHT_ENV) return true;
+    if (l.sidedness != X2_SIDE_TWO && planeDistance(l, p) < threshold) return false;
+    if (!cull) return true;
+    const float hx = l.half_width, hy = l.half_height;
+    const float cx[4] = {hx, hx, -hx, -hx}, cy[4] = {hy, -hy, hy, -hy};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (dot(n, (rows(l.local_to_render, V3{cx[i], cy[i], 0.0f}) + ld3(l.position)) - p) > threshold) return true;
+    return false;
+}
+__device__ __forceinline__ float powerHeuristic(float f, float g) { return (f * f) / (f * f + g * g); }
+
+// ---------------------------------------------------------------------------------------------------------------
+// Lobes: Lambert and GGX (shade_lobes.h:481-556; shade_lobes_eval.h:11, 93-146, 609-668; shade_math.h:135-262).
+constexpr uint32_t Reflection = 1u, Transmission = 2u, Diffuse = 4u, Glossy = 16u, Mirror = 64u;
+constexpr uint32_t AllSurfaceSides = Reflection | Transmission, AllLobes = Diffuse | Glossy | Mirror;
+constexpr uint32_t AllLobeTypes = AllSurfaceSides | AllLobes;
+__device__ __forceinline__ bool matchesFlags(uint32_t type, uint32_t flags) {
+    return (type & AllSurfaceSides & flags) && (type & AllLobes & flags);
+}
+__device__ __forceinline__ uint32_t lobeRayMask(uint32_t t) {
+    uint32_t mask = 0;
+    if (t & 1u) mask |= t & 0x54u;
+    if (t & 2u) mask |= (t & 0x54u) << 1;
+    return mask;
+}
+struct Frame {
+    V3 x, y, z;
+};
+__device__ __forceinline__ Frame frameOf(V3 N) {
+    const float sign = copysignf(1.0f, N.z), a = -1.0f / (sign + N.z), b = N.x * N.y * a;
+    return {{1.0f + sign * N.x * N.x * a, sign * b, -sign * N.x}, {b, sign + N.y * N.y * a, -N.y}, N};
+}
+__device__ __forceinline__ V3 localToGlobal(const Frame& f, V3 v) { return (f.x * v.x + f.y * v.y) + f.z * v.z; }
+__device__ __forceinline__ V3 localSpherical(float cosTheta, float sinTheta, float phi) {
+    float s, c;
+    sincosPoly(phi, s, c);
+    return {sinTheta * c, sinTheta * s, cosTheta};
+}
+// A hit's Bsdf: lobe 0 Lambert (albedo * diffuse weight), lobe 1 GGX (scale = specular weight), either omitted at
+// weight 0. Both share the shading frame.
+struct Bsdf {
+    uint32_t count, type;
+    uint32_t kind[2], ltype[2];
+    V3 albedo, scale;  // Lambert's albedo; GGX's scale
+    float roughness;   // GGX: max(0.001, r^2)
+    Frame frame;
+    V3 emission;
+};
+__device__ __forceinline__ V3 evalLambert(const Bsdf& b, V3 wi, float& pdf) {
+    const V3 N = b.frame.z;
+    const float cosWi = fmaxf(dot(N, wi), 0.0f);
+    pdf = cosWi * OneOverPi;
+    return (((1.0f * b.albedo) * V3{1, 1, 1}) * OneOverPi) * cosWi;
+}
+__device__ __forceinline__ float widenRoughness(float roughness, float cosNO) {
+    return (1.2f - 0.2f * sqrtf(cosNO)) * roughness;
+}
+__device__ V3 evalGGX(const Bsdf& b, V3 wo, V3 wi, float& pdf) {
+    const V3 N = b.frame.z;
+    const float cosNO = dot(N, wo), cosNI = dot(N, wi), roughness = b.roughness;
+    pdf = 0.0f;
+    if (cosNO <= 0.0f || cosNI <= 0.0f) return {0, 0, 0};
+    V3 m = wo + wi;
+    {
+        const float l = length(m);
+        if (!(l > 0.001f)) return {0, 0, 0};
+        m = m * (1.0f / l);
+    }
+    const float cosMI = dot(m, wi);
+    if (cosMI <= 0.0f) return {0, 0, 0};
+    const float cosThetaM = dot(m, N);
+    if (cosThetaM <= 0.0f) return {0, 0, 0};
+    float alpha2 = roughness * roughness;
+    const float cosThetaM2 = cosThetaM * cosThetaM, piCosThetaM4 = Pi * cosThetaM2 * cosThetaM2;
+    const float tanThetaM2 = (1.0f - cosThetaM2) / cosThetaM2;
+    float tmp = alpha2 + tanThetaM2, D4 = 0.25f * alpha2 / (piCosThetaM4 * tmp * tmp);
+    const float invCosNO = 1.0f / cosNO, invCosNI = 1.0f / cosNI;
+    const float G1o = 2.0f / (1.0f + sqrtf(1.0f + alpha2 * (1.0f - cosNO * cosNO) * (invCosNO * invCosNO)));
+    const float G1i = 2.0f / (1.0f + sqrtf(1.0f + alpha2 * (1.0f - cosNI * cosNI) * (invCosNI * invCosNI)));
+    const V3 fr = b.scale * G1o * G1i * D4 * invCosNO * 1.0f;
+    const float w2 = 0.0f;
+    const float alpha = widenRoughness(roughness, cosNO);
+    alpha2 = alpha * alpha;
+    tmp = alpha2 + tanThetaM2;
+    D4 = 0.25f * alpha2 / (piCosThetaM4 * tmp * tmp);
+    const float p2 = 0.0f, p1 = D4 * cosThetaM / cosMI;
+    pdf = (1.0f - w2) * p1 + w2 * p2;
+    return 1.0f * fr;
+}
+__device__ __forceinline__ V3 lobeEval(const Bsdf& b, uint32_t q, V3 wo, V3 wi, float& pdf) {
+    return b.kind[q] == 0 ? evalLambert(b, wi, pdf) : evalGGX(b, wo, wi, pdf);
+}
+__device__ V3 lobeSample(const Bsdf& b, uint32_t q, V3 wo, float r1, float r2, V3& wi, float& pdf) {
+    if (b.kind[q] == 0) {
+        const float cosTheta = r1 > 1.0f ? 1.0f : sqrtf(r1), sinTheta = sqrtf(1.0f - r1);
+        wi = localToGlobal(b.frame, localSpherical(cosTheta, sinTheta, r2 * TwoPi));
+        return evalLambert(b, wi, pdf);
+    }
+    const float cosNO = dot(b.frame.z, wo);
+    if (cosNO <= 0.0f) {
+        wi = {0, 0, 0};
+        pdf = 0.0f;
+        return {0, 0, 0};
+    }
+    const float w2 = 0.0f, u1 = (r1 - w2) / (1.0f - w2);
+    const float alpha = widenRoughness(b.roughness, cosNO), alpha2 = alpha * alpha;
+    const float tanThetaM2 = alpha2 * u1 / (1.0f - u1), cosThetaM = 1.0f / sqrtf(1.0f + tanThetaM2);
+    const float sinThetaM = cosThetaM * sqrtf(tanThetaM2);
+    const V3 m = localToGlobal(b.frame, localSpherical(cosThetaM, sinThetaM, 2.0f * Pi * r2));
+    const float NdotWo = dot(m, wo);
+    wi = -1.0f * wo + 2.0f * NdotWo * m;
+    return evalGGX(b, wo, wi, pdf);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The estimator (async_integrator.h:1986-3042, the subset above). Pending rays live in the lane's private memory.
+struct Pending {
+    V3 origin, dir, T;
+    uint64_t branch;
+    uint32_t depths;  // depth | nonMirrorDepth << 8 | diffuseDepth << 16 | glossyDepth << 24
+    uint32_t heldLight;
+    float heldFar;
+    V3 held;
+};
+constexpr uint32_t MaxPending = 32, PendingReserve = 16;
+__device__ __forceinline__ float bounceThinning(uint32_t top, uint32_t planned) {
+    const uint32_t room = top < PendingReserve ? PendingReserve - top : 1u;
+    return planned > room ? float(room) / float(planned) : 1.0f;
+}
+__device__ __forceinline__ bool roomForChild(uint32_t top, float u, float& weight) {
+    if (top >= MaxPending) return false;
+    const float chance = float(MaxPending - top) / float(PendingReserve + 1);
+    if (u >= chance) return false;
+    weight = 1.0f / chance;
+    return true;
+}
+__device__ __forceinline__ bool roulette(float lum, float threshold, float u, float& scale) {
+    scale = 1;
+    if (lum >= threshold) return true;
+    if (lum < u * threshold) return false;
+    scale = threshold / lum;
+    return true;
+}
+__device__ __forceinline__ V3 smartClamp(V3 a, float clamp) {
+    if (clamp <= 0) return a;
+    const float m = fmaxf(a.x, fmaxf(a.y, a.z));
+    return m > clamp ? a * (clamp / m) : a;
+}
+__device__ __forceinline__ uint32_t surfaceFlags(V3 Ng, V3 wo, V3 wi) {
+    return dot(Ng, wo) * dot(Ng, wi) > 0.0f ? (AllLobeTypes & ~Transmission) : (AllLobeTypes & ~Reflection);
+}
+
+__device__ bool traceHit(const RenderParams& P, const Ray& ray, Hit& h, Counters& k) {
+    ++k.traces;
+    float t;
+    const uint32_t tri = closestHit(P, ray.origin, ray.direction, ray.nearT, ray.farT, t, k);
+    h.geom = ~0u;
+    if (tri == ~0u) return true;
+    const TriRecord& r = P.tris[tri];
+    h.geom = r.geom;
+    h.tri = tri;
+    return geometryHit(h, ray, V3{r.ax, r.ay, r.az}, V3{r.bx, r.by, r.bz}, V3{r.cx, r.cy, r.cz});
+}
+__device__ __forceinline__ bool visible(const RenderParams& P, V3 o, V3 d, float nearT, float farT, Counters& k) {
+    ++k.shadows;
+    return !anyHit(P, o, d, nearT, farT, k);
+}
+
+// One sample; 0 Done (rgba set), 2 Error.
+__device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, double rgba[4], Counters& k) {
+    const x2_config& c = P.cfg;
+    const Identity root{pixel, index, c.seed, 0};
+    float x, y;
+    {
+        const float r0 = draw(root, 0, c.sampling), r1 = draw(root, 1, c.sampling), w = c.filter_width;
+        if (!finite1(w) || w <= 0) return 2;
+        x = (r0 - .5f) * w + .5f;
+        y = (r1 - .5f) * w + .5f;
+        if (!finite1(x) || !finite1(y)) return 2;
+    }
+    Ray camera;
+    if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, camera)) return 2;
+    Pending stack[MaxPending];
+    uint32_t top = 0;
+    {
+        Pending& p = stack[top++];
+        p.origin = camera.origin;
+        p.dir = camera.direction;
+        p.T = {1, 1, 1};
+        p.branch = 0;
+        p.depths = 0;
+        p.heldLight = ~0u;
+        p.heldFar = 0;
+        p.held = {0, 0, 0};
+    }
+    V3 radiance{0, 0, 0};
+    float alpha = 0.0f;
+    const uint32_t rrMin = c.rr_min_depth > 1 ? c.rr_min_depth : 1u;
+    while (top) {
+        const Pending pv = stack[--top];
+        const uint32_t depth = pv.depths & 0xffu, nonMirrorDepth = (pv.depths >> 8) & 0xffu,
+                       diffuseDepth = (pv.depths >> 16) & 0xffu, glossyDepth = pv.depths >> 24;
+        const bool fromCamera = depth == 0;
+        const Ray ray{pv.origin, pv.dir, fromCamera ? camera.nearT : 0.0f, fromCamera ? camera.farT : MaxFloat};
+        Hit hit;
+        if (!traceHit(P, ray, hit, k)) return 2;
+        if (pv.heldLight != ~0u) {
+            const float kept = hit.geom != ~0u && hit.t < pv.heldFar ? 0.0f : 1.0f;
+            if (kept > 0.0f) radiance = radiance + (kept < 1.0f ? pv.held * kept : pv.held);
+        }
+        const float hitAlpha = 1.0f - (0.0f + 0.0f + 0.0f) / 3.0f;
+        if (hit.geom == ~0u) {
+            if (depth == 0) alpha += 1.0f * (1.0f - (1.0f + 1.0f + 1.0f) / 3.0f);
+            continue;
+        }
+        // shade::hitState for a flat triangle: Ng = (b - a) x (c - a), turned to face the ray.
+        const TriRecord& tr = P.tris[hit.tri];
+        const V3 a{tr.ax, tr.ay, tr.az}, b{tr.bx, tr.by, tr.bz}, cc{tr.cx, tr.cy, tr.cz};
+        V3 ng = cross(b - a, cc - a);
+        float l = length(ng);
+        if (!(l > 0)) {
+            const double ux = double(b.x) - a.x, uy = double(b.y) - a.y, uz = double(b.z) - a.z;
+            const double vx = double(cc.x) - a.x, vy = double(cc.y) - a.y, vz = double(cc.z) - a.z;
+            const double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+            const double dl = sqrt(nx * nx + ny * ny + nz * nz);
+            ng = dl > 0 ? V3{float(nx / dl), float(ny / dl), float(nz / dl)} : -ray.direction;
+            l = length(ng);
+            if (!(l > 0)) return 2;
+        }
+        V3 Ng = ng / l, N = Ng;
+        const V3 Pt = hit.point, wo = -ray.direction;
+        if (!(dot(Ng, ray.direction) < 0.0f)) {
+            N = -N;
+            Ng = -Ng;
+        }
+        if (!finite3(N) || !finite3(Ng)) return 2;
+        Bsdf bsdf;
+        {
+            const x2_material& m = P.materials[tr.material];
+            bsdf.count = 0;
+            bsdf.type = 0;
+            bsdf.frame = frameOf(N);
+            bsdf.albedo = ld3(m.albedo) * m.diffuse_weight;
+            bsdf.scale = {m.specular_weight, m.specular_weight, m.specular_weight};
+            bsdf.roughness = fmaxf(0.001f, m.roughness * m.roughness);
+            if (m.diffuse_weight > 0.0f) {
+                bsdf.kind[bsdf.count] = 0;
+                bsdf.ltype[bsdf.count++] = Reflection | Diffuse;
+                bsdf.type |= Reflection | Diffuse;
+            }
+            if (m.specular_weight > 0.0f) {
+                bsdf.kind[bsdf.count] = 1;
+                bsdf.ltype[bsdf.count++] = Reflection | Glossy;
+                bsdf.type |= Reflection | Glossy;
+            }
+            bsdf.emission = ld3(m.emission);
+        }
+        if (depth == 0) alpha += 1.0f * hitAlpha;
+        radiance = radiance + pv.T * bsdf.emission;
+        if (bsdf.count == 0) continue;
+        uint32_t indirectFlags = 0;
+        if (c.bsdf_samples > 0 && c.max_depth > 0 && depth < c.max_depth) {
+            if (diffuseDepth < c.max_diffuse_depth) indirectFlags |= AllSurfaceSides | Diffuse;
+            if (glossyDepth < c.max_glossy_depth) indirectFlags |= AllSurfaceSides | Glossy;
+            if (0u < c.max_mirror_depth) indirectFlags |= AllSurfaceSides | Mirror;
+        }
+        const bool doIndirect = indirectFlags != 0;
+        const V3 cullN = (bsdf.type & Reflection) ? Ng : -Ng;
+        const bool cull = (bsdf.type & AllSurfaceSides) != AllSurfaceSides;
+        const bool split = nonMirrorDepth == 0;
+        const uint32_t nl = split ? c.light_samples : 1;
+        const bool clampHere = nonMirrorDepth >= c.clamp_depth;
+        const bool rr = nonMirrorDepth >= rrMin && c.rr_threshold > 0;
+        const uint32_t nk = split ? c.bsdf_samples : 1;  // no mirror lobes
+        const uint32_t planned = nk * bsdf.count;
+        const float thinning = bounceThinning(top, planned);
+        const V3 above = hit.origin, below = hit.below;
+
+        // Light samples (:2615-2785).
+        const uint64_t lightBranch = childBranch(pv.branch, 0xfffe, 0);
+        for (uint32_t i = 0; i < c.light_count; ++i) {
+            const x2_light& L = P.lights[i];
+            if (!canIlluminate(L, Pt, cull, cullN)) continue;
+            const uint64_t lightAddress = childBranch(lightBranch, i, 0);
+            Identity lid = root;
+            lid.branch = lightAddress;
+            Stream ls(lid, depth);
+            for (uint32_t s = 0; s < nl; ++s) {
+                float r[3];
+                r[0] = ls.next();
+                r[1] = ls.next();
+                r[2] = ls.next();
+                const float u = ls.next();
+                if (split) {
+                    const Identity strat{pixel, uint64_t(index) * nl + s, c.seed, lightAddress};
+                    r[0] = draw(strat, CameraDimensions + 2, c.sampling);
+                    r[1] = draw(strat, CameraDimensions + 3, c.sampling);
+                }
+                V3 wi;
+                LightHit lh;
+                if (!lightSample(L, Pt, cull, cullN, r, wi, lh)) continue;
+                float pdfL;
+                const V3 Li = lightEval(L, wi, lh, pdfL);
+                if (pdfL == 0.0f || isExactlyZero(Li) || !(fabsf(pdfL) < 3.4e38f) || !finite3(Li)) continue;
+                const V3 factor = pv.T * Li * (1.0f / (float(nl) * pdfL * 1.0f));
+                const uint32_t flags = surfaceFlags(Ng, wo, wi);
+                V3 t{0, 0, 0};
+                bool any = false;
+                for (uint32_t q = 0; q < bsdf.count; ++q) {
+                    if (!matchesFlags(bsdf.ltype[q], flags)) continue;
+                    if (!(lobeRayMask(bsdf.ltype[q]) & L.visibility_mask)) continue;
+                    float pdf;
+                    const V3 f = lobeEval(bsdf, q, wo, wi, pdf);
+                    if (pdf == 0.0f || isExactlyZero(f)) continue;
+                    V3 tk = factor * f * powerHeuristic(float(nl) * pdfL, float(nk) * pdf);
+                    if (clampHere) tk = smartClamp(tk, c.clamp_value);
+                    t = t + tk;
+                    any = true;
+                }
+                if (!any) continue;
+                if (rr) {
+                    float scale;
+                    if (!roulette(luminance(t), c.rr_threshold, u, scale)) continue;
+                    t = t * scale;
+                }
+                if (!finite3(t) || isExactlyZero(t)) continue;
+                const V3 shadowOrigin = dot(wi, Ng) >= 0.0f ? above : below;
+                const float shadowNear = 0.0f, shadowFar = fmaxf((lh.distance - 0.0f) * HitEpsilonEnd, 2.0f * shadowNear);
+                if (visible(P, shadowOrigin, wi, shadowNear, shadowFar, k)) radiance = radiance + t;
+            }
+        }
+
+        // BSDF samples (:2787-3031).
+        for (uint32_t q = 0; q < bsdf.count; ++q) {
+            const uint32_t ltype = bsdf.ltype[q];
+            const uint32_t lobeMask = lobeRayMask(ltype);
+            const bool lobeIndirect = matchesFlags(ltype, indirectFlags);
+            const uint64_t lobeBranch = childBranch(pv.branch, q, 0xffff);
+            for (uint32_t i = 0; i < nk; ++i) {
+                const uint64_t sampleBranch = childBranch(lobeBranch, i, 0);
+                Identity bid = root;
+                bid.branch = sampleBranch;
+                Stream bs(bid, depth);
+                float r1 = bs.next(), r2 = bs.next();
+                const float choice = bs.next(), u = bs.next();
+                if (split) {
+                    const Identity strat{pixel, uint64_t(index) * nk + i, c.seed, lobeBranch};
+                    r1 = draw(strat, CameraDimensions, c.sampling);
+                    r2 = draw(strat, CameraDimensions + 1, c.sampling);
+                }
+                V3 wi;
+                float pdf;
+                const V3 f = lobeSample(bsdf, q, wo, r1, r2, wi, pdf);
+                if (!(pdf != 0.0f && !isExactlyZero(f)) || !finite3(f) || !finite3(wi)) continue;
+                if (!matchesFlags(ltype, surfaceFlags(Ng, wo, wi))) continue;
+                const V3 pt = pv.T * f * (1.0f / (float(nk) * pdf));
+                uint32_t hits = 0;
+                for (uint32_t j = 0; j < c.light_count; ++j) {
+                    LightHit lh;
+                    if ((lobeMask & P.lights[j].visibility_mask) && lightIntersect(P.lights[j], Pt, wi, MaxFloat, lh)) ++hits;
+                }
+                V3 direct{0, 0, 0};
+                float lightDistance = 0;
+                bool lightHit = false;
+                uint32_t lightIndex = 0;
+                if (hits) {
+                    const uint32_t chosen = uint32_t(choice * float(hits)) < hits ? uint32_t(choice * float(hits)) : hits - 1;
+                    for (uint32_t j = 0, seen = 0; j < c.light_count; ++j) {
+                        LightHit lh;
+                        if (!(lobeMask & P.lights[j].visibility_mask) || !lightIntersect(P.lights[j], Pt, wi, MaxFloat, lh))
+                            continue;
+                        if (seen++ != chosen) continue;
+                        float pdfL;
+                        const V3 Li = lightEval(P.lights[j], wi, lh, pdfL) * float(hits);
+                        if (!(pdfL == 0.0f || isExactlyZero(Li)) && fabsf(pdfL) < 3.4e38f && finite3(Li)) {
+                            direct = Li * pt * powerHeuristic(float(nk) * pdf, float(nl) * pdfL);
+                            lightDistance = lh.distance;
+                            lightHit = true;
+                            lightIndex = j;
+                        }
+                        break;
+                    }
+                }
+                if (!lightHit && !doIndirect) continue;
+                if (clampHere) direct = smartClamp(direct, c.clamp_value);
+                V3 indirect = doIndirect ? pt : V3{0, 0, 0};
+                if (rr) {
+                    float scale;
+                    if (!roulette(fmaxf(luminance(direct), luminance(indirect)), c.rr_threshold, u, scale)) continue;
+                    direct = direct * scale;
+                    indirect = indirect * scale;
+                }
+                const V3 origin = dot(wi, Ng) >= 0.0f ? above : below;
+                float shadowFar = 0.0f, kept = 0.0f;
+                const bool bounces = lobeIndirect && !isExactlyZero(indirect) && finite3(indirect);
+                bool held = false;
+                if (lightHit && !isBlack(direct) && finite3(direct)) {
+                    shadowFar = fmaxf((lightDistance - 0.0f) * HitEpsilonEnd, 2.0f * 0.0f);
+                    held = bounces && thinning >= 1.0f && top + PendingReserve < MaxPending && lobeMask &&
+                           (c.shadow_reuse & lobeMask) == lobeMask;
+                    if (!held) kept = visible(P, origin, wi, 0.0f, shadowFar, k) ? 1.0f : 0.0f;
+                }
+                const V3 directBefore = direct;
+                if (kept > 0.0f && !held) {
+                    if (kept < 1.0f) direct = direct * kept;
+                    radiance = radiance + direct;
+                }
+                if (!bounces) continue;
+                float keptShare = 1.0f;
+                if (thinning < 1.0f || top + PendingReserve >= MaxPending) {
+                    Identity kid = bid;
+                    kid.branch = childBranch(sampleBranch, 0xfff7, 0);
+                    Stream chance(kid, depth);
+                    if (thinning < 1.0f) {
+                        if (chance.next() >= thinning) continue;
+                        keptShare = 1.0f / thinning;
+                    }
+                    if (top + PendingReserve >= MaxPending) {
+                        float tight = 1.0f;
+                        if (!roomForChild(top, chance.next(), tight)) continue;
+                        keptShare *= tight;
+                    }
+                }
+                Pending& child = stack[top++];
+                child.origin = origin;
+                child.dir = wi;
+                child.T = indirect * keptShare;
+                child.branch = sampleBranch;
+                child.depths = (depth + 1) | ((nonMirrorDepth + 1) << 8) |
+                               ((diffuseDepth + ((ltype & Diffuse) ? 1u : 0u)) << 16) |
+                               ((glossyDepth + ((ltype & Glossy) ? 1u : 0u)) << 24);
+                child.heldLight = held ? lightIndex : ~0u;
+                child.heldFar = shadowFar;
+                child.held = directBefore;
+            }
+        }
+    }
+    rgba[0] = radiance.x;
+    rgba[1] = radiance.y;
+    rgba[2] = radiance.z;
+    rgba[3] = alpha;
+    return 0;
+}
+
+__device__ __forceinline__ uint32_t tilePixel(uint32_t tile, uint32_t position, const x2_config& c) {
+    const uint32_t tilesX = (c.width + 7) / 8;
+    const uint32_t x = (tile % tilesX) * 8 + position % 8, y = (tile / tilesX) * 8 + position / 8;
+    if (!(x < c.width && y < c.height)) return ~0u;
+    if (int(x) < c.viewport[0] || int(y) < c.viewport[1] || int(x) >= c.viewport[2] || int(y) >= c.viewport[3])
+        return ~0u;
+    return y * c.width + x;
+}
+
+__device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
+    unsigned long long s = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
+
+// One thread per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
+// position; the chunk comes from the mailbox ring. Every lane reaches the end (no early return) so the wave's
+// counter sums see all 64 lanes.
+__global__ void __launch_bounds__(256) renderKernel(const RenderParams P) {
+    const unsigned long long item = P.itemBegin + (unsigned long long)blockIdx.x * 256u + threadIdx.x;
+    Counters k;
+    uint32_t errors = 0;
+    if (item < P.itemEnd) {
+        const unsigned long long chunkIndex = item / P.itemsPerChunk;
+        const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
+        const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
+        const uint32_t s = offset / TilePixels, position = offset % TilePixels;
+        const uint32_t pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
+        if (pixel != ~0u) {
+            double rgba[4];
+            if (integrate(P, pixel, chunk.first + s, rgba, k) == 0) {
+                double* f = P.sums + size_t(pixel) * 4;
+                atomicAdd(f + 0, rgba[0]);
+                atomicAdd(f + 1, rgba[1]);
+                atomicAdd(f + 2, rgba[2]);
+                atomicAdd(f + 3, rgba[3]);
+                atomicAdd(P.counts + pixel, 1u);
+            } else {
+                errors = 1;
+            }
+        }
+    }
+    if (errors) atomicAdd(&P.state->errors, errors);
+    if (P.counters) {
+        const unsigned long long t = waveSum(k.traces), sh = waveSum(k.shadows), n = waveSum(k.nodes),
+                                 tr = waveSum(k.triangles);
+        if ((threadIdx.x & 63u) == 0) {
+            atomicAdd(&P.state->traces, t);
+            atomicAdd(&P.state->shadows, sh);
+            atomicAdd(&P.state->nodes, n);
+            atomicAdd(&P.state->triangles, tr);
+        }
+    }
+}
+
+__global__ void clearState(LaunchState* s) {
+    if (threadIdx.x == 0) {
+        s->traces = s->shadows = s->nodes = s->triangles = 0;
+        s->errors = 0;
+    }
+}
+
+}  // namespace x2
+
+// Host-side launchers the lane calls (x2_lane.hip).
+namespace x2 {
+hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
+    const unsigned long long items = p.itemEnd - p.itemBegin;
+    const unsigned blocks = unsigned((items + 255) / 256);
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(renderKernel, dim3(blocks), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+hipError_t launchClearState(LaunchState* s, hipStream_t stream) {
+    hipLaunchKernelGGL(clearState, dim3(1), dim3(64), 0, stream, s);
+    return hipGetLastError();
+}
+hipError_t uploadSobol(const uint32_t* table) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(kSobolDirections), table, sizeof(uint32_t) * 17 * 32, 0, hipMemcpyHostToDevice);
+}
+}  // namespace x2

@@ -1,0 +1,498 @@
+// x2_lane.hip — the device's host side behind the C-ABI (include/x2dev.h): the counterpart of the reference's
+// GpuLane (xpu2/interactive/async_gpu.h:82) over the HIP runtime. One stream; a mailbox ring of chunks in device
+// memory fed from pinned staging; at most two launches in flight, each bracketed by HIP events and followed by a
+// readback of its counters; a film of FP64 sums and u32 counts downloaded to pinned host memory on request. The
+// broker counterpart (x2dev_render_frame) follows Renderer::render's GPU-lane sequence (async_render.h:318-1081).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "x2_bvh.h"
+#include "x2_layout.h"
+
+namespace x2 {
+hipError_t launchRender(const RenderParams& p, hipStream_t stream);
+hipError_t launchClearState(LaunchState* s, hipStream_t stream);
+hipError_t uploadSobol(const uint32_t* table);
+}  // namespace x2
+
+namespace {
+using namespace x2;
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& what) {
+    g_error = what;
+    return code;
+}
+int hipFail(hipError_t e, const char* where) {
+    return fail(X2_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+#define X2_HIP(call)                                   \
+    do {                                               \
+        const hipError_t e_ = (call);                  \
+        if (e_ != hipSuccess) return hipFail(e_, #call); \
+    } while (0)
+
+// Sobol direction numbers (sample_stream.h:82-100) from the Joe-Kuo primitive polynomials (new-joe-kuo-6.21201,
+// dimensions 2..17: degree s, coefficient bits a, initial m), by the standard recurrence; row 0 is van der Corput.
+void sobolDirections(uint32_t out[17 * 32]) {
+    static const uint32_t s[16] = {1, 2, 3, 3, 4, 4, 5, 5, 5, 5, 5, 5, 6, 6, 6, 6};
+    static const uint32_t a[16] = {0, 1, 1, 2, 1, 4, 2, 4, 7, 11, 13, 14, 1, 13, 16, 19};
+    static const uint32_t m0[16][6] = {{1},          {1, 3},        {1, 3, 1},        {1, 1, 1},
+                                       {1, 1, 3, 3}, {1, 3, 5, 13}, {1, 1, 5, 5, 17}, {1, 1, 5, 5, 5},
+                                       {1, 1, 7, 11, 19}, {1, 1, 5, 1, 1}, {1, 1, 1, 3, 11}, {1, 3, 5, 5, 31},
+                                       {1, 3, 3, 9, 7, 49}, {1, 1, 1, 15, 21, 21}, {1, 3, 1, 13, 27, 49},
+                                       {1, 1, 1, 15, 7, 5}};
+    for (uint32_t k = 0; k < 32; ++k) out[k] = 0x80000000u >> k;
+    for (uint32_t d = 1; d < 17; ++d) {
+        const uint32_t deg = s[d - 1], coef = a[d - 1];
+        uint32_t m[33] = {0};
+        for (uint32_t i = 1; i <= deg; ++i) m[i] = m0[d - 1][i - 1];
+        for (uint32_t i = deg + 1; i <= 32; ++i) {
+            uint32_t v = m[i - deg] ^ (m[i - deg] << deg);
+            for (uint32_t j = 1; j < deg; ++j)
+                if ((coef >> (deg - 1 - j)) & 1u) v ^= m[i - j] << j;
+            m[i] = v;
+        }
+        for (uint32_t k = 1; k <= 32; ++k) out[d * 32 + k - 1] = m[k] << (32 - k);
+    }
+}
+
+struct Flight {
+    uint64_t launch, first, count, samples;
+    uint32_t slot;
+};
+}  // namespace
+
+struct x2dev {
+    int device = 0;
+    x2dev_options opt{};
+    hipStream_t stream = nullptr;
+    uint32_t cus = 0, launchThreads = 0;
+    // scene
+    Node8* nodes = nullptr;
+    TriRecord* tris = nullptr;
+    x2_material* materials = nullptr;
+    x2_light* lights = nullptr;
+    uint32_t nodeCount = 0, triCount = 0, lightCount = 0, materialCount = 0, maxStack = 0;
+    bool prepared = false;
+    // film
+    double* sums = nullptr;
+    uint32_t* counts = nullptr;
+    size_t filmPixels = 0;
+    double* hostSums = nullptr;
+    uint32_t* hostCounts = nullptr;
+    hipEvent_t filmDone = nullptr;
+    bool filmQueued = false;
+    // mailbox ring, staging, per-slot state and readback
+    x2_chunk* ring = nullptr;
+    x2_chunk* staging[Slots] = {};
+    LaunchState* state = nullptr;
+    LaunchState* readback = nullptr;
+    hipEvent_t begun[Slots] = {}, ended[Slots] = {}, done[Slots] = {};
+    std::deque<Flight> inFlight;
+    uint64_t published = 0, launches = 0;
+    // frame
+    x2_config cfg{};
+    uint32_t itemsPerChunk = 0;
+    bool begun_ = false;
+};
+
+namespace {
+void release(x2dev* d) {
+    if (!d) return;
+    (void)hipSetDevice(d->device);
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    (void)hipFree(d->nodes);
+    (void)hipFree(d->tris);
+    (void)hipFree(d->materials);
+    (void)hipFree(d->lights);
+    (void)hipFree(d->sums);
+    (void)hipFree(d->counts);
+    (void)hipFree(d->ring);
+    (void)hipFree(d->state);
+    if (d->hostSums) (void)hipHostFree(d->hostSums);
+    if (d->hostCounts) (void)hipHostFree(d->hostCounts);
+    if (d->readback) (void)hipHostFree(d->readback);
+    for (uint32_t i = 0; i < Slots; ++i) {
+        if (d->staging[i]) (void)hipHostFree(d->staging[i]);
+        if (d->begun[i]) (void)hipEventDestroy(d->begun[i]);
+        if (d->ended[i]) (void)hipEventDestroy(d->ended[i]);
+        if (d->done[i]) (void)hipEventDestroy(d->done[i]);
+    }
+    if (d->filmDone) (void)hipEventDestroy(d->filmDone);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+uint64_t oldestChunk(const x2dev* d) { return d->inFlight.empty() ? d->published : d->inFlight.front().first; }
+
+int validateConfig(const x2dev* d, const x2_config& c) {
+    if (!c.width || !c.height || !c.spp) return fail(X2_ERR_ARGUMENT, "width, height and spp must be positive");
+    if (uint64_t(c.width) * c.height > (1ull << 31)) return fail(X2_ERR_ARGUMENT, "image too large");
+    if (c.bsdf_samples == 0 || c.bsdf_samples > 64 || c.light_samples == 0 || c.light_samples > 64)
+        return fail(X2_ERR_ARGUMENT, "bsdf_samples and light_samples must be 1..64 (async_integrator.h:1989)");
+    if (c.max_depth > 250 || c.max_diffuse_depth > 250 || c.max_glossy_depth > 250)
+        return fail(X2_ERR_ARGUMENT, "depth limits above 250 are not supported");
+    if (c.filter != 0) return fail(X2_ERR_ARGUMENT, "only the box pixel filter is implemented");
+    if (c.light_count > d->lightCount) return fail(X2_ERR_ARGUMENT, "config.light_count exceeds the scene's lights");
+    return X2_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* x2dev_last_error(void) { return g_error.c_str(); }
+
+int x2dev_create(const x2dev_options* options, x2dev** out) {
+    if (!out) return fail(X2_ERR_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    x2dev* d = new x2dev;
+    d->opt.device = options ? options->device : 0;
+    d->opt.max_chunks_per_launch = options && options->max_chunks_per_launch ? options->max_chunks_per_launch : 16384u;
+    d->opt.ring_chunks = options && options->ring_chunks ? options->ring_chunks : (1u << 17);
+    d->opt.counters = options ? options->counters : 1u;
+    d->device = d->opt.device;
+    if (d->opt.ring_chunks < d->opt.max_chunks_per_launch) {
+        delete d;
+        return fail(X2_ERR_ARGUMENT, "ring_chunks must hold at least one launch");
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || d->device < 0 || d->device >= count) {
+        delete d;
+        return fail(X2_ERR_HIP, e != hipSuccess ? hipGetErrorString(e) : "no such HIP device");
+    }
+#define X2_CREATE(call)                 \
+    do {                                \
+        const hipError_t e_ = (call);   \
+        if (e_ != hipSuccess) {         \
+            const int r_ = hipFail(e_, #call); \
+            release(d);                 \
+            return r_;                  \
+        }                               \
+    } while (0)
+    X2_CREATE(hipSetDevice(d->device));
+    hipDeviceProp_t prop;
+    X2_CREATE(hipGetDeviceProperties(&prop, d->device));
+    d->cus = uint32_t(prop.multiProcessorCount);
+    d->launchThreads = uint32_t(prop.multiProcessorCount) * uint32_t(prop.maxThreadsPerMultiProcessor);
+    X2_CREATE(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    X2_CREATE(hipMalloc(&d->ring, size_t(d->opt.ring_chunks) * sizeof(x2_chunk)));
+    X2_CREATE(hipMalloc(&d->state, Slots * sizeof(LaunchState)));
+    X2_CREATE(hipHostMalloc(&d->readback, Slots * sizeof(LaunchState), hipHostMallocDefault));
+    for (uint32_t i = 0; i < Slots; ++i) {
+        X2_CREATE(hipHostMalloc(&d->staging[i], size_t(d->opt.max_chunks_per_launch) * sizeof(x2_chunk), hipHostMallocDefault));
+        X2_CREATE(hipEventCreate(&d->begun[i]));
+        X2_CREATE(hipEventCreate(&d->ended[i]));
+        X2_CREATE(hipEventCreateWithFlags(&d->done[i], hipEventDisableTiming));
+    }
+    X2_CREATE(hipEventCreateWithFlags(&d->filmDone, hipEventDisableTiming));
+    uint32_t table[17 * 32];
+    sobolDirections(table);
+    X2_CREATE(uploadSobol(table));
+#undef X2_CREATE
+    *out = d;
+    return X2_OK;
+}
+
+int x2dev_prepare(x2dev* d, const x2_scene* s) {
+    if (!d || !s) return fail(X2_ERR_ARGUMENT, "null device or scene");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "prepare with launches in flight");
+    if ((s->triangle_count && (!s->positions || !s->indices)) || !s->geom_first || !s->geom_material ||
+        !s->materials || (s->light_count && !s->lights))
+        return fail(X2_ERR_ARGUMENT, "scene arrays missing");
+    for (uint32_t i = 0; i < s->light_count; ++i)
+        if (s->lights[i].type != X2_LIGHT_ENV && s->lights[i].type != X2_LIGHT_RECT)
+            return fail(X2_ERR_ARGUMENT, "only environment and rectangle lights are implemented");
+    Bvh8Result bvh;
+    std::string err;
+    if (!buildBvh8(*s, bvh, err)) return fail(X2_ERR_ARGUMENT, err);
+    if (bvh.maxStack > uint32_t(TraversalStack))
+        return fail(X2_ERR_ARGUMENT, "BVH8 too deep for the traversal stack (" + std::to_string(bvh.maxStack) + ")");
+    X2_HIP(hipSetDevice(d->device));
+    (void)hipFree(d->nodes);
+    (void)hipFree(d->tris);
+    (void)hipFree(d->materials);
+    (void)hipFree(d->lights);
+    d->nodes = nullptr;
+    d->tris = nullptr;
+    d->materials = nullptr;
+    d->lights = nullptr;
+    d->prepared = false;
+    const size_t nb = bvh.nodes.size() * sizeof(Node8), tb = std::max<size_t>(1, bvh.tris.size()) * sizeof(TriRecord);
+    X2_HIP(hipMalloc(&d->nodes, nb));
+    X2_HIP(hipMalloc(&d->tris, tb));
+    X2_HIP(hipMalloc(&d->materials, std::max<size_t>(1, s->material_count) * sizeof(x2_material)));
+    X2_HIP(hipMalloc(&d->lights, std::max<size_t>(1, s->light_count) * sizeof(x2_light)));
+    X2_HIP(hipMemcpy(d->nodes, bvh.nodes.data(), nb, hipMemcpyHostToDevice));
+    if (!bvh.tris.empty()) X2_HIP(hipMemcpy(d->tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice));
+    X2_HIP(hipMemcpy(d->materials, s->materials, s->material_count * sizeof(x2_material), hipMemcpyHostToDevice));
+    if (s->light_count) X2_HIP(hipMemcpy(d->lights, s->lights, s->light_count * sizeof(x2_light), hipMemcpyHostToDevice));
+    d->nodeCount = uint32_t(bvh.nodes.size());
+    d->triCount = uint32_t(bvh.tris.size());
+    d->lightCount = s->light_count;
+    d->materialCount = s->material_count;
+    d->maxStack = bvh.maxStack;
+    d->prepared = true;
+    return X2_OK;
+}
+
+int x2dev_begin(x2dev* d, const x2_config* c, uint32_t samples_per_chunk) {
+    if (!d || !c) return fail(X2_ERR_ARGUMENT, "null device or config");
+    if (!d->prepared) return fail(X2_ERR_STATE, "begin before prepare");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "GPU frame overlap (async_gpu.h:1374)");
+    if (samples_per_chunk == 0 || samples_per_chunk > 1024) return fail(X2_ERR_ARGUMENT, "samples_per_chunk must be 1..1024");
+    const int v = validateConfig(d, *c);
+    if (v) return v;
+    X2_HIP(hipSetDevice(d->device));
+    const size_t pixels = size_t(c->width) * c->height;
+    if (pixels != d->filmPixels) {
+        (void)hipFree(d->sums);
+        (void)hipFree(d->counts);
+        if (d->hostSums) (void)hipHostFree(d->hostSums);
+        if (d->hostCounts) (void)hipHostFree(d->hostCounts);
+        d->sums = nullptr;
+        d->counts = nullptr;
+        d->hostSums = nullptr;
+        d->hostCounts = nullptr;
+        d->filmPixels = 0;
+        X2_HIP(hipMalloc(&d->sums, pixels * 4 * sizeof(double)));
+        X2_HIP(hipMalloc(&d->counts, pixels * sizeof(uint32_t)));
+        X2_HIP(hipHostMalloc(&d->hostSums, pixels * 4 * sizeof(double), hipHostMallocDefault));
+        X2_HIP(hipHostMalloc(&d->hostCounts, pixels * sizeof(uint32_t), hipHostMallocDefault));
+        d->filmPixels = pixels;
+    }
+    X2_HIP(hipMemsetAsync(d->sums, 0, pixels * 4 * sizeof(double), d->stream));
+    X2_HIP(hipMemsetAsync(d->counts, 0, pixels * sizeof(uint32_t), d->stream));
+    d->cfg = *c;
+    d->itemsPerChunk = samples_per_chunk * TilePixels;
+    d->published = 0;
+    d->launches = 0;
+    d->filmQueued = false;
+    d->begun_ = true;
+    return X2_OK;
+}
+
+int x2dev_ready(const x2dev* d) { return d && d->begun_ && d->inFlight.size() < 2 ? 1 : 0; }
+uint32_t x2dev_in_flight(const x2dev* d) { return d ? uint32_t(d->inFlight.size()) : 0u; }
+uint32_t x2dev_max_chunks_per_launch(const x2dev* d) { return d ? d->opt.max_chunks_per_launch : 0u; }
+uint32_t x2dev_launch_threads(const x2dev* d) { return d ? d->launchThreads : 0u; }
+
+int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samples) {
+    if (!d) return fail(X2_ERR_ARGUMENT, "null device");
+    if (!d->begun_) return fail(X2_ERR_STATE, "launch before begin");
+    if (!count) return 0;
+    if (!chunks) return fail(X2_ERR_ARGUMENT, "null chunks");
+    if (d->inFlight.size() >= 2 || count > d->opt.max_chunks_per_launch)
+        return fail(X2_ERR_BACKPRESSURE, "GPU launch backpressure (async_gpu.h:1435)");
+    if (d->published + count - oldestChunk(d) > d->opt.ring_chunks) return fail(X2_ERR_BACKPRESSURE, "mailbox ring full");
+    const uint32_t spc = d->itemsPerChunk / TilePixels;
+    const uint32_t tiles = ((d->cfg.width + 7) / 8) * ((d->cfg.height + 7) / 8);
+    for (uint32_t i = 0; i < count; ++i)
+        if (chunks[i].count > spc || chunks[i].tile >= tiles || chunks[i].first > 0xffffffffu - chunks[i].count)
+            return fail(X2_ERR_ARGUMENT, "chunk out of range (tile, or count above samples_per_chunk)");
+    X2_HIP(hipSetDevice(d->device));
+    const uint32_t slot = uint32_t(d->launches % Slots);
+    std::memcpy(d->staging[slot], chunks, size_t(count) * sizeof(x2_chunk));
+    for (uint32_t done = 0; done < count;) {  // the ring wraps
+        const uint64_t at = (d->published + done) % d->opt.ring_chunks;
+        const uint32_t n = uint32_t(std::min<uint64_t>(count - done, d->opt.ring_chunks - at));
+        X2_HIP(hipMemcpyAsync(d->ring + at, d->staging[slot] + done, size_t(n) * sizeof(x2_chunk), hipMemcpyHostToDevice,
+                              d->stream));
+        done += n;
+    }
+    X2_HIP(launchClearState(d->state + slot, d->stream));
+    RenderParams p;
+    p.cfg = d->cfg;
+    p.nodes = d->nodes;
+    p.tris = d->tris;
+    p.materials = d->materials;
+    p.lights = d->lights;
+    p.ring = d->ring;
+    p.sums = d->sums;
+    p.counts = d->counts;
+    p.state = d->state + slot;
+    p.itemBegin = d->published * d->itemsPerChunk;
+    p.itemEnd = (d->published + count) * d->itemsPerChunk;
+    p.itemsPerChunk = d->itemsPerChunk;
+    p.ringCapacity = d->opt.ring_chunks;
+    p.counters = d->opt.counters;
+    p.pad = 0;
+    X2_HIP(hipEventRecord(d->begun[slot], d->stream));
+    X2_HIP(launchRender(p, d->stream));
+    X2_HIP(hipEventRecord(d->ended[slot], d->stream));
+    X2_HIP(hipMemcpyAsync(d->readback + slot, d->state + slot, sizeof(LaunchState), hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipEventRecord(d->done[slot], d->stream));
+    d->inFlight.push_back(Flight{d->launches, d->published, count, samples, slot});
+    d->published += count;
+    ++d->launches;
+    return 1;
+}
+
+int x2dev_poll(x2dev* d, x2_completion* out) {
+    if (!d || !out) return fail(X2_ERR_ARGUMENT, "null device or completion");
+    if (d->inFlight.empty()) return 0;
+    const Flight f = d->inFlight.front();
+    const hipError_t q = hipEventQuery(d->done[f.slot]);
+    if (q == hipErrorNotReady) return 0;
+    if (q != hipSuccess) return hipFail(q, "hipEventQuery");
+    d->inFlight.pop_front();
+    const LaunchState& s = d->readback[f.slot];
+    float ms = 0;
+    X2_HIP(hipEventElapsedTime(&ms, d->begun[f.slot], d->ended[f.slot]));
+    out->chunks = f.count;
+    out->items = f.count * uint64_t(d->itemsPerChunk);
+    out->samples = f.samples;
+    out->errors = s.errors;
+    out->pad = 0;
+    out->device_seconds = ms / 1000.0;
+    out->rays_trace = s.traces;
+    out->rays_shadow = s.shadows;
+    out->nodes_visited = s.nodes;
+    out->triangles_tested = s.triangles;
+    return 1;
+}
+
+int x2dev_take_deferred(x2dev* d, x2_parked* out, uint32_t capacity, uint32_t* taken) {
+    (void)out;
+    (void)capacity;
+    if (!d || !taken) return fail(X2_ERR_ARGUMENT, "null device or count");
+    *taken = 0;  // this device renders every sample it claims: nothing is deferred to the CPU lane
+    return X2_OK;
+}
+
+int x2dev_download_film(x2dev* d) {
+    if (!d) return fail(X2_ERR_ARGUMENT, "null device");
+    if (!d->begun_) return fail(X2_ERR_STATE, "download before begin");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "Film download before the last launch was collected");
+    X2_HIP(hipSetDevice(d->device));
+    X2_HIP(hipMemcpyAsync(d->hostSums, d->sums, d->filmPixels * 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipMemcpyAsync(d->hostCounts, d->counts, d->filmPixels * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipEventRecord(d->filmDone, d->stream));
+    d->filmQueued = true;
+    return X2_OK;
+}
+
+int x2dev_film_ready(x2dev* d) {
+    if (!d || !d->filmQueued) return 0;
+    const hipError_t q = hipEventQuery(d->filmDone);
+    if (q == hipErrorNotReady) return 0;
+    if (q != hipSuccess) return hipFail(q, "hipEventQuery");
+    return 1;
+}
+
+int x2dev_film(const x2dev* d, const double** sums, const uint32_t** counts) {
+    if (!d || !sums || !counts) return fail(X2_ERR_ARGUMENT, "null argument");
+    *sums = d->hostSums;
+    *counts = d->hostCounts;
+    return X2_OK;
+}
+
+int x2dev_device_film(const x2dev* d, void** sums, void** counts) {
+    if (!d || !sums || !counts) return fail(X2_ERR_ARGUMENT, "null argument");
+    *sums = d->sums;
+    *counts = d->counts;
+    return X2_OK;
+}
+
+void x2dev_destroy(x2dev* d) { release(d); }
+
+// The broker's GPU-lane sequence for one uniform frame (async_render.h:318-1081 with the GPU lane alone): chunks in
+// the work grid's order (async_grid.h:129-175: sample ranges 1, 1, 2, then samples_per_chunk; each range over the
+// 64-pixel cells' tiles), launches of about windowMs at the measured rate (launchSize, async_render.h:1674-1718:
+// firstChunks first, never under two items a resident thread), at most two in flight, completions polled, the film
+// downloaded and every pixel's count audited against spp (async_render.h:1035-1038).
+int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats* stats) {
+    using clock = std::chrono::steady_clock;
+    const auto start = clock::now();
+    int r = x2dev_begin(d, c, spc);
+    if (r) return r;
+    std::vector<x2_chunk> chunks;
+    {
+        const uint32_t tilesX = (c->width + 7) / 8, tilesY = (c->height + 7) / 8;
+        const uint32_t cellsX = (c->width + 63) / 64, cellsY = (c->height + 63) / 64;
+        std::vector<std::pair<uint32_t, uint32_t>> ranges;
+        for (uint32_t first = 0; first < c->spp;) {
+            const uint32_t size = first < 2 ? 1u : first < 4 ? 2u : spc;
+            ranges.push_back({first, std::min(size, c->spp - first)});
+            first += ranges.back().second;
+        }
+        for (const auto& rg : ranges)
+            for (uint32_t cy = 0; cy < cellsY; ++cy)
+                for (uint32_t cx = 0; cx < cellsX; ++cx)
+                    for (uint32_t ty = cy * 8; ty < std::min(tilesY, (cy + 1) * 8); ++ty)
+                        for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
+                            chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
+    }
+    x2_frame_stats st{};
+    const double windowMs = 10.0;
+    const uint64_t firstChunks = 2048, cap = d->opt.max_chunks_per_launch;
+    const uint64_t least = (uint64_t(d->launchThreads) * 2 + d->itemsPerChunk - 1) / d->itemsPerChunk;
+    double rate = 0;  // items per device second, from completed launches
+    size_t next = 0;
+    while (next < chunks.size() || !d->inFlight.empty()) {
+        x2_completion done;
+        while ((r = x2dev_poll(d, &done)) == 1) {
+            ++st.launches;
+            st.chunks += done.chunks;
+            st.samples += done.samples;
+            st.errors += done.errors;
+            st.device_seconds += done.device_seconds;
+            st.rays_trace += done.rays_trace;
+            st.rays_shadow += done.rays_shadow;
+            st.nodes_visited += done.nodes_visited;
+            st.triangles_tested += done.triangles_tested;
+            if (done.device_seconds > 0) {
+                const double now = double(done.items) / done.device_seconds;
+                rate = rate ? 0.5 * rate + 0.5 * now : now;
+            }
+        }
+        if (r < 0) return r;
+        while (x2dev_ready(d) && next < chunks.size()) {
+            const uint64_t remaining = chunks.size() - next;
+            uint64_t want = rate ? std::max<uint64_t>(uint64_t(rate * windowMs / 1000.0 / d->itemsPerChunk), least)
+                                 : std::max<uint64_t>(firstChunks, least);
+            want = std::min<uint64_t>({want, cap, remaining});
+            uint64_t samples = 0;
+            for (uint64_t i = 0; i < want; ++i) {
+                const x2_chunk& ch = chunks[next + i];
+                for (uint32_t p = 0; p < TilePixels; ++p) {
+                    const uint32_t tilesX = (c->width + 7) / 8;
+                    const uint32_t x = (ch.tile % tilesX) * 8 + p % 8, y = (ch.tile / tilesX) * 8 + p / 8;
+                    if (x < c->width && y < c->height) samples += ch.count;
+                }
+            }
+            r = x2dev_launch(d, chunks.data() + next, uint32_t(want), samples);
+            if (r < 0) return r;
+            next += want;
+        }
+        if (!d->inFlight.empty()) {
+            const hipError_t e = hipEventSynchronize(d->done[d->inFlight.front().slot]);
+            if (e != hipSuccess) return hipFail(e, "hipEventSynchronize");
+        }
+    }
+    r = x2dev_download_film(d);
+    if (r) return r;
+    X2_HIP(hipEventSynchronize(d->filmDone));
+    for (uint32_t y = 0; y < c->height; ++y)
+        for (uint32_t x = 0; x < c->width; ++x) {
+            if (int(x) < c->viewport[0] || int(y) < c->viewport[1] || int(x) >= c->viewport[2] || int(y) >= c->viewport[3])
+                continue;
+            if (d->hostCounts[size_t(y) * c->width + x] != c->spp) ++st.audit_failures;
+        }
+    st.wall_seconds = std::chrono::duration<double>(clock::now() - start).count();
+    if (stats) *stats = st;
+    if (st.audit_failures) return fail(X2_ERR_STATE, "Sample audit failure (async_render.h:1035)");
+    if (st.errors) return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
+    return X2_OK;
+}
+
+}  // extern "C"

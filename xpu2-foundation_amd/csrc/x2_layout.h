@@ -1,0 +1,62 @@
+// x2_layout.h — the device's data layout in HBM, shared by the host side (BVH8 builder, lane) and the kernels.
+//
+//   BVH8 node     256 B: the 8 children's AABBs as six float[8] planes (192 B), 8 child references (32 B), 32 B pad.
+//                 A reference is a node index (bit 31 clear), a leaf (bit 31 set: first triangle << 3 | count - 1,
+//                 1..8 triangles from first, first < 2^28), or EmptyChild with an inverted box that no ray enters.
+//   triangle      48 B: three float4, the vertices in xyz and (geom, prim, material) as the w words, in leaf order.
+//   film          FP64 sums[pixel][4] (rgb, alpha) + u32 counts[pixel] (async_gpu_abi.h:182-183 sums/counts).
+#pragma once
+#include <stdint.h>
+
+#include "../../include/x2dev.h"
+
+namespace x2 {
+
+constexpr uint32_t EmptyChild = 0xffffffffu;
+constexpr uint32_t LeafBit = 0x80000000u;
+constexpr uint32_t MaxLeafTriangles = 8;
+constexpr uint32_t TilePixels = 64;
+constexpr uint32_t Slots = 4;  // launches whose per-launch counters may be live at once (async_gpu_abi.h:14)
+constexpr int TraversalStack = 128;  // per-lane traversal stack entries; prepare() checks the BVH fits
+
+struct alignas(256) Node8 {
+    float lox[8], hix[8], loy[8], hiy[8], loz[8], hiz[8];
+    uint32_t child[8];
+    uint32_t pad[8];
+};
+static_assert(sizeof(Node8) == X2_NODE_BYTES, "BVH8 node is 256 B");
+
+struct alignas(16) TriRecord {
+    float ax, ay, az;
+    uint32_t geom;
+    float bx, by, bz;
+    uint32_t prim;
+    float cx, cy, cz;
+    uint32_t material;
+};
+static_assert(sizeof(TriRecord) == X2_TRIANGLE_BYTES, "triangle record is 48 B");
+
+// Per-launch totals the kernel adds up (one atomic per wave): errors and traversal counters.
+struct alignas(64) LaunchState {
+    unsigned long long traces, shadows, nodes, triangles;
+    uint32_t errors, pad[7];
+};
+
+// Everything a render launch reads, by value.
+struct RenderParams {
+    x2_config cfg;
+    const Node8* nodes;
+    const TriRecord* tris;
+    const x2_material* materials;
+    const x2_light* lights;
+    const x2_chunk* ring;
+    double* sums;
+    uint32_t* counts;
+    LaunchState* state;
+    unsigned long long itemBegin, itemEnd;
+    uint32_t itemsPerChunk, ringCapacity;
+    uint32_t counters;  // nonzero: count rays, nodes, triangles
+    uint32_t pad;
+};
+
+}  // namespace x2

@@ -1,0 +1,155 @@
+"""ctypes mirror of include/x2dev.h: the boundary records and the device library's entry points.
+
+The structures are field-for-field the C ones; sizes are asserted against the C declarations at import so a layout
+drift fails loudly. The library is the in-tree ``libx2dev.so`` built by ``__graft_entry__.build()``.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libx2dev.so")
+
+X2_OK = 0
+X2_LIGHT_ENV = 0
+X2_LIGHT_RECT = 1
+X2_SIDE_REGULAR = 0
+X2_SIDE_REVERSE = 1
+X2_SIDE_TWO = 2
+NODE_BYTES = 256
+TRIANGLE_BYTES = 48
+FILM_BYTES_PER_PATH = 72
+
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int32
+f32 = ctypes.c_float
+f64 = ctypes.c_double
+
+
+class Chunk(ctypes.Structure):
+    _fields_ = [("tile", u32), ("first", u32), ("count", u32), ("cell", u32)]
+
+
+class Parked(ctypes.Structure):
+    _fields_ = [("pixel", u32), ("index", u32), ("tries", u32), ("pass_", u32)]
+
+
+class Light(ctypes.Structure):
+    _fields_ = [
+        ("type", u32), ("sidedness", u32), ("hemispherical", u32), ("visibility_mask", u32),
+        ("radiance", f32 * 3), ("inv_area", f32),
+        ("position", f32 * 3), ("plane_offset", f32),
+        ("direction", f32 * 3), ("scale", f32),
+        ("local_to_render", f32 * 9), ("half_width", f32),
+        ("render_to_local", f32 * 9), ("half_height", f32),
+        ("render_to_local_rot", f32 * 9), ("pad", f32),
+    ]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [
+        ("albedo", f32 * 3), ("roughness", f32), ("diffuse_weight", f32), ("specular_weight", f32),
+        ("emission", f32 * 3), ("albedo_map", u32),
+    ]
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.POINTER(f32)), ("indices", ctypes.POINTER(u32)),
+        ("geom_first", ctypes.POINTER(u32)), ("geom_material", ctypes.POINTER(u32)),
+        ("materials", ctypes.POINTER(Material)), ("lights", ctypes.POINTER(Light)),
+        ("vertex_count", u32), ("triangle_count", u32), ("geom_count", u32), ("material_count", u32),
+        ("light_count", u32), ("pad", u32),
+    ]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("width", u32), ("height", u32), ("spp", u32), ("seed", u32),
+        ("sampling", u32), ("filter", u32), ("filter_width", f32), ("near_clip", f32), ("far_clip", f32),
+        ("raster_to_camera", f32 * 16), ("camera_to_render", f32 * 16),
+        ("light_count", u32), ("bsdf_samples", u32), ("light_samples", u32),
+        ("max_depth", u32), ("max_diffuse_depth", u32), ("max_glossy_depth", u32), ("max_mirror_depth", u32),
+        ("clamp_depth", u32), ("clamp_value", f32), ("rr_threshold", f32), ("rr_min_depth", u32),
+        ("shadow_reuse", u32), ("viewport", i32 * 4), ("pad", u32 * 2),
+    ]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [
+        ("chunks", u64), ("items", u64), ("samples", u64), ("errors", u32), ("pad", u32),
+        ("device_seconds", f64), ("rays_trace", u64), ("rays_shadow", u64), ("nodes_visited", u64),
+        ("triangles_tested", u64),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", i32), ("max_chunks_per_launch", u32), ("ring_chunks", u32), ("counters", u32)]
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [
+        ("launches", u64), ("chunks", u64), ("samples", u64), ("errors", u32), ("audit_failures", u32),
+        ("device_seconds", f64), ("wall_seconds", f64), ("rays_trace", u64), ("rays_shadow", u64),
+        ("nodes_visited", u64), ("triangles_tested", u64),
+    ]
+
+
+# Sizes of the C structures (include/x2dev.h); checked against ctypes here and against the C compiler by a
+# CPU test that compiles the header.
+EXPECTED_SIZES = {
+    "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
+    "x2_material": (Material, 40), "x2_scene": (Scene, 72), "x2_config": (Config, 236),
+    "x2_completion": (Completion, 72), "x2dev_options": (Options, 16), "x2_frame_stats": (FrameStats, 80),
+}
+for _name, (_cls, _size) in EXPECTED_SIZES.items():
+    assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
+
+# Every entry point include/x2dev.h declares, with its ctypes signature.
+SIGNATURES = {
+    "x2dev_create": (ctypes.c_int, [ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p)]),
+    "x2dev_prepare": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Scene)]),
+    "x2dev_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32]),
+    "x2dev_ready": (ctypes.c_int, [ctypes.c_void_p]),
+    "x2dev_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Chunk), u32, u64]),
+    "x2dev_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Completion)]),
+    "x2dev_take_deferred": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Parked), u32, ctypes.POINTER(u32)]),
+    "x2dev_download_film": (ctypes.c_int, [ctypes.c_void_p]),
+    "x2dev_film_ready": (ctypes.c_int, [ctypes.c_void_p]),
+    "x2dev_film": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(f64)),
+                                  ctypes.POINTER(ctypes.POINTER(u32))]),
+    "x2dev_in_flight": (u32, [ctypes.c_void_p]),
+    "x2dev_max_chunks_per_launch": (u32, [ctypes.c_void_p]),
+    "x2dev_launch_threads": (u32, [ctypes.c_void_p]),
+    "x2dev_device_film": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "x2dev_destroy": (None, [ctypes.c_void_p]),
+    "x2dev_last_error": (ctypes.c_char_p, []),
+    "x2dev_render_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32,
+                                          ctypes.POINTER(FrameStats)]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads the device library; raises if it is not built (there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"x2dev library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code, what):
+    if code < 0:
+        err = load().x2dev_last_error()
+        raise RuntimeError(f"{what} failed ({code}): {err.decode() if err else ''}")
+    return code

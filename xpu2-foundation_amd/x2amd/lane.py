@@ -1,0 +1,106 @@
+"""GpuLane: the reference's GPU-lane interface (xpu2/interactive/async_gpu.h:82-1731) over the C-ABI device.
+
+Method names and meaning follow the reference: begin(samples_per_chunk) zeroes the film; ready() allows at most two
+launches in flight; launch(chunks, samples) copies the chunks and queues a render, False when there was nothing to
+run; poll() never waits and returns the oldest finished launch's Completion or None; download_film()/film_ready()
+bring the film to the host; sums()/counts() view it. Failures raise RuntimeError, as the reference's cpu::need
+throws std::runtime_error. There is no CPU fallback: without the built library every call raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+class GpuLane:
+    def __init__(self, device=0, max_chunks_per_launch=16384, ring_chunks=1 << 17, counters=True):
+        self.lib = abi.load()
+        opts = abi.Options(device, max_chunks_per_launch, ring_chunks, 1 if counters else 0)
+        h = ctypes.c_void_p()
+        abi.check(self.lib.x2dev_create(ctypes.byref(opts), ctypes.byref(h)), "x2dev_create")
+        self.h = h
+        self.config = None
+        self._scene = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.x2dev_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def prepare(self, scene):
+        """Uploads a scenes.SceneData (BVH8 built on the host)."""
+        abi.check(self.lib.x2dev_prepare(self.h, ctypes.byref(scene.c)), "x2dev_prepare")
+        self._scene = scene
+
+    def begin(self, config, samples_per_chunk=4):
+        abi.check(self.lib.x2dev_begin(self.h, ctypes.byref(config), samples_per_chunk), "x2dev_begin")
+        self.config = config
+
+    def ready(self):
+        return bool(self.lib.x2dev_ready(self.h))
+
+    def in_flight(self):
+        return int(self.lib.x2dev_in_flight(self.h))
+
+    def launch_threads(self):
+        return int(self.lib.x2dev_launch_threads(self.h))
+
+    def max_chunks_per_launch(self):
+        return int(self.lib.x2dev_max_chunks_per_launch(self.h))
+
+    def launch(self, chunks, samples=0):
+        """chunks: a ctypes array of abi.Chunk (or a slice copy). Returns True when queued."""
+        n = len(chunks)
+        ptr = ctypes.cast(chunks, ctypes.POINTER(abi.Chunk)) if n else None
+        return abi.check(self.lib.x2dev_launch(self.h, ptr, n, samples), "x2dev_launch") == 1
+
+    def poll(self):
+        c = abi.Completion()
+        r = abi.check(self.lib.x2dev_poll(self.h, ctypes.byref(c)), "x2dev_poll")
+        return c if r == 1 else None
+
+    def take_deferred(self):
+        buf = (abi.Parked * 1)()
+        n = ctypes.c_uint32()
+        abi.check(self.lib.x2dev_take_deferred(self.h, buf, 1, ctypes.byref(n)), "x2dev_take_deferred")
+        return [buf[i] for i in range(n.value)]
+
+    def download_film(self):
+        abi.check(self.lib.x2dev_download_film(self.h), "x2dev_download_film")
+
+    def film_ready(self):
+        return abi.check(self.lib.x2dev_film_ready(self.h), "x2dev_film_ready") == 1
+
+    def film(self):
+        """(sums [pixels, 4] float64, counts [pixels] uint32), copies of the downloaded film."""
+        s = ctypes.POINTER(ctypes.c_double)()
+        c = ctypes.POINTER(ctypes.c_uint32)()
+        abi.check(self.lib.x2dev_film(self.h, ctypes.byref(s), ctypes.byref(c)), "x2dev_film")
+        n = self.config.width * self.config.height
+        sums = np.ctypeslib.as_array(s, shape=(n * 4,)).reshape(n, 4).copy()
+        counts = np.ctypeslib.as_array(c, shape=(n,)).copy()
+        return sums, counts
+
+    def render_frame(self, config, samples_per_chunk=4):
+        """One uniform frame through the broker counterpart; returns (sums, counts, FrameStats)."""
+        st = abi.FrameStats()
+        r = self.lib.x2dev_render_frame(self.h, ctypes.byref(config), samples_per_chunk, ctypes.byref(st))
+        self.config = config
+        if r < 0:
+            err = self.lib.x2dev_last_error()
+            raise RuntimeError(f"x2dev_render_frame failed ({r}): {err.decode() if err else ''}")
+        sums, counts = self.film()
+        return sums, counts, st
+
+
+def render_frame(scene, config, samples_per_chunk=4, device=0):
+    lane = GpuLane(device)
+    try:
+        lane.prepare(scene)
+        return lane.render_frame(config, samples_per_chunk)
+    finally:
+        lane.close()
