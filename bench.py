@@ -22,7 +22,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def cpu_baseline(scene, width, height, spp, seconds_hint):
-    """The oracle (a CPU restatement of the reference's integrator) over 1920x1080 at `spp`, all host threads."""
+    """The oracle (a CPU restatement of the reference's integrator, -O3, no contraction) over 1920x1080 at `spp`
+    (about 10-20 s on a 16-thread host share), one worker thread per host thread claiming 8x8 tiles."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from x2amd import scenes
@@ -49,7 +50,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
-    ap.add_argument("--cpu-spp", type=int, default=4)
+    ap.add_argument("--cpu-spp", type=int, default=0, help="CPU baseline spp (0: 64 with >= 12 host threads, else 16)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -110,7 +111,8 @@ def main():
                      "launches": totals["launches"], "algorithmic_bytes": bytes_total},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, args.cpu_spp, 20)
+        cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)
+        out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp, 20)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 

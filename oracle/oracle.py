@@ -90,6 +90,10 @@ class OracleScene:
             self.lib.oracle_render(self.h, ctypes.addressof(config), threads,
                                    sums.ctypes.data_as(P(f64)), counts.ctypes.data_as(P(u32)), counters)
         else:
+            if isinstance(chunks, list):  # a slice of a ctypes array is a list of records
+                chunks = (type(chunks[0]) * len(chunks))(*chunks) if chunks else None
+            if chunks is None:
+                return sums, counts, (0, 0, 0)
             self.lib.oracle_render_chunks(self.h, ctypes.addressof(config), ctypes.addressof(chunks), len(chunks),
                                           threads, sums.ctypes.data_as(P(f64)), counts.ctypes.data_as(P(u32)),
                                           counters)
