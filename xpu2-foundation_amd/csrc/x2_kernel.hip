@@ -12,6 +12,7 @@
 // conservative (Ize 2013) and decide what is tested, never the result.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "x2_layout.h"
@@ -163,6 +164,21 @@ __device__ __forceinline__ bool woopHit(const WoopRay& w, V3 a, V3 b, V3 c, floa
 constexpr float FarScale = 1.0f + 2.0f * (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
 constexpr float Slack = 1.0f + 0x1p-12f;
 
+// A lane's traversal stack: the first LdsDepth entries in LDS, lane-interleaved (entry e of lane l at
+// lds[e * 256 + l], bank-conflict free), deeper ones in private memory. Entries are (child ref, entry distance).
+struct TravStack {
+    uint2* lds;
+    uint32_t lane;
+    uint2 spill[TraversalStack - LdsDepth > 0 ? TraversalStack - LdsDepth : 1];
+    __device__ __forceinline__ void put(int i, uint2 e) {
+        if (LdsDepth > 0 && i < LdsDepth) lds[i * 256 + lane] = e;
+        else spill[i - LdsDepth] = e;
+    }
+    __device__ __forceinline__ uint2 get(int i) const {
+        return (LdsDepth > 0 && i < LdsDepth) ? lds[i * 256 + lane] : spill[i - LdsDepth];
+    }
+};
+
 struct Counters {
     uint32_t traces = 0, shadows = 0, nodes = 0, triangles = 0;
 };
@@ -245,14 +261,12 @@ __device__ __forceinline__ bool keyLess(float t, uint32_t g, uint32_t p, float b
 
 // Closest hit among triangles with t in [near, far]; returns the triangle record index (~0u: none).
 __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, float& tOut,
-                               Counters& k) {
+                               Counters& k, TravStack& st) {
     const WoopRay w = woopSetup(org, dir, nearT, farT);
     const BoxRay br = boxRay(org, dir);
     const float lo = nearT > 0 ? nearT / Slack : nearT * Slack;
     uint32_t best = ~0u, bg = ~0u, bp = ~0u;
     float bestT = farT;
-    uint32_t stackRef[TraversalStack];
-    float stackT[TraversalStack];
     int sp = 0;
     uint32_t ref = 0;
     for (;;) {
@@ -288,8 +302,7 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
 #pragma unroll
             for (int i = 7; i >= 1; --i)
                 if (key[i] != ~0ull) {
-                    stackRef[sp] = uint32_t(key[i]);
-                    stackT[sp] = __uint_as_float(uint32_t(key[i] >> 32));
+                    st.put(sp, make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32)));
                     ++sp;
                 }
             if (key[0] != ~0ull) {
@@ -302,8 +315,9 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
         while (sp > 0) {
             --sp;
             const float bound = best == ~0u ? farT : bestT;
-            if (stackT[sp] <= (bound > 0 ? bound * Slack : bound / Slack)) {
-                ref = stackRef[sp];
+            const uint2 e = st.get(sp);
+            if (__uint_as_float(e.y) <= (bound > 0 ? bound * Slack : bound / Slack)) {
+                ref = e.x;
                 found = true;
                 break;
             }
@@ -315,11 +329,10 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
 }
 
 // Any hit with t in [near, far] (a shadow ray; TERMINATE_ON_FIRST_HIT, async_gpu_device.h:521-533).
-__device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, Counters& k) {
+__device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, Counters& k, TravStack& st) {
     const WoopRay w = woopSetup(org, dir, nearT, farT);
     const BoxRay br = boxRay(org, dir);
     const float lo = nearT > 0 ? nearT / Slack : nearT * Slack, hi = farT > 0 ? farT * Slack : farT / Slack;
-    uint32_t stackRef[TraversalStack];
     int sp = 0;
     uint32_t ref = 0;
     for (;;) {
@@ -338,10 +351,10 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
             nodeChildren(P.nodes[ref], br, lo, hi, key);
 #pragma unroll
             for (int i = 0; i < 8; ++i)
-                if (key[i] != ~0ull) stackRef[sp++] = uint32_t(key[i]);
+                if (key[i] != ~0ull) st.put(sp++, make_uint2(uint32_t(key[i]), 0u));
         }
         if (sp == 0) return false;
-        ref = stackRef[--sp];
+        ref = st.get(--sp).x;
     }
 }
 
@@ -628,10 +641,10 @@ __device__ __forceinline__ uint32_t surfaceFlags(V3 Ng, V3 wo, V3 wi) {
     return dot(Ng, wo) * dot(Ng, wi) > 0.0f ? (AllLobeTypes & ~Transmission) : (AllLobeTypes & ~Reflection);
 }
 
-__device__ bool traceHit(const RenderParams& P, const Ray& ray, Hit& h, Counters& k) {
+__device__ bool traceHit(const RenderParams& P, const Ray& ray, Hit& h, Counters& k, TravStack& st) {
     ++k.traces;
     float t;
-    const uint32_t tri = closestHit(P, ray.origin, ray.direction, ray.nearT, ray.farT, t, k);
+    const uint32_t tri = closestHit(P, ray.origin, ray.direction, ray.nearT, ray.farT, t, k, st);
     h.geom = ~0u;
     if (tri == ~0u) return true;
     const TriRecord& r = P.tris[tri];
@@ -639,13 +652,15 @@ __device__ bool traceHit(const RenderParams& P, const Ray& ray, Hit& h, Counters
     h.tri = tri;
     return geometryHit(h, ray, V3{r.ax, r.ay, r.az}, V3{r.bx, r.by, r.bz}, V3{r.cx, r.cy, r.cz});
 }
-__device__ __forceinline__ bool visible(const RenderParams& P, V3 o, V3 d, float nearT, float farT, Counters& k) {
+__device__ __forceinline__ bool visible(const RenderParams& P, V3 o, V3 d, float nearT, float farT, Counters& k,
+                                        TravStack& st) {
     ++k.shadows;
-    return !anyHit(P, o, d, nearT, farT, k);
+    return !anyHit(P, o, d, nearT, farT, k, st);
 }
 
 // One sample; 0 Done (rgba set), 2 Error.
-__device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, double rgba[4], Counters& k) {
+__device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, double rgba[4], Counters& k,
+                         TravStack& st) {
     const x2_config& c = P.cfg;
     const Identity root{pixel, index, c.seed, 0};
     float x, y;
@@ -681,7 +696,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
         const bool fromCamera = depth == 0;
         const Ray ray{pv.origin, pv.dir, fromCamera ? camera.nearT : 0.0f, fromCamera ? camera.farT : MaxFloat};
         Hit hit;
-        if (!traceHit(P, ray, hit, k)) return 2;
+        if (!traceHit(P, ray, hit, k, st)) return 2;
         if (pv.heldLight != ~0u) {
             const float kept = hit.geom != ~0u && hit.t < pv.heldFar ? 0.0f : 1.0f;
             if (kept > 0.0f) radiance = radiance + (kept < 1.0f ? pv.held * kept : pv.held);
@@ -804,7 +819,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 if (!finite3(t) || isExactlyZero(t)) continue;
                 const V3 shadowOrigin = dot(wi, Ng) >= 0.0f ? above : below;
                 const float shadowNear = 0.0f, shadowFar = fmaxf((lh.distance - 0.0f) * HitEpsilonEnd, 2.0f * shadowNear);
-                if (visible(P, shadowOrigin, wi, shadowNear, shadowFar, k)) radiance = radiance + t;
+                if (visible(P, shadowOrigin, wi, shadowNear, shadowFar, k, st)) radiance = radiance + t;
             }
         }
 
@@ -876,7 +891,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                     shadowFar = fmaxf((lightDistance - 0.0f) * HitEpsilonEnd, 2.0f * 0.0f);
                     held = bounces && thinning >= 1.0f && top + PendingReserve < MaxPending && lobeMask &&
                            (c.shadow_reuse & lobeMask) == lobeMask;
-                    if (!held) kept = visible(P, origin, wi, 0.0f, shadowFar, k) ? 1.0f : 0.0f;
+                    if (!held) kept = visible(P, origin, wi, 0.0f, shadowFar, k, st) ? 1.0f : 0.0f;
                 }
                 const V3 directBefore = direct;
                 if (kept > 0.0f && !held) {
@@ -936,33 +951,64 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
     return s;
 }
 
-// One thread per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
-// position; the chunk comes from the mailbox ring. Every lane reaches the end (no early return) so the wave's
-// counter sums see all 64 lanes.
-__global__ void __launch_bounds__(256) renderKernel(const RenderParams P) {
-    const unsigned long long item = P.itemBegin + (unsigned long long)blockIdx.x * 256u + threadIdx.x;
+// Launch shape knobs (measured on MI355X at the headline workload, profiles/renderKernel_icospheres_1080p64.md):
+// persistent lanes claiming samples one at a time, and an occupancy target of 3 waves per SIMD (168 VGPRs) were the
+// fastest of the variants tried; an LDS traversal stack was slower than private memory at that occupancy.
+#ifndef X2_PERSISTENT
+#define X2_PERSISTENT 1
+#endif
+#ifndef X2_WAVES_PER_EU
+#define X2_WAVES_PER_EU 3
+#endif
+#if X2_WAVES_PER_EU
+#define X2_OCCUPANCY __attribute__((amdgpu_waves_per_eu(X2_WAVES_PER_EU)))
+#else
+#define X2_OCCUPANCY
+#endif
+
+// One sample per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
+// position; the chunk comes from the mailbox ring.
+__device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long long item, Counters& k, TravStack& st,
+                                           uint32_t& errors) {
+    const unsigned long long chunkIndex = item / P.itemsPerChunk;
+    const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
+    const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
+    const uint32_t s = offset / TilePixels, position = offset % TilePixels;
+    const uint32_t pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
+    if (pixel == ~0u) return;
+    double rgba[4];
+    if (integrate(P, pixel, chunk.first + s, rgba, k, st) == 0) {
+        double* f = P.sums + size_t(pixel) * 4;
+        atomicAdd(f + 0, rgba[0]);
+        atomicAdd(f + 1, rgba[1]);
+        atomicAdd(f + 2, rgba[2]);
+        atomicAdd(f + 3, rgba[3]);
+        atomicAdd(P.counts + pixel, 1u);
+    } else {
+        ++errors;
+    }
+}
+
+// A lane per item (the 64 lanes of a wave are one 8x8 tile at one sample index, so camera rays start coherent), or
+// persistent lanes claiming items one at a time (the reference's Feed loop) when built with X2_PERSISTENT. Every lane
+// reaches the end so the wave's counter sums see all 64 lanes.
+__global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) {
+    __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
+    TravStack st;
+    st.lds = lds;
+    st.lane = threadIdx.x;
     Counters k;
     uint32_t errors = 0;
-    if (item < P.itemEnd) {
-        const unsigned long long chunkIndex = item / P.itemsPerChunk;
-        const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
-        const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
-        const uint32_t s = offset / TilePixels, position = offset % TilePixels;
-        const uint32_t pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
-        if (pixel != ~0u) {
-            double rgba[4];
-            if (integrate(P, pixel, chunk.first + s, rgba, k) == 0) {
-                double* f = P.sums + size_t(pixel) * 4;
-                atomicAdd(f + 0, rgba[0]);
-                atomicAdd(f + 1, rgba[1]);
-                atomicAdd(f + 2, rgba[2]);
-                atomicAdd(f + 3, rgba[3]);
-                atomicAdd(P.counts + pixel, 1u);
-            } else {
-                errors = 1;
-            }
-        }
+#if X2_PERSISTENT
+    for (;;) {
+        const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
+        if (item >= P.itemEnd) break;
+        renderItem(P, item, k, st, errors);
     }
+#else
+    const unsigned long long item = P.itemBegin + (unsigned long long)blockIdx.x * 256u + threadIdx.x;
+    if (item < P.itemEnd) renderItem(P, item, k, st, errors);
+#endif
     if (errors) atomicAdd(&P.state->errors, errors);
     if (P.counters) {
         const unsigned long long t = waveSum(k.traces), sh = waveSum(k.shadows), n = waveSum(k.nodes),
@@ -979,6 +1025,7 @@ __global__ void __launch_bounds__(256) renderKernel(const RenderParams P) {
 __global__ void clearState(LaunchState* s) {
     if (threadIdx.x == 0) {
         s->traces = s->shadows = s->nodes = s->triangles = 0;
+        s->claimed = 0;
         s->errors = 0;
     }
 }
@@ -988,10 +1035,21 @@ __global__ void clearState(LaunchState* s) {
 // Host-side launchers the lane calls (x2_lane.hip).
 namespace x2 {
 hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
+    static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
+    if (X2_PERSISTENT && !resident) {
+        int dev = 0, perCu = 0;
+        hipDeviceProp_t prop;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, renderKernel, 256, 0);
+        if (e != hipSuccess) return e;
+        resident = std::max(1, perCu) * prop.multiProcessorCount;
+    }
     const unsigned long long items = p.itemEnd - p.itemBegin;
-    const unsigned blocks = unsigned((items + 255) / 256);
+    unsigned long long blocks = (items + 255) / 256;
+    if (X2_PERSISTENT && blocks > (unsigned long long)resident) blocks = resident;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(renderKernel, dim3(blocks), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(renderKernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 hipError_t launchClearState(LaunchState* s, hipStream_t stream) {

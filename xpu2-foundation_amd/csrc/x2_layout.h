@@ -18,6 +18,10 @@ constexpr uint32_t MaxLeafTriangles = 8;
 constexpr uint32_t TilePixels = 64;
 constexpr uint32_t Slots = 4;  // launches whose per-launch counters may be live at once (async_gpu_abi.h:14)
 constexpr int TraversalStack = 128;  // per-lane traversal stack entries; prepare() checks the BVH fits
+#ifndef X2_LDS_DEPTH
+#define X2_LDS_DEPTH 0
+#endif
+constexpr int LdsDepth = X2_LDS_DEPTH;  // of which the first entries live in LDS (8 B x 256 lanes each)
 
 struct alignas(256) Node8 {
     float lox[8], hix[8], loy[8], hiy[8], loz[8], hiz[8];
@@ -36,10 +40,11 @@ struct alignas(16) TriRecord {
 };
 static_assert(sizeof(TriRecord) == X2_TRIANGLE_BYTES, "triangle record is 48 B");
 
-// Per-launch totals the kernel adds up (one atomic per wave): errors and traversal counters.
+// Per-launch state: the claim counter and the totals the kernel adds up (one atomic per wave).
 struct alignas(64) LaunchState {
     unsigned long long traces, shadows, nodes, triangles;
-    uint32_t errors, pad[7];
+    unsigned long long claimed;  // items claimed by the launch's persistent lanes
+    uint32_t errors, pad[5];
 };
 
 // Everything a render launch reads, by value.

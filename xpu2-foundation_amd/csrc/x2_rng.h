@@ -99,6 +99,7 @@ struct Stream {
     Identity id;
     uint32_t dimension, group;
     Words w;
+    __device__ __forceinline__ Stream() : id{0, 0, 0, 0}, dimension(0), group(~0u), w{{0, 0, 0, 0}} {}
     __device__ __forceinline__ Stream(const Identity& i, uint32_t bounce)
         : id(i), dimension(CameraDimensions + bounce * DimensionsPerBounce), group(~0u), w{{0, 0, 0, 0}} {}
     __device__ __forceinline__ float next() {

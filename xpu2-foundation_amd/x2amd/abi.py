@@ -132,11 +132,13 @@ SIGNATURES = {
 _lib = None
 
 
-def load(path=LIB_PATH):
-    """Loads the device library; raises if it is not built (there is no fallback path)."""
+def load(path=None):
+    """Loads the device library (LIB_PATH unless a path is given); raises if it is not built (there is no fallback
+    path)."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"x2dev library not built: {path} (run __graft_entry__.build())")
     lib = ctypes.CDLL(path)
