@@ -72,11 +72,20 @@ typedef struct x2_material {
     float diffuse_weight;
     float specular_weight;
     float emission[3];
-    uint32_t albedo_map;   /* reserved (0: none) */
+    uint32_t albedo_map;   /* 0: none; k: the scene's texture k-1 multiplies the albedo */
 } x2_material;
 
+/* An RGB FP32 texture, row-major texels (y * width + x) * 3, read by bilinear filtering at level 0 with periodic
+ * addressing and the texture's t flipped (texture_core.h:144-153, :179-181 with unit scale and no offset). */
+typedef struct x2_texture {
+    uint32_t width, height;
+    const float* texels;
+} x2_texture;
+
 /* A triangle scene. Triangles are ordered mesh by mesh; mesh g owns triangles [geom_first[g], geom_first[g+1]),
- * and a triangle's primitive id is its index less geom_first of its mesh. Meshes are two-sided. */
+ * and a triangle's primitive id is its index less geom_first of its mesh. Meshes are two-sided. A material whose
+ * albedo_map is k > 0 multiplies its albedo by texture k-1 at the hit's (s, t), interpolated from per-vertex uvs
+ * (shade_material.h:121-160; a triangle without uvs uses (0,0), (1,0), (1,1)). */
 typedef struct x2_scene {
     const float* positions;        /* vertex_count x 3 */
     const uint32_t* indices;       /* triangle_count x 3 */
@@ -85,6 +94,9 @@ typedef struct x2_scene {
     const x2_material* materials;  /* material_count */
     const x2_light* lights;        /* light_count */
     uint32_t vertex_count, triangle_count, geom_count, material_count, light_count, pad;
+    const float* uvs;              /* vertex_count x 2, or null */
+    const x2_texture* textures;    /* texture_count */
+    uint32_t texture_count, pad2;
 } x2_scene;
 
 /* The integrator's settings: the used subset of `mixed::Config` (gpu_jobs.h:10-44), plus rr_min_depth (an
@@ -119,6 +131,14 @@ typedef struct x2_completion {
     uint64_t nodes_visited;    /* BVH8 nodes fetched (256 B each) */
     uint64_t triangles_tested; /* triangle records fetched (48 B each) */
 } x2_completion;
+
+/* Totals over a frame's launches (x2dev_render_frame, x2dev_adjoint_frame). */
+typedef struct x2_frame_stats {
+    uint64_t launches, chunks, samples;
+    uint32_t errors, audit_failures;
+    double device_seconds, wall_seconds;
+    uint64_t rays_trace, rays_shadow, nodes_visited, triangles_tested;
+} x2_frame_stats;
 
 typedef struct x2dev_options {
     int32_t device;                 /* HIP device ordinal */
@@ -165,6 +185,26 @@ void x2dev_destroy(x2dev* d);
 /* The last failure's text on the calling thread ("" when none). */
 const char* x2dev_last_error(void);
 
+/* Parameter updates between frames, for optimisation loops: a texture's texels (texels_on_device != 0: a device
+ * pointer, copied on the device's stream) and a material record. No counterpart in the reference (its scene is
+ * rebuilt by the plan); they stand beside prepare. */
+int x2dev_set_texture(x2dev* d, uint32_t texture, const float* texels, int texels_on_device);
+int x2dev_set_material(x2dev* d, uint32_t material, const x2_material* m);
+
+/* The film's mean rgb per pixel (sums / counts; 0 where a pixel has no sample) into a caller-owned device buffer of
+ * width * height * 3 floats, on the device's stream, waited for before return. */
+int x2dev_resolve_image(x2dev* d, float* device_rgb);
+
+/* The path-replay adjoint of one uniform frame: every sample of the frame is rendered again from its identity (the
+ * same random numbers as the forward frame), and the image adjoint d_image (device, width * height * 3 floats:
+ * dLoss / d(mean rgb of the pixel)) is carried back through it to the gradients of the textures' texels and the
+ * materials' roughness (device buffers the caller owns and zeroes: grad_textures, the scene's textures' texels
+ * concatenated in order; grad_roughness, material_count floats). Sampling, MIS weights and Russian roulette are
+ * held fixed (detached), so the result is an unbiased estimate of the gradient of the expected image's loss.
+ * No counterpart in the reference (it has no differentiable code). */
+int x2dev_adjoint_frame(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, const float* d_image,
+                        float* grad_textures, float* grad_roughness, x2_frame_stats* stats);
+
 /* Algorithmic byte sizes the roofline counts: one BVH8 node and one triangle record. */
 #define X2_NODE_BYTES 256u
 #define X2_TRIANGLE_BYTES 48u
@@ -174,12 +214,6 @@ const char* x2dev_last_error(void);
  * the work grid makes them (async_grid.h:129-175, progressive ranges 1,1,2,spc over 64-px cells), launches sized
  * to the measured rate with at most two in flight, completions polled, the film downloaded and every pixel's count
  * audited against spp (async_render.h:1035). stats (may be null) receives totals over the frame's launches. */
-typedef struct x2_frame_stats {
-    uint64_t launches, chunks, samples;
-    uint32_t errors, audit_failures;
-    double device_seconds, wall_seconds;
-    uint64_t rays_trace, rays_shadow, nodes_visited, triangles_tested;
-} x2_frame_stats;
 int x2dev_render_frame(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, x2_frame_stats* stats);
 
 #ifdef __cplusplus

@@ -48,6 +48,12 @@ def load():
         "oracle_sample": (c_int, [vp, vp, u32, u32, P(f64), P(u64)]),
         "oracle_render": (c_int, [vp, vp, u32, P(f64), P(u32), P(u64)]),
         "oracle_render_chunks": (c_int, [vp, vp, vp, u32, u32, P(f64), P(u32), P(u64)]),
+        "oracle_adjoint": (c_int, [vp, vp, u32, P(f32), P(f64), P(f64), P(u64)]),
+        "oracle_set_texture": (None, [vp, u32, P(f32)]),
+        "oracle_set_material": (None, [vp, u32, vp]),
+        "oracle_set_eval_roughness": (None, [vp, u32, f32]),
+        "oracle_ggx_roughness_log_derivative": (f32, [f32, P(f32), P(f32), P(f32)]),
+        "oracle_texture_lookup": (c_int, [vp, f32, f32, P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -98,6 +104,26 @@ class OracleScene:
                                           threads, sums.ctypes.data_as(P(f64)), counts.ctypes.data_as(P(u32)),
                                           counters)
         return sums, counts, tuple(counters)
+
+    def adjoint(self, config, d_image, texture_floats, material_count, threads=None):
+        """FP64 gradients (textures concatenated, roughness per material) for the image adjoint d_image [pixels, 3]."""
+        d = np.ascontiguousarray(d_image, dtype=np.float32).reshape(-1)
+        gt = np.zeros(max(1, texture_floats), np.float64)
+        gr = np.zeros(material_count, np.float64)
+        counters = (u64 * 3)()
+        self.lib.oracle_adjoint(self.h, ctypes.addressof(config), threads or os.cpu_count() or 1,
+                                d.ctypes.data_as(P(f32)), gt.ctypes.data_as(P(f64)), gr.ctypes.data_as(P(f64)), counters)
+        return gt[:texture_floats], gr, tuple(counters)
+
+    def set_texture(self, index, texels):
+        t = np.ascontiguousarray(texels, dtype=np.float32)
+        self.lib.oracle_set_texture(self.h, index, t.ctypes.data_as(P(f32)))
+
+    def set_material(self, index, material):
+        self.lib.oracle_set_material(self.h, index, ctypes.addressof(material))
+
+    def set_eval_roughness(self, material, r):
+        self.lib.oracle_set_eval_roughness(self.h, material, r)
 
     def sample(self, config, pixel, index):
         rgba = (f64 * 4)()

@@ -2,7 +2,7 @@
 // in place from the reference tree (the Makefile's `ref` target, output oracle/_ref/libref.so). The tests call it to
 // check the oracle's restatement against the reference's code on the same inputs. It wraps, it does not restate:
 // every computation below is the reference's (sample_stream.h, surface_core.h, camera_core.h, triangle_hit.h,
-// resume_core.h, light_core.h, shade_lobes_eval.h). The shared integrator (async_integrator.h) is not buildable here
+// resume_core.h, light_core.h, shade_lobes_eval.h, texture_core.h). The shared integrator (async_integrator.h) is not buildable here
 // (it includes NanoVDB, which the image lacks), so it is not wrapped.
 #include <cstdint>
 #include <cstring>
@@ -13,7 +13,9 @@
 #include "sample_stream.h"
 #include "shade_lobes_eval.h"
 #include "surface_core.h"
+#include "texture_core.h"
 #include "triangle_hit.h"
+#include <vector>
 
 #include "../include/x2dev.h"
 
@@ -157,5 +159,35 @@ void ref_lobe_sample(uint32_t kind, const float N[3], const float Ng[3], const f
     V3 w;
     put(f, ii::shade::lobeSample(g_bsdf, rec, s, r1, r2, w, *pdf));
     put(wi, w);
+}
+// The reference's bilinear filter (tex::bilinear on level 0, periodic, float texels) at the lookup's (s, t) =
+// (u, 1 - v) with unit scale and no offset; the flat row-major texels are laid into its 32 x 32 tile pages first.
+int ref_bilinear(uint32_t width, uint32_t height, const float* texels, float u, float v, float out[3]) {
+    namespace tx = ii::tex;
+    const uint32_t nx = (width + tx::Tile - 1) / tx::Tile, ny = (height + tx::Tile - 1) / tx::Tile;
+    std::vector<std::vector<V3>> tiles(size_t(nx) * ny);
+    std::vector<const V3*> pages(size_t(nx) * ny);
+    for (uint32_t ty = 0; ty < ny; ++ty)
+        for (uint32_t tx0 = 0; tx0 < nx; ++tx0) {
+            const uint32_t stride = std::min<uint32_t>(tx::Tile, width - tx0 * tx::Tile);
+            const uint32_t rows = std::min<uint32_t>(tx::Tile, height - ty * tx::Tile);
+            std::vector<V3>& t = tiles[size_t(ty) * nx + tx0];
+            t.resize(size_t(stride) * rows);
+            for (uint32_t y = 0; y < rows; ++y)
+                for (uint32_t x = 0; x < stride; ++x)
+                    t[size_t(y) * stride + x] = v3(texels + (size_t(ty * tx::Tile + y) * width + tx0 * tx::Tile + x) * 3);
+            pages[size_t(ty) * nx + tx0] = t.data();
+        }
+    tx::Image image;
+    image.count = 1;
+    image.levels[0].width = width;
+    image.levels[0].height = height;
+    image.levels[0].pages = pages.data();
+    tx::NoSink sink;
+    V3 r;
+    const auto status = tx::bilinear(image, 0, u, 1 - v, true, r, sink);
+    if (status != tx::Ready) return 0;
+    put(out, r);
+    return 1;
 }
 }

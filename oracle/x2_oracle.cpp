@@ -288,7 +288,7 @@ bool woopHit(const WoopRay& w, V3 a, V3 b, V3 c, float& t) {
 // triangle's float vertices and the ray, the point rounded once, the conservative offset origins.
 struct Hit {
     U32 geom = ~0u, prim = 0;
-    float t = 0, u = 0, v = 0;
+    float t = 0, u = 0, v = 0;  // u, v: the FP64 barycentrics rounded once (resume_core.h:30)
     V3 normal{0, 0, 0}, origin{0, 0, 0}, below{0, 0, 0}, point{0, 0, 0};
 };
 bool coordinates(const Ray& ray, V3 a, V3 b, V3 c, double& t, double& u, double& v) {
@@ -343,10 +343,20 @@ struct BNode {
     float lo[3], hi[3];
     U32 a, n;  // n > 0: leaf of triangles order[a, a+n); n == 0: children a and a + 1
 };
+struct Texture {
+    U32 width, height;
+    std::vector<float> texels;  // RGB, row-major
+    size_t gradOffset;          // in the adjoint's concatenated gradient
+};
 struct Scene {
     std::vector<V3> tri;  // 3 vertices per triangle, scene order
+    std::vector<float> triUV;  // 6 per triangle, or empty: (0,0), (1,0), (1,1)
+    std::vector<Texture> textures;
     std::vector<U32> geomOf, primOf, material;
     std::vector<x2_material> materials;
+    // Test hook: per-material roughness used only for the value of the GGX lobe (not for sampling, pdfs or MIS),
+    // negative = the material's own. Central differences over it isolate d f / d roughness with the sampling fixed.
+    std::vector<float> evalRoughness;
     std::vector<x2_light> lights;
     std::vector<BNode> nodes;
     std::vector<U32> order;
@@ -676,6 +686,7 @@ struct Lobe {
     V3 scale, albedo;
     Frame frame;
     float roughness;  // GGX: max(0.001, r^2)
+    float evalRoughness;  // the same, for the lobe's value only (the evalRoughness test hook)
 };
 struct Slice { V3 Ng, wo; U32 flags; };
 struct Bsdf {
@@ -714,7 +725,7 @@ V3 evalGGX(const Lobe& l, const Slice& s, V3 wi, float& pdf) {  // shade_lobes_e
     if (cosMI <= 0.0f) return {0, 0, 0};
     const float cosThetaM = dot(m, l.frame.z);
     if (cosThetaM <= 0.0f) return {0, 0, 0};
-    float alpha2 = roughness * roughness;
+    float alpha2 = l.evalRoughness * l.evalRoughness;
     const float cosThetaM2 = cosThetaM * cosThetaM, piCosThetaM4 = Pi * cosThetaM2 * cosThetaM2;
     const float tanThetaM2 = (1.0f - cosThetaM2) / cosThetaM2;
     float tmp = alpha2 + tanThetaM2, D4 = 0.25f * alpha2 / (piCosThetaM4 * tmp * tmp);
@@ -763,10 +774,9 @@ U32 surfaceFlags(const Slice& s, V3 wi) {  // shade_lobes.h:323-326
 }
 // The material's Bsdf at a hit: a Lambert lobe and a GGX lobe about the shading normal (lambertLobe,
 // cookTorranceLobe with KindGGX), in that order.
-void buildBsdf(const x2_material& m, V3 N, Bsdf& b) {
+void buildBsdf(const x2_material& m, V3 N, Bsdf& b, V3 albedo, float evalRoughness = -1.0f) {
     b.count = 0;
     b.type = 0;
-    const V3 albedo = load3(m.albedo);
     const Frame f = frameOf(N);
     if (m.diffuse_weight > 0.0f) {
         Lobe& l = b.lobes[b.count++];
@@ -776,6 +786,7 @@ void buildBsdf(const x2_material& m, V3 N, Bsdf& b) {
         l.albedo = albedo * m.diffuse_weight;
         l.scale = {1, 1, 1};
         l.roughness = 1.0f;
+        l.evalRoughness = 1.0f;
         b.type |= l.type;
     }
     if (m.specular_weight > 0.0f) {
@@ -786,9 +797,96 @@ void buildBsdf(const x2_material& m, V3 N, Bsdf& b) {
         l.albedo = {0, 0, 0};
         l.scale = {m.specular_weight, m.specular_weight, m.specular_weight};
         l.roughness = fmaxf(0.001f, m.roughness * m.roughness);
+        l.evalRoughness = evalRoughness < 0.0f ? l.roughness : fmaxf(0.001f, evalRoughness * evalRoughness);
         b.type |= l.type;
     }
     b.selfEmission = load3(m.emission);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Textures (texture_core.h:144-153 bilinear, :179-181 lookup with unit scale and no offset; level 0, periodic).
+struct TexTap {
+    U32 tex = 0, x0 = 0, y0 = 0;
+    float fs = 0, ft = 0;
+    V3 color{0, 0, 0};
+};
+U32 wrapTexel(int x, U32 n) {
+    const int m = x % int(n);
+    return U32(m < 0 ? m + int(n) : m);
+}
+V3 texelAt(const Texture& t, U32 x, U32 y) { return load3(&t.texels[(size_t(y) * t.width + x) * 3]); }
+bool textureTap(const Texture& t, U32 id, float u, float v, TexTap& tap) {
+    const float su = 1.0f * u + 0.0f, tv = 1 - (1.0f * v + 0.0f);
+    float fs = su * float(t.width) - .5f, ft = tv * float(t.height) - .5f;
+    if (!finite(fs) || !finite(ft) || fabsf(fs) > 1.e8f || fabsf(ft) > 1.e8f) return false;
+    const int x = int(floorf(fs)), y = int(floorf(ft));
+    fs -= float(x);
+    ft -= float(y);
+    const U32 x0 = wrapTexel(x, t.width), x1 = wrapTexel(x + 1, t.width);
+    const U32 y0 = wrapTexel(y, t.height), y1 = wrapTexel(y + 1, t.height);
+    const V3 a = texelAt(t, x0, y0), b = texelAt(t, x1, y0), c = texelAt(t, x0, y1), d = texelAt(t, x1, y1);
+    tap.color = ((a * (1 - fs)) + (b * fs)) * (1 - ft) + ((c * (1 - fs)) + (d * fs)) * ft;
+    tap.tex = id;
+    tap.x0 = x0;
+    tap.y0 = y0;
+    tap.fs = fs;
+    tap.ft = ft;
+    return finite3(tap.color);
+}
+// d ln f / d r of the GGX lobe for MoonRay's input roughness r (alpha = r^2, A = alpha^2 = r^4; 0 where alpha is
+// clamped): D4 ~ A / (A + tan^2 theta_m)^2, G1 = 2 / (1 + sqrt(1 + A tan^2 theta)) for wo and wi.
+float ggxRoughnessLogDerivative(float r, V3 N, V3 wo, V3 wi) {
+    const float alpha = r * r;
+    if (!(alpha > 0.001f)) return 0.0f;
+    const float A = alpha * alpha;
+    const float co = dot(N, wo), ci = dot(N, wi);
+    if (co <= 0.0f || ci <= 0.0f) return 0.0f;
+    V3 m = wo + wi;
+    const float l = length(m);
+    if (!(l > 0.001f)) return 0.0f;
+    m = m * (1.0f / l);
+    const float cm = dot(m, N);
+    if (cm <= 0.0f) return 0.0f;
+    const float tm = (1.0f - cm * cm) / (cm * cm), to = (1.0f - co * co) / (co * co), ti = (1.0f - ci * ci) / (ci * ci);
+    const float so = sqrtf(1.0f + A * to), si = sqrtf(1.0f + A * ti);
+    const float dA = 1.0f / A - 2.0f / (A + tm) - to / (2.0f * so * (1.0f + so)) - ti / (2.0f * si * (1.0f + si));
+    return dA * (4.0f * r * r * r);
+}
+// The adjoint's gradient sinks and open subtrees (the device's bookkeeping, accumulated here in FP64).
+enum : U32 { SinkNone = 0, SinkTexture = 1, SinkRoughness = 2 };
+struct Sink {
+    U32 kind = SinkNone, material = 0;
+    float g = 0;
+    TexTap tap;
+};
+struct Record {
+    U32 pos;
+    V3 mark;
+    Sink sink;
+};
+struct Adjoint {
+    const float* dImage = nullptr;  // per pixel rgb
+    double* gradTextures = nullptr;
+    double* gradRoughness = nullptr;
+};
+void scatterTexture(const Scene& S, const Adjoint& A, const TexTap& t, V3 R, V3 delta) {
+    const Texture& d = S.textures[t.tex];
+    const float cs[3] = {t.color.x > 0.0f ? delta.x * R.x / t.color.x : 0.0f,
+                         t.color.y > 0.0f ? delta.y * R.y / t.color.y : 0.0f,
+                         t.color.z > 0.0f ? delta.z * R.z / t.color.z : 0.0f};
+    if (cs[0] == 0.0f && cs[1] == 0.0f && cs[2] == 0.0f) return;
+    const U32 x1 = t.x0 + 1 == d.width ? 0 : t.x0 + 1, y1 = t.y0 + 1 == d.height ? 0 : t.y0 + 1;
+    const U32 xs[4] = {t.x0, x1, t.x0, x1}, ys[4] = {t.y0, t.y0, y1, y1};
+    const float ws[4] = {(1 - t.fs) * (1 - t.ft), t.fs * (1 - t.ft), (1 - t.fs) * t.ft, t.fs * t.ft};
+    for (int k = 0; k < 4; ++k) {
+        double* q = A.gradTextures + d.gradOffset + (size_t(ys[k]) * d.width + xs[k]) * 3;
+        for (int c = 0; c < 3; ++c) q[c] += double(cs[c] * ws[k]);
+    }
+}
+void scatterSink(const Scene& S, const Adjoint& A, const Sink& s, V3 R, V3 delta) {
+    if (s.kind == SinkTexture) scatterTexture(S, A, s.tap, R, delta);
+    else if (s.kind == SinkRoughness)
+        A.gradRoughness[s.material] += double((delta.x * R.x + delta.y * R.y + delta.z * R.z) * s.g);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -849,7 +947,8 @@ bool visible(const Scene& s, V3 o, V3 d, float nearT, float farT, Counters& k) {
     return traverse(s, o, d, nearT, farT, true, false, t) == ~0u;
 }
 
-int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double out[4], Counters& k) {
+int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double out[4], Counters& k,
+              const Adjoint* adj = nullptr) {
     if (pixel >= c.width * c.height || c.bsdf_samples == 0 || c.bsdf_samples > 64 || c.light_samples == 0 ||
         c.light_samples > 64)
         return 2;
@@ -874,8 +973,25 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
     V3 radiance{0, 0, 0};
     float alpha = 0.0f;
     const U32 rrMin = c.rr_min_depth > 1 ? c.rr_min_depth : 1u;
+    // The adjoint (adj): the pixel's adjoint per sample, each pending ray's sink by stack position, and the open
+    // subtrees; a subtree's radiance is what the running radiance gains until a pop below its position.
+    V3 delta{0, 0, 0};
+    Sink sinks[MaxPending];
+    std::vector<Record> recs;
+    if (adj) {
+        const float inv = 1.0f / float(c.spp);
+        delta = V3{adj->dImage[size_t(pixel) * 3] * inv, adj->dImage[size_t(pixel) * 3 + 1] * inv,
+                   adj->dImage[size_t(pixel) * 3 + 2] * inv};
+    }
     while (top) {
         Pending pv = stack[--top];
+        if (adj) {
+            while (!recs.empty() && recs.back().pos > top) {
+                scatterSink(S, *adj, recs.back().sink, radiance - recs.back().mark, delta);
+                recs.pop_back();
+            }
+            if (recs.size() < 34) recs.push_back(Record{top, radiance, sinks[top]});
+        }
         const bool fromCamera = pv.depth == 0;
         const Ray ray{pv.origin, pv.dir, fromCamera ? camera.nearT : pv.nearT, fromCamera ? camera.farT : pv.farT};
         const Hit hit = trace(S, ray, k);
@@ -912,7 +1028,26 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
         }
         if (!finite3(N) || !finite3(Ng)) return 2;
         Bsdf bsdf;
-        buildBsdf(S.materials[S.material[tri]], N, bsdf);
+        const x2_material& mat = S.materials[S.material[tri]];
+        V3 albedo = load3(mat.albedo);
+        Sink texSink, roughSink;
+        if (mat.albedo_map) {  // the map at the hit's (s, t) (shade_material.h:121-160; cpu_jobs.h:46 apply)
+            float uv[6] = {0, 0, 1, 0, 1, 1};
+            if (!S.triUV.empty())
+                for (int q = 0; q < 6; ++q) uv[q] = S.triUV[size_t(tri) * 6 + q];
+            const float w = 1.0f - hit.u - hit.v;
+            const float ss = w * uv[0] + hit.u * uv[2] + hit.v * uv[4], tt = w * uv[1] + hit.u * uv[3] + hit.v * uv[5];
+            TexTap tap;
+            if (!textureTap(S.textures[mat.albedo_map - 1], mat.albedo_map - 1, ss, tt, tap)) return 2;
+            albedo = albedo * tap.color;
+            texSink.kind = SinkTexture;
+            texSink.tap = tap;
+        }
+        if (mat.specular_weight > 0.0f && mat.roughness * mat.roughness > 0.001f) {
+            roughSink.kind = SinkRoughness;
+            roughSink.material = S.material[tri];
+        }
+        buildBsdf(mat, N, bsdf, albedo, S.evalRoughness.empty() ? -1.0f : S.evalRoughness[S.material[tri]]);
         if (pv.depth == 0) alpha += 1.0f * hitAlpha;
         radiance = radiance + pv.T * bsdf.selfEmission;
         if (bsdf.count == 0) continue;
@@ -965,6 +1100,7 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                 const V3 factor = pv.T * Li * (1.0f / (float(nl) * pdfL * 1.0f));
                 const U32 flags = surfaceFlags(slice, wi);
                 V3 t{0, 0, 0};
+                V3 tkq[2] = {{0, 0, 0}, {0, 0, 0}};
                 bool any = false;
                 for (U32 q = 0; q < bsdf.count; ++q) {
                     const Lobe& lobe = bsdf.lobes[q];
@@ -976,13 +1112,16 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                     V3 tk = factor * f * powerHeuristic(float(nl) * pdfL, float(lobeSamples[q]) * pdf);
                     if (clampHere) tk = smartClamp(tk, c.clamp_value);
                     t = t + tk;
+                    tkq[q] = tk;
                     any = true;
                 }
                 if (!any) continue;
+                float rrScale = 1.0f;
                 if (rr) {
                     float scale;
                     if (!roulette(luminance(t), c.rr_threshold, u, scale)) continue;
                     t = t * scale;
+                    rrScale = scale;
                 }
                 if (!finite3(t) || isExactlyZero(t)) continue;
                 const V3 shadowOrigin = dot(wi, Ng) >= 0.0f ? above : below;
@@ -992,6 +1131,16 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                 if (kept > 0.0f) {
                     if (kept < 1.0f) t = t * kept;
                     radiance = radiance + t;
+                    if (adj)
+                        for (U32 q = 0; q < bsdf.count; ++q) {
+                            const V3 share = tkq[q] * rrScale;
+                            if (bsdf.lobes[q].kind == KindLambert && texSink.kind == SinkTexture)
+                                scatterTexture(S, *adj, texSink.tap, share, delta);
+                            if (bsdf.lobes[q].kind == KindGGX && roughSink.kind == SinkRoughness)
+                                adj->gradRoughness[roughSink.material] +=
+                                    double((delta.x * share.x + delta.y * share.y + delta.z * share.z) *
+                                           ggxRoughnessLogDerivative(mat.roughness, bsdf.lobes[q].frame.z, wo, wi));
+                        }
                 }
             }
         }
@@ -1072,9 +1221,17 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                     if (!held) kept = visible(S, origin, wi, shadowNear, shadowFar, k) ? 1.0f : 0.0f;
                 }
                 const V3 directBefore = direct;
+                Sink lobeSink;
+                if (lobe.kind == KindLambert) {
+                    lobeSink = texSink;
+                } else if (roughSink.kind == SinkRoughness) {
+                    lobeSink = roughSink;
+                    lobeSink.g = ggxRoughnessLogDerivative(mat.roughness, lobe.frame.z, wo, wi);
+                }
                 if (kept > 0.0f && !held) {
                     if (kept < 1.0f) direct = direct * kept;
                     radiance = radiance + direct;
+                    if (adj) scatterSink(S, *adj, lobeSink, direct, delta);
                 }
                 if (!bounces) continue;
                 float keptShare = 1.0f;
@@ -1092,6 +1249,7 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                         keptShare *= tight;
                     }
                 }
+                sinks[top] = lobeSink;
                 Pending child{};
                 child.origin = origin;
                 child.dir = wi;
@@ -1114,6 +1272,11 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
             }
         }
     }
+    if (adj)
+        while (!recs.empty()) {
+            scatterSink(S, *adj, recs.back().sink, radiance - recs.back().mark, delta);
+            recs.pop_back();
+        }
     out[0] = radiance.x;
     out[1] = radiance.y;
     out[2] = radiance.z;
@@ -1197,6 +1360,51 @@ void renderChunks(const Scene& S, const x2_config& c, const std::vector<x2_chunk
     counters[1] += shadows;
     counters[2] += errors;
 }
+// The adjoint over every sample of a uniform frame, tiles over worker threads, each with its own FP64 gradients,
+// summed at the end.
+void adjointFrame(const Scene& S, const x2_config& c, U32 threads, const float* dImage, double* gradTex, double* gradR,
+                  U64 counters[3]) {
+    const U32 tiles = ((c.width + 7) / 8) * ((c.height + 7) / 8);
+    size_t texFloats = 0;
+    for (const Texture& t : S.textures) texFloats += size_t(t.width) * t.height * 3;
+    threads = std::max<U32>(1, threads);
+    std::vector<std::vector<double>> gt(threads, std::vector<double>(texFloats, 0.0)), gr(threads, std::vector<double>(S.materials.size(), 0.0));
+    std::atomic<U32> next{0};
+    std::atomic<U64> traces{0}, shadows{0}, errors{0};
+    auto worker = [&](U32 w) {
+        Counters k;
+        U64 errs = 0;
+        Adjoint A;
+        A.dImage = dImage;
+        A.gradTextures = gt[w].data();
+        A.gradRoughness = gr[w].data();
+        for (;;) {
+            const U32 tile = next.fetch_add(1);
+            if (tile >= tiles) break;
+            for (U32 s = 0; s < c.spp; ++s)
+                for (U32 pos = 0; pos < 64; ++pos) {
+                    const U32 pixel = tilePixel(tile, pos, c);
+                    if (pixel == ~0u) continue;
+                    double rgba[4];
+                    if (integrate(S, c, pixel, s, rgba, k, &A) != 0) ++errs;
+                }
+        }
+        traces += k.traces;
+        shadows += k.shadows;
+        errors += errs;
+    };
+    std::vector<std::thread> pool;
+    for (U32 t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& t : pool) t.join();
+    for (U32 w = 0; w < threads; ++w) {
+        for (size_t i = 0; i < texFloats; ++i) gradTex[i] += gt[w][i];
+        for (size_t i = 0; i < S.materials.size(); ++i) gradR[i] += gr[w][i];
+    }
+    counters[0] += traces;
+    counters[1] += shadows;
+    counters[2] += errors;
+}
 }  // namespace
 
 struct oracle_scene {
@@ -1238,7 +1446,7 @@ static Lobe testLobe(uint32_t kind, const float N[3], float roughness) {
     m.diffuse_weight = kind == 0 ? 1.0f : 0.0f;
     m.specular_weight = kind == 1 ? 1.0f : 0.0f;
     Bsdf b;
-    buildBsdf(m, load3(N), b);
+    buildBsdf(m, load3(N), b, load3(m.albedo));
     return b.lobes[0];
 }
 void oracle_lobe_eval(uint32_t kind, const float N[3], const float Ng[3], const float wo[3], const float wi[3],
@@ -1322,6 +1530,26 @@ oracle_scene* oracle_scene_create(const x2_scene* in) {
         }
     for (U32 t = 0; t < in->triangle_count; ++t)
         for (int v = 0; v < 3; ++v) s.tri[size_t(t) * 3 + v] = load3(in->positions + size_t(in->indices[size_t(t) * 3 + v]) * 3);
+    if (in->uvs) {
+        s.triUV.resize(size_t(in->triangle_count) * 6);
+        for (U32 t = 0; t < in->triangle_count; ++t)
+            for (int v = 0; v < 3; ++v) {
+                const float* uv = in->uvs + size_t(in->indices[size_t(t) * 3 + v]) * 2;
+                s.triUV[size_t(t) * 6 + v * 2] = uv[0];
+                s.triUV[size_t(t) * 6 + v * 2 + 1] = uv[1];
+            }
+    }
+    size_t gradOffset = 0;
+    for (U32 i = 0; i < in->texture_count; ++i) {
+        const x2_texture& t = in->textures[i];
+        Texture tex;
+        tex.width = t.width;
+        tex.height = t.height;
+        tex.texels.assign(t.texels, t.texels + size_t(t.width) * t.height * 3);
+        tex.gradOffset = gradOffset;
+        gradOffset += size_t(t.width) * t.height * 3;
+        s.textures.push_back(std::move(tex));
+    }
     s.materials.assign(in->materials, in->materials + in->material_count);
     if (in->light_count) s.lights.assign(in->lights, in->lights + in->light_count);
     buildBvh(s);
@@ -1353,6 +1581,34 @@ int oracle_render(const oracle_scene* o, const x2_config* c, uint32_t threads, d
     for (U32 t = 0; t < tiles; ++t) chunks.push_back({t, 0, c->spp, 0});
     renderChunks(o->s, *c, chunks, threads, sums, counts, counters);
     return 0;
+}
+int oracle_adjoint(const oracle_scene* o, const x2_config* c, uint32_t threads, const float* d_image,
+                   double* grad_textures, double* grad_roughness, uint64_t counters[3]) {
+    adjointFrame(o->s, *c, threads, d_image, grad_textures, grad_roughness, counters);
+    return 0;
+}
+void oracle_set_texture(oracle_scene* o, uint32_t texture, const float* texels) {
+    Texture& t = o->s.textures.at(texture);
+    t.texels.assign(texels, texels + size_t(t.width) * t.height * 3);
+}
+void oracle_set_material(oracle_scene* o, uint32_t material, const x2_material* m) { o->s.materials.at(material) = *m; }
+void oracle_set_eval_roughness(oracle_scene* o, uint32_t material, float r) {
+    if (o->s.evalRoughness.empty()) o->s.evalRoughness.assign(o->s.materials.size(), -1.0f);
+    o->s.evalRoughness.at(material) = r;
+}
+float oracle_ggx_roughness_log_derivative(float r, const float N[3], const float wo[3], const float wi[3]) {
+    return ggxRoughnessLogDerivative(r, load3(N), load3(wo), load3(wi));
+}
+int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]) {
+    Texture tex;
+    tex.width = t->width;
+    tex.height = t->height;
+    tex.texels.assign(t->texels, t->texels + size_t(t->width) * t->height * 3);
+    tex.gradOffset = 0;
+    TexTap tap;
+    if (!textureTap(tex, 0, u, v, tap)) return 0;
+    store3(rgb, tap.color);
+    return 1;
 }
 int oracle_render_chunks(const oracle_scene* o, const x2_config* c, const x2_chunk* chunks, uint32_t n,
                          uint32_t threads, double* sums, uint32_t* counts, uint64_t counters[3]) {

@@ -73,6 +73,21 @@ int oracle_render(const oracle_scene* s, const x2_config* c, uint32_t threads, d
 int oracle_render_chunks(const oracle_scene* s, const x2_config* c, const x2_chunk* chunks, uint32_t n,
                          uint32_t threads, double* sums, uint32_t* counts, uint64_t counters[3]);
 
+/* The path-replay adjoint of a uniform frame (the device's algorithm, FP64 gradients): d_image is dLoss / d(mean
+ * rgb) per pixel; grad_textures (the scene's textures' texels concatenated) and grad_roughness (per material) are
+ * added to. counters: traces, shadow rays, errors. */
+int oracle_adjoint(const oracle_scene* s, const x2_config* c, uint32_t threads, const float* d_image,
+                   double* grad_textures, double* grad_roughness, uint64_t counters[3]);
+/* Parameter updates (the device's x2dev_set_texture / x2dev_set_material). */
+void oracle_set_texture(oracle_scene* s, uint32_t texture, const float* texels);
+void oracle_set_material(oracle_scene* s, uint32_t material, const x2_material* m);
+// Test hook: the GGX lobe's value (not its sampling, pdf or MIS weights) uses roughness r for this material;
+// r < 0 restores the material's own.
+void oracle_set_eval_roughness(oracle_scene* s, uint32_t material, float r);
+/* d ln f / d r of the GGX lobe, and one bilinear texture lookup at (u, v). */
+float oracle_ggx_roughness_log_derivative(float r, const float N[3], const float wo[3], const float wi[3]);
+int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]);
+
 #ifdef __cplusplus
 }
 #endif

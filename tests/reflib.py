@@ -33,6 +33,7 @@ def load():
         "ref_light_can_illuminate": (c_int, [vp, P(f32), P(f32)]),
         "ref_lobe_eval": (None, [u32, P(f32), P(f32), P(f32), P(f32), f32, P(f32), P(f32)]),
         "ref_lobe_sample": (None, [u32, P(f32), P(f32), P(f32), f32, f32, f32, P(f32), P(f32), P(f32)]),
+        "ref_bilinear": (c_int, [u32, u32, P(f32), f32, f32, P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,194 @@
+"""The textured scene and the path-replay adjoint on the device, through the C-ABI.
+
+- forward parity of the textured Cornell box against the oracle (counts exact, radiance within 1e-3);
+- the device adjoint against the oracle's FP64 adjoint at the same sample identities;
+- the device adjoint against central differences (h = 1e-3) of the device's own renders on 64 random texels of the
+  differentiable Cornell box at its full size (512 x 512, 32 spp, depth 8, 512 x 512 texture, L2 loss against a
+  target rendered from parameters perturbed by N(0, 0.1)): agreement within 1e-2 relative;
+- the roughness gradients against differences taken with the sampling held fixed (the estimator the adjoint
+  differentiates; the oracle's evalRoughness hook), within 1e-2 relative;
+- the torch.autograd.Function over the same entry points.
+Device buffers come from torch (device memory only)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from helpers import mean_relative_difference
+from x2amd import abi, scenes
+from x2amd.lane import GpuLane
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def image_of(lane, config):
+    sums, counts, st = lane.render_frame(config)
+    assert (counts == config.spp).all() and st.errors == 0
+    return sums[:, :3] / counts[:, None].astype(np.float64)
+
+
+def device_adjoint(lane, scene, config, d_image, texture_floats):
+    g = torch.from_numpy(np.ascontiguousarray(d_image, np.float32).reshape(-1)).to(DEV)
+    gt = torch.zeros(max(1, texture_floats), dtype=torch.float32, device=DEV)
+    gr = torch.zeros(scene.material_count, dtype=torch.float32, device=DEV)
+    torch.cuda.synchronize()
+    st = lane.adjoint_frame(config, g.data_ptr(), gt.data_ptr(), gr.data_ptr())
+    assert st.errors == 0
+    return gt.cpu().numpy().astype(np.float64)[:texture_floats], gr.cpu().numpy().astype(np.float64), st
+
+
+@pytest.fixture(scope="module")
+def small():
+    scene, config, params = scenes.cornell_textured(width=128, height=128, spp=8, max_depth=8)
+    lane = GpuLane(0)
+    lane.prepare(scene)
+    yield scene, config, params, lane
+    lane.close()
+
+
+def test_textured_forward_parity(small):
+    scene, config, params, lane = small
+    sums, counts, st = lane.render_frame(config)
+    ref_sums, ref_counts, ref_k = oracle.OracleScene(scene).render(config)
+    assert (counts == config.spp).all() and (ref_counts == config.spp).all()
+    rel = mean_relative_difference(sums, counts, ref_sums, ref_counts)
+    print(f"textured cornell 128x128x8: rel {rel:.3e}, traces {st.rays_trace} vs {ref_k[0]}")
+    assert rel <= 1e-3
+    assert st.rays_trace == ref_k[0] and st.rays_shadow == ref_k[1]
+
+
+def test_adjoint_matches_oracle_fp64(small):
+    scene, config, params, lane = small
+    rng = np.random.default_rng(8)
+    d_image = rng.normal(size=(config.width * config.height, 3)).astype(np.float32)
+    n = params["texture"].size
+    gt, gr, st = device_adjoint(lane, scene, config, d_image, n)
+    ot, orr, ok = oracle.OracleScene(scene).adjoint(config, d_image, n, scene.material_count)
+    assert ok[2] == 0
+    assert st.rays_trace == ok[0] and st.rays_shadow == ok[1], "the replay traces the forward paths"
+    err = np.linalg.norm(gt - ot) / np.linalg.norm(ot)
+    print(f"texel gradient vs FP64 oracle: relative L2 {err:.3e}; roughness {gr[1:5]} vs {orr[1:5]}")
+    assert err <= 1e-4
+    assert np.all(np.abs(gt - ot) <= 1e-3 * np.abs(ot) + 1e-5 * np.abs(ot).max())
+    assert np.allclose(gr, orr, rtol=1e-4, atol=1e-6 * np.abs(orr).max())
+
+
+def test_roughness_adjoint_matches_detached_differences(small):
+    """Differences over the lobe's value with sampling held (the oracle's hook) at the same sample identities."""
+    scene, config, params, lane = small
+    rng = np.random.default_rng(9)
+    d_image = rng.normal(size=(config.width * config.height, 3)).astype(np.float32)
+    _, gr, _ = device_adjoint(lane, scene, config, d_image, params["texture"].size)
+    o = oracle.OracleScene(scene)
+    h = 1e-3
+
+    def loss():
+        sums, counts, _ = o.render(config)
+        return float(np.sum(sums[:, :3] / counts[:, None] * d_image))
+
+    for m, r in zip(params["roughness_materials"], params["roughness"]):
+        o.set_eval_roughness(m, float(r) + h)
+        lp = loss()
+        o.set_eval_roughness(m, float(r) - h)
+        lm = loss()
+        o.set_eval_roughness(m, -1.0)
+        fd = (lp - lm) / (2 * h)
+        print(f"material {m}: device adjoint {gr[m]:.6e}, detached difference {fd:.6e}")
+        assert abs(gr[m] - fd) <= 1e-2 * abs(fd), (m, gr[m], fd)
+
+
+@pytest.fixture(scope="module")
+def full():
+    """The differentiable Cornell box at full size and its target."""
+    scene, config, params = scenes.cornell_textured()
+    lane = GpuLane(0)
+    lane.prepare(scene)
+    rng = np.random.default_rng(1234)
+    tex_t = np.clip(params["texture"] + rng.normal(0, 0.1, params["texture"].shape), 0.01, 1.0).astype(np.float32)
+    rough_t = np.clip(params["roughness"] + rng.normal(0, 0.1, 4), 0.05, 0.95)
+    lane.set_texture(0, tex_t)
+    for m, r in zip(params["roughness_materials"], rough_t):
+        mat = abi.Material.from_buffer_copy(scene.materials[m])
+        mat.roughness = float(r)
+        lane.set_material(m, mat)
+    target = image_of(lane, config)
+    lane.set_texture(0, params["texture"])
+    for m in params["roughness_materials"]:
+        lane.set_material(m, scene.materials[m])
+    yield scene, config, params, lane, target
+    lane.close()
+
+
+def test_texel_gradients_match_central_differences(full):
+    scene, config, params, lane, target = full
+    image = image_of(lane, config)
+    loss0 = float(np.sum((image - target) ** 2))
+    d_image = (2 * (image - target)).astype(np.float32)
+    tex = params["texture"]
+    gt, gr, st = device_adjoint(lane, scene, config, d_image, tex.size)
+    rng = np.random.default_rng(64)
+    picks = rng.choice(tex.size, 64, replace=False)
+    h = 1e-3
+    fd = []
+    flat = tex.reshape(-1).copy()
+    for i in picks:
+        flat[i] = tex.reshape(-1)[i] + h
+        lane.set_texture(0, flat.reshape(tex.shape))
+        lp = float(np.sum((image_of(lane, config) - target) ** 2))
+        flat[i] = tex.reshape(-1)[i] - h
+        lane.set_texture(0, flat.reshape(tex.shape))
+        lm = float(np.sum((image_of(lane, config) - target) ** 2))
+        flat[i] = tex.reshape(-1)[i]
+        fd.append((lp - lm) / (2 * h))
+    lane.set_texture(0, tex)
+    fd, adj = np.array(fd), gt[picks]
+    rel = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-12)
+    big = np.abs(fd) >= 1e-2 * np.abs(fd).max()
+    print(f"loss {loss0:.6e}; 64 texels: median rel {np.median(rel):.3e}, max rel (|fd| >= 1% of max) "
+          f"{rel[big].max():.3e}, vector rel {np.linalg.norm(adj - fd) / np.linalg.norm(fd):.3e}")
+    assert np.linalg.norm(adj - fd) <= 1e-2 * np.linalg.norm(fd)
+    assert np.all(np.abs(adj - fd) <= 1e-2 * np.abs(fd) + 1e-4 * np.abs(fd).max())
+
+
+def test_autograd_function_steps_downhill(small):
+    from x2amd.diff import DiffRenderer
+    scene, config, params, _ = small
+    r = DiffRenderer(scene, config, params["roughness_materials"])
+    try:
+        tex = torch.tensor(params["texture"], device=DEV, requires_grad=True)
+        rough = torch.tensor(params["roughness"], device=DEV, requires_grad=True)
+        with torch.no_grad():
+            target = r(torch.clamp(tex + 0.1 * torch.randn_like(tex), 0.01, 1.0), rough + 0.05)
+        image = r(tex, rough)
+        loss = ((image - target) ** 2).sum()
+        loss.backward()
+        assert tex.grad is not None and rough.grad is not None
+        assert torch.isfinite(tex.grad).all() and torch.isfinite(rough.grad).all()
+        # the Function's backward is the adjoint entry point with dLoss/dimage = 2 (image - target)
+        gt, gr = r.adjoint(tex, rough, 2 * (image.detach() - target))
+        assert torch.allclose(tex.grad, gt, rtol=1e-4, atol=1e-6 * float(gt.abs().max()))
+        assert torch.allclose(rough.grad, gr, rtol=1e-4)
+        with torch.no_grad():
+            step = 0.5 * float(loss) / float((tex.grad ** 2).sum())  # a small fraction of the linearised distance
+            after = ((r(tex - 0.1 * step * tex.grad, rough) - target) ** 2).sum()
+        print(f"loss {float(loss):.6e} -> {float(after):.6e} after one texture step")
+        assert float(after) < float(loss)
+    finally:
+        r.close()
+
+
+def test_adjoint_argument_errors(small):
+    scene, config, params, lane = small
+    g = torch.zeros(config.width * config.height * 3, device=DEV)
+    gr = torch.zeros(scene.material_count, device=DEV)
+    with pytest.raises(RuntimeError):
+        lane.adjoint_frame(config, g.data_ptr(), 0, gr.data_ptr())  # texture gradient missing
+    deep = scenes.cornell_textured(width=128, height=128, spp=8, max_depth=40)[1]
+    gt = torch.zeros(params["texture"].size, device=DEV)
+    with pytest.raises(RuntimeError):
+        lane.adjoint_frame(deep, g.data_ptr(), gt.data_ptr(), gr.data_ptr())
+    with pytest.raises(RuntimeError):
+        lane.set_texture(3, params["texture"])  # no such texture

@@ -199,6 +199,7 @@ bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
     // Collapse to BVH8. Each wide node opens its children greedily by area until it has 8.
     out.nodes.clear();
     out.tris.clear();
+    out.triUV.clear();
     out.tris.reserve(n);
     out.maxStack = 0;
     out.depth = 0;
@@ -227,6 +228,12 @@ bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
             r.prim = primOf[t];
             r.material = materialOf[t];
             out.tris.push_back(r);
+            if (s.uvs)
+                for (int v = 0; v < 3; ++v) {
+                    const float* uv = s.uvs + size_t(s.indices[size_t(t) * 3 + v]) * 2;
+                    out.triUV.push_back(uv[0]);
+                    out.triUV.push_back(uv[1]);
+                }
         }
         return LeafBit | (first << 3) | (leaf.count - 1);
     };

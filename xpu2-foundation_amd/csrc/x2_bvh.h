@@ -9,6 +9,7 @@ namespace x2 {
 struct Bvh8Result {
     std::vector<Node8> nodes;
     std::vector<TriRecord> tris;
+    std::vector<float> triUV;  // 6 floats per triangle record when the scene has uvs
     uint32_t maxStack = 0, depth = 0;  // traversal stack entries a ray can need; wide-tree depth
 };
 bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err);

@@ -363,7 +363,7 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
 // the point rounded once, conservative offset origins above and below the surface.
 struct Hit {
     uint32_t geom, tri;
-    float t;
+    float t, u, v;
     V3 normal, origin, below, point;
 };
 __device__ bool geometryHit(Hit& h, const Ray& ray, V3 a, V3 b, V3 c) {
@@ -384,6 +384,8 @@ __device__ bool geometryHit(Hit& h, const Ray& ray, V3 a, V3 b, V3 c) {
     const double pt = ((fx * qx + fy * qy) + fz * qz) / determinant;
     if (!finite1(float(pt)) || !finite1(float(pu)) || !finite1(float(pv))) return false;
     h.t = float(pt);
+    h.u = float(pu);
+    h.v = float(pv);
     const V3 p{float(ax + ((double(b.x) - ax) * pu + (double(c.x) - ax) * pv)),
                float(ay + ((double(b.y) - ay) * pu + (double(c.y) - ay) * pv)),
                float(az + ((double(b.z) - az) * pu + (double(c.z) - az) * pv))};
@@ -605,6 +607,106 @@ __device__ V3 lobeSample(const Bsdf& b, uint32_t q, V3 wo, float r1, float r2, V
 
 // ---------------------------------------------------------------------------------------------------------------
 // The estimator (async_integrator.h:1986-3042, the subset above). Pending rays live in the lane's private memory.
+// ---------------------------------------------------------------------------------------------------------------
+// Textures (texture_core.h:144-153 bilinear, :179-181 lookup with unit scale and no offset, level 0, periodic).
+struct TexTap {
+    uint32_t tex, x0, y0;
+    float fs, ft;
+    V3 color;
+};
+__device__ __forceinline__ uint32_t wrapTexel(int x, uint32_t n) {
+    const int m = x % int(n);
+    return uint32_t(m < 0 ? m + int(n) : m);
+}
+__device__ __forceinline__ V3 texel(const TexDesc& t, uint32_t x, uint32_t y) {
+    const float* p = t.texels + (size_t(y) * t.width + x) * 3;
+    return {p[0], p[1], p[2]};
+}
+__device__ bool textureTap(const TexDesc& t, uint32_t id, float u, float v, TexTap& tap) {
+    const float su = 1.0f * u + 0.0f, tv = 1 - (1.0f * v + 0.0f);
+    float fs = su * float(t.width) - .5f, ft = tv * float(t.height) - .5f;
+    if (!finite1(fs) || !finite1(ft) || fabsf(fs) > 1.e8f || fabsf(ft) > 1.e8f) return false;
+    const int x = int(floorf(fs)), y = int(floorf(ft));
+    fs -= float(x);
+    ft -= float(y);
+    const uint32_t x0 = wrapTexel(x, t.width), x1 = wrapTexel(x + 1, t.width);
+    const uint32_t y0 = wrapTexel(y, t.height), y1 = wrapTexel(y + 1, t.height);
+    const V3 a = texel(t, x0, y0), b = texel(t, x1, y0), c = texel(t, x0, y1), d = texel(t, x1, y1);
+    tap.color = ((a * (1 - fs)) + (b * fs)) * (1 - ft) + ((c * (1 - fs)) + (d * fs)) * ft;
+    tap.tex = id;
+    tap.x0 = x0;
+    tap.y0 = y0;
+    tap.fs = fs;
+    tap.ft = ft;
+    return finite3(tap.color);
+}
+
+// d ln f / d r of the GGX lobe (shade_lobes_eval.h:93-121) at (wo, wi) for MoonRay's input roughness r (the lobe's
+// alpha = r^2, alpha^2 = A = r^4; 0 where alpha is clamped to 0.001): D4 ~ A / (A + tan^2 theta_m)^2 and Smith's
+// G1 = 2 / (1 + sqrt(1 + A tan^2 theta)) for wo and wi; scale and 1 / cos(theta_o) do not depend on r.
+__device__ float ggxRoughnessLogDerivative(float r, V3 N, V3 wo, V3 wi) {
+    const float alpha = r * r;
+    if (!(alpha > 0.001f)) return 0.0f;
+    const float A = alpha * alpha;
+    const float co = dot(N, wo), ci = dot(N, wi);
+    if (co <= 0.0f || ci <= 0.0f) return 0.0f;
+    V3 m = wo + wi;
+    const float l = length(m);
+    if (!(l > 0.001f)) return 0.0f;
+    m = m * (1.0f / l);
+    const float cm = dot(m, N);
+    if (cm <= 0.0f) return 0.0f;
+    const float tm = (1.0f - cm * cm) / (cm * cm), to = (1.0f - co * co) / (co * co), ti = (1.0f - ci * ci) / (ci * ci);
+    const float so = sqrtf(1.0f + A * to), si = sqrtf(1.0f + A * ti);
+    const float dA = 1.0f / A - 2.0f / (A + tm) - to / (2.0f * so * (1.0f + so)) - ti / (2.0f * si * (1.0f + si));
+    return dA * (4.0f * r * r * r);
+}
+
+// The adjoint's bookkeeping. A gradient sink is what a contribution's derivative goes to: the texels of a textured
+// Lambert lobe (its bilinear taps) or a material's roughness (with the GGX log-derivative at the lobe's direction).
+enum : uint32_t { SinkNone = 0, SinkTexture = 1, SinkRoughness = 2 };
+struct Sink {
+    uint32_t kind, material;
+    float g;
+    TexTap tap;
+};
+// An open subtree: the pending ray popped at stack position pos, the running radiance when it was popped, and the
+// sink of the lobe that made it. Its subtree's radiance is what the running radiance gains until the next pop at a
+// position below pos (the stack is last in, first out).
+struct Record {
+    uint32_t pos;
+    V3 mark;
+    Sink sink;
+};
+constexpr uint32_t MaxRecords = 34;
+
+__device__ void scatterTexture(const RenderParams& P, const TexTap& t, V3 R, V3 delta) {
+    const TexDesc& d = P.textures[t.tex];
+    const float cs[3] = {t.color.x > 0.0f ? delta.x * R.x / t.color.x : 0.0f,
+                         t.color.y > 0.0f ? delta.y * R.y / t.color.y : 0.0f,
+                         t.color.z > 0.0f ? delta.z * R.z / t.color.z : 0.0f};
+    if (cs[0] == 0.0f && cs[1] == 0.0f && cs[2] == 0.0f) return;
+    const uint32_t x1 = t.x0 + 1 == d.width ? 0 : t.x0 + 1, y1 = t.y0 + 1 == d.height ? 0 : t.y0 + 1;
+    const uint32_t xs[4] = {t.x0, x1, t.x0, x1}, ys[4] = {t.y0, t.y0, y1, y1};
+    const float ws[4] = {(1 - t.fs) * (1 - t.ft), t.fs * (1 - t.ft), (1 - t.fs) * t.ft, t.fs * t.ft};
+    float* g = P.gradTextures + d.gradOffset;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float* q = g + (size_t(ys[k]) * d.width + xs[k]) * 3;
+        atomicAdd(q + 0, cs[0] * ws[k]);
+        atomicAdd(q + 1, cs[1] * ws[k]);
+        atomicAdd(q + 2, cs[2] * ws[k]);
+    }
+}
+__device__ __forceinline__ void addRoughness(const RenderParams& P, uint32_t m, float v, float* gr) {
+    if (m < RoughnessSlots) gr[m] += v;
+    else atomicAdd(P.gradRoughness + m, v);
+}
+__device__ __forceinline__ void scatterSink(const RenderParams& P, const Sink& s, V3 R, V3 delta, float* gr) {
+    if (s.kind == SinkTexture) scatterTexture(P, s.tap, R, delta);
+    else if (s.kind == SinkRoughness) addRoughness(P, s.material, (delta.x * R.x + delta.y * R.y + delta.z * R.z) * s.g, gr);
+}
+
 struct Pending {
     V3 origin, dir, T;
     uint64_t branch;
@@ -658,9 +760,12 @@ __device__ __forceinline__ bool visible(const RenderParams& P, V3 o, V3 d, float
     return !anyHit(P, o, d, nearT, farT, k, st);
 }
 
-// One sample; 0 Done (rgba set), 2 Error.
+// One sample; 0 Done (rgba set), 2 Error. With Adj, the sample is replayed for the adjoint: the same estimator, and
+// every contribution's derivative is carried to its sinks, scaled by the pixel's adjoint (gr: the lane's roughness
+// gradient accumulators).
+template <bool Adj>
 __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, double rgba[4], Counters& k,
-                         TravStack& st) {
+                         TravStack& st, float* gr) {
     const x2_config& c = P.cfg;
     const Identity root{pixel, index, c.seed, 0};
     float x, y;
@@ -689,8 +794,26 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
     V3 radiance{0, 0, 0};
     float alpha = 0.0f;
     const uint32_t rrMin = c.rr_min_depth > 1 ? c.rr_min_depth : 1u;
+    // The adjoint's state: the pixel's adjoint per sample, the sinks of the pending rays by stack position, and the
+    // open subtrees (the ancestors of the vertex being shaded).
+    V3 delta{0, 0, 0};
+    Sink sinks[Adj ? MaxPending : 1];
+    Record recs[Adj ? MaxRecords : 1];
+    uint32_t nrec = 0;
+    if constexpr (Adj) {
+        const float inv = 1.0f / float(c.spp);
+        delta = V3{P.dImage[size_t(pixel) * 3] * inv, P.dImage[size_t(pixel) * 3 + 1] * inv, P.dImage[size_t(pixel) * 3 + 2] * inv};
+        sinks[0].kind = SinkNone;
+    }
     while (top) {
         const Pending pv = stack[--top];
+        if constexpr (Adj) {
+            while (nrec && recs[nrec - 1].pos > top) {
+                --nrec;
+                scatterSink(P, recs[nrec].sink, radiance - recs[nrec].mark, delta, gr);
+            }
+            if (nrec < MaxRecords) recs[nrec++] = Record{top, radiance, sinks[top]};
+        }
         const uint32_t depth = pv.depths & 0xffu, nonMirrorDepth = (pv.depths >> 8) & 0xffu,
                        diffuseDepth = (pv.depths >> 16) & 0xffu, glossyDepth = pv.depths >> 24;
         const bool fromCamera = depth == 0;
@@ -728,12 +851,38 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
         }
         if (!finite3(N) || !finite3(Ng)) return 2;
         Bsdf bsdf;
+        float rInput = 0.0f;  // the material's input roughness (the adjoint's GGX derivative)
+        Sink texSink, roughSink;
+        texSink.kind = SinkNone;
+        roughSink.kind = SinkNone;
         {
             const x2_material& m = P.materials[tr.material];
             bsdf.count = 0;
             bsdf.type = 0;
             bsdf.frame = frameOf(N);
-            bsdf.albedo = ld3(m.albedo) * m.diffuse_weight;
+            V3 albedo = ld3(m.albedo);
+            if (m.albedo_map) {  // the map at the hit's (s, t) (shade_material.h:139-141; cpu_jobs.h:46 apply)
+                float uv[6] = {0, 0, 1, 0, 1, 1};
+                if (P.triUV)
+                    for (int q = 0; q < 6; ++q) uv[q] = P.triUV[size_t(hit.tri) * 6 + q];
+                const float w = 1.0f - hit.u - hit.v;
+                const float ss = w * uv[0] + hit.u * uv[2] + hit.v * uv[4], tt = w * uv[1] + hit.u * uv[3] + hit.v * uv[5];
+                TexTap tap;
+                if (!textureTap(P.textures[m.albedo_map - 1], m.albedo_map - 1, ss, tt, tap)) return 2;
+                albedo = albedo * tap.color;
+                if constexpr (Adj) {
+                    texSink.kind = SinkTexture;
+                    texSink.tap = tap;
+                }
+            }
+            if constexpr (Adj) {
+                if (m.specular_weight > 0.0f && m.roughness * m.roughness > 0.001f) {
+                    roughSink.kind = SinkRoughness;
+                    roughSink.material = tr.material;
+                }
+            }
+            bsdf.albedo = albedo * m.diffuse_weight;
+            rInput = m.roughness;
             bsdf.scale = {m.specular_weight, m.specular_weight, m.specular_weight};
             bsdf.roughness = fmaxf(0.001f, m.roughness * m.roughness);
             if (m.diffuse_weight > 0.0f) {
@@ -798,6 +947,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 const V3 factor = pv.T * Li * (1.0f / (float(nl) * pdfL * 1.0f));
                 const uint32_t flags = surfaceFlags(Ng, wo, wi);
                 V3 t{0, 0, 0};
+                V3 tkq[2] = {{0, 0, 0}, {0, 0, 0}};  // each lobe's share, for the adjoint
                 bool any = false;
                 for (uint32_t q = 0; q < bsdf.count; ++q) {
                     if (!matchesFlags(bsdf.ltype[q], flags)) continue;
@@ -808,18 +958,32 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                     V3 tk = factor * f * powerHeuristic(float(nl) * pdfL, float(nk) * pdf);
                     if (clampHere) tk = smartClamp(tk, c.clamp_value);
                     t = t + tk;
+                    if constexpr (Adj) tkq[q] = tk;
                     any = true;
                 }
                 if (!any) continue;
+                float rrScale = 1.0f;
                 if (rr) {
                     float scale;
                     if (!roulette(luminance(t), c.rr_threshold, u, scale)) continue;
                     t = t * scale;
+                    rrScale = scale;
                 }
                 if (!finite3(t) || isExactlyZero(t)) continue;
                 const V3 shadowOrigin = dot(wi, Ng) >= 0.0f ? above : below;
                 const float shadowNear = 0.0f, shadowFar = fmaxf((lh.distance - 0.0f) * HitEpsilonEnd, 2.0f * shadowNear);
-                if (visible(P, shadowOrigin, wi, shadowNear, shadowFar, k, st)) radiance = radiance + t;
+                if (visible(P, shadowOrigin, wi, shadowNear, shadowFar, k, st)) {
+                    radiance = radiance + t;
+                    if constexpr (Adj) {
+                        for (uint32_t q = 0; q < bsdf.count; ++q) {
+                            const V3 share = tkq[q] * rrScale;
+                            if (bsdf.kind[q] == 0 && texSink.kind == SinkTexture) scatterTexture(P, texSink.tap, share, delta);
+                            if (bsdf.kind[q] == 1 && roughSink.kind == SinkRoughness)
+                                addRoughness(P, roughSink.material, (delta.x * share.x + delta.y * share.y + delta.z * share.z) *
+                                                 ggxRoughnessLogDerivative(rInput, bsdf.frame.z, wo, wi), gr);
+                        }
+                    }
+                }
             }
         }
 
@@ -894,9 +1058,19 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                     if (!held) kept = visible(P, origin, wi, 0.0f, shadowFar, k, st) ? 1.0f : 0.0f;
                 }
                 const V3 directBefore = direct;
+                Sink lobeSink;
+                lobeSink.kind = SinkNone;
+                if constexpr (Adj) {
+                    if (bsdf.kind[q] == 0) lobeSink = texSink;
+                    else if (roughSink.kind == SinkRoughness) {
+                        lobeSink = roughSink;
+                        lobeSink.g = ggxRoughnessLogDerivative(rInput, bsdf.frame.z, wo, wi);
+                    }
+                }
                 if (kept > 0.0f && !held) {
                     if (kept < 1.0f) direct = direct * kept;
                     radiance = radiance + direct;
+                    if constexpr (Adj) scatterSink(P, lobeSink, direct, delta, gr);
                 }
                 if (!bounces) continue;
                 float keptShare = 1.0f;
@@ -914,6 +1088,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                         keptShare *= tight;
                     }
                 }
+                if constexpr (Adj) sinks[top] = lobeSink;
                 Pending& child = stack[top++];
                 child.origin = origin;
                 child.dir = wi;
@@ -926,6 +1101,12 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 child.heldFar = shadowFar;
                 child.held = directBefore;
             }
+        }
+    }
+    if constexpr (Adj) {
+        while (nrec) {
+            --nrec;
+            scatterSink(P, recs[nrec].sink, radiance - recs[nrec].mark, delta, gr);
         }
     }
     rgba[0] = radiance.x;
@@ -967,9 +1148,11 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
 #endif
 
 // One sample per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
-// position; the chunk comes from the mailbox ring.
+// position; the chunk comes from the mailbox ring. The forward render adds the sample to the film; the adjoint
+// replays it and scatters its gradients.
+template <bool Adj>
 __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long long item, Counters& k, TravStack& st,
-                                           uint32_t& errors) {
+                                           uint32_t& errors, float* gr) {
     const unsigned long long chunkIndex = item / P.itemsPerChunk;
     const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
     const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
@@ -977,13 +1160,15 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
     const uint32_t pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
     if (pixel == ~0u) return;
     double rgba[4];
-    if (integrate(P, pixel, chunk.first + s, rgba, k, st) == 0) {
-        double* f = P.sums + size_t(pixel) * 4;
-        atomicAdd(f + 0, rgba[0]);
-        atomicAdd(f + 1, rgba[1]);
-        atomicAdd(f + 2, rgba[2]);
-        atomicAdd(f + 3, rgba[3]);
-        atomicAdd(P.counts + pixel, 1u);
+    if (integrate<Adj>(P, pixel, chunk.first + s, rgba, k, st, gr) == 0) {
+        if constexpr (!Adj) {
+            double* f = P.sums + size_t(pixel) * 4;
+            atomicAdd(f + 0, rgba[0]);
+            atomicAdd(f + 1, rgba[1]);
+            atomicAdd(f + 2, rgba[2]);
+            atomicAdd(f + 3, rgba[3]);
+            atomicAdd(P.counts + pixel, 1u);
+        }
     } else {
         ++errors;
     }
@@ -992,24 +1177,35 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
 // A lane per item (the 64 lanes of a wave are one 8x8 tile at one sample index, so camera rays start coherent), or
 // persistent lanes claiming items one at a time (the reference's Feed loop) when built with X2_PERSISTENT. Every lane
 // reaches the end so the wave's counter sums see all 64 lanes.
-__global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) {
+template <bool Adj>
+__device__ __forceinline__ void runLaunch(const RenderParams& P) {
     __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
     TravStack st;
     st.lds = lds;
     st.lane = threadIdx.x;
     Counters k;
     uint32_t errors = 0;
+    float gr[Adj ? RoughnessSlots : 1];
+    for (uint32_t i = 0; i < (Adj ? RoughnessSlots : 1); ++i) gr[i] = 0.0f;
 #if X2_PERSISTENT
     for (;;) {
         const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
         if (item >= P.itemEnd) break;
-        renderItem(P, item, k, st, errors);
+        renderItem<Adj>(P, item, k, st, errors, gr);
     }
 #else
     const unsigned long long item = P.itemBegin + (unsigned long long)blockIdx.x * 256u + threadIdx.x;
-    if (item < P.itemEnd) renderItem(P, item, k, st, errors);
+    if (item < P.itemEnd) renderItem<Adj>(P, item, k, st, errors, gr);
 #endif
     if (errors) atomicAdd(&P.state->errors, errors);
+    if constexpr (Adj) {
+        for (uint32_t m = 0; m < RoughnessSlots; ++m) {  // one atomic per wave and material
+            float v = gr[m];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if ((threadIdx.x & 63u) == 0 && v != 0.0f) atomicAdd(P.gradRoughness + m, v);
+        }
+    }
     if (P.counters) {
         const unsigned long long t = waveSum(k.traces), sh = waveSum(k.shadows), n = waveSum(k.nodes),
                                  tr = waveSum(k.triangles);
@@ -1020,6 +1216,17 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderPar
             atomicAdd(&P.state->triangles, tr);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) { runLaunch<false>(P); }
+__global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) { runLaunch<true>(P); }
+
+// The film's mean rgb per pixel (sums / counts).
+__global__ void resolveKernel(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= pixels) return;
+    const double n = double(counts[i]);
+    for (int c = 0; c < 3; ++c) rgb[size_t(i) * 3 + c] = n > 0 ? float(sums[size_t(i) * 4 + c] / n) : 0.0f;
 }
 
 __global__ void clearState(LaunchState* s) {
@@ -1034,14 +1241,16 @@ __global__ void clearState(LaunchState* s) {
 
 // Host-side launchers the lane calls (x2_lane.hip).
 namespace x2 {
-hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
+template <bool Adj>
+hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
     static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
     if (X2_PERSISTENT && !resident) {
         int dev = 0, perCu = 0;
         hipDeviceProp_t prop;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
-        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, renderKernel, 256, 0);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, Adj ? adjointKernel : renderKernel, 256, 0);
         if (e != hipSuccess) return e;
         resident = std::max(1, perCu) * prop.multiProcessorCount;
     }
@@ -1049,7 +1258,13 @@ hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     unsigned long long blocks = (items + 255) / 256;
     if (X2_PERSISTENT && blocks > (unsigned long long)resident) blocks = resident;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(renderKernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(Adj ? adjointKernel : renderKernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+hipError_t launchRender(const RenderParams& p, hipStream_t stream) { return launchFrame<false>(p, stream); }
+hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<true>(p, stream); }
+hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream) {
+    hipLaunchKernelGGL(resolveKernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, sums, counts, pixels, rgb);
     return hipGetLastError();
 }
 hipError_t launchClearState(LaunchState* s, hipStream_t stream) {

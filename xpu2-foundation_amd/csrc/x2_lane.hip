@@ -18,6 +18,8 @@
 
 namespace x2 {
 hipError_t launchRender(const RenderParams& p, hipStream_t stream);
+hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream);
+hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream);
 hipError_t launchClearState(LaunchState* s, hipStream_t stream);
 hipError_t uploadSobol(const uint32_t* table);
 }  // namespace x2
@@ -82,6 +84,10 @@ struct x2dev {
     x2_material* materials = nullptr;
     x2_light* lights = nullptr;
     uint32_t nodeCount = 0, triCount = 0, lightCount = 0, materialCount = 0, maxStack = 0;
+    float* triUV = nullptr;
+    std::vector<float*> texels;   // one device buffer per texture
+    std::vector<TexDesc> texDesc;  // host copy of the descriptors
+    TexDesc* textures = nullptr;
     bool prepared = false;
     // film
     double* sums = nullptr;
@@ -103,6 +109,10 @@ struct x2dev {
     x2_config cfg{};
     uint32_t itemsPerChunk = 0;
     bool begun_ = false;
+    // an adjoint frame's inputs and outputs (device pointers the caller owns), null in a forward frame
+    const float* dImage = nullptr;
+    float* gradTextures = nullptr;
+    float* gradRoughness = nullptr;
 };
 
 namespace {
@@ -114,6 +124,9 @@ void release(x2dev* d) {
     (void)hipFree(d->tris);
     (void)hipFree(d->materials);
     (void)hipFree(d->lights);
+    (void)hipFree(d->triUV);
+    (void)hipFree(d->textures);
+    for (float* t : d->texels) (void)hipFree(t);
     (void)hipFree(d->sums);
     (void)hipFree(d->counts);
     (void)hipFree(d->ring);
@@ -212,6 +225,13 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     for (uint32_t i = 0; i < s->light_count; ++i)
         if (s->lights[i].type != X2_LIGHT_ENV && s->lights[i].type != X2_LIGHT_RECT)
             return fail(X2_ERR_ARGUMENT, "only environment and rectangle lights are implemented");
+    if (s->texture_count > MaxTextures || (s->texture_count && !s->textures))
+        return fail(X2_ERR_ARGUMENT, "at most 16 textures, and their records");
+    for (uint32_t i = 0; i < s->texture_count; ++i)
+        if (!s->textures[i].texels || !s->textures[i].width || !s->textures[i].height)
+            return fail(X2_ERR_ARGUMENT, "a texture without texels");
+    for (uint32_t i = 0; i < s->material_count; ++i)
+        if (s->materials[i].albedo_map > s->texture_count) return fail(X2_ERR_ARGUMENT, "albedo_map names no texture");
     Bvh8Result bvh;
     std::string err;
     if (!buildBvh8(*s, bvh, err)) return fail(X2_ERR_ARGUMENT, err);
@@ -222,6 +242,13 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     (void)hipFree(d->tris);
     (void)hipFree(d->materials);
     (void)hipFree(d->lights);
+    (void)hipFree(d->triUV);
+    (void)hipFree(d->textures);
+    for (float* t : d->texels) (void)hipFree(t);
+    d->texels.clear();
+    d->texDesc.clear();
+    d->triUV = nullptr;
+    d->textures = nullptr;
     d->nodes = nullptr;
     d->tris = nullptr;
     d->materials = nullptr;
@@ -236,6 +263,25 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     if (!bvh.tris.empty()) X2_HIP(hipMemcpy(d->tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice));
     X2_HIP(hipMemcpy(d->materials, s->materials, s->material_count * sizeof(x2_material), hipMemcpyHostToDevice));
     if (s->light_count) X2_HIP(hipMemcpy(d->lights, s->lights, s->light_count * sizeof(x2_light), hipMemcpyHostToDevice));
+    if (!bvh.triUV.empty()) {
+        X2_HIP(hipMalloc(&d->triUV, bvh.triUV.size() * sizeof(float)));
+        X2_HIP(hipMemcpy(d->triUV, bvh.triUV.data(), bvh.triUV.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    unsigned long long gradOffset = 0;
+    for (uint32_t i = 0; i < s->texture_count; ++i) {
+        const x2_texture& t = s->textures[i];
+        const size_t bytes = size_t(t.width) * t.height * 3 * sizeof(float);
+        float* buf = nullptr;
+        X2_HIP(hipMalloc(&buf, bytes));
+        d->texels.push_back(buf);
+        X2_HIP(hipMemcpy(buf, t.texels, bytes, hipMemcpyHostToDevice));
+        d->texDesc.push_back(TexDesc{buf, t.width, t.height, gradOffset});
+        gradOffset += size_t(t.width) * t.height * 3;
+    }
+    if (!d->texDesc.empty()) {
+        X2_HIP(hipMalloc(&d->textures, d->texDesc.size() * sizeof(TexDesc)));
+        X2_HIP(hipMemcpy(d->textures, d->texDesc.data(), d->texDesc.size() * sizeof(TexDesc), hipMemcpyHostToDevice));
+    }
     d->nodeCount = uint32_t(bvh.nodes.size());
     d->triCount = uint32_t(bvh.tris.size());
     d->lightCount = s->light_count;
@@ -273,6 +319,9 @@ int x2dev_begin(x2dev* d, const x2_config* c, uint32_t samples_per_chunk) {
     X2_HIP(hipMemsetAsync(d->sums, 0, pixels * 4 * sizeof(double), d->stream));
     X2_HIP(hipMemsetAsync(d->counts, 0, pixels * sizeof(uint32_t), d->stream));
     d->cfg = *c;
+    d->dImage = nullptr;
+    d->gradTextures = nullptr;
+    d->gradRoughness = nullptr;
     d->itemsPerChunk = samples_per_chunk * TilePixels;
     d->published = 0;
     d->launches = 0;
@@ -314,8 +363,13 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     p.cfg = d->cfg;
     p.nodes = d->nodes;
     p.tris = d->tris;
+    p.triUV = d->triUV;
+    p.textures = d->textures;
     p.materials = d->materials;
     p.lights = d->lights;
+    p.dImage = d->dImage;
+    p.gradTextures = d->gradTextures;
+    p.gradRoughness = d->gradRoughness;
     p.ring = d->ring;
     p.sums = d->sums;
     p.counts = d->counts;
@@ -327,7 +381,7 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     p.counters = d->opt.counters;
     p.pad = 0;
     X2_HIP(hipEventRecord(d->begun[slot], d->stream));
-    X2_HIP(launchRender(p, d->stream));
+    X2_HIP(d->dImage ? launchAdjoint(p, d->stream) : launchRender(p, d->stream));
     X2_HIP(hipEventRecord(d->ended[slot], d->stream));
     X2_HIP(hipMemcpyAsync(d->readback + slot, d->state + slot, sizeof(LaunchState), hipMemcpyDeviceToHost, d->stream));
     X2_HIP(hipEventRecord(d->done[slot], d->stream));
@@ -405,16 +459,46 @@ int x2dev_device_film(const x2dev* d, void** sums, void** counts) {
 
 void x2dev_destroy(x2dev* d) { release(d); }
 
-// The broker's GPU-lane sequence for one uniform frame (async_render.h:318-1081 with the GPU lane alone): chunks in
-// the work grid's order (async_grid.h:129-175: sample ranges 1, 1, 2, then samples_per_chunk; each range over the
-// 64-pixel cells' tiles), launches of about windowMs at the measured rate (launchSize, async_render.h:1674-1718:
-// firstChunks first, never under two items a resident thread), at most two in flight, completions polled, the film
-// downloaded and every pixel's count audited against spp (async_render.h:1035-1038).
-int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats* stats) {
-    using clock = std::chrono::steady_clock;
-    const auto start = clock::now();
-    int r = x2dev_begin(d, c, spc);
-    if (r) return r;
+int x2dev_set_texture(x2dev* d, uint32_t texture, const float* texels, int texels_on_device) {
+    if (!d || !texels) return fail(X2_ERR_ARGUMENT, "null device or texels");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "texture update with launches in flight");
+    if (texture >= d->texDesc.size()) return fail(X2_ERR_ARGUMENT, "no such texture");
+    const TexDesc& t = d->texDesc[texture];
+    X2_HIP(hipSetDevice(d->device));
+    X2_HIP(hipMemcpyAsync(d->texels[texture], texels, size_t(t.width) * t.height * 3 * sizeof(float),
+                          texels_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, d->stream));
+    X2_HIP(hipStreamSynchronize(d->stream));
+    return X2_OK;
+}
+
+int x2dev_set_material(x2dev* d, uint32_t material, const x2_material* m) {
+    if (!d || !m) return fail(X2_ERR_ARGUMENT, "null device or material");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "material update with launches in flight");
+    if (material >= d->materialCount) return fail(X2_ERR_ARGUMENT, "no such material");
+    if (m->albedo_map > d->texDesc.size()) return fail(X2_ERR_ARGUMENT, "albedo_map names no texture");
+    X2_HIP(hipSetDevice(d->device));
+    X2_HIP(hipMemcpyAsync(d->materials + material, m, sizeof(x2_material), hipMemcpyHostToDevice, d->stream));
+    X2_HIP(hipStreamSynchronize(d->stream));
+    return X2_OK;
+}
+
+int x2dev_resolve_image(x2dev* d, float* device_rgb) {
+    if (!d || !device_rgb) return fail(X2_ERR_ARGUMENT, "null device or image");
+    if (!d->begun_ || !d->filmPixels) return fail(X2_ERR_STATE, "no film to resolve");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "resolve with launches in flight");
+    X2_HIP(hipSetDevice(d->device));
+    X2_HIP(launchResolve(d->sums, d->counts, uint32_t(d->filmPixels), device_rgb, d->stream));
+    X2_HIP(hipStreamSynchronize(d->stream));
+    return X2_OK;
+}
+
+namespace {
+// The broker's GPU-lane sequence over one uniform frame's chunks (async_render.h:318-1081 with the GPU lane alone):
+// chunks in the work grid's order (async_grid.h:129-175: sample ranges 1, 1, 2, then samples_per_chunk; each range
+// over the 64-pixel cells' tiles), launches of about windowMs at the measured rate (launchSize,
+// async_render.h:1674-1718: firstChunks first, never under two items a resident thread), at most two in flight,
+// completions polled until the last.
+int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
     std::vector<x2_chunk> chunks;
     {
         const uint32_t tilesX = (c->width + 7) / 8, tilesY = (c->height + 7) / 8;
@@ -432,12 +516,12 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
                         for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
                             chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
     }
-    x2_frame_stats st{};
     const double windowMs = 10.0;
     const uint64_t firstChunks = 2048, cap = d->opt.max_chunks_per_launch;
     const uint64_t least = (uint64_t(d->launchThreads) * 2 + d->itemsPerChunk - 1) / d->itemsPerChunk;
     double rate = 0;  // items per device second, from completed launches
     size_t next = 0;
+    int r = 0;
     while (next < chunks.size() || !d->inFlight.empty()) {
         x2_completion done;
         while ((r = x2dev_poll(d, &done)) == 1) {
@@ -462,10 +546,10 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
                                  : std::max<uint64_t>(firstChunks, least);
             want = std::min<uint64_t>({want, cap, remaining});
             uint64_t samples = 0;
+            const uint32_t tilesX = (c->width + 7) / 8;
             for (uint64_t i = 0; i < want; ++i) {
                 const x2_chunk& ch = chunks[next + i];
                 for (uint32_t p = 0; p < TilePixels; ++p) {
-                    const uint32_t tilesX = (c->width + 7) / 8;
                     const uint32_t x = (ch.tile % tilesX) * 8 + p % 8, y = (ch.tile / tilesX) * 8 + p / 8;
                     if (x < c->width && y < c->height) samples += ch.count;
                 }
@@ -479,6 +563,20 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
             if (e != hipSuccess) return hipFail(e, "hipEventSynchronize");
         }
     }
+    return X2_OK;
+}
+}  // namespace
+
+// One uniform frame: runFrame, then the film downloaded and every pixel's count audited against spp
+// (async_render.h:1035-1038).
+int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats* stats) {
+    using clock = std::chrono::steady_clock;
+    const auto start = clock::now();
+    int r = x2dev_begin(d, c, spc);
+    if (r) return r;
+    x2_frame_stats st{};
+    r = runFrame(d, c, spc, st);
+    if (r) return r;
     r = x2dev_download_film(d);
     if (r) return r;
     X2_HIP(hipEventSynchronize(d->filmDone));
@@ -492,6 +590,41 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
     if (stats) *stats = st;
     if (st.audit_failures) return fail(X2_ERR_STATE, "Sample audit failure (async_render.h:1035)");
     if (st.errors) return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
+    return X2_OK;
+}
+
+// The adjoint of one uniform frame: the same chunks as runFrame, each sample replayed by the adjoint kernel. The
+// film is left as it was (the forward frame's), so x2dev_resolve_image still reads the forward image.
+int x2dev_adjoint_frame(x2dev* d, const x2_config* c, uint32_t spc, const float* d_image, float* grad_textures,
+                        float* grad_roughness, x2_frame_stats* stats) {
+    using clock = std::chrono::steady_clock;
+    const auto start = clock::now();
+    if (!d || !c || !d_image || !grad_roughness || (!d->texDesc.empty() && !grad_textures))
+        return fail(X2_ERR_ARGUMENT, "null device, config, image adjoint or gradient buffer");
+    if (!d->prepared) return fail(X2_ERR_STATE, "adjoint before prepare");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "adjoint with launches in flight");
+    if (spc == 0 || spc > 1024) return fail(X2_ERR_ARGUMENT, "samples_per_chunk must be 1..1024");
+    int r = validateConfig(d, *c);
+    if (r) return r;
+    if (c->max_depth > 32) return fail(X2_ERR_ARGUMENT, "the adjoint supports depth limits up to 32");
+    X2_HIP(hipSetDevice(d->device));
+    d->cfg = *c;
+    d->itemsPerChunk = spc * TilePixels;
+    d->published = 0;
+    d->launches = 0;
+    d->begun_ = true;
+    d->dImage = d_image;
+    d->gradTextures = grad_textures;
+    d->gradRoughness = grad_roughness;
+    x2_frame_stats st{};
+    r = runFrame(d, c, spc, st);
+    d->dImage = nullptr;
+    d->gradTextures = nullptr;
+    d->gradRoughness = nullptr;
+    if (r) return r;
+    st.wall_seconds = std::chrono::duration<double>(clock::now() - start).count();
+    if (stats) *stats = st;
+    if (st.errors) return fail(X2_ERR_STATE, "GPU sample failure in the adjoint");
     return X2_OK;
 }
 

@@ -47,13 +47,28 @@ struct alignas(64) LaunchState {
     uint32_t errors, pad[5];
 };
 
+// A texture on the device: its texels (RGB f32, row-major) and where its gradient starts in the adjoint's buffer.
+struct TexDesc {
+    const float* texels;
+    uint32_t width, height;
+    unsigned long long gradOffset;  // in floats
+};
+constexpr uint32_t MaxTextures = 16;
+constexpr uint32_t RoughnessSlots = 8;  // materials whose roughness gradient a lane accumulates in registers
+
 // Everything a render launch reads, by value.
 struct RenderParams {
     x2_config cfg;
     const Node8* nodes;
     const TriRecord* tris;
+    const float* triUV;  // 6 floats per triangle record (leaf order), or null: (0,0), (1,0), (1,1)
+    const TexDesc* textures;
     const x2_material* materials;
     const x2_light* lights;
+    // the adjoint's image adjoint (dLoss / d mean rgb per pixel) and gradient outputs (adjoint launches only)
+    const float* dImage;
+    float* gradTextures;
+    float* gradRoughness;
     const x2_chunk* ring;
     double* sums;
     uint32_t* counts;

@@ -53,6 +53,10 @@ class Material(ctypes.Structure):
     ]
 
 
+class Texture(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("texels", ctypes.POINTER(f32))]
+
+
 class Scene(ctypes.Structure):
     _fields_ = [
         ("positions", ctypes.POINTER(f32)), ("indices", ctypes.POINTER(u32)),
@@ -60,6 +64,7 @@ class Scene(ctypes.Structure):
         ("materials", ctypes.POINTER(Material)), ("lights", ctypes.POINTER(Light)),
         ("vertex_count", u32), ("triangle_count", u32), ("geom_count", u32), ("material_count", u32),
         ("light_count", u32), ("pad", u32),
+        ("uvs", ctypes.POINTER(f32)), ("textures", ctypes.POINTER(Texture)), ("texture_count", u32), ("pad2", u32),
     ]
 
 
@@ -99,7 +104,7 @@ class FrameStats(ctypes.Structure):
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
     "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
-    "x2_material": (Material, 40), "x2_scene": (Scene, 72), "x2_config": (Config, 236),
+    "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_scene": (Scene, 96), "x2_config": (Config, 236),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 16), "x2_frame_stats": (FrameStats, 80),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
@@ -127,6 +132,11 @@ SIGNATURES = {
     "x2dev_last_error": (ctypes.c_char_p, []),
     "x2dev_render_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32,
                                           ctypes.POINTER(FrameStats)]),
+    "x2dev_set_texture": (ctypes.c_int, [ctypes.c_void_p, u32, ctypes.c_void_p, ctypes.c_int]),
+    "x2dev_set_material": (ctypes.c_int, [ctypes.c_void_p, u32, ctypes.POINTER(Material)]),
+    "x2dev_resolve_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "x2dev_adjoint_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(FrameStats)]),
 }
 
 _lib = None

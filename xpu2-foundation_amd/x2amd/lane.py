@@ -85,6 +85,32 @@ class GpuLane:
         counts = np.ctypeslib.as_array(c, shape=(n,)).copy()
         return sums, counts
 
+    def set_texture(self, index, texels, on_device=False):
+        """texels: a float32 array (host) or a device pointer (int) when on_device."""
+        if on_device:
+            ptr = ctypes.c_void_p(int(texels))
+        else:
+            arr = np.ascontiguousarray(texels, dtype=np.float32)
+            ptr = arr.ctypes.data_as(ctypes.c_void_p)
+        abi.check(self.lib.x2dev_set_texture(self.h, index, ptr, 1 if on_device else 0), "x2dev_set_texture")
+
+    def set_material(self, index, material):
+        abi.check(self.lib.x2dev_set_material(self.h, index, ctypes.byref(material)), "x2dev_set_material")
+
+    def resolve_image(self, device_rgb_ptr):
+        abi.check(self.lib.x2dev_resolve_image(self.h, ctypes.c_void_p(int(device_rgb_ptr))), "x2dev_resolve_image")
+
+    def adjoint_frame(self, config, d_image_ptr, grad_textures_ptr, grad_roughness_ptr, samples_per_chunk=4):
+        """The path-replay adjoint (device pointers: dLoss/dI per pixel rgb, texel and roughness gradients)."""
+        st = abi.FrameStats()
+        r = self.lib.x2dev_adjoint_frame(self.h, ctypes.byref(config), samples_per_chunk,
+                                         ctypes.c_void_p(int(d_image_ptr)), ctypes.c_void_p(int(grad_textures_ptr or 0)),
+                                         ctypes.c_void_p(int(grad_roughness_ptr)), ctypes.byref(st))
+        if r < 0:
+            err = self.lib.x2dev_last_error()
+            raise RuntimeError(f"x2dev_adjoint_frame failed ({r}): {err.decode() if err else ''}")
+        return st
+
     def render_frame(self, config, samples_per_chunk=4):
         """One uniform frame through the broker counterpart; returns (sums, counts, FrameStats)."""
         st = abi.FrameStats()

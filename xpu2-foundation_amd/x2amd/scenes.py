@@ -34,7 +34,8 @@ def frame_seed(frame=0, eye=0, lock=False):
 class SceneData:
     """A flat triangle scene and the ctypes view of it (kept alive with the arrays)."""
 
-    def __init__(self, positions, indices, geom_first, geom_material, materials, lights, name):
+    def __init__(self, positions, indices, geom_first, geom_material, materials, lights, name, uvs=None,
+                 textures=()):
         self.name = name
         self.positions = np.ascontiguousarray(positions, dtype=np.float32)
         self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
@@ -56,14 +57,27 @@ class SceneData:
         self.c.geom_count = len(self.geom_first) - 1
         self.c.material_count = self.material_count
         self.c.light_count = self.light_count
+        self.uvs = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32)
+        if self.uvs is not None:
+            assert self.uvs.shape == (self.positions.shape[0], 2)
+            self.c.uvs = self.uvs.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        self.texture_arrays = [np.ascontiguousarray(t, dtype=np.float32) for t in textures]
+        self.textures = (abi.Texture * max(1, len(self.texture_arrays)))()
+        for i, t in enumerate(self.texture_arrays):
+            assert t.ndim == 3 and t.shape[2] == 3
+            self.textures[i].height, self.textures[i].width = t.shape[0], t.shape[1]
+            self.textures[i].texels = t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        self.c.textures = ctypes.cast(self.textures, ctypes.POINTER(abi.Texture))
+        self.c.texture_count = len(self.texture_arrays)
 
     @property
     def triangle_count(self):
         return int(self.indices.shape[0])
 
 
-def material(albedo, diffuse=1.0, specular=0.0, roughness=0.5, emission=(0.0, 0.0, 0.0)):
+def material(albedo, diffuse=1.0, specular=0.0, roughness=0.5, emission=(0.0, 0.0, 0.0), albedo_map=0):
     m = abi.Material()
+    m.albedo_map = albedo_map
     m.albedo[:] = [float(a) for a in albedo]
     m.roughness = float(roughness)
     m.diffuse_weight = float(diffuse)
@@ -358,3 +372,68 @@ def uniform_chunks(config, samples_per_chunk=4, progressive=True):
     for i, c in enumerate(out):
         arr[i].tile, arr[i].first, arr[i].count, arr[i].cell = c
     return arr
+
+
+# ------------------------------------------------------------------------------------------------------------------
+# The differentiable Cornell box.
+def value_noise_texture(size, seed, cells=16, lo=0.1, hi=0.9):
+    """A size x size RGB texture of seeded value noise in [lo, hi]: a random lattice of cells x cells per channel,
+    smoothstep-interpolated, periodic (it tiles with periodic addressing)."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((size, size, 3), np.float64)
+    x = (np.arange(size) + 0.5) / size * cells
+    i0 = np.floor(x).astype(np.int64)
+    f = x - i0
+    w = f * f * (3 - 2 * f)
+    for ch in range(3):
+        lat = rng.uniform(lo, hi, (cells, cells))
+        a = lat[np.ix_(i0 % cells, i0 % cells)]
+        b = lat[np.ix_(i0 % cells, (i0 + 1) % cells)]
+        c = lat[np.ix_((i0 + 1) % cells, i0 % cells)]
+        d = lat[np.ix_((i0 + 1) % cells, (i0 + 1) % cells)]
+        wy, wx = w[:, None], w[None, :]
+        out[:, :, ch] = (a * (1 - wx) + b * wx) * (1 - wy) + (c * (1 - wx) + d * wx) * wy
+    return out.astype(np.float32)
+
+
+def cornell_textured(width=512, height=512, spp=32, max_depth=8, texture_size=512, texture=None, roughness=None,
+                     seed=7):
+    """The Cornell box with its back wall's albedo a texture_size^2 RGB FP32 map (value noise in [0.1, 0.9]) and four
+    50/50 Lambert + GGX materials whose roughness is a parameter (floor and ceiling, left, right, the boxes).
+    Russian roulette is off so the fixed-seed estimator is differentiable in the parameters (no roulette scale).
+    Returns (scene, config, params) with params = {"texture": HxWx3, "roughness": [4], "roughness_materials": [...]}."""
+    grey, red, green = (0.73, 0.73, 0.73), (0.65, 0.05, 0.05), (0.12, 0.45, 0.15)
+    tex = value_noise_texture(texture_size, seed) if texture is None else np.asarray(texture, np.float32)
+    rough = np.array([0.3, 0.5, 0.2, 0.6] if roughness is None else roughness, np.float64)
+    walls = [
+        (_quad((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1)), 1),  # floor
+        (_quad((-1, 1, 1), (-1, 1, -1), (1, 1, -1), (1, 1, 1)), 1),  # ceiling
+        (_quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1)), 0),  # back: textured
+        (_quad((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1)), 2),  # left
+        (_quad((1, -1, 1), (1, 1, 1), (1, 1, -1), (1, -1, -1)), 3),  # right
+    ]
+    meshes = [((np.asarray(v, np.float64), t), m) for (v, t), m in walls]
+    meshes.append((_box((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 15.0), 4))
+    meshes.append((_box((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -18.0), 4))
+    positions, indices, first, mats, uvs = [], [], [0], [], []
+    base = 0
+    for k, ((v, t), m) in enumerate(meshes):
+        positions.append(np.asarray(v, np.float64))
+        indices.append(np.asarray(t, np.int64) + base)
+        base += len(v)
+        first.append(first[-1] + len(t))
+        mats.append(m)
+        # the back wall's corners (-1,-1), (1,-1), (1,1), (-1,1) map to (0,0), (1,0), (1,1), (0,1)
+        uvs.append(np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float64) if k == 2 else np.zeros((len(v), 2)))
+    materials = [material((1.0, 1.0, 1.0), albedo_map=1),
+                 material(grey, diffuse=0.5, specular=0.5, roughness=rough[0]),
+                 material(red, diffuse=0.5, specular=0.5, roughness=rough[1]),
+                 material(green, diffuse=0.5, specular=0.5, roughness=rough[2]),
+                 material(grey, diffuse=0.5, specular=0.5, roughness=rough[3])]
+    lights = [rect_light((0.0, 0.999, 0.0), (0.25, 0.0, 0.0), (0.0, 0.0, 0.2), (17.0, 12.0, 4.0))]
+    scene = SceneData(np.concatenate(positions), np.concatenate(indices), first, mats, materials, lights,
+                      "cornell_textured", uvs=np.concatenate(uvs), textures=[tex])
+    config = make_config(width, height, spp, eye=(0.0, 0.0, 3.75), target=(0.0, 0.0, 0.0), light_count=1,
+                         max_depth=max_depth, rr_threshold=0.0)
+    params = {"texture": tex, "roughness": rough.astype(np.float32), "roughness_materials": [1, 2, 3, 4]}
+    return scene, config, params
