@@ -7,7 +7,9 @@ in flight, the film downloaded and every pixel's count audited. The scene is res
 
 Prints one JSON line: Mpaths/s (value), Mrays/s, the roofline of the render kernel (algorithmic bytes: 256 B per
 BVH8 node fetched + 48 B per triangle record tested + 72 B of film per path, over the kernel's own HIP-event time)
-and the CPU baseline (the oracle on this host's cores over a bounded sample of the same workload).
+and the CPU baseline (the oracle on this host's cores over a bounded sample of the same workload). The adjoint
+leg times one forward frame and one path-replay adjoint frame of the differentiable Cornell box (512x512, 32 spp):
+adjoint_step_over_forward is the adjoint frame's time over the forward frame's.
 """
 import argparse
 import json
@@ -42,6 +44,73 @@ def cpu_baseline(scene, width, height, spp, seconds_hint):
     }
 
 
+def adjoint_leg(reps):
+    """The differentiable Cornell box (512x512, 32 spp, depth 8, 512x512 RGB FP32 albedo map, 4 GGX materials): one
+    forward frame against one path-replay adjoint frame of an L2 image loss (dLoss/dimage resident on the device),
+    each timed over `reps` frames after a warm-up."""
+    import numpy as np
+    from x2amd import abi, scenes
+    from x2amd.hipmem import DeviceBuffer
+    from x2amd.lane import GpuLane
+
+    scene, config, params = scenes.cornell_textured()
+    lane = GpuLane(0)
+    lane.prepare(scene)
+    rng = np.random.default_rng(1234)
+    lane.set_texture(0, np.clip(params["texture"] + rng.normal(0, 0.1, params["texture"].shape), 0.01, 1.0))
+    for m, r in zip(params["roughness_materials"], params["roughness"] + rng.normal(0, 0.1, 4)):
+        mat = abi.Material.from_buffer_copy(scene.materials[m])
+        mat.roughness = float(np.clip(r, 0.05, 0.95))
+        lane.set_material(m, mat)
+    sums, counts, _ = lane.render_frame(config)
+    target = sums[:, :3] / counts[:, None]
+    lane.set_texture(0, params["texture"])
+    for m in params["roughness_materials"]:
+        lane.set_material(m, scene.materials[m])
+    sums, counts, _ = lane.render_frame(config)
+    d_image = (2 * (sums[:, :3] / counts[:, None] - target)).astype(np.float32)
+    dbuf, gt, gr = DeviceBuffer(d_image.nbytes), DeviceBuffer(params["texture"].nbytes), DeviceBuffer(4 * 64)
+    dbuf.upload(d_image)
+    fw = dict(wall=0.0, kernel=0.0, traces=0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, _, st = lane.render_frame(config)
+        fw["kernel"] += st.device_seconds
+        fw["traces"] += st.rays_trace + st.rays_shadow
+    fw["wall"] = time.perf_counter() - t0
+    gt.zero()
+    gr.zero()
+    lane.adjoint_frame(config, dbuf.ptr, gt.ptr, gr.ptr)  # warm-up
+    ad = dict(wall=0.0, kernel=0.0, traces=0)
+    for _ in range(reps):
+        gt.zero()
+        gr.zero()
+        t0 = time.perf_counter()
+        st = lane.adjoint_frame(config, dbuf.ptr, gt.ptr, gr.ptr)
+        ad["wall"] += time.perf_counter() - t0
+        ad["kernel"] += st.device_seconds
+        ad["traces"] += st.rays_trace + st.rays_shadow
+    grads = gt.download(np.float32, params["texture"].size)
+    for b in (dbuf, gt, gr):
+        b.free()
+    lane.close()
+    assert np.isfinite(grads).all() and np.abs(grads).sum() > 0
+    paths = config.width * config.height * config.spp
+    return {
+        "workload": f"differentiable cornell {config.width}x{config.height}, {config.spp} spp, depth "
+                    f"{config.max_depth}, {params['texture'].shape[1]}x{params['texture'].shape[0]} albedo map, "
+                    "4 GGX roughness, L2 loss",
+        "reps": reps,
+        "forward_ms": fw["wall"] / reps * 1e3, "adjoint_ms": ad["wall"] / reps * 1e3,
+        "forward_kernel_ms": fw["kernel"] / reps * 1e3, "adjoint_kernel_ms": ad["kernel"] / reps * 1e3,
+        "adjoint_over_forward": ad["wall"] / fw["wall"],
+        "step_over_forward": (fw["wall"] + ad["wall"]) / fw["wall"],
+        "forward_mpaths_s": paths * reps / fw["wall"] / 1e6,
+        "adjoint_mpaths_s": paths * reps / ad["wall"] / 1e6,
+        "rays_replayed_equal": fw["traces"] == ad["traces"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +121,8 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--cpu-spp", type=int, default=0, help="CPU baseline spp (0: 64 with >= 12 host threads, else 16)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (the film reduce "
@@ -110,6 +181,9 @@ def main():
                      "kernel": "renderKernel", "avg_launch_ms": per_launch_s * 1e3,
                      "launches": totals["launches"], "algorithmic_bytes": bytes_total},
     }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(args.adjoint_reps)
+        out["adjoint_step_over_forward"] = out["adjoint"]["adjoint_over_forward"]
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)
         out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp, 20)
