@@ -82,6 +82,27 @@ typedef struct x2_texture {
     const float* texels;
 } x2_texture;
 
+/* A heterogeneous participating medium filling a render-space box (the build's own volume model: the reference binds
+ * volumes to closed meshes and reads NanoVDB grids, async_integrator.h:415-1291). Two grids of res[0] x res[1] x res[2]
+ * IEEE half values, x fastest, each res a multiple of 8: density and temperature (kelvin; null: no emission). The
+ * device keeps them as 8^3 bricks, only the bricks holding a nonzero density. A point's values are trilinear in the
+ * voxel centres; extinction = density * sigma_t_scale (grey), scattering = albedo * extinction, Henyey-Greenstein
+ * phase g, emission = emission_scale * density * ramp(temperature), ramp being ramp_size rgb values over
+ * [0, ramp_max_kelvin] (a blackbody colour ramp), linearly interpolated. */
+typedef struct x2_volume {
+    float bounds_min[3], bounds_max[3];
+    uint32_t res[3];
+    float sigma_t_scale;
+    float albedo[3];
+    float g;
+    float emission_scale;
+    float ramp_max_kelvin;
+    uint32_t ramp_size, pad;
+    const uint16_t* density;
+    const uint16_t* temperature;
+    const float* ramp;             /* ramp_size x 3 */
+} x2_volume;
+
 /* A triangle scene. Triangles are ordered mesh by mesh; mesh g owns triangles [geom_first[g], geom_first[g+1]),
  * and a triangle's primitive id is its index less geom_first of its mesh. Meshes are two-sided. A material whose
  * albedo_map is k > 0 multiplies its albedo by texture k-1 at the hit's (s, t), interpolated from per-vertex uvs
@@ -97,6 +118,8 @@ typedef struct x2_scene {
     const float* uvs;              /* vertex_count x 2, or null */
     const x2_texture* textures;    /* texture_count */
     uint32_t texture_count, pad2;
+    const x2_volume* volumes;      /* volume_count (0 or 1) */
+    uint32_t volume_count, pad3;
 } x2_scene;
 
 /* The integrator's settings: the used subset of `mixed::Config` (gpu_jobs.h:10-44), plus rr_min_depth (an
@@ -119,6 +142,12 @@ typedef struct x2_config {
     uint32_t shadow_reuse;      /* bounce-ray reuse: ray visibility bits whose bounces stand for shadow rays */
     int32_t viewport[4];        /* min x, min y, max x, max y, half open */
     uint32_t pad[2];
+    /* Volumes (mixed::Config, gpu_jobs.h:19-28; read only when the scene has a volume). */
+    uint32_t max_volume_depth, volume_illumination_samples;
+    float volume_transmittance_threshold;
+    float inv_volume_quality;   /* march step = voxel size * inv_volume_quality * max(depth - 1, 1) */
+    float volume_attenuation_factor, volume_contribution_factor, volume_phase_attenuation_factor;
+    uint32_t pad_volume;
 } x2_config;
 
 /* What the host learns when a launch has finished (async_gpu.h:65 Completion), plus traversal counters. */

@@ -348,6 +348,18 @@ struct Texture {
     std::vector<float> texels;  // RGB, row-major
     size_t gradOffset;          // in the adjoint's concatenated gradient
 };
+// A box-bounded brick grid (the medium; see the volume section below for how it is read).
+struct Volume {
+    bool present = false, hasTemperature = false;
+    V3 bmin{0, 0, 0}, bmax{0, 0, 0}, albedo{0, 0, 0};
+    U32 res[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
+    float scaleVox[3] = {0, 0, 0}, brickExtent[3] = {0, 0, 0}, minFeature = 0;
+    float sigmaTScale = 0, g = 0, emissionScale = 0, rampMax = 1;
+    U32 rampSize = 0;
+    std::vector<float> ramp, majorant;
+    std::vector<U32> table;  // per brick: its pool index, ~0u when empty
+    std::vector<uint16_t> density, temperature;  // pools of 512 halves a brick, z, y, x within
+};
 struct Scene {
     std::vector<V3> tri;  // 3 vertices per triangle, scene order
     std::vector<float> triUV;  // 6 per triangle, or empty: (0,0), (1,0), (1,1)
@@ -358,6 +370,7 @@ struct Scene {
     // negative = the material's own. Central differences over it isolate d f / d roughness with the sampling fixed.
     std::vector<float> evalRoughness;
     std::vector<x2_light> lights;
+    Volume volume;  // volume.present: the scene has a medium
     std::vector<BNode> nodes;
     std::vector<U32> order;
     U32 triangles = 0;
@@ -900,6 +913,7 @@ struct Pending {
     U32 heldLight;
     float heldFar;
     V3 held;
+    U32 volumeDepth;  // the volume scattering the path has been through (async_integrator.h:210)
 };
 constexpr U32 MaxPending = 32, PendingReserve = 16;  // async_integrator.h:250-267
 float bounceThinning(U32 top, U32 planned) {
@@ -945,6 +959,494 @@ bool visible(const Scene& s, V3 o, V3 d, float nearT, float farT, Counters& k) {
     ++k.shadows;
     float t;
     return traverse(s, o, d, nearT, farT, true, false, t) == ~0u;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Volumes. The structure is the reference's computeRadianceVolume (async_integrator.h:1152-1291): one march along the
+// ray (decoupled ray marching, VolumeMarch::run :650-798) records steps; single scattering per light by equi-angular
+// and distance sampling over those steps with MIS (volumeScatterDirect :917-1012); the Wrenninge approximation of
+// multiple scattering (volumeMultipleScatter :1060-1149); the throughput takes the march's transmittance. The medium
+// is the build's own (one box-bounded brick grid, trilinear FP16, grey extinction, a blackbody ramp for emission) and
+// so is the shadow transmittance: ratio tracking under per-brick majorants in place of the reference's transmittance
+// ray march (:428-476). Steps whose end point has zero density are not recorded (they carry nothing), and the
+// oracle keeps its steps in a table where the device marches again from checkpoints.
+constexpr float VolEpsilon = 1.1920928955078125e-07f;  // volume::Epsilon
+constexpr float HitEpsilonStart = 1e-5f;               // volume::HitEpsilonStart
+constexpr float OneOverFourPi = 0.0795774715459476678844f;
+constexpr U32 MarchSteps = 1u << 16;  // lattice steps a march takes at most (the reference caps at volume::MarchLimit)
+float halfToFloat(uint16_t h) {  // IEEE binary16, exactly
+    const U32 s = U32(h & 0x8000u) << 16, e = (h >> 10) & 31u, m = h & 1023u;
+    if (e == 0) {
+        const float v = float(m) * (1.0f / 16777216.0f);
+        return s ? -v : v;
+    }
+    const U32 u = s | (e == 31 ? 0x7F800000u | (m << 13) : ((e + 112u) << 23) | (m << 13));
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+float lerp(float a, float b, float w) { return a + (b - a) * w; }
+float voxel(const Volume& v, const std::vector<uint16_t>& pool, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || U32(i) >= v.res[0] || U32(j) >= v.res[1] || U32(k) >= v.res[2]) return 0.0f;
+    const U32 e = v.table[(U32(k >> 3) * v.nb[1] + U32(j >> 3)) * v.nb[0] + U32(i >> 3)];
+    if (e == ~0u) return 0.0f;
+    return halfToFloat(pool[size_t(e) * 512 + (U32(k & 7) * 8 + U32(j & 7)) * 8 + U32(i & 7)]);
+}
+// Trilinear in the voxel centres (voxel i's centre at i + 0.5 in grid units), x first, then y, then z.
+float sampleGrid(const Volume& v, const std::vector<uint16_t>& pool, V3 p) {
+    float q[3];
+    for (int a = 0; a < 3; ++a) {
+        q[a] = (comp(p, a) - comp(v.bmin, a)) * v.scaleVox[a] - 0.5f;
+        if (!(q[a] > -1.0f && q[a] < float(v.res[a]))) return 0.0f;
+    }
+    const float fx0 = floorf(q[0]), fy0 = floorf(q[1]), fz0 = floorf(q[2]);
+    const int i = int(fx0), j = int(fy0), k = int(fz0);
+    const float fx = q[0] - fx0, fy = q[1] - fy0, fz = q[2] - fz0;
+    const float c00 = lerp(voxel(v, pool, i, j, k), voxel(v, pool, i + 1, j, k), fx);
+    const float c10 = lerp(voxel(v, pool, i, j + 1, k), voxel(v, pool, i + 1, j + 1, k), fx);
+    const float c01 = lerp(voxel(v, pool, i, j, k + 1), voxel(v, pool, i + 1, j, k + 1), fx);
+    const float c11 = lerp(voxel(v, pool, i, j + 1, k + 1), voxel(v, pool, i + 1, j + 1, k + 1), fx);
+    return lerp(lerp(c00, c10, fy), lerp(c01, c11, fy), fz);
+}
+U32 brickAt(const Volume& v, V3 p) {
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+        const float q = floorf((comp(p, a) - comp(v.bmin, a)) * v.scaleVox[a] * 0.125f);
+        b[a] = int(fminf(fmaxf(q, 0.0f), float(v.nb[a] - 1)));
+    }
+    return (U32(b[2]) * v.nb[1] + U32(b[1])) * v.nb[0] + U32(b[0]);
+}
+// Density at p: zero without a look at the voxels where the brick's majorant (its voxels and a one-voxel apron) is.
+float densityAt(const Volume& v, V3 p) { return v.majorant[brickAt(v, p)] > 0.0f ? sampleGrid(v, v.density, p) : 0.0f; }
+V3 rampAt(const Volume& v, float kelvin) {
+    float x = kelvin / v.rampMax * float(v.rampSize - 1);
+    x = fminf(fmaxf(x, 0.0f), float(v.rampSize - 1));
+    const float fi = floorf(x);
+    const U32 i = U32(fi);
+    if (i + 1 >= v.rampSize) return load3(&v.ramp[size_t(v.rampSize - 1) * 3]);
+    const float f = x - fi;
+    const float* a = &v.ramp[size_t(i) * 3];
+    return {lerp(a[0], a[3], f), lerp(a[1], a[4], f), lerp(a[2], a[5], f)};
+}
+struct VCoeff {
+    float sigmaT;
+    V3 sigmaS, emission;
+};
+VCoeff coefficientsAt(const Volume& v, V3 p, bool withEmission) {
+    VCoeff k{0.0f, {0, 0, 0}, {0, 0, 0}};
+    const float d = densityAt(v, p);
+    if (d == 0.0f) return k;
+    k.sigmaT = d * v.sigmaTScale;
+    k.sigmaS = v.albedo * k.sigmaT;
+    if (withEmission && v.hasTemperature && v.emissionScale > 0.0f)
+        k.emission = rampAt(v, sampleGrid(v, v.temperature, p)) * (v.emissionScale * d);
+    return k;
+}
+// The ray's stretch inside the box, clipped to [tnear, tfar].
+bool clipBox(const Volume& v, V3 o, V3 d, float tnear, float tfar, float& a, float& b) {
+    a = tnear;
+    b = tfar;
+    for (int ax = 0; ax < 3; ++ax) {
+        const float oa = comp(o, ax), da = comp(d, ax), lo = comp(v.bmin, ax), hi = comp(v.bmax, ax);
+        if (da == 0.0f) {
+            if (oa < lo || oa > hi) return false;
+            continue;
+        }
+        float t0 = (lo - oa) / da, t1 = (hi - oa) / da;
+        if (t0 > t1) std::swap(t0, t1);
+        a = fmaxf(a, t0);
+        b = fminf(b, t1);
+    }
+    return a < b;
+}
+bool insideBox(const Volume& v, V3 p) {
+    return p.x >= v.bmin.x && p.x <= v.bmax.x && p.y >= v.bmin.y && p.y <= v.bmax.y && p.z >= v.bmin.z && p.z <= v.bmax.z;
+}
+// Ratio tracking (Novak et al. 2014) along [tnear, tfar] inside the box, brick by brick (a 3D DDA over the brick
+// grid): free flights at the brick's majorant extinction, each collision keeping 1 - density / majorant of the
+// estimate; nothing once it falls below the transmittance threshold (as the reference's transmittance gives up).
+float transmittance(const Volume& v, V3 o, V3 d, float tnear, float tfar, float scale, float threshold, Stream& s) {
+    float a, b;
+    if (!clipBox(v, o, d, tnear, tfar, a, b)) return 1.0f;
+    const V3 entry = o + d * a;
+    int cell[3], step[3];
+    float tNext[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        const float q = floorf((comp(entry, ax) - comp(v.bmin, ax)) * v.scaleVox[ax] * 0.125f);
+        cell[ax] = int(fminf(fmaxf(q, 0.0f), float(v.nb[ax] - 1)));
+        const float da = comp(d, ax);
+        step[ax] = da > 0.0f ? 1 : -1;
+        tNext[ax] = da == 0.0f ? FLT_MAX
+                               : (comp(v.bmin, ax) + float(cell[ax] + (da > 0.0f ? 1 : 0)) * v.brickExtent[ax] - comp(o, ax)) / da;
+    }
+    float T = 1.0f, t = a;
+    while (t < b) {
+        const int ax = tNext[0] <= tNext[1] ? (tNext[0] <= tNext[2] ? 0 : 2) : (tNext[1] <= tNext[2] ? 1 : 2);
+        const float tEnd = fminf(tNext[ax], b);
+        const float m = v.majorant[(U32(cell[2]) * v.nb[1] + U32(cell[1])) * v.nb[0] + U32(cell[0])];
+        if (m > 0.0f && tEnd > t) {
+            const float mu = m * v.sigmaTScale * scale;
+            float tt = t;
+            for (;;) {
+                tt = tt - logf(1.0f - s.next()) / mu;
+                if (!(tt < tEnd)) break;
+                T *= 1.0f - fminf(1.0f, densityAt(v, o + d * tt) / m);
+                if (T < threshold) return 0.0f;
+            }
+        }
+        t = fmaxf(t, tEnd);
+        cell[ax] += step[ax];
+        if (cell[ax] < 0 || U32(cell[ax]) >= v.nb[ax]) break;
+        const float da = comp(d, ax);
+        tNext[ax] = (comp(v.bmin, ax) + float(cell[ax] + (da > 0.0f ? 1 : 0)) * v.brickExtent[ax] - comp(o, ax)) / da;
+    }
+    return T;
+}
+// Henyey-Greenstein (volume_core.h:550-564).
+float phaseEval(float g, V3 wo, V3 wi) {
+    if (fabsf(g) <= 1e-3f) return OneOverFourPi;
+    const float cosTheta = dot(wo, wi), denom = 1 + g * g + 2 * g * cosTheta;
+    return fabsf(denom) < VolEpsilon ? 0.0f : OneOverFourPi * (1 - g * g) / (denom * sqrtf(denom));
+}
+V3 phaseSample(float g, V3 wo, float u1, float u2) {
+    if (fabsf(g) <= 1e-3f) {
+        const float cosTheta = 1.0f - 2.0f * u1, sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        return localSpherical(cosTheta, sinTheta, u2 * TwoPi);
+    }
+    u1 = fmaxf(u1, VolEpsilon);
+    const float sqrTerm = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+    const float cosTheta = (1.0f + g * g - sqrTerm * sqrTerm) / (2.0f * g);
+    const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+    return localToGlobal(frameOf(-wo), localSpherical(cosTheta, sinTheta, TwoPi * u2));
+}
+// Distance sampling in exp(-sigma t) over [a, b] and equi-angular sampling (volume_core.h:566-573).
+float sampleDistanceExponential(float u, float sigma, float a, float b) {
+    return a - logf(1.0f - u * (1.0f - expf(sigma * (a - b)))) / sigma;
+}
+float pdfDistanceExponential(float t, float sigma, float a, float b) {
+    return sigma / (expf((t - a) * sigma) - expf((t - b) * sigma));
+}
+float sampleEquiAngular(float u, float D, float thetaA, float thetaB) { return D * tanf((1 - u) * thetaA + u * thetaB); }
+float pdfEquiAngular(float t, float D, float thetaA, float thetaB) { return D / ((thetaB - thetaA) * (D * D + t * t)); }
+bool finitef(float x) { return x - x == 0.0f; }
+// A rect light's equi-angular pivot (volume_core.h:630-660, a rectangle without a texture).
+V3 lightPivot(const x2_light& l, const float r[3]) {
+    if (l.type != X2_LIGHT_RECT) return load3(l.position);
+    const V3 local{(0.5f - r[0]) * 2.0f * l.half_width, (0.5f - r[1]) * 2.0f * l.half_height, 0.0f};
+    return load3(l.position) + rows(l.local_to_render, local);
+}
+// One recorded step of a march: extinction, scattering, the transmittance before it, where it starts, its length.
+struct VStep {
+    float sigmaT;
+    V3 sigmaS;
+    float tr, tStart, delta;
+};
+struct MarchResult {
+    std::vector<VStep> steps;
+    std::vector<float> cdf;
+    float total = 1.0f, vt = 1.0f;
+    V3 emitted{0, 0, 0};
+    bool reached = false;
+};
+// The march over [a, b): step ends at a + stepSize * (u + k) (a lattice, each from the one before; the last step
+// ends at b), each step's coefficients at its end (VolumeMarch::run's heterogeneous stretch).
+void marchVolume(const Volume& v, const x2_config& c, V3 o, V3 d, float a, float b, U32 depth, bool sampleEmission,
+                 Stream marching, MarchResult& m) {
+    const float u = marching.next();
+    const float stepSize = v.minFeature * c.inv_volume_quality * fmaxf(float(int(depth) - 1), 1.0f);
+    for (U32 k = 0; k < MarchSteps; ++k) {
+        const float e = a + stepSize * (u + float(k));
+        const bool last = !(e < b);
+        if (last && !(e - b > VolEpsilon)) break;
+        if (m.vt < c.volume_transmittance_threshold) {
+            m.reached = true;
+            break;
+        }
+        const float at = last ? b : e, start = k == 0 ? a : a + stepSize * (u + float(k - 1));
+        const VCoeff q = coefficientsAt(v, o + d * at, sampleEmission);
+        if (q.sigmaT != 0.0f) {
+            const float delta = at - start;
+            m.steps.push_back(VStep{q.sigmaT, q.sigmaS, m.vt, start, delta});
+            if (sampleEmission) m.emitted = m.emitted + q.emission * (m.vt * delta);
+            m.vt *= expf(-q.sigmaT * delta);
+        }
+        if (last) break;
+    }
+    // tabulate (volume_core.h:595-612): each step's weight its scattering through what lies before it.
+    const U32 n = U32(m.steps.size());
+    if (!n) return;
+    float sum = 0.0f;
+    std::vector<float> w(n);
+    for (U32 i = 0; i < n; ++i) {
+        w[i] = (luminance(m.steps[i].sigmaS) * m.steps[i].delta) * m.steps[i].tr;
+        sum += w[i];
+    }
+    m.cdf.assign(n, 0.0f);
+    if (sum != 0.0f) {
+        float partial = 0.0f;
+        for (U32 i = 0; i + 1 < n; ++i) {
+            partial += w[i];
+            if (partial > sum) partial = sum;
+            m.cdf[i] = partial;
+        }
+    } else {
+        for (U32 i = 0; i + 1 < n; ++i) m.cdf[i] = float(i + 1) / float(n);
+        sum = 1.0f;
+    }
+    m.cdf[n - 1] = sum;
+    m.total = sum;
+}
+U32 pickStep(const MarchResult& m, float r, float& pdf, float& remapped) {  // sampleDiscrete
+    const U32 n = U32(m.steps.size());
+    const float rw = r * m.total;
+    U32 i = 0;
+    while (i + 1 < n && m.cdf[i] < rw) ++i;
+    const float prev = i > 0 ? m.cdf[i - 1] : 0.0f, value = m.cdf[i] - prev;
+    pdf = value / m.total;
+    remapped = (rw - prev) / value;
+    return i;
+}
+// findInterval: the last step starting at or before t, its pdfDiscrete, and whether it holds t (a t between
+// recorded steps lies in a step of zero density).
+U32 stepAt(const MarchResult& m, float t, float& pdf, bool& holds) {
+    const U32 n = U32(m.steps.size());
+    U32 k = 0;
+    while (k + 1 < n && m.steps[k + 1].tStart <= t) ++k;
+    pdf = ((m.cdf[k] - (k > 0 ? m.cdf[k - 1] : 0.0f)) / m.total);
+    holds = m.steps[k].tStart <= t && t <= m.steps[k].tStart + m.steps[k].delta;
+    return k;
+}
+// estimateInScatteringSourceTermDirect (:575-591): a light sample from P, its shadow ray, the transmittance
+// through the medium on its way (ratio tracking), the phase function.
+V3 volumeSourceTerm(const Scene& S, const x2_config& c, Stream& s, const x2_light& L, V3 P, V3 rayDir, float g,
+                    const float ul[3], float scale, Counters& k) {
+    V3 wi;
+    LightHit lh;
+    const float r[3] = {ul[0], ul[1], ul[2]};
+    if (!lightSample(L, P, nullptr, r, wi, lh)) return {0, 0, 0};
+    float pdfL;
+    const V3 Li = lightEval(L, wi, lh, pdfL);
+    if (pdfL == 0.0f || isExactlyZero(Li) || !finitef(pdfL)) return {0, 0, 0};
+    const float near = HitEpsilonStart, far = fmaxf(HitEpsilonEnd * lh.distance, near);
+    if (!visible(S, P, wi, near, far, k)) return {0, 0, 0};
+    const float tr = transmittance(S.volume, P, wi, 0.0f, far, scale, c.volume_transmittance_threshold, s);
+    const float ph = phaseEval(g, -rayDir, wi);
+    return Li * (tr * (ph / pdfL));
+}
+// The dense grids as bricks: a brick is kept when any of its density voxels is nonzero (temperature is kept for the
+// same bricks and reads zero elsewhere); a brick's majorant is the largest density among its own voxels and its 26
+// neighbours' (a point in the brick interpolates voxels at most one away from it).
+bool packVolume(const x2_volume& in, Volume& v) {
+    if (!in.density || in.sigma_t_scale < 0.0f) return false;
+    if (in.temperature && (!in.ramp || in.ramp_size < 2 || !(in.ramp_max_kelvin > 0.0f))) return false;
+    v = Volume{};
+    v.bmin = load3(in.bounds_min);
+    v.bmax = load3(in.bounds_max);
+    float feature = FLT_MAX;
+    for (int a = 0; a < 3; ++a) {
+        if (in.res[a] == 0 || in.res[a] % 8 || in.res[a] > 4096) return false;
+        const float ext = comp(v.bmax, a) - comp(v.bmin, a);
+        if (!(ext > 0.0f)) return false;
+        v.res[a] = in.res[a];
+        v.nb[a] = in.res[a] / 8;
+        v.scaleVox[a] = float(in.res[a]) / ext;
+        v.brickExtent[a] = ext / float(v.nb[a]);
+        feature = fminf(feature, ext / float(in.res[a]));
+    }
+    v.minFeature = feature;
+    v.sigmaTScale = in.sigma_t_scale;
+    v.albedo = load3(in.albedo);
+    v.g = in.g;
+    v.emissionScale = in.emission_scale;
+    v.hasTemperature = in.temperature != nullptr;
+    if (v.hasTemperature) {
+        v.rampMax = in.ramp_max_kelvin;
+        v.rampSize = in.ramp_size;
+        v.ramp.assign(in.ramp, in.ramp + size_t(in.ramp_size) * 3);
+    }
+    const U32 nx = v.nb[0], ny = v.nb[1], nz = v.nb[2];
+    const size_t bricks = size_t(nx) * ny * nz;
+    v.table.assign(bricks, ~0u);
+    std::vector<float> brickMax(bricks, 0.0f);
+    U32 used = 0;
+    for (U32 bz = 0; bz < nz; ++bz)
+        for (U32 by = 0; by < ny; ++by)
+            for (U32 bx = 0; bx < nx; ++bx) {
+                const size_t b = (size_t(bz) * ny + by) * nx + bx;
+                bool any = false;
+                float mx = 0.0f;
+                for (U32 z = 0; z < 8; ++z)
+                    for (U32 y = 0; y < 8; ++y)
+                        for (U32 x = 0; x < 8; ++x) {
+                            const uint16_t h = in.density[(size_t(bz * 8 + z) * v.res[1] + by * 8 + y) * v.res[0] + bx * 8 + x];
+                            if (h & 0x7fffu) any = true;
+                            mx = fmaxf(mx, halfToFloat(h));
+                        }
+                if (!any) continue;
+                brickMax[b] = mx;
+                v.table[b] = used++;
+                for (U32 z = 0; z < 8; ++z)
+                    for (U32 y = 0; y < 8; ++y)
+                        for (U32 x = 0; x < 8; ++x) {
+                            const size_t src = (size_t(bz * 8 + z) * v.res[1] + by * 8 + y) * v.res[0] + bx * 8 + x;
+                            v.density.push_back(in.density[src]);
+                            if (v.hasTemperature) v.temperature.push_back(in.temperature[src]);
+                        }
+            }
+    v.majorant.assign(bricks, 0.0f);
+    for (U32 bz = 0; bz < nz; ++bz)
+        for (U32 by = 0; by < ny; ++by)
+            for (U32 bx = 0; bx < nx; ++bx) {
+                float m = 0.0f;
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int x = int(bx) + dx, y = int(by) + dy, z = int(bz) + dz;
+                            if (x < 0 || y < 0 || z < 0 || U32(x) >= nx || U32(y) >= ny || U32(z) >= nz) continue;
+                            m = fmaxf(m, brickMax[(size_t(z) * ny + U32(y)) * nx + U32(x)]);
+                        }
+                v.majorant[(size_t(bz) * ny + by) * nx + bx] = m;
+            }
+    v.present = true;
+    return true;
+}
+struct VolumeState {
+    V3 T;
+    U32 depth, nonMirrorDepth, volumeDepth;
+    U64 branch;
+};
+// computeRadianceVolume for one ray: adds the medium's light to radiance, returns the alpha transmittance and takes
+// the march's transmittance into pv.T.
+float volumeRadiance(const Scene& S, const x2_config& c, const Identity& root, VolumeState& pv, V3 o, V3 d,
+                     float tnear, float tfar, V3& radiance, Counters& k) {
+    const Volume& v = S.volume;
+    float a, b;
+    if (!clipBox(v, o, d, tnear, tfar, a, b)) return 1.0f;
+    const U64 volumeBranch = childBranch(pv.branch, 0xfffc, pv.volumeDepth);
+    Identity vid = root;
+    vid.branch = childBranch(volumeBranch, 0, 0);
+    const bool sampleEmission = pv.depth == 0;
+    MarchResult m;
+    marchVolume(v, c, o, d, a, b, pv.depth, sampleEmission, Stream(vid, pv.depth, 0), m);
+    const U32 count = U32(m.steps.size());
+    if (count == 0) return 1.0f;
+    const float vt = m.reached ? 0.0f : m.vt;
+    const V3 T = pv.T;
+    if (sampleEmission && !isExactlyZero(m.emitted)) radiance = radiance + T * m.emitted;
+    if (pv.volumeDepth < c.max_volume_depth) {
+        const float firstStart = m.steps[0].tStart, rayEnd = m.steps[count - 1].tStart + m.steps[count - 1].delta;
+        // integrateVolumeScatteringDirect (:917-1012)
+        const bool highQuality = pv.nonMirrorDepth == 0;
+        const U32 samplesPerLight = highQuality ? c.light_samples : 1,
+                  scatterCount = highQuality ? c.volume_illumination_samples : 1;
+        if (scatterCount > 0) {
+            const float invN = 1.0f / float(samplesPerLight * scatterCount);
+            for (U32 li = 0; li < c.light_count; ++li) {
+                const x2_light& L = S.lights[li];
+                const bool equiAngular = L.type == X2_LIGHT_RECT;
+                V3 direct{0, 0, 0};
+                for (U32 ls = 0; ls < samplesPerLight; ++ls)
+                    for (U32 ts = 0; ts < scatterCount; ++ts) {
+                        Identity sid = root;
+                        sid.branch = childBranch(volumeBranch, 1 + li, ls * scatterCount + ts);
+                        Stream ss(sid, pv.depth, 0);
+                        float D = 0.0f, thetaA = 0.0f, thetaB = 0.0f, offset = 0.0f;
+                        if (equiAngular) {
+                            const float ue = ss.next();
+                            const float uel[3] = {ss.next(), ss.next(), ss.next()};
+                            const V3 pivot = lightPivot(L, uel);
+                            offset = dot(pivot - o, d);
+                            const V3 foot = o + d * offset;
+                            D = length(pivot - foot);
+                            thetaA = atan2f(firstStart - offset, D);
+                            thetaB = atan2f(rayEnd - offset, D);
+                            const float te = sampleEquiAngular(ue, D, thetaA, thetaB);
+                            const float pdfTe = pdfEquiAngular(te, D, thetaA, thetaB);
+                            if (pdfTe != 0.0f && finitef(pdfTe)) {
+                                const float tePrime = te + offset;
+                                float pdfK;
+                                bool holds;
+                                const VStep& vp = m.steps[stepAt(m, tePrime, pdfK, holds)];
+                                if (holds && !isExactlyZero(vp.sigmaS)) {
+                                    const float trScatter = vp.tr * expf(-vp.sigmaT * (tePrime - vp.tStart));
+                                    float mis = 1.0f;
+                                    const float pdfTd =
+                                        pdfK * pdfDistanceExponential(tePrime, vp.sigmaT, vp.tStart, vp.tStart + vp.delta);
+                                    if (finitef(pdfTd)) mis = powerHeuristic(pdfTe, pdfTd);
+                                    const V3 Ls = volumeSourceTerm(S, c, ss, L, o + d * tePrime, d, v.g, uel, 1.0f, k);
+                                    direct = direct + T * vp.sigmaS * (Ls * trScatter) * (invN * mis / pdfTe);
+                                }
+                            }
+                        }
+                        // distanceVolumeScatteringDirect
+                        const float ud = ss.next();
+                        const float udl[3] = {ss.next(), ss.next(), ss.next()};
+                        float pdfDensity, udRemapped;
+                        const VStep& vp = m.steps[pickStep(m, ud, pdfDensity, udRemapped)];
+                        if (pdfDensity == 0.0f || isExactlyZero(vp.sigmaS)) continue;
+                        const float t0 = vp.tStart, t1 = vp.tStart + vp.delta;
+                        const float td = sampleDistanceExponential(udRemapped, vp.sigmaT, t0, t1);
+                        const float pdfTd = pdfDensity * pdfDistanceExponential(td, vp.sigmaT, t0, t1);
+                        if (pdfTd == 0.0f || !finitef(pdfTd)) continue;
+                        const float trScatter = vp.tr * expf(-vp.sigmaT * (td - t0));
+                        float mis = 1.0f;
+                        if (equiAngular) {
+                            const float pdfTe = pdfEquiAngular(td - offset, D, thetaA, thetaB);
+                            if (finitef(pdfTe)) mis = powerHeuristic(pdfTd, pdfTe);
+                        }
+                        const V3 Ls = volumeSourceTerm(S, c, ss, L, o + d * td, d, v.g, udl, 1.0f, k);
+                        direct = direct + T * vp.sigmaS * (Ls * trScatter) * (invN * mis / pdfTd);
+                    }
+                radiance = radiance + direct;
+            }
+        }
+        ++pv.volumeDepth;
+        // approximateVolumeMultipleScattering (:1060-1149)
+        if (pv.volumeDepth < c.max_volume_depth && c.volume_illumination_samples > 0) {
+            const float invN = 1.0f / float(c.volume_illumination_samples);
+            for (U32 i = 0; i < c.volume_illumination_samples; ++i) {
+                Identity mid = root;
+                mid.branch = childBranch(volumeBranch, 0xffff, i);
+                Stream ms(mid, pv.depth, 0);
+                float pdfDensity, utRemapped;
+                const VStep& vp = m.steps[pickStep(m, ms.next(), pdfDensity, utRemapped)];
+                float sigmaT = vp.sigmaT;
+                const float t0 = vp.tStart, t1 = vp.tStart + vp.delta;
+                const float td = sampleDistanceExponential(utRemapped, sigmaT, t0, t1);
+                const float pdfT = pdfDensity * pdfDistanceExponential(td, sigmaT, t0, t1);
+                if (pdfT == 0.0f || !finitef(pdfT)) continue;
+                const float trScatter = vp.tr * expf(-vp.sigmaT * (td - t0));
+                V3 scatterPoint = o + d * td;
+                V3 throughput = T * vp.sigmaS * (trScatter / pdfT);
+                const float up0 = ms.next(), up1 = ms.next();
+                V3 scatterDir = phaseSample(v.g, -d, up0, up1);
+                for (U32 depth = pv.volumeDepth; depth < c.max_volume_depth; ++depth) {
+                    if (luminance(throughput) < c.volume_transmittance_threshold) break;
+                    const float ai = powf(c.volume_attenuation_factor, float(depth)),
+                                bi = powf(c.volume_contribution_factor, float(depth)),
+                                ci = powf(c.volume_phase_attenuation_factor, float(depth));
+                    const float tFreePath = -logf(1.0f - ms.next()) / sigmaT;
+                    const V3 rayDir = scatterDir;
+                    scatterPoint = scatterPoint + scatterDir * tFreePath;
+                    if (!insideBox(v, scatterPoint)) break;
+                    const VCoeff kk = coefficientsAt(v, scatterPoint, false);
+                    const float g = v.g * ci;
+                    sigmaT = kk.sigmaT;
+                    if (fabsf(sigmaT) < VolEpsilon) break;
+                    throughput = throughput * kk.sigmaS * (1.0f / sigmaT);
+                    for (U32 li = 0; li < c.light_count; ++li) {
+                        const float usl[3] = {ms.next(), ms.next(), ms.next()};
+                        const V3 Ls = volumeSourceTerm(S, c, ms, S.lights[li], scatterPoint, rayDir, g, usl, ai, k);
+                        radiance = radiance + throughput * Ls * (invN * bi);
+                    }
+                    const float uq0 = ms.next(), uq1 = ms.next();
+                    scatterDir = phaseSample(g, -rayDir, uq0, uq1);
+                }
+            }
+        }
+    }
+    pv.T = pv.T * vt;
+    return vt;
 }
 
 int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double out[4], Counters& k,
@@ -999,9 +1501,19 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
             const float kept = hit.geom != ~0u && hit.t < pv.heldFar ? 0.0f : 1.0f;
             if (kept > 0.0f) radiance = radiance + (kept < 1.0f ? pv.held * kept : pv.held);
         }
+        // The medium the ray passes on its way (:2106-2120): its light, and the throughput through it.
+        float volumeAlpha = 1.0f;
+        if (S.volume.present) {
+            if (adj) return 2;  // the adjoint does not differentiate through media
+            VolumeState vs{pv.T, pv.depth, pv.nonMirrorDepth, pv.volumeDepth, pv.branch};
+            volumeAlpha = volumeRadiance(S, c, root, vs, ray.origin, ray.direction, ray.nearT,
+                                         hit.geom == ~0u ? ray.farT : hit.t, radiance, k);
+            pv.T = vs.T;
+            pv.volumeDepth = vs.volumeDepth;
+        }
         const float hitAlpha = 1.0f - (0.0f + 0.0f + 0.0f) / 3.0f;
-        if (hit.geom == ~0u) {
-            if (pv.depth == 0) alpha += 1.0f * (1.0f - (1.0f + 1.0f + 1.0f) / 3.0f);  // camera pixel weight 1
+        if (hit.geom == ~0u) {  // camera pixel weight 1; a medium's opacity as the alpha (:2160-2161)
+            if (pv.depth == 0) alpha += 1.0f * (1.0f - (volumeAlpha + volumeAlpha + volumeAlpha) / 3.0f);
             continue;
         }
         // shade::hitState (shade_material.h:121-289) for a flat triangle without normals: Ng = (b-a) x (c-a),
@@ -1130,6 +1642,13 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                 const float kept = visible(S, shadowOrigin, wi, shadowNear, shadowFar, k) ? 1.0f : 0.0f;
                 if (kept > 0.0f) {
                     if (kept < 1.0f) t = t * kept;
+                    if (S.volume.present) {  // the medium on the shadow ray's way (:2725-2732)
+                        Identity tid = lid;
+                        tid.branch = childBranch(lightAddress, s, 0xfffb);
+                        Stream vs(tid, pv.depth, 0);
+                        t = t * transmittance(S.volume, shadowOrigin, wi, shadowNear, shadowFar, 1.0f,
+                                              c.volume_transmittance_threshold, vs);
+                    }
                     radiance = radiance + t;
                     if (adj)
                         for (U32 q = 0; q < bsdf.count; ++q) {
@@ -1216,8 +1735,8 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                 bool held = false;
                 if (lightHit && !isBlack(direct) && finite3(direct)) {
                     lightShadowSpan(lightDistance, shadowNear, shadowFar);
-                    held = bounces && thinning >= 1.0f && top + PendingReserve < MaxPending && lobeMask &&
-                           (c.shadow_reuse & lobeMask) == lobeMask && shadowNear == 0.0f;
+                    held = !S.volume.present && bounces && thinning >= 1.0f && top + PendingReserve < MaxPending &&
+                           lobeMask && (c.shadow_reuse & lobeMask) == lobeMask && shadowNear == 0.0f;
                     if (!held) kept = visible(S, origin, wi, shadowNear, shadowFar, k) ? 1.0f : 0.0f;
                 }
                 const V3 directBefore = direct;
@@ -1230,6 +1749,13 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                 }
                 if (kept > 0.0f && !held) {
                     if (kept < 1.0f) direct = direct * kept;
+                    if (S.volume.present) {  // (:2939-2945)
+                        Identity tid = bid;
+                        tid.branch = childBranch(sampleBranch, 0xfffb, 0);
+                        Stream vs(tid, pv.depth, 0);
+                        direct = direct * transmittance(S.volume, origin, wi, shadowNear, shadowFar, 1.0f,
+                                                        c.volume_transmittance_threshold, vs);
+                    }
                     radiance = radiance + direct;
                     if (adj) scatterSink(S, *adj, lobeSink, direct, delta);
                 }
@@ -1263,6 +1789,7 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                 child.farT = 3.402823466e+38f;
                 child.nearT = 0.0f;
                 child.heldLight = ~0u;
+                child.volumeDepth = pv.volumeDepth + 1;  // (:3000)
                 if (held) {
                     child.heldLight = lightIndex;
                     child.heldFar = shadowFar;
@@ -1552,6 +2079,10 @@ oracle_scene* oracle_scene_create(const x2_scene* in) {
     }
     s.materials.assign(in->materials, in->materials + in->material_count);
     if (in->light_count) s.lights.assign(in->lights, in->lights + in->light_count);
+    if (in->volume_count && in->volumes && !packVolume(in->volumes[0], s.volume)) {
+        delete o;
+        return nullptr;
+    }
     buildBvh(s);
     return o;
 }
@@ -1609,6 +2140,35 @@ int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]) {
     if (!textureTap(tex, 0, u, v, tap)) return 0;
     store3(rgb, tap.color);
     return 1;
+}
+float oracle_volume_density(const oracle_scene* o, const float p[3]) {
+    return o->s.volume.present ? densityAt(o->s.volume, load3(p)) : 0.0f;
+}
+float oracle_volume_transmittance(const oracle_scene* o, const x2_config* c, const float org[3], const float dir[3],
+                                  float tnear, float tfar, uint32_t pixel, uint64_t index, uint64_t branch) {
+    if (!o->s.volume.present) return 1.0f;
+    Stream s(Identity{pixel, index, c->seed, branch}, 0, 0);
+    return transmittance(o->s.volume, load3(org), load3(dir), tnear, tfar, 1.0f, c->volume_transmittance_threshold, s);
+}
+float oracle_phase_eval(float g, const float wo[3], const float wi[3]) { return phaseEval(g, load3(wo), load3(wi)); }
+void oracle_phase_sample(float g, const float wo[3], float u1, float u2, float wi[3]) {
+    store3(wi, phaseSample(g, load3(wo), u1, u2));
+}
+int oracle_volume_march(const oracle_scene* o, const x2_config* c, const float org[3], const float dir[3], float tnear,
+                        float tfar, uint32_t depth, uint64_t branch, float out[8]) {
+    const Volume& v = o->s.volume;
+    float a, b;
+    if (!v.present || !clipBox(v, load3(org), load3(dir), tnear, tfar, a, b)) return 0;
+    MarchResult m;
+    marchVolume(v, *c, load3(org), load3(dir), a, b, depth, depth == 0, Stream(Identity{0, 0, c->seed, branch}, depth, 0), m);
+    const U32 n = U32(m.steps.size());
+    out[0] = m.reached ? 0.0f : m.vt;
+    store3(out + 1, m.emitted);
+    out[4] = n ? m.steps[0].tStart : 0.0f;
+    out[5] = n ? m.steps[n - 1].tStart + m.steps[n - 1].delta : 0.0f;
+    out[6] = m.total;
+    out[7] = m.reached ? 1.0f : 0.0f;
+    return int(n);
 }
 int oracle_render_chunks(const oracle_scene* o, const x2_config* c, const x2_chunk* chunks, uint32_t n,
                          uint32_t threads, double* sums, uint32_t* counts, uint64_t counters[3]) {

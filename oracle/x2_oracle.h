@@ -87,6 +87,16 @@ void oracle_set_eval_roughness(oracle_scene* s, uint32_t material, float r);
 /* d ln f / d r of the GGX lobe, and one bilinear texture lookup at (u, v). */
 float oracle_ggx_roughness_log_derivative(float r, const float N[3], const float wo[3], const float wi[3]);
 int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]);
+/* The medium, piece by piece (for the tests): density at p; one ratio-tracking transmittance estimate along a ray
+ * with its stream at (pixel, index, the config's seed, branch); the Henyey-Greenstein phase; one march (returns its
+ * recorded steps; out: transmittance, emitted rgb, first step start, last step end, weight total, reached). */
+float oracle_volume_density(const oracle_scene* s, const float p[3]);
+float oracle_volume_transmittance(const oracle_scene* s, const x2_config* c, const float org[3], const float dir[3],
+                                  float tnear, float tfar, uint32_t pixel, uint64_t index, uint64_t branch);
+float oracle_phase_eval(float g, const float wo[3], const float wi[3]);
+void oracle_phase_sample(float g, const float wo[3], float u1, float u2, float wi[3]);
+int oracle_volume_march(const oracle_scene* s, const x2_config* c, const float org[3], const float dir[3], float tnear,
+                        float tfar, uint32_t depth, uint64_t branch, float out[8]);
 
 #ifdef __cplusplus
 }

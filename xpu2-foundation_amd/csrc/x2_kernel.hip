@@ -760,10 +760,488 @@ __device__ __forceinline__ bool visible(const RenderParams& P, V3 o, V3 d, float
     return !anyHit(P, o, d, nearT, farT, k, st);
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// The participating medium (the structure of computeRadianceVolume, async_integrator.h:1152-1291, over this
+// device's own medium: a box-bounded brick grid of FP16 density and temperature, trilinear, grey extinction, a
+// blackbody ramp for emission; shadow transmittance by ratio tracking under per-brick majorants in place of the
+// reference's transmittance march, :428-476). The first march along a ray keeps no step table (a march may take
+// hundreds of steps): it leaves checkpoints (MarchRecord, :622-648, 8 slots with stride doubling) and each lookup
+// the light sampling needs marches again from the nearest one (ReplaySteps, :823-888), bit for bit the same steps.
+constexpr float VolEpsilon = 1.1920928955078125e-07f;
+constexpr float HitEpsilonStart = 1e-5f;
+constexpr float OneOverFourPi = 0.0795774715459476678844f;
+constexpr uint32_t MarchSteps = 1u << 16;  // lattice steps a march takes at most
+__device__ __forceinline__ float halfValue(uint16_t h) { return float(__builtin_bit_cast(_Float16, h)); }
+__device__ __forceinline__ float lerpf(float a, float b, float w) { return a + (b - a) * w; }
+__device__ __forceinline__ float voxelAt(const VolumeDesc& v, const uint16_t* pool, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || uint32_t(i) >= v.res[0] || uint32_t(j) >= v.res[1] || uint32_t(k) >= v.res[2]) return 0.0f;
+    const uint32_t e = v.table[(uint32_t(k >> 3) * v.nb[1] + uint32_t(j >> 3)) * v.nb[0] + uint32_t(i >> 3)];
+    if (e == ~0u) return 0.0f;
+    return halfValue(pool[size_t(e) * 512 + (uint32_t(k & 7) * 8 + uint32_t(j & 7)) * 8 + uint32_t(i & 7)]);
+}
+__device__ float sampleGrid(const VolumeDesc& v, const uint16_t* pool, V3 p) {
+    float q[3];
+    for (int a = 0; a < 3; ++a) {
+        q[a] = (comp(p, a) - v.bmin[a]) * v.scaleVox[a] - 0.5f;
+        if (!(q[a] > -1.0f && q[a] < float(v.res[a]))) return 0.0f;
+    }
+    const float fx0 = floorf(q[0]), fy0 = floorf(q[1]), fz0 = floorf(q[2]);
+    const int i = int(fx0), j = int(fy0), k = int(fz0);
+    const float fx = q[0] - fx0, fy = q[1] - fy0, fz = q[2] - fz0;
+    const float c00 = lerpf(voxelAt(v, pool, i, j, k), voxelAt(v, pool, i + 1, j, k), fx);
+    const float c10 = lerpf(voxelAt(v, pool, i, j + 1, k), voxelAt(v, pool, i + 1, j + 1, k), fx);
+    const float c01 = lerpf(voxelAt(v, pool, i, j, k + 1), voxelAt(v, pool, i + 1, j, k + 1), fx);
+    const float c11 = lerpf(voxelAt(v, pool, i, j + 1, k + 1), voxelAt(v, pool, i + 1, j + 1, k + 1), fx);
+    return lerpf(lerpf(c00, c10, fy), lerpf(c01, c11, fy), fz);
+}
+__device__ __forceinline__ uint32_t brickAt(const VolumeDesc& v, V3 p) {
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+        const float q = floorf((comp(p, a) - v.bmin[a]) * v.scaleVox[a] * 0.125f);
+        b[a] = int(fminf(fmaxf(q, 0.0f), float(v.nb[a] - 1)));
+    }
+    return (uint32_t(b[2]) * v.nb[1] + uint32_t(b[1])) * v.nb[0] + uint32_t(b[0]);
+}
+__device__ __forceinline__ float densityAt(const VolumeDesc& v, V3 p) {
+    return v.majorant[brickAt(v, p)] > 0.0f ? sampleGrid(v, v.density, p) : 0.0f;
+}
+__device__ V3 rampAt(const VolumeDesc& v, float kelvin) {
+    float x = kelvin / v.rampMax * float(v.rampSize - 1);
+    x = fminf(fmaxf(x, 0.0f), float(v.rampSize - 1));
+    const float fi = floorf(x);
+    const uint32_t i = uint32_t(fi);
+    if (i + 1 >= v.rampSize) return ld3(v.ramp + size_t(v.rampSize - 1) * 3);
+    const float f = x - fi;
+    const float* a = v.ramp + size_t(i) * 3;
+    return {lerpf(a[0], a[3], f), lerpf(a[1], a[4], f), lerpf(a[2], a[5], f)};
+}
+struct VCoeff {
+    float sigmaT;
+    V3 sigmaS, emission;
+};
+__device__ VCoeff coefficientsAt(const VolumeDesc& v, V3 p, bool withEmission) {
+    VCoeff k{0.0f, {0, 0, 0}, {0, 0, 0}};
+    const float d = densityAt(v, p);
+    if (d == 0.0f) return k;
+    k.sigmaT = d * v.sigmaTScale;
+    k.sigmaS = ld3(v.albedo) * k.sigmaT;
+    if (withEmission && v.temperature && v.emissionScale > 0.0f)
+        k.emission = rampAt(v, sampleGrid(v, v.temperature, p)) * (v.emissionScale * d);
+    return k;
+}
+__device__ bool clipBox(const VolumeDesc& v, V3 o, V3 d, float tnear, float tfar, float& a, float& b) {
+    a = tnear;
+    b = tfar;
+    for (int ax = 0; ax < 3; ++ax) {
+        const float oa = comp(o, ax), da = comp(d, ax), lo = v.bmin[ax], hi = v.bmax[ax];
+        if (da == 0.0f) {
+            if (oa < lo || oa > hi) return false;
+            continue;
+        }
+        float t0 = (lo - oa) / da, t1 = (hi - oa) / da;
+        if (t0 > t1) {
+            const float s = t0;
+            t0 = t1;
+            t1 = s;
+        }
+        a = fmaxf(a, t0);
+        b = fminf(b, t1);
+    }
+    return a < b;
+}
+__device__ __forceinline__ bool insideBox(const VolumeDesc& v, V3 p) {
+    return p.x >= v.bmin[0] && p.x <= v.bmax[0] && p.y >= v.bmin[1] && p.y <= v.bmax[1] && p.z >= v.bmin[2] &&
+           p.z <= v.bmax[2];
+}
+// Ratio tracking brick by brick (a 3D DDA over the brick grid): free flights at the brick's majorant extinction,
+// each collision keeping 1 - density / majorant; nothing below the transmittance threshold.
+__device__ float transmittance(const VolumeDesc& v, V3 o, V3 d, float tnear, float tfar, float scale, float threshold,
+                               Stream& s) {
+    float a, b;
+    if (!clipBox(v, o, d, tnear, tfar, a, b)) return 1.0f;
+    const V3 entry = o + d * a;
+    int cell[3], step[3];
+    float tNext[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        const float q = floorf((comp(entry, ax) - v.bmin[ax]) * v.scaleVox[ax] * 0.125f);
+        cell[ax] = int(fminf(fmaxf(q, 0.0f), float(v.nb[ax] - 1)));
+        const float da = comp(d, ax);
+        step[ax] = da > 0.0f ? 1 : -1;
+        tNext[ax] = da == 0.0f ? FLT_MAX
+                               : (v.bmin[ax] + float(cell[ax] + (da > 0.0f ? 1 : 0)) * v.brickExtent[ax] - comp(o, ax)) / da;
+    }
+    float T = 1.0f, t = a;
+    while (t < b) {
+        const int ax = tNext[0] <= tNext[1] ? (tNext[0] <= tNext[2] ? 0 : 2) : (tNext[1] <= tNext[2] ? 1 : 2);
+        const float tEnd = fminf(tNext[ax], b);
+        const float m = v.majorant[(uint32_t(cell[2]) * v.nb[1] + uint32_t(cell[1])) * v.nb[0] + uint32_t(cell[0])];
+        if (m > 0.0f && tEnd > t) {
+            const float mu = m * v.sigmaTScale * scale;
+            float tt = t;
+            for (;;) {
+                tt = tt - logf(1.0f - s.next()) / mu;
+                if (!(tt < tEnd)) break;
+                T *= 1.0f - fminf(1.0f, densityAt(v, o + d * tt) / m);
+                if (T < threshold) return 0.0f;
+            }
+        }
+        t = fmaxf(t, tEnd);
+        cell[ax] += step[ax];
+        if (cell[ax] < 0 || uint32_t(cell[ax]) >= v.nb[ax]) break;
+        const float da = comp(d, ax);
+        tNext[ax] = (v.bmin[ax] + float(cell[ax] + (da > 0.0f ? 1 : 0)) * v.brickExtent[ax] - comp(o, ax)) / da;
+    }
+    return T;
+}
+// Henyey-Greenstein (volume_core.h:550-564), distance and equi-angular sampling (:566-573).
+__device__ float phaseEval(float g, V3 wo, V3 wi) {
+    if (fabsf(g) <= 1e-3f) return OneOverFourPi;
+    const float cosTheta = dot(wo, wi), denom = 1 + g * g + 2 * g * cosTheta;
+    return fabsf(denom) < VolEpsilon ? 0.0f : OneOverFourPi * (1 - g * g) / (denom * sqrtf(denom));
+}
+__device__ V3 phaseSample(float g, V3 wo, float u1, float u2) {
+    if (fabsf(g) <= 1e-3f) {
+        const float cosTheta = 1.0f - 2.0f * u1, sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        return localSpherical(cosTheta, sinTheta, u2 * TwoPi);
+    }
+    u1 = fmaxf(u1, VolEpsilon);
+    const float sqrTerm = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+    const float cosTheta = (1.0f + g * g - sqrTerm * sqrTerm) / (2.0f * g);
+    const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+    return localToGlobal(frameOf(-wo), localSpherical(cosTheta, sinTheta, TwoPi * u2));
+}
+__device__ __forceinline__ float sampleDistanceExponential(float u, float sigma, float a, float b) {
+    return a - logf(1.0f - u * (1.0f - expf(sigma * (a - b)))) / sigma;
+}
+__device__ __forceinline__ float pdfDistanceExponential(float t, float sigma, float a, float b) {
+    return sigma / (expf((t - a) * sigma) - expf((t - b) * sigma));
+}
+__device__ __forceinline__ float pdfEquiAngular(float t, float D, float thetaA, float thetaB) {
+    return D / ((thetaB - thetaA) * (D * D + t * t));
+}
+__device__ __forceinline__ bool finitef(float x) { return x - x == 0.0f; }
+
+// A march: step ends on the lattice a + stepSize * (u + k), the last step ending at b, each step's coefficients at
+// its end; a step of zero density is not a step (it carries nothing).
+struct VStep {
+    float sigmaT;
+    V3 sigmaS;
+    float tr, tStart, delta;
+};
+struct MarchCk {  // the march as a recorded step begins: its lattice index, transmittance, weights so far, count
+    uint32_t k;
+    float vt, partial;
+    uint32_t count;
+};
+struct MarchRecord {
+    static constexpr uint32_t Slots = 8;
+    MarchCk at[Slots];
+    uint32_t used = 0, stride = 8, last = 0;
+    __device__ bool wants(uint32_t count) const {
+        if (count == 0 || count == last || count % stride) return false;
+        return used < Slots || count % (2 * stride) == 0;
+    }
+    __device__ MarchCk& slot(uint32_t count) {
+        if (used == Slots) {
+            for (uint32_t i = 0; i < Slots / 2; ++i) at[i] = at[2 * i + 1];
+            used = Slots / 2;
+            stride *= 2;
+        }
+        last = count;
+        return at[used++];
+    }
+};
+struct March {
+    V3 o, d;
+    float a, b, u, stepSize;
+    // the first march's totals
+    uint32_t count;
+    float vt, total;
+    bool reached, weightless;
+    V3 emitted;
+    float firstStart, lastEnd;
+    MarchRecord rec;
+    __device__ float endOf(uint32_t k) const { return a + stepSize * (u + float(k)); }
+    __device__ float startOf(uint32_t k) const { return k == 0 ? a : a + stepSize * (u + float(k - 1)); }
+};
+// Runs the march from step k0 with the state (vt, count) there; emit(step, index) returns false to stop. first: the
+// first march (records checkpoints, adds emission, sets the totals).
+template <bool First, class Emit>
+__device__ void runMarch(const VolumeDesc& v, const x2_config& c, March& m, uint32_t k0, float vt, float partial,
+                         uint32_t count, bool sampleEmission, Emit&& emit) {
+    for (uint32_t k = k0; k < MarchSteps; ++k) {
+        const float e = m.endOf(k);
+        const bool last = !(e < m.b);
+        if (last && !(e - m.b > VolEpsilon)) break;
+        if (vt < c.volume_transmittance_threshold) {
+            if constexpr (First) m.reached = true;
+            break;
+        }
+        const float at = last ? m.b : e, start = m.startOf(k);
+        const VCoeff q = coefficientsAt(v, m.o + m.d * at, First && sampleEmission);
+        if (q.sigmaT != 0.0f) {
+            const float delta = at - start;
+            const VStep s{q.sigmaT, q.sigmaS, vt, start, delta};
+            const float w = (luminance(q.sigmaS) * delta) * vt;
+            if constexpr (First) {
+                if (m.rec.wants(count)) m.rec.slot(count) = MarchCk{k, vt, partial, count};
+                if (count == 0) m.firstStart = start;
+                m.lastEnd = start + delta;
+                if (sampleEmission) m.emitted = m.emitted + q.emission * (vt * delta);
+            }
+            if (!emit(s, count, w)) return;
+            partial += w;
+            ++count;
+            vt *= expf(-q.sigmaT * delta);
+        }
+        if (last) break;
+    }
+    if constexpr (First) {
+        m.count = count;
+        m.vt = vt;
+        m.total = partial;
+    }
+}
+__device__ void firstMarch(const VolumeDesc& v, const x2_config& c, March& m, bool sampleEmission) {
+    m.count = 0;
+    m.reached = false;
+    m.emitted = {0, 0, 0};
+    m.firstStart = m.lastEnd = 0.0f;
+    runMarch<true>(v, c, m, 0, 1.0f, 0.0f, 0, sampleEmission, [](const VStep&, uint32_t, float) { return true; });
+    m.weightless = m.total == 0.0f;
+    if (m.weightless) m.total = 1.0f;
+}
+// The cdf as the oracle's table gives it: weights summed in order, each partial no more than the total, the last the
+// total; even where none weighs.
+__device__ __forceinline__ float cdfAt(const March& m, float& partial, float w, uint32_t index) {
+    if (index + 1 >= m.count) return m.total;
+    if (m.weightless) return float(index + 1) / float(m.count);
+    partial += w;
+    if (partial > m.total) partial = m.total;
+    return partial;
+}
+__device__ __forceinline__ float cdfBefore(const March& m, const MarchCk& ck) {
+    return m.weightless ? float(ck.count) / float(m.count) : ck.partial;
+}
+// sampleDiscrete by marching again from the last checkpoint short of r * total.
+__device__ VStep pickStep(const VolumeDesc& v, const x2_config& c, March& m, float r, float& pdf, float& remapped) {
+    const float rw = r * m.total;
+    int from = -1;
+    for (int j = int(m.rec.used) - 1; j >= 0; --j)
+        if (m.rec.at[j].count < m.count && cdfBefore(m, m.rec.at[j]) < rw) {
+            from = j;
+            break;
+        }
+    const MarchCk ck = from >= 0 ? m.rec.at[from] : MarchCk{0, 1.0f, 0.0f, 0};
+    float partial = ck.partial, prev = from >= 0 ? cdfBefore(m, ck) : 0.0f, found = m.total, before = 0.0f;
+    VStep chosen{0.0f, {0, 0, 0}, 0.0f, 0.0f, 0.0f};
+    runMarch<false>(v, c, m, ck.k, ck.vt, ck.partial, ck.count, false, [&](const VStep& s, uint32_t i, float w) {
+        const float here = cdfAt(m, partial, w, i);
+        chosen = s;
+        if (here >= rw || i + 1 >= m.count) {
+            found = here;
+            before = prev;
+            return false;
+        }
+        prev = here;
+        return true;
+    });
+    const float value = found - before;
+    pdf = value / m.total;
+    remapped = (rw - before) / value;
+    return chosen;
+}
+// findInterval by marching again: the last step starting at or before t, its pdfDiscrete, whether it holds t.
+__device__ VStep stepAt(const VolumeDesc& v, const x2_config& c, March& m, float t, float& pdf, bool& holds) {
+    int from = -1;
+    for (int j = int(m.rec.used) - 1; j >= 0; --j)
+        if (m.rec.at[j].count < m.count && m.startOf(m.rec.at[j].k) <= t) {
+            from = j;
+            break;
+        }
+    const MarchCk ck = from >= 0 ? m.rec.at[from] : MarchCk{0, 1.0f, 0.0f, 0};
+    float partial = ck.partial, here = from >= 0 ? cdfBefore(m, ck) : 0.0f, kept = 0.0f, keptPrev = 0.0f;
+    VStep chosen{0.0f, {0, 0, 0}, 0.0f, 0.0f, 0.0f};
+    runMarch<false>(v, c, m, ck.k, ck.vt, ck.partial, ck.count, false, [&](const VStep& s, uint32_t i, float w) {
+        if (i > 0 && s.tStart > t) return false;
+        const float before = here;
+        here = cdfAt(m, partial, w, i);
+        chosen = s;
+        kept = here;
+        keptPrev = i > 0 ? before : 0.0f;
+        return i + 1 < m.count;
+    });
+    pdf = (kept - keptPrev) / m.total;
+    holds = chosen.tStart <= t && t <= chosen.tStart + chosen.delta;
+    return chosen;
+}
+// estimateInScatteringSourceTermDirect (:575-591).
+__device__ V3 volumeSourceTerm(const RenderParams& P, Stream& s, const x2_light& L, V3 Pt, V3 rayDir, float g,
+                               const float ul[3], float scale, Counters& k, TravStack& st) {
+    V3 wi;
+    LightHit lh;
+    if (!lightSample(L, Pt, false, V3{0, 0, 0}, ul, wi, lh)) return {0, 0, 0};
+    float pdfL;
+    const V3 Li = lightEval(L, wi, lh, pdfL);
+    if (pdfL == 0.0f || isExactlyZero(Li) || !finitef(pdfL)) return {0, 0, 0};
+    const float near = HitEpsilonStart, far = fmaxf(HitEpsilonEnd * lh.distance, near);
+    if (!visible(P, Pt, wi, near, far, k, st)) return {0, 0, 0};
+    const float tr = transmittance(P.vol, Pt, wi, 0.0f, far, scale, P.cfg.volume_transmittance_threshold, s);
+    const float ph = phaseEval(g, -rayDir, wi);
+    return Li * (tr * (ph / pdfL));
+}
+// computeRadianceVolume for one ray: the medium's light into radiance, the march's transmittance into T; returns
+// the alpha transmittance.
+__device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3& T, uint32_t depth,
+                                uint32_t nonMirrorDepth, uint32_t& volumeDepth, uint64_t branch, V3 o, V3 d,
+                                float tnear, float tfar, V3& radiance, Counters& k, TravStack& st) {
+    const VolumeDesc& v = P.vol;
+    const x2_config& c = P.cfg;
+    March m;
+    if (!clipBox(v, o, d, tnear, tfar, m.a, m.b)) return 1.0f;
+    m.o = o;
+    m.d = d;
+    const uint64_t volumeBranch = childBranch(branch, 0xfffc, volumeDepth);
+    Identity vid = root;
+    vid.branch = childBranch(volumeBranch, 0, 0);
+    {
+        Stream marching(vid, depth);
+        m.u = marching.next();
+    }
+    m.stepSize = v.minFeature * c.inv_volume_quality * fmaxf(float(int(depth) - 1), 1.0f);
+    const bool sampleEmission = depth == 0;
+    firstMarch(v, c, m, sampleEmission);
+    if (m.count == 0) return 1.0f;
+    const float vt = m.reached ? 0.0f : m.vt;
+    const V3 T0 = T;
+    if (sampleEmission && !isExactlyZero(m.emitted)) radiance = radiance + T0 * m.emitted;
+    if (volumeDepth < c.max_volume_depth) {
+        const float firstStart = m.firstStart, rayEnd = m.lastEnd;
+        const bool highQuality = nonMirrorDepth == 0;
+        const uint32_t samplesPerLight = highQuality ? c.light_samples : 1,
+                       scatterCount = highQuality ? c.volume_illumination_samples : 1;
+        if (scatterCount > 0) {
+            const float invN = 1.0f / float(samplesPerLight * scatterCount);
+            for (uint32_t li = 0; li < c.light_count; ++li) {
+                const x2_light& L = P.lights[li];
+                const bool equiAngular = L.type == X2_LIGHT_RECT;
+                V3 direct{0, 0, 0};
+                for (uint32_t ls = 0; ls < samplesPerLight; ++ls)
+                    for (uint32_t ts = 0; ts < scatterCount; ++ts) {
+                        Identity sid = root;
+                        sid.branch = childBranch(volumeBranch, 1 + li, ls * scatterCount + ts);
+                        Stream ss(sid, depth);
+                        float D = 0.0f, thetaA = 0.0f, thetaB = 0.0f, offset = 0.0f;
+                        if (equiAngular) {
+                            const float ue = ss.next();
+                            float uel[3];
+                            uel[0] = ss.next();
+                            uel[1] = ss.next();
+                            uel[2] = ss.next();
+                            const V3 local{(0.5f - uel[0]) * 2.0f * L.half_width, (0.5f - uel[1]) * 2.0f * L.half_height, 0.0f};
+                            const V3 pivot = ld3(L.position) + rows(L.local_to_render, local);
+                            offset = dot(pivot - o, d);
+                            const V3 foot = o + d * offset;
+                            D = length(pivot - foot);
+                            thetaA = atan2f(firstStart - offset, D);
+                            thetaB = atan2f(rayEnd - offset, D);
+                            const float te = D * tanf((1 - ue) * thetaA + ue * thetaB);
+                            const float pdfTe = pdfEquiAngular(te, D, thetaA, thetaB);
+                            if (pdfTe != 0.0f && finitef(pdfTe)) {
+                                const float tePrime = te + offset;
+                                float pdfK;
+                                bool holds;
+                                const VStep vp = stepAt(v, c, m, tePrime, pdfK, holds);
+                                if (holds && !isExactlyZero(vp.sigmaS)) {
+                                    const float trScatter = vp.tr * expf(-vp.sigmaT * (tePrime - vp.tStart));
+                                    float mis = 1.0f;
+                                    const float pdfTd =
+                                        pdfK * pdfDistanceExponential(tePrime, vp.sigmaT, vp.tStart, vp.tStart + vp.delta);
+                                    if (finitef(pdfTd)) mis = powerHeuristic(pdfTe, pdfTd);
+                                    const V3 Ls = volumeSourceTerm(P, ss, L, o + d * tePrime, d, v.g, uel, 1.0f, k, st);
+                                    direct = direct + T0 * vp.sigmaS * (Ls * trScatter) * (invN * mis / pdfTe);
+                                }
+                            }
+                        }
+                        const float ud = ss.next();
+                        float udl[3];
+                        udl[0] = ss.next();
+                        udl[1] = ss.next();
+                        udl[2] = ss.next();
+                        float pdfDensity, udRemapped;
+                        const VStep vp = pickStep(v, c, m, ud, pdfDensity, udRemapped);
+                        if (pdfDensity == 0.0f || isExactlyZero(vp.sigmaS)) continue;
+                        const float t0 = vp.tStart, t1 = vp.tStart + vp.delta;
+                        const float td = sampleDistanceExponential(udRemapped, vp.sigmaT, t0, t1);
+                        const float pdfTd = pdfDensity * pdfDistanceExponential(td, vp.sigmaT, t0, t1);
+                        if (pdfTd == 0.0f || !finitef(pdfTd)) continue;
+                        const float trScatter = vp.tr * expf(-vp.sigmaT * (td - t0));
+                        float mis = 1.0f;
+                        if (equiAngular) {
+                            const float pdfTe = pdfEquiAngular(td - offset, D, thetaA, thetaB);
+                            if (finitef(pdfTe)) mis = powerHeuristic(pdfTd, pdfTe);
+                        }
+                        const V3 Ls = volumeSourceTerm(P, ss, L, o + d * td, d, v.g, udl, 1.0f, k, st);
+                        direct = direct + T0 * vp.sigmaS * (Ls * trScatter) * (invN * mis / pdfTd);
+                    }
+                radiance = radiance + direct;
+            }
+        }
+        ++volumeDepth;
+        if (volumeDepth < c.max_volume_depth && c.volume_illumination_samples > 0) {
+            const float invN = 1.0f / float(c.volume_illumination_samples);
+            for (uint32_t i = 0; i < c.volume_illumination_samples; ++i) {
+                Identity mid = root;
+                mid.branch = childBranch(volumeBranch, 0xffff, i);
+                Stream ms(mid, depth);
+                float pdfDensity, utRemapped;
+                const VStep vp = pickStep(v, c, m, ms.next(), pdfDensity, utRemapped);
+                float sigmaT = vp.sigmaT;
+                const float t0 = vp.tStart, t1 = vp.tStart + vp.delta;
+                const float td = sampleDistanceExponential(utRemapped, sigmaT, t0, t1);
+                const float pdfT = pdfDensity * pdfDistanceExponential(td, sigmaT, t0, t1);
+                if (pdfT == 0.0f || !finitef(pdfT)) continue;
+                const float trScatter = vp.tr * expf(-vp.sigmaT * (td - t0));
+                V3 scatterPoint = o + d * td;
+                V3 throughput = T0 * vp.sigmaS * (trScatter / pdfT);
+                const float up0 = ms.next(), up1 = ms.next();
+                V3 scatterDir = phaseSample(v.g, -d, up0, up1);
+                for (uint32_t vd = volumeDepth; vd < c.max_volume_depth; ++vd) {
+                    if (luminance(throughput) < c.volume_transmittance_threshold) break;
+                    const float ai = powf(c.volume_attenuation_factor, float(vd)),
+                                bi = powf(c.volume_contribution_factor, float(vd)),
+                                ci = powf(c.volume_phase_attenuation_factor, float(vd));
+                    const float tFreePath = -logf(1.0f - ms.next()) / sigmaT;
+                    const V3 rayDir = scatterDir;
+                    scatterPoint = scatterPoint + scatterDir * tFreePath;
+                    if (!insideBox(v, scatterPoint)) break;
+                    const VCoeff kk = coefficientsAt(v, scatterPoint, false);
+                    const float g = v.g * ci;
+                    sigmaT = kk.sigmaT;
+                    if (fabsf(sigmaT) < VolEpsilon) break;
+                    throughput = throughput * kk.sigmaS * (1.0f / sigmaT);
+                    for (uint32_t li = 0; li < c.light_count; ++li) {
+                        float usl[3];
+                        usl[0] = ms.next();
+                        usl[1] = ms.next();
+                        usl[2] = ms.next();
+                        const V3 Ls = volumeSourceTerm(P, ms, P.lights[li], scatterPoint, rayDir, g, usl, ai, k, st);
+                        radiance = radiance + throughput * Ls * (invN * bi);
+                    }
+                    const float uq0 = ms.next(), uq1 = ms.next();
+                    scatterDir = phaseSample(g, -rayDir, uq0, uq1);
+                }
+            }
+        }
+    }
+    T = T * vt;
+    return vt;
+}
+
 // One sample; 0 Done (rgba set), 2 Error. With Adj, the sample is replayed for the adjoint: the same estimator, and
 // every contribution's derivative is carried to its sinks, scaled by the pixel's adjoint (gr: the lane's roughness
 // gradient accumulators).
-template <bool Adj>
+template <bool Adj, bool Vol>
 __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, double rgba[4], Counters& k,
                          TravStack& st, float* gr) {
     const x2_config& c = P.cfg;
@@ -787,7 +1265,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
         p.T = {1, 1, 1};
         p.branch = 0;
         p.depths = 0;
-        p.heldLight = ~0u;
+        p.heldLight = Vol ? 0u : ~0u;
         p.heldFar = 0;
         p.held = {0, 0, 0};
     }
@@ -820,13 +1298,20 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
         const Ray ray{pv.origin, pv.dir, fromCamera ? camera.nearT : 0.0f, fromCamera ? camera.farT : MaxFloat};
         Hit hit;
         if (!traceHit(P, ray, hit, k, st)) return 2;
-        if (pv.heldLight != ~0u) {
+        // With a medium there is no bounce-ray reuse (:2919), and heldLight holds the path's volume depth.
+        uint32_t volumeDepth = Vol ? pv.heldLight : 0u;
+        if (!Vol && pv.heldLight != ~0u) {
             const float kept = hit.geom != ~0u && hit.t < pv.heldFar ? 0.0f : 1.0f;
             if (kept > 0.0f) radiance = radiance + (kept < 1.0f ? pv.held * kept : pv.held);
         }
+        float volumeAlpha = 1.0f;
+        V3 T = pv.T;
+        if constexpr (Vol)  // the medium on the ray's way (:2106-2120): its light, and the throughput through it
+            volumeAlpha = volumeRadiance(P, root, T, depth, nonMirrorDepth, volumeDepth, pv.branch, ray.origin,
+                                         ray.direction, ray.nearT, hit.geom == ~0u ? ray.farT : hit.t, radiance, k, st);
         const float hitAlpha = 1.0f - (0.0f + 0.0f + 0.0f) / 3.0f;
         if (hit.geom == ~0u) {
-            if (depth == 0) alpha += 1.0f * (1.0f - (1.0f + 1.0f + 1.0f) / 3.0f);
+            if (depth == 0) alpha += 1.0f * (1.0f - (volumeAlpha + volumeAlpha + volumeAlpha) / 3.0f);
             continue;
         }
         // shade::hitState for a flat triangle: Ng = (b - a) x (c - a), turned to face the ray.
@@ -898,7 +1383,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
             bsdf.emission = ld3(m.emission);
         }
         if (depth == 0) alpha += 1.0f * hitAlpha;
-        radiance = radiance + pv.T * bsdf.emission;
+        radiance = radiance + T * bsdf.emission;
         if (bsdf.count == 0) continue;
         uint32_t indirectFlags = 0;
         if (c.bsdf_samples > 0 && c.max_depth > 0 && depth < c.max_depth) {
@@ -944,7 +1429,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 float pdfL;
                 const V3 Li = lightEval(L, wi, lh, pdfL);
                 if (pdfL == 0.0f || isExactlyZero(Li) || !(fabsf(pdfL) < 3.4e38f) || !finite3(Li)) continue;
-                const V3 factor = pv.T * Li * (1.0f / (float(nl) * pdfL * 1.0f));
+                const V3 factor = T * Li * (1.0f / (float(nl) * pdfL * 1.0f));
                 const uint32_t flags = surfaceFlags(Ng, wo, wi);
                 V3 t{0, 0, 0};
                 V3 tkq[2] = {{0, 0, 0}, {0, 0, 0}};  // each lobe's share, for the adjoint
@@ -973,6 +1458,13 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 const V3 shadowOrigin = dot(wi, Ng) >= 0.0f ? above : below;
                 const float shadowNear = 0.0f, shadowFar = fmaxf((lh.distance - 0.0f) * HitEpsilonEnd, 2.0f * shadowNear);
                 if (visible(P, shadowOrigin, wi, shadowNear, shadowFar, k, st)) {
+                    if constexpr (Vol) {  // the medium on the shadow ray's way (:2725-2732)
+                        Identity tid = lid;
+                        tid.branch = childBranch(lightAddress, s, 0xfffb);
+                        Stream vs(tid, depth);
+                        t = t * transmittance(P.vol, shadowOrigin, wi, shadowNear, shadowFar, 1.0f,
+                                              c.volume_transmittance_threshold, vs);
+                    }
                     radiance = radiance + t;
                     if constexpr (Adj) {
                         for (uint32_t q = 0; q < bsdf.count; ++q) {
@@ -1010,7 +1502,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 const V3 f = lobeSample(bsdf, q, wo, r1, r2, wi, pdf);
                 if (!(pdf != 0.0f && !isExactlyZero(f)) || !finite3(f) || !finite3(wi)) continue;
                 if (!matchesFlags(ltype, surfaceFlags(Ng, wo, wi))) continue;
-                const V3 pt = pv.T * f * (1.0f / (float(nk) * pdf));
+                const V3 pt = T * f * (1.0f / (float(nk) * pdf));
                 uint32_t hits = 0;
                 for (uint32_t j = 0; j < c.light_count; ++j) {
                     LightHit lh;
@@ -1053,7 +1545,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 bool held = false;
                 if (lightHit && !isBlack(direct) && finite3(direct)) {
                     shadowFar = fmaxf((lightDistance - 0.0f) * HitEpsilonEnd, 2.0f * 0.0f);
-                    held = bounces && thinning >= 1.0f && top + PendingReserve < MaxPending && lobeMask &&
+                    held = !Vol && bounces && thinning >= 1.0f && top + PendingReserve < MaxPending && lobeMask &&
                            (c.shadow_reuse & lobeMask) == lobeMask;
                     if (!held) kept = visible(P, origin, wi, 0.0f, shadowFar, k, st) ? 1.0f : 0.0f;
                 }
@@ -1069,6 +1561,13 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 }
                 if (kept > 0.0f && !held) {
                     if (kept < 1.0f) direct = direct * kept;
+                    if constexpr (Vol) {  // (:2939-2945)
+                        Identity tid = bid;
+                        tid.branch = childBranch(sampleBranch, 0xfffb, 0);
+                        Stream vs(tid, depth);
+                        direct = direct * transmittance(P.vol, origin, wi, 0.0f, shadowFar, 1.0f,
+                                                        c.volume_transmittance_threshold, vs);
+                    }
                     radiance = radiance + direct;
                     if constexpr (Adj) scatterSink(P, lobeSink, direct, delta, gr);
                 }
@@ -1097,7 +1596,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
                 child.depths = (depth + 1) | ((nonMirrorDepth + 1) << 8) |
                                ((diffuseDepth + ((ltype & Diffuse) ? 1u : 0u)) << 16) |
                                ((glossyDepth + ((ltype & Glossy) ? 1u : 0u)) << 24);
-                child.heldLight = held ? lightIndex : ~0u;
+                child.heldLight = Vol ? volumeDepth + 1 : held ? lightIndex : ~0u;  // (:3000)
                 child.heldFar = shadowFar;
                 child.held = directBefore;
             }
@@ -1150,7 +1649,7 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
 // One sample per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
 // position; the chunk comes from the mailbox ring. The forward render adds the sample to the film; the adjoint
 // replays it and scatters its gradients.
-template <bool Adj>
+template <bool Adj, bool Vol>
 __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long long item, Counters& k, TravStack& st,
                                            uint32_t& errors, float* gr) {
     const unsigned long long chunkIndex = item / P.itemsPerChunk;
@@ -1160,7 +1659,7 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
     const uint32_t pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
     if (pixel == ~0u) return;
     double rgba[4];
-    if (integrate<Adj>(P, pixel, chunk.first + s, rgba, k, st, gr) == 0) {
+    if (integrate<Adj, Vol>(P, pixel, chunk.first + s, rgba, k, st, gr) == 0) {
         if constexpr (!Adj) {
             double* f = P.sums + size_t(pixel) * 4;
             atomicAdd(f + 0, rgba[0]);
@@ -1177,7 +1676,7 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
 // A lane per item (the 64 lanes of a wave are one 8x8 tile at one sample index, so camera rays start coherent), or
 // persistent lanes claiming items one at a time (the reference's Feed loop) when built with X2_PERSISTENT. Every lane
 // reaches the end so the wave's counter sums see all 64 lanes.
-template <bool Adj>
+template <bool Adj, bool Vol>
 __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
     TravStack st;
@@ -1191,11 +1690,11 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     for (;;) {
         const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
         if (item >= P.itemEnd) break;
-        renderItem<Adj>(P, item, k, st, errors, gr);
+        renderItem<Adj, Vol>(P, item, k, st, errors, gr);
     }
 #else
     const unsigned long long item = P.itemBegin + (unsigned long long)blockIdx.x * 256u + threadIdx.x;
-    if (item < P.itemEnd) renderItem<Adj>(P, item, k, st, errors, gr);
+    if (item < P.itemEnd) renderItem<Adj, Vol>(P, item, k, st, errors, gr);
 #endif
     if (errors) atomicAdd(&P.state->errors, errors);
     if constexpr (Adj) {
@@ -1218,8 +1717,10 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     }
 }
 
-__global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) { runLaunch<false>(P); }
-__global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) { runLaunch<true>(P); }
+__global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) { runLaunch<false, false>(P); }
+__global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) { runLaunch<true, false>(P); }
+// A scene with a medium: the same estimator with the medium's light and transmittance on every ray.
+__global__ void __launch_bounds__(256) renderVolumeKernel(const RenderParams P) { runLaunch<false, true>(P); }
 
 // The film's mean rgb per pixel (sums / counts).
 __global__ void resolveKernel(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb) {
@@ -1241,8 +1742,9 @@ __global__ void clearState(LaunchState* s) {
 
 // Host-side launchers the lane calls (x2_lane.hip).
 namespace x2 {
-template <bool Adj>
+template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium
 hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
+    const auto kernel = Kind == 0 ? renderKernel : Kind == 1 ? adjointKernel : renderVolumeKernel;
     static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
     if (X2_PERSISTENT && !resident) {
         int dev = 0, perCu = 0;
@@ -1250,7 +1752,7 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
         if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, Adj ? adjointKernel : renderKernel, 256, 0);
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0);
         if (e != hipSuccess) return e;
         resident = std::max(1, perCu) * prop.multiProcessorCount;
     }
@@ -1258,11 +1760,13 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
     unsigned long long blocks = (items + 255) / 256;
     if (X2_PERSISTENT && blocks > (unsigned long long)resident) blocks = resident;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(Adj ? adjointKernel : renderKernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
-hipError_t launchRender(const RenderParams& p, hipStream_t stream) { return launchFrame<false>(p, stream); }
-hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<true>(p, stream); }
+hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
+    return p.vol.table ? launchFrame<2>(p, stream) : launchFrame<0>(p, stream);
+}
+hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }
 hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream) {
     hipLaunchKernelGGL(resolveKernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, sums, counts, pixels, rgb);
     return hipGetLastError();

@@ -15,6 +15,7 @@
 
 #include "x2_bvh.h"
 #include "x2_layout.h"
+#include "x2_volume.h"
 
 namespace x2 {
 hipError_t launchRender(const RenderParams& p, hipStream_t stream);
@@ -88,6 +89,9 @@ struct x2dev {
     std::vector<float*> texels;   // one device buffer per texture
     std::vector<TexDesc> texDesc;  // host copy of the descriptors
     TexDesc* textures = nullptr;
+    // the medium's bricks (volDesc.table == null: none)
+    VolumeDesc volDesc{};
+    void* volBuffers[5] = {};  // table, density, temperature, majorant, ramp
     bool prepared = false;
     // film
     double* sums = nullptr;
@@ -127,6 +131,7 @@ void release(x2dev* d) {
     (void)hipFree(d->triUV);
     (void)hipFree(d->textures);
     for (float* t : d->texels) (void)hipFree(t);
+    for (void* b : d->volBuffers) (void)hipFree(b);
     (void)hipFree(d->sums);
     (void)hipFree(d->counts);
     (void)hipFree(d->ring);
@@ -156,6 +161,10 @@ int validateConfig(const x2dev* d, const x2_config& c) {
         return fail(X2_ERR_ARGUMENT, "depth limits above 250 are not supported");
     if (c.filter != 0) return fail(X2_ERR_ARGUMENT, "only the box pixel filter is implemented");
     if (c.light_count > d->lightCount) return fail(X2_ERR_ARGUMENT, "config.light_count exceeds the scene's lights");
+    if (d->volDesc.table && (c.max_volume_depth > 16 || c.volume_illumination_samples > 64 ||
+                             !(c.inv_volume_quality > 0.0f) || !(c.volume_transmittance_threshold >= 0.0f)))
+        return fail(X2_ERR_ARGUMENT, "volume settings budget (cpu_scene.h:3168): depth <= 16, samples <= 64, "
+                                     "a positive quality");
     return X2_OK;
 }
 }  // namespace
@@ -232,6 +241,13 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
             return fail(X2_ERR_ARGUMENT, "a texture without texels");
     for (uint32_t i = 0; i < s->material_count; ++i)
         if (s->materials[i].albedo_map > s->texture_count) return fail(X2_ERR_ARGUMENT, "albedo_map names no texture");
+    if (s->volume_count > 1 || (s->volume_count && !s->volumes))
+        return fail(X2_ERR_ARGUMENT, "at most one volume, and its record");
+    VolumeHost vol;
+    if (s->volume_count) {
+        std::string verr;
+        if (!packVolume(s->volumes[0], vol, verr)) return fail(X2_ERR_ARGUMENT, verr);
+    }
     Bvh8Result bvh;
     std::string err;
     if (!buildBvh8(*s, bvh, err)) return fail(X2_ERR_ARGUMENT, err);
@@ -245,6 +261,11 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     (void)hipFree(d->triUV);
     (void)hipFree(d->textures);
     for (float* t : d->texels) (void)hipFree(t);
+    for (void*& b : d->volBuffers) {
+        (void)hipFree(b);
+        b = nullptr;
+    }
+    d->volDesc = VolumeDesc{};
     d->texels.clear();
     d->texDesc.clear();
     d->triUV = nullptr;
@@ -281,6 +302,27 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     if (!d->texDesc.empty()) {
         X2_HIP(hipMalloc(&d->textures, d->texDesc.size() * sizeof(TexDesc)));
         X2_HIP(hipMemcpy(d->textures, d->texDesc.data(), d->texDesc.size() * sizeof(TexDesc), hipMemcpyHostToDevice));
+    }
+    if (s->volume_count) {
+        VolumeDesc v = vol.desc;
+        auto upload = [&](int slot, const void* src, size_t bytes) -> hipError_t {
+            hipError_t e = hipMalloc(&d->volBuffers[slot], std::max<size_t>(bytes, 4));
+            if (e == hipSuccess && bytes) e = hipMemcpy(d->volBuffers[slot], src, bytes, hipMemcpyHostToDevice);
+            return e;
+        };
+        X2_HIP(upload(0, vol.table.data(), vol.table.size() * sizeof(uint32_t)));
+        X2_HIP(upload(1, vol.density.data(), vol.density.size() * sizeof(uint16_t)));
+        X2_HIP(upload(3, vol.majorant.data(), vol.majorant.size() * sizeof(float)));
+        if (!vol.temperature.empty() || !vol.ramp.empty()) {
+            X2_HIP(upload(2, vol.temperature.data(), vol.temperature.size() * sizeof(uint16_t)));
+            X2_HIP(upload(4, vol.ramp.data(), vol.ramp.size() * sizeof(float)));
+        }
+        v.table = static_cast<const uint32_t*>(d->volBuffers[0]);
+        v.density = static_cast<const uint16_t*>(d->volBuffers[1]);
+        v.temperature = s->volumes[0].temperature ? static_cast<const uint16_t*>(d->volBuffers[2]) : nullptr;
+        v.majorant = static_cast<const float*>(d->volBuffers[3]);
+        v.ramp = static_cast<const float*>(d->volBuffers[4]);
+        d->volDesc = v;
     }
     d->nodeCount = uint32_t(bvh.nodes.size());
     d->triCount = uint32_t(bvh.tris.size());
@@ -380,6 +422,7 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     p.ringCapacity = d->opt.ring_chunks;
     p.counters = d->opt.counters;
     p.pad = 0;
+    p.vol = d->volDesc;
     X2_HIP(hipEventRecord(d->begun[slot], d->stream));
     X2_HIP(d->dImage ? launchAdjoint(p, d->stream) : launchRender(p, d->stream));
     X2_HIP(hipEventRecord(d->ended[slot], d->stream));
@@ -607,6 +650,7 @@ int x2dev_adjoint_frame(x2dev* d, const x2_config* c, uint32_t spc, const float*
     int r = validateConfig(d, *c);
     if (r) return r;
     if (c->max_depth > 32) return fail(X2_ERR_ARGUMENT, "the adjoint supports depth limits up to 32");
+    if (d->volDesc.table) return fail(X2_ERR_ARGUMENT, "the adjoint does not differentiate through a medium");
     X2_HIP(hipSetDevice(d->device));
     d->cfg = *c;
     d->itemsPerChunk = spc * TilePixels;

@@ -56,6 +56,23 @@ struct TexDesc {
 constexpr uint32_t MaxTextures = 16;
 constexpr uint32_t RoughnessSlots = 8;  // materials whose roughness gradient a lane accumulates in registers
 
+// The medium on the device (x2_volume as bricks): the brick table (pool index or ~0u), the FP16 pools (512 halves a
+// brick, z, y, x within), each brick's majorant density, the emission ramp, and the derived sizes (host-computed in
+// float so that every reader rounds alike). table == null: no medium.
+struct VolumeDesc {
+    float bmin[3], bmax[3];
+    uint32_t res[3], nb[3];
+    float scaleVox[3], brickExtent[3];
+    float minFeature, sigmaTScale, g, emissionScale, rampMax;
+    float albedo[3];
+    uint32_t rampSize, pad;
+    const uint32_t* table;
+    const uint16_t* density;
+    const uint16_t* temperature;  // null: no emission
+    const float* majorant;
+    const float* ramp;
+};
+
 // Everything a render launch reads, by value.
 struct RenderParams {
     x2_config cfg;
@@ -77,6 +94,7 @@ struct RenderParams {
     uint32_t itemsPerChunk, ringCapacity;
     uint32_t counters;  // nonzero: count rays, nodes, triangles
     uint32_t pad;
+    VolumeDesc vol;
 };
 
 }  // namespace x2

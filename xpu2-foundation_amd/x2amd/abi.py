@@ -57,6 +57,15 @@ class Texture(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("texels", ctypes.POINTER(f32))]
 
 
+class Volume(ctypes.Structure):
+    _fields_ = [
+        ("bounds_min", f32 * 3), ("bounds_max", f32 * 3), ("res", u32 * 3), ("sigma_t_scale", f32),
+        ("albedo", f32 * 3), ("g", f32), ("emission_scale", f32), ("ramp_max_kelvin", f32), ("ramp_size", u32),
+        ("pad", u32), ("density", ctypes.POINTER(ctypes.c_uint16)), ("temperature", ctypes.POINTER(ctypes.c_uint16)),
+        ("ramp", ctypes.POINTER(f32)),
+    ]
+
+
 class Scene(ctypes.Structure):
     _fields_ = [
         ("positions", ctypes.POINTER(f32)), ("indices", ctypes.POINTER(u32)),
@@ -65,6 +74,7 @@ class Scene(ctypes.Structure):
         ("vertex_count", u32), ("triangle_count", u32), ("geom_count", u32), ("material_count", u32),
         ("light_count", u32), ("pad", u32),
         ("uvs", ctypes.POINTER(f32)), ("textures", ctypes.POINTER(Texture)), ("texture_count", u32), ("pad2", u32),
+        ("volumes", ctypes.POINTER(Volume)), ("volume_count", u32), ("pad3", u32),
     ]
 
 
@@ -77,6 +87,9 @@ class Config(ctypes.Structure):
         ("max_depth", u32), ("max_diffuse_depth", u32), ("max_glossy_depth", u32), ("max_mirror_depth", u32),
         ("clamp_depth", u32), ("clamp_value", f32), ("rr_threshold", f32), ("rr_min_depth", u32),
         ("shadow_reuse", u32), ("viewport", i32 * 4), ("pad", u32 * 2),
+        ("max_volume_depth", u32), ("volume_illumination_samples", u32), ("volume_transmittance_threshold", f32),
+        ("inv_volume_quality", f32), ("volume_attenuation_factor", f32), ("volume_contribution_factor", f32),
+        ("volume_phase_attenuation_factor", f32), ("pad_volume", u32),
     ]
 
 
@@ -104,7 +117,8 @@ class FrameStats(ctypes.Structure):
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
     "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
-    "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_scene": (Scene, 96), "x2_config": (Config, 236),
+    "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
+    "x2_config": (Config, 268),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 16), "x2_frame_stats": (FrameStats, 80),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():

@@ -35,7 +35,7 @@ class SceneData:
     """A flat triangle scene and the ctypes view of it (kept alive with the arrays)."""
 
     def __init__(self, positions, indices, geom_first, geom_material, materials, lights, name, uvs=None,
-                 textures=()):
+                 textures=(), volume=None):
         self.name = name
         self.positions = np.ascontiguousarray(positions, dtype=np.float32)
         self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
@@ -69,6 +69,10 @@ class SceneData:
             self.textures[i].texels = t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         self.c.textures = ctypes.cast(self.textures, ctypes.POINTER(abi.Texture))
         self.c.texture_count = len(self.texture_arrays)
+        self.volume = volume  # a VolumeData or None
+        if volume is not None:
+            self.c.volumes = ctypes.pointer(volume.c)
+            self.c.volume_count = 1
 
     @property
     def triangle_count(self):
@@ -191,6 +195,14 @@ def make_config(width, height, spp, eye, target, up=(0, 1, 0), fov=40.0, light_c
     c.rr_min_depth = rr_min_depth
     c.shadow_reuse = shadow_reuse
     c.viewport[:] = list(FULL_VIEWPORT)
+    # the volume settings' defaults (gpu_jobs.h:19-28)
+    c.max_volume_depth = 1
+    c.volume_illumination_samples = 4
+    c.volume_transmittance_threshold = 0.005
+    c.inv_volume_quality = 2.0
+    c.volume_attenuation_factor = 0.65
+    c.volume_contribution_factor = 0.65
+    c.volume_phase_attenuation_factor = 0.5
     return c
 
 
@@ -437,3 +449,133 @@ def cornell_textured(width=512, height=512, spp=32, max_depth=8, texture_size=51
                          max_depth=max_depth, rr_threshold=0.0)
     params = {"texture": tex, "roughness": rough.astype(np.float32), "roughness_materials": [1, 2, 3, 4]}
     return scene, config, params
+
+
+# ------------------------------------------------------------------------------------------------------------------
+# Participating media.
+class VolumeData:
+    """A box-bounded medium (x2_volume): density and temperature grids as float arrays [z, y, x] (stored as IEEE
+    halves, x fastest), a blackbody ramp for the emission."""
+
+    def __init__(self, bounds_min, bounds_max, density, temperature=None, sigma_t_scale=4.0, albedo=(0.8, 0.8, 0.8),
+                 g=0.6, emission_scale=0.0, ramp=None, ramp_max_kelvin=1500.0):
+        d = np.asarray(density, np.float32)
+        assert d.ndim == 3 and all(n % 8 == 0 for n in d.shape)
+        self.density = np.ascontiguousarray(d.astype(np.float16)).view(np.uint16)
+        self.temperature = None if temperature is None else \
+            np.ascontiguousarray(np.asarray(temperature, np.float32).astype(np.float16)).view(np.uint16)
+        self.ramp = None if ramp is None else np.ascontiguousarray(ramp, np.float32)
+        c = self.c = abi.Volume()
+        c.bounds_min[:] = [float(x) for x in bounds_min]
+        c.bounds_max[:] = [float(x) for x in bounds_max]
+        c.res[:] = [d.shape[2], d.shape[1], d.shape[0]]
+        c.sigma_t_scale = sigma_t_scale
+        c.albedo[:] = [float(x) for x in albedo]
+        c.g = g
+        c.emission_scale = emission_scale
+        c.ramp_max_kelvin = ramp_max_kelvin
+        u16p = ctypes.POINTER(ctypes.c_uint16)
+        c.density = self.density.ctypes.data_as(u16p)
+        if self.temperature is not None:
+            c.temperature = self.temperature.ctypes.data_as(u16p)
+            c.ramp = self.ramp.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            c.ramp_size = self.ramp.shape[0]
+
+    def occupancy(self):
+        """The fraction of 8^3 bricks holding a nonzero density."""
+        d = self.density.reshape(self.density.shape[0] // 8, 8, self.density.shape[1] // 8, 8,
+                                 self.density.shape[2] // 8, 8)
+        return float(((d & 0x7FFF) != 0).any(axis=(1, 3, 5)).mean())
+
+
+def _cmf_lobe(lam, mu, s1, s2):
+    s = np.where(lam < mu, s1, s2)
+    return np.exp(-0.5 * ((lam - mu) / s) ** 2)
+
+
+def blackbody_ramp(max_kelvin=1500.0, size=256):
+    """Linear sRGB of Planck's law at size temperatures over [0, max_kelvin]: the spectrum (360-830 nm, 1 nm) against
+    the CIE 1931 observer in Wyman, Sloan and Shirley's multi-lobe fit (JCGT 2013), XYZ to linear sRGB (IEC
+    61966-2-1), negatives clipped, scaled so that the luminance at max_kelvin is 1."""
+    lam = np.arange(360.0, 831.0)
+    xb = 1.056 * _cmf_lobe(lam, 599.8, 37.9, 31.0) + 0.362 * _cmf_lobe(lam, 442.0, 16.0, 26.7) \
+        - 0.065 * _cmf_lobe(lam, 501.1, 20.4, 26.2)
+    yb = 0.821 * _cmf_lobe(lam, 568.8, 46.9, 40.5) + 0.286 * _cmf_lobe(lam, 530.9, 16.3, 31.1)
+    zb = 1.217 * _cmf_lobe(lam, 437.0, 11.8, 36.0) + 0.681 * _cmf_lobe(lam, 459.0, 26.0, 13.8)
+    h, c, k = 6.62607015e-34, 2.99792458e8, 1.380649e-23
+    m = lam * 1e-9
+    temps = np.linspace(0.0, max_kelvin, size)
+    out = np.zeros((size, 3))
+    to_rgb = np.array([[3.2406, -1.5372, -0.4986], [-0.9689, 1.8758, 0.0415], [0.0557, -0.2040, 1.0570]])
+    for i, t in enumerate(temps):
+        if t <= 0:
+            continue
+        with np.errstate(over="ignore"):
+            b = 2 * h * c * c / m ** 5 / np.expm1(np.minimum(h * c / (m * k * t), 700.0))
+        xyz = np.array([(b * xb).sum(), (b * yb).sum(), (b * zb).sum()])
+        out[i] = np.maximum(to_rgb @ xyz, 0.0)
+    top = out[-1] @ np.array([0.299, 0.587, 0.114])
+    return (out / top).astype(np.float32)
+
+
+def _value_noise(res, cells, rng):
+    """Seeded value noise on a res^3 grid of voxel centres over `cells` lattice cells, smoothstep-interpolated."""
+    lat = rng.random((cells + 1, cells + 1, cells + 1)).astype(np.float32)
+    x = (np.arange(res, dtype=np.float32) + 0.5) / res * cells
+    i0 = np.minimum(np.floor(x).astype(np.int64), cells - 1)
+    f = x - i0
+    w = f * f * (3 - 2 * f)
+    out = np.empty((res, res, res), np.float32)
+    for z in range(res):  # one z slice at a time keeps the temporaries small
+        kz, wz = i0[z], w[z]
+        a = lat[kz][np.ix_(i0, i0)] * (1 - wz) + lat[kz + 1][np.ix_(i0, i0)] * wz
+        b = lat[kz][np.ix_(i0, i0 + 1)] * (1 - wz) + lat[kz + 1][np.ix_(i0, i0 + 1)] * wz
+        c = lat[kz][np.ix_(i0 + 1, i0)] * (1 - wz) + lat[kz + 1][np.ix_(i0 + 1, i0)] * wz
+        d = lat[kz][np.ix_(i0 + 1, i0 + 1)] * (1 - wz) + lat[kz + 1][np.ix_(i0 + 1, i0 + 1)] * wz
+        wy, wx = w[:, None], w[None, :]
+        out[z] = (a * (1 - wx) + b * wx) * (1 - wy) + (c * (1 - wx) + d * wx) * wy
+    return out
+
+
+def fbm_grid(res, seed, octaves=4, base_cells=4):
+    """Fractional Brownian motion of value noise in [0, 1] (gain 0.5, lacunarity 2)."""
+    rng = np.random.default_rng(seed)
+    total = np.zeros((res, res, res), np.float32)
+    norm = 0.0
+    for o in range(octaves):
+        amp = 0.5 ** o
+        total += amp * _value_noise(res, base_cells * 2 ** o, rng)
+        norm += amp
+    return total / norm
+
+
+def cloud_volume(res=256, seed=17, threshold=0.4, bounds_min=(-2.0, 0.2, -2.0), bounds_max=(2.0, 4.2, 2.0)):
+    """The participating medium: fBm density thresholded at 0.4 under a spherical falloff (about 30% of the 8^3 bricks
+    occupied at 256^3), extinction scale 4, albedo 0.8, HG g = 0.6, and a temperature field of 0-1500 K (a hot core
+    with fBm detail) driving blackbody emission."""
+    n = fbm_grid(res, seed)
+    x = (np.arange(res, dtype=np.float32) + 0.5) / res * 2 - 1
+    r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+    shaped = n * np.clip(1.5 - r, 0.0, 1.0) + 0.3 * np.clip(1.0 - r, 0.0, 1.0)
+    density = np.clip((shaped - threshold) / (1 - threshold), 0.0, 1.0).astype(np.float32)
+    hot = fbm_grid(res // 2, seed + 1, octaves=3, base_cells=3).repeat(2, 0).repeat(2, 1).repeat(2, 2)
+    temperature = (1500.0 * np.clip(1.3 * (1.0 - r / 0.8) + 0.6 * (hot - 0.5), 0.0, 1.0)).astype(np.float32)
+    return VolumeData(bounds_min, bounds_max, density, temperature, sigma_t_scale=4.0, albedo=(0.8, 0.8, 0.8),
+                      g=0.6, emission_scale=2.0, ramp=blackbody_ramp(1500.0), ramp_max_kelvin=1500.0)
+
+
+def clouds(width=1280, height=720, spp=32, res=256, max_depth=16, max_volume_depth=2, seed=17, volume=None):
+    """Participating media: the cloud over a grey ground quad, one rectangle light above it and a constant
+    environment of 0.2; 1280x720, 32 spp, depth 16. Returns (scene, config)."""
+    vol = volume if volume is not None else cloud_volume(res, seed)
+    s = 50.0
+    pos = np.array([[-s, 0, -s], [s, 0, -s], [s, 0, s], [-s, 0, s]], np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2]])
+    mats = [material((0.5, 0.5, 0.5))]
+    lights = [rect_light((0.0, 8.0, 1.0), (1.5, 0.0, 0.0), (0.0, 0.0, 1.5), (12.0, 11.0, 10.0)),
+              env_light((0.2, 0.2, 0.2))]
+    scene = SceneData(pos, idx, [0, 2], [0], mats, lights, "clouds", volume=vol)
+    config = make_config(width, height, spp, eye=(0.0, 2.6, 9.5), target=(0.0, 2.1, 0.0), light_count=2,
+                         max_depth=max_depth)
+    config.max_volume_depth = max_volume_depth
+    return scene, config
