@@ -9,7 +9,8 @@ Prints one JSON line: Mpaths/s (value), Mrays/s, the roofline of the render kern
 BVH8 node fetched + 48 B per triangle record tested + 72 B of film per path, over the kernel's own HIP-event time)
 and the CPU baseline (the oracle on this host's cores over a bounded sample of the same workload). The adjoint
 leg times one forward frame and one path-replay adjoint frame of the differentiable Cornell box (512x512, 32 spp):
-adjoint_step_over_forward is the adjoint frame's time over the forward frame's.
+adjoint_step_over_forward is the adjoint frame's time over the forward frame's. The volume leg renders the
+participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp).
 """
 import argparse
 import json
@@ -111,6 +112,47 @@ def adjoint_leg(reps):
     }
 
 
+def volume_leg(reps, cpu):
+    """Participating media: the cloud (256^3 brick-sparse FP16 density and temperature, about 31% of the 8^3 bricks
+    occupied, extinction scale 4, HG g = 0.6, blackbody emission to 1500 K) over a ground quad under a rectangle light
+    and a constant environment, 1280x720 at 32 spp, depth 16, volume depth 2; `reps` frames after a warm-up."""
+    from x2amd import scenes
+    from x2amd.lane import GpuLane
+
+    scene, config = scenes.clouds()
+    lane = GpuLane(0)
+    lane.prepare(scene)
+    lane.render_frame(config)
+    wall, kernel, rays = 0.0, 0.0, 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, counts, st = lane.render_frame(config)
+        wall += time.perf_counter() - t0
+        assert st.audit_failures == 0 and st.errors == 0
+        kernel += st.device_seconds
+        rays += st.rays_trace + st.rays_shadow
+    lane.close()
+    paths = config.width * config.height * config.spp * reps
+    out = {
+        "workload": f"clouds {scene.volume.c.res[0]}^3 grid ({scene.volume.occupancy():.3f} of bricks occupied), "
+                    f"{config.width}x{config.height}, {config.spp} spp, depth {config.max_depth}, volume depth "
+                    f"{config.max_volume_depth}, rect + env lights",
+        "reps": reps, "ms_per_frame": wall / reps * 1e3, "kernel_ms_per_frame": kernel / reps * 1e3,
+        "mpaths_per_s": paths / wall / 1e6, "mrays_per_s": rays / wall / 1e6,
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        cores = len(os.sched_getaffinity(0))
+        _, small = scenes.clouds(width=640, height=360, spp=4, volume=scene.volume)
+        t = time.perf_counter()
+        oracle.OracleScene(scene).render(small, threads=cores)
+        dt = time.perf_counter() - t
+        out["cpu"] = {"mpaths_per_s": 640 * 360 * 4 / dt / 1e6, "cores": cores, "kind": "port",
+                      "sample": f"640x360 at 4 spp of the same scene, {dt:.2f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-reps", type=int, default=5)
+    ap.add_argument("--no-volume", action="store_true")
+    ap.add_argument("--volume-reps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (the film reduce "
@@ -184,6 +228,8 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adjoint_reps)
         out["adjoint_step_over_forward"] = out["adjoint"]["adjoint_over_forward"]
+    if not args.no_volume:
+        out["volume"] = volume_leg(args.volume_reps, not args.no_cpu)
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)
         out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp, 20)
