@@ -10,7 +10,8 @@ BVH8 node fetched + 48 B per triangle record tested + 72 B of film per path, ove
 and the CPU baseline (the oracle on this host's cores over a bounded sample of the same workload). The adjoint
 leg times one forward frame and one path-replay adjoint frame of the differentiable Cornell box (512x512, 32 spp):
 adjoint_step_over_forward is the adjoint frame's time over the forward frame's. The volume leg renders the
-participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp).
+participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp); the batch leg the 64-job batched
+workload, persistent kernel from a hipGraph against one launch per job.
 """
 import argparse
 import json
@@ -153,6 +154,47 @@ def volume_leg(reps, cpu):
     return out
 
 
+def batch_leg(reps):
+    """Batched renders: 64 seeded icosphere jobs of log-uniform 10k-500k triangles, square images of log-uniform
+    side 64-1024 and spp 1-64, rendered by one persistent kernel from a device queue replayed from a hipGraph
+    (its shape chosen by the library's autotuner), against one launch per job in the same order."""
+    import numpy as np
+    from x2amd import scenes
+    from x2amd.batch import Batch
+
+    jobs = scenes.batch_jobs()
+    b = Batch(0)
+    t = time.perf_counter()
+    for scene, config in jobs:
+        b.add(scene, config)
+    add_s = time.perf_counter() - t
+    t = time.perf_counter()
+    tuning = b.tune()
+    tune_s = time.perf_counter() - t
+    b.run()  # captures the graph
+    paths = sum(c.width * c.height * c.spp for _, c in jobs)
+    out = {"workload": f"{len(jobs)} jobs, {sum(s.triangle_count for s, _ in jobs)} triangles, "
+                       f"{sum(c.width * c.height for _, c in jobs)} pixels, {paths} paths",
+           "add_s": add_s, "tune_s": tune_s,
+           "tuning": {"block_threads": tuning.block_threads, "claim_items": tuning.claim_items,
+                      "waves_per_simd": tuning.waves_per_simd, "one_sample_pass_ms": tuning.seconds * 1e3}}
+    for label, per_job in (("persistent_graph", False), ("per_job_launches", True)):
+        wall = dev = 0.0
+        secs = None
+        for _ in range(reps):
+            st, secs = b.run(per_job_launches=per_job)
+            assert st.errors == 0
+            wall += st.wall_seconds
+            dev += st.device_seconds
+        out[label] = {"mpaths_per_s": paths * reps / wall / 1e6, "device_mpaths_per_s": paths * reps / dev / 1e6,
+                      "ms_per_batch": wall / reps * 1e3, "launches": int(st.launches),
+                      "job_p50_ms": float(np.percentile(secs, 50) * 1e3),
+                      "job_p99_ms": float(np.percentile(secs, 99) * 1e3)}
+    b.close()
+    out["speedup_vs_per_job"] = out["persistent_graph"]["mpaths_per_s"] / out["per_job_launches"]["mpaths_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +209,8 @@ def main():
     ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--no-volume", action="store_true")
     ap.add_argument("--volume-reps", type=int, default=2)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (the film reduce "
@@ -230,6 +274,8 @@ def main():
         out["adjoint_step_over_forward"] = out["adjoint"]["adjoint_over_forward"]
     if not args.no_volume:
         out["volume"] = volume_leg(args.volume_reps, not args.no_cpu)
+    if not args.no_batch:
+        out["batch"] = batch_leg(args.batch_reps)
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)
         out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp, 20)

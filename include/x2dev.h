@@ -245,6 +245,41 @@ int x2dev_adjoint_frame(x2dev* d, const x2_config* config, uint32_t samples_per_
  * audited against spp (async_render.h:1035). stats (may be null) receives totals over the frame's launches. */
 int x2dev_render_frame(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, x2_frame_stats* stats);
 
+/* ---- Batched jobs. No counterpart in the reference, which renders one frame at a time and decided against CUDA
+ * graphs (ASYNC.md:176-177): many independent render jobs, each its own scene, settings and film, run in one
+ * persistent kernel that claims work from a device queue (the jobs' items end to end, shortest job first by
+ * paths x log2(triangles)); the whole batch (film clears, a start stamp, the kernel) is captured once in a hipGraph
+ * and replayed. The kernel's shape (workgroup threads, items a wave claims, the occupancy target) is chosen by
+ * x2batch_tune, which times each candidate on a one-sample pass over every job. */
+typedef struct x2batch x2batch;
+typedef struct x2_batch_tuning {
+    uint32_t block_threads;   /* 64, 128 or 256 */
+    uint32_t claim_items;     /* items a wave claims at once: a multiple of 64 */
+    uint32_t waves_per_simd;  /* occupancy target the kernel was built for: 2 or 3 */
+    uint32_t pad;
+    double seconds;           /* the tuning pass's device time with this shape (x2batch_tune) */
+} x2_batch_tuning;
+typedef struct x2_batch_stats {
+    double wall_seconds, device_seconds;  /* host clock around the run; HIP events around its device work */
+    uint64_t paths, rays_trace, rays_shadow;
+    uint32_t jobs, errors, launches, graph; /* launches: kernel launches issued; graph: 1 when replayed from one */
+    x2_batch_tuning tuning;
+} x2_batch_stats;
+int x2batch_create(int device, x2batch** out);
+/* Builds the job's BVH8, uploads its scene and allocates its film; *job receives its index (submission order). */
+int x2batch_add_job(x2batch* b, const x2_scene* scene, const x2_config* config, uint32_t* job);
+/* Times the candidate shapes; the fastest becomes the batch's shape and is written to *best (may be null). */
+int x2batch_tune(x2batch* b, x2_batch_tuning* best);
+int x2batch_set_tuning(x2batch* b, const x2_batch_tuning* tuning);
+/* Renders every job once (every pixel, every sample index). per_job_launches = 0: the persistent kernel over the
+ * whole queue, from the captured graph; 1: one launch per job, in the same order (the comparison). job_seconds
+ * (job_count doubles, may be null): each job's completion time from the run's start, by the device clock. */
+int x2batch_run(x2batch* b, int per_job_launches, double* job_seconds, x2_batch_stats* stats);
+/* The job's film (downloaded into host memory the batch owns, valid until the next run). */
+int x2batch_job_film(x2batch* b, uint32_t job, const double** sums, const uint32_t** counts);
+uint32_t x2batch_job_count(const x2batch* b);
+void x2batch_destroy(x2batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1722,6 +1722,89 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderPa
 // A scene with a medium: the same estimator with the medium's light and transmittance on every ray.
 __global__ void __launch_bounds__(256) renderVolumeKernel(const RenderParams P) { runLaunch<false, true>(P); }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Batched jobs: many independent frames (each job its own scene, settings and film, a RenderParams in device
+// memory) in one persistent launch. The device queue is the jobs' items laid end to end in queue order (first[p]
+// = the items before position p); each wave claims `claim` items (a multiple of 64) with one atomic, finds each
+// 64-item group's job by binary search, and renders it as the frame kernel renders an item: a group is one 8x8 tile
+// at one sample index of one job (item = (sample * tiles + tile) * 64 + position). A job's last group stamps the
+// wall clock, the job's completion time.
+template <int Waves>
+__device__ __forceinline__ void runBatch(const BatchParams& B) {
+    TravStack st;
+    st.lds = nullptr;
+    st.lane = threadIdx.x;
+    Counters k;
+    uint32_t errors = 0;
+    float gr[1] = {0.0f};
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long begin = B.first[B.posBegin], end = B.first[B.posEnd];
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(B.head, (unsigned long long)B.claim);
+        base = __shfl(base, 0, 64) + begin;
+        if (base >= end) break;
+        for (uint32_t g = 0; g < B.claim; g += 64) {
+            const unsigned long long group = base + g;
+            if (group >= end) break;
+            uint32_t lo = B.posBegin, hi = B.posEnd - 1;  // the last position whose first item is <= group
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) / 2;
+                if (B.first[mid] <= group) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint32_t job = B.order[lo];
+            const RenderParams& P = B.jobs[job];
+            const unsigned long long local = group - B.first[lo];
+            const uint32_t tiles = ((P.cfg.width + 7) / 8) * ((P.cfg.height + 7) / 8);
+            const uint32_t sample = uint32_t(local / (64ull * tiles)), tile = uint32_t((local / 64) % tiles);
+            const uint32_t pixel = tilePixel(tile, lane, P.cfg);
+            if (pixel != ~0u) {
+                double rgba[4];
+                if (integrate<false, false>(P, pixel, sample, rgba, k, st, gr) == 0) {
+                    double* f = P.sums + size_t(pixel) * 4;
+                    atomicAdd(f + 0, rgba[0]);
+                    atomicAdd(f + 1, rgba[1]);
+                    atomicAdd(f + 2, rgba[2]);
+                    atomicAdd(f + 3, rgba[3]);
+                    atomicAdd(P.counts + pixel, 1u);
+                } else {
+                    ++errors;
+                }
+            }
+            if (lane == 0) {
+                const unsigned long long items = B.first[lo + 1] - B.first[lo];
+                if (atomicAdd(B.jobDone + job, 64ull) + 64 == items) B.jobFinish[job] = wall_clock64();
+            }
+        }
+    }
+    if (errors) atomicAdd(&B.state->errors, errors);
+    const unsigned long long t = waveSum(k.traces), sh = waveSum(k.shadows);
+    if (lane == 0) {
+        atomicAdd(&B.state->traces, t);
+        atomicAdd(&B.state->shadows, sh);
+    }
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) batchKernel2(const BatchParams B) {
+    runBatch<2>(B);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) batchKernel3(const BatchParams B) {
+    runBatch<3>(B);
+}
+// Zeroes a list of regions (films, counters) with plain stores, so a captured graph can replay it.
+__global__ void fillKernel(const FillRegion* regions, uint32_t n) {
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t* w = regions[r].words;
+        const unsigned long long count = regions[r].count;
+        for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+            w[i] = 0u;
+    }
+}
+__global__ void stampKernel(unsigned long long* at) {
+    if (threadIdx.x == 0) *at = wall_clock64();
+}
+
 // The film's mean rgb per pixel (sums / counts).
 __global__ void resolveKernel(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -1773,6 +1856,32 @@ hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pi
 }
 hipError_t launchClearState(LaunchState* s, hipStream_t stream) {
     hipLaunchKernelGGL(clearState, dim3(1), dim3(64), 0, stream, s);
+    return hipGetLastError();
+}
+// The batched queue's launchers (x2_batch.hip): the persistent grid is what the chip holds of the chosen kernel at
+// the chosen workgroup size.
+hipError_t batchResident(int waves, uint32_t threads, int& blocks) {
+    int dev = 0, perCu = 0;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, waves == 2 ? batchKernel2 : batchKernel3, int(threads), 0);
+    if (e != hipSuccess) return e;
+    blocks = std::max(1, perCu) * prop.multiProcessorCount;
+    return hipSuccess;
+}
+hipError_t launchBatch(const BatchParams& b, int waves, uint32_t threads, int blocks, hipStream_t stream) {
+    if (waves == 2) hipLaunchKernelGGL(batchKernel2, dim3(unsigned(blocks)), dim3(threads), 0, stream, b);
+    else hipLaunchKernelGGL(batchKernel3, dim3(unsigned(blocks)), dim3(threads), 0, stream, b);
+    return hipGetLastError();
+}
+hipError_t launchFill(const FillRegion* regions, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(fillKernel, dim3(1024), dim3(256), 0, stream, regions, n);
+    return hipGetLastError();
+}
+hipError_t launchStamp(unsigned long long* at, hipStream_t stream) {
+    hipLaunchKernelGGL(stampKernel, dim3(1), dim3(64), 0, stream, at);
     return hipGetLastError();
 }
 hipError_t uploadSobol(const uint32_t* table) {

@@ -74,6 +74,11 @@ struct Flight {
 };
 }  // namespace
 
+namespace x2 {  // for the other host files (x2_batch.hip)
+void sobolTable(uint32_t out[17 * 32]) { sobolDirections(out); }
+int reportFailure(int code, const std::string& what) { return fail(code, what); }
+}  // namespace x2
+
 struct x2dev {
     int device = 0;
     x2dev_options opt{};

@@ -97,4 +97,24 @@ struct RenderParams {
     VolumeDesc vol;
 };
 
+// The batched queue's launch record and totals (x2_kernel.hip runBatch, x2_batch.hip).
+struct BatchState {
+    unsigned long long traces, shadows;
+    uint32_t errors, pad;
+};
+struct BatchParams {
+    const RenderParams* jobs;
+    const uint32_t* order;             // queue position -> job
+    const unsigned long long* first;   // [positions + 1] item prefix over queue positions
+    unsigned long long* head;          // the queue's claim counter
+    unsigned long long* jobDone;       // per job: items finished
+    unsigned long long* jobFinish;     // per job: wall clock when its last group finished
+    BatchState* state;
+    uint32_t posBegin, posEnd, claim, pad;
+};
+// A region the batch clears before a run: `count` 32-bit words at `words`.
+struct FillRegion {
+    uint32_t* words;
+    unsigned long long count;
+};
 }  // namespace x2

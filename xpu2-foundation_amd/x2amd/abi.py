@@ -113,6 +113,18 @@ class FrameStats(ctypes.Structure):
     ]
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("block_threads", u32), ("claim_items", u32), ("waves_per_simd", u32), ("pad", u32),
+                ("seconds", f64)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("wall_seconds", f64), ("device_seconds", f64), ("paths", u64), ("rays_trace", u64), ("rays_shadow", u64),
+        ("jobs", u32), ("errors", u32), ("launches", u32), ("graph", u32), ("tuning", BatchTuning),
+    ]
+
+
 # Sizes of the C structures (include/x2dev.h); checked against ctypes here and against the C compiler by a
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
@@ -120,6 +132,7 @@ EXPECTED_SIZES = {
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 268),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 16), "x2_frame_stats": (FrameStats, 80),
+    "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
     assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
@@ -151,6 +164,16 @@ SIGNATURES = {
     "x2dev_resolve_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "x2dev_adjoint_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(FrameStats)]),
+    "x2batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "x2batch_add_job": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Scene), ctypes.POINTER(Config),
+                                       ctypes.POINTER(u32)]),
+    "x2batch_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchTuning)]),
+    "x2batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchTuning)]),
+    "x2batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(f64), ctypes.POINTER(BatchStats)]),
+    "x2batch_job_film": (ctypes.c_int, [ctypes.c_void_p, u32, ctypes.POINTER(ctypes.POINTER(f64)),
+                                        ctypes.POINTER(ctypes.POINTER(u32))]),
+    "x2batch_job_count": (u32, [ctypes.c_void_p]),
+    "x2batch_destroy": (None, [ctypes.c_void_p]),
 }
 
 _lib = None

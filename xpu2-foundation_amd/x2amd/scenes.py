@@ -579,3 +579,25 @@ def clouds(width=1280, height=720, spp=32, res=256, max_depth=16, max_volume_dep
                          max_depth=max_depth)
     config.max_volume_depth = max_volume_depth
     return scene, config
+
+
+# ------------------------------------------------------------------------------------------------------------------
+# Batched jobs.
+def batch_jobs(count=64, seed=3, min_tris=10_000, max_tris=500_000, min_side=64, max_side=1024, max_spp=64):
+    """`count` seeded icosphere scenes in the headline's style, each of about log-uniform [min_tris, max_tris]
+    triangles (n fBm-displaced icospheres of the largest level that gives at least two), a square image of log-uniform
+    side in [min_side, max_side] (a multiple of 8) and spp uniform in 1..max_spp. Returns [(scene, config)]."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(count):
+        target = math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))
+        for level in (6, 5, 4, 3):
+            per = 20 * 4 ** level
+            if target / per >= 2 or level == 3:
+                n = int(min(64, max(1, round(target / per))))
+                break
+        side = int(max(8, round(math.exp(rng.uniform(math.log(min_side), math.log(max_side))) / 8) * 8))
+        spp = int(rng.integers(1, max_spp + 1))
+        scene, config = icospheres(width=side, height=side, spp=spp, count=n, level=level, seed=seed * 1000 + i)
+        jobs.append((scene, config))
+    return jobs
