@@ -195,6 +195,34 @@ def batch_leg(reps):
     return out
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB (the runtime's blit kernel), run under rocprofv3 --pmc to scale
+    FETCH_SIZE and WRITE_SIZE: the counters' reading for a copy of known size."""
+    import numpy as np
+    from x2amd.hipmem import DeviceBuffer
+
+    n = mib << 20
+    a, b = DeviceBuffer(n), DeviceBuffer(n)
+    a.upload(np.ones(n // 4, np.float32))
+    if a.lib.hipMemcpy(b.ptr, a.ptr, n, 3) != 0:  # device to device
+        raise RuntimeError("hipMemcpy device to device failed")
+    if a.lib.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+    a.free()
+    b.free()
+    print(json.dumps({"copy_mib": mib}))
+
+
+def traffic_per_path():
+    """HBM bytes per path of the render kernel at the headline workload, from the committed rocprofv3 --pmc
+    measurement (FETCH_SIZE + WRITE_SIZE, calibrated by a copy), or None."""
+    path = os.path.join(ROOT, "profiles", "renderKernel_icospheres_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,8 +238,12 @@ def main():
     ap.add_argument("--no-volume", action="store_true")
     ap.add_argument("--volume-reps", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, help="MiB: only run a device copy (for --pmc scaling)")
     ap.add_argument("--batch-reps", type=int, default=3)
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (the film reduce "
                                    "across GPUs is a design note, not built)"}))
@@ -245,6 +277,10 @@ def main():
     rays = totals["traces"] + totals["shadows"]
     bytes_total = 256 * totals["nodes"] + 48 * totals["tris"] + 72 * paths
     per_launch_s = totals["kernel_s"] / max(1, totals["launches"])
+    measured = traffic_per_path()
+    traffic = None  # HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    if measured:
+        traffic = measured["hbm_bytes_per_path"] * paths / max(1, totals["launches"])
     achieved = bytes_total / max(totals["kernel_s"], 1e-12) / 1e9
     out = {
         "metric": "Mpaths/s + Mrays/s at 1080p 64spp (fraction of HBM roofline); adjoint step/forward",
@@ -265,7 +301,7 @@ def main():
         "rays_per_path": rays / paths,
         "prepare_s": prepare_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "renderKernel", "avg_launch_ms": per_launch_s * 1e3,
                      "launches": totals["launches"], "algorithmic_bytes": bytes_total},
     }
