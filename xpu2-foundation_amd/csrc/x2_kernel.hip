@@ -259,6 +259,26 @@ __device__ __forceinline__ bool keyLess(float t, uint32_t g, uint32_t p, float b
     return t < bt || (t == bt && (g < bg || (g == bg && p < bp)));
 }
 
+// Traversal order is "speculative while-while" (Aila and Laine 2009): a lane descends inner nodes, postponing the
+// first leaf it reaches, until every lane of the wave still descending holds a leaf; then the wave tests leaves
+// together. Node and leaf work stay apart, so a wave is not split between the two on every step. The closest hit
+// is the least key whatever the order, so the result is the same as a depth-first walk's.
+constexpr uint32_t Sentinel = 0x7fffffffu;  // an empty stack's pop: neither a node index nor a leaf
+
+#ifndef X2_WHILE_WHILE
+#define X2_WHILE_WHILE 1
+#endif
+
+__device__ __forceinline__ void sort8(unsigned long long key[8]) {  // 19-comparator network: nearest first
+    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[4], key[5]); cswap(key[6], key[7]);
+    cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[4], key[6]); cswap(key[5], key[7]);
+    cswap(key[1], key[2]); cswap(key[5], key[6]); cswap(key[0], key[4]); cswap(key[3], key[7]);
+    cswap(key[1], key[5]); cswap(key[2], key[6]);
+    cswap(key[1], key[4]); cswap(key[3], key[6]);
+    cswap(key[2], key[4]); cswap(key[3], key[5]);
+    cswap(key[3], key[4]);
+}
+
 // Closest hit among triangles with t in [near, far]; returns the triangle record index (~0u: none).
 __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, float& tOut,
                                Counters& k, TravStack& st) {
@@ -268,94 +288,145 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
     uint32_t best = ~0u, bg = ~0u, bp = ~0u;
     float bestT = farT;
     int sp = 0;
-    uint32_t ref = 0;
-    for (;;) {
-        if (ref & LeafBit) {
-            const uint32_t first = (ref & 0x7fffffffu) >> 3, count = (ref & 7u) + 1;
-            for (uint32_t i = 0; i < count; ++i) {
-                const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
-                const float4 a = t4[0], b = t4[1], c = t4[2];
-                ++k.triangles;
-                float t;
-                if (!woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) continue;
-                const uint32_t g = __float_as_uint(a.w), p = __float_as_uint(b.w);
-                if (best == ~0u || keyLess(t, g, p, bestT, bg, bp)) {
-                    best = first + i;
-                    bestT = t;
-                    bg = g;
-                    bp = p;
-                }
-            }
-        } else {
-            ++k.nodes;
-            const float hi = best == ~0u ? (farT > 0 ? farT * Slack : farT / Slack) : (bestT > 0 ? bestT * Slack : bestT / Slack);
-            unsigned long long key[8];
-            nodeChildren(P.nodes[ref], br, lo, hi, key);
-            // Sort the 8 keys (19-comparator network): nearest first.
-            cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[4], key[5]); cswap(key[6], key[7]);
-            cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[4], key[6]); cswap(key[5], key[7]);
-            cswap(key[1], key[2]); cswap(key[5], key[6]); cswap(key[0], key[4]); cswap(key[3], key[7]);
-            cswap(key[1], key[5]); cswap(key[2], key[6]);
-            cswap(key[1], key[4]); cswap(key[3], key[6]);
-            cswap(key[2], key[4]); cswap(key[3], key[5]);
-            cswap(key[3], key[4]);
-#pragma unroll
-            for (int i = 7; i >= 1; --i)
-                if (key[i] != ~0ull) {
-                    st.put(sp, make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32)));
-                    ++sp;
-                }
-            if (key[0] != ~0ull) {
-                ref = uint32_t(key[0]);
-                continue;
-            }
-        }
-        // Pop the nearest entry still worth visiting.
-        bool found = false;
+    // The nearest stacked entry still worth visiting, or Sentinel.
+    auto pop = [&]() -> uint32_t {
         while (sp > 0) {
             --sp;
             const float bound = best == ~0u ? farT : bestT;
             const uint2 e = st.get(sp);
-            if (__uint_as_float(e.y) <= (bound > 0 ? bound * Slack : bound / Slack)) {
-                ref = e.x;
-                found = true;
-                break;
+            if (__uint_as_float(e.y) <= (bound > 0 ? bound * Slack : bound / Slack)) return e.x;
+        }
+        return Sentinel;
+    };
+    auto visitNode = [&](uint32_t node) -> uint32_t {  // pushes the far children, returns the nearest (or a pop)
+        ++k.nodes;
+        const float hi = best == ~0u ? (farT > 0 ? farT * Slack : farT / Slack) : (bestT > 0 ? bestT * Slack : bestT / Slack);
+        unsigned long long key[8];
+        nodeChildren(P.nodes[node], br, lo, hi, key);
+        sort8(key);
+#pragma unroll
+        for (int i = 7; i >= 1; --i)
+            if (key[i] != ~0ull) {
+                st.put(sp, make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32)));
+                ++sp;
+            }
+        return key[0] != ~0ull ? uint32_t(key[0]) : pop();
+    };
+    auto visitLeaf = [&](uint32_t leafRef) {
+        const uint32_t first = (leafRef & 0x7fffffffu) >> 3, count = (leafRef & 7u) + 1;
+        for (uint32_t i = 0; i < count; ++i) {
+            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
+            const float4 a = t4[0], b = t4[1], c = t4[2];
+            ++k.triangles;
+            float t;
+            if (!woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) continue;
+            const uint32_t g = __float_as_uint(a.w), p = __float_as_uint(b.w);
+            if (best == ~0u || keyLess(t, g, p, bestT, bg, bp)) {
+                best = first + i;
+                bestT = t;
+                bg = g;
+                bp = p;
             }
         }
-        if (!found) break;
+    };
+#if X2_WHILE_WHILE
+    uint32_t ref = 0, leaf = 0;  // leaf: a postponed leaf reference (0: none)
+    for (;;) {
+        while (ref < Sentinel) {  // an inner node
+            ref = visitNode(ref);
+            if ((ref & LeafBit) && leaf == 0) {
+                leaf = ref;
+                ref = pop();
+            }
+            if (!__any(leaf == 0)) break;
+        }
+        while (leaf || (ref & LeafBit)) {
+            if (!leaf) {
+                leaf = ref;
+                ref = pop();
+            }
+            visitLeaf(leaf);
+            leaf = 0;
+        }
+        if (ref == Sentinel) break;
     }
+#else
+    uint32_t ref = 0;
+    for (;;) {
+        if (ref & LeafBit) {
+            visitLeaf(ref);
+            ref = pop();
+        } else {
+            ref = visitNode(ref);
+        }
+        if (ref == Sentinel) break;
+    }
+#endif
     tOut = bestT;
     return best;
 }
 
-// Any hit with t in [near, far] (a shadow ray; TERMINATE_ON_FIRST_HIT, async_gpu_device.h:521-533).
+// Any hit with t in [near, far] (a shadow ray; TERMINATE_ON_FIRST_HIT, async_gpu_device.h:521-533), children
+// unsorted, the same while-while order.
 __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, Counters& k, TravStack& st) {
     const WoopRay w = woopSetup(org, dir, nearT, farT);
     const BoxRay br = boxRay(org, dir);
     const float lo = nearT > 0 ? nearT / Slack : nearT * Slack, hi = farT > 0 ? farT * Slack : farT / Slack;
     int sp = 0;
+    auto pop = [&]() -> uint32_t { return sp > 0 ? st.get(--sp).x : Sentinel; };
+    auto visitNode = [&](uint32_t node) -> uint32_t {
+        ++k.nodes;
+        unsigned long long key[8];
+        nodeChildren(P.nodes[node], br, lo, hi, key);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (key[i] != ~0ull) st.put(sp++, make_uint2(uint32_t(key[i]), 0u));
+        return pop();
+    };
+    auto visitLeaf = [&](uint32_t leafRef) -> bool {
+        const uint32_t first = (leafRef & 0x7fffffffu) >> 3, count = (leafRef & 7u) + 1;
+        for (uint32_t i = 0; i < count; ++i) {
+            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
+            const float4 a = t4[0], b = t4[1], c = t4[2];
+            ++k.triangles;
+            float t;
+            if (woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) return true;
+        }
+        return false;
+    };
+#if X2_WHILE_WHILE
+    uint32_t ref = 0, leaf = 0;
+    for (;;) {
+        while (ref < Sentinel) {
+            ref = visitNode(ref);
+            if ((ref & LeafBit) && leaf == 0) {
+                leaf = ref;
+                ref = pop();
+            }
+            if (!__any(leaf == 0)) break;
+        }
+        while (leaf || (ref & LeafBit)) {
+            if (!leaf) {
+                leaf = ref;
+                ref = pop();
+            }
+            if (visitLeaf(leaf)) return true;
+            leaf = 0;
+        }
+        if (ref == Sentinel) return false;
+    }
+#else
     uint32_t ref = 0;
     for (;;) {
         if (ref & LeafBit) {
-            const uint32_t first = (ref & 0x7fffffffu) >> 3, count = (ref & 7u) + 1;
-            for (uint32_t i = 0; i < count; ++i) {
-                const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
-                const float4 a = t4[0], b = t4[1], c = t4[2];
-                ++k.triangles;
-                float t;
-                if (woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) return true;
-            }
+            if (visitLeaf(ref)) return true;
+            ref = pop();
         } else {
-            ++k.nodes;
-            unsigned long long key[8];
-            nodeChildren(P.nodes[ref], br, lo, hi, key);
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (key[i] != ~0ull) st.put(sp++, make_uint2(uint32_t(key[i]), 0u));
+            ref = visitNode(ref);
         }
-        if (sp == 0) return false;
-        ref = st.get(--sp).x;
+        if (ref == Sentinel) return false;
     }
+#endif
 }
 
 // ---------------------------------------------------------------------------------------------------------------
