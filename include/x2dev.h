@@ -234,6 +234,19 @@ int x2dev_resolve_image(x2dev* d, float* device_rgb);
 int x2dev_adjoint_frame(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, const float* d_image,
                         float* grad_textures, float* grad_roughness, x2_frame_stats* stats);
 
+/* Diagnostics (no counterpart in the reference): the device's sample-identity RNG evaluated on the device, for a
+ * twin test against a host restatement. For query i: words[i] = samplingWord(identity, dimension, policy)
+ * (sample_stream.h:127-144), stream_out[i * draws ...] = the first `draws` values of paths::Stream(identity,
+ * bounce) (async_integrator.h:151-176), branches[i] = childBranch(branch, a, b) (async_integrator.h:281-286).
+ * Host arrays, copied in and out; n <= 2^20, draws <= 256. */
+typedef struct x2_rng_query {
+    uint32_t pixel, dimension, policy, bounce;
+    uint64_t index, stream, branch;
+    uint32_t a, b;
+} x2_rng_query;
+int x2dev_rng_twin(x2dev* d, const x2_rng_query* queries, uint32_t n, uint32_t draws, uint32_t* words,
+                   float* stream_out, uint64_t* branches);
+
 /* Algorithmic byte sizes the roofline counts: one BVH8 node and one triangle record. */
 #define X2_NODE_BYTES 256u
 #define X2_TRIANGLE_BYTES 48u

@@ -850,6 +850,9 @@ __device__ __forceinline__ float voxelAt(const VolumeDesc& v, const uint16_t* po
     if (e == ~0u) return 0.0f;
     return halfValue(pool[size_t(e) * 512 + (uint32_t(k & 7) * 8 + uint32_t(j & 7)) * 8 + uint32_t(i & 7)]);
 }
+#ifndef X2_BRICK_FAST
+#define X2_BRICK_FAST 1
+#endif
 __device__ float sampleGrid(const VolumeDesc& v, const uint16_t* pool, V3 p) {
     float q[3];
     for (int a = 0; a < 3; ++a) {
@@ -859,10 +862,25 @@ __device__ float sampleGrid(const VolumeDesc& v, const uint16_t* pool, V3 p) {
     const float fx0 = floorf(q[0]), fy0 = floorf(q[1]), fz0 = floorf(q[2]);
     const int i = int(fx0), j = int(fy0), k = int(fz0);
     const float fx = q[0] - fx0, fy = q[1] - fy0, fz = q[2] - fz0;
-    const float c00 = lerpf(voxelAt(v, pool, i, j, k), voxelAt(v, pool, i + 1, j, k), fx);
-    const float c10 = lerpf(voxelAt(v, pool, i, j + 1, k), voxelAt(v, pool, i + 1, j + 1, k), fx);
-    const float c01 = lerpf(voxelAt(v, pool, i, j, k + 1), voxelAt(v, pool, i + 1, j, k + 1), fx);
-    const float c11 = lerpf(voxelAt(v, pool, i, j + 1, k + 1), voxelAt(v, pool, i + 1, j + 1, k + 1), fx);
+    float t[8];  // the 8 voxels, x fastest
+#if X2_BRICK_FAST
+    if (i >= 0 && j >= 0 && k >= 0 && (i & 7) != 7 && (j & 7) != 7 && (k & 7) != 7) {
+        // all 8 in one brick (i + 1 < res as i + 1 is inside the brick): one table lookup
+        const uint32_t e = v.table[(uint32_t(k >> 3) * v.nb[1] + uint32_t(j >> 3)) * v.nb[0] + uint32_t(i >> 3)];
+        if (e == ~0u) return 0.0f;
+        const uint16_t* b = pool + size_t(e) * 512 + (uint32_t(k & 7) * 8 + uint32_t(j & 7)) * 8 + uint32_t(i & 7);
+        t[0] = halfValue(b[0]), t[1] = halfValue(b[1]), t[2] = halfValue(b[8]), t[3] = halfValue(b[9]);
+        t[4] = halfValue(b[64]), t[5] = halfValue(b[65]), t[6] = halfValue(b[72]), t[7] = halfValue(b[73]);
+    } else
+#endif
+    {
+        t[0] = voxelAt(v, pool, i, j, k), t[1] = voxelAt(v, pool, i + 1, j, k);
+        t[2] = voxelAt(v, pool, i, j + 1, k), t[3] = voxelAt(v, pool, i + 1, j + 1, k);
+        t[4] = voxelAt(v, pool, i, j, k + 1), t[5] = voxelAt(v, pool, i + 1, j, k + 1);
+        t[6] = voxelAt(v, pool, i, j + 1, k + 1), t[7] = voxelAt(v, pool, i + 1, j + 1, k + 1);
+    }
+    const float c00 = lerpf(t[0], t[1], fx), c10 = lerpf(t[2], t[3], fx);
+    const float c01 = lerpf(t[4], t[5], fx), c11 = lerpf(t[6], t[7], fx);
     return lerpf(lerpf(c00, c10, fy), lerpf(c01, c11, fy), fz);
 }
 __device__ __forceinline__ uint32_t brickAt(const VolumeDesc& v, V3 p) {
@@ -1004,8 +1022,11 @@ struct MarchCk {  // the march as a recorded step begins: its lattice index, tra
     float vt, partial;
     uint32_t count;
 };
+#ifndef X2_MARCH_SLOTS
+#define X2_MARCH_SLOTS 16  // 8, 4, 32 measured slower
+#endif
 struct MarchRecord {
-    static constexpr uint32_t Slots = 8;
+    static constexpr uint32_t Slots = X2_MARCH_SLOTS;
     MarchCk at[Slots];
     uint32_t used = 0, stride = 8, last = 0;
     __device__ bool wants(uint32_t count) const {
@@ -1791,7 +1812,17 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
 __global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) { runLaunch<false, false>(P); }
 __global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) { runLaunch<true, false>(P); }
 // A scene with a medium: the same estimator with the medium's light and transmittance on every ray.
-__global__ void __launch_bounds__(256) renderVolumeKernel(const RenderParams P) { runLaunch<false, true>(P); }
+#ifndef X2_VOLUME_WAVES
+#define X2_VOLUME_WAVES 2  // 256 VGPRs, no accumulation registers: 2 waves/SIMD (1 without the bound)
+#endif
+#if X2_VOLUME_WAVES
+#define X2_VOLUME_OCCUPANCY __attribute__((amdgpu_waves_per_eu(X2_VOLUME_WAVES)))
+#else
+#define X2_VOLUME_OCCUPANCY
+#endif
+__global__ void __launch_bounds__(256) X2_VOLUME_OCCUPANCY renderVolumeKernel(const RenderParams P) {
+    runLaunch<false, true>(P);
+}
 
 // ---------------------------------------------------------------------------------------------------------------
 // Batched jobs: many independent frames (each job its own scene, settings and film, a RenderParams in device
@@ -1876,6 +1907,19 @@ __global__ void stampKernel(unsigned long long* at) {
     if (threadIdx.x == 0) *at = wall_clock64();
 }
 
+// The RNG twin (diagnostics): the device's RNG functions on a list of identities.
+__global__ void rngTwinKernel(const x2_rng_query* q, uint32_t n, uint32_t draws, uint32_t* words, float* out,
+                              uint64_t* branches) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const x2_rng_query r = q[i];
+    const Identity id{r.pixel, r.index, r.stream, r.branch};
+    words[i] = samplingWord(id, r.dimension, r.policy);
+    Stream s(id, r.bounce);
+    for (uint32_t k = 0; k < draws; ++k) out[size_t(i) * draws + k] = s.next();
+    branches[i] = childBranch(r.branch, r.a, r.b);
+}
+
 // The film's mean rgb per pixel (sums / counts).
 __global__ void resolveKernel(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -1953,6 +1997,11 @@ hipError_t launchFill(const FillRegion* regions, uint32_t n, hipStream_t stream)
 }
 hipError_t launchStamp(unsigned long long* at, hipStream_t stream) {
     hipLaunchKernelGGL(stampKernel, dim3(1), dim3(64), 0, stream, at);
+    return hipGetLastError();
+}
+hipError_t launchRngTwin(const x2_rng_query* q, uint32_t n, uint32_t draws, uint32_t* words, float* out,
+                         uint64_t* branches, hipStream_t stream) {
+    hipLaunchKernelGGL(rngTwinKernel, dim3((n + 255) / 256), dim3(256), 0, stream, q, n, draws, words, out, branches);
     return hipGetLastError();
 }
 hipError_t uploadSobol(const uint32_t* table) {

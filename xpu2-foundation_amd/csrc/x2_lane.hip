@@ -23,6 +23,8 @@ hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream);
 hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream);
 hipError_t launchClearState(LaunchState* s, hipStream_t stream);
 hipError_t uploadSobol(const uint32_t* table);
+hipError_t launchRngTwin(const x2_rng_query* q, uint32_t n, uint32_t draws, uint32_t* words, float* out,
+                         uint64_t* branches, hipStream_t stream);
 }  // namespace x2
 
 namespace {
@@ -517,6 +519,35 @@ int x2dev_set_texture(x2dev* d, uint32_t texture, const float* texels, int texel
                           texels_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, d->stream));
     X2_HIP(hipStreamSynchronize(d->stream));
     return X2_OK;
+}
+
+int x2dev_rng_twin(x2dev* d, const x2_rng_query* queries, uint32_t n, uint32_t draws, uint32_t* words,
+                   float* stream_out, uint64_t* branches) {
+    if (!d || !queries || !words || !branches || (draws && !stream_out)) return fail(X2_ERR_ARGUMENT, "null argument");
+    if (n == 0 || n > (1u << 20) || draws > 256) return fail(X2_ERR_ARGUMENT, "n must be 1..2^20, draws <= 256");
+    X2_HIP(hipSetDevice(d->device));
+    void *dq = nullptr, *dw = nullptr, *ds = nullptr, *db = nullptr;
+    auto done = [&](int r) {
+        (void)hipFree(dq);
+        (void)hipFree(dw);
+        (void)hipFree(ds);
+        (void)hipFree(db);
+        return r;
+    };
+    hipError_t e = hipMalloc(&dq, size_t(n) * sizeof(x2_rng_query));
+    if (e == hipSuccess) e = hipMalloc(&dw, size_t(n) * 4);
+    if (e == hipSuccess) e = hipMalloc(&ds, std::max<size_t>(4, size_t(n) * draws * 4));
+    if (e == hipSuccess) e = hipMalloc(&db, size_t(n) * 8);
+    if (e == hipSuccess) e = hipMemcpy(dq, queries, size_t(n) * sizeof(x2_rng_query), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launchRngTwin(static_cast<const x2_rng_query*>(dq), n, draws, static_cast<uint32_t*>(dw),
+                          static_cast<float*>(ds), static_cast<uint64_t*>(db), d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e == hipSuccess) e = hipMemcpy(words, dw, size_t(n) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && draws) e = hipMemcpy(stream_out, ds, size_t(n) * draws * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(branches, db, size_t(n) * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(hipFail(e, "x2dev_rng_twin"));
+    return done(X2_OK);
 }
 
 int x2dev_set_material(x2dev* d, uint32_t material, const x2_material* m) {

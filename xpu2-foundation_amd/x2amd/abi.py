@@ -113,6 +113,11 @@ class FrameStats(ctypes.Structure):
     ]
 
 
+class RngQuery(ctypes.Structure):
+    _fields_ = [("pixel", u32), ("dimension", u32), ("policy", u32), ("bounce", u32), ("index", u64), ("stream", u64),
+                ("branch", u64), ("a", u32), ("b", u32)]
+
+
 class BatchTuning(ctypes.Structure):
     _fields_ = [("block_threads", u32), ("claim_items", u32), ("waves_per_simd", u32), ("pad", u32),
                 ("seconds", f64)]
@@ -132,7 +137,7 @@ EXPECTED_SIZES = {
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 268),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 16), "x2_frame_stats": (FrameStats, 80),
-    "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80),
+    "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
     assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
@@ -164,6 +169,8 @@ SIGNATURES = {
     "x2dev_resolve_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "x2dev_adjoint_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(FrameStats)]),
+    "x2dev_rng_twin": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RngQuery), u32, u32, ctypes.POINTER(u32),
+                                      ctypes.POINTER(f32), ctypes.POINTER(u64)]),
     "x2batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "x2batch_add_job": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Scene), ctypes.POINTER(Config),
                                        ctypes.POINTER(u32)]),
