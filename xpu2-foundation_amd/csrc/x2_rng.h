@@ -14,12 +14,13 @@ extern __constant__ uint32_t kSobolDirections[17 * 32];
 
 struct Words { uint32_t v[4]; };
 
-// Philox4x32-10 (sample_stream.h:31-38): 10 rounds of two 32x32->64 multiplies and a Weyl key schedule.
+// Philox4x32-10 (sample_stream.h:31-38): 10 rounds of two 32x32->64 multiplies and a Weyl key schedule. Each
+// product is one 64-bit multiply (v_mad_u64_u32), its halves the round's high and low words.
 __device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t ka, uint32_t kb) {
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
-        const uint32_t alo = c0 * 0xd2511f53u, ahi = __umulhi(c0, 0xd2511f53u);
-        const uint32_t blo = c2 * 0xcd9e8d57u, bhi = __umulhi(c2, 0xcd9e8d57u);
+        const uint64_t pa = uint64_t(c0) * 0xd2511f53u, pb = uint64_t(c2) * 0xcd9e8d57u;
+        const uint32_t alo = uint32_t(pa), ahi = uint32_t(pa >> 32), blo = uint32_t(pb), bhi = uint32_t(pb >> 32);
         c0 = bhi ^ c1 ^ ka;
         c1 = blo;
         c2 = ahi ^ c3 ^ kb;
@@ -90,6 +91,25 @@ __device__ __forceinline__ uint32_t samplingWord(const Identity& id, uint32_t di
 }
 __device__ __forceinline__ float draw(const Identity& id, uint32_t dimension, uint32_t policy) {
     return unit(samplingWord(id, dimension, policy));
+}
+// Two draws of an even dimension and the next (0/1 camera, 16/17 first-hit BSDF, 18/19 first-hit light), the Philox
+// block they share computed once: Philox words of one group, or the Sobol pair's one seed block. The same words as
+// two draw() calls.
+__device__ __forceinline__ void drawPair(const Identity& id, uint32_t dimension, uint32_t policy, float& u0, float& u1) {
+    const bool camera = dimension < 5, diffuse = dimension == 16, light = dimension == 18;
+    const bool sobolPair = (camera && (policy & 1)) || (diffuse && (policy & 2)) || (light && (policy & 4));
+    if (!sobolPair) {
+        const Words w = block(id, dimension >> 2);
+        u0 = unit(pick(w, dimension & 3));
+        u1 = unit(pick(w, (dimension + 1) & 3));
+        return;
+    }
+    const uint32_t d = camera ? dimension : dimension - 14;  // 0, 2 or 4: the pair (d, d + 1) within one seed block
+    Identity shift = id;
+    shift.index = id.index >> 32;
+    const Words seeds = block(shift, 0x534f424fu + (d >> 2));
+    u0 = unit(owenScramble(sobol(uint32_t(id.index), d), pick(seeds, d & 3)));
+    u1 = unit(owenScramble(sobol(uint32_t(id.index), d + 1), pick(seeds, (d + 1) & 3)));
 }
 
 constexpr uint32_t CameraDimensions = 16, DimensionsPerBounce = 256;  // sample_stream.h:149
