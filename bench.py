@@ -11,7 +11,8 @@ and the CPU baseline (the oracle on this host's cores over a bounded sample of t
 leg times one forward frame and one path-replay adjoint frame of the differentiable Cornell box (512x512, 32 spp):
 adjoint_step_over_forward is the adjoint frame's time over the forward frame's. The volume leg renders the
 participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp); the batch leg the 64-job batched
-workload, persistent kernel from a hipGraph against one launch per job.
+workload, persistent kernel from a hipGraph against one launch per job; the direct leg the prototype estimator on
+the headline scene. roofline.copy_ceiling_gbs is a measured device copy's bandwidth beside the published peak.
 """
 import argparse
 import json
@@ -195,6 +196,56 @@ def batch_leg(reps):
     return out
 
 
+def direct_leg(reps):
+    """The prototype estimator (integrateDirect: camera hit, emission, one cosine-sampled Lambert ray to a constant
+    sky of 0.2) on the headline scene at 1920x1080 x 64 spp; the reference quotes its GPU lane rate on this
+    estimator (400-530 M samples/s on other hardware and scenes, ASYNC_RESULTS.md:58)."""
+    from x2amd import scenes
+    from x2amd.lane import GpuLane
+
+    scene, config = scenes.icospheres()
+    config.direct_only, config.surface_limit = 1, 1
+    config.sky[:] = [0.2, 0.2, 0.2]
+    lane = GpuLane(0)
+    lane.prepare(scene)
+    lane.render_frame(config)
+    wall, rays = 0.0, 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, _, st = lane.render_frame(config)
+        wall += time.perf_counter() - t0
+        assert st.errors == 0 and st.audit_failures == 0
+        rays += st.rays_trace + st.rays_shadow
+    lane.close()
+    paths = config.width * config.height * config.spp * reps
+    return {"workload": f"icospheres {scene.triangle_count} tris, {config.width}x{config.height}, {config.spp} spp, "
+                        "prototype estimator, surface limit 1, sky 0.2",
+            "reps": reps, "ms_per_frame": wall / reps * 1e3, "mpaths_per_s": paths / wall / 1e6,
+            "mrays_per_s": rays / wall / 1e6}
+
+
+def copy_ceiling(mib=1024):
+    """GB/s of a plain device-to-device copy (read + write bytes over the copy's time), the chip's practical
+    bandwidth beside the 8 TB/s peak."""
+    import numpy as np
+    from x2amd.hipmem import DeviceBuffer
+
+    n = mib << 20
+    a, b = DeviceBuffer(n), DeviceBuffer(n)
+    a.zero()
+    best = 1e30
+    for _ in range(3):
+        a.lib.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        if a.lib.hipMemcpy(b.ptr, a.ptr, n, 3) != 0:
+            raise RuntimeError("hipMemcpy device to device failed")
+        a.lib.hipDeviceSynchronize()
+        best = min(best, time.perf_counter() - t0)
+    a.free()
+    b.free()
+    return 2 * n / best / 1e9
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (the runtime's blit kernel), run under rocprofv3 --pmc to scale
     FETCH_SIZE and WRITE_SIZE: the counters' reading for a copy of known size."""
@@ -238,6 +289,7 @@ def main():
     ap.add_argument("--no-volume", action="store_true")
     ap.add_argument("--volume-reps", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB: only run a device copy (for --pmc scaling)")
     ap.add_argument("--batch-reps", type=int, default=3)
     args = ap.parse_args()
@@ -312,6 +364,9 @@ def main():
         out["volume"] = volume_leg(args.volume_reps, not args.no_cpu)
     if not args.no_batch:
         out["batch"] = batch_leg(args.batch_reps)
+    if not args.no_direct:
+        out["direct"] = direct_leg(2)
+    out["roofline"]["copy_ceiling_gbs"] = copy_ceiling()
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)
         out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp, 20)

@@ -148,6 +148,12 @@ typedef struct x2_config {
     float inv_volume_quality;   /* march step = voxel size * inv_volume_quality * max(depth - 1, 1) */
     float volume_attenuation_factor, volume_contribution_factor, volume_phase_attenuation_factor;
     uint32_t pad_volume;
+    /* The estimator: 0 the path tracer (integratePaths); nonzero the prototype (integrateDirect,
+     * async_integrator.h:61-126, mixed::Config::paths == 0): the camera hit's emission and surface_limit (1..8)
+     * cosine-sampled Lambert bounces, the last looking for a constant sky of radiance sky. */
+    uint32_t direct_only, surface_limit;
+    float sky[3];
+    uint32_t pad_direct;
 } x2_config;
 
 /* What the host learns when a launch has finished (async_gpu.h:65 Completion), plus traversal counters. */

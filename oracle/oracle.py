@@ -54,6 +54,7 @@ def load():
         "oracle_set_eval_roughness": (None, [vp, u32, f32]),
         "oracle_ggx_roughness_log_derivative": (f32, [f32, P(f32), P(f32), P(f32)]),
         "oracle_texture_lookup": (c_int, [vp, f32, f32, P(f32)]),
+        "oracle_surface_lambert": (c_int, [P(f32), P(f32), f32, f32, P(f32), P(f32), P(f32)]),
         "oracle_volume_density": (f32, [vp, P(f32)]),
         "oracle_volume_transmittance": (f32, [vp, vp, P(f32), P(f32), f32, f32, u32, u64, u64]),
         "oracle_phase_eval": (f32, [f32, P(f32), P(f32)]),

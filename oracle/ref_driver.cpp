@@ -160,6 +160,16 @@ void ref_lobe_sample(uint32_t kind, const float N[3], const float Ng[3], const f
     put(f, ii::shade::lobeSample(g_bsdf, rec, s, r1, r2, w, *pdf));
     put(wi, w);
 }
+// The prototype estimator's Lambert sample (surface::lambert, surface_core.h:158-165).
+int ref_surface_lambert(const float albedo[3], const float N[3], float u, float v, float wi[3], float f[3], float* pdf) {
+    ii::surface::Lambert m{v3(albedo), V3{0, 0, 0}};
+    ii::surface::Sample s{};
+    if (!ii::surface::lambert(m, v3(N), u, v, s)) return 0;
+    put(wi, s.wi);
+    put(f, s.f);
+    *pdf = s.pdf;
+    return 1;
+}
 // The reference's bilinear filter (tex::bilinear on level 0, periodic, float texels) at the lookup's (s, t) =
 // (u, 1 - v) with unit scale and no offset; the flat row-major texels are laid into its 32 x 32 tile pages first.
 int ref_bilinear(uint32_t width, uint32_t height, const float* texels, float u, float v, float out[3]) {

@@ -1449,8 +1449,93 @@ float volumeRadiance(const Scene& S, const x2_config& c, const Identity& root, V
     return vt;
 }
 
+// The prototype estimator (async_integrator.h:61-126; Config::paths == 0): camera hit, emission, cosine-sampled
+// Lambert bounces (surface::lambert, surface_core.h:158-165) to a constant sky; the surface_limit-th interaction
+// looks along its sampled direction for the sky. Albedo: the material's Lambert lobe (albedo x diffuse weight, x map).
+bool surfaceLambert(V3 albedo, V3 N, float u, float v, V3& wi, V3& f, float& pdf) {
+    if (!finite3(albedo) || albedo.x < 0 || albedo.y < 0 || albedo.z < 0 || !finite3(N) ||
+        fabsf(dot(N, N) - 1) > 2e-5f || !finite(u) || !finite(v) || u < 0 || u > 1 || v < 0 || v > 1)
+        return false;
+    float sn, cs;
+    sincosPoly(v * (2 * Pi), sn, cs);
+    const float s = sqrtf(1 - u);
+    wi = localToGlobal(frameOf(N), V3{s * cs, s * sn, sqrtf(u)});
+    const float cosine = fmaxf(dot(N, wi), 0.0f);
+    pdf = cosine * OneOverPi;
+    f = (albedo * OneOverPi) * cosine;
+    return true;
+}
+int integrateDirect(const Scene& S, const x2_config& c, U32 pixel, U32 index, double out[4], Counters& k) {
+    if (c.surface_limit == 0 || c.surface_limit > 8 || pixel >= c.width * c.height) return 2;
+    const Identity id{pixel, index, c.seed, 0};
+    float x, y;
+    if (!boxFilter(c.filter_width, draw(id, 0, c.sampling), x) || !boxFilter(c.filter_width, draw(id, 1, c.sampling), y))
+        return 2;
+    Ray ray;
+    if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, ray)) return 2;
+    double radiance[3] = {0, 0, 0}, throughput[3] = {1, 1, 1};
+    U32 depth = 0, primaryHit = 0;
+    const V3 sky = load3(c.sky);
+    for (;;) {
+        const Hit hit = trace(S, ray, k);
+        if (hit.geom == ~0u) {
+            if (depth) {
+                radiance[0] += throughput[0] * sky.x;
+                radiance[1] += throughput[1] * sky.y;
+                radiance[2] += throughput[2] * sky.z;
+            }
+            break;
+        }
+        if (hit.normal.x == 0 && hit.normal.y == 0 && hit.normal.z == 0) return 2;
+        if (!depth) primaryHit = 1;
+        const U32 tri = hit.prim;
+        const x2_material& m = S.materials[S.material[tri]];
+        V3 albedo = load3(m.albedo) * m.diffuse_weight;
+        if (m.albedo_map) {
+            float uv[6] = {0, 0, 1, 0, 1, 1};
+            if (!S.triUV.empty())
+                for (int q = 0; q < 6; ++q) uv[q] = S.triUV[size_t(tri) * 6 + q];
+            const float w = 1.0f - hit.u - hit.v;
+            TexTap tap;
+            if (!textureTap(S.textures[m.albedo_map - 1], m.albedo_map - 1, w * uv[0] + hit.u * uv[2] + hit.v * uv[4],
+                            w * uv[1] + hit.u * uv[3] + hit.v * uv[5], tap))
+                return 2;
+            albedo = albedo * tap.color;
+        }
+        const U32 dimension = CameraDimensions + 2 * depth;
+        V3 wi, f;
+        float pdf;
+        if (!surfaceLambert(albedo, hit.normal, draw(id, dimension, c.sampling), draw(id, dimension + 1, c.sampling), wi,
+                            f, pdf))
+            return 2;
+        const V3 e = load3(m.emission);
+        radiance[0] += throughput[0] * e.x;
+        radiance[1] += throughput[1] * e.y;
+        radiance[2] += throughput[2] * e.z;
+        if (pdf <= 0) break;
+        throughput[0] *= f.x / pdf;
+        throughput[1] *= f.y / pdf;
+        throughput[2] *= f.z / pdf;
+        if (++depth == c.surface_limit) {
+            if (visible(S, hit.origin, wi, 0.0f, 3.402823466e+38f, k)) {
+                radiance[0] += throughput[0] * sky.x;
+                radiance[1] += throughput[1] * sky.y;
+                radiance[2] += throughput[2] * sky.z;
+            }
+            break;
+        }
+        ray = Ray{hit.origin, wi, 0.0f, 3.402823466e+38f};
+    }
+    out[0] = radiance[0];
+    out[1] = radiance[1];
+    out[2] = radiance[2];
+    out[3] = primaryHit;
+    return 0;
+}
+
 int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double out[4], Counters& k,
               const Adjoint* adj = nullptr) {
+    if (c.direct_only) return adj ? 2 : integrateDirect(S, c, pixel, index, out, k);
     if (pixel >= c.width * c.height || c.bsdf_samples == 0 || c.bsdf_samples > 64 || c.light_samples == 0 ||
         c.light_samples > 64)
         return 2;
@@ -2129,6 +2214,14 @@ void oracle_set_eval_roughness(oracle_scene* o, uint32_t material, float r) {
 }
 float oracle_ggx_roughness_log_derivative(float r, const float N[3], const float wo[3], const float wi[3]) {
     return ggxRoughnessLogDerivative(r, load3(N), load3(wo), load3(wi));
+}
+int oracle_surface_lambert(const float albedo[3], const float N[3], float u, float v, float wi[3], float f[3],
+                           float* pdf) {
+    V3 w, ff;
+    if (!surfaceLambert(load3(albedo), load3(N), u, v, w, ff, *pdf)) return 0;
+    store3(wi, w);
+    store3(f, ff);
+    return 1;
 }
 int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]) {
     Texture tex;

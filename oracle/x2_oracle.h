@@ -87,6 +87,9 @@ void oracle_set_eval_roughness(oracle_scene* s, uint32_t material, float r);
 /* d ln f / d r of the GGX lobe, and one bilinear texture lookup at (u, v). */
 float oracle_ggx_roughness_log_derivative(float r, const float N[3], const float wo[3], const float wi[3]);
 int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]);
+/* The prototype estimator's Lambert sample (surface::lambert): 1 with (wi, f, pdf), 0 for an invalid input. */
+int oracle_surface_lambert(const float albedo[3], const float N[3], float u, float v, float wi[3], float f[3],
+                           float* pdf);
 /* The medium, piece by piece (for the tests): density at p; one ratio-tracking transmittance estimate along a ray
  * with its stream at (pixel, index, the config's seed, branch); the Henyey-Greenstein phase; one march (returns its
  * recorded steps; out: transmittance, emitted rgb, first step start, last step end, weight total, reached). */

@@ -34,6 +34,7 @@ def load():
         "ref_lobe_eval": (None, [u32, P(f32), P(f32), P(f32), P(f32), f32, P(f32), P(f32)]),
         "ref_lobe_sample": (None, [u32, P(f32), P(f32), P(f32), f32, f32, f32, P(f32), P(f32), P(f32)]),
         "ref_bilinear": (c_int, [u32, u32, P(f32), f32, f32, P(f32)]),
+        "ref_surface_lambert": (c_int, [P(f32), P(f32), f32, f32, P(f32), P(f32), P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

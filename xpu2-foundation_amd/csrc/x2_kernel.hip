@@ -1707,6 +1707,92 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
     return 0;
 }
 
+// The prototype estimator (async_integrator.h:61-126; mixed::Config::paths == 0): the camera hit, its emission, and
+// cosine-sampled Lambert bounces (surface::lambert, surface_core.h:158-165) toward a constant sky, the
+// surface_limit-th interaction looking along its sampled direction for the sky. Its albedo is the material's Lambert
+// lobe (albedo x diffuse weight, times its map).
+__device__ int integrateDirect(const RenderParams& P, uint32_t pixel, uint32_t index, double rgba[4], Counters& k,
+                               TravStack& st) {
+    const x2_config& c = P.cfg;
+    if (c.surface_limit == 0 || c.surface_limit > 8) return 2;
+    const Identity id{pixel, index, c.seed, 0};
+    float x, y;
+    {
+        float r0, r1;
+        drawPair(id, 0, c.sampling, r0, r1);
+        const float w = c.filter_width;
+        if (!finite1(w) || w <= 0) return 2;
+        x = (r0 - .5f) * w + .5f;
+        y = (r1 - .5f) * w + .5f;
+        if (!finite1(x) || !finite1(y)) return 2;
+    }
+    Ray ray;
+    if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, ray)) return 2;
+    double radiance[3] = {0, 0, 0}, throughput[3] = {1, 1, 1};
+    uint32_t depth = 0, primaryHit = 0;
+    const V3 sky = ld3(c.sky);
+    for (;;) {
+        Hit hit;
+        if (!traceHit(P, ray, hit, k, st)) return 2;
+        if (hit.geom == ~0u) {
+            if (depth) {
+                radiance[0] += throughput[0] * sky.x;
+                radiance[1] += throughput[1] * sky.y;
+                radiance[2] += throughput[2] * sky.z;
+            }
+            break;
+        }
+        if (!depth) primaryHit = 1;
+        const TriRecord& tr = P.tris[hit.tri];
+        const x2_material& m = P.materials[tr.material];
+        V3 albedo = ld3(m.albedo) * m.diffuse_weight;
+        if (m.albedo_map) {
+            float uv[6] = {0, 0, 1, 0, 1, 1};
+            if (P.triUV)
+                for (int q = 0; q < 6; ++q) uv[q] = P.triUV[size_t(hit.tri) * 6 + q];
+            const float w = 1.0f - hit.u - hit.v;
+            TexTap tap;
+            if (!textureTap(P.textures[m.albedo_map - 1], m.albedo_map - 1, w * uv[0] + hit.u * uv[2] + hit.v * uv[4],
+                            w * uv[1] + hit.u * uv[3] + hit.v * uv[5], tap))
+                return 2;
+            albedo = albedo * tap.color;
+        }
+        const V3 emission = ld3(m.emission), N = hit.normal;
+        const uint32_t dimension = CameraDimensions + 2 * depth;
+        const float u = draw(id, dimension, c.sampling), v = draw(id, dimension + 1, c.sampling);
+        if (!finite3(albedo) || albedo.x < 0 || albedo.y < 0 || albedo.z < 0 || !finite3(N) ||
+            fabsf(dot(N, N) - 1) > 2e-5f || !finite1(u) || !finite1(v) || u < 0 || u > 1 || v < 0 || v > 1)
+            return 2;
+        float sn, cs;
+        sincosPoly(v * (2 * Pi), sn, cs);
+        const float sr = sqrtf(1 - u);
+        const V3 wi = localToGlobal(frameOf(N), V3{sr * cs, sr * sn, sqrtf(u)});
+        const float cosine = fmaxf(dot(N, wi), 0.0f), pdf = cosine * OneOverPi;
+        const V3 f = (albedo * OneOverPi) * cosine;
+        radiance[0] += throughput[0] * emission.x;
+        radiance[1] += throughput[1] * emission.y;
+        radiance[2] += throughput[2] * emission.z;
+        if (pdf <= 0) break;
+        throughput[0] *= f.x / pdf;
+        throughput[1] *= f.y / pdf;
+        throughput[2] *= f.z / pdf;
+        if (++depth == c.surface_limit) {
+            if (visible(P, hit.origin, wi, 0.0f, MaxFloat, k, st)) {
+                radiance[0] += throughput[0] * sky.x;
+                radiance[1] += throughput[1] * sky.y;
+                radiance[2] += throughput[2] * sky.z;
+            }
+            break;
+        }
+        ray = Ray{hit.origin, wi, 0.0f, MaxFloat};
+    }
+    rgba[0] = radiance[0];
+    rgba[1] = radiance[1];
+    rgba[2] = radiance[2];
+    rgba[3] = primaryHit;
+    return 0;
+}
+
 __device__ __forceinline__ uint32_t tilePixel(uint32_t tile, uint32_t position, const x2_config& c) {
     const uint32_t tilesX = (c.width + 7) / 8;
     const uint32_t x = (tile % tilesX) * 8 + position % 8, y = (tile / tilesX) * 8 + position / 8;
@@ -1741,7 +1827,7 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
 // One sample per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
 // position; the chunk comes from the mailbox ring. The forward render adds the sample to the film; the adjoint
 // replays it and scatters its gradients.
-template <bool Adj, bool Vol>
+template <bool Adj, bool Vol, bool Direct = false>
 __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long long item, Counters& k, TravStack& st,
                                            uint32_t& errors, float* gr) {
     const unsigned long long chunkIndex = item / P.itemsPerChunk;
@@ -1751,7 +1837,10 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
     const uint32_t pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
     if (pixel == ~0u) return;
     double rgba[4];
-    if (integrate<Adj, Vol>(P, pixel, chunk.first + s, rgba, k, st, gr) == 0) {
+    int status;
+    if constexpr (Direct) status = integrateDirect(P, pixel, chunk.first + s, rgba, k, st);
+    else status = integrate<Adj, Vol>(P, pixel, chunk.first + s, rgba, k, st, gr);
+    if (status == 0) {
         if constexpr (!Adj) {
             double* f = P.sums + size_t(pixel) * 4;
             atomicAdd(f + 0, rgba[0]);
@@ -1768,7 +1857,7 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
 // A lane per item (the 64 lanes of a wave are one 8x8 tile at one sample index, so camera rays start coherent), or
 // persistent lanes claiming items one at a time (the reference's Feed loop) when built with X2_PERSISTENT. Every lane
 // reaches the end so the wave's counter sums see all 64 lanes.
-template <bool Adj, bool Vol>
+template <bool Adj, bool Vol, bool Direct = false>
 __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
     TravStack st;
@@ -1782,11 +1871,11 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     for (;;) {
         const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
         if (item >= P.itemEnd) break;
-        renderItem<Adj, Vol>(P, item, k, st, errors, gr);
+        renderItem<Adj, Vol, Direct>(P, item, k, st, errors, gr);
     }
 #else
     const unsigned long long item = P.itemBegin + (unsigned long long)blockIdx.x * 256u + threadIdx.x;
-    if (item < P.itemEnd) renderItem<Adj, Vol>(P, item, k, st, errors, gr);
+    if (item < P.itemEnd) renderItem<Adj, Vol, Direct>(P, item, k, st, errors, gr);
 #endif
     if (errors) atomicAdd(&P.state->errors, errors);
     if constexpr (Adj) {
@@ -1823,6 +1912,8 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderPa
 __global__ void __launch_bounds__(256) X2_VOLUME_OCCUPANCY renderVolumeKernel(const RenderParams P) {
     runLaunch<false, true>(P);
 }
+// The prototype estimator's kernel (x2_config.direct_only).
+__global__ void __launch_bounds__(256) directKernel(const RenderParams P) { runLaunch<false, false, true>(P); }
 
 // ---------------------------------------------------------------------------------------------------------------
 // Batched jobs: many independent frames (each job its own scene, settings and film, a RenderParams in device
@@ -1940,9 +2031,9 @@ __global__ void clearState(LaunchState* s) {
 
 // Host-side launchers the lane calls (x2_lane.hip).
 namespace x2 {
-template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium
+template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium, 3 the prototype estimator
 hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
-    const auto kernel = Kind == 0 ? renderKernel : Kind == 1 ? adjointKernel : renderVolumeKernel;
+    const auto kernel = Kind == 0 ? renderKernel : Kind == 1 ? adjointKernel : Kind == 2 ? renderVolumeKernel : directKernel;
     static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
     if (X2_PERSISTENT && !resident) {
         int dev = 0, perCu = 0;
@@ -1962,6 +2053,7 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
     return hipGetLastError();
 }
 hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
+    if (p.cfg.direct_only) return launchFrame<3>(p, stream);
     return p.vol.table ? launchFrame<2>(p, stream) : launchFrame<0>(p, stream);
 }
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }

@@ -168,6 +168,9 @@ int validateConfig(const x2dev* d, const x2_config& c) {
         return fail(X2_ERR_ARGUMENT, "depth limits above 250 are not supported");
     if (c.filter != 0) return fail(X2_ERR_ARGUMENT, "only the box pixel filter is implemented");
     if (c.light_count > d->lightCount) return fail(X2_ERR_ARGUMENT, "config.light_count exceeds the scene's lights");
+    if (c.direct_only && (c.surface_limit == 0 || c.surface_limit > 8))
+        return fail(X2_ERR_ARGUMENT, "the prototype estimator takes surface_limit 1..8 (async_integrator.h:65)");
+    if (c.direct_only && d->volDesc.table) return fail(X2_ERR_ARGUMENT, "the prototype estimator renders no media");
     if (d->volDesc.table && (c.max_volume_depth > 16 || c.volume_illumination_samples > 64 ||
                              !(c.inv_volume_quality > 0.0f) || !(c.volume_transmittance_threshold >= 0.0f)))
         return fail(X2_ERR_ARGUMENT, "volume settings budget (cpu_scene.h:3168): depth <= 16, samples <= 64, "
@@ -687,6 +690,7 @@ int x2dev_adjoint_frame(x2dev* d, const x2_config* c, uint32_t spc, const float*
     if (r) return r;
     if (c->max_depth > 32) return fail(X2_ERR_ARGUMENT, "the adjoint supports depth limits up to 32");
     if (d->volDesc.table) return fail(X2_ERR_ARGUMENT, "the adjoint does not differentiate through a medium");
+    if (c->direct_only) return fail(X2_ERR_ARGUMENT, "the adjoint replays the path tracer, not the prototype estimator");
     X2_HIP(hipSetDevice(d->device));
     d->cfg = *c;
     d->itemsPerChunk = spc * TilePixels;

@@ -90,6 +90,7 @@ class Config(ctypes.Structure):
         ("max_volume_depth", u32), ("volume_illumination_samples", u32), ("volume_transmittance_threshold", f32),
         ("inv_volume_quality", f32), ("volume_attenuation_factor", f32), ("volume_contribution_factor", f32),
         ("volume_phase_attenuation_factor", f32), ("pad_volume", u32),
+        ("direct_only", u32), ("surface_limit", u32), ("sky", f32 * 3), ("pad_direct", u32),
     ]
 
 
@@ -135,7 +136,7 @@ class BatchStats(ctypes.Structure):
 EXPECTED_SIZES = {
     "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
-    "x2_config": (Config, 268),
+    "x2_config": (Config, 292),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 16), "x2_frame_stats": (FrameStats, 80),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48),
 }

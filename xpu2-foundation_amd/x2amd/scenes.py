@@ -203,6 +203,9 @@ def make_config(width, height, spp, eye, target, up=(0, 1, 0), fov=40.0, light_c
     c.volume_attenuation_factor = 0.65
     c.volume_contribution_factor = 0.65
     c.volume_phase_attenuation_factor = 0.5
+    c.direct_only = 0  # the path tracer; 1: the prototype estimator (integrateDirect)
+    c.surface_limit = 1
+    c.sky[:] = [0.0, 0.0, 0.0]
     return c
 
 
