@@ -351,6 +351,8 @@ def main():
                                f"{config.spp} spp, depth {config.max_depth}, 3 rect + env lights"},
         "mrays_per_s": rays / wall / 1e6,
         "rays_per_path": rays / paths,
+        "nodes_per_ray": totals["nodes"] / max(1, rays),
+        "triangles_per_ray": totals["tris"] / max(1, rays),
         "prepare_s": prepare_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -43,9 +43,15 @@ struct BinNode {
     uint32_t left = 0, right = 0, first = 0, count = 0;  // count > 0: leaf
 };
 
+#ifndef X2_MIN_LEAF
+#define X2_MIN_LEAF 4
+#endif
+#ifndef X2_TRAVERSAL_COST
+#define X2_TRAVERSAL_COST 1.0f
+#endif
 constexpr int Bins = 32;
-constexpr uint32_t MinLeaf = 4;
-constexpr float TraversalCost = 1.0f, IntersectCost = 1.0f;
+constexpr uint32_t MinLeaf = X2_MIN_LEAF;  // a leaf at or below this many triangles; the SAH decides up to 8
+constexpr float TraversalCost = X2_TRAVERSAL_COST, IntersectCost = 1.0f;
 
 struct Builder {
     const std::vector<Aabb>& boxes;
