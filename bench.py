@@ -3,7 +3,8 @@
 Workload (the headline configuration): 2,621,442 triangles (32 fBm-displaced level-6 icospheres over a ground quad),
 50/50 Lambert + GGX materials, 3 rectangle lights + a constant environment, 1920x1080 at 64 spp, depth 12. A step is
 one uniform frame through the broker counterpart (x2dev_render_frame): 132,710,400 paths, chunks launched at most two
-in flight, the film downloaded and every pixel's count audited. The scene is resident in HBM before timing starts.
+in flight (up to LAUNCH_CHUNKS chunks a launch: fewer, larger launches leave fewer idle lanes at launch ends), the
+film downloaded and every pixel's count audited. The scene is resident in HBM before timing starts.
 
 Prints one JSON line: Mpaths/s (value), Mrays/s, the roofline of the render kernel (algorithmic bytes: 256 B per
 BVH8 node fetched + 48 B per triangle record tested + 72 B of film per path, over the kernel's own HIP-event time)
@@ -24,6 +25,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "xpu2-foundation_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+LAUNCH_CHUNKS = 131072  # max_chunks_per_launch of every leg's lane (the reference's default is 16384)
+
+
+def make_lane(wave_slots=0):
+    from x2amd.lane import GpuLane
+    return GpuLane(0, max_chunks_per_launch=LAUNCH_CHUNKS, ring_chunks=2 * LAUNCH_CHUNKS, wave_slots=wave_slots)
 
 
 def cpu_baseline(scene, width, height, spp, seconds_hint):
@@ -54,10 +63,9 @@ def adjoint_leg(reps):
     import numpy as np
     from x2amd import abi, scenes
     from x2amd.hipmem import DeviceBuffer
-    from x2amd.lane import GpuLane
 
     scene, config, params = scenes.cornell_textured()
-    lane = GpuLane(0)
+    lane = make_lane()
     lane.prepare(scene)
     rng = np.random.default_rng(1234)
     lane.set_texture(0, np.clip(params["texture"] + rng.normal(0, 0.1, params["texture"].shape), 0.01, 1.0))
@@ -119,10 +127,9 @@ def volume_leg(reps, cpu):
     occupied, extinction scale 4, HG g = 0.6, blackbody emission to 1500 K) over a ground quad under a rectangle light
     and a constant environment, 1280x720 at 32 spp, depth 16, volume depth 2; `reps` frames after a warm-up."""
     from x2amd import scenes
-    from x2amd.lane import GpuLane
 
     scene, config = scenes.clouds()
-    lane = GpuLane(0)
+    lane = make_lane()
     lane.prepare(scene)
     lane.render_frame(config)
     wall, kernel, rays = 0.0, 0.0, 0
@@ -201,12 +208,11 @@ def direct_leg(reps):
     sky of 0.2) on the headline scene at 1920x1080 x 64 spp; the reference quotes its GPU lane rate on this
     estimator (400-530 M samples/s on other hardware and scenes, ASYNC_RESULTS.md:58)."""
     from x2amd import scenes
-    from x2amd.lane import GpuLane
 
     scene, config = scenes.icospheres()
     config.direct_only, config.surface_limit = 1, 1
     config.sky[:] = [0.2, 0.2, 0.2]
-    lane = GpuLane(0)
+    lane = make_lane()
     lane.prepare(scene)
     lane.render_frame(config)
     wall, rays = 0.0, 0
@@ -275,6 +281,7 @@ def traffic_per_path():
 
 
 def main():
+    global LAUNCH_CHUNKS
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -292,7 +299,11 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB: only run a device copy (for --pmc scaling)")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--launch-chunks", type=int, default=LAUNCH_CHUNKS, help="max_chunks_per_launch of the lanes")
+    ap.add_argument("--wave-slots", type=int, default=0,
+                    help="render the headline in the wavefront form with this many path slots (0: one kernel)")
     args = ap.parse_args()
+    LAUNCH_CHUNKS = args.launch_chunks
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -302,10 +313,9 @@ def main():
         sys.exit(2)
 
     from x2amd import scenes
-    from x2amd.lane import GpuLane
 
     scene, config = scenes.icospheres(width=args.width, height=args.height, spp=args.spp)
-    lane = GpuLane(0)
+    lane = make_lane(args.wave_slots)
     t = time.perf_counter()
     lane.prepare(scene)
     prepare_s = time.perf_counter() - t
@@ -354,6 +364,8 @@ def main():
         "nodes_per_ray": totals["nodes"] / max(1, rays),
         "triangles_per_ray": totals["tris"] / max(1, rays),
         "prepare_s": prepare_s,
+        "launch": {"max_chunks_per_launch": LAUNCH_CHUNKS, "launches_per_frame": totals["launches"] / args.steps,
+                   "form": f"wavefront, {args.wave_slots} path slots" if args.wave_slots else "one kernel"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "renderKernel", "avg_launch_ms": per_launch_s * 1e3,

@@ -180,6 +180,10 @@ typedef struct x2dev_options {
     uint32_t max_chunks_per_launch; /* 16384 in the reference (async_gpu.h:43) */
     uint32_t ring_chunks;           /* mailbox ring capacity, in chunks (2^17 in the reference) */
     uint32_t counters;              /* nonzero: count rays, nodes and triangles per launch */
+    uint32_t wave_slots;            /* 0: the surface estimator runs as one kernel with path regeneration (the
+                                       default); else the wavefront form with this many path slots (a multiple of
+                                       256): shading and traversal in separate kernels. The same samples either way.
+                                       No reference counterpart (the reference runs one OptiX kernel). */
 } x2dev_options;
 
 typedef struct x2dev x2dev;

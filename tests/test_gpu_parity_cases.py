@@ -2,7 +2,8 @@
 oracle's bit for bit (per-sample radiance is computed by the same float expressions in the same order), with
 identical ray counts: ragged image sizes and a partial viewport, several light and BSDF samples (stratified
 first-hit sets), smart clamping, Russian roulette from the first bounce and none at all, hemispherical and
-two-sided/reverse-sided lights, emissive and GGX-only materials, direct lighting only."""
+two-sided/reverse-sided lights, emissive and GGX-only materials, direct lighting only. Every case runs twice: with
+the surface estimator as one kernel, and in the wavefront form (shading and traversal in separate kernels)."""
 import math
 
 import numpy as np
@@ -13,10 +14,19 @@ from x2amd import abi, scenes
 from x2amd.lane import GpuLane
 
 pytestmark = pytest.mark.gpu
+FORM = {"wave_slots": 0}
+
+
+@pytest.fixture(autouse=True, params=[0, 512], ids=["one_kernel", "wavefront"])
+def form(request):
+    """Runs each case in both forms of the surface estimator (wave_slots 0: one kernel)."""
+    FORM["wave_slots"] = request.param
+    yield request.param
+    FORM["wave_slots"] = 0
 
 
 def render_both(scene, config):
-    lane = GpuLane(0)
+    lane = GpuLane(0, wave_slots=FORM["wave_slots"])
     try:
         lane.prepare(scene)
         sums, counts, st = lane.render_frame(config, samples_per_chunk=1)

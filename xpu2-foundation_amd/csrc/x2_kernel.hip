@@ -787,6 +787,7 @@ struct Pending {
     V3 held;
 };
 constexpr uint32_t MaxPending = 32, PendingReserve = 16;
+static_assert(sizeof(Pending) == PendingBytes && MaxPending == MaxPendingRecords, "the slot arrays' pending records");
 __device__ __forceinline__ float bounceThinning(uint32_t top, uint32_t planned) {
     const uint32_t room = top < PendingReserve ? PendingReserve - top : 1u;
     return planned > room ? float(room) / float(planned) : 1.0f;
@@ -1347,8 +1348,8 @@ struct PathState {
 };
 
 // Starts sample (pixel, index): its camera ray becomes the one pending ray. False: an invalid value (Error).
-template <bool Adj, bool Vol>
-__device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixel, uint32_t index, PathState<Adj>& S) {
+template <bool Adj, bool Vol, class St>
+__device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixel, uint32_t index, St& S) {
     const x2_config& c = P.cfg;
     const Identity root{pixel, index, c.seed, 0};
     float x, y;
@@ -1387,37 +1388,35 @@ __device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixe
     return true;
 }
 
-// Shades the sample's last pending ray: traces it, adds the light found there and pushes the rays it spawns. False:
-// an invalid value (Error; the sample adds nothing). With Adj every contribution's derivative is carried to its
-// sinks, scaled by the pixel's adjoint (gr: the lane's roughness gradient accumulators).
-template <bool Adj, bool Vol>
-__device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixel, uint32_t index, PathState<Adj>& S,
-                                            Counters& k, TravStack& st, float* gr) {
+// How a vertex's shadow rays are answered: InlineTrace traces each one where it is made; the wavefront form queues
+// it with the contribution it would add (WaveQueue) and adds that once the ray is traced.
+struct InlineTrace {
+    __device__ __forceinline__ bool shadow(const RenderParams& P, V3 o, V3 d, float nearT, float farT, V3, Counters& k,
+                                           TravStack& st) {
+        return visible(P, o, d, nearT, farT, k, st);
+    }
+};
+
+// Shades a traced vertex (pv, along ray, found hit): adds the light found there and pushes the rays it spawns onto
+// the sample's pending stack. False: an invalid value (Error; the sample adds nothing). With Adj every
+// contribution's derivative is carried to its sinks, scaled by the pixel's adjoint (gr: the lane's roughness
+// gradient accumulators). St is the sample's state (PathState, or a wavefront path slot), Tr answers shadow rays:
+// a shadow ray's contribution is added here only when Tr.shadow says it is unoccluded.
+template <bool Adj, bool Vol, class St, class Tr>
+__device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, uint32_t index, St& S, Tr& tracer,
+                                         const Pending& pv, const Ray& ray, const Hit& hit, Counters& k, TravStack& st,
+                                         float* gr) {
     const x2_config& c = P.cfg;
     const uint32_t rrMin = c.rr_min_depth > 1 ? c.rr_min_depth : 1u;
     const Identity root{pixel, index, c.seed, 0};
-    Pending* const stack = S.stack;
+    auto& stack = S.stack;
     uint32_t& top = S.top;
     V3& radiance = S.radiance;
     float& alpha = S.alpha;
     const V3& delta = S.delta;
-    Sink* const sinks = S.sinks;
-    Record* const recs = S.recs;
-    uint32_t& nrec = S.nrec;
-    const Pending pv = stack[--top];
-    if constexpr (Adj) {
-        while (nrec && recs[nrec - 1].pos > top) {
-            --nrec;
-            scatterSink(P, recs[nrec].sink, radiance - recs[nrec].mark, delta, gr);
-        }
-        if (nrec < MaxRecords) recs[nrec++] = Record{top, radiance, sinks[top]};
-    }
+    auto& sinks = S.sinks;
     const uint32_t depth = pv.depths & 0xffu, nonMirrorDepth = (pv.depths >> 8) & 0xffu,
                    diffuseDepth = (pv.depths >> 16) & 0xffu, glossyDepth = pv.depths >> 24;
-    const bool fromCamera = depth == 0;
-    const Ray ray{pv.origin, pv.dir, fromCamera ? S.cameraNear : 0.0f, fromCamera ? S.cameraFar : MaxFloat};
-    Hit hit;
-    if (!traceHit(P, ray, hit, k, st)) return false;
     // With a medium there is no bounce-ray reuse (:2919), and heldLight holds the path's volume depth.
     uint32_t volumeDepth = Vol ? pv.heldLight : 0u;
     if (!Vol && pv.heldLight != ~0u) {
@@ -1576,7 +1575,7 @@ __device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixe
             if (!finite3(t) || isExactlyZero(t)) continue;
             const V3 shadowOrigin = dot(wi, Ng) >= 0.0f ? above : below;
             const float shadowNear = 0.0f, shadowFar = fmaxf((lh.distance - 0.0f) * HitEpsilonEnd, 2.0f * shadowNear);
-            if (visible(P, shadowOrigin, wi, shadowNear, shadowFar, k, st)) {
+            if (tracer.shadow(P, shadowOrigin, wi, shadowNear, shadowFar, t, k, st)) {
                 if constexpr (Vol) {  // the medium on the shadow ray's way (:2725-2732)
                     Identity tid = lid;
                     tid.branch = childBranch(lightAddress, s, 0xfffb);
@@ -1665,7 +1664,7 @@ __device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixe
                 shadowFar = fmaxf((lightDistance - 0.0f) * HitEpsilonEnd, 2.0f * 0.0f);
                 held = !Vol && bounces && thinning >= 1.0f && top + PendingReserve < MaxPending && lobeMask &&
                        (c.shadow_reuse & lobeMask) == lobeMask;
-                if (!held) kept = visible(P, origin, wi, 0.0f, shadowFar, k, st) ? 1.0f : 0.0f;
+                if (!held) kept = tracer.shadow(P, origin, wi, 0.0f, shadowFar, direct, k, st) ? 1.0f : 0.0f;
             }
             const V3 directBefore = direct;
             Sink lobeSink;
@@ -1720,6 +1719,26 @@ __device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixe
         }
     }
     return true;
+}
+
+// Shades the sample's last pending ray: traces it, then shadeHit with shadow rays traced inline.
+template <bool Adj, bool Vol, class St>
+__device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixel, uint32_t index, St& S, Counters& k,
+                                            TravStack& st, float* gr) {
+    const Pending pv = S.stack[--S.top];
+    if constexpr (Adj) {
+        while (S.nrec && S.recs[S.nrec - 1].pos > S.top) {
+            --S.nrec;
+            scatterSink(P, S.recs[S.nrec].sink, S.radiance - S.recs[S.nrec].mark, S.delta, gr);
+        }
+        if (S.nrec < MaxRecords) S.recs[S.nrec++] = Record{S.top, S.radiance, S.sinks[S.top]};
+    }
+    const bool fromCamera = (pv.depths & 0xffu) == 0;
+    const Ray ray{pv.origin, pv.dir, fromCamera ? S.cameraNear : 0.0f, fromCamera ? S.cameraFar : MaxFloat};
+    Hit hit;
+    if (!traceHit(P, ray, hit, k, st)) return false;
+    InlineTrace tracer;
+    return shadeHit<Adj, Vol>(P, pixel, index, S, tracer, pv, ray, hit, k, st, gr);
 }
 
 // Closes a finished sample's open subtrees; with Adj each carries its share of the radiance to its sink.
@@ -1944,34 +1963,38 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
 #ifndef X2_REGEN_LANES
 #define X2_REGEN_LANES 32
 #endif
+// The launch's next sample: claims items until one lies in the image and the viewport (false: none left).
+__device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel, uint32_t& index) {
+    for (;;) {
+        const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
+        if (item >= P.itemEnd) return false;
+        const unsigned long long chunkIndex = item / P.itemsPerChunk;
+        const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
+        const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
+        const uint32_t s = offset / TilePixels, position = offset % TilePixels;
+        pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
+        if (pixel == ~0u) continue;
+        index = chunk.first + s;
+        return true;
+    }
+}
+// A finished sample into the film (async_gpu.cu:331-336).
+__device__ __forceinline__ void filmAdd(const RenderParams& P, uint32_t pixel, V3 r, float a) {
+    double* f = P.sums + size_t(pixel) * 4;
+    atomicAdd(f + 0, double(r.x));
+    atomicAdd(f + 1, double(r.y));
+    atomicAdd(f + 2, double(r.z));
+    atomicAdd(f + 3, double(a));
+    atomicAdd(P.counts + pixel, 1u);
+}
 template <bool Adj>
 struct LaneFeed {
     static constexpr uint32_t Regen = X2_REGEN_LANES;  // finished lanes of a wave that take new samples together
     const RenderParams& P;
     uint32_t& errors;
-    __device__ bool next(uint32_t& pixel, uint32_t& index) {
-        for (;;) {
-            const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
-            if (item >= P.itemEnd) return false;
-            const unsigned long long chunkIndex = item / P.itemsPerChunk;
-            const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
-            const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
-            const uint32_t s = offset / TilePixels, position = offset % TilePixels;
-            pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
-            if (pixel == ~0u) continue;
-            index = chunk.first + s;
-            return true;
-        }
-    }
+    __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem(P, pixel, index); }
     __device__ void done(uint32_t pixel, V3 r, float a) {
-        if constexpr (!Adj) {
-            double* f = P.sums + size_t(pixel) * 4;
-            atomicAdd(f + 0, double(r.x));
-            atomicAdd(f + 1, double(r.y));
-            atomicAdd(f + 2, double(r.z));
-            atomicAdd(f + 3, double(a));
-            atomicAdd(P.counts + pixel, 1u);
-        }
+        if constexpr (!Adj) filmAdd(P, pixel, r, a);
     }
     __device__ void failed() { ++errors; }
 };
@@ -2035,6 +2058,338 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderPar
 __global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) {
     runLaunch<true, false, false, X2_PERSISTENT>(P);
 }
+// ---------------------------------------------------------------------------------------------------------------
+// The wavefront form of the surface estimator: the same samples, vertex for vertex, with the traversal taken out of
+// the shading kernel so that each kind of work runs at its own occupancy. An iteration is two launches.
+// waveShadeKernel, one thread a path slot: adds the unoccluded contributions of the slot's last vertex's shadow rays,
+// shades the vertex whose closest hit the last trace pass found (its shadow rays become queue entries, each with the
+// contribution it adds), then moves the slot's next pending ray into the closest-hit queue — finishing the sample
+// into the film and starting the launch's next item when the stack is empty. waveTraceKernel traces both queues.
+// Contributions are added in the order the per-lane loop adds them, so every sample's radiance is bit for bit the
+// per-lane loop's. waveDrainKernel ends a launch: it finishes whatever the iterations left (slots still busy and
+// unclaimed items) with the per-lane loop, so a launch is complete whatever the number of iterations.
+
+// A slot's pending records in the slot arrays: level i at base[i * slots] (base: the slot's level 0).
+struct LevelStack {
+    Pending* base;
+    uint32_t slots;
+    __device__ __forceinline__ Pending& operator[](uint32_t i) const { return base[size_t(i) * slots]; }
+};
+// A path slot's sample while a pass works on it: loaded from the slot arrays, stored back at the pass's end.
+struct WaveSlot {
+    LevelStack stack;
+    uint32_t top;
+    V3 radiance;
+    float alpha, cameraNear, cameraFar;
+    V3 delta;  // the adjoint's (unused here)
+    Sink sinks[1];
+    Record recs[1];
+    uint32_t nrec;
+};
+__device__ __forceinline__ void loadSlot(const WaveParams& W, uint32_t slot, WaveSlot& S, uint32_t& pixel,
+                                         uint32_t& index) {
+    const uint2 id = reinterpret_cast<const uint2*>(W.sample)[slot];
+    pixel = id.x;
+    index = id.y;
+    S.top = W.top[slot];
+    const float4* a = reinterpret_cast<const float4*>(W.accum) + size_t(slot) * 2;
+    const float4 a0 = a[0], a1 = a[1];
+    S.radiance = V3{a0.x, a0.y, a0.z};
+    S.alpha = a0.w;
+    S.cameraNear = a1.x;
+    S.cameraFar = a1.y;
+}
+__device__ __forceinline__ void storeSlot(const WaveParams& W, uint32_t slot, const WaveSlot& S, uint32_t pixel,
+                                          uint32_t index) {
+    reinterpret_cast<uint2*>(W.sample)[slot] = make_uint2(pixel, index);
+    W.top[slot] = S.top;
+    float4* a = reinterpret_cast<float4*>(W.accum) + size_t(slot) * 2;
+    a[0] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, S.alpha);
+    a[1] = make_float4(S.cameraNear, S.cameraFar, 0.0f, 0.0f);
+}
+// Shadow rays as queue entries at the slot's positions: the ray and the contribution it adds when unoccluded. A
+// vertex makes at most light_count x light_samples + 2 x bsdf_samples of them (W.perVertex).
+struct WaveQueue {
+    const WaveParams& W;
+    uint32_t parity, slot, n;
+    __device__ __forceinline__ bool shadow(const RenderParams&, V3 o, V3 d, float nearT, float farT, V3 value, Counters&,
+                                           TravStack&) {
+        if (n >= W.perVertex) return false;  // beyond the bound the queues are sized for: never
+        const size_t e = size_t(slot) * W.perVertex + n++;
+        float4* r = reinterpret_cast<float4*>(W.shadowRays[parity]) + e * 2;
+        r[0] = make_float4(o.x, o.y, o.z, nearT);
+        r[1] = make_float4(d.x, d.y, d.z, farT);
+        reinterpret_cast<float4*>(W.shadowValue[parity])[e] = make_float4(value.x, value.y, value.z, 0.0f);
+        return false;
+    }
+};
+// The launch still has unclaimed items.
+__device__ __forceinline__ bool feedOpen(const RenderParams& P) {
+    const unsigned long long claimed = *reinterpret_cast<volatile unsigned long long*>(&P.state->claimed);
+    return P.itemBegin + claimed < P.itemEnd;
+}
+// An exclusive prefix sum over the wave's 64 lanes (every lane active); total = the wave's sum.
+__device__ __forceinline__ uint32_t waveExclusive(uint32_t v, uint32_t& total) {
+    uint32_t x = v;
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= uint32_t(o)) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// The slot's traced vertex: its closest hit (found by the trace pass) reconstructed and shaded with Tr answering
+// the shadow rays. False: an invalid value (Error).
+template <class Tr>
+__device__ __forceinline__ bool shadeTraced(const RenderParams& P, const WaveParams& W, uint32_t slot, uint32_t pixel,
+                                            uint32_t index, WaveSlot& S, Tr& tracer, Counters& k, TravStack& st) {
+    const Pending pv = static_cast<const Pending*>(W.current)[slot];
+    const bool fromCamera = (pv.depths & 0xffu) == 0;
+    const Ray ray{pv.origin, pv.dir, fromCamera ? S.cameraNear : 0.0f, fromCamera ? S.cameraFar : MaxFloat};
+    Hit hit;
+    hit.geom = ~0u;
+    const uint32_t tri = W.hitTri[slot];
+    if (tri != ~0u) {
+        const TriRecord& r = P.tris[tri];
+        hit.geom = r.geom;
+        hit.tri = tri;
+        if (!geometryHit(hit, ray, V3{r.ax, r.ay, r.az}, V3{r.bx, r.by, r.bz}, V3{r.cx, r.cy, r.cz})) return false;
+    }
+    float gr[1] = {0.0f};
+    return shadeHit<false, false>(P, pixel, index, S, tracer, pv, ray, hit, k, st, gr);
+}
+// Adds the unoccluded contributions of the slot's queued shadow rays (queue parity q), in the order they were made.
+__device__ __forceinline__ void resolveShadows(const WaveParams& W, uint32_t slot, uint32_t q, WaveSlot& S) {
+    const uint32_t n = W.shadowCount[slot];
+    for (uint32_t j = 0; j < n; ++j) {
+        const size_t e = size_t(slot) * W.perVertex + j;
+        if (W.shadowVis[q][e]) {
+            const float4 v = reinterpret_cast<const float4*>(W.shadowValue[q])[e];
+            S.radiance = S.radiance + V3{v.x, v.y, v.z};
+        }
+    }
+}
+
+#ifndef X2_SHADE_WAVES
+#define X2_SHADE_WAVES X2_WAVES_PER_EU
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(X2_SHADE_WAVES)))
+waveShadeKernel(const RenderParams P, const WaveParams W, uint32_t p) {
+    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;  // slots come in whole workgroups
+    uint32_t errors = 0, queuedRay = 0, queuedShadows = 0;
+    {
+        uint32_t fl = W.flags[slot];
+        if (fl || feedOpen(P)) {
+            WaveSlot S;
+            S.stack = LevelStack{static_cast<Pending*>(W.stack) + slot, W.slots};
+            S.top = 0;
+            uint32_t pixel = 0, index = 0;
+            if (fl & WaveLive) loadSlot(W, slot, S, pixel, index);
+            if (fl & WaveShadows) {
+                resolveShadows(W, slot, p ^ 1u, S);
+                fl &= ~WaveShadows;
+            }
+            if (fl & WaveTraced) {
+                fl &= ~WaveTraced;
+                Counters k;
+                TravStack st;
+                st.lds = nullptr;
+                st.lane = threadIdx.x;
+                WaveQueue queue{W, p, slot, 0};
+                if (!shadeTraced(P, W, slot, pixel, index, S, queue, k, st)) {
+                    ++errors;
+                    fl = 0;
+                    S.top = 0;
+                } else if (queue.n) {
+                    W.shadowCount[slot] = queue.n;
+                    queuedShadows = queue.n;
+                    fl |= WaveShadows;
+                }
+            }
+            for (;;) {
+                if (fl & WaveLive) {
+                    if (S.top) {  // the next pending ray into the closest-hit queue
+                        const Pending pv = S.stack[--S.top];
+                        static_cast<Pending*>(W.current)[slot] = pv;
+                        const bool fromCamera = (pv.depths & 0xffu) == 0;
+                        float4* ray = reinterpret_cast<float4*>(W.rays[p]) + size_t(slot) * 2;
+                        ray[0] = make_float4(pv.origin.x, pv.origin.y, pv.origin.z, fromCamera ? S.cameraNear : 0.0f);
+                        ray[1] = make_float4(pv.dir.x, pv.dir.y, pv.dir.z, fromCamera ? S.cameraFar : MaxFloat);
+                        queuedRay = 1;
+                        fl |= WaveTraced;
+                        break;
+                    }
+                    if (fl & WaveShadows) break;  // the last vertex's shadow rays are still being traced
+                    filmAdd(P, pixel, S.radiance, S.alpha);
+                    fl = 0;
+                }
+                if (!claimItem(P, pixel, index)) break;
+                if (!startSample<false, false>(P, pixel, index, S)) {
+                    ++errors;
+                    continue;
+                }
+                fl = WaveLive;
+            }
+            W.flags[slot] = fl;
+            if (fl) storeSlot(W, slot, S, pixel, index);
+        }
+    }
+    if (errors) atomicAdd(&P.state->errors, errors);
+    // The workgroup's queued rays into the compacted lists: one atomic per list and workgroup.
+    __shared__ uint32_t waveRays[4], waveShadows[4], base[2];
+    uint32_t totalRays, totalShadows;
+    const uint32_t atRay = waveExclusive(queuedRay, totalRays), atShadow = waveExclusive(queuedShadows, totalShadows);
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        waveRays[wave] = totalRays;
+        waveShadows[wave] = totalShadows;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0, sh = 0;
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t a = waveRays[w], b = waveShadows[w];
+            waveRays[w] = r;
+            waveShadows[w] = sh;
+            r += a;
+            sh += b;
+        }
+        base[0] = r ? atomicAdd(&W.counts->rays[p], r) : 0u;
+        base[1] = sh ? atomicAdd(&W.counts->shadows[p], sh) : 0u;
+    }
+    __syncthreads();
+    if (queuedRay) W.rayList[p][base[0] + waveRays[wave] + atRay] = slot;
+    for (uint32_t j = 0; j < queuedShadows; ++j)
+        W.shadowList[p][base[1] + waveShadows[wave] + atShadow + j] = slot * W.perVertex + j;
+}
+
+#ifndef X2_TRACE_WAVES
+#define X2_TRACE_WAVES 6
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(X2_TRACE_WAVES)))
+waveTraceKernel(const RenderParams P, const WaveParams W, uint32_t p) {
+    __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
+    TravStack st;
+    st.lds = lds;
+    st.lane = threadIdx.x;
+    Counters k;
+    const uint32_t nr = W.counts->rays[p], ns = W.counts->shadows[p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the other parity's lists are free: the next shading pass fills them
+        W.counts->rays[p ^ 1u] = 0;
+        W.counts->shadows[p ^ 1u] = 0;
+        W.counts->traced += nr;
+    }
+    const uint32_t total = nr + ns, stride = gridDim.x * 256u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < total; i += stride) {
+        if (i < nr) {
+            const uint32_t slot = W.rayList[p][i];
+            const float4* r = reinterpret_cast<const float4*>(W.rays[p]) + size_t(slot) * 2;
+            const float4 a = r[0], b = r[1];
+            float t;
+            ++k.traces;
+            W.hitTri[slot] = closestHit(P, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, t, k, st);
+        } else {
+            const uint32_t e = W.shadowList[p][i - nr];
+            const float4* r = reinterpret_cast<const float4*>(W.shadowRays[p]) + size_t(e) * 2;
+            const float4 a = r[0], b = r[1];
+            ++k.shadows;
+            W.shadowVis[p][e] = anyHit(P, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, k, st) ? 0u : 1u;
+        }
+    }
+    if (P.counters && blockIdx.x < W.partialBlocks) {  // per-workgroup partial sums, added up at the launch's end
+        __shared__ unsigned long long sums[4][4];
+        const unsigned long long v[4] = {waveSum(k.traces), waveSum(k.shadows), waveSum(k.nodes), waveSum(k.triangles)};
+        if ((threadIdx.x & 63u) == 0)
+            for (int c = 0; c < 4; ++c) sums[threadIdx.x >> 6][c] = v[c];
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            unsigned long long* q = W.partials + size_t(blockIdx.x) * 4 + threadIdx.x;
+            *q += sums[0][threadIdx.x] + sums[1][threadIdx.x] + sums[2][threadIdx.x] + sums[3][threadIdx.x];
+        }
+    }
+}
+
+// The launch's end (parity p: the last trace pass's queues): each busy slot's sample is finished with shadow rays
+// traced inline, then the lane takes the launch's remaining items with the per-lane loop.
+__global__ void __launch_bounds__(256) X2_OCCUPANCY waveDrainKernel(const RenderParams P, const WaveParams W,
+                                                                     uint32_t p) {
+    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;  // every lane reaches the end (the counter sums)
+    const uint32_t fl = slot < W.slots ? W.flags[slot] : 0u;
+    __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
+    TravStack st;
+    st.lds = lds;
+    st.lane = threadIdx.x;
+    Counters k;
+    uint32_t errors = 0;
+    if (fl) {
+        WaveSlot S;
+        S.stack = LevelStack{static_cast<Pending*>(W.stack) + slot, W.slots};
+        uint32_t pixel, index;
+        loadSlot(W, slot, S, pixel, index);
+        if (fl & WaveShadows) resolveShadows(W, slot, p, S);
+        float gr[1] = {0.0f};
+        bool ok = true;
+        if (fl & WaveTraced) {
+            InlineTrace inl;
+            ok = shadeTraced(P, W, slot, pixel, index, S, inl, k, st);
+        }
+        while (ok && S.top) ok = shadeVertex<false, false>(P, pixel, index, S, k, st, gr);
+        if (ok) filmAdd(P, pixel, S.radiance, S.alpha);
+        else ++errors;
+        W.flags[slot] = 0;
+    }
+    if (slot < W.slots && feedOpen(P)) {
+        float gr[1] = {0.0f};
+        LaneFeed<false> feed{P, errors};
+        runSamples<false, false>(P, feed, k, st, gr);
+    }
+    if (errors) atomicAdd(&P.state->errors, errors);
+    const unsigned long long drained = waveSum(k.traces);
+    if ((threadIdx.x & 63u) == 0 && drained) atomicAdd(&W.counts->traced, drained);
+    if (P.counters) {
+        const unsigned long long t = waveSum(k.traces), sh = waveSum(k.shadows), n = waveSum(k.nodes),
+                                 tr = waveSum(k.triangles);
+        if ((threadIdx.x & 63u) == 0) {
+            atomicAdd(&P.state->traces, t);
+            atomicAdd(&P.state->shadows, sh);
+            atomicAdd(&P.state->nodes, n);
+            atomicAdd(&P.state->triangles, tr);
+        }
+    }
+}
+
+__global__ void waveBeginKernel(const WaveParams W) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        W.counts->rays[0] = W.counts->rays[1] = 0;
+        W.counts->shadows[0] = W.counts->shadows[1] = 0;
+        W.counts->traced = 0;
+    }
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < W.partialBlocks * 4u; i += gridDim.x * 256u) W.partials[i] = 0;
+}
+// The trace passes' per-workgroup counters into the launch's totals (one workgroup).
+__global__ void waveEndKernel(const RenderParams P, const WaveParams W) {
+    __shared__ unsigned long long sums[256][4];
+    unsigned long long v[4] = {0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < W.partialBlocks; b += 256u)
+        for (int c = 0; c < 4; ++c) v[c] += W.partials[size_t(b) * 4 + c];
+    for (int c = 0; c < 4; ++c) sums[threadIdx.x][c] = v[c];
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int c = 0; c < 4; ++c) sums[threadIdx.x][c] += sums[threadIdx.x + h][c];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && P.counters) {
+        atomicAdd(&P.state->traces, sums[0][0]);
+        atomicAdd(&P.state->shadows, sums[0][1]);
+        atomicAdd(&P.state->nodes, sums[0][2]);
+        atomicAdd(&P.state->triangles, sums[0][3]);
+    }
+}
+
 // A scene with a medium: the same estimator with the medium's light and transmittance on every ray.
 #ifndef X2_VOLUME_WAVES
 #define X2_VOLUME_WAVES 2  // 256 VGPRs, no accumulation registers: 2 waves/SIMD (1 without the bound)
@@ -2195,6 +2550,28 @@ hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     return p.vol.table ? launchFrame<2>(p, stream) : launchFrame<0>(p, stream);
 }
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }
+// The wavefront form over one launch's items: its counters cleared, `iterations` shading and trace passes, the drain.
+hipError_t launchWave(const RenderParams& p, const WaveParams& w, uint32_t iterations, hipStream_t stream) {
+    static int traceBlocks = 0;  // the trace pass is persistent: as many workgroups as the chip holds at once
+    if (!traceBlocks) {
+        int dev = 0, perCu = 0;
+        hipDeviceProp_t prop;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, waveTraceKernel, 256, 0);
+        if (e != hipSuccess) return e;
+        traceBlocks = std::max(1, perCu) * prop.multiProcessorCount;
+    }
+    const unsigned shadeBlocks = w.slots / 256u, blocks = unsigned(std::min<int>(traceBlocks, int(w.partialBlocks)));
+    hipLaunchKernelGGL(waveBeginKernel, dim3(16), dim3(256), 0, stream, w);
+    for (uint32_t i = 0; i < iterations; ++i) {
+        hipLaunchKernelGGL(waveShadeKernel, dim3(shadeBlocks), dim3(256), 0, stream, p, w, i & 1u);
+        hipLaunchKernelGGL(waveTraceKernel, dim3(blocks), dim3(256), 0, stream, p, w, i & 1u);
+    }
+    hipLaunchKernelGGL(waveDrainKernel, dim3(shadeBlocks), dim3(256), 0, stream, p, w, (iterations + 1u) & 1u);
+    hipLaunchKernelGGL(waveEndKernel, dim3(1), dim3(256), 0, stream, p, w);
+    return hipGetLastError();
+}
 hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream) {
     hipLaunchKernelGGL(resolveKernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, sums, counts, pixels, rgb);
     return hipGetLastError();

@@ -20,6 +20,7 @@
 namespace x2 {
 hipError_t launchRender(const RenderParams& p, hipStream_t stream);
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream);
+hipError_t launchWave(const RenderParams& p, const WaveParams& w, uint32_t iterations, hipStream_t stream);
 hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream);
 hipError_t launchClearState(LaunchState* s, hipStream_t stream);
 hipError_t uploadSobol(const uint32_t* table);
@@ -73,7 +74,10 @@ void sobolDirections(uint32_t out[17 * 32]) {
 struct Flight {
     uint64_t launch, first, count, samples;
     uint32_t slot;
+    bool wave;  // run in the wavefront form
 };
+
+
 }  // namespace
 
 namespace x2 {  // for the other host files (x2_batch.hip)
@@ -113,6 +117,11 @@ struct x2dev {
     x2_chunk* staging[Slots] = {};
     LaunchState* state = nullptr;
     LaunchState* readback = nullptr;
+    // the wavefront form: one allocation holding every slot array and queue, and each launch's counters read back
+    WaveParams wave{};
+    void* waveBlock = nullptr;
+    WaveCounts* waveReadback = nullptr;
+    double waveVertices = 0;  // closest-hit rays an item took in the last wavefront launches
     hipEvent_t begun[Slots] = {}, ended[Slots] = {}, done[Slots] = {};
     std::deque<Flight> inFlight;
     uint64_t published = 0, launches = 0;
@@ -146,6 +155,8 @@ void release(x2dev* d) {
     if (d->hostSums) (void)hipHostFree(d->hostSums);
     if (d->hostCounts) (void)hipHostFree(d->hostCounts);
     if (d->readback) (void)hipHostFree(d->readback);
+    (void)hipFree(d->waveBlock);
+    if (d->waveReadback) (void)hipHostFree(d->waveReadback);
     for (uint32_t i = 0; i < Slots; ++i) {
         if (d->staging[i]) (void)hipHostFree(d->staging[i]);
         if (d->begun[i]) (void)hipEventDestroy(d->begun[i]);
@@ -192,6 +203,11 @@ int x2dev_create(const x2dev_options* options, x2dev** out) {
     d->opt.ring_chunks = options && options->ring_chunks ? options->ring_chunks : (1u << 17);
     d->opt.counters = options ? options->counters : 1u;
     d->device = d->opt.device;
+    d->opt.wave_slots = options ? options->wave_slots : 0u;
+    if (d->opt.wave_slots % 256 || d->opt.wave_slots > (1u << 24)) {
+        delete d;
+        return fail(X2_ERR_ARGUMENT, "wave_slots must be a multiple of 256, at most 2^24");
+    }
     if (d->opt.ring_chunks < d->opt.max_chunks_per_launch) {
         delete d;
         return fail(X2_ERR_ARGUMENT, "ring_chunks must hold at least one launch");
@@ -220,6 +236,7 @@ int x2dev_create(const x2dev_options* options, x2dev** out) {
     X2_CREATE(hipMalloc(&d->ring, size_t(d->opt.ring_chunks) * sizeof(x2_chunk)));
     X2_CREATE(hipMalloc(&d->state, Slots * sizeof(LaunchState)));
     X2_CREATE(hipHostMalloc(&d->readback, Slots * sizeof(LaunchState), hipHostMallocDefault));
+    X2_CREATE(hipHostMalloc(&d->waveReadback, Slots * sizeof(WaveCounts), hipHostMallocDefault));
     for (uint32_t i = 0; i < Slots; ++i) {
         X2_CREATE(hipHostMalloc(&d->staging[i], size_t(d->opt.max_chunks_per_launch) * sizeof(x2_chunk), hipHostMallocDefault));
         X2_CREATE(hipEventCreate(&d->begun[i]));
@@ -387,6 +404,57 @@ uint32_t x2dev_in_flight(const x2dev* d) { return d ? uint32_t(d->inFlight.size(
 uint32_t x2dev_max_chunks_per_launch(const x2dev* d) { return d ? d->opt.max_chunks_per_launch : 0u; }
 uint32_t x2dev_launch_threads(const x2dev* d) { return d ? d->launchThreads : 0u; }
 
+namespace {
+// The surface estimator of a forward frame runs in the wavefront form when the build has path slots.
+bool waveForm(const x2dev* d) { return d->opt.wave_slots && !d->dImage && !d->volDesc.table && !d->cfg.direct_only; }
+// Allocates the slot arrays and queues for `perVertex` shadow rays a vertex (once; again if a frame needs more).
+int ensureWave(x2dev* d, uint32_t perVertex) {
+    if (d->waveBlock && d->wave.perVertex >= perVertex) return X2_OK;
+    X2_HIP(hipStreamSynchronize(d->stream));
+    (void)hipFree(d->waveBlock);
+    d->waveBlock = nullptr;
+    const size_t n = d->opt.wave_slots, e = n * perVertex;
+    constexpr uint32_t partialBlocks = 8192;  // trace-pass workgroups with their own counters (at most)
+    const size_t sizes[] = {
+        n * 4,  n * 8,  n * 4,  n * 32, size_t(MaxPendingRecords) * n * PendingBytes, n * PendingBytes, n * 4, n * 4,
+        n * 32, n * 32, n * 4,  n * 4,  e * 32, e * 32, e * 16, e * 16, e * 4, e * 4, e * 4, e * 4,
+        sizeof(WaveCounts), size_t(partialBlocks) * 4 * 8};
+    constexpr size_t parts = sizeof(sizes) / sizeof(sizes[0]);
+    size_t offsets[parts], total = 0;
+    for (size_t i = 0; i < parts; ++i) {
+        offsets[i] = total;
+        total += (sizes[i] + 255) & ~size_t(255);
+    }
+    X2_HIP(hipMalloc(&d->waveBlock, total));
+    char* b = static_cast<char*>(d->waveBlock);
+    WaveParams& w = d->wave;
+    w.slots = uint32_t(n);
+    w.perVertex = perVertex;
+    w.flags = reinterpret_cast<uint32_t*>(b + offsets[0]);
+    w.sample = reinterpret_cast<uint32_t*>(b + offsets[1]);
+    w.top = reinterpret_cast<uint32_t*>(b + offsets[2]);
+    w.accum = reinterpret_cast<float*>(b + offsets[3]);
+    w.stack = b + offsets[4];
+    w.current = b + offsets[5];
+    w.hitTri = reinterpret_cast<uint32_t*>(b + offsets[6]);
+    w.shadowCount = reinterpret_cast<uint32_t*>(b + offsets[7]);
+    for (int q = 0; q < 2; ++q) {
+        w.rays[q] = reinterpret_cast<float*>(b + offsets[8 + q]);
+        w.rayList[q] = reinterpret_cast<uint32_t*>(b + offsets[10 + q]);
+        w.shadowRays[q] = reinterpret_cast<float*>(b + offsets[12 + q]);
+        w.shadowValue[q] = reinterpret_cast<float*>(b + offsets[14 + q]);
+        w.shadowList[q] = reinterpret_cast<uint32_t*>(b + offsets[16 + q]);
+        w.shadowVis[q] = reinterpret_cast<uint32_t*>(b + offsets[18 + q]);
+    }
+    w.counts = reinterpret_cast<WaveCounts*>(b + offsets[20]);
+    w.partials = reinterpret_cast<unsigned long long*>(b + offsets[21]);
+    w.partialBlocks = partialBlocks;
+    w.pad = 0;
+    X2_HIP(hipMemsetAsync(w.flags, 0, n * 4, d->stream));  // every slot empty
+    return X2_OK;
+}
+}  // namespace
+
 int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samples) {
     if (!d) return fail(X2_ERR_ARGUMENT, "null device");
     if (!d->begun_) return fail(X2_ERR_STATE, "launch before begin");
@@ -433,12 +501,31 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     p.counters = d->opt.counters;
     p.pad = 0;
     p.vol = d->volDesc;
+    const bool wave = waveForm(d);
+    if (wave) {
+        const uint32_t perVertex = d->cfg.light_count * std::max(1u, d->cfg.light_samples) +
+                                   2u * std::max(1u, d->cfg.bsdf_samples);
+        const int r = ensureWave(d, perVertex);
+        if (r) return r;
+    }
     X2_HIP(hipEventRecord(d->begun[slot], d->stream));
-    X2_HIP(d->dImage ? launchAdjoint(p, d->stream) : launchRender(p, d->stream));
+    if (wave) {
+        // Iterations: each busy slot shades one vertex an iteration, plus one to finish a sample; the drain takes
+        // whatever is left, so the estimate only sets how much of the launch runs in the wavefront form.
+        const double perItem = (d->waveVertices > 0 ? d->waveVertices : 2.0) + 1.0;
+        const double want = double(count) * d->itemsPerChunk * perItem / double(d->wave.slots);
+        const uint32_t iterations = uint32_t(std::min(4096.0, std::ceil(want))) + 8u;
+        X2_HIP(launchWave(p, d->wave, iterations, d->stream));
+    } else {
+        X2_HIP(d->dImage ? launchAdjoint(p, d->stream) : launchRender(p, d->stream));
+    }
     X2_HIP(hipEventRecord(d->ended[slot], d->stream));
     X2_HIP(hipMemcpyAsync(d->readback + slot, d->state + slot, sizeof(LaunchState), hipMemcpyDeviceToHost, d->stream));
+    if (wave)
+        X2_HIP(hipMemcpyAsync(d->waveReadback + slot, d->wave.counts, sizeof(WaveCounts), hipMemcpyDeviceToHost,
+                              d->stream));
     X2_HIP(hipEventRecord(d->done[slot], d->stream));
-    d->inFlight.push_back(Flight{d->launches, d->published, count, samples, slot});
+    d->inFlight.push_back(Flight{d->launches, d->published, count, samples, slot, wave});
     d->published += count;
     ++d->launches;
     return 1;
@@ -453,6 +540,10 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
     if (q != hipSuccess) return hipFail(q, "hipEventQuery");
     d->inFlight.pop_front();
     const LaunchState& s = d->readback[f.slot];
+    if (f.wave) {
+        const double v = double(d->waveReadback[f.slot].traced) / double(f.count * uint64_t(d->itemsPerChunk));
+        d->waveVertices = d->waveVertices > 0 ? 0.5 * (d->waveVertices + v) : v;
+    }
     float ms = 0;
     X2_HIP(hipEventElapsedTime(&ms, d->begun[f.slot], d->ended[f.slot]));
     out->chunks = f.count;
@@ -598,7 +689,9 @@ int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
                         for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
                             chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
     }
-    const double windowMs = 10.0;
+    // A launch's device time once the rate is known: long enough that max_chunks_per_launch bounds it at the
+    // headline's rate (a launch's last samples leave lanes idle, so fewer, larger launches waste less).
+    const double windowMs = 100.0;
     const uint64_t firstChunks = 2048, cap = d->opt.max_chunks_per_launch;
     const uint64_t least = (uint64_t(d->launchThreads) * 2 + d->itemsPerChunk - 1) / d->itemsPerChunk;
     double rate = 0;  // items per device second, from completed launches

@@ -97,6 +97,40 @@ struct RenderParams {
     VolumeDesc vol;
 };
 
+// The wavefront form of the surface estimator (x2_kernel.hip waveShadeKernel / waveTraceKernel / waveDrainKernel):
+// `slots` path slots, each holding one sample between its vertices, in SoA arrays over the slots; and two sets of ray
+// queues (ping-pong by iteration parity). A shading pass writes a slot's closest-hit ray at the slot's position and
+// its shadow rays (at most `perVertex`, each with the contribution it adds when unoccluded) at slot * perVertex + j;
+// each workgroup then appends its queued positions to the compacted lists with one atomic.
+//   pending records   72 B, level-major: level i of slot s at stack[i * slots + s] (MaxPending levels)
+//   ray               two float4: origin + near, direction + far
+struct WaveCounts {
+    uint32_t rays[2], shadows[2];  // per parity: the compacted lists' lengths
+    unsigned long long traced;     // closest-hit rays traced in the launch (the host's iteration estimate)
+};
+struct WaveParams {
+    uint32_t slots, perVertex;
+    uint32_t* flags;         // [slots] WaveLive | WaveTraced | WaveShadows
+    uint32_t* sample;        // [slots * 2] pixel, sample index
+    uint32_t* top;           // [slots] pending records
+    float* accum;            // [slots * 8] radiance rgb, alpha, camera near, camera far, 2 unused
+    void* stack;             // [MaxPending * slots] pending records
+    void* current;           // [slots] the pending record whose ray is being traced
+    uint32_t* hitTri;        // [slots] its closest hit (a triangle record, or ~0u)
+    uint32_t* shadowCount;   // [slots] shadow rays of the slot's last vertex
+    float* rays[2];          // [slots * 8] by slot
+    uint32_t* rayList[2];    // [slots] queued slots, compacted
+    float* shadowRays[2];    // [slots * perVertex * 8] by slot and entry
+    float* shadowValue[2];   // [slots * perVertex * 4] contribution rgb, unused
+    uint32_t* shadowList[2]; // [slots * perVertex] queued entries, compacted
+    uint32_t* shadowVis[2];  // [slots * perVertex] 1 unoccluded
+    WaveCounts* counts;
+    unsigned long long* partials;  // [partialBlocks * 4] the trace passes' counters by workgroup
+    uint32_t partialBlocks, pad;
+};
+constexpr uint32_t WaveLive = 1, WaveTraced = 2, WaveShadows = 4;
+constexpr uint32_t MaxPendingRecords = 32, PendingBytes = 72;  // a sample's pending stack (x2_kernel.hip Pending)
+
 // The batched queue's launch record and totals (x2_kernel.hip runBatch, x2_batch.hip).
 struct BatchState {
     unsigned long long traces, shadows;

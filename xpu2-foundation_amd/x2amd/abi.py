@@ -103,7 +103,8 @@ class Completion(ctypes.Structure):
 
 
 class Options(ctypes.Structure):
-    _fields_ = [("device", i32), ("max_chunks_per_launch", u32), ("ring_chunks", u32), ("counters", u32)]
+    _fields_ = [("device", i32), ("max_chunks_per_launch", u32), ("ring_chunks", u32), ("counters", u32),
+                ("wave_slots", u32)]
 
 
 class FrameStats(ctypes.Structure):
@@ -137,7 +138,7 @@ EXPECTED_SIZES = {
     "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 292),
-    "x2_completion": (Completion, 72), "x2dev_options": (Options, 16), "x2_frame_stats": (FrameStats, 80),
+    "x2_completion": (Completion, 72), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 80),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
