@@ -268,6 +268,51 @@ int x2dev_rng_twin(x2dev* d, const x2_rng_query* queries, uint32_t n, uint32_t d
  * audited against spp (async_render.h:1035). stats (may be null) receives totals over the frame's launches. */
 int x2dev_render_frame(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, x2_frame_stats* stats);
 
+/* ---- Adaptive frames (MoonRay's adaptive sampling as the reference restates it: async_adaptive.h,
+ * async_adaptive_core.h, async_integrator.h:3077-3125). A pass is a range of sample counts; a chunk of a pass
+ * carries first = pass << X2_PASS_SHIFT | offset, and each active pixel of its tile takes sample indices from its
+ * own base, so a pixel's indices stay contiguous. The film also keeps the odd-index sums, from which the device works
+ * out every pixel's error for the host's error trees. ---- */
+#define X2_PASS_SHIFT 16u
+#define X2_PASS_SLOTS 4u
+#define X2_NO_SLOT 0xffffffffu /* x2dev_upload_pass: no uniform base, the pass starts from the device's counts */
+
+typedef struct x2_adaptive {
+    uint32_t min_samples; /* passes that start below this sample every pixel (MoonRay's min samples) */
+    uint32_t max_samples; /* the most samples a pixel takes: the schedule's end */
+    float target_error;   /* a region tree's node converges below this error */
+    uint32_t passes;      /* out: passes rendered */
+    uint32_t tree_updates; /* out: times the trees were rebuilt */
+    uint32_t pad;
+} x2_adaptive;
+
+/* GpuLane::begin(spc, adaptive = true) (async_gpu.h:1372-1422): begin, with the odd-index film cleared and chunks
+ * read through the pass tables. */
+int x2dev_begin_adaptive(x2dev* d, const x2_config* config, uint32_t samples_per_chunk);
+/* GpuLane::uploadPass (async_gpu.h:1906-1916): pass slot `slot`'s tile masks (one u64 a tile, bit y*8+x) and sample
+ * bases: every pixel uniform_base, or X2_NO_SLOT for the device's counts (every earlier pass completed). Queued on
+ * the lane's stream ahead of any launch that reads it. */
+int x2dev_upload_pass(x2dev* d, uint32_t slot, const uint64_t* masks, uint32_t uniform_base);
+/* GpuLane::requestErrors / errorsReady / errors (async_gpu.h:1836-1899; the mode-2 launch, async_gpu.cu:364-369):
+ * every pixel's error behind the launches already queued, copied to a pinned host buffer. */
+int x2dev_request_errors(x2dev* d);
+int x2dev_errors_ready(x2dev* d);
+int x2dev_errors(const x2dev* d, const float** errors);
+/* GpuLane::odd() (async_gpu.h:1706): the odd-index sums, downloaded with the film in an adaptive frame. */
+int x2dev_odd(const x2dev* d, const double** odd);
+/* One adaptive frame through the broker's sequence (async_render.h:1219-1440) with the host's error trees:
+ * config->spp is ignored (a->max_samples bounds every pixel); counts are audited against the planned samples. */
+int x2dev_render_adaptive(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, x2_adaptive* a,
+                          x2_frame_stats* stats);
+/* The host's tree step on its own (Regions::update + sampleArea, async_adaptive.h:99-145): from every pixel's error
+ * (width x height), each tile's active pixels; regions_done: nonzero when every region's tree converged. Needs no
+ * device. */
+int x2_adaptive_masks(uint32_t width, uint32_t height, float target_error, const float* errors, uint64_t* masks,
+                      uint32_t* regions_done);
+/* pixelError (async_integrator.h:3109-3125) on the host over a film: errors[i] from sums[i*4..], odd[i*4..],
+ * counts[i]. Needs no device. */
+int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors);
+
 /* ---- Batched jobs. No counterpart in the reference, which renders one frame at a time and decided against CUDA
  * graphs (ASYNC.md:176-177): many independent render jobs, each its own scene, settings and film, run in one
  * persistent kernel that claims work from a device queue (the jobs' items end to end, shortest job first by

@@ -48,6 +48,7 @@ def load():
         "oracle_sample": (c_int, [vp, vp, u32, u32, P(f64), P(u64)]),
         "oracle_render": (c_int, [vp, vp, u32, P(f64), P(u32), P(u64)]),
         "oracle_render_chunks": (c_int, [vp, vp, vp, u32, u32, P(f64), P(u32), P(u64)]),
+        "oracle_render_pixels": (c_int, [vp, vp, P(u32), P(u32), u32, P(f64), P(f64), P(u32), P(u64)]),
         "oracle_adjoint": (c_int, [vp, vp, u32, P(f32), P(f64), P(f64), P(u64)]),
         "oracle_set_texture": (None, [vp, u32, P(f32)]),
         "oracle_set_material": (None, [vp, u32, vp]),
@@ -130,6 +131,17 @@ class OracleScene:
 
     def set_eval_roughness(self, material, r):
         self.lib.oracle_set_eval_roughness(self.h, material, r)
+
+    def render_pixels(self, config, first, count, sums, odd, counts, threads=None):
+        """Adds pixel p's samples first[p] .. first[p] + count[p] - 1 into the film (sums, odd-index sums, counts)."""
+        counters = (u64 * 3)()
+        f = np.ascontiguousarray(first, np.uint32)
+        n = np.ascontiguousarray(count, np.uint32)
+        self.lib.oracle_render_pixels(self.h, ctypes.addressof(config), f.ctypes.data_as(P(u32)),
+                                      n.ctypes.data_as(P(u32)), threads or os.cpu_count() or 1,
+                                      sums.ctypes.data_as(P(f64)), odd.ctypes.data_as(P(f64)),
+                                      counts.ctypes.data_as(P(u32)), counters)
+        return tuple(counters)
 
     def sample(self, config, pixel, index):
         rgba = (f64 * 4)()

@@ -2268,4 +2268,46 @@ int oracle_render_chunks(const oracle_scene* o, const x2_config* c, const x2_chu
     renderChunks(o->s, *c, std::vector<x2_chunk>(chunks, chunks + n), threads, sums, counts, counters);
     return 0;
 }
+// An adaptive pass's samples (async_gpu.cu:331-345, :440-446): pixel p renders indices first[p] .. first[p] +
+// count[p] - 1, added into the running film; the odd-index samples also into odd.
+int oracle_render_pixels(const oracle_scene* o, const x2_config* c, const uint32_t* first, const uint32_t* count,
+                         uint32_t threads, double* sums, double* odd, uint32_t* counts, uint64_t counters[3]) {
+    const Scene& S = o->s;
+    std::atomic<U32> nextRow{0};
+    std::atomic<U64> traces{0}, shadows{0}, errors{0};
+    auto worker = [&]() {
+        Counters k;
+        U64 errs = 0;
+        for (;;) {
+            const U32 y = nextRow.fetch_add(1);
+            if (y >= c->height) break;
+            for (U32 x = 0; x < c->width; ++x) {
+                const U32 pixel = y * c->width + x;
+                for (U32 s = 0; s < count[pixel]; ++s) {
+                    const U32 index = first[pixel] + s;
+                    double rgba[4];
+                    const int st = integrate(S, *c, pixel, index, rgba, k);
+                    if (st == 0) {
+                        for (int q = 0; q < 4; ++q) sums[size_t(pixel) * 4 + q] += rgba[q];
+                        if (index & 1u)
+                            for (int q = 0; q < 4; ++q) odd[size_t(pixel) * 4 + q] += rgba[q];
+                        ++counts[pixel];
+                    } else if (st != 3) {
+                        ++errs;
+                    }
+                }
+            }
+        }
+        traces += k.traces;
+        shadows += k.shadows;
+        errors += errs;
+    };
+    std::vector<std::thread> pool;
+    for (U32 t = 0; t < std::max<U32>(1, threads); ++t) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    counters[0] = traces;
+    counters[1] = shadows;
+    counters[2] = errors;
+    return 0;
+}
 }

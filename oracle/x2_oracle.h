@@ -72,6 +72,8 @@ int oracle_render(const oracle_scene* s, const x2_config* c, uint32_t threads, d
 /* The same over chunks (tile, first, count), each sample of each chunk once: the film of a partial frame. */
 int oracle_render_chunks(const oracle_scene* s, const x2_config* c, const x2_chunk* chunks, uint32_t n,
                          uint32_t threads, double* sums, uint32_t* counts, uint64_t counters[3]);
+int oracle_render_pixels(const oracle_scene* s, const x2_config* c, const uint32_t* first, const uint32_t* count,
+                         uint32_t threads, double* sums, double* odd, uint32_t* counts, uint64_t counters[3]);
 
 /* The path-replay adjoint of a uniform frame (the device's algorithm, FP64 gradients): d_image is dLoss / d(mean
  * rgb) per pixel; grad_textures (the scene's textures' texels concatenated) and grad_roughness (per material) are

@@ -1783,7 +1783,7 @@ __device__ void runSamples(const RenderParams& P, Feed& feed, Counters& k, TravS
         const bool idle = S.top == 0;
         if (idle && live) {
             finishSample<Adj>(P, S, gr);
-            feed.done(pixel, S.radiance, S.alpha);
+            feed.done(pixel, index, S.radiance, S.alpha);
             live = false;
         }
         // A finished lane takes its next sample once Feed::Regen lanes of the wave are finished (or all are), so that
@@ -1931,28 +1931,57 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
 // One sample per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
 // position; the chunk comes from the mailbox ring. The forward render adds the sample to the film; the adjoint
 // replays it and scatters its gradients.
-template <bool Adj, bool Vol, bool Direct = false>
-__device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long long item, Counters& k, TravStack& st,
-                                           uint32_t& errors, float* gr) {
+// An item's sample (false: none — outside the image or viewport, past the chunk's count, or, in an adaptive frame,
+// a pixel its pass leaves out). An adaptive chunk's first is its pass and offset; the sample index starts from the
+// pixel's own base, so a pixel's indices stay contiguous (async_gpu.cu:440-446).
+__device__ __forceinline__ bool decodeItem(const RenderParams& P, unsigned long long item, uint32_t& pixel,
+                                           uint32_t& index) {
     const unsigned long long chunkIndex = item / P.itemsPerChunk;
     const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
     const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
     const uint32_t s = offset / TilePixels, position = offset % TilePixels;
-    const uint32_t pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
-    if (pixel == ~0u) return;
+    if (s >= chunk.count) return false;
+    pixel = tilePixel(chunk.tile, position, P.cfg);
+    if (pixel == ~0u) return false;
+    if (P.adaptive) {
+        const uint32_t slot = (chunk.first >> PassShift) % PassSlots;
+        if (!(P.passMask[slot][chunk.tile] >> position & 1ull)) return false;
+        index = P.passBase[slot][pixel] + (chunk.first & ((1u << PassShift) - 1u)) + s;
+    } else {
+        index = chunk.first + s;
+    }
+    return true;
+}
+// A finished sample into the film (async_gpu.cu:331-345): rgb + alpha, the count, and in an adaptive frame the
+// odd-index sums (MoonRay's beauty odd, the error's second half-image).
+__device__ __forceinline__ void filmAdd(const RenderParams& P, uint32_t pixel, uint32_t index, double r, double g,
+                                        double b, double a) {
+    double* f = P.sums + size_t(pixel) * 4;
+    atomicAdd(f + 0, r);
+    atomicAdd(f + 1, g);
+    atomicAdd(f + 2, b);
+    atomicAdd(f + 3, a);
+    atomicAdd(P.counts + pixel, 1u);
+    if (P.odd && (index & 1u)) {
+        double* o = P.odd + size_t(pixel) * 4;
+        atomicAdd(o + 0, r);
+        atomicAdd(o + 1, g);
+        atomicAdd(o + 2, b);
+        atomicAdd(o + 3, a);
+    }
+}
+
+template <bool Adj, bool Vol, bool Direct = false>
+__device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long long item, Counters& k, TravStack& st,
+                                           uint32_t& errors, float* gr) {
+    uint32_t pixel, index;
+    if (!decodeItem(P, item, pixel, index)) return;
     double rgba[4];
     int status;
-    if constexpr (Direct) status = integrateDirect(P, pixel, chunk.first + s, rgba, k, st);
-    else status = integrate<Adj, Vol>(P, pixel, chunk.first + s, rgba, k, st, gr);
+    if constexpr (Direct) status = integrateDirect(P, pixel, index, rgba, k, st);
+    else status = integrate<Adj, Vol>(P, pixel, index, rgba, k, st, gr);
     if (status == 0) {
-        if constexpr (!Adj) {
-            double* f = P.sums + size_t(pixel) * 4;
-            atomicAdd(f + 0, rgba[0]);
-            atomicAdd(f + 1, rgba[1]);
-            atomicAdd(f + 2, rgba[2]);
-            atomicAdd(f + 3, rgba[3]);
-            atomicAdd(P.counts + pixel, 1u);
-        }
+        if constexpr (!Adj) filmAdd(P, pixel, index, rgba[0], rgba[1], rgba[2], rgba[3]);
     } else {
         ++errors;
     }
@@ -1968,24 +1997,8 @@ __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel
     for (;;) {
         const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
         if (item >= P.itemEnd) return false;
-        const unsigned long long chunkIndex = item / P.itemsPerChunk;
-        const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
-        const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
-        const uint32_t s = offset / TilePixels, position = offset % TilePixels;
-        pixel = s < chunk.count ? tilePixel(chunk.tile, position, P.cfg) : ~0u;
-        if (pixel == ~0u) continue;
-        index = chunk.first + s;
-        return true;
+        if (decodeItem(P, item, pixel, index)) return true;
     }
-}
-// A finished sample into the film (async_gpu.cu:331-336).
-__device__ __forceinline__ void filmAdd(const RenderParams& P, uint32_t pixel, V3 r, float a) {
-    double* f = P.sums + size_t(pixel) * 4;
-    atomicAdd(f + 0, double(r.x));
-    atomicAdd(f + 1, double(r.y));
-    atomicAdd(f + 2, double(r.z));
-    atomicAdd(f + 3, double(a));
-    atomicAdd(P.counts + pixel, 1u);
 }
 template <bool Adj>
 struct LaneFeed {
@@ -1993,8 +2006,8 @@ struct LaneFeed {
     const RenderParams& P;
     uint32_t& errors;
     __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem(P, pixel, index); }
-    __device__ void done(uint32_t pixel, V3 r, float a) {
-        if constexpr (!Adj) filmAdd(P, pixel, r, a);
+    __device__ void done(uint32_t pixel, uint32_t index, V3 r, float a) {
+        if constexpr (!Adj) filmAdd(P, pixel, index, r.x, r.y, r.z, a);
     }
     __device__ void failed() { ++errors; }
 };
@@ -2223,7 +2236,7 @@ waveShadeKernel(const RenderParams P, const WaveParams W, uint32_t p) {
                         break;
                     }
                     if (fl & WaveShadows) break;  // the last vertex's shadow rays are still being traced
-                    filmAdd(P, pixel, S.radiance, S.alpha);
+                    filmAdd(P, pixel, index, S.radiance.x, S.radiance.y, S.radiance.z, S.alpha);
                     fl = 0;
                 }
                 if (!claimItem(P, pixel, index)) break;
@@ -2337,7 +2350,7 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY waveDrainKernel(const Render
             ok = shadeTraced(P, W, slot, pixel, index, S, inl, k, st);
         }
         while (ok && S.top) ok = shadeVertex<false, false>(P, pixel, index, S, k, st, gr);
-        if (ok) filmAdd(P, pixel, S.radiance, S.alpha);
+        if (ok) filmAdd(P, pixel, index, S.radiance.x, S.radiance.y, S.radiance.z, S.alpha);
         else ++errors;
         W.flags[slot] = 0;
     }
@@ -2359,6 +2372,34 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY waveDrainKernel(const Render
             atomicAdd(&P.state->triangles, tr);
         }
     }
+}
+
+// Every pixel's adaptive error (async_gpu.cu:364-369, pixelError async_integrator.h:3109-3125): the luma of the
+// difference between the even- and odd-index half-images over the square root of the pixel's luma, plus the alpha
+// difference; -1 until both halves have a sample. Each product and sum rounds on its own.
+__global__ void errorsKernel(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels,
+                             float* errors) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= pixels) return;
+    const uint32_t samples = counts[i];
+    const float n = float(samples), nOdd = float(samples / 2), nEven = n - nOdd;
+    if (nOdd <= 0 || nEven <= 0) {
+        errors[i] = -1.0f;
+        return;
+    }
+    float gap[4], mean[3];
+    for (int c = 0; c < 4; ++c) {
+        const float t = float(sums[size_t(i) * 4 + c]), o = float(odd[size_t(i) * 4 + c]);
+        if (t != t || o != o) {
+            errors[i] = 0.0f;
+            return;
+        }
+        gap[c] = fabsf((t * nOdd + -(o * n)) / (nOdd * nEven));
+        if (c < 3) mean[c] = t / n;
+    }
+    const float lumaGap = 0.299f * gap[0] + 0.587f * gap[1] + 0.114f * gap[2];
+    const float lumaMean = 0.299f * mean[0] + 0.587f * mean[1] + 0.114f * mean[2];
+    errors[i] = lumaMean <= 0 ? gap[3] : lumaGap / sqrtf(lumaMean) + gap[3];
 }
 
 __global__ void waveBeginKernel(const WaveParams W) {
@@ -2550,6 +2591,11 @@ hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     return p.vol.table ? launchFrame<2>(p, stream) : launchFrame<0>(p, stream);
 }
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }
+hipError_t launchErrors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(errorsKernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, sums, odd, counts, pixels, errors);
+    return hipGetLastError();
+}
 // The wavefront form over one launch's items: its counters cleared, `iterations` shading and trace passes, the drain.
 hipError_t launchWave(const RenderParams& p, const WaveParams& w, uint32_t iterations, hipStream_t stream) {
     static int traceBlocks = 0;  // the trace pass is persistent: as many workgroups as the chip holds at once

@@ -16,11 +16,14 @@
 #include "x2_bvh.h"
 #include "x2_layout.h"
 #include "x2_volume.h"
+#include "x2_adaptive.h"
 
 namespace x2 {
 hipError_t launchRender(const RenderParams& p, hipStream_t stream);
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream);
 hipError_t launchWave(const RenderParams& p, const WaveParams& w, uint32_t iterations, hipStream_t stream);
+hipError_t launchErrors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors,
+                        hipStream_t stream);
 hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream);
 hipError_t launchClearState(LaunchState* s, hipStream_t stream);
 hipError_t uploadSobol(const uint32_t* table);
@@ -122,6 +125,19 @@ struct x2dev {
     void* waveBlock = nullptr;
     WaveCounts* waveReadback = nullptr;
     double waveVertices = 0;  // closest-hit rays an item took in the last wavefront launches
+    // an adaptive frame: the odd-index film, the per-pixel errors and the pass tables (async_gpu.h:1836-1916)
+    bool adaptive = false;
+    size_t adaptivePixels = 0, adaptiveTiles = 0;
+    double* odd = nullptr;
+    double* hostOdd = nullptr;
+    float* errors = nullptr;
+    float* hostErrors = nullptr;
+    hipEvent_t errorsDone = nullptr;
+    bool errorsQueued = false;
+    unsigned long long* passMask[PassSlots] = {};
+    uint32_t* passBase[PassSlots] = {};
+    unsigned long long* passStaging[PassSlots] = {};
+    hipEvent_t passCopied[PassSlots] = {};
     hipEvent_t begun[Slots] = {}, ended[Slots] = {}, done[Slots] = {};
     std::deque<Flight> inFlight;
     uint64_t published = 0, launches = 0;
@@ -164,6 +180,17 @@ void release(x2dev* d) {
         if (d->done[i]) (void)hipEventDestroy(d->done[i]);
     }
     if (d->filmDone) (void)hipEventDestroy(d->filmDone);
+    (void)hipFree(d->odd);
+    (void)hipFree(d->errors);
+    if (d->hostOdd) (void)hipHostFree(d->hostOdd);
+    if (d->hostErrors) (void)hipHostFree(d->hostErrors);
+    if (d->errorsDone) (void)hipEventDestroy(d->errorsDone);
+    for (uint32_t i = 0; i < PassSlots; ++i) {
+        (void)hipFree(d->passMask[i]);
+        (void)hipFree(d->passBase[i]);
+        if (d->passStaging[i]) (void)hipHostFree(d->passStaging[i]);
+        if (d->passCopied[i]) (void)hipEventDestroy(d->passCopied[i]);
+    }
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -244,6 +271,8 @@ int x2dev_create(const x2dev_options* options, x2dev** out) {
         X2_CREATE(hipEventCreateWithFlags(&d->done[i], hipEventDisableTiming));
     }
     X2_CREATE(hipEventCreateWithFlags(&d->filmDone, hipEventDisableTiming));
+    X2_CREATE(hipEventCreateWithFlags(&d->errorsDone, hipEventDisableTiming));
+    for (uint32_t i = 0; i < PassSlots; ++i) X2_CREATE(hipEventCreateWithFlags(&d->passCopied[i], hipEventDisableTiming));
     uint32_t table[17 * 32];
     sobolDirections(table);
     X2_CREATE(uploadSobol(table));
@@ -360,7 +389,54 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     return X2_OK;
 }
 
+namespace {
+// The buffers of an adaptive frame at this image size: the odd film and errors (device and pinned host copies) and
+// the four pass slots' tables.
+int ensureAdaptive(x2dev* d, size_t pixels, size_t tiles) {
+    if (d->adaptivePixels == pixels && d->adaptiveTiles == tiles) return X2_OK;
+    (void)hipFree(d->odd);
+    (void)hipFree(d->errors);
+    if (d->hostOdd) (void)hipHostFree(d->hostOdd);
+    if (d->hostErrors) (void)hipHostFree(d->hostErrors);
+    d->odd = nullptr;
+    d->errors = nullptr;
+    d->hostOdd = nullptr;
+    d->hostErrors = nullptr;
+    d->adaptivePixels = d->adaptiveTiles = 0;
+    for (uint32_t i = 0; i < PassSlots; ++i) {
+        (void)hipFree(d->passMask[i]);
+        (void)hipFree(d->passBase[i]);
+        if (d->passStaging[i]) (void)hipHostFree(d->passStaging[i]);
+        d->passMask[i] = nullptr;
+        d->passBase[i] = nullptr;
+        d->passStaging[i] = nullptr;
+    }
+    X2_HIP(hipMalloc(&d->odd, pixels * 4 * sizeof(double)));
+    X2_HIP(hipMalloc(&d->errors, pixels * sizeof(float)));
+    X2_HIP(hipHostMalloc(&d->hostOdd, pixels * 4 * sizeof(double), hipHostMallocDefault));
+    X2_HIP(hipHostMalloc(&d->hostErrors, pixels * sizeof(float), hipHostMallocDefault));
+    for (uint32_t i = 0; i < PassSlots; ++i) {
+        X2_HIP(hipMalloc(&d->passMask[i], tiles * sizeof(unsigned long long)));
+        X2_HIP(hipMalloc(&d->passBase[i], pixels * sizeof(uint32_t)));
+        X2_HIP(hipHostMalloc(&d->passStaging[i], tiles * sizeof(unsigned long long), hipHostMallocDefault));
+        X2_HIP(hipMemsetAsync(d->passMask[i], 0, tiles * sizeof(unsigned long long), d->stream));
+    }
+    d->adaptivePixels = pixels;
+    d->adaptiveTiles = tiles;
+    return X2_OK;
+}
+int beginFrame(x2dev* d, const x2_config* c, uint32_t samples_per_chunk, bool adaptive);
+}  // namespace
+
 int x2dev_begin(x2dev* d, const x2_config* c, uint32_t samples_per_chunk) {
+    return beginFrame(d, c, samples_per_chunk, false);
+}
+int x2dev_begin_adaptive(x2dev* d, const x2_config* c, uint32_t samples_per_chunk) {
+    return beginFrame(d, c, samples_per_chunk, true);
+}
+
+namespace {
+int beginFrame(x2dev* d, const x2_config* c, uint32_t samples_per_chunk, bool adaptive) {
     if (!d || !c) return fail(X2_ERR_ARGUMENT, "null device or config");
     if (!d->prepared) return fail(X2_ERR_STATE, "begin before prepare");
     if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "GPU frame overlap (async_gpu.h:1374)");
@@ -395,9 +471,18 @@ int x2dev_begin(x2dev* d, const x2_config* c, uint32_t samples_per_chunk) {
     d->published = 0;
     d->launches = 0;
     d->filmQueued = false;
+    d->errorsQueued = false;
+    d->adaptive = adaptive;
+    if (adaptive) {
+        if (d->volDesc.table || c->direct_only) return fail(X2_ERR_ARGUMENT, "adaptive frames render the path tracer");
+        const int r = ensureAdaptive(d, pixels, size_t((c->width + 7) / 8) * ((c->height + 7) / 8));
+        if (r) return r;
+        X2_HIP(hipMemsetAsync(d->odd, 0, pixels * 4 * sizeof(double), d->stream));
+    }
     d->begun_ = true;
     return X2_OK;
 }
+}  // namespace
 
 int x2dev_ready(const x2dev* d) { return d && d->begun_ && d->inFlight.size() < 2 ? 1 : 0; }
 uint32_t x2dev_in_flight(const x2dev* d) { return d ? uint32_t(d->inFlight.size()) : 0u; }
@@ -499,8 +584,13 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     p.itemsPerChunk = d->itemsPerChunk;
     p.ringCapacity = d->opt.ring_chunks;
     p.counters = d->opt.counters;
-    p.pad = 0;
+    p.adaptive = d->adaptive ? 1u : 0u;
     p.vol = d->volDesc;
+    p.odd = d->adaptive ? d->odd : nullptr;
+    for (uint32_t i = 0; i < PassSlots; ++i) {
+        p.passMask[i] = d->adaptive ? d->passMask[i] : nullptr;
+        p.passBase[i] = d->adaptive ? d->passBase[i] : nullptr;
+    }
     const bool wave = waveForm(d);
     if (wave) {
         const uint32_t perVertex = d->cfg.light_count * std::max(1u, d->cfg.light_samples) +
@@ -574,6 +664,8 @@ int x2dev_download_film(x2dev* d) {
     X2_HIP(hipSetDevice(d->device));
     X2_HIP(hipMemcpyAsync(d->hostSums, d->sums, d->filmPixels * 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     X2_HIP(hipMemcpyAsync(d->hostCounts, d->counts, d->filmPixels * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
+    if (d->adaptive)
+        X2_HIP(hipMemcpyAsync(d->hostOdd, d->odd, d->filmPixels * 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     X2_HIP(hipEventRecord(d->filmDone, d->stream));
     d->filmQueued = true;
     return X2_OK;
@@ -591,6 +683,58 @@ int x2dev_film(const x2dev* d, const double** sums, const uint32_t** counts) {
     if (!d || !sums || !counts) return fail(X2_ERR_ARGUMENT, "null argument");
     *sums = d->hostSums;
     *counts = d->hostCounts;
+    return X2_OK;
+}
+
+int x2dev_odd(const x2dev* d, const double** odd) {
+    if (!d || !odd) return fail(X2_ERR_ARGUMENT, "null argument");
+    if (!d->adaptive) return fail(X2_ERR_STATE, "the odd-index film is kept in adaptive frames only");
+    *odd = d->hostOdd;
+    return X2_OK;
+}
+
+int x2dev_upload_pass(x2dev* d, uint32_t slot, const uint64_t* masks, uint32_t uniform_base) {
+    if (!d || !masks) return fail(X2_ERR_ARGUMENT, "null device or masks");
+    if (!d->begun_ || !d->adaptive) return fail(X2_ERR_STATE, "pass tables before an adaptive begin");
+    if (slot >= PassSlots) return fail(X2_ERR_ARGUMENT, "pass slot out of range (4 slots)");
+    X2_HIP(hipSetDevice(d->device));
+    X2_HIP(hipEventSynchronize(d->passCopied[slot]));  // the slot's last upload has left its staging
+    std::memcpy(d->passStaging[slot], masks, d->adaptiveTiles * sizeof(unsigned long long));
+    X2_HIP(hipMemcpyAsync(d->passMask[slot], d->passStaging[slot], d->adaptiveTiles * sizeof(unsigned long long),
+                          hipMemcpyHostToDevice, d->stream));
+    if (uniform_base != X2_NO_SLOT)
+        X2_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(d->passBase[slot]), int(uniform_base),
+                                 d->adaptivePixels, d->stream));
+    else  // a later pass starts from the counts every earlier pass has left
+        X2_HIP(hipMemcpyAsync(d->passBase[slot], d->counts, d->adaptivePixels * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, d->stream));
+    X2_HIP(hipEventRecord(d->passCopied[slot], d->stream));
+    return X2_OK;
+}
+
+int x2dev_request_errors(x2dev* d) {
+    if (!d) return fail(X2_ERR_ARGUMENT, "null device");
+    if (!d->begun_ || !d->adaptive) return fail(X2_ERR_STATE, "errors before an adaptive begin");
+    X2_HIP(hipSetDevice(d->device));
+    X2_HIP(launchErrors(d->sums, d->odd, d->counts, uint32_t(d->adaptivePixels), d->errors, d->stream));
+    X2_HIP(hipMemcpyAsync(d->hostErrors, d->errors, d->adaptivePixels * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipEventRecord(d->errorsDone, d->stream));
+    d->errorsQueued = true;
+    return X2_OK;
+}
+
+int x2dev_errors_ready(x2dev* d) {
+    if (!d || !d->errorsQueued) return 0;
+    const hipError_t q = hipEventQuery(d->errorsDone);
+    if (q == hipErrorNotReady) return 0;
+    if (q != hipSuccess) return hipFail(q, "hipEventQuery");
+    return 1;
+}
+
+int x2dev_errors(const x2dev* d, const float** errors) {
+    if (!d || !errors) return fail(X2_ERR_ARGUMENT, "null argument");
+    if (!d->adaptive) return fail(X2_ERR_STATE, "errors are kept in adaptive frames only");
+    *errors = d->hostErrors;
     return X2_OK;
 }
 
@@ -671,6 +815,9 @@ namespace {
 // over the 64-pixel cells' tiles), launches of about windowMs at the measured rate (launchSize,
 // async_render.h:1674-1718: firstChunks first, never under two items a resident thread), at most two in flight,
 // completions polled until the last.
+int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<uint32_t>* chunkSamples,
+              x2_frame_stats& st);
+
 int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
     std::vector<x2_chunk> chunks;
     {
@@ -689,6 +836,14 @@ int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
                         for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
                             chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
     }
+    return runChunks(d, chunks, nullptr, st);
+}
+
+// Launches the chunks in order, at most two launches in flight, sized by the measured rate, and collects every
+// completion (chunkSamples: each chunk's samples, else every in-image pixel of its tile).
+int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<uint32_t>* chunkSamples,
+              x2_frame_stats& st) {
+    const x2_config* c = &d->cfg;
     // A launch's device time once the rate is known: long enough that max_chunks_per_launch bounds it at the
     // headline's rate (a launch's last samples leave lanes idle, so fewer, larger launches waste less).
     const double windowMs = 100.0;
@@ -723,6 +878,10 @@ int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
             uint64_t samples = 0;
             const uint32_t tilesX = (c->width + 7) / 8;
             for (uint64_t i = 0; i < want; ++i) {
+                if (chunkSamples) {
+                    samples += (*chunkSamples)[next + i];
+                    continue;
+                }
                 const x2_chunk& ch = chunks[next + i];
                 for (uint32_t p = 0; p < TilePixels; ++p) {
                     const uint32_t x = (ch.tile % tilesX) * 8 + p % 8, y = (ch.tile / tilesX) * 8 + p / 8;
@@ -765,6 +924,107 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
     if (stats) *stats = st;
     if (st.audit_failures) return fail(X2_ERR_STATE, "Sample audit failure (async_render.h:1035)");
     if (st.errors) return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
+    return X2_OK;
+}
+
+// One adaptive frame (the broker's adaptive passes, async_render.h:1219-1440, with the host's region trees,
+// async_adaptive.h): MoonRay's schedule of passes; the passes below min_samples sample every pixel; before each later
+// pass the device works out every pixel's error (only the errors cross to the host), the trees rebuild from them,
+// and the pass samples the pixels their open leaves cover, each from its own count. A pass with no samples ends the
+// frame. The film is downloaded and every pixel's count audited against the samples its passes planned.
+int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptive* a, x2_frame_stats* stats) {
+    using clock = std::chrono::steady_clock;
+    const auto start = clock::now();
+    if (!d || !c || !a) return fail(X2_ERR_ARGUMENT, "null device, config or adaptive settings");
+    if (a->max_samples == 0 || a->max_samples > 65535 || !(a->target_error > 0.0f))
+        return fail(X2_ERR_ARGUMENT, "max_samples must be 1..65535 and target_error positive");
+    x2_config cfg = *c;
+    cfg.spp = a->max_samples;
+    int r = x2dev_begin_adaptive(d, &cfg, spc);
+    if (r) return r;
+    const uint32_t width = cfg.width, height = cfg.height, tilesX = (width + 7) / 8, tilesY = (height + 7) / 8;
+    const uint32_t cellsX = (width + 63) / 64, tiles = tilesX * tilesY;
+    const std::vector<adaptive::Pass> passes = adaptive::schedule(a->max_samples);
+    if (passes.size() > MaxPasses) return fail(X2_ERR_ARGUMENT, "adaptive pass budget (160 passes)");
+    adaptive::Regions regions(int(width), int(height), a->target_error);
+    std::vector<uint64_t> inImage(tiles, 0), masks(tiles);
+    for (uint32_t t = 0; t < tiles; ++t)
+        for (uint32_t q = 0; q < TilePixels; ++q) {
+            const uint32_t x = (t % tilesX) * 8 + q % 8, y = (t / tilesX) * 8 + q / 8;
+            if (x < width && y < height && int(x) >= cfg.viewport[0] && int(y) >= cfg.viewport[1] &&
+                int(x) < cfg.viewport[2] && int(y) < cfg.viewport[3])
+                inImage[t] |= 1ull << q;
+        }
+    std::vector<uint32_t> planned(size_t(width) * height, 0);
+    x2_frame_stats st{};
+    a->passes = 0;
+    a->tree_updates = 0;
+    for (uint32_t j = 0; j < passes.size(); ++j) {
+        const bool uniform = passes[j].start < a->min_samples;
+        const uint32_t size = passes[j].end - passes[j].start;
+        if (!uniform) {  // every pass before this one has completed: the trees see all of them
+            r = x2dev_request_errors(d);
+            if (r) return r;
+            X2_HIP(hipEventSynchronize(d->errorsDone));
+            regions.update(d->hostErrors);
+            ++a->tree_updates;
+        }
+        uint64_t samples = 0;
+        std::vector<x2_chunk> chunks;
+        std::vector<uint32_t> chunkSamples;
+        for (uint32_t cy = 0; cy < (tilesY + 7) / 8; ++cy)  // the frame loop's cell order
+            for (uint32_t cx = 0; cx < cellsX; ++cx)
+                for (uint32_t ty = cy * 8; ty < std::min(tilesY, (cy + 1) * 8); ++ty)
+                    for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx) {
+                        const uint32_t t = ty * tilesX + tx;
+                        uint64_t active = inImage[t];
+                        if (!uniform) active &= regions.sampleArea(int(tx * 8), int(ty * 8));
+                        masks[t] = active;
+                        if (!active) continue;
+                        const uint32_t n = uint32_t(__builtin_popcountll(active));
+                        for (uint32_t q = 0; q < TilePixels; ++q)
+                            if (active >> q & 1) planned[size_t(ty * 8 + q / 8) * width + tx * 8 + q % 8] += size;
+                        samples += uint64_t(n) * size;
+                        for (uint32_t off = 0; off < size; off += spc) {
+                            const uint32_t count = std::min(spc, size - off);
+                            chunks.push_back({t, (j << PassShift) | off, count, cy * cellsX + cx});
+                            chunkSamples.push_back(n * count);
+                        }
+                    }
+        if (!samples) break;  // the trees have converged: nothing changes after this
+        r = x2dev_upload_pass(d, j % PassSlots, masks.data(), uniform ? passes[j].start : X2_NO_SLOT);
+        if (r) return r;
+        r = runChunks(d, chunks, &chunkSamples, st);
+        if (r) return r;
+        ++a->passes;
+    }
+    r = x2dev_download_film(d);
+    if (r) return r;
+    X2_HIP(hipEventSynchronize(d->filmDone));
+    for (size_t i = 0; i < planned.size(); ++i)
+        if (d->hostCounts[i] != planned[i]) ++st.audit_failures;
+    st.wall_seconds = std::chrono::duration<double>(clock::now() - start).count();
+    if (stats) *stats = st;
+    if (st.audit_failures) return fail(X2_ERR_STATE, "Sample audit failure (async_render.h:1035)");
+    if (st.errors) return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
+    return X2_OK;
+}
+
+int x2_adaptive_masks(uint32_t width, uint32_t height, float target_error, const float* errors, uint64_t* masks,
+                      uint32_t* regions_done) {
+    if (!errors || !masks || !width || !height) return fail(X2_ERR_ARGUMENT, "null buffer or empty image");
+    if (!(target_error > 0.0f)) return fail(X2_ERR_ARGUMENT, "target_error must be positive");
+    adaptive::Regions regions(int(width), int(height), target_error);
+    regions.update(errors);
+    const uint32_t tilesX = (width + 7) / 8, tilesY = (height + 7) / 8;
+    for (uint32_t t = 0; t < tilesX * tilesY; ++t) masks[t] = regions.sampleArea(int(t % tilesX) * 8, int(t / tilesX) * 8);
+    if (regions_done) *regions_done = regions.done() ? 1u : 0u;
+    return X2_OK;
+}
+
+int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors) {
+    if (!sums || !odd || !counts || !errors) return fail(X2_ERR_ARGUMENT, "null buffer");
+    for (uint32_t i = 0; i < pixels; ++i) errors[i] = adaptive::pixelError(sums + size_t(i) * 4, odd + size_t(i) * 4, counts[i]);
     return X2_OK;
 }
 

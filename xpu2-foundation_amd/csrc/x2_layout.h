@@ -93,9 +93,15 @@ struct RenderParams {
     unsigned long long itemBegin, itemEnd;
     uint32_t itemsPerChunk, ringCapacity;
     uint32_t counters;  // nonzero: count rays, nodes, triangles
-    uint32_t pad;
+    uint32_t adaptive;  // nonzero: an adaptive frame (the chunks carry pass and offset; odd-index sums kept)
     VolumeDesc vol;
+    // An adaptive frame's film of odd-index sums and its pass tables (async_gpu.cu:337-345, :440-446): a chunk's
+    // first = pass << PassShift | offset; its tile's active pixels (passMask) take index passBase[pixel] + offset + s.
+    double* odd;
+    const unsigned long long* passMask[4];
+    const uint32_t* passBase[4];
 };
+constexpr uint32_t PassSlots = 4, PassShift = 16, MaxPasses = 160;  // async_integrator.h:3083
 
 // The wavefront form of the surface estimator (x2_kernel.hip waveShadeKernel / waveTraceKernel / waveDrainKernel):
 // `slots` path slots, each holding one sample between its vertices, in SoA arrays over the slots; and two sets of ray

@@ -10,6 +10,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libx2dev.so")
 
 X2_OK = 0
+X2_PASS_SHIFT = 16
+X2_PASS_SLOTS = 4
+X2_NO_SLOT = 0xFFFFFFFF
 X2_LIGHT_ENV = 0
 X2_LIGHT_RECT = 1
 X2_SIDE_REGULAR = 0
@@ -115,6 +118,11 @@ class FrameStats(ctypes.Structure):
     ]
 
 
+class Adaptive(ctypes.Structure):
+    _fields_ = [("min_samples", u32), ("max_samples", u32), ("target_error", f32), ("passes", u32),
+                ("tree_updates", u32), ("pad", u32)]
+
+
 class RngQuery(ctypes.Structure):
     _fields_ = [("pixel", u32), ("dimension", u32), ("policy", u32), ("bounce", u32), ("index", u64), ("stream", u64),
                 ("branch", u64), ("a", u32), ("b", u32)]
@@ -139,7 +147,7 @@ EXPECTED_SIZES = {
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 292),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 80),
-    "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48),
+    "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 24),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
     assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
@@ -173,6 +181,17 @@ SIGNATURES = {
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(FrameStats)]),
     "x2dev_rng_twin": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RngQuery), u32, u32, ctypes.POINTER(u32),
                                       ctypes.POINTER(f32), ctypes.POINTER(u64)]),
+    "x2dev_begin_adaptive": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32]),
+    "x2dev_upload_pass": (ctypes.c_int, [ctypes.c_void_p, u32, ctypes.POINTER(u64), u32]),
+    "x2dev_request_errors": (ctypes.c_int, [ctypes.c_void_p]),
+    "x2dev_errors_ready": (ctypes.c_int, [ctypes.c_void_p]),
+    "x2dev_errors": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(f32))]),
+    "x2dev_odd": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(f64))]),
+    "x2dev_render_adaptive": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.POINTER(Adaptive),
+                                             ctypes.POINTER(FrameStats)]),
+    "x2_adaptive_masks": (ctypes.c_int, [u32, u32, f32, ctypes.POINTER(f32), ctypes.POINTER(u64), ctypes.POINTER(u32)]),
+    "x2_pixel_errors": (ctypes.c_int, [ctypes.POINTER(f64), ctypes.POINTER(f64), ctypes.POINTER(u32), u32,
+                                       ctypes.POINTER(f32)]),
     "x2batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "x2batch_add_job": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Scene), ctypes.POINTER(Config),
                                        ctypes.POINTER(u32)]),

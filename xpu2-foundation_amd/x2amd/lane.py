@@ -40,6 +40,47 @@ class GpuLane:
         abi.check(self.lib.x2dev_begin(self.h, ctypes.byref(config), samples_per_chunk), "x2dev_begin")
         self.config = config
 
+    def begin_adaptive(self, config, samples_per_chunk=4):
+        """GpuLane::begin(spc, adaptive=true): the odd-index film kept, chunks read through the pass tables."""
+        abi.check(self.lib.x2dev_begin_adaptive(self.h, ctypes.byref(config), samples_per_chunk),
+                  "x2dev_begin_adaptive")
+        self.config = config
+
+    def upload_pass(self, slot, masks, uniform_base=abi.X2_NO_SLOT):
+        """GpuLane::uploadPass: a pass slot's tile masks (uint64 a tile) and bases (uniform_base, or the counts)."""
+        m = np.ascontiguousarray(masks, np.uint64)
+        abi.check(self.lib.x2dev_upload_pass(self.h, slot, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                             uniform_base), "x2dev_upload_pass")
+
+    def errors(self):
+        """GpuLane::requestErrors, then errors(): every pixel's adaptive error (float32), waited for."""
+        abi.check(self.lib.x2dev_request_errors(self.h), "x2dev_request_errors")
+        while not abi.check(self.lib.x2dev_errors_ready(self.h), "x2dev_errors_ready"):
+            pass
+        e = ctypes.POINTER(ctypes.c_float)()
+        abi.check(self.lib.x2dev_errors(self.h, ctypes.byref(e)), "x2dev_errors")
+        return np.ctypeslib.as_array(e, shape=(self.config.width * self.config.height,)).copy()
+
+    def odd(self):
+        """The odd-index sums [pixels, 4] of the downloaded film (adaptive frames)."""
+        o = ctypes.POINTER(ctypes.c_double)()
+        abi.check(self.lib.x2dev_odd(self.h, ctypes.byref(o)), "x2dev_odd")
+        n = self.config.width * self.config.height
+        return np.ctypeslib.as_array(o, shape=(n * 4,)).reshape(n, 4).copy()
+
+    def render_adaptive(self, config, min_samples, max_samples, target_error, samples_per_chunk=4):
+        """One adaptive frame; returns (sums, odd, counts, FrameStats, x2_adaptive with passes and tree updates)."""
+        st = abi.FrameStats()
+        a = abi.Adaptive(min_samples, max_samples, target_error, 0, 0, 0)
+        r = self.lib.x2dev_render_adaptive(self.h, ctypes.byref(config), samples_per_chunk, ctypes.byref(a),
+                                           ctypes.byref(st))
+        self.config = config
+        if r < 0:
+            err = self.lib.x2dev_last_error()
+            raise RuntimeError(f"x2dev_render_adaptive failed ({r}): {err.decode() if err else ''}")
+        sums, counts = self.film()
+        return sums, self.odd(), counts, st, a
+
     def ready(self):
         return bool(self.lib.x2dev_ready(self.h))
 
@@ -130,3 +171,30 @@ def render_frame(scene, config, samples_per_chunk=4, device=0):
         return lane.render_frame(config, samples_per_chunk)
     finally:
         lane.close()
+
+
+def adaptive_masks(width, height, target_error, errors):
+    """The host's tree step (x2_adaptive_masks): each tile's active pixels from every pixel's error, and whether every
+    region's tree converged. Needs no device."""
+    lib = abi.load()
+    e = np.ascontiguousarray(errors, np.float32)
+    masks = np.zeros(((width + 7) // 8) * ((height + 7) // 8), np.uint64)
+    done = ctypes.c_uint32(0)
+    abi.check(lib.x2_adaptive_masks(width, height, target_error, e.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                    masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(done)),
+              "x2_adaptive_masks")
+    return masks, bool(done.value)
+
+
+def pixel_errors(sums, odd, counts):
+    """pixelError over a film on the host (x2_pixel_errors). Needs no device."""
+    lib = abi.load()
+    s = np.ascontiguousarray(sums, np.float64)
+    o = np.ascontiguousarray(odd, np.float64)
+    c = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros(len(c), np.float32)
+    abi.check(lib.x2_pixel_errors(s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  o.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(c),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "x2_pixel_errors")
+    return out
