@@ -283,7 +283,8 @@ typedef struct x2_adaptive {
     float target_error;   /* a region tree's node converges below this error */
     uint32_t passes;      /* out: passes rendered */
     uint32_t tree_updates; /* out: times the trees were rebuilt */
-    uint32_t pad;
+    uint32_t device_chain; /* nonzero: the device pass chain (async_adaptive.cu) builds the trees and masks and the
+                              frame's passes are queued at once; zero: the host's trees between passes */
 } x2_adaptive;
 
 /* GpuLane::begin(spc, adaptive = true) (async_gpu.h:1372-1422): begin, with the odd-index film cleared and chunks
@@ -300,8 +301,10 @@ int x2dev_errors_ready(x2dev* d);
 int x2dev_errors(const x2dev* d, const float** errors);
 /* GpuLane::odd() (async_gpu.h:1706): the odd-index sums, downloaded with the film in an adaptive frame. */
 int x2dev_odd(const x2dev* d, const double** odd);
-/* One adaptive frame through the broker's sequence (async_render.h:1219-1440) with the host's error trees:
- * config->spp is ignored (a->max_samples bounds every pixel); counts are audited against the planned samples. */
+/* One adaptive frame through the broker's sequence (async_render.h:1219-1440): the host's error trees between passes,
+ * or with a->device_chain the device pass chain (GpuLane::beginChain / launchChain, async_gpu.h:1960-2085), the
+ * uniform stage one step and each later pass one. config->spp is ignored (a->max_samples bounds every pixel); counts
+ * are audited against the planned samples. */
 int x2dev_render_adaptive(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, x2_adaptive* a,
                           x2_frame_stats* stats);
 /* The host's tree step on its own (Regions::update + sampleArea, async_adaptive.h:99-145): from every pixel's error

@@ -86,3 +86,26 @@ def test_adaptive_frame_matches_oracle(target, wave_slots):
     ref = A.pixel_errors(rs, ro, rc)
     assert (errors.view(np.uint32) == ref.view(np.uint32)).mean() >= 0.99
     assert np.abs(errors - ref).max() <= 1e-5 * max(1.0, float(np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("target", [5e-4, 2e-3])
+def test_device_chain_matches_host_trees(target):
+    """The device pass chain (trees, masks and active tiles made on the device, every pass queued at once) samples
+    exactly the pixels the host's trees do; the uniform stage is one step there, so only the film's FP64 order differs."""
+    scene, config = scenes.cornell(width=72, height=56, spp=64)
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        hs, ho, hc, hst, ha = lane.render_adaptive(config, 4, 64, target)
+        cs, co, cc, cst, ca = lane.render_adaptive(config, 4, 64, target, device_chain=True)
+    finally:
+        lane.close()
+    print(f"target {target}: host passes {ha.passes} tree updates {ha.tree_updates}; chain passes {ca.passes} tree "
+          f"updates {ca.tree_updates}; samples a pixel {hc.mean():.1f}; chain {cst.device_seconds * 1e3:.2f} ms device")
+    assert cst.errors == 0 and cst.audit_failures == 0
+    assert (cc == hc).all() and hc.min() < hc.max()
+    assert ca.passes == ha.passes
+    assert mean_relative_difference(cs, cc, hs, hc) <= 1e-12
+    assert mean_relative_difference(co, np.maximum(cc // 2, 1), ho, np.maximum(hc // 2, 1)) <= 1e-12
+    _, _, rc, history = A.adaptive_frame(oracle.OracleScene(scene), config, 4, 64, target)
+    assert (cc == rc).all() and len(history) == ca.passes

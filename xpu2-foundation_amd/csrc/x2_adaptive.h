@@ -40,6 +40,7 @@ public:
     // The pixels of the 8x8 tile at (tx, ty) still to sample (bit y * 8 + x), filled to their bounding rectangle.
     uint64_t sampleArea(int tx, int ty) const;
     const std::vector<Node>& nodes() const { return nodes_; }
+    IBox root() const { return root_; }
 
 private:
     void updateNode(int index);
@@ -60,6 +61,8 @@ public:
     size_t count() const { return trees_.size(); }
     const Tree& tree(size_t i) const { return trees_[i]; }
     int perDim() const { return perDim_; }
+    int regionWidth() const { return regionW_; }
+    int regionHeight() const { return regionH_; }
 
 private:
     int width_, height_, perDim_ = 1, regionW_ = 0, regionH_ = 0;

@@ -1,0 +1,122 @@
+// x2_adaptive.hip — the host side of an adaptive frame (x2_adaptive.h): MoonRay's pass schedule, pixel error and
+// region error trees (after Dammertz et al. 2010, "A hierarchical automatic stopping condition for Monte Carlo
+// global illumination"), as the reference restates them in async_adaptive.h / async_adaptive_core.h. The node
+// arithmetic is x2_adaptive_core.h's, which the device's pass chain shares. Built without floating-point
+// contraction: each product and sum rounds on its own, as the reference's mulRn/addRn do.
+#include "x2_adaptive.h"
+
+#include <math.h>
+
+#include <algorithm>
+
+#include "x2_adaptive_core.h"
+
+namespace x2::adaptive {
+
+std::vector<Pass> schedule(uint32_t maxSamples) {
+    std::vector<Pass> out{{0, 1}};
+    uint32_t at = 1, left = maxSamples > 0 ? maxSamples - 1 : 0;
+    for (uint32_t size = 4; left > size && out.size() < 999; size += 2) {
+        out.push_back({at, at + size});
+        at += size;
+        left -= size;
+    }
+    if (left) out.push_back({at, at + left});
+    return out;
+}
+
+float pixelError(const double total[4], const double odd[4], uint32_t samples) {
+    return pixelErrorOf(total, odd, samples);
+}
+
+Tree::Tree(IBox root, float target)
+    : root_(root), target_(target), sat_(size_t(root.x1 - root.x0 + 1) * size_t(root.y1 - root.y0 + 1), 0.0) {
+    nodes_.push_back({{float(root.x0), float(root.y0), float(root.x1), float(root.y1)}, -1, 0});
+}
+
+void Tree::update(const float* errors, int stride) {
+    nodes_.resize(1);
+    nodes_[0].children = -1;
+    nodes_[0].complete = 0;
+    const int w = root_.x1 - root_.x0;
+    const size_t w1 = size_t(w) + 1;
+    for (int y = root_.y0; y < root_.y1; ++y)  // a pixel without both halves yet: the whole region samples
+        for (int x = root_.x0; x < root_.x1; ++x)
+            if (errors[size_t(y) * stride + x] < 0) return;
+    for (int y = root_.y0; y < root_.y1; ++y) {
+        const size_t r = size_t(y - root_.y0);
+        double run = 0;
+        for (int x = 0; x < w; ++x) {
+            run += double(errors[size_t(y) * stride + root_.x0 + x]);
+            sat_[(r + 1) * w1 + size_t(x) + 1] = sat_[r * w1 + size_t(x) + 1] + run;
+        }
+    }
+    updateNode(0);
+}
+
+// Breadth first: every node of a level is judged, its children appended; then each inner node, deepest first, is
+// complete when both children are.
+void Tree::updateNode(int index) {
+    const Table t{sat_.data(), root_.x0, root_.y0, root_.x1 - root_.x0 + 1};
+    for (size_t i = size_t(index); i < nodes_.size(); ++i) {
+        FBox a, b;
+        const Fate f = judge(t, root_, nodes_[i].bounds, target_, a, b);
+        if (f == Converged) nodes_[i].complete = 1;
+        if (f != Split) continue;
+        nodes_[i].children = int(nodes_.size());
+        nodes_.push_back({a, -1, 0});
+        nodes_.push_back({b, -1, 0});
+    }
+    for (size_t i = nodes_.size(); i-- > size_t(index);) {
+        Node& n = nodes_[i];
+        if (n.children >= 0) n.complete = nodes_[size_t(n.children)].complete && nodes_[size_t(n.children) + 1].complete;
+    }
+}
+
+uint64_t Tree::sampleArea(int tx, int ty) const {
+    uint64_t bits = 0;
+    for (const Node& n : nodes_) {
+        if (n.children >= 0 || n.complete) continue;  // open leaves only
+        const FBox c{std::max(n.bounds.x0, float(tx)), std::max(n.bounds.y0, float(ty)), std::min(n.bounds.x1, float(tx + 8)),
+                     std::min(n.bounds.y1, float(ty + 8))};
+        if (c.x1 <= c.x0 || c.y1 <= c.y0) continue;
+        const IBox r = outward(c);
+        for (int y = std::max(r.y0, ty); y < std::min(r.y1, ty + 8); ++y)
+            for (int x = std::max(r.x0, tx); x < std::min(r.x1, tx + 8); ++x) bits |= 1ull << ((y - ty) * 8 + (x - tx));
+    }
+    return fillRectangle(bits);
+}
+
+Regions::Regions(int width, int height, float target) : width_(width), height_(height) {
+    constexpr int tile = 8, overlap = 8, most = 4;
+    auto upTo8 = [](int n) { return (n + 7) / 8 * 8; };
+    if (tile + overlap >= width || tile + overlap >= height) {
+        regionW_ = width;
+        regionH_ = height;
+    } else {
+        perDim_ = std::max(1, std::min({most, width / (tile + overlap), height / (tile + overlap)}));
+        regionW_ = upTo8(1 + (width - 1) / perDim_);
+        regionH_ = upTo8(1 + (height - 1) / perDim_);
+    }
+    for (int i = 0; i < perDim_ * perDim_; ++i) {
+        const int rx = i % perDim_, ry = i / perDim_;
+        const IBox b{std::max(rx * regionW_ - overlap, 0), std::max(ry * regionH_ - overlap, 0),
+                     std::min((rx + 1) * regionW_ + overlap, width), std::min((ry + 1) * regionH_ + overlap, height)};
+        trees_.emplace_back(b, target);
+    }
+}
+
+void Regions::update(const float* errors) {
+    for (Tree& t : trees_) t.update(errors, width_);
+}
+
+bool Regions::done() const {
+    return std::all_of(trees_.begin(), trees_.end(), [](const Tree& t) { return t.done(); });
+}
+
+uint64_t Regions::sampleArea(int tx, int ty) const {
+    const int rx = std::min(tx / regionW_, perDim_ - 1), ry = std::min(ty / regionH_, perDim_ - 1);
+    return trees_[size_t(ry * perDim_ + rx)].sampleArea(tx, ty);
+}
+
+}  // namespace x2::adaptive

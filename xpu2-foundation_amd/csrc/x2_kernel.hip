@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cfloat>
 
+#include "x2_adaptive_core.h"
 #include "x2_layout.h"
 #include "x2_rng.h"
 
@@ -1995,6 +1996,21 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
 // The launch's next sample: claims items until one lies in the image and the viewport (false: none left).
 __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel, uint32_t& index) {
     for (;;) {
+        if (P.chain) {  // the device pass chain's render: its active tiles' pixels, chainSize samples each
+            const unsigned long long perTile = (unsigned long long)P.chainSize * TilePixels;
+            const unsigned long long item = atomicAdd(&P.chain->claimed, 1ull);
+            const ChainState* cs = P.chain;
+            if (*reinterpret_cast<const volatile uint32_t*>(&cs->stop) ||
+                item >= (unsigned long long)(*reinterpret_cast<const volatile uint32_t*>(&cs->activeTiles)) * perTile)
+                return false;
+            const uint32_t t = uint32_t(item / perTile), rest = uint32_t(item % perTile);
+            const uint32_t position = rest % TilePixels;
+            if (!(P.chainMasks[t] >> position & 1ull)) continue;
+            pixel = tilePixel(P.chainTiles[t], position, P.cfg);
+            if (pixel == ~0u) continue;
+            index = P.chainBase[pixel] + rest / TilePixels;
+            return true;
+        }
         const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
         if (item >= P.itemEnd) return false;
         if (decodeItem(P, item, pixel, index)) return true;
@@ -2380,26 +2396,189 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY waveDrainKernel(const Render
 __global__ void errorsKernel(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels,
                              float* errors) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= pixels) return;
-    const uint32_t samples = counts[i];
-    const float n = float(samples), nOdd = float(samples / 2), nEven = n - nOdd;
-    if (nOdd <= 0 || nEven <= 0) {
-        errors[i] = -1.0f;
+    if (i < pixels) errors[i] = adaptive::pixelErrorOf(sums + size_t(i) * 4, odd + size_t(i) * 4, counts[i]);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The device pass chain (async_adaptive.cu:22-177): a frame's adaptive passes queued at once. Each pass: every
+// pixel's error (chainErrorsKernel), each region's tree rebuilt from them (chainTreesKernel: the summed-area table
+// row by row then column by column, then the nodes a level at a time, each judged with the host's arithmetic,
+// x2_adaptive_core.h), each tile's active pixels and their sample bases (chainMasksKernel), the active tiles in
+// the work grid's cell order (chainCompactKernel), then the render over them. Once the trees have converged or a
+// pass comes out empty, every later step does nothing.
+__device__ __forceinline__ bool chainHalted(const ChainParams& C) {
+    return *reinterpret_cast<const volatile uint32_t*>(&C.state->stop) != 0;
+}
+
+__global__ void chainErrorsKernel(const ChainParams C) {
+    if (chainHalted(C)) return;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < C.width * C.height) C.errors[i] = adaptive::pixelErrorOf(C.sums + size_t(i) * 4, C.odd + size_t(i) * 4, C.counts[i]);
+}
+
+__global__ void __launch_bounds__(512) chainTreesKernel(const ChainParams C) {
+    if (chainHalted(C)) return;
+    const uint32_t r = blockIdx.x;
+    const ChainRegion R = C.regions[r];
+    const adaptive::IBox root{R.x0, R.y0, R.x1, R.y1};
+    const int w = R.x1 - R.x0, h = R.y1 - R.y0, w1 = w + 1;
+    double* sat = C.sat + R.satOffset;
+    adaptive::Node* nodes = static_cast<adaptive::Node*>(C.nodes) + R.nodeOffset;
+    __shared__ int notReady, open, count;
+    if (threadIdx.x == 0) {
+        notReady = 0;
+        open = 0;
+        count = 1;
+        nodes[0] = adaptive::Node{{float(R.x0), float(R.y0), float(R.x1), float(R.y1)}, -1, 0};
+        if (r == 0) ++C.state->treeUpdates;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x <= w; x += blockDim.x) sat[x] = 0;
+    for (int y = threadIdx.x; y < h; y += blockDim.x) {  // each row's running sums
+        const float* e = C.errors + size_t(R.y0 + y) * C.width + R.x0;
+        double* out = sat + size_t(y + 1) * w1;
+        double run = 0;
+        bool missing = false;
+        out[0] = 0;
+        for (int x = 0; x < w; ++x) {
+            const float v = e[x];
+            missing |= v < 0;
+            run += double(v);
+            out[x + 1] = run;
+        }
+        if (missing) notReady = 1;
+    }
+    __syncthreads();
+    if (notReady) {  // a pixel without both halves yet: the root alone, open
+        if (threadIdx.x == 0) C.regionDone[r] = 0;
         return;
     }
-    float gap[4], mean[3];
-    for (int c = 0; c < 4; ++c) {
-        const float t = float(sums[size_t(i) * 4 + c]), o = float(odd[size_t(i) * 4 + c]);
-        if (t != t || o != o) {
-            errors[i] = 0.0f;
+    for (int x = threadIdx.x + 1; x <= w; x += blockDim.x)  // then down each column
+        for (int y = 1; y <= h; ++y) sat[size_t(y) * w1 + x] += sat[size_t(y - 1) * w1 + x];
+    __syncthreads();
+    const adaptive::Table t{sat, R.x0, R.y0, w1};
+    const int capacity = int(R.nodeCapacity);
+    int begin = 0, end = 1;
+    while (begin < end) {
+        for (int i = begin + int(threadIdx.x); i < end; i += blockDim.x) {
+            adaptive::FBox a, b;
+            const adaptive::Fate f = adaptive::judge(t, root, nodes[i].bounds, C.target, a, b);
+            if (f == adaptive::Converged) {
+                nodes[i].complete = 1;
+            } else if (f == adaptive::Split) {
+                const int c = atomicAdd(&count, 2);
+                if (c + 2 <= capacity) {
+                    nodes[c] = adaptive::Node{a, -1, 0};
+                    nodes[c + 1] = adaptive::Node{b, -1, 0};
+                    nodes[i].children = c;
+                } else {
+                    open = 1;  // beyond MoonRay's own bound on a region's nodes: never
+                }
+            } else {
+                open = 1;
+            }
+        }
+        __syncthreads();
+        begin = end;
+        end = count < capacity ? count : capacity;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) C.regionDone[r] = open ? 0u : 1u;
+}
+
+// The tile's pixels under an open leaf of its region's tree, filled to their bounding rectangle (inner nodes are
+// never marked complete here: a subtree is skipped when its node converged, or when it misses the tile).
+__device__ uint64_t openArea(const adaptive::Node* nodes, int tx, int ty) {
+    const float t0 = float(tx), t1 = float(ty), t2 = float(tx + 8), t3 = float(ty + 8);
+    int stack[128], top = 0;
+    stack[top++] = 0;
+    uint64_t bits = 0;
+    while (top) {
+        const adaptive::Node n = nodes[stack[--top]];
+        if (n.complete) continue;
+        const adaptive::FBox c{n.bounds.x0 < t0 ? t0 : n.bounds.x0, n.bounds.y0 < t1 ? t1 : n.bounds.y0,
+                               t2 < n.bounds.x1 ? t2 : n.bounds.x1, t3 < n.bounds.y1 ? t3 : n.bounds.y1};
+        if (c.x1 <= c.x0 || c.y1 <= c.y0) continue;
+        if (n.children >= 0) {
+            if (top + 2 > 128) return ~0ull;  // deeper than the stack: the whole tile
+            stack[top++] = n.children;
+            stack[top++] = n.children + 1;
+            continue;
+        }
+        const adaptive::IBox b = adaptive::outward(c);
+        for (int y = adaptive::max2(b.y0, ty); y < adaptive::min2(b.y1, ty + 8); ++y)
+            for (int x = adaptive::max2(b.x0, tx); x < adaptive::min2(b.x1, tx + 8); ++x)
+                bits |= 1ull << ((y - ty) * 8 + (x - tx));
+    }
+    return adaptive::fillRectangle(bits);
+}
+
+// One thread a tile: its active pixels for the pass (all of them in the uniform stage), each pixel's first sample
+// index and its planned count. Once every region's tree has converged, the chain stops instead.
+__global__ void chainMasksKernel(const ChainParams C, uint32_t uniform, uint32_t size) {
+    if (chainHalted(C)) return;
+    if (!uniform) {
+        bool converged = true;
+        for (uint32_t r = 0; r < C.regionCount; ++r) converged &= C.regionDone[r] != 0;
+        if (converged) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) C.state->stop = 1;
             return;
         }
-        gap[c] = fabsf((t * nOdd + -(o * n)) / (nOdd * nEven));
-        if (c < 3) mean[c] = t / n;
     }
-    const float lumaGap = 0.299f * gap[0] + 0.587f * gap[1] + 0.114f * gap[2];
-    const float lumaMean = 0.299f * mean[0] + 0.587f * mean[1] + 0.114f * mean[2];
-    errors[i] = lumaMean <= 0 ? gap[3] : lumaGap / sqrtf(lumaMean) + gap[3];
+    const uint32_t tile = blockIdx.x * 256u + threadIdx.x;
+    if (tile >= C.tiles) return;
+    unsigned long long mask = C.inImage[tile];
+    const int tx = int(tile % C.tilesX) * 8, ty = int(tile / C.tilesX) * 8;
+    if (!uniform && mask) {
+        const uint32_t rx = uint32_t(tx) / C.regionW < C.perDim - 1 ? uint32_t(tx) / C.regionW : C.perDim - 1;
+        const uint32_t ry = uint32_t(ty) / C.regionH < C.perDim - 1 ? uint32_t(ty) / C.regionH : C.perDim - 1;
+        mask &= openArea(static_cast<const adaptive::Node*>(C.nodes) + C.regions[ry * C.perDim + rx].nodeOffset, tx, ty);
+    }
+    C.tileMask[tile] = mask;
+    for (uint32_t q = 0; q < TilePixels; ++q)
+        if (mask >> q & 1ull) {
+            const uint32_t pixel = (uint32_t(ty) + q / 8) * C.width + uint32_t(tx) + q % 8;
+            C.base[pixel] = C.planned[pixel];
+            C.planned[pixel] += size;
+        }
+}
+
+// One workgroup: the pass's active tiles in cell order, for its render. A pass with no active pixel ends the chain.
+__global__ void __launch_bounds__(1024) chainCompactKernel(const ChainParams C) {
+    if (chainHalted(C)) return;
+    __shared__ uint32_t scan[1024];
+    const uint32_t tid = threadIdx.x, per = (C.tiles + 1023u) / 1024u, first = tid * per;
+    const uint32_t last = first + per < C.tiles ? first + per : C.tiles;
+    uint32_t mine = 0;
+    for (uint32_t k = first; k < last; ++k) mine += C.tileMask[C.tileOrder[k]] ? 1u : 0u;
+    scan[tid] = mine;
+    __syncthreads();
+    for (uint32_t offset = 1; offset < 1024u; offset <<= 1) {
+        const uint32_t v = tid >= offset ? scan[tid - offset] : 0u;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    uint32_t at = scan[tid] - mine;
+    for (uint32_t k = first; k < last; ++k) {
+        const uint32_t tile = C.tileOrder[k];
+        const unsigned long long m = C.tileMask[tile];
+        if (m) {
+            C.activeTiles[at] = tile;
+            C.activeMasks[at] = m;
+            ++at;
+        }
+    }
+    if (tid == 1023u) {
+        const uint32_t total = scan[1023];
+        if (total) {
+            C.state->activeTiles = total;
+            C.state->claimed = 0;
+            ++C.state->passes;
+        } else {
+            C.state->stop = 1;
+        }
+    }
 }
 
 __global__ void waveBeginKernel(const WaveParams W) {
@@ -2579,7 +2758,9 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
         if (e != hipSuccess) return e;
         resident = std::max(1, perCu) * prop.multiProcessorCount;
     }
-    const unsigned long long items = p.itemEnd - p.itemBegin;
+    // The device pass chain's render learns its items on the device: a full persistent grid.
+    if (p.chain && !X2_PERSISTENT) return hipErrorInvalidValue;
+    const unsigned long long items = p.chain ? ~0ull >> 1 : p.itemEnd - p.itemBegin;
     unsigned long long blocks = (items + 255) / 256;
     if (X2_PERSISTENT && blocks > (unsigned long long)resident) blocks = resident;
     if (!blocks) return hipSuccess;
@@ -2591,6 +2772,18 @@ hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     return p.vol.table ? launchFrame<2>(p, stream) : launchFrame<0>(p, stream);
 }
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }
+// One step of the device pass chain: errors and trees (adaptive passes), masks and compaction, the render.
+hipError_t launchChainStep(const ChainParams& c, const RenderParams& p, bool uniform, hipStream_t stream) {
+    if (!uniform) {
+        hipLaunchKernelGGL(chainErrorsKernel, dim3((c.width * c.height + 255) / 256), dim3(256), 0, stream, c);
+        hipLaunchKernelGGL(chainTreesKernel, dim3(c.regionCount), dim3(512), 0, stream, c);
+    }
+    hipLaunchKernelGGL(chainMasksKernel, dim3((c.tiles + 255) / 256), dim3(256), 0, stream, c, uniform ? 1u : 0u,
+                       p.chainSize);
+    hipLaunchKernelGGL(chainCompactKernel, dim3(1), dim3(1024), 0, stream, c);
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launchFrame<0>(p, stream);
+}
 hipError_t launchErrors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors,
                         hipStream_t stream) {
     hipLaunchKernelGGL(errorsKernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, sums, odd, counts, pixels, errors);

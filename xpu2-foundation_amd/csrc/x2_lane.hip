@@ -16,12 +16,13 @@
 #include "x2_bvh.h"
 #include "x2_layout.h"
 #include "x2_volume.h"
-#include "x2_adaptive.h"
+#include "x2_adaptive_core.h"
 
 namespace x2 {
 hipError_t launchRender(const RenderParams& p, hipStream_t stream);
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream);
 hipError_t launchWave(const RenderParams& p, const WaveParams& w, uint32_t iterations, hipStream_t stream);
+hipError_t launchChainStep(const ChainParams& c, const RenderParams& p, bool uniform, hipStream_t stream);
 hipError_t launchErrors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors,
                         hipStream_t stream);
 hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream);
@@ -490,6 +491,35 @@ uint32_t x2dev_max_chunks_per_launch(const x2dev* d) { return d ? d->opt.max_chu
 uint32_t x2dev_launch_threads(const x2dev* d) { return d ? d->launchThreads : 0u; }
 
 namespace {
+// What every launch of the frame reads (its items and, for the chain, its pass set by the caller).
+RenderParams renderParams(const x2dev* d, uint32_t slot) {
+    RenderParams p{};
+    p.cfg = d->cfg;
+    p.nodes = d->nodes;
+    p.tris = d->tris;
+    p.triUV = d->triUV;
+    p.textures = d->textures;
+    p.materials = d->materials;
+    p.lights = d->lights;
+    p.dImage = d->dImage;
+    p.gradTextures = d->gradTextures;
+    p.gradRoughness = d->gradRoughness;
+    p.ring = d->ring;
+    p.sums = d->sums;
+    p.counts = d->counts;
+    p.state = d->state + slot;
+    p.itemsPerChunk = d->itemsPerChunk;
+    p.ringCapacity = d->opt.ring_chunks;
+    p.counters = d->opt.counters;
+    p.adaptive = d->adaptive ? 1u : 0u;
+    p.vol = d->volDesc;
+    p.odd = d->adaptive ? d->odd : nullptr;
+    for (uint32_t i = 0; i < PassSlots; ++i) {
+        p.passMask[i] = d->adaptive ? d->passMask[i] : nullptr;
+        p.passBase[i] = d->adaptive ? d->passBase[i] : nullptr;
+    }
+    return p;
+}
 // The surface estimator of a forward frame runs in the wavefront form when the build has path slots.
 bool waveForm(const x2dev* d) { return d->opt.wave_slots && !d->dImage && !d->volDesc.table && !d->cfg.direct_only; }
 // Allocates the slot arrays and queues for `perVertex` shadow rays a vertex (once; again if a frame needs more).
@@ -564,33 +594,9 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
         done += n;
     }
     X2_HIP(launchClearState(d->state + slot, d->stream));
-    RenderParams p;
-    p.cfg = d->cfg;
-    p.nodes = d->nodes;
-    p.tris = d->tris;
-    p.triUV = d->triUV;
-    p.textures = d->textures;
-    p.materials = d->materials;
-    p.lights = d->lights;
-    p.dImage = d->dImage;
-    p.gradTextures = d->gradTextures;
-    p.gradRoughness = d->gradRoughness;
-    p.ring = d->ring;
-    p.sums = d->sums;
-    p.counts = d->counts;
-    p.state = d->state + slot;
+    RenderParams p = renderParams(d, slot);
     p.itemBegin = d->published * d->itemsPerChunk;
     p.itemEnd = (d->published + count) * d->itemsPerChunk;
-    p.itemsPerChunk = d->itemsPerChunk;
-    p.ringCapacity = d->opt.ring_chunks;
-    p.counters = d->opt.counters;
-    p.adaptive = d->adaptive ? 1u : 0u;
-    p.vol = d->volDesc;
-    p.odd = d->adaptive ? d->odd : nullptr;
-    for (uint32_t i = 0; i < PassSlots; ++i) {
-        p.passMask[i] = d->adaptive ? d->passMask[i] : nullptr;
-        p.passBase[i] = d->adaptive ? d->passBase[i] : nullptr;
-    }
     const bool wave = waveForm(d);
     if (wave) {
         const uint32_t perVertex = d->cfg.light_count * std::max(1u, d->cfg.light_samples) +
@@ -927,6 +933,120 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
     return X2_OK;
 }
 
+namespace {
+// The device pass chain over a frame (GpuLane::beginChain / launchChain, async_gpu.h:1960-2085): the regions' bounds,
+// node room (MoonRay's maxNodes) and summed-area tables, the tiles in cell order; then every step queued at once —
+// the uniform stage as one step, each later pass as one — and one wait. `planned` receives each pixel's samples.
+int runChain(x2dev* d, const adaptive::Regions& regions, const std::vector<adaptive::Pass>& passes,
+             const std::vector<uint64_t>& inImage, uint32_t minSamples, float target, std::vector<uint32_t>& planned,
+             x2_adaptive& a, x2_frame_stats& st) {
+    const uint32_t width = d->cfg.width, height = d->cfg.height, tilesX = (width + 7) / 8, tilesY = (height + 7) / 8;
+    const uint32_t tiles = tilesX * tilesY, cellsX = (width + 63) / 64, n = uint32_t(regions.count());
+    const size_t pixels = size_t(width) * height;
+    std::vector<ChainRegion> bounds(n);
+    uint64_t nodes = 0, sat = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const adaptive::IBox b = regions.tree(i).root();
+        const int w = b.x1 - b.x0, h = b.y1 - b.y0;
+        const float cells = (float(w) / adaptive::MinNodeSize) * (float(h) / adaptive::MinNodeSize);
+        const uint32_t room = uint32_t(std::max(1, 2 * int(ceilf(cells)) - 1));  // AdaptiveRegionTree::maxNodes
+        bounds[i] = {b.x0, b.y0, b.x1, b.y1, uint32_t(nodes), room, sat};
+        nodes += room;
+        sat += uint64_t(w + 1) * uint64_t(h + 1);
+    }
+    std::vector<uint32_t> order;
+    for (uint32_t cy = 0; cy < (tilesY + 7) / 8; ++cy)
+        for (uint32_t cx = 0; cx < cellsX; ++cx)
+            for (uint32_t ty = cy * 8; ty < std::min(tilesY, (cy + 1) * 8); ++ty)
+                for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx) order.push_back(ty * tilesX + tx);
+    // One allocation for the chain's tables (freed when the frame returns).
+    const size_t sizes[] = {n * sizeof(ChainRegion), nodes * sizeof(adaptive::Node), sat * sizeof(double), n * 4,
+                            tiles * 8, tiles * 8, tiles * 4, tiles * 4, tiles * 8, pixels * 4, pixels * 4,
+                            sizeof(ChainState)};
+    size_t offsets[12], total = 0;
+    for (int i = 0; i < 12; ++i) {
+        offsets[i] = total;
+        total += (sizes[i] + 255) & ~size_t(255);
+    }
+    char* block = nullptr;
+    X2_HIP(hipMalloc(&block, total));
+    struct Free {
+        char* p;
+        ~Free() { (void)hipFree(p); }
+    } release{block};
+    ChainParams c{};
+    c.sums = d->sums;
+    c.odd = d->odd;
+    c.counts = d->counts;
+    c.errors = d->errors;
+    c.regions = reinterpret_cast<const ChainRegion*>(block + offsets[0]);
+    c.nodes = block + offsets[1];
+    c.sat = reinterpret_cast<double*>(block + offsets[2]);
+    c.regionDone = reinterpret_cast<uint32_t*>(block + offsets[3]);
+    c.tileMask = reinterpret_cast<unsigned long long*>(block + offsets[4]);
+    c.inImage = reinterpret_cast<const unsigned long long*>(block + offsets[5]);
+    c.tileOrder = reinterpret_cast<const uint32_t*>(block + offsets[6]);
+    c.activeTiles = reinterpret_cast<uint32_t*>(block + offsets[7]);
+    c.activeMasks = reinterpret_cast<unsigned long long*>(block + offsets[8]);
+    c.planned = reinterpret_cast<uint32_t*>(block + offsets[9]);
+    c.base = reinterpret_cast<uint32_t*>(block + offsets[10]);
+    c.state = reinterpret_cast<ChainState*>(block + offsets[11]);
+    c.width = width;
+    c.height = height;
+    c.tiles = tiles;
+    c.tilesX = tilesX;
+    c.regionCount = n;
+    c.perDim = uint32_t(regions.perDim());
+    c.regionW = uint32_t(regions.regionWidth());
+    c.regionH = uint32_t(regions.regionHeight());
+    c.target = target;
+    X2_HIP(hipMemcpyAsync(block + offsets[0], bounds.data(), sizes[0], hipMemcpyHostToDevice, d->stream));
+    X2_HIP(hipMemcpyAsync(block + offsets[5], inImage.data(), sizes[5], hipMemcpyHostToDevice, d->stream));
+    X2_HIP(hipMemcpyAsync(block + offsets[6], order.data(), sizes[6], hipMemcpyHostToDevice, d->stream));
+    X2_HIP(hipMemsetAsync(block + offsets[3], 0, sizes[3], d->stream));
+    X2_HIP(hipMemsetAsync(block + offsets[9], 0, sizes[9], d->stream));
+    X2_HIP(hipMemsetAsync(block + offsets[11], 0, sizes[11], d->stream));
+    X2_HIP(launchClearState(d->state, d->stream));
+    RenderParams p = renderParams(d, 0);
+    p.chain = c.state;
+    p.chainTiles = c.activeTiles;
+    p.chainMasks = c.activeMasks;
+    p.chainBase = c.base;
+    uint32_t uniformEnd = 0;
+    while (uniformEnd < passes.size() && passes[uniformEnd].start < minSamples) ++uniformEnd;
+    X2_HIP(hipEventRecord(d->begun[0], d->stream));
+    if (uniformEnd) {
+        p.chainSize = uniformEnd < passes.size() ? passes[uniformEnd].start : passes.back().end;
+        X2_HIP(launchChainStep(c, p, true, d->stream));
+    }
+    for (uint32_t j = uniformEnd; j < passes.size(); ++j) {
+        p.chainSize = passes[j].end - passes[j].start;
+        X2_HIP(launchChainStep(c, p, false, d->stream));
+    }
+    X2_HIP(hipEventRecord(d->ended[0], d->stream));
+    ChainState cs{};
+    X2_HIP(hipMemcpyAsync(d->readback, d->state, sizeof(LaunchState), hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipMemcpyAsync(&cs, c.state, sizeof(ChainState), hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipMemcpyAsync(planned.data(), c.planned, pixels * 4, hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipStreamSynchronize(d->stream));
+    float ms = 0;
+    X2_HIP(hipEventElapsedTime(&ms, d->begun[0], d->ended[0]));
+    const LaunchState& ls = d->readback[0];
+    st.launches += 1;
+    st.errors += ls.errors;
+    st.device_seconds += ms / 1000.0;
+    st.rays_trace += ls.traces;
+    st.rays_shadow += ls.shadows;
+    st.nodes_visited += ls.nodes;
+    st.triangles_tested += ls.triangles;
+    for (uint32_t v : planned) st.samples += v;
+    // The passes rendered, counted as the reference counts a chain's (async_render.h:1069).
+    a.passes = uniformEnd + (cs.passes ? cs.passes - (uniformEnd ? 1u : 0u) : 0u);
+    a.tree_updates = cs.treeUpdates;
+    return X2_OK;
+}
+}  // namespace
+
 // One adaptive frame (the broker's adaptive passes, async_render.h:1219-1440, with the host's region trees,
 // async_adaptive.h): MoonRay's schedule of passes; the passes below min_samples sample every pixel; before each later
 // pass the device works out every pixel's error (only the errors cross to the host), the trees rebuild from them,
@@ -959,7 +1079,11 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
     x2_frame_stats st{};
     a->passes = 0;
     a->tree_updates = 0;
-    for (uint32_t j = 0; j < passes.size(); ++j) {
+    if (a->device_chain) {
+        r = runChain(d, regions, passes, inImage, a->min_samples, a->target_error, planned, *a, st);
+        if (r) return r;
+    }
+    for (uint32_t j = 0; j < passes.size() && !a->device_chain; ++j) {
         const bool uniform = passes[j].start < a->min_samples;
         const uint32_t size = passes[j].end - passes[j].start;
         if (!uniform) {  // every pass before this one has completed: the trees see all of them

@@ -100,8 +100,50 @@ struct RenderParams {
     double* odd;
     const unsigned long long* passMask[4];
     const uint32_t* passBase[4];
+    // The device pass chain's render (async_gpu.cu:448-466, Feed phase 2): items over the chain's active tiles, each
+    // chainSize samples of its active pixels from chainBase; null: chunks from the ring.
+    struct ChainState* chain;
+    const uint32_t* chainTiles;
+    const unsigned long long* chainMasks;
+    const uint32_t* chainBase;
+    uint32_t chainSize, chainPad;
 };
 constexpr uint32_t PassSlots = 4, PassShift = 16, MaxPasses = 160;  // async_integrator.h:3083
+
+// The device pass chain (x2_kernel.hip chain*Kernel; async_adaptive.cu): a frame's adaptive passes queued at once,
+// each pass's trees, masks and active tiles made on the device from the errors of every pass before it.
+struct ChainState {
+    unsigned long long claimed;  // the current pass's items claimed
+    uint32_t stop;               // the trees converged or a pass came out empty: every later step does nothing
+    uint32_t activeTiles;        // the current pass's active tiles
+    uint32_t passes, treeUpdates, pad[2];
+};
+struct ChainRegion {
+    int x0, y0, x1, y1;
+    uint32_t nodeOffset, nodeCapacity;
+    unsigned long long satOffset;
+};
+struct ChainParams {
+    const double* sums;
+    const double* odd;
+    const uint32_t* counts;
+    float* errors;
+    const ChainRegion* regions;
+    void* nodes;  // adaptive::Node
+    double* sat;
+    uint32_t* regionDone;
+    unsigned long long* tileMask;
+    const unsigned long long* inImage;
+    const uint32_t* tileOrder;  // the tiles in the work grid's cell order
+    uint32_t* activeTiles;
+    unsigned long long* activeMasks;
+    uint32_t* planned;
+    uint32_t* base;
+    ChainState* state;
+    uint32_t width, height, tiles, tilesX, regionCount, perDim, regionW, regionH;
+    float target;
+    uint32_t pad;
+};
 
 // The wavefront form of the surface estimator (x2_kernel.hip waveShadeKernel / waveTraceKernel / waveDrainKernel):
 // `slots` path slots, each holding one sample between its vertices, in SoA arrays over the slots; and two sets of ray

@@ -120,7 +120,7 @@ class FrameStats(ctypes.Structure):
 
 class Adaptive(ctypes.Structure):
     _fields_ = [("min_samples", u32), ("max_samples", u32), ("target_error", f32), ("passes", u32),
-                ("tree_updates", u32), ("pad", u32)]
+                ("tree_updates", u32), ("device_chain", u32)]
 
 
 class RngQuery(ctypes.Structure):

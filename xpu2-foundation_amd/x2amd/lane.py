@@ -68,10 +68,11 @@ class GpuLane:
         n = self.config.width * self.config.height
         return np.ctypeslib.as_array(o, shape=(n * 4,)).reshape(n, 4).copy()
 
-    def render_adaptive(self, config, min_samples, max_samples, target_error, samples_per_chunk=4):
-        """One adaptive frame; returns (sums, odd, counts, FrameStats, x2_adaptive with passes and tree updates)."""
+    def render_adaptive(self, config, min_samples, max_samples, target_error, samples_per_chunk=4, device_chain=False):
+        """One adaptive frame (host trees, or the device pass chain); returns (sums, odd, counts, FrameStats,
+        x2_adaptive with passes and tree updates)."""
         st = abi.FrameStats()
-        a = abi.Adaptive(min_samples, max_samples, target_error, 0, 0, 0)
+        a = abi.Adaptive(min_samples, max_samples, target_error, 0, 0, 1 if device_chain else 0)
         r = self.lib.x2dev_render_adaptive(self.h, ctypes.byref(config), samples_per_chunk, ctypes.byref(a),
                                            ctypes.byref(st))
         self.config = config
