@@ -13,7 +13,8 @@ leg times one forward frame and one path-replay adjoint frame of the differentia
 adjoint_step_over_forward is the adjoint frame's time over the forward frame's. The volume leg renders the
 participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp); the batch leg the 64-job batched
 workload, persistent kernel from a hipGraph against one launch per job; the direct leg the prototype estimator on
-the headline scene. roofline.copy_ceiling_gbs is a measured device copy's bandwidth beside the published peak.
+the headline scene; the adaptive leg an adaptive frame of the headline scene through the device pass chain and
+through the host's trees. roofline.copy_ceiling_gbs is a measured device copy's bandwidth beside the published peak.
 """
 import argparse
 import json
@@ -230,6 +231,38 @@ def direct_leg(reps):
             "mrays_per_s": rays / wall / 1e6}
 
 
+def adaptive_leg(reps, target=3e-4, min_samples=16, max_samples=64):
+    """An adaptive frame of the headline scene (MoonRay's schedule to 64 spp, the passes below 16 samples uniform,
+    region trees judged against target): the device pass chain (trees and masks on the device, every pass queued at
+    once) against the host's trees between passes. samples_per_pixel is what the trees asked for; the uniform frame
+    at 64 spp is the headline."""
+    from x2amd import scenes
+
+    scene, config = scenes.icospheres()
+    lane = make_lane()
+    lane.prepare(scene)
+    lane.render_adaptive(config, min_samples, max_samples, target, device_chain=True)
+    chain, device, spp = 0.0, 0.0, 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, _, counts, st, a = lane.render_adaptive(config, min_samples, max_samples, target, device_chain=True)
+        chain += time.perf_counter() - t0
+        device += st.device_seconds
+        assert st.errors == 0 and st.audit_failures == 0
+        spp = float(counts.mean())
+    t0 = time.perf_counter()
+    _, _, host_counts, hst, ha = lane.render_adaptive(config, min_samples, max_samples, target)
+    host = time.perf_counter() - t0
+    lane.close()
+    samples = float(counts.sum())
+    return {"workload": f"icospheres {scene.triangle_count} tris, {config.width}x{config.height}, adaptive to "
+                        f"{max_samples} spp, uniform below {min_samples}, target error {target}",
+            "reps": reps, "passes": a.passes, "samples_per_pixel": spp,
+            "chain_ms_per_frame": chain / reps * 1e3, "chain_device_ms": device / reps * 1e3,
+            "chain_mpaths_per_s": samples / (chain / reps) / 1e6,
+            "host_trees_ms_per_frame": host * 1e3, "same_samples": bool((host_counts == counts).all())}
+
+
 def copy_ceiling(mib=1024):
     """GB/s of a plain device-to-device copy (read + write bytes over the copy's time), the chip's practical
     bandwidth beside the 8 TB/s peak."""
@@ -297,6 +330,7 @@ def main():
     ap.add_argument("--volume-reps", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB: only run a device copy (for --pmc scaling)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--launch-chunks", type=int, default=LAUNCH_CHUNKS, help="max_chunks_per_launch of the lanes")
@@ -380,6 +414,8 @@ def main():
         out["batch"] = batch_leg(args.batch_reps)
     if not args.no_direct:
         out["direct"] = direct_leg(2)
+    if not args.no_adaptive:
+        out["adaptive"] = adaptive_leg(2)
     out["roofline"]["copy_ceiling_gbs"] = copy_ceiling()
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)

@@ -8,6 +8,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <thread>
 
 #include "x2_adaptive_core.h"
 
@@ -106,8 +107,12 @@ Regions::Regions(int width, int height, float target) : width_(width), height_(h
     }
 }
 
+// The regions are independent: each rebuilds on a thread of its own (the reference's tree pool, async_render.h:205).
 void Regions::update(const float* errors) {
-    for (Tree& t : trees_) t.update(errors, width_);
+    std::vector<std::thread> pool;
+    pool.reserve(trees_.size());
+    for (Tree& t : trees_) pool.emplace_back([&t, errors, this] { t.update(errors, width_); });
+    for (std::thread& th : pool) th.join();
 }
 
 bool Regions::done() const {
