@@ -285,6 +285,11 @@ typedef struct x2_adaptive {
     uint32_t tree_updates; /* out: times the trees were rebuilt */
     uint32_t device_chain; /* nonzero: the device pass chain (async_adaptive.cu) builds the trees and masks and the
                               frame's passes are queued at once; zero: the host's trees between passes */
+    uint32_t cellular;      /* nonzero: cellular generations of tiles decide each pass (adaptive::Cellular,
+                               async_adaptive.h:147-317) in place of the region trees (host side only) */
+    uint32_t sample_budget; /* cellular: a frame by sampling generations spends this many samples (0: none) */
+    float noise_floor;      /* with a budget: the error no tile need go below (0: none) */
+    uint32_t pad;
 } x2_adaptive;
 
 /* GpuLane::begin(spc, adaptive = true) (async_gpu.h:1372-1422): begin, with the odd-index film cleared and chunks
@@ -312,6 +317,13 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* config, uint32_t samples_pe
  * device. */
 int x2_adaptive_masks(uint32_t width, uint32_t height, float target_error, const float* errors, uint64_t* masks,
                       uint32_t* regions_done);
+/* One generation of the cellular rules on the host (Cellular::update, async_adaptive.h:183-230): from every tile's
+ * state alive_in (1 live), the image's errors, the tiles' in-image masks and, with sample_budget > 0, each pixel's
+ * planned samples and the samples left, the next generation in alive_out and each tile's error over the target in
+ * ratios; returns the live tiles (or a negative status). Needs no device. */
+int x2_cellular_step(uint32_t width, uint32_t height, float target_error, const uint8_t* alive_in,
+                     const uint64_t* in_image, const float* errors, const uint32_t* planned, double samples_left,
+                     uint32_t max_samples, float noise_floor, uint8_t* alive_out, float* ratios);
 /* pixelError (async_integrator.h:3109-3125) on the host over a film: errors[i] from sums[i*4..], odd[i*4..],
  * counts[i]. Needs no device. */
 int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors);

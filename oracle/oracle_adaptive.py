@@ -310,3 +310,162 @@ def adaptive_frame(oracle_scene, config, min_samples, max_samples, target, threa
         history.append(masks)
         oracle_scene.render_pixels(config, first, count, sums, odd, counts, threads)
     return sums, odd, counts, history
+
+
+class Cellular:
+    """Cellular generations of the image's tiles (async_adaptive.h:147-317): a tile lives on its own error (ratio >= 1)
+    or, down to SURVIVE with 3 live neighbours of 8, and comes back from BIRTH with 5. With a budget the target is the
+    one the samples left would bring every tile to (budgetTarget, bisected in log space, float64 throughout)."""
+    SURVIVE, BIRTH = F(0.5), F(0.75)
+
+    def __init__(self, width, height, target):
+        self.w, self.h = width, height
+        self.tx, self.ty = (width + 7) // 8, (height + 7) // 8
+        self.target = F(target)
+        n = self.tx * self.ty
+        self.alive = [1] * n
+        self.dead = []
+        self.ratio = [F(0)] * n
+        self.error = [F(0)] * n
+        self.pixels = [0.0] * n
+        self.samples = [0.0] * n
+
+    def dead_forever(self, in_image):
+        self.dead = [0] * len(self.alive)
+        for t in range(len(self.alive)):
+            if not int(in_image[t]):
+                self.dead[t] = 1
+                self.alive[t] = 0
+
+    def budget_target(self, planned, remaining, most):
+        lo, hi = 3.4e38, 0.0
+        live = [t for t in range(len(self.error)) if not (self.dead and self.dead[t]) and self.error[t] > 0 and self.samples[t] > 0]
+        for t in live:
+            lo, hi = min(lo, float(self.error[t])), max(hi, float(self.error[t]))
+        if remaining <= 0 or hi <= 0:
+            return F(3.4e38)
+
+        def cost(target):
+            c = 0.0
+            for t in live:
+                n = self.samples[t]
+                r = float(self.error[t]) / target
+                c += self.pixels[t] * (min(max(n * r * r, n), float(most)) - min(n, float(most)))
+            return c
+
+        low, high = lo * 1e-3, hi
+        if cost(low) <= remaining:
+            return F(low)
+        for _ in range(48):
+            middle = math.sqrt(low * high)
+            if cost(middle) > remaining:
+                low = middle
+            else:
+                high = middle
+        return F(high)
+
+    def update(self, errors, budget=None):
+        """budget: (planned per pixel, samples left, most, floor) or None. Returns the live tiles."""
+        for ty in range(self.ty):
+            for tx in range(self.tx):
+                t = ty * self.tx + tx
+                total, most, rendered, n, unknown = 0.0, 0.0, 0.0, 0, False
+                for y in range(ty * 8, min(self.h, ty * 8 + 8)):
+                    for x in range(tx * 8, min(self.w, tx * 8 + 8)):
+                        i = y * self.w + x
+                        e = errors[i]
+                        if e < 0:
+                            unknown = True
+                        else:
+                            total += float(e)
+                        n += 1
+                        if budget is not None:
+                            s = float(budget[0][i])
+                            most = max(most, s)
+                            rendered += 1 if s > 0 else 0
+                self.error[t] = F(-1) if unknown else area_error(F(total), F(n))
+                self.pixels[t] = rendered
+                self.samples[t] = most
+        target = self.target
+        if budget is not None:
+            target = max(self.budget_target(budget[0], budget[1], budget[2]), F(budget[3]))
+        self.ratio = [F(3.4e38) if e < 0 else F(e / target) for e in self.error]
+        nxt, live = [0] * len(self.alive), 0
+        for ty in range(self.ty):
+            for tx in range(self.tx):
+                t = ty * self.tx + tx
+                k = sum(1 for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+                        if (dx or dy) and 0 <= tx + dx < self.tx and 0 <= ty + dy < self.ty
+                        and self.alive[(ty + dy) * self.tx + tx + dx])
+                r = self.ratio[t]
+                if self.alive[t]:
+                    stays = r >= 1 or (r >= self.SURVIVE and k >= 3)
+                else:
+                    stays = r >= 1 or (r >= self.BIRTH and k >= 5)
+                if self.dead and self.dead[t]:
+                    stays = False
+                nxt[t] = 1 if stays else 0
+                live += nxt[t]
+        self.alive = nxt
+        return live
+
+
+def cellular_frame(oracle_scene, config, min_samples, max_samples, target, sample_budget=0, noise_floor=0.0,
+                   threads=None):
+    """A frame by cellular generations (async_render.h:1362-1440, 1561-1563), optionally by a sample budget: (sums,
+    odd, counts, per-pass masks). Ties among the tiles a budget keeps are broken by tile index."""
+    w, h = config.width, config.height
+    tiles_x = (w + 7) // 8
+    inside = in_image(config)
+    cells = Cellular(w, h, target)
+    cells.dead_forever(inside)
+    sums = np.zeros((w * h, 4), np.float64)
+    odd = np.zeros((w * h, 4), np.float64)
+    counts = np.zeros(w * h, np.uint32)
+    planned = np.zeros(w * h, np.uint32)
+    spent, history = 0.0, []
+    for start, end in schedule(max_samples):
+        uniform = start < min_samples
+        size = end - start
+        if not uniform:
+            if sample_budget and spent >= sample_budget:
+                break
+            budget = (planned, max(0.0, sample_budget - spent), max_samples, noise_floor) if sample_budget else None
+            if cells.update(pixel_errors(sums, odd, counts), budget) == 0:
+                break
+        dropped = None
+        if not uniform and sample_budget:
+            live = [(cells.ratio[t], t) for t in range(len(inside)) if int(inside[t]) and cells.alive[t]]
+            wanted = sum(bin(int(inside[t])).count("1") * size for _, t in live)
+            left = max(0.0, sample_budget - spent)
+            if wanted > left:
+                live.sort(key=lambda x: -float(x[0]))  # stable: ties keep tile order
+                dropped = [1] * len(inside)
+                taken = 0.0
+                for _, t in live:
+                    cost = bin(int(inside[t])).count("1") * size
+                    if taken + cost > left:
+                        break
+                    taken += cost
+                    dropped[t] = 0
+        masks = inside.copy()
+        if not uniform:
+            for t in range(len(masks)):
+                if not (cells.alive[t] and (dropped is None or not dropped[t])):
+                    masks[t] = 0
+        first = np.zeros(w * h, np.uint32)
+        count = np.zeros(w * h, np.uint32)
+        for t in np.nonzero(masks)[0]:
+            m = int(masks[t])
+            for q in range(64):
+                if m >> q & 1:
+                    p = ((t // tiles_x) * 8 + q // 8) * w + (t % tiles_x) * 8 + q % 8
+                    first[p] = counts[p]
+                    count[p] = size
+        if not count.any():
+            break
+        planned += count
+        spent += float(count.sum())
+        history.append(masks)
+        oracle_scene.render_pixels(config, first, count, sums, odd, counts, threads)
+    return sums, odd, counts, history

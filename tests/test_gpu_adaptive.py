@@ -109,3 +109,25 @@ def test_device_chain_matches_host_trees(target):
     assert mean_relative_difference(co, np.maximum(cc // 2, 1), ho, np.maximum(hc // 2, 1)) <= 1e-12
     _, _, rc, history = A.adaptive_frame(oracle.OracleScene(scene), config, 4, 64, target)
     assert (cc == rc).all() and len(history) == ca.passes
+
+
+@pytest.mark.parametrize("target,budget", [(1e-3, 0), (3e-3, 0), (1e-3, 64 * 64 * 24), (1e-3, 64 * 64 * 12)])
+def test_cellular_frame_matches_oracle(target, budget):
+    """Cellular generations (and frames by a sample budget): the same tiles live at every generation, so every pixel's
+    count equals the oracle's; a budget is spent exactly."""
+    scene, config = scenes.cornell(width=64, height=64, spp=64)
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        sums, odd, counts, st, a = lane.render_adaptive(config, 4, 64, target, cellular=True, sample_budget=budget)
+    finally:
+        lane.close()
+    rs, ro, rc, history = A.cellular_frame(oracle.OracleScene(scene), config, 4, 64, target, budget)
+    print(f"cellular target {target} budget {budget}: passes {a.passes}, generations {a.tree_updates}, samples "
+          f"{int(counts.sum())} ({counts.mean():.1f} a pixel, {counts.min()}..{counts.max()})")
+    assert st.errors == 0 and st.audit_failures == 0
+    assert a.passes == len(history)
+    assert (counts == rc).all()
+    assert mean_relative_difference(sums, counts, rs, rc) <= TOL
+    if budget:
+        assert int(counts.sum()) == budget

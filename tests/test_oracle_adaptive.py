@@ -174,3 +174,35 @@ def test_trees_converge_and_stay_open():
     r = A.Regions(w, h, F(1e-3))
     r.update(loud)
     assert all(len(t.nodes) == 1 for t in r.trees)
+
+
+@pytest.mark.parametrize("w,h", [(64, 64), (37, 23), (100, 60)])
+@pytest.mark.parametrize("budget", [False, True])
+def test_cellular_step_matches_oracle(w, h, budget):
+    """One generation of the cellular rules: the product's host step equals the oracle's, and the rules hold."""
+    rng = np.random.default_rng(w * 7 + h + (1000 if budget else 0))
+    errors = error_image(rng, w, h, "missing")
+    tiles_x, tiles_y = (w + 7) // 8, (h + 7) // 8
+    cfg = type("C", (), {"width": w, "height": h, "viewport": [2, 1, w - 3, h]})
+    inside = A.in_image(cfg)
+    inside[rng.integers(0, len(inside), 2)] = 0  # a few tiles outside the render region
+    alive = (rng.random(len(inside)) < 0.7).astype(np.uint8) & (inside != 0)
+    planned = rng.integers(1, 40, w * h).astype(np.uint32) if budget else None
+    left = float(rng.integers(1000, 60000)) if budget else 0.0
+    target = 1e-3
+    o = A.Cellular(w, h, target)
+    o.dead_forever(inside)
+    o.alive = [int(v) for v in alive]
+    live = o.update(errors, (planned, left, 64, 0.0) if budget else None)
+    nxt, ratios, plive = L.cellular_step(w, h, target, alive, inside, errors, planned, left, 64, 0.0)
+    assert plive == live and list(nxt) == o.alive
+    assert (ratios.view(np.uint32) == np.array(o.ratio, np.float32).view(np.uint32)).all()
+    assert 0 < live < len(inside)
+    for t in range(len(inside)):
+        r = float(o.ratio[t])
+        if not int(inside[t]):
+            assert not nxt[t]
+        elif r >= 1:
+            assert nxt[t]
+        elif r < 0.5:
+            assert not nxt[t]

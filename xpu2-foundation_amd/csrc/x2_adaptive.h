@@ -69,4 +69,33 @@ private:
     std::vector<Tree> trees_;
 };
 
+// Cellular generations (async_adaptive.h:147-317, the preview session's default): each adaptive pass a generation of
+// the image's 8x8 tiles as cells. A tile's error is MoonRay's node error over it against the target; its own error
+// keeps a tile alive at 1 or more, live neighbours keep it alive down to Survive (3 of its 8) and bring a dead one
+// back from Birth (5 of 8). With a budget the target is the one the samples still to spend would bring every tile to.
+struct Budget {
+    const std::vector<uint32_t>* samples;  // each pixel's samples planned so far
+    double remaining;                      // the frame's samples still to spend
+    uint32_t most;                         // the most samples a pixel may take
+    float floor;                           // the error no tile need go below (0: none)
+};
+class Cellular {
+public:
+    Cellular(int width, int height, float target);
+    void deadForever(const std::vector<uint64_t>& inImage);  // tiles without a pixel in the render region
+    size_t update(const float* errors, const Budget* budget);  // the next generation; its live tiles
+    bool alive(size_t tile) const { return alive_[tile] != 0; }
+    float ratio(size_t tile) const { return ratio_[tile]; }
+    const std::vector<char>& aliveTiles() const { return alive_; }
+    void setAlive(const std::vector<char>& a) { alive_ = a; }
+
+private:
+    float budgetTarget(const Budget& b);
+    int width_, height_, tilesX_, tilesY_;
+    float target_, lastTarget_ = 0;
+    std::vector<char> alive_, next_, dead_;
+    std::vector<float> ratio_, error_;
+    std::vector<double> pixels_, samples_;
+};
+
 }  // namespace x2::adaptive

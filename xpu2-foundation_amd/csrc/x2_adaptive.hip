@@ -124,4 +124,89 @@ uint64_t Regions::sampleArea(int tx, int ty) const {
     return trees_[size_t(ry * perDim_ + rx)].sampleArea(tx, ty);
 }
 
+Cellular::Cellular(int width, int height, float target)
+    : width_(width), height_(height), tilesX_((width + 7) / 8), tilesY_((height + 7) / 8), target_(target),
+      alive_(size_t(tilesX_) * tilesY_, 1), next_(alive_.size(), 0), ratio_(alive_.size(), 0.0f), error_(alive_.size(), 0.0f),
+      pixels_(alive_.size(), 0.0), samples_(alive_.size(), 0.0) {}
+
+void Cellular::deadForever(const std::vector<uint64_t>& inImage) {
+    dead_.assign(alive_.size(), 0);
+    for (size_t t = 0; t < alive_.size() && t < inImage.size(); ++t)
+        if (!inImage[t]) dead_[t] = 1, alive_[t] = 0;
+}
+
+size_t Cellular::update(const float* errors, const Budget* budget) {
+    for (int ty = 0; ty < tilesY_; ++ty)
+        for (int tx = 0; tx < tilesX_; ++tx) {
+            const size_t t = size_t(ty) * tilesX_ + tx;
+            double sum = 0, most = 0, rendered = 0;
+            int n = 0;
+            bool unknown = false;
+            for (int y = ty * 8; y < std::min(height_, ty * 8 + 8); ++y)
+                for (int x = tx * 8; x < std::min(width_, tx * 8 + 8); ++x) {
+                    const size_t i = size_t(y) * size_t(width_) + size_t(x);
+                    if (errors[i] < 0) unknown = true;
+                    else sum += errors[i];
+                    ++n;
+                    if (budget) {
+                        const double s = (*budget->samples)[i];
+                        most = std::max(most, s);
+                        rendered += s > 0 ? 1 : 0;
+                    }
+                }
+            error_[t] = unknown ? -1.0f : nodeError(float(sum), float(n));
+            pixels_[t] = rendered;
+            samples_[t] = most;
+        }
+    const float target = budget ? std::max(budgetTarget(*budget), budget->floor) : target_;
+    for (size_t t = 0; t < ratio_.size(); ++t) ratio_[t] = error_[t] < 0 ? 3.4e38f : error_[t] / target;
+    size_t live = 0;
+    constexpr float Survive = 0.5f, Birth = 0.75f;
+    constexpr int SurviveNeighbours = 3, BirthNeighbours = 5;
+    for (int ty = 0; ty < tilesY_; ++ty)
+        for (int tx = 0; tx < tilesX_; ++tx) {
+            const size_t t = size_t(ty) * tilesX_ + tx;
+            int around = 0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int x = tx + dx, y = ty + dy;
+                    if ((dx || dy) && x >= 0 && y >= 0 && x < tilesX_ && y < tilesY_ && alive_[size_t(y) * tilesX_ + x]) ++around;
+                }
+            const float r = ratio_[t];
+            const bool stays = alive_[t] ? (r >= 1.0f || (r >= Survive && around >= SurviveNeighbours))
+                                         : (r >= 1.0f || (r >= Birth && around >= BirthNeighbours));
+            next_[t] = stays && (dead_.empty() || !dead_[t]) ? 1 : 0;
+            live += next_[t] ? 1 : 0;
+        }
+    alive_.swap(next_);
+    return live;
+}
+
+// The target whose cost — each tile's rendered pixels times the samples that bring its error down to it, n (error /
+// target)^2 less the n it has, at most the most a pixel may take — is the samples left; bisected in log space (the
+// cost falls as the target rises). No samples left, or nothing left to refine: no tile lives on its own error.
+float Cellular::budgetTarget(const Budget& b) {
+    double lo = 3.4e38, hi = 0;
+    for (size_t t = 0; t < error_.size(); ++t)
+        if ((dead_.empty() || !dead_[t]) && error_[t] > 0 && samples_[t] > 0)
+            lo = std::min(lo, double(error_[t])), hi = std::max(hi, double(error_[t]));
+    if (b.remaining <= 0 || hi <= 0) return lastTarget_ = 3.4e38f;
+    auto cost = [&](double target) {
+        double c = 0;
+        for (size_t t = 0; t < error_.size(); ++t) {
+            if ((!dead_.empty() && dead_[t]) || error_[t] <= 0 || samples_[t] <= 0) continue;
+            const double n = samples_[t], r = double(error_[t]) / target;
+            c += pixels_[t] * (std::min(std::max(n * r * r, n), double(b.most)) - std::min(n, double(b.most)));
+        }
+        return c;
+    };
+    double low = lo * 1e-3, high = hi;
+    if (cost(low) <= b.remaining) return lastTarget_ = float(low);
+    for (int i = 0; i < 48; ++i) {
+        const double middle = sqrt(low * high);
+        (cost(middle) > b.remaining ? low : high) = middle;
+    }
+    return lastTarget_ = float(high);
+}
+
 }  // namespace x2::adaptive

@@ -1058,6 +1058,7 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
     if (!d || !c || !a) return fail(X2_ERR_ARGUMENT, "null device, config or adaptive settings");
     if (a->max_samples == 0 || a->max_samples > 65535 || !(a->target_error > 0.0f))
         return fail(X2_ERR_ARGUMENT, "max_samples must be 1..65535 and target_error positive");
+    if (a->cellular && a->device_chain) return fail(X2_ERR_ARGUMENT, "cellular generations run on the host, not the chain");
     x2_config cfg = *c;
     cfg.spp = a->max_samples;
     int r = x2dev_begin_adaptive(d, &cfg, spc);
@@ -1067,6 +1068,7 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
     const std::vector<adaptive::Pass> passes = adaptive::schedule(a->max_samples);
     if (passes.size() > MaxPasses) return fail(X2_ERR_ARGUMENT, "adaptive pass budget (160 passes)");
     adaptive::Regions regions(int(width), int(height), a->target_error);
+    adaptive::Cellular cells(int(width), int(height), a->target_error);
     std::vector<uint64_t> inImage(tiles, 0), masks(tiles);
     for (uint32_t t = 0; t < tiles; ++t)
         for (uint32_t q = 0; q < TilePixels; ++q) {
@@ -1075,7 +1077,9 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
                 int(x) < cfg.viewport[2] && int(y) < cfg.viewport[3])
                 inImage[t] |= 1ull << q;
         }
+    if (a->cellular) cells.deadForever(inImage);
     std::vector<uint32_t> planned(size_t(width) * height, 0);
+    double spent = 0;  // the samples of the passes generated (a frame by sampling generations)
     x2_frame_stats st{};
     a->passes = 0;
     a->tree_updates = 0;
@@ -1086,12 +1090,44 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
     for (uint32_t j = 0; j < passes.size() && !a->device_chain; ++j) {
         const bool uniform = passes[j].start < a->min_samples;
         const uint32_t size = passes[j].end - passes[j].start;
-        if (!uniform) {  // every pass before this one has completed: the trees see all of them
+        const bool budgeted = a->cellular && a->sample_budget;
+        if (!uniform) {  // every pass before this one has completed: the trees (or cells) see all of them
+            if (budgeted && spent >= double(a->sample_budget)) break;  // the budget is spent
             r = x2dev_request_errors(d);
             if (r) return r;
             X2_HIP(hipEventSynchronize(d->errorsDone));
-            regions.update(d->hostErrors);
             ++a->tree_updates;
+            if (a->cellular) {
+                const adaptive::Budget budget{&planned, std::max(0.0, double(a->sample_budget) - spent), a->max_samples,
+                                              a->noise_floor};
+                if (!cells.update(d->hostErrors, budgeted ? &budget : nullptr)) break;  // no tile lives: converged
+            } else {
+                regions.update(d->hostErrors);
+            }
+        }
+        // A frame by sampling generations: a pass no larger than the samples left buys, the tiles furthest above the
+        // target first (ties by tile), async_render.h:1372-1394.
+        std::vector<char> dropped;
+        if (!uniform && budgeted) {
+            std::vector<std::pair<float, uint32_t>> live;
+            double wanted = 0;
+            for (uint32_t t = 0; t < tiles; ++t)
+                if (inImage[t] && cells.alive(t)) {
+                    live.push_back({cells.ratio(t), t});
+                    wanted += double(__builtin_popcountll(inImage[t])) * size;
+                }
+            const double left = std::max(0.0, double(a->sample_budget) - spent);
+            if (wanted > left) {
+                std::stable_sort(live.begin(), live.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+                dropped.assign(tiles, 1);
+                double taken = 0;
+                for (const auto& [ratio, t] : live) {
+                    const double cost = double(__builtin_popcountll(inImage[t])) * size;
+                    if (taken + cost > left) break;
+                    taken += cost;
+                    dropped[t] = 0;
+                }
+            }
         }
         uint64_t samples = 0;
         std::vector<x2_chunk> chunks;
@@ -1102,7 +1138,10 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
                     for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx) {
                         const uint32_t t = ty * tilesX + tx;
                         uint64_t active = inImage[t];
-                        if (!uniform) active &= regions.sampleArea(int(tx * 8), int(ty * 8));
+                        if (!uniform && a->cellular)
+                            active &= cells.alive(t) && (dropped.empty() || !dropped[t]) ? ~0ull : 0ull;
+                        else if (!uniform)
+                            active &= regions.sampleArea(int(tx * 8), int(ty * 8));
                         masks[t] = active;
                         if (!active) continue;
                         const uint32_t n = uint32_t(__builtin_popcountll(active));
@@ -1116,6 +1155,7 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
                         }
                     }
         if (!samples) break;  // the trees have converged: nothing changes after this
+        spent += double(samples);
         r = x2dev_upload_pass(d, j % PassSlots, masks.data(), uniform ? passes[j].start : X2_NO_SLOT);
         if (r) return r;
         r = runChunks(d, chunks, &chunkSamples, st);
@@ -1144,6 +1184,27 @@ int x2_adaptive_masks(uint32_t width, uint32_t height, float target_error, const
     for (uint32_t t = 0; t < tilesX * tilesY; ++t) masks[t] = regions.sampleArea(int(t % tilesX) * 8, int(t / tilesX) * 8);
     if (regions_done) *regions_done = regions.done() ? 1u : 0u;
     return X2_OK;
+}
+
+int x2_cellular_step(uint32_t width, uint32_t height, float target_error, const uint8_t* alive_in,
+                     const uint64_t* in_image, const float* errors, const uint32_t* planned, double samples_left,
+                     uint32_t max_samples, float noise_floor, uint8_t* alive_out, float* ratios) {
+    if (!alive_in || !in_image || !errors || !alive_out || !width || !height) return fail(X2_ERR_ARGUMENT, "null buffer");
+    const size_t tiles = size_t((width + 7) / 8) * ((height + 7) / 8);
+    adaptive::Cellular cells(int(width), int(height), target_error);
+    cells.deadForever(std::vector<uint64_t>(in_image, in_image + tiles));
+    std::vector<char> alive(tiles);
+    for (size_t t = 0; t < tiles; ++t) alive[t] = alive_in[t] && in_image[t] ? 1 : 0;  // dead tiles stay dead
+    cells.setAlive(alive);
+    std::vector<uint32_t> plan;
+    if (planned) plan.assign(planned, planned + size_t(width) * height);
+    const adaptive::Budget budget{&plan, samples_left, max_samples, noise_floor};
+    const size_t live = cells.update(errors, planned ? &budget : nullptr);
+    for (size_t t = 0; t < tiles; ++t) {
+        alive_out[t] = cells.alive(t) ? 1 : 0;
+        if (ratios) ratios[t] = cells.ratio(t);
+    }
+    return int(live);
 }
 
 int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors) {

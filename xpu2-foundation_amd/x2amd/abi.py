@@ -120,7 +120,8 @@ class FrameStats(ctypes.Structure):
 
 class Adaptive(ctypes.Structure):
     _fields_ = [("min_samples", u32), ("max_samples", u32), ("target_error", f32), ("passes", u32),
-                ("tree_updates", u32), ("device_chain", u32)]
+                ("tree_updates", u32), ("device_chain", u32),
+                ("cellular", u32), ("sample_budget", u32), ("noise_floor", f32), ("pad", u32)]
 
 
 class RngQuery(ctypes.Structure):
@@ -147,7 +148,7 @@ EXPECTED_SIZES = {
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 292),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 80),
-    "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 24),
+    "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
     assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
@@ -190,6 +191,9 @@ SIGNATURES = {
     "x2dev_render_adaptive": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.POINTER(Adaptive),
                                              ctypes.POINTER(FrameStats)]),
     "x2_adaptive_masks": (ctypes.c_int, [u32, u32, f32, ctypes.POINTER(f32), ctypes.POINTER(u64), ctypes.POINTER(u32)]),
+    "x2_cellular_step": (ctypes.c_int, [u32, u32, f32, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64),
+                                        ctypes.POINTER(f32), ctypes.POINTER(u32), f64, u32, f32,
+                                        ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(f32)]),
     "x2_pixel_errors": (ctypes.c_int, [ctypes.POINTER(f64), ctypes.POINTER(f64), ctypes.POINTER(u32), u32,
                                        ctypes.POINTER(f32)]),
     "x2batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),

@@ -68,11 +68,13 @@ class GpuLane:
         n = self.config.width * self.config.height
         return np.ctypeslib.as_array(o, shape=(n * 4,)).reshape(n, 4).copy()
 
-    def render_adaptive(self, config, min_samples, max_samples, target_error, samples_per_chunk=4, device_chain=False):
-        """One adaptive frame (host trees, or the device pass chain); returns (sums, odd, counts, FrameStats,
-        x2_adaptive with passes and tree updates)."""
+    def render_adaptive(self, config, min_samples, max_samples, target_error, samples_per_chunk=4, device_chain=False,
+                        cellular=False, sample_budget=0, noise_floor=0.0):
+        """One adaptive frame (host trees, the device pass chain, or cellular generations with an optional sample
+        budget); returns (sums, odd, counts, FrameStats, x2_adaptive with passes and tree updates)."""
         st = abi.FrameStats()
-        a = abi.Adaptive(min_samples, max_samples, target_error, 0, 0, 1 if device_chain else 0)
+        a = abi.Adaptive(min_samples, max_samples, target_error, 0, 0, 1 if device_chain else 0, 1 if cellular else 0,
+                         sample_budget, noise_floor, 0)
         r = self.lib.x2dev_render_adaptive(self.h, ctypes.byref(config), samples_per_chunk, ctypes.byref(a),
                                            ctypes.byref(st))
         self.config = config
@@ -185,6 +187,26 @@ def adaptive_masks(width, height, target_error, errors):
                                     masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(done)),
               "x2_adaptive_masks")
     return masks, bool(done.value)
+
+
+def cellular_step(width, height, target_error, alive, in_image, errors, planned=None, samples_left=0.0, max_samples=0,
+                  noise_floor=0.0):
+    """One generation of the cellular rules on the host (x2_cellular_step): (next alive uint8 per tile, ratios, live)."""
+    lib = abi.load()
+    a = np.ascontiguousarray(alive, np.uint8)
+    m = np.ascontiguousarray(in_image, np.uint64)
+    e = np.ascontiguousarray(errors, np.float32)
+    out = np.zeros_like(a)
+    ratios = np.zeros(len(a), np.float32)
+    p = None if planned is None else np.ascontiguousarray(planned, np.uint32)
+    live = abi.check(lib.x2_cellular_step(width, height, target_error, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                          m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                          e.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                          None if p is None else p.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                          samples_left, max_samples, noise_floor,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                          ratios.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "x2_cellular_step")
+    return out, ratios, live
 
 
 def pixel_errors(sums, odd, counts):
