@@ -304,6 +304,11 @@ int x2dev_upload_pass(x2dev* d, uint32_t slot, const uint64_t* masks, uint32_t u
 int x2dev_request_errors(x2dev* d);
 int x2dev_errors_ready(x2dev* d);
 int x2dev_errors(const x2dev* d, const float** errors);
+/* GpuLane::seedFilm (async_gpu.h:1594-1609): a frame taken up from a saved film starts with its sums, odd-index sums
+ * (may be null outside adaptive frames) and counts, so the samples still to come, and an adaptive frame's bases and
+ * errors, go on from them. After begin / begin_adaptive, ahead of any launch; the host buffers are copied before it
+ * returns. */
+int x2dev_seed_film(x2dev* d, const double* sums, const double* odd, const uint32_t* counts);
 /* GpuLane::odd() (async_gpu.h:1706): the odd-index sums, downloaded with the film in an adaptive frame. */
 int x2dev_odd(const x2dev* d, const double** odd);
 /* One adaptive frame through the broker's sequence (async_render.h:1219-1440): the host's error trees between passes,

@@ -692,6 +692,20 @@ int x2dev_film(const x2dev* d, const double** sums, const uint32_t** counts) {
     return X2_OK;
 }
 
+int x2dev_seed_film(x2dev* d, const double* sums, const double* odd, const uint32_t* counts) {
+    if (!d || !sums || !counts) return fail(X2_ERR_ARGUMENT, "null device, sums or counts");
+    if (!d->begun_) return fail(X2_ERR_STATE, "seed before begin");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "seed with launches in flight");
+    X2_HIP(hipSetDevice(d->device));
+    const size_t pixels = d->filmPixels;
+    X2_HIP(hipMemcpyAsync(d->sums, sums, pixels * 4 * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    X2_HIP(hipMemcpyAsync(d->counts, counts, pixels * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream));
+    if (odd && d->adaptive)
+        X2_HIP(hipMemcpyAsync(d->odd, odd, pixels * 4 * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    X2_HIP(hipStreamSynchronize(d->stream));  // the caller's buffers may go once this returns
+    return X2_OK;
+}
+
 int x2dev_odd(const x2dev* d, const double** odd) {
     if (!d || !odd) return fail(X2_ERR_ARGUMENT, "null argument");
     if (!d->adaptive) return fail(X2_ERR_STATE, "the odd-index film is kept in adaptive frames only");

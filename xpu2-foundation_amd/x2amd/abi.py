@@ -187,6 +187,7 @@ SIGNATURES = {
     "x2dev_request_errors": (ctypes.c_int, [ctypes.c_void_p]),
     "x2dev_errors_ready": (ctypes.c_int, [ctypes.c_void_p]),
     "x2dev_errors": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(f32))]),
+    "x2dev_seed_film": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(f64), ctypes.POINTER(f64), ctypes.POINTER(u32)]),
     "x2dev_odd": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(f64))]),
     "x2dev_render_adaptive": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.POINTER(Adaptive),
                                              ctypes.POINTER(FrameStats)]),

@@ -61,6 +61,15 @@ class GpuLane:
         abi.check(self.lib.x2dev_errors(self.h, ctypes.byref(e)), "x2dev_errors")
         return np.ctypeslib.as_array(e, shape=(self.config.width * self.config.height,)).copy()
 
+    def seed_film(self, sums, counts, odd=None):
+        """GpuLane::seedFilm: the frame starts from a saved film (after begin, ahead of any launch)."""
+        s = np.ascontiguousarray(sums, np.float64)
+        c = np.ascontiguousarray(counts, np.uint32)
+        o = None if odd is None else np.ascontiguousarray(odd, np.float64)
+        abi.check(self.lib.x2dev_seed_film(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           None if o is None else o.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))), "x2dev_seed_film")
+
     def odd(self):
         """The odd-index sums [pixels, 4] of the downloaded film (adaptive frames)."""
         o = ctypes.POINTER(ctypes.c_double)()
