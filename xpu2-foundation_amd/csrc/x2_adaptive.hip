@@ -74,13 +74,24 @@ void Tree::updateNode(int index) {
     }
 }
 
+// Depth first from the root, skipping converged subtrees and those that miss the tile; each open leaf's overlap
+// with the tile, rounded outward, is sampled.
 uint64_t Tree::sampleArea(int tx, int ty) const {
+    const FBox tile{float(tx), float(ty), float(tx + 8), float(ty + 8)};
+    std::vector<int> stack{0};
     uint64_t bits = 0;
-    for (const Node& n : nodes_) {
-        if (n.children >= 0 || n.complete) continue;  // open leaves only
-        const FBox c{std::max(n.bounds.x0, float(tx)), std::max(n.bounds.y0, float(ty)), std::min(n.bounds.x1, float(tx + 8)),
-                     std::min(n.bounds.y1, float(ty + 8))};
+    while (!stack.empty()) {
+        const Node& n = nodes_[size_t(stack.back())];
+        stack.pop_back();
+        if (n.complete) continue;
+        const FBox c{std::max(n.bounds.x0, tile.x0), std::max(n.bounds.y0, tile.y0), std::min(n.bounds.x1, tile.x1),
+                     std::min(n.bounds.y1, tile.y1)};
         if (c.x1 <= c.x0 || c.y1 <= c.y0) continue;
+        if (n.children >= 0) {
+            stack.push_back(n.children);
+            stack.push_back(n.children + 1);
+            continue;
+        }
         const IBox r = outward(c);
         for (int y = std::max(r.y0, ty); y < std::min(r.y1, ty + 8); ++y)
             for (int x = std::max(r.x0, tx); x < std::min(r.x1, tx + 8); ++x) bits |= 1ull << ((y - ty) * 8 + (x - tx));

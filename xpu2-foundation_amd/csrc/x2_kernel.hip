@@ -1994,15 +1994,18 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
 #define X2_REGEN_LANES 32
 #endif
 // The launch's next sample: claims items until one lies in the image and the viewport (false: none left).
-__device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel, uint32_t& index) {
+// The device pass chain's render: its items, fixed by the compaction before the render (0: the chain has stopped).
+__device__ __forceinline__ unsigned long long chainItems(const RenderParams& P) {
+    if (!P.chain || P.chain->stop) return 0;
+    return (unsigned long long)P.chain->activeTiles * P.chainSize * TilePixels;
+}
+__device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel, uint32_t& index,
+                                          unsigned long long chainEnd = 0) {
     for (;;) {
         if (P.chain) {  // the device pass chain's render: its active tiles' pixels, chainSize samples each
             const unsigned long long perTile = (unsigned long long)P.chainSize * TilePixels;
             const unsigned long long item = atomicAdd(&P.chain->claimed, 1ull);
-            const ChainState* cs = P.chain;
-            if (*reinterpret_cast<const volatile uint32_t*>(&cs->stop) ||
-                item >= (unsigned long long)(*reinterpret_cast<const volatile uint32_t*>(&cs->activeTiles)) * perTile)
-                return false;
+            if (item >= chainEnd) return false;
             const uint32_t t = uint32_t(item / perTile), rest = uint32_t(item % perTile);
             const uint32_t position = rest % TilePixels;
             if (!(P.chainMasks[t] >> position & 1ull)) continue;
@@ -2021,7 +2024,8 @@ struct LaneFeed {
     static constexpr uint32_t Regen = X2_REGEN_LANES;  // finished lanes of a wave that take new samples together
     const RenderParams& P;
     uint32_t& errors;
-    __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem(P, pixel, index); }
+    unsigned long long chainEnd = chainItems(P);  // read once: the compaction wrote it before this render
+    __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem(P, pixel, index, chainEnd); }
     __device__ void done(uint32_t pixel, uint32_t index, V3 r, float a) {
         if constexpr (!Adj) filmAdd(P, pixel, index, r.x, r.y, r.z, a);
     }

@@ -113,10 +113,11 @@ constexpr uint32_t PassSlots = 4, PassShift = 16, MaxPasses = 160;  // async_int
 // The device pass chain (x2_kernel.hip chain*Kernel; async_adaptive.cu): a frame's adaptive passes queued at once,
 // each pass's trees, masks and active tiles made on the device from the errors of every pass before it.
 struct ChainState {
-    unsigned long long claimed;  // the current pass's items claimed
+    unsigned long long claimed;  // the current pass's items claimed (a line of its own: every claim's atomic)
+    uint32_t padClaimed[30];
     uint32_t stop;               // the trees converged or a pass came out empty: every later step does nothing
     uint32_t activeTiles;        // the current pass's active tiles
-    uint32_t passes, treeUpdates, pad[2];
+    uint32_t passes, treeUpdates, pad[28];
 };
 struct ChainRegion {
     int x0, y0, x1, y1;
