@@ -14,7 +14,7 @@ adjoint_step_over_forward is the adjoint frame's time over the forward frame's. 
 participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp); the batch leg the 64-job batched
 workload, persistent kernel from a hipGraph against one launch per job; the direct leg the prototype estimator on
 the headline scene; the adaptive leg an adaptive frame of the headline scene through the device pass chain and
-through the host's trees. roofline.copy_ceiling_gbs is a measured device copy's bandwidth beside the published peak.
+through the host's trees; the grid leg the headline frame from the work grid through the broker. roofline.copy_ceiling_gbs is a measured device copy's bandwidth beside the published peak.
 """
 import argparse
 import json
@@ -263,6 +263,38 @@ def adaptive_leg(reps, target=3e-4, min_samples=16, max_samples=64):
             "host_trees_ms_per_frame": host * 1e3, "same_samples": bool((host_counts == counts).all())}
 
 
+def grid_leg(reps, windows=(100.0, 10.0)):
+    """The headline frame from the work grid through the broker (x2dev_render_grid, the GPU alone): the grid's
+    progressive sample ranges (1, 1, 2, then 4) handed out by the frontier, the GPU's 32 cellular workers each taking a
+    share of a launch, launches sized to about window_ms of the GPU's measured rate (100 ms, and the reference's
+    default of 10 ms). Each frame's time includes building its grid."""
+    from x2amd import grid as G
+    from x2amd import scenes
+
+    scene, config = scenes.icospheres()
+    lane = make_lane()
+    lane.prepare(scene)
+    out = {"workload": f"icospheres {scene.triangle_count} tris, {config.width}x{config.height}, {config.spp} spp, "
+                       "work grid of 64-pixel cells, GPU alone", "reps": reps}
+    for window in windows:
+        lane.render_grid(config, G.WorkGrid(config.width, config.height, config.spp, G.options(64, 4, 1)),
+                         window_ms=window)
+        wall, launches, chunks = 0.0, 0, 0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            g = G.WorkGrid(config.width, config.height, config.spp, G.options(64, 4, 1))
+            _, counts, fr = lane.render_grid(config, g, window_ms=window)
+            wall += time.perf_counter() - t0
+            assert fr.errors == 0 and (counts == config.spp).all()
+            launches += fr.launches
+            chunks = fr.gpu_chunks
+        paths = config.width * config.height * config.spp * reps
+        out[f"window_{int(window)}ms"] = {"ms_per_frame": wall / reps * 1e3, "mpaths_per_s": paths / wall / 1e6,
+                                          "launches_per_frame": launches / reps, "chunks_per_frame": chunks}
+    lane.close()
+    return out
+
+
 def copy_ceiling(mib=1024):
     """GB/s of a plain device-to-device copy (read + write bytes over the copy's time), the chip's practical
     bandwidth beside the 8 TB/s peak."""
@@ -331,6 +363,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-adaptive", action="store_true")
+    ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB: only run a device copy (for --pmc scaling)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--launch-chunks", type=int, default=LAUNCH_CHUNKS, help="max_chunks_per_launch of the lanes")
@@ -416,6 +449,8 @@ def main():
         out["direct"] = direct_leg(2)
     if not args.no_adaptive:
         out["adaptive"] = adaptive_leg(2)
+    if not args.no_grid:
+        out["grid"] = grid_leg(2)
     out["roofline"]["copy_ceiling_gbs"] = copy_ceiling()
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)

@@ -368,6 +368,115 @@ int x2batch_job_film(x2batch* b, uint32_t job, const double** sums, const uint32
 uint32_t x2batch_job_count(const x2batch* b);
 void x2batch_destroy(x2batch* b);
 
+/* ---- The work grid and the broker (async_grid.h:38-439, async_render.h:590-830,1668-1718). A frame's chunks wait in
+ * the grid's cells; the GPU lane's broker and the CPU lane's workers claim them by the cellular rules, each chunk
+ * exactly once, so a frame can be rendered by the GPU alone or by both devices together. Host code only: every call
+ * here but x2dev_render_grid runs without a device. Cells and devices are indices; X2_NO_CELL: none. */
+#define X2_DEVICE_CPU 0u
+#define X2_DEVICE_GPU 1u
+#define X2_NO_CELL 0xffffffffu
+/* GridOptions (async_grid.h:103-113). */
+typedef struct x2_grid_options {
+    uint32_t cell_size;         /* pixels, a multiple of 8 (64) */
+    uint32_t samples_per_chunk; /* sample indices a chunk holds at most (4) */
+    int32_t mode;               /* 0: CPU only, 1: GPU only, 2: both (2) */
+    uint32_t progressive;       /* nonzero: ranges of 1, 1, 2, then samples_per_chunk, handed out by the frontier */
+    uint32_t sample_first;      /* a share of the frame: its sample indices [sample_first, sample_first + */
+    uint32_t sample_count;      /* sample_count) (0: all of them) */
+} x2_grid_options;
+typedef struct x2_grid x2_grid;
+/* Grid::Grid (async_grid.h:131-177): owners (owner_count == the cell count, else ignored) X2_DEVICE_* per cell, null for
+ * the checkerboard; empty != 0: the cells start empty (chunks appended later); viewport {x0, y0, x1, y1} half open, or
+ * null. The defaults above apply to a null options. */
+int x2_grid_create(uint32_t width, uint32_t height, uint32_t spp, const x2_grid_options* options, const uint8_t* owners,
+                   uint32_t owner_count, int empty, const int32_t* viewport, x2_grid** out);
+void x2_grid_destroy(x2_grid* g);
+uint32_t x2_grid_cell_count(const x2_grid* g);
+uint64_t x2_grid_total_chunks(const x2_grid* g);
+uint32_t x2_grid_owner(const x2_grid* g, uint32_t cell);
+/* A cell's unclaimed chunks (0 once closed) and those a device may claim now (its frontier allowing). */
+uint32_t x2_grid_remaining(const x2_grid* g, uint32_t cell);
+uint32_t x2_grid_claimable(const x2_grid* g, uint32_t cell, uint32_t device);
+int x2_grid_empty(const x2_grid* g);
+/* Grid::claim (async_grid.h:231-237): up to n chunks of the cell into out; returns how many. */
+uint32_t x2_grid_claim(x2_grid* g, uint32_t cell, uint32_t n, x2_chunk* out, uint32_t device);
+/* Grid::append (:182-186), the coordinator only: more chunks for a cell. */
+int x2_grid_append(x2_grid* g, uint32_t cell, const x2_chunk* chunks, uint32_t n);
+/* Grid::choose (:325-359): the cell a worker of the device works next from `current` (X2_NO_CELL: none yet), or
+ * X2_NO_CELL when it may take nothing now; salt spreads workers over equal cells. Grid::seed (:362-368): the w-th of
+ * n workers' starting cell. */
+uint32_t x2_grid_choose(const x2_grid* g, uint32_t device, uint32_t current, uint32_t salt);
+uint32_t x2_grid_seed(const x2_grid* g, uint32_t device, uint32_t worker, uint32_t workers);
+/* The progressive order's frontier of a device, and the frame's sample ranges (first, count pairs into out, up to
+ * capacity; returns how many there are). */
+uint32_t x2_grid_frontier(const x2_grid* g, uint32_t device);
+uint32_t x2_grid_ranges(const x2_grid* g, uint32_t* out, uint32_t capacity);
+/* Important parts first (:250-268): each cell's score (below zero: unknown); cells above the mean are important. */
+int x2_grid_set_scores(x2_grid* g, const float* scores, uint32_t n);
+int x2_grid_important(const x2_grid* g, uint32_t cell);
+int x2_grid_reserved(const x2_grid* g, uint32_t cell);
+/* The scene indexer's per-cell probability that the GPU is the efficient owner (device_suit.h). */
+int x2_grid_set_suitability(x2_grid* g, const float* p, uint32_t n);
+/* The rights of passage (:279-308): the GPU's samples a second, one CPU worker's, the CPU workers, a GPU launch's
+ * target seconds; the time of the GPU's last claim; whether a device may work the other's reserved cell now. */
+void x2_grid_set_rates(x2_grid* g, double gpu_samples, double cpu_worker_samples, uint32_t cpu_workers,
+                       double window_seconds);
+void x2_grid_gpu_claimed(x2_grid* g);
+int x2_grid_passage(const x2_grid* g, uint32_t device, uint32_t cell);
+/* A drain: nothing more is handed out and the grid counts as empty. */
+void x2_grid_close(x2_grid* g);
+/* Samples rendered in a cell by a device (for the ownership flips), and a GPU sample re-rendered by the CPU. */
+void x2_grid_rendered(x2_grid* g, uint32_t cell, uint32_t device, uint64_t samples);
+uint64_t x2_grid_rendered_by(const x2_grid* g, uint32_t cell, uint32_t device);
+void x2_grid_deferred_to_cpu(x2_grid* g, uint32_t tile);
+uint32_t x2_grid_cell_of(const x2_grid* g, uint32_t tile);
+uint32_t x2_grid_tile_pixels(const x2_grid* g, uint32_t tile);
+
+/* Ownership (async_grid.h:366-437): owners persisting across frames. x2_ownership_owners fills cells owners and
+ * returns 1 once known for that many cells, 0 for the checkerboard; x2_ownership_finished records a finished frame. */
+typedef struct x2_ownership x2_ownership;
+int x2_ownership_create(x2_ownership** out);
+void x2_ownership_destroy(x2_ownership* o);
+int x2_ownership_owners(const x2_ownership* o, uint32_t cells, uint8_t* owners);
+int x2_ownership_finished(x2_ownership* o, const x2_grid* g);
+
+/* The CPU lane the broker drives (CpuLane, async_cpu.h:1584-1673): `threads` workers, each calling render for the
+ * chunks it claims, on its own thread; render adds the chunk's samples to the caller's own film and returns 0, or
+ * nonzero to fail the frame. */
+typedef struct x2_cpu_lane {
+    uint32_t threads;
+    uint32_t pad;
+    void* context;
+    int (*render)(void* context, uint32_t worker, const x2_chunk* chunk);
+} x2_cpu_lane;
+/* RenderOptions (async_render.h:43-69): a GPU launch's target duration, the first launches' size before the GPU's
+ * rate is known, and the GPU's cellular workers (each a share of a launch). Zero fields take the defaults 10 ms, 2048,
+ * 32. */
+typedef struct x2_broker_options {
+    double window_ms;
+    uint32_t first_chunks;
+    uint32_t gpu_workers;
+} x2_broker_options;
+typedef struct x2_grid_frame {
+    uint64_t gpu_chunks, cpu_chunks;     /* chunks each device rendered */
+    uint64_t gpu_samples, cpu_samples;   /* pixel samples each device rendered */
+    uint64_t gpu_stolen, cpu_stolen;     /* chunks taken from cells the other device owns */
+    uint64_t launches, errors;           /* GPU launches; device sample errors */
+    double seconds, gpu_rate, cpu_rate;  /* the frame's wall time; the GPU's samples a second (device time); the CPU
+                                            lane's, over the frame */
+    uint64_t rays_trace, rays_shadow, nodes_visited, triangles_tested;
+} x2_grid_frame;
+/* The broker's frame (Renderer::render, async_render.h:590-830): begins the device (x2dev_begin with the grid's
+ * samples_per_chunk), starts the CPU lane's workers (none for a null cpu or mode 1), and feeds the GPU from the grid
+ * (launchSize, async_render.h:1668-1718: launches of about window_ms at its measured rate, the last ones sized so both
+ * lanes finish together) until every chunk is rendered; then downloads the GPU's film (x2dev_film). The grid's chunks
+ * are consumed: a grid renders one frame. */
+int x2dev_render_grid(x2dev* d, const x2_config* config, x2_grid* grid, const x2_cpu_lane* cpu,
+                      const x2_broker_options* options, x2_grid_frame* out);
+/* GpuLane::chunkRoom (async_gpu.h:1725-1729): the chunks the next launch may take (the launch cap, and the mailbox
+ * ring's free room). */
+uint32_t x2dev_chunk_room(const x2dev* d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,11 +12,13 @@ from x2amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "x2dev.h")
+ENTRY = r"\b(x2(?:dev|batch)?_\w+)\s*\("  # the lane's, the batch's and the host calls' (x2_grid_*, x2_adaptive_masks ...)
 
 
 def declared_entry_points():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(x2dev_\w+)\s*\(", text)))
+    text = "".join(open(os.path.join(ROOT, "include", f)).read() for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+                   if f.endswith(".h"))
+    return sorted(set(re.findall(ENTRY, text)))
 
 
 @pytest.mark.parametrize("compiler,ext", [("gcc", "c"), ("g++", "cpp")])
@@ -37,12 +39,12 @@ def test_header_compiles_and_sizes_match(compiler, ext):
 def test_library_exports_every_declared_entry_point():
     lib = abi.load()
     declared = declared_entry_points()
-    assert len(declared) >= 17
+    assert len(declared) >= 70
     for name in declared:
         assert hasattr(lib, name), name
         assert name in abi.SIGNATURES, f"{name} has no ctypes signature"
     nm = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], check=True, capture_output=True, text=True).stdout
-    exported = set(re.findall(r"\bT (x2dev_\w+)", nm))
+    exported = set(re.findall(r"\bT (x2\w+)", nm))
     assert set(declared) <= exported
 
 

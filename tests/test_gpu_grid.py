@@ -1,0 +1,161 @@
+"""Frames from the work grid through the broker (x2dev_render_grid): the GPU alone, and the GPU beside a CPU lane.
+
+The GPU alone renders the film the frame loop renders (counts exact, sums to the FP64 atomics' order), with a
+viewport and a share of the frame's samples too. Beside a CPU lane — here a stand-in that records the chunks it
+claims and takes time over each, so the rights of passage and the end-of-frame split come into play — the two lanes
+claim every chunk exactly once: the GPU's film plus the recorded chunks rendered afterwards equals the whole frame.
+Ownership across frames follows the shares the lanes rendered."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from helpers import drain
+from x2amd import abi, scenes
+from x2amd import grid as G
+from x2amd.lane import GpuLane
+
+pytestmark = pytest.mark.gpu
+
+
+def film(lane):
+    lane.download_film()
+    while not lane.film_ready():
+        pass
+    return lane.film()
+
+
+def render_chunks(lane, config, chunks, spc):
+    """The chunks (tile, first, count) rendered on a fresh begin of the lane; returns its film."""
+    lane.begin(config, spc)
+    cap = lane.max_chunks_per_launch()
+    for i in range(0, len(chunks), cap):
+        part = chunks[i:i + cap]
+        arr = (abi.Chunk * len(part))(*[abi.Chunk(t, f, c, 0) for t, f, c in part])
+        while not lane.ready():
+            lane.poll()
+        assert lane.launch(arr, samples=0)
+    drain(lane)
+    return film(lane)
+
+
+def close(a_sums, b_sums):
+    return np.abs(a_sums - b_sums).max() <= 1e-12 * max(1.0, float(np.abs(b_sums).max()))
+
+
+@pytest.mark.parametrize("window_ms", [0.0, 100.0])
+def test_gpu_alone_equals_frame_loop(window_ms):
+    scene, config = scenes.cornell(width=96, height=80, spp=12)
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        ws, wc, _ = lane.render_frame(config)
+        g = G.WorkGrid(config.width, config.height, config.spp, G.options(32, 4, 1))
+        gs, gc, fr = lane.render_grid(config, g, window_ms=window_ms)
+    finally:
+        lane.close()
+    print(f"window {window_ms} ms: launches {fr.launches}, chunks {fr.gpu_chunks}, {fr.seconds * 1e3:.1f} ms")
+    assert fr.errors == 0 and fr.cpu_chunks == 0 and fr.gpu_chunks == g.total_chunks()
+    assert fr.gpu_samples == config.width * config.height * config.spp
+    assert (gc == wc).all() and (gc == config.spp).all()
+    assert close(gs, ws)
+
+
+def test_viewport_and_sample_share():
+    """A viewport (the render region) and a share of the samples [5, 12): only those, each pixel the indices a whole
+    frame gives it."""
+    scene, config = scenes.cornell(width=80, height=64, spp=16)
+    vp = [9, 5, 70, 50]
+    config.viewport[:] = vp
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        g = G.WorkGrid(config.width, config.height, config.spp, G.options(32, 4, 1, True, 5, 7), viewport=vp)
+        gs, gc, fr = lane.render_grid(config, g)
+        ranges = g.ranges()
+        chunks = [(t, f, c) for t in range(((80 + 7) // 8) * ((64 + 7) // 8)) for f, c in ranges
+                  if g.tile_pixels(t)]
+        rs, rc = render_chunks(lane, config, chunks, 4)
+    finally:
+        lane.close()
+    y, x = np.divmod(np.arange(80 * 64), 80)
+    inside = (x >= vp[0]) & (x < vp[2]) & (y >= vp[1]) & (y < vp[3])
+    assert ranges[0] == (5, 1) and sum(c for _, c in ranges) == 7
+    assert (gc[inside] == 7).all() and (gc[~inside] == 0).all()
+    assert (gc == rc).all() and close(gs, rs)
+    assert fr.gpu_samples == int(inside.sum()) * 7
+
+
+class SlowCpuLane:
+    """A CPU lane stand-in: records each chunk it is handed and takes `seconds` over it."""
+
+    def __init__(self, seconds):
+        self.seconds = seconds
+        self.lock = threading.Lock()
+        self.chunks = []
+
+    def __call__(self, worker, chunk):
+        time.sleep(self.seconds)
+        with self.lock:
+            self.chunks.append((chunk.tile, chunk.first, chunk.count))
+        return 0
+
+
+@pytest.mark.parametrize("mode,cell,seconds,size", [(2, 32, 0.0005, 384), (2, 64, 0.002, 512), (0, 32, 0.0, 96)])
+def test_co_rendered_frame_equals_whole_frame(mode, cell, seconds, size):
+    """Frames of 23,040 and 40,960 chunks: larger than the GPU's first launches (at least two samples a launch thread),
+    so the CPU lane's workers claim beside them."""
+    scene, config = scenes.cornell(width=size, height=size, spp=32 if mode else 8)
+    cpu = SlowCpuLane(seconds)
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        ws, wc, _ = lane.render_frame(config)
+        g = G.WorkGrid(config.width, config.height, config.spp, G.options(cell, 4, mode))
+        gs, gc, fr = lane.render_grid(config, g, cpu_render=cpu, cpu_threads=4, window_ms=2.0, first_chunks=64)
+        cs, cc = render_chunks(lane, config, cpu.chunks, 4)
+    finally:
+        lane.close()
+    print(f"mode {mode} cell {cell}: GPU {fr.gpu_chunks} chunks in {fr.launches} launches (stolen {fr.gpu_stolen}), "
+          f"CPU {fr.cpu_chunks} (stolen {fr.cpu_stolen}), {fr.seconds * 1e3:.1f} ms")
+    assert fr.errors == 0
+    assert fr.cpu_chunks == len(cpu.chunks) and fr.gpu_chunks + fr.cpu_chunks == g.total_chunks()
+    assert len(set((t, f) for t, f, _ in cpu.chunks)) == len(cpu.chunks)
+    assert (gc + cc == config.spp).all(), "every pixel's samples exactly once between the lanes"
+    assert close(gs + cs, ws)
+    if mode == 0:
+        assert fr.gpu_chunks == 0 and fr.launches == 0
+    else:
+        assert fr.gpu_chunks > 0 and fr.cpu_chunks > 0
+
+
+def test_ownership_follows_the_lanes():
+    """Frames from grids whose owners come from the last frames: a slow CPU lane leaves most cells to the GPU, and the
+    owners flip towards it after the 8-frame window; every frame stays whole."""
+    scene, config = scenes.cornell(width=128, height=128, spp=8)
+    own = G.Ownership()
+    lane = GpuLane(0)
+    flipped = []
+    try:
+        lane.prepare(scene)
+        for frame in range(10):
+            owners = own.owners(16)
+            g = G.WorkGrid(config.width, config.height, config.spp, G.options(32, 4, 2), owners=owners)
+            cpu = SlowCpuLane(0.004)
+            gs, gc, fr = lane.render_grid(config, g, cpu_render=cpu, cpu_threads=2, window_ms=2.0, first_chunks=32)
+            assert fr.errors == 0 and fr.gpu_chunks + fr.cpu_chunks == g.total_chunks()
+            cpu_counts = np.zeros_like(gc)
+            for t, f, c in cpu.chunks:
+                tx = t % 16
+                ty = t // 16
+                for yy in range(ty * 8, ty * 8 + 8):
+                    cpu_counts[yy * 128 + tx * 8: yy * 128 + tx * 8 + 8] += c
+            assert (gc + cpu_counts == config.spp).all(), frame
+            own.finished(g)
+            flipped.append(sum(1 for i in range(16) if g.owner(i) == G.GPU))
+    finally:
+        lane.close()
+    after = own.owners(16)
+    print(f"GPU-owned cells by frame: {flipped}; after: {int((after == G.GPU).sum())}")
+    assert flipped[0] == 8 and int((after == G.GPU).sum()) > 8

@@ -18,6 +18,9 @@ X2_LIGHT_RECT = 1
 X2_SIDE_REGULAR = 0
 X2_SIDE_REVERSE = 1
 X2_SIDE_TWO = 2
+X2_DEVICE_CPU = 0
+X2_DEVICE_GPU = 1
+X2_NO_CELL = 0xFFFFFFFF
 NODE_BYTES = 256
 TRIANGLE_BYTES = 48
 FILM_BYTES_PER_PATH = 72
@@ -141,6 +144,31 @@ class BatchStats(ctypes.Structure):
     ]
 
 
+class GridOptions(ctypes.Structure):
+    _fields_ = [("cell_size", u32), ("samples_per_chunk", u32), ("mode", i32), ("progressive", u32),
+                ("sample_first", u32), ("sample_count", u32)]
+
+
+CPU_RENDER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, u32, ctypes.POINTER(Chunk))
+
+
+class CpuLane(ctypes.Structure):
+    _fields_ = [("threads", u32), ("pad", u32), ("context", ctypes.c_void_p), ("render", CPU_RENDER)]
+
+
+class BrokerOptions(ctypes.Structure):
+    _fields_ = [("window_ms", f64), ("first_chunks", u32), ("gpu_workers", u32)]
+
+
+class GridFrame(ctypes.Structure):
+    _fields_ = [
+        ("gpu_chunks", u64), ("cpu_chunks", u64), ("gpu_samples", u64), ("cpu_samples", u64), ("gpu_stolen", u64),
+        ("cpu_stolen", u64), ("launches", u64), ("errors", u64), ("seconds", f64), ("gpu_rate", f64),
+        ("cpu_rate", f64), ("rays_trace", u64), ("rays_shadow", u64), ("nodes_visited", u64),
+        ("triangles_tested", u64),
+    ]
+
+
 # Sizes of the C structures (include/x2dev.h); checked against ctypes here and against the C compiler by a
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
@@ -149,6 +177,8 @@ EXPECTED_SIZES = {
     "x2_config": (Config, 292),
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 80),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),
+    "x2_grid_options": (GridOptions, 24), "x2_cpu_lane": (CpuLane, 24), "x2_broker_options": (BrokerOptions, 16),
+    "x2_grid_frame": (GridFrame, 120),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
     assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
@@ -207,6 +237,42 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.POINTER(u32))]),
     "x2batch_job_count": (u32, [ctypes.c_void_p]),
     "x2batch_destroy": (None, [ctypes.c_void_p]),
+    "x2_grid_create": (ctypes.c_int, [u32, u32, u32, ctypes.POINTER(GridOptions), ctypes.POINTER(ctypes.c_uint8), u32,
+                                      ctypes.c_int, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_void_p)]),
+    "x2_grid_destroy": (None, [ctypes.c_void_p]),
+    "x2_grid_cell_count": (u32, [ctypes.c_void_p]),
+    "x2_grid_total_chunks": (u64, [ctypes.c_void_p]),
+    "x2_grid_owner": (u32, [ctypes.c_void_p, u32]),
+    "x2_grid_remaining": (u32, [ctypes.c_void_p, u32]),
+    "x2_grid_claimable": (u32, [ctypes.c_void_p, u32, u32]),
+    "x2_grid_empty": (ctypes.c_int, [ctypes.c_void_p]),
+    "x2_grid_claim": (u32, [ctypes.c_void_p, u32, u32, ctypes.POINTER(Chunk), u32]),
+    "x2_grid_append": (ctypes.c_int, [ctypes.c_void_p, u32, ctypes.POINTER(Chunk), u32]),
+    "x2_grid_choose": (u32, [ctypes.c_void_p, u32, u32, u32]),
+    "x2_grid_seed": (u32, [ctypes.c_void_p, u32, u32, u32]),
+    "x2_grid_frontier": (u32, [ctypes.c_void_p, u32]),
+    "x2_grid_ranges": (u32, [ctypes.c_void_p, ctypes.POINTER(u32), u32]),
+    "x2_grid_set_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(f32), u32]),
+    "x2_grid_important": (ctypes.c_int, [ctypes.c_void_p, u32]),
+    "x2_grid_reserved": (ctypes.c_int, [ctypes.c_void_p, u32]),
+    "x2_grid_set_suitability": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(f32), u32]),
+    "x2_grid_set_rates": (None, [ctypes.c_void_p, f64, f64, u32, f64]),
+    "x2_grid_gpu_claimed": (None, [ctypes.c_void_p]),
+    "x2_grid_passage": (ctypes.c_int, [ctypes.c_void_p, u32, u32]),
+    "x2_grid_close": (None, [ctypes.c_void_p]),
+    "x2_grid_rendered": (None, [ctypes.c_void_p, u32, u32, u64]),
+    "x2_grid_rendered_by": (u64, [ctypes.c_void_p, u32, u32]),
+    "x2_grid_deferred_to_cpu": (None, [ctypes.c_void_p, u32]),
+    "x2_grid_cell_of": (u32, [ctypes.c_void_p, u32]),
+    "x2_grid_tile_pixels": (u32, [ctypes.c_void_p, u32]),
+    "x2_ownership_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "x2_ownership_destroy": (None, [ctypes.c_void_p]),
+    "x2_ownership_owners": (ctypes.c_int, [ctypes.c_void_p, u32, ctypes.POINTER(ctypes.c_uint8)]),
+    "x2_ownership_finished": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "x2dev_render_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), ctypes.c_void_p,
+                                         ctypes.POINTER(CpuLane), ctypes.POINTER(BrokerOptions),
+                                         ctypes.POINTER(GridFrame)]),
+    "x2dev_chunk_room": (u32, [ctypes.c_void_p]),
 }
 
 _lib = None

@@ -175,6 +175,36 @@ class GpuLane:
         sums, counts = self.film()
         return sums, counts, st
 
+    def chunk_room(self):
+        return self.lib.x2dev_chunk_room(self.h)
+
+    def render_grid(self, config, grid, cpu_render=None, cpu_threads=0, window_ms=0.0, first_chunks=0, gpu_workers=0):
+        """One frame from a work grid (x2amd.grid.WorkGrid) through the broker (x2dev_render_grid): the GPU fed by
+        launches of about window_ms and, with cpu_render (called as cpu_render(worker, chunk) on cpu_threads worker
+        threads, chunk an x2_chunk), a CPU lane beside it. Returns (sums, counts, GridFrame): the GPU's film."""
+        lane = None
+        keep = None
+        if cpu_render is not None and cpu_threads:
+            def call(_context, worker, chunk):
+                try:
+                    return int(cpu_render(worker, chunk.contents) or 0)
+                except Exception:
+                    return 1
+            keep = abi.CPU_RENDER(call)
+            lane = abi.CpuLane(cpu_threads, 0, None, keep)
+        opts = abi.BrokerOptions(window_ms, first_chunks, gpu_workers)
+        out = abi.GridFrame()
+        r = self.lib.x2dev_render_grid(self.h, ctypes.byref(config), grid.handle,
+                                       ctypes.byref(lane) if lane is not None else None, ctypes.byref(opts),
+                                       ctypes.byref(out))
+        self.config = config
+        del keep
+        if r < 0:
+            err = self.lib.x2dev_last_error()
+            raise RuntimeError(f"x2dev_render_grid failed ({r}): {err.decode() if err else ''}")
+        sums, counts = self.film()
+        return sums, counts, out
+
 
 def render_frame(scene, config, samples_per_chunk=4, device=0):
     lane = GpuLane(device)
