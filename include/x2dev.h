@@ -24,6 +24,7 @@ extern "C" {
 #define X2_ERR_STATE (-3)      /* a call out of order (launch before begin, download with launches in flight ...) */
 #define X2_ERR_BACKPRESSURE (-4) /* launch() while not ready(), or a launch larger than the mailbox ring allows */
 #define X2_ERR_MEMORY (-5)     /* device or pinned allocation failed */
+#define X2_ERR_FILE (-6)       /* a file could not be written or read, or is not the save asked for */
 
 /* The unit of work: one 8x8 tile of the image and a range of sample indices (async_integrator.h:3057 Chunk). */
 typedef struct x2_chunk {
@@ -473,6 +474,58 @@ typedef struct x2_grid_frame {
  * are consumed: a grid renders one frame. */
 int x2dev_render_grid(x2dev* d, const x2_config* config, x2_grid* grid, const x2_cpu_lane* cpu,
                       const x2_broker_options* options, x2_grid_frame* out);
+/* ---- The on-disk formats either side of a frame. Host code (x2dev_render_staged excepted). */
+/* A film as a checkpoint render saves it (StagedFilm, xpu2/interactive/xpu2_session.cc:1338-1420): sums[pixel*4 + c]
+ * FP64, counts[pixel], odd (the odd-index sums, or null), `done` of `spp` samples a pixel rendered. */
+typedef struct x2_film_file {
+    uint32_t width, height, spp, done;
+    const double* sums;
+    const uint32_t* counts;
+    const double* odd;
+} x2_film_file;
+/* StagedFilm::save (xpu2_session.cc:1416-1452): the "X2RS" version 1 file (magic, version, width, height, spp, done,
+ * AOV channels 0, closest filters 0; then the arrays sums, counts, odd, AOVs, depths, each its u64 length and its
+ * elements; then 0 Cryptomatte categories), written to path.part and then renamed over path. */
+int x2_film_save(const char* path, const x2_film_file* film);
+/* StagedFilm::load (xpu2_session.cc:1454-1513): the save at path into caller-owned width*height films (odd may be
+ * null; *has_odd says whether the save held one); X2_ERR_FILE, the film untouched, when there is no save, it is not
+ * one, it is of another render (size, spp, outputs) or it is cut short. */
+int x2_film_load(const char* path, uint32_t width, uint32_t height, uint32_t spp, double* sums, uint32_t* counts,
+                 double* odd, uint32_t* done, uint32_t* has_odd);
+
+/* The beauty output as a scanline OpenEXR file (RenderOutputWriter through OIIO, aov_write.h:176-255): channels A, B,
+ * G, R (each pixel's sums / counts; 0 where it has no sample) as half or float and, with weight, "weight" (the count,
+ * float); the display window the whole image, the data window the region (the film's pixels [x0, x1) x [y0, y1),
+ * the whole image when empty); the film's rows run from the bottom, the file's from the top. Compression: */
+#define X2_EXR_NONE 0u
+#define X2_EXR_ZIPS 2u /* zlib, one scanline a block */
+#define X2_EXR_ZIP 3u  /* zlib, 16 scanlines a block (OIIO's default for EXR) */
+typedef struct x2_exr_image {
+    uint32_t width, height;
+    const double* sums;
+    const uint32_t* counts;
+    int32_t region[4];
+    uint32_t half, compression, weight, pad;
+} x2_exr_image;
+int x2_write_exr(const char* path, const x2_exr_image* image);
+
+/* A checkpoint render (stagedFrame, xpu2_session.cc:1540-1640): the uniform frame in stages of stage_samples sample
+ * indices a pixel (0: a sixteenth of spp, at least 1), every pixel alike; after each stage the film is saved to path
+ * (null: not saved); with resume, a save at path of the same render is taken up (x2dev_seed_film) and the frame goes
+ * on from its samples (none, or another render's: the frame begins anew). max_stages (0: all) stops after that many
+ * stages, as a render drained part way. The device's film (x2dev_film) holds the frame so far at return. */
+typedef struct x2_staged {
+    const char* path;
+    uint32_t stage_samples, resume, max_stages, pad;
+} x2_staged;
+typedef struct x2_staged_frame {
+    uint32_t stages, saves, launches, errors;
+    uint32_t resumed, resumed_from, done, pad;
+    double seconds, device_seconds;
+} x2_staged_frame;
+int x2dev_render_staged(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, const x2_staged* stages,
+                        x2_staged_frame* out);
+
 /* GpuLane::chunkRoom (async_gpu.h:1725-1729): the chunks the next launch may take (the launch cap, and the mailbox
  * ring's free room). */
 uint32_t x2dev_chunk_room(const x2dev* d);

@@ -18,6 +18,10 @@ X2_LIGHT_RECT = 1
 X2_SIDE_REGULAR = 0
 X2_SIDE_REVERSE = 1
 X2_SIDE_TWO = 2
+X2_ERR_FILE = -6
+X2_EXR_NONE = 0
+X2_EXR_ZIPS = 2
+X2_EXR_ZIP = 3
 X2_DEVICE_CPU = 0
 X2_DEVICE_GPU = 1
 X2_NO_CELL = 0xFFFFFFFF
@@ -169,6 +173,25 @@ class GridFrame(ctypes.Structure):
     ]
 
 
+class FilmFile(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("spp", u32), ("done", u32), ("sums", ctypes.POINTER(f64)),
+                ("counts", ctypes.POINTER(u32)), ("odd", ctypes.POINTER(f64))]
+
+
+class ExrImage(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("sums", ctypes.POINTER(f64)), ("counts", ctypes.POINTER(u32)),
+                ("region", i32 * 4), ("half", u32), ("compression", u32), ("weight", u32), ("pad", u32)]
+
+
+class Staged(ctypes.Structure):
+    _fields_ = [("path", ctypes.c_char_p), ("stage_samples", u32), ("resume", u32), ("max_stages", u32), ("pad", u32)]
+
+
+class StagedFrame(ctypes.Structure):
+    _fields_ = [("stages", u32), ("saves", u32), ("launches", u32), ("errors", u32), ("resumed", u32),
+                ("resumed_from", u32), ("done", u32), ("pad", u32), ("seconds", f64), ("device_seconds", f64)]
+
+
 # Sizes of the C structures (include/x2dev.h); checked against ctypes here and against the C compiler by a
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
@@ -178,7 +201,8 @@ EXPECTED_SIZES = {
     "x2_completion": (Completion, 72), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 80),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),
     "x2_grid_options": (GridOptions, 24), "x2_cpu_lane": (CpuLane, 24), "x2_broker_options": (BrokerOptions, 16),
-    "x2_grid_frame": (GridFrame, 120),
+    "x2_grid_frame": (GridFrame, 120), "x2_film_file": (FilmFile, 40), "x2_exr_image": (ExrImage, 56),
+    "x2_staged": (Staged, 24), "x2_staged_frame": (StagedFrame, 48),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
     assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
@@ -273,6 +297,12 @@ SIGNATURES = {
                                          ctypes.POINTER(CpuLane), ctypes.POINTER(BrokerOptions),
                                          ctypes.POINTER(GridFrame)]),
     "x2dev_chunk_room": (u32, [ctypes.c_void_p]),
+    "x2_film_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FilmFile)]),
+    "x2_film_load": (ctypes.c_int, [ctypes.c_char_p, u32, u32, u32, ctypes.POINTER(f64), ctypes.POINTER(u32),
+                                    ctypes.POINTER(f64), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    "x2_write_exr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ExrImage)]),
+    "x2dev_render_staged": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.POINTER(Staged),
+                                           ctypes.POINTER(StagedFrame)]),
 }
 
 _lib = None

@@ -206,6 +206,21 @@ class GpuLane:
         return sums, counts, out
 
 
+    def render_staged(self, config, path=None, stage_samples=0, resume=False, max_stages=0, samples_per_chunk=4):
+        """A checkpoint render (x2dev_render_staged): the frame in stages, its film saved to path after each, taken
+        up from a save at path with resume. Returns (sums, counts, StagedFrame): the film so far."""
+        st = abi.Staged(path.encode() if path else None, stage_samples, 1 if resume else 0, max_stages, 0)
+        out = abi.StagedFrame()
+        r = self.lib.x2dev_render_staged(self.h, ctypes.byref(config), samples_per_chunk, ctypes.byref(st),
+                                         ctypes.byref(out))
+        self.config = config
+        if r < 0:
+            err = self.lib.x2dev_last_error()
+            raise RuntimeError(f"x2dev_render_staged failed ({r}): {err.decode() if err else ''}")
+        sums, counts = self.film()
+        return sums, counts, out
+
+
 def render_frame(scene, config, samples_per_chunk=4, device=0):
     lane = GpuLane(device)
     try:
