@@ -334,6 +334,34 @@ int x2_cellular_step(uint32_t width, uint32_t height, float target_error, const 
  * counts[i]. Needs no device. */
 int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors);
 
+/* ---- On-demand texture tiles and partial shading (GpuLane's fine-detail cache, async_gpu.h:2911-3200, and its
+ * parked and provisional samples, async_gpu.cu:274-295,350-357; texture_core.h's tiles). After prepare, every texture's
+ * level 0 is kept by the host as 32x32 tiles of half-float rgb (a float texture's levels are held in half floats,
+ * texture_core.h:30-31) and the device holds pool_tiles of them at a time (a full pool gives up its oldest load), plus
+ * each texture's coarse level (level 0's halves averaged over boxes, at most coarse_side texels a side; 0: 64),
+ * always resident. A sample whose texture lookup finds a tile missing asks for every tile of its footprint and is
+ * parked: it ends, and runs again in a later launch (ahead of its chunks, or in a launch of parked samples alone,
+ * x2dev_launch with no chunks) once x2dev_launch has sent up the tiles asked for. With partial it finishes from the
+ * coarse level instead, into the provisional film, and is parked to run again in full. The frame calls
+ * (x2dev_render_frame, x2dev_render_adaptive with host trees, x2dev_render_grid, x2dev_render_staged) run parked
+ * samples until none is left, so their films are the resident frame's. The surface estimator's forward frames in the
+ * one-kernel form only: the wavefront form, the adjoint, the prototype estimator, the device pass chain and media
+ * refuse paged textures; x2dev_set_texture refuses a paged one. No counterpart for the pool size in the reference,
+ * which sizes its cache from the card's memory. */
+typedef struct x2_paging {
+    uint32_t pool_tiles, coarse_side, partial, pad;
+} x2_paging;
+typedef struct x2_paging_stats {
+    uint64_t parked_outstanding; /* parked, not yet run again (GpuLane::parkedOutstanding, async_gpu.h:1427) */
+    uint64_t parked_total, provisional, misses, uploads, evictions, overflow;
+    uint32_t tiles, pool_tiles, resident, pad;
+} x2_paging_stats;
+int x2dev_page_textures(x2dev* d, const x2_paging* paging);
+int x2dev_paging_stats(const x2dev* d, x2_paging_stats* out);
+uint64_t x2dev_parked_outstanding(const x2dev* d);
+/* The provisional film (partial shading; downloaded with the film): each provisional sample's rgb + alpha and count. */
+int x2dev_provisional_film(const x2dev* d, const double** sums, const uint32_t** counts);
+
 /* ---- Batched jobs. No counterpart in the reference, which renders one frame at a time and decided against CUDA
  * graphs (ASYNC.md:176-177): many independent render jobs, each its own scene, settings and film, run in one
  * persistent kernel that claims work from a device queue (the jobs' items end to end, shortest job first by

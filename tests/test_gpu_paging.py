@@ -1,0 +1,151 @@
+"""On-demand texture tiles on the device (x2dev_page_textures; the reference's fine-detail cache and parked samples,
+async_gpu.h:2911-3200, async_gpu.cu:350-357): a frame whose texture lives in a small pool of 32x32 half-float tiles —
+samples parked on missing tiles and run again once their tiles are sent up, the pool evicting as it fills — equals
+the frame with the texture resident (the same half-float texels: counts exact, sums to the FP64 atomics' order), and
+so with partial shading (provisional samples from the coarse level, then run again in full), through the frame loop,
+the work grid's broker and the staged checkpoint render. Calls that read resident textures refuse paged ones."""
+import numpy as np
+import pytest
+
+from x2amd import grid as G
+from x2amd import scenes
+from x2amd.lane import GpuLane
+
+pytestmark = pytest.mark.gpu
+
+
+def close(a, b):
+    return np.abs(a - b).max() <= 1e-12 * max(1.0, float(np.abs(b).max()))
+
+
+def textured(size=128, spp=8, tex=256):
+    scene, config, params = scenes.cornell_textured(width=size, height=size, spp=spp, texture_size=tex)
+    half = params["texture"].astype(np.float16).astype(np.float32)  # the texels the tiles hold
+    resident, _, _ = scenes.cornell_textured(width=size, height=size, spp=spp, texture=half)
+    return scene, resident, config
+
+
+@pytest.fixture(scope="module")
+def frames():
+    scene, resident, config = textured()
+    lane = GpuLane(0)
+    try:
+        lane.prepare(resident)
+        ws, wc, _ = lane.render_frame(config)
+    finally:
+        lane.close()
+    return scene, config, ws, wc
+
+
+@pytest.mark.parametrize("pool,partial", [(64, False), (64, True), (80, True)])
+def test_paged_frame_equals_resident(frames, pool, partial):
+    scene, config, ws, wc = frames
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        lane.page_textures(pool, partial=partial)
+        ps, pc, st = lane.render_frame(config)
+        g = lane.paging_stats()
+        prov = lane.provisional_film()
+    finally:
+        lane.close()
+    print(f"pool {pool} partial {partial}: parked {g.parked_total}, provisional {g.provisional}, misses {g.misses}, "
+          f"uploads {g.uploads}, evictions {g.evictions}, resident {g.resident}/{g.tiles}, launches {st.launches}")
+    assert st.errors == 0 and st.audit_failures == 0 and g.overflow == 0 and g.parked_outstanding == 0
+    assert g.tiles == 64 and g.parked_total > 0 and g.uploads >= g.resident
+    assert (pc == wc).all() and close(ps, ws)
+    assert g.evictions == 0  # the pool holds the frame's working set (the whole texture: level 0 is read everywhere)
+    if partial:
+        assert g.provisional > 0 and int(prov[1].sum()) == g.provisional
+        assert np.isfinite(prov[0]).all()
+    else:
+        assert g.provisional == 0 and int(prov[1].sum()) == 0
+
+
+def test_frames_with_other_tiles_evict(tmp_path):
+    """Camera hits only (depth 0): the left half of the image reads some of the texture's tiles, the right half others.
+    In a pool of 48 of the 64 tiles the second frame evicts the first's; each equals its resident frame."""
+    scene, resident, config = textured(128, 4, 256)
+    config.max_depth = 0
+    halves = [[0, 0, 64, 128], [64, 0, 128, 128]]
+    want = []
+    lane = GpuLane(0)
+    try:
+        lane.prepare(resident)
+        for vp in halves:
+            config.viewport[:] = vp
+            want.append(lane.render_frame(config)[:2])
+    finally:
+        lane.close()
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        lane.page_textures(48)
+        got = []
+        for vp in halves:
+            config.viewport[:] = vp
+            got.append(lane.render_frame(config)[:2])
+        g = lane.paging_stats()
+    finally:
+        lane.close()
+    print(f"two halves: misses {g.misses}, uploads {g.uploads}, evictions {g.evictions}, resident {g.resident}")
+    assert g.evictions > 0 and g.overflow == 0 and g.parked_outstanding == 0
+    for (gs, gc), (ws, wc) in zip(got, want):
+        assert (gc == wc).all() and close(gs, ws)
+
+
+def test_paged_frames_through_the_grid_and_stages(frames, tmp_path):
+    scene, config, ws, wc = frames
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        lane.page_textures(64, partial=True)
+        g = G.WorkGrid(config.width, config.height, config.spp, G.options(64, 4, 1))
+        gs, gc, fr = lane.render_grid(config, g, window_ms=5.0)
+        ss, sc, sf = lane.render_staged(config, str(tmp_path / "paged.xpu2resume"), stage_samples=3)
+        # a second frame on the same lane: its tiles are resident from the first
+        rs, rc, _ = lane.render_frame(config)
+        st = lane.paging_stats()
+    finally:
+        lane.close()
+    assert fr.errors == 0 and (gc == wc).all() and close(gs, ws)
+    assert sf.errors == 0 and (sc == wc).all() and close(ss, ws)
+    assert (rc == wc).all() and close(rs, ws) and st.parked_outstanding == 0
+
+
+def test_a_pool_too_small_for_a_path_fails_the_frame(frames):
+    """A pool far smaller than what the frame's paths read: samples park again and again, and the frame fails with the
+    reason rather than running on (the reference's frame fails so too, async_render.h:816-818)."""
+    scene, config, _, _ = frames
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        lane.page_textures(8)
+        with pytest.raises(RuntimeError, match="cannot hold"):
+            lane.render_frame(config)
+    finally:
+        lane.close()
+
+
+def test_calls_that_read_resident_textures_refuse_paged_ones():
+    scene, _, config = textured(32, 2, 64)
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        lane.page_textures(4)
+        with pytest.raises(RuntimeError, match="paged"):
+            lane.set_texture(0, np.zeros((64, 64, 3), np.float32))
+        with pytest.raises(RuntimeError, match="paged"):
+            lane.page_textures(4)
+        config.direct_only, config.surface_limit = 1, 1
+        with pytest.raises(RuntimeError, match="paged textures"):
+            lane.render_frame(config)
+    finally:
+        lane.close()
+    wave = GpuLane(0, wave_slots=4096)
+    try:
+        wave.prepare(scene)
+        with pytest.raises(RuntimeError, match="wavefront"):
+            wave.page_textures(4)
+    finally:
+        wave.close()

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/x2dev.h"
+#include "x2_half.h"
 
 namespace x2 {
 int reportFailure(int code, const std::string& what);  // sets x2dev_last_error's text (x2_lane.hip)
@@ -58,30 +59,7 @@ struct PartFile {
     }
 };
 
-// IEEE binary16 from binary32, rounding to nearest even (as Imath's half and OIIO convert).
-uint16_t toHalf(float value) {
-    uint32_t x;
-    std::memcpy(&x, &value, 4);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t e = (x >> 23) & 0xffu;
-    uint32_t m = x & 0x7fffffu;
-    if (e == 0xff) return uint16_t(sign | 0x7c00u | (m >> 13) | (m && !(m >> 13)));  // inf, or a NaN kept a NaN
-    const int exp = int(e) - 127 + 15;
-    if (exp >= 31) return uint16_t(sign | 0x7c00u);  // overflow
-    if (exp <= 0) {                                  // a subnormal half, or zero
-        if (exp < -10) return uint16_t(sign);
-        m |= 0x800000u;
-        const int shift = 14 - exp;  // 14..24
-        uint32_t h = m >> shift;
-        const uint32_t rest = m & ((1u << shift) - 1), half = 1u << (shift - 1);
-        if (rest > half || (rest == half && (h & 1u))) ++h;
-        return uint16_t(sign | h);
-    }
-    uint32_t h = (uint32_t(exp) << 10) | (m >> 13);
-    const uint32_t rest = m & 0x1fffu;
-    if (rest > 0x1000u || (rest == 0x1000u && (h & 1u))) ++h;  // may carry into the exponent, to infinity at most
-    return uint16_t(sign | h);
-}
+using x2::toHalf;
 
 void put32(std::vector<uint8_t>& b, uint32_t v) {
     for (int i = 0; i < 4; ++i) b.push_back(uint8_t(v >> (8 * i)));
@@ -314,7 +292,12 @@ int x2dev_render_staged(x2dev* d, const x2_config* config, uint32_t samples_per_
     }
     // Cell by cell (64-pixel cells), a stage's sample ranges in turn, as the frame loop queues them.
     std::vector<x2_chunk> chunks;
-    const uint32_t cap = std::max(1u, std::min(x2dev_max_chunks_per_launch(d), x2dev_chunk_room(d)));
+    x2_completion c;
+    auto collect = [&]() {
+        const int q = x2dev_poll(d, &c);
+        if (q == 1) out->errors += c.errors, out->device_seconds += c.device_seconds;
+        return q;
+    };
     while (done < spp) {
         if (s->max_stages && out->stages >= s->max_stages) break;
         const uint32_t first = done, end = std::min(spp, done + step);
@@ -325,20 +308,26 @@ int x2dev_render_staged(x2dev* d, const x2_config* config, uint32_t samples_per_
                     for (uint32_t ty = cy * 8; ty < std::min(tilesY, cy * 8 + 8); ++ty)
                         for (uint32_t tx = cx * 8; tx < std::min(tilesX, cx * 8 + 8); ++tx)
                             chunks.push_back({ty * tilesX + tx, f, std::min(spc, end - f), cy * ((tilesX + 7) / 8) + cx});
-        x2_completion c;
         for (size_t at = 0; at < chunks.size();) {
-            while (!x2dev_ready(d)) {
-                if ((r = x2dev_poll(d, &c)) < 0) return r;
-                if (r == 1) out->errors += c.errors, out->device_seconds += c.device_seconds;
+            uint32_t room = 0;
+            while (!x2dev_ready(d) || !(room = x2dev_chunk_room(d))) {  // room: the ring, and parked samples' ring
+                if ((r = collect()) < 0) return r;
+                if (!x2dev_in_flight(d) && !room && x2dev_ready(d) && !(room = x2dev_chunk_room(d)))
+                    return reportFailure(X2_ERR_STATE, "no room for a launch with nothing in flight");
             }
-            const uint32_t n = uint32_t(std::min<size_t>(cap, chunks.size() - at));
+            const uint32_t n = uint32_t(std::min<size_t>(room, chunks.size() - at));
             if ((r = x2dev_launch(d, chunks.data() + at, n, 0)) < 0) return r;
             ++out->launches;
             at += n;
         }
-        while (x2dev_in_flight(d)) {
-            if ((r = x2dev_poll(d, &c)) < 0) return r;
-            if (r == 1) out->errors += c.errors, out->device_seconds += c.device_seconds;
+        // Paged textures: the stage's parked samples run again until none is left.
+        for (uint32_t idle = 0; x2dev_in_flight(d) || x2dev_parked_outstanding(d);) {
+            if ((r = collect()) < 0) return r;
+            if (x2dev_ready(d) && x2dev_parked_outstanding(d)) {
+                if (++idle > 100000) return reportFailure(X2_ERR_STATE, "parked samples do not finish");
+                if ((r = x2dev_launch(d, nullptr, 0, 0)) < 0) return r;
+                ++out->launches;
+            }
         }
         done = end;
         ++out->stages;

@@ -464,7 +464,19 @@ struct Broker {
             bool launched = false;
             while (useGpu && x2dev_ready(d)) {
                 const uint32_t want = launchSize(cpuThreads);
-                if (!want) break;
+                if (!want) {
+                    // Paged textures: parked samples run again in a launch of their own once no chunk is left.
+                    if (grid.empty() && x2dev_parked_outstanding(d)) {
+                        const int r = x2dev_launch(d, nullptr, 0, 0);
+                        if (r < 0) {
+                            status = r;
+                            break;
+                        }
+                        ++out.launches;
+                        launched = true;
+                    }
+                    break;
+                }
                 claim.resize(want);
                 uint32_t got = 0;
                 // The GPU's cellular workers take the launch's shares, each from its cell, moving by the rules. A
@@ -503,7 +515,7 @@ struct Broker {
                 launched = true;
             }
             if (status < 0) break;
-            if (grid.empty() && !x2dev_in_flight(d)) break;
+            if (grid.empty() && !x2dev_in_flight(d) && !x2dev_parked_outstanding(d)) break;
             if (!launched) std::this_thread::sleep_for(std::chrono::microseconds(x2dev_in_flight(d) ? 20 : 100));
         }
         if (status < 0) {

@@ -10,9 +10,11 @@
 // Built with -ffp-contract=off and correctly rounded division and square root, so every float expression of the
 // estimator rounds as written, as the host oracle's does. Only the box tests use fused multiply-adds: they are
 // conservative (Ize 2013) and decide what is tested, never the result.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cfloat>
 
 #include "x2_adaptive_core.h"
@@ -695,6 +697,7 @@ __device__ __forceinline__ V3 texel(const TexDesc& t, uint32_t x, uint32_t y) {
     return {p[0], p[1], p[2]};
 }
 __device__ bool textureTap(const TexDesc& t, uint32_t id, float u, float v, TexTap& tap) {
+    if (t.paged) return false;  // its level 0 is in tiles: pagedTap reads it
     const float su = 1.0f * u + 0.0f, tv = 1 - (1.0f * v + 0.0f);
     float fs = su * float(t.width) - .5f, ft = tv * float(t.height) - .5f;
     if (!finite1(fs) || !finite1(ft) || fabsf(fs) > 1.e8f || fabsf(ft) > 1.e8f) return false;
@@ -711,6 +714,71 @@ __device__ bool textureTap(const TexDesc& t, uint32_t id, float u, float v, TexT
     tap.fs = fs;
     tap.ft = ft;
     return finite3(tap.color);
+}
+
+// On-demand tiles (texture_core.h:37-60; the GPU's fine-detail cache, async_gpu.h:2911-3200). A level-0 texel through
+// the page table: half-float rgb in its tile's pool slot. False: the tile is not resident; it is asked for once (its
+// flag set, its id on the launch's miss list; with the list full the flag is cleared, for a later lookup to ask).
+__device__ __forceinline__ bool pagedTexel(const RenderParams& P, const TexDesc& t, uint32_t x, uint32_t y, V3& out) {
+    const uint32_t tile = t.tileBase + (y >> 5) * t.tilesX + (x >> 5);
+    const uint32_t slot = P.tileSlot[tile];
+    if (slot == NoTileSlot) {
+        if (atomicExch(P.tileAsked + tile, 1u) == 0u) {
+            const uint32_t at = atomicAdd(&P.paging->missCount, 1u);
+            if (at < P.missCap) P.missList[at] = tile;
+            else atomicExch(P.tileAsked + tile, 0u);
+        }
+        return false;
+    }
+    const uint16_t* h = P.tilePool + (size_t(slot) * TexTileTexels + (y & (TexTile - 1)) * TexTile + (x & (TexTile - 1))) * 3;
+    out = V3{__half2float(__ushort_as_half(h[0])), __half2float(__ushort_as_half(h[1])),
+             __half2float(__ushort_as_half(h[2]))};
+    return true;
+}
+enum : int { TapOk = 0, TapInvalid = 1, TapMiss = 2, TapCoarse = 3 };
+// textureTap over a paged level 0: the same footprint and arithmetic, every tap's tile asked for at once when any is
+// missing (one wait for the sample, not one a tile, texture_core.h:56-58). TapMiss: a tile is missing; with partial
+// shading TapCoarse instead, the footprint read from the coarse level (the sample is then provisional).
+__device__ int pagedTap(const RenderParams& P, const TexDesc& t, uint32_t id, float u, float v, TexTap& tap) {
+    const float su = 1.0f * u + 0.0f, tv = 1 - (1.0f * v + 0.0f);
+    float fs = su * float(t.width) - .5f, ft = tv * float(t.height) - .5f;
+    if (!finite1(fs) || !finite1(ft) || fabsf(fs) > 1.e8f || fabsf(ft) > 1.e8f) return TapInvalid;
+    const int x = int(floorf(fs)), y = int(floorf(ft));
+    fs -= float(x);
+    ft -= float(y);
+    const uint32_t x0 = wrapTexel(x, t.width), x1 = wrapTexel(x + 1, t.width);
+    const uint32_t y0 = wrapTexel(y, t.height), y1 = wrapTexel(y + 1, t.height);
+    V3 a, b, c, d;
+    bool ok = pagedTexel(P, t, x0, y0, a);
+    ok &= pagedTexel(P, t, x1, y0, b);
+    ok &= pagedTexel(P, t, x0, y1, c);
+    ok &= pagedTexel(P, t, x1, y1, d);
+    tap.tex = id;
+    if (ok) {
+        tap.color = ((a * (1 - fs)) + (b * fs)) * (1 - ft) + ((c * (1 - fs)) + (d * fs)) * ft;
+        tap.x0 = x0;
+        tap.y0 = y0;
+        tap.fs = fs;
+        tap.ft = ft;
+        return finite3(tap.color) ? TapOk : TapInvalid;
+    }
+    if (!P.partial) return TapMiss;
+    float cs = su * float(t.coarseW) - .5f, ct = tv * float(t.coarseH) - .5f;
+    const int cx = int(floorf(cs)), cy = int(floorf(ct));
+    cs -= float(cx);
+    ct -= float(cy);
+    const uint32_t u0 = wrapTexel(cx, t.coarseW), u1 = wrapTexel(cx + 1, t.coarseW);
+    const uint32_t v0 = wrapTexel(cy, t.coarseH), v1 = wrapTexel(cy + 1, t.coarseH);
+    auto at = [&](uint32_t i, uint32_t j) {
+        const float* q = t.coarse + (size_t(j) * t.coarseW + i) * 3;
+        return V3{q[0], q[1], q[2]};
+    };
+    tap.color = ((at(u0, v0) * (1 - cs)) + (at(u1, v0) * cs)) * (1 - ct) + ((at(u0, v1) * (1 - cs)) + (at(u1, v1) * cs)) * ct;
+    tap.x0 = u0;
+    tap.y0 = v0;
+    tap.fs = cs;
+    tap.ft = ct;
+    return finite3(tap.color) ? TapCoarse : TapInvalid;
 }
 
 // d ln f / d r of the GGX lobe (shade_lobes_eval.h:93-121) at (wo, wi) for MoonRay's input roughness r (the lobe's
@@ -1335,9 +1403,13 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
 // A sample's state between its vertices: the pending rays (a stack: a vertex's children are pushed after it, so the
 // path tree is walked depth first), the radiance and alpha so far and the camera ray's clip distances; with Adj the
 // adjoint's state too: the pixel's adjoint per sample, the sinks of the pending rays by stack position, and the open
-// subtrees (the ancestors of the vertex being shaded).
-template <bool Adj>
-struct PathState {
+// subtrees (the ancestors of the vertex being shaded). With Pg (the paged render) it keeps the sample's paging flags.
+struct SampleFlagsNone {};
+struct SampleFlagsKept {
+    uint32_t flags;  // SampleParked, SampleProvisional (on-demand texture tiles)
+};
+template <bool Adj, bool Pg = false>
+struct PathState : std::conditional_t<Pg, SampleFlagsKept, SampleFlagsNone> {
     Pending stack[MaxPending];
     uint32_t top;
     V3 radiance;
@@ -1347,6 +1419,23 @@ struct PathState {
     Record recs[Adj ? MaxRecords : 1];
     uint32_t nrec;
 };
+
+// A sample that met a missing texture tile: parked (it ends and runs again once the tile is resident), or, with
+// partial shading, provisional (it finished from the coarse level, into the provisional film, and runs again).
+enum : uint32_t { SampleParked = 1u, SampleProvisional = 2u };
+template <class St>
+struct HasSampleFlags {
+    static constexpr bool value = false;
+};
+template <bool A>
+struct HasSampleFlags<PathState<A, true>> {
+    static constexpr bool value = true;
+};
+template <class St>
+__device__ __forceinline__ uint32_t sampleFlags(const St& S) {
+    if constexpr (HasSampleFlags<St>::value) return S.flags;
+    else return 0u;
+}
 
 // Starts sample (pixel, index): its camera ray becomes the one pending ray. False: an invalid value (Error).
 template <bool Adj, bool Vol, class St>
@@ -1379,6 +1468,7 @@ __device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixe
     S.top = 1;
     S.radiance = V3{0, 0, 0};
     S.alpha = 0.0f;
+    if constexpr (HasSampleFlags<St>::value) S.flags = 0u;
     if constexpr (Adj) {
         const float inv = 1.0f / float(c.spp);
         S.delta = V3{P.dImage[size_t(pixel) * 3] * inv, P.dImage[size_t(pixel) * 3 + 1] * inv,
@@ -1473,7 +1563,23 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
             const float w = 1.0f - hit.u - hit.v;
             const float ss = w * uv[0] + hit.u * uv[2] + hit.v * uv[4], tt = w * uv[1] + hit.u * uv[3] + hit.v * uv[5];
             TexTap tap;
-            if (!textureTap(P.textures[m.albedo_map - 1], m.albedo_map - 1, ss, tt, tap)) return false;
+            const TexDesc& td = P.textures[m.albedo_map - 1];
+            if constexpr (HasSampleFlags<St>::value && !Adj) {  // the paged render (renderPagedKernel)
+                if (td.paged) {
+                    const int r = pagedTap(P, td, m.albedo_map - 1, ss, tt, tap);
+                    if (r == TapInvalid) return false;
+                    if (r == TapMiss) {  // the sample waits for its tile: it ends here, parked, and runs again
+                        S.flags |= SampleParked;
+                        S.top = 0;
+                        return true;
+                    }
+                    if (r == TapCoarse) S.flags |= SampleProvisional;
+                } else if (!textureTap(td, m.albedo_map - 1, ss, tt, tap)) {
+                    return false;
+                }
+            } else if (!textureTap(td, m.albedo_map - 1, ss, tt, tap)) {  // false for a paged texture
+                return false;
+            }
             albedo = albedo * tap.color;
             if constexpr (Adj) {
                 texSink.kind = SinkTexture;
@@ -1743,8 +1849,8 @@ __device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixe
 }
 
 // Closes a finished sample's open subtrees; with Adj each carries its share of the radiance to its sink.
-template <bool Adj>
-__device__ __forceinline__ void finishSample(const RenderParams& P, PathState<Adj>& S, float* gr) {
+template <bool Adj, class St>
+__device__ __forceinline__ void finishSample(const RenderParams& P, St& S, float* gr) {
     if constexpr (Adj) {
         while (S.nrec) {
             --S.nrec;
@@ -1776,7 +1882,7 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
 // (Error; it adds nothing).
 template <bool Adj, bool Vol, class Feed>
 __device__ void runSamples(const RenderParams& P, Feed& feed, Counters& k, TravStack& st, float* gr) {
-    PathState<Adj> S;
+    PathState<Adj, Feed::Paged> S;
     S.top = 0;
     uint32_t pixel = 0, index = 0;
     bool live = false;
@@ -1784,7 +1890,7 @@ __device__ void runSamples(const RenderParams& P, Feed& feed, Counters& k, TravS
         const bool idle = S.top == 0;
         if (idle && live) {
             finishSample<Adj>(P, S, gr);
-            feed.done(pixel, index, S.radiance, S.alpha);
+            feed.done(pixel, index, S.radiance, S.alpha, sampleFlags(S));
             live = false;
         }
         // A finished lane takes its next sample once Feed::Regen lanes of the wave are finished (or all are), so that
@@ -1999,6 +2105,7 @@ __device__ __forceinline__ unsigned long long chainItems(const RenderParams& P) 
     if (!P.chain || P.chain->stop) return 0;
     return (unsigned long long)P.chain->activeTiles * P.chainSize * TilePixels;
 }
+template <bool Pg = false>
 __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel, uint32_t& index,
                                           unsigned long long chainEnd = 0) {
     for (;;) {
@@ -2014,20 +2121,56 @@ __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel
             index = P.chainBase[pixel] + rest / TilePixels;
             return true;
         }
-        const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
+        unsigned long long k = atomicAdd(&P.state->claimed, 1ull);
+        if constexpr (Pg) {
+            if (k < P.parkedCount) {  // the parked samples this launch runs again, ahead of its chunks
+                const x2_parked r = P.parked[(P.parkedBegin + k) % P.parkCap];
+                pixel = r.pixel;
+                index = r.index;
+                return true;
+            }
+            k -= P.parkedCount;
+        }
+        const unsigned long long item = P.itemBegin + k;
         if (item >= P.itemEnd) return false;
         if (decodeItem(P, item, pixel, index)) return true;
     }
 }
-template <bool Adj>
+// A parked sample onto the ring (async_gpu.cu:350-357): the host sizes launches so the ring never overwrites a record
+// not yet run again; past that bound a sample is counted as lost (a failure of the frame), never written.
+__device__ __forceinline__ void parkSample(const RenderParams& P, uint32_t pixel, uint32_t index) {
+    const unsigned long long at = atomicAdd(&P.paging->parkTail, 1ull);
+    if (at >= P.parkedBegin + P.parkCap) {
+        atomicAdd(&P.paging->parkOverflow, 1u);
+        return;
+    }
+    P.parked[at % P.parkCap] = x2_parked{pixel, index, 0u, 0u};
+}
+template <bool Adj, bool Pg = false>
 struct LaneFeed {
     static constexpr uint32_t Regen = X2_REGEN_LANES;  // finished lanes of a wave that take new samples together
+    static constexpr bool Paged = Pg;                  // texture tiles on demand: parked and provisional samples
     const RenderParams& P;
     uint32_t& errors;
     unsigned long long chainEnd = chainItems(P);  // read once: the compaction wrote it before this render
-    __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem(P, pixel, index, chainEnd); }
-    __device__ void done(uint32_t pixel, uint32_t index, V3 r, float a) {
-        if constexpr (!Adj) filmAdd(P, pixel, index, r.x, r.y, r.z, a);
+    __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem<Pg>(P, pixel, index, chainEnd); }
+    __device__ void done(uint32_t pixel, uint32_t index, V3 r, float a, uint32_t flags) {
+        if constexpr (!Adj) {
+            if (Pg && flags) {  // a parked or provisional sample (on-demand texture tiles) runs again
+                if (flags & SampleProvisional) {
+                    double* f = P.provSums + size_t(pixel) * 4;
+                    atomicAdd(f + 0, double(r.x));
+                    atomicAdd(f + 1, double(r.y));
+                    atomicAdd(f + 2, double(r.z));
+                    atomicAdd(f + 3, double(a));
+                    atomicAdd(P.provCounts + pixel, 1u);
+                    atomicAdd(&P.paging->provisional, 1u);
+                }
+                parkSample(P, pixel, index);
+                return;
+            }
+            filmAdd(P, pixel, index, r.x, r.y, r.z, a);
+        }
     }
     __device__ void failed() { ++errors; }
 };
@@ -2036,7 +2179,7 @@ struct LaneFeed {
 // persistent lanes claiming items one at a time (the reference's Feed loop) when built with X2_PERSISTENT: with
 // Regen a lane claims its next item when its sample's last vertex is shaded (runSamples over LaneFeed), without it
 // when the whole sample is (renderItem). Every lane reaches the end so the wave's counter sums see all 64 lanes.
-template <bool Adj, bool Vol, bool Direct = false, bool Regen = false>
+template <bool Adj, bool Vol, bool Direct = false, bool Regen = false, bool Paged = false>
 __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
     TravStack st;
@@ -2048,7 +2191,7 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     for (uint32_t i = 0; i < (Adj ? RoughnessSlots : 1); ++i) gr[i] = 0.0f;
 #if X2_PERSISTENT
     if constexpr (Regen) {
-        LaneFeed<Adj> feed{P, errors};
+        LaneFeed<Adj, Paged> feed{P, errors};
         runSamples<Adj, Vol>(P, feed, k, st, gr);
     } else {
         for (;;) {
@@ -2087,6 +2230,12 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
 // forward and 1.52x adjoint on the textured Cornell box (RR off, long uneven paths).
 __global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) {
     runLaunch<false, false, false, X2_PERSISTENT>(P);
+}
+// The same render with texture tiles on demand: samples that meet a missing tile park (or, with partial shading,
+// finish provisionally) and the launch runs the parked samples of earlier launches ahead of its chunks. Its own
+// instantiation, so the render above carries none of it.
+__global__ void __launch_bounds__(256) X2_OCCUPANCY renderPagedKernel(const RenderParams P) {
+    runLaunch<false, false, false, X2_PERSISTENT, true>(P);
 }
 __global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) {
     runLaunch<true, false, false, X2_PERSISTENT>(P);
@@ -2750,7 +2899,11 @@ __global__ void clearState(LaunchState* s) {
 namespace x2 {
 template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium, 3 the prototype estimator
 hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
-    const auto kernel = Kind == 0 ? renderKernel : Kind == 1 ? adjointKernel : Kind == 2 ? renderVolumeKernel : directKernel;
+    const auto kernel = Kind == 0   ? renderKernel
+                        : Kind == 1 ? adjointKernel
+                        : Kind == 2 ? renderVolumeKernel
+                        : Kind == 3 ? directKernel
+                                    : renderPagedKernel;
     static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
     if (X2_PERSISTENT && !resident) {
         int dev = 0, perCu = 0;
@@ -2764,7 +2917,8 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
     }
     // The device pass chain's render learns its items on the device: a full persistent grid.
     if (p.chain && !X2_PERSISTENT) return hipErrorInvalidValue;
-    const unsigned long long items = p.chain ? ~0ull >> 1 : p.itemEnd - p.itemBegin;
+    // A launch's items: its chunks' and the parked samples it runs again (a launch may hold parked samples alone).
+    const unsigned long long items = p.chain ? ~0ull >> 1 : p.itemEnd - p.itemBegin + p.parkedCount;
     unsigned long long blocks = (items + 255) / 256;
     if (X2_PERSISTENT && blocks > (unsigned long long)resident) blocks = resident;
     if (!blocks) return hipSuccess;
@@ -2773,6 +2927,7 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
 }
 hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     if (p.cfg.direct_only) return launchFrame<3>(p, stream);
+    if (p.tileSlot) return launchFrame<4>(p, stream);  // texture tiles on demand
     return p.vol.table ? launchFrame<2>(p, stream) : launchFrame<0>(p, stream);
 }
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }

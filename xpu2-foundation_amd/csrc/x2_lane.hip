@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "x2_bvh.h"
+#include "x2_half.h"
 #include "x2_layout.h"
 #include "x2_volume.h"
 #include "x2_adaptive_core.h"
@@ -79,6 +80,40 @@ struct Flight {
     uint64_t launch, first, count, samples;
     uint32_t slot;
     bool wave;  // run in the wavefront form
+    uint64_t parkedBegin = 0, parkedCount = 0, items = 0;  // parked samples it ran again; items it could park
+};
+
+// On-demand texture tiles (GpuLane's fine-detail cache, async_gpu.h:2911-3200): every paged texture's level 0 as
+// 32x32 tiles of half-float rgb on the host, a pool of them on the device with the scene-wide page table, the tiles the
+// kernels asked for, the ring of parked samples and the provisional film.
+struct Paging {
+    bool on = false, partial = false;
+    uint32_t tiles = 0, poolTiles = 0, missCap = 0, uploadCap = 0;
+    uint64_t parkCap = 0;
+    std::vector<uint16_t> hostTiles;  // tile t's texels at t * TexTileTexels * 3
+    std::vector<uint32_t> slotOf;     // host copy of the page table
+    std::vector<uint32_t> tileIn;     // each pool slot's tile (NoTileSlot: free)
+    std::deque<uint32_t> loadOrder;   // pool slots, oldest load first (the next evicted)
+    std::vector<uint8_t> queued;      // asked for and not yet sent up
+    std::vector<uint32_t> pending;
+    std::vector<float*> coarse;       // each texture's coarse level (device)
+    uint32_t* tileSlot = nullptr;
+    uint32_t* tileAsked = nullptr;
+    uint32_t* missList = nullptr;
+    uint16_t* pool = nullptr;
+    PagingState* state = nullptr;
+    x2_parked* parked = nullptr;
+    double* provSums = nullptr;
+    uint32_t* provCounts = nullptr;
+    double* hostProvSums = nullptr;
+    uint32_t* hostProvCounts = nullptr;
+    size_t provPixels = 0;
+    PagingState* readback = nullptr;             // one per launch slot (pinned)
+    uint32_t* missReadback[Slots] = {};          // missCap entries per launch slot (pinned)
+    uint16_t* upload[Slots] = {};                // tile staging per launch slot (pinned)
+    uint32_t* tableStaging[Slots] = {};          // the page table per launch slot (pinned)
+    uint64_t parkTail = 0, parkIssued = 0, inflightItems = 0;
+    uint64_t parkedTotal = 0, provisional = 0, misses = 0, uploads = 0, evictions = 0, overflow = 0;
 };
 
 
@@ -150,13 +185,37 @@ struct x2dev {
     const float* dImage = nullptr;
     float* gradTextures = nullptr;
     float* gradRoughness = nullptr;
+    Paging pg;
 };
 
 namespace {
+void releasePaging(x2dev* d) {
+    Paging& g = d->pg;
+    (void)hipFree(g.tileSlot);
+    (void)hipFree(g.tileAsked);
+    (void)hipFree(g.missList);
+    (void)hipFree(g.pool);
+    (void)hipFree(g.state);
+    (void)hipFree(g.parked);
+    (void)hipFree(g.provSums);
+    (void)hipFree(g.provCounts);
+    for (float* c : g.coarse) (void)hipFree(c);
+    if (g.hostProvSums) (void)hipHostFree(g.hostProvSums);
+    if (g.hostProvCounts) (void)hipHostFree(g.hostProvCounts);
+    if (g.readback) (void)hipHostFree(g.readback);
+    for (uint32_t i = 0; i < Slots; ++i) {
+        if (g.missReadback[i]) (void)hipHostFree(g.missReadback[i]);
+        if (g.upload[i]) (void)hipHostFree(g.upload[i]);
+        if (g.tableStaging[i]) (void)hipHostFree(g.tableStaging[i]);
+    }
+    g = Paging{};
+}
+
 void release(x2dev* d) {
     if (!d) return;
     (void)hipSetDevice(d->device);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
+    releasePaging(d);
     (void)hipFree(d->nodes);
     (void)hipFree(d->tris);
     (void)hipFree(d->materials);
@@ -318,6 +377,7 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     (void)hipFree(d->triUV);
     (void)hipFree(d->textures);
     for (float* t : d->texels) (void)hipFree(t);
+    releasePaging(d);  // a new scene's textures are resident until paged again
     for (void*& b : d->volBuffers) {
         (void)hipFree(b);
         b = nullptr;
@@ -437,6 +497,40 @@ int x2dev_begin_adaptive(x2dev* d, const x2_config* c, uint32_t samples_per_chun
 }
 
 namespace {
+// A frame's paging state: the parked ring (three launches of the cap), the provisional film, the counters cleared.
+int beginPaging(x2dev* d, size_t pixels) {
+    Paging& g = d->pg;
+    const uint64_t cap = std::max<uint64_t>(uint64_t(3) * d->opt.max_chunks_per_launch * d->itemsPerChunk, 1u << 20);
+    if (cap > g.parkCap) {
+        (void)hipFree(g.parked);
+        g.parked = nullptr;
+        g.parkCap = 0;
+        X2_HIP(hipMalloc(&g.parked, cap * sizeof(x2_parked)));
+        g.parkCap = cap;
+    }
+    if (pixels != g.provPixels) {
+        (void)hipFree(g.provSums);
+        (void)hipFree(g.provCounts);
+        if (g.hostProvSums) (void)hipHostFree(g.hostProvSums);
+        if (g.hostProvCounts) (void)hipHostFree(g.hostProvCounts);
+        g.provSums = nullptr;
+        g.provCounts = nullptr;
+        g.hostProvSums = nullptr;
+        g.hostProvCounts = nullptr;
+        g.provPixels = 0;
+        X2_HIP(hipMalloc(&g.provSums, pixels * 4 * sizeof(double)));
+        X2_HIP(hipMalloc(&g.provCounts, pixels * sizeof(uint32_t)));
+        X2_HIP(hipHostMalloc(&g.hostProvSums, pixels * 4 * sizeof(double), hipHostMallocDefault));
+        X2_HIP(hipHostMalloc(&g.hostProvCounts, pixels * sizeof(uint32_t), hipHostMallocDefault));
+        g.provPixels = pixels;
+    }
+    X2_HIP(hipMemsetAsync(g.provSums, 0, pixels * 4 * sizeof(double), d->stream));
+    X2_HIP(hipMemsetAsync(g.provCounts, 0, pixels * sizeof(uint32_t), d->stream));
+    X2_HIP(hipMemsetAsync(g.state, 0, sizeof(PagingState), d->stream));
+    g.parkTail = g.parkIssued = g.inflightItems = 0;
+    g.parkedTotal = g.provisional = g.overflow = 0;
+    return X2_OK;
+}
 int beginFrame(x2dev* d, const x2_config* c, uint32_t samples_per_chunk, bool adaptive) {
     if (!d || !c) return fail(X2_ERR_ARGUMENT, "null device or config");
     if (!d->prepared) return fail(X2_ERR_STATE, "begin before prepare");
@@ -474,6 +568,11 @@ int beginFrame(x2dev* d, const x2_config* c, uint32_t samples_per_chunk, bool ad
     d->filmQueued = false;
     d->errorsQueued = false;
     d->adaptive = adaptive;
+    if (d->pg.on) {
+        if (c->direct_only) return fail(X2_ERR_STATE, "paged textures: the prototype estimator reads resident textures");
+        const int r = beginPaging(d, pixels);
+        if (r) return r;
+    }
     if (adaptive) {
         if (d->volDesc.table || c->direct_only) return fail(X2_ERR_ARGUMENT, "adaptive frames render the path tracer");
         const int r = ensureAdaptive(d, pixels, size_t((c->width + 7) / 8) * ((c->height + 7) / 8));
@@ -492,9 +591,16 @@ uint32_t x2dev_launch_threads(const x2dev* d) { return d ? d->launchThreads : 0u
 uint32_t x2dev_chunk_room(const x2dev* d) {
     if (!d) return 0u;
     const uint64_t used = d->published - oldestChunk(d);
-    const uint64_t room = used >= d->opt.ring_chunks ? 0 : d->opt.ring_chunks - used;
+    uint64_t room = used >= d->opt.ring_chunks ? 0 : d->opt.ring_chunks - used;
+    if (d->pg.on && d->itemsPerChunk) {  // what the launch may park, on top of what is parked and may park in flight
+        const Paging& g = d->pg;
+        const uint64_t bound = g.parkTail + g.inflightItems + (g.parkTail - g.parkIssued);
+        const uint64_t limit = g.parkIssued + g.parkCap;
+        room = std::min<uint64_t>(room, bound >= limit ? 0 : (limit - bound) / d->itemsPerChunk);
+    }
     return uint32_t(std::min<uint64_t>(room, d->opt.max_chunks_per_launch));
 }
+uint64_t x2dev_parked_outstanding(const x2dev* d) { return d && d->pg.on ? d->pg.parkTail - d->pg.parkIssued : 0u; }
 
 namespace {
 // What every launch of the frame reads (its items and, for the chain, its pass set by the caller).
@@ -518,6 +624,20 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
     p.ringCapacity = d->opt.ring_chunks;
     p.counters = d->opt.counters;
     p.adaptive = d->adaptive ? 1u : 0u;
+    if (d->pg.on) {
+        const Paging& g = d->pg;
+        p.tileSlot = g.tileSlot;
+        p.tilePool = g.pool;
+        p.tileAsked = g.tileAsked;
+        p.missList = g.missList;
+        p.paging = g.state;
+        p.parked = g.parked;
+        p.provSums = g.provSums;
+        p.provCounts = g.provCounts;
+        p.parkCap = uint32_t(std::min<uint64_t>(g.parkCap, 0xffffffffu));
+        p.missCap = g.missCap;
+        p.partial = g.partial ? 1u : 0u;
+    }
     p.vol = d->volDesc;
     p.odd = d->adaptive ? d->odd : nullptr;
     for (uint32_t i = 0; i < PassSlots; ++i) {
@@ -576,11 +696,176 @@ int ensureWave(x2dev* d, uint32_t perVertex) {
 }
 }  // namespace
 
+namespace {
+// GpuLane::commitTiles (async_gpu.h:3164-3200): the tiles asked for go up into the pool, the oldest asked first, at
+// most uploadCap a launch; a full pool gives up its oldest load. Then the page table, and every tile's asked-for flag
+// cleared (a tile evicted later is asked for again; one asked for by a launch still in flight may be asked twice,
+// which the host's queue keeps once). Stream order keeps every copy after the launches before and ahead of the next.
+int commitTiles(x2dev* d, uint32_t slot) {
+    Paging& g = d->pg;
+    if (g.pending.empty()) return X2_OK;
+    const size_t tileBytes = size_t(TexTileTexels) * 3 * sizeof(uint16_t);
+    const size_t n = std::min<size_t>({g.pending.size(), size_t(g.uploadCap), size_t(g.poolTiles)});
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t tile = g.pending[i];
+        uint32_t at;
+        if (g.loadOrder.size() < g.poolTiles) {
+            at = uint32_t(g.loadOrder.size());
+        } else {
+            at = g.loadOrder.front();
+            g.loadOrder.pop_front();
+            if (g.tileIn[at] != NoTileSlot) {
+                g.slotOf[g.tileIn[at]] = NoTileSlot;
+                ++g.evictions;
+            }
+        }
+        g.loadOrder.push_back(at);
+        g.tileIn[at] = tile;
+        g.slotOf[tile] = at;
+        g.queued[tile] = 0;
+        std::memcpy(g.upload[slot] + i * TexTileTexels * 3, g.hostTiles.data() + size_t(tile) * TexTileTexels * 3,
+                    tileBytes);
+        X2_HIP(hipMemcpyAsync(g.pool + size_t(at) * TexTileTexels * 3, g.upload[slot] + i * TexTileTexels * 3,
+                              tileBytes, hipMemcpyHostToDevice, d->stream));
+        ++g.uploads;
+    }
+    g.pending.erase(g.pending.begin(), g.pending.begin() + std::ptrdiff_t(n));
+    std::memcpy(g.tableStaging[slot], g.slotOf.data(), size_t(g.tiles) * sizeof(uint32_t));
+    X2_HIP(hipMemcpyAsync(g.tileSlot, g.tableStaging[slot], size_t(g.tiles) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          d->stream));
+    X2_HIP(hipMemsetAsync(g.tileAsked, 0, size_t(g.tiles) * sizeof(uint32_t), d->stream));
+    return X2_OK;
+}
+}  // namespace
+
+int x2dev_page_textures(x2dev* d, const x2_paging* o) {
+    if (!d || !o) return fail(X2_ERR_ARGUMENT, "null device or paging");
+    if (!d->prepared) return fail(X2_ERR_STATE, "paging before prepare");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "paging with launches in flight");
+    if (d->pg.on) return fail(X2_ERR_STATE, "the scene's textures are paged already");
+    if (d->texDesc.empty()) return fail(X2_ERR_STATE, "the scene has no texture to page");
+    if (d->volDesc.table) return fail(X2_ERR_STATE, "paged textures: the medium kernel reads resident textures");
+    if (d->opt.wave_slots) return fail(X2_ERR_STATE, "paged textures: the wavefront form reads resident textures");
+    if (o->pool_tiles == 0 || o->pool_tiles > (1u << 22)) return fail(X2_ERR_ARGUMENT, "pool_tiles must be 1..2^22");
+    const uint32_t coarseSide = o->coarse_side ? o->coarse_side : 64u;
+    X2_HIP(hipSetDevice(d->device));
+    Paging g;
+    g.on = true;
+    g.partial = o->partial != 0;
+    g.poolTiles = o->pool_tiles;
+    g.missCap = 65536;
+    g.uploadCap = std::min<uint32_t>(o->pool_tiles, 1024u);
+    std::vector<TexDesc> desc = d->texDesc;
+    std::vector<float> texels;
+    for (size_t t = 0; t < desc.size(); ++t) {
+        TexDesc& td = desc[t];
+        const uint32_t w = td.width, h = td.height, tx = (w + TexTile - 1) / TexTile, ty = (h + TexTile - 1) / TexTile;
+        texels.resize(size_t(w) * h * 3);
+        X2_HIP(hipMemcpy(texels.data(), d->texels[t], texels.size() * sizeof(float), hipMemcpyDeviceToHost));
+        td.tileBase = g.tiles;
+        td.tilesX = tx;
+        g.tiles += tx * ty;
+        // Level 0 in tiles of half floats (a float file's levels are held in half floats, texture_core.h:30-31),
+        // a partial tile's texels past the image edge zero (never read: taps wrap inside the image).
+        const size_t base = g.hostTiles.size();
+        g.hostTiles.resize(base + size_t(tx) * ty * TexTileTexels * 3, 0);
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < w; ++x) {
+                const size_t tile = size_t(y / TexTile) * tx + x / TexTile;
+                uint16_t* dst = g.hostTiles.data() + base + (tile * TexTileTexels + (y % TexTile) * TexTile + x % TexTile) * 3;
+                for (int c = 0; c < 3; ++c) dst[c] = x2::toHalf(texels[(size_t(y) * w + x) * 3 + c]);
+            }
+        // The coarse level: level 0 (as its halves read) averaged over boxes, at most coarseSide a side.
+        uint32_t cw = w, ch = h;
+        while (cw > coarseSide) cw = (cw + 1) / 2;
+        while (ch > coarseSide) ch = (ch + 1) / 2;
+        std::vector<float> coarse(size_t(cw) * ch * 3);
+        for (uint32_t j = 0; j < ch; ++j)
+            for (uint32_t i = 0; i < cw; ++i) {
+                const uint32_t x0 = uint32_t(uint64_t(i) * w / cw), x1 = std::max(x0 + 1, uint32_t(uint64_t(i + 1) * w / cw));
+                const uint32_t y0 = uint32_t(uint64_t(j) * h / ch), y1 = std::max(y0 + 1, uint32_t(uint64_t(j + 1) * h / ch));
+                double acc[3] = {0, 0, 0};
+                for (uint32_t y = y0; y < y1; ++y)
+                    for (uint32_t x = x0; x < x1; ++x)
+                        for (int c = 0; c < 3; ++c) acc[c] += x2::fromHalf(x2::toHalf(texels[(size_t(y) * w + x) * 3 + c]));
+                const double area = double(x1 - x0) * double(y1 - y0);
+                for (int c = 0; c < 3; ++c) coarse[(size_t(j) * cw + i) * 3 + c] = float(acc[c] / area);
+            }
+        float* dc = nullptr;
+        X2_HIP(hipMalloc(&dc, coarse.size() * sizeof(float)));
+        g.coarse.push_back(dc);
+        X2_HIP(hipMemcpy(dc, coarse.data(), coarse.size() * sizeof(float), hipMemcpyHostToDevice));
+        td.coarse = dc;
+        td.coarseW = cw;
+        td.coarseH = ch;
+        td.paged = 1;
+        td.texels = nullptr;
+    }
+    g.slotOf.assign(g.tiles, NoTileSlot);
+    g.tileIn.assign(g.poolTiles, NoTileSlot);
+    g.queued.assign(g.tiles, 0);
+    const size_t tileBytes = size_t(TexTileTexels) * 3 * sizeof(uint16_t);
+    X2_HIP(hipMalloc(&g.tileSlot, size_t(g.tiles) * sizeof(uint32_t)));
+    X2_HIP(hipMalloc(&g.tileAsked, size_t(g.tiles) * sizeof(uint32_t)));
+    X2_HIP(hipMalloc(&g.missList, size_t(g.missCap) * sizeof(uint32_t)));
+    X2_HIP(hipMalloc(&g.pool, size_t(g.poolTiles) * tileBytes));
+    X2_HIP(hipMalloc(&g.state, sizeof(PagingState)));
+    X2_HIP(hipMemcpy(g.tileSlot, g.slotOf.data(), size_t(g.tiles) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    X2_HIP(hipMemset(g.tileAsked, 0, size_t(g.tiles) * sizeof(uint32_t)));
+    X2_HIP(hipMemset(g.state, 0, sizeof(PagingState)));
+    X2_HIP(hipHostMalloc(&g.readback, Slots * sizeof(PagingState), hipHostMallocDefault));
+    for (uint32_t i = 0; i < Slots; ++i) {
+        X2_HIP(hipHostMalloc(&g.missReadback[i], size_t(g.missCap) * sizeof(uint32_t), hipHostMallocDefault));
+        X2_HIP(hipHostMalloc(&g.upload[i], size_t(g.uploadCap) * tileBytes, hipHostMallocDefault));
+        X2_HIP(hipHostMalloc(&g.tableStaging[i], size_t(g.tiles) * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    // The descriptors now read the tiles; level 0's resident copies are let go.
+    X2_HIP(hipMemcpy(d->textures, desc.data(), desc.size() * sizeof(TexDesc), hipMemcpyHostToDevice));
+    for (float*& t : d->texels) {
+        (void)hipFree(t);
+        t = nullptr;
+    }
+    d->texDesc = desc;
+    releasePaging(d);
+    d->pg = std::move(g);
+    d->begun_ = false;  // a frame begun before paging reads the old descriptors: begin again
+    return X2_OK;
+}
+
+int x2dev_paging_stats(const x2dev* d, x2_paging_stats* out) {
+    if (!d || !out) return fail(X2_ERR_ARGUMENT, "null device or stats");
+    std::memset(out, 0, sizeof *out);
+    const Paging& g = d->pg;
+    if (!g.on) return X2_OK;
+    out->parked_outstanding = g.parkTail - g.parkIssued;
+    out->parked_total = g.parkedTotal;
+    out->provisional = g.provisional;
+    out->misses = g.misses;
+    out->uploads = g.uploads;
+    out->evictions = g.evictions;
+    out->overflow = g.overflow;
+    out->tiles = g.tiles;
+    out->pool_tiles = g.poolTiles;
+    uint32_t resident = 0;
+    for (uint32_t t : g.tileIn) resident += t != NoTileSlot;
+    out->resident = resident;
+    return X2_OK;
+}
+
+int x2dev_provisional_film(const x2dev* d, const double** sums, const uint32_t** counts) {
+    if (!d || !sums || !counts) return fail(X2_ERR_ARGUMENT, "null argument");
+    if (!d->pg.on || !d->pg.hostProvSums) return fail(X2_ERR_STATE, "no provisional film: the textures are not paged");
+    *sums = d->pg.hostProvSums;
+    *counts = d->pg.hostProvCounts;
+    return X2_OK;
+}
+
 int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samples) {
     if (!d) return fail(X2_ERR_ARGUMENT, "null device");
     if (!d->begun_) return fail(X2_ERR_STATE, "launch before begin");
-    if (!count) return 0;
-    if (!chunks) return fail(X2_ERR_ARGUMENT, "null chunks");
+    const uint64_t parkedCount = d->pg.on ? d->pg.parkTail - d->pg.parkIssued : 0;
+    if (!count && !parkedCount) return 0;  // nothing to run: no chunks and no parked sample
+    if (count && !chunks) return fail(X2_ERR_ARGUMENT, "null chunks");
     if (d->inFlight.size() >= 2 || count > d->opt.max_chunks_per_launch)
         return fail(X2_ERR_BACKPRESSURE, "GPU launch backpressure (async_gpu.h:1435)");
     if (d->published + count - oldestChunk(d) > d->opt.ring_chunks) return fail(X2_ERR_BACKPRESSURE, "mailbox ring full");
@@ -591,7 +876,22 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
             return fail(X2_ERR_ARGUMENT, "chunk out of range (tile, or count above samples_per_chunk)");
     X2_HIP(hipSetDevice(d->device));
     const uint32_t slot = uint32_t(d->launches % Slots);
-    std::memcpy(d->staging[slot], chunks, size_t(count) * sizeof(x2_chunk));
+    const bool wave = waveForm(d);
+    uint64_t parkedBegin = 0;
+    const uint64_t items = parkedCount + uint64_t(count) * d->itemsPerChunk;
+    if (d->pg.on) {
+        Paging& g = d->pg;
+        if (wave) return fail(X2_ERR_STATE, "paged textures: the wavefront form reads resident textures");
+        if (parkedCount > 0xffffffffu) return fail(X2_ERR_BACKPRESSURE, "too many parked samples for one launch");
+        if (g.parkTail + g.inflightItems + items > g.parkIssued + g.parkCap)
+            return fail(X2_ERR_BACKPRESSURE, "parked-sample ring would overflow (async_gpu.h:1447)");
+        const int r = commitTiles(d, slot);  // the tiles asked for so far, before the samples that wait for them
+        if (r) return r;
+        parkedBegin = g.parkIssued;
+        g.parkIssued += parkedCount;
+        g.inflightItems += items;
+    }
+    if (count) std::memcpy(d->staging[slot], chunks, size_t(count) * sizeof(x2_chunk));
     for (uint32_t done = 0; done < count;) {  // the ring wraps
         const uint64_t at = (d->published + done) % d->opt.ring_chunks;
         const uint32_t n = uint32_t(std::min<uint64_t>(count - done, d->opt.ring_chunks - at));
@@ -603,7 +903,8 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     RenderParams p = renderParams(d, slot);
     p.itemBegin = d->published * d->itemsPerChunk;
     p.itemEnd = (d->published + count) * d->itemsPerChunk;
-    const bool wave = waveForm(d);
+    p.parkedBegin = parkedBegin;
+    p.parkedCount = uint32_t(parkedCount);
     if (wave) {
         const uint32_t perVertex = d->cfg.light_count * std::max(1u, d->cfg.light_samples) +
                                    2u * std::max(1u, d->cfg.bsdf_samples);
@@ -626,8 +927,20 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     if (wave)
         X2_HIP(hipMemcpyAsync(d->waveReadback + slot, d->wave.counts, sizeof(WaveCounts), hipMemcpyDeviceToHost,
                               d->stream));
+    if (d->pg.on) {  // the launch's paging counters and miss list, then its counters cleared for the next launch
+        Paging& g = d->pg;
+        X2_HIP(hipMemcpyAsync(g.readback + slot, g.state, sizeof(PagingState), hipMemcpyDeviceToHost, d->stream));
+        X2_HIP(hipMemcpyAsync(g.missReadback[slot], g.missList, size_t(g.missCap) * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, d->stream));
+        X2_HIP(hipMemsetAsync(reinterpret_cast<char*>(g.state) + offsetof(PagingState, missCount), 0,
+                              sizeof(PagingState) - offsetof(PagingState, missCount), d->stream));
+    }
     X2_HIP(hipEventRecord(d->done[slot], d->stream));
-    d->inFlight.push_back(Flight{d->launches, d->published, count, samples, slot, wave});
+    Flight flight{d->launches, d->published, count, samples, slot, wave};
+    flight.parkedBegin = parkedBegin;
+    flight.parkedCount = parkedCount;
+    flight.items = items;
+    d->inFlight.push_back(flight);
     d->published += count;
     ++d->launches;
     return 1;
@@ -642,6 +955,28 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
     if (q != hipSuccess) return hipFail(q, "hipEventQuery");
     d->inFlight.pop_front();
     const LaunchState& s = d->readback[f.slot];
+    uint32_t lost = 0;
+    if (d->pg.on) {  // the samples it parked, the tiles it asked for
+        Paging& g = d->pg;
+        const PagingState& rb = g.readback[f.slot];
+        const uint64_t tail = std::min<uint64_t>(rb.parkTail, f.parkedBegin + g.parkCap);
+        if (tail > g.parkTail) {
+            g.parkedTotal += tail - g.parkTail;
+            g.parkTail = tail;
+        }
+        g.inflightItems -= std::min(g.inflightItems, f.items);
+        g.provisional += rb.provisional;
+        g.overflow += rb.parkOverflow;
+        lost = rb.parkOverflow;
+        const uint32_t n = std::min(rb.missCount, g.missCap);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t tile = g.missReadback[f.slot][i];
+            if (tile >= g.tiles || g.queued[tile] || g.slotOf[tile] != NoTileSlot) continue;
+            g.queued[tile] = 1;
+            g.pending.push_back(tile);
+            ++g.misses;
+        }
+    }
     if (f.wave) {
         const double v = double(d->waveReadback[f.slot].traced) / double(f.count * uint64_t(d->itemsPerChunk));
         d->waveVertices = d->waveVertices > 0 ? 0.5 * (d->waveVertices + v) : v;
@@ -651,7 +986,7 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
     out->chunks = f.count;
     out->items = f.count * uint64_t(d->itemsPerChunk);
     out->samples = f.samples;
-    out->errors = s.errors;
+    out->errors = s.errors + lost;  // a sample the parked ring could not hold is lost: a failure
     out->pad = 0;
     out->device_seconds = ms / 1000.0;
     out->rays_trace = s.traces;
@@ -678,6 +1013,12 @@ int x2dev_download_film(x2dev* d) {
     X2_HIP(hipMemcpyAsync(d->hostCounts, d->counts, d->filmPixels * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
     if (d->adaptive)
         X2_HIP(hipMemcpyAsync(d->hostOdd, d->odd, d->filmPixels * 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    if (d->pg.on && d->pg.provPixels == d->filmPixels) {
+        X2_HIP(hipMemcpyAsync(d->pg.hostProvSums, d->pg.provSums, d->filmPixels * 4 * sizeof(double),
+                              hipMemcpyDeviceToHost, d->stream));
+        X2_HIP(hipMemcpyAsync(d->pg.hostProvCounts, d->pg.provCounts, d->filmPixels * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, d->stream));
+    }
     X2_HIP(hipEventRecord(d->filmDone, d->stream));
     d->filmQueued = true;
     return X2_OK;
@@ -777,6 +1118,7 @@ int x2dev_set_texture(x2dev* d, uint32_t texture, const float* texels, int texel
     if (!d || !texels) return fail(X2_ERR_ARGUMENT, "null device or texels");
     if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "texture update with launches in flight");
     if (texture >= d->texDesc.size()) return fail(X2_ERR_ARGUMENT, "no such texture");
+    if (d->pg.on) return fail(X2_ERR_STATE, "a paged texture's texels are the tiles x2dev_page_textures made");
     const TexDesc& t = d->texDesc[texture];
     X2_HIP(hipSetDevice(d->device));
     X2_HIP(hipMemcpyAsync(d->texels[texture], texels, size_t(t.width) * t.height * 3 * sizeof(float),
@@ -876,12 +1218,16 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
     // A launch's device time once the rate is known: long enough that max_chunks_per_launch bounds it at the
     // headline's rate (a launch's last samples leave lanes idle, so fewer, larger launches waste less).
     const double windowMs = 100.0;
-    const uint64_t firstChunks = 2048, cap = d->opt.max_chunks_per_launch;
+    const uint64_t firstChunks = 2048;
     const uint64_t least = (uint64_t(d->launchThreads) * 2 + d->itemsPerChunk - 1) / d->itemsPerChunk;
     double rate = 0;  // items per device second, from completed launches
     size_t next = 0;
     int r = 0;
-    while (next < chunks.size() || !d->inFlight.empty()) {
+    // Paged textures: parked samples run again (in the next launches, or in launches of them alone at the end) until
+    // none is left; a stretch of launches of parked samples alone that ends none means the pool cannot hold what they
+    // need (async_render.h:816-818).
+    uint64_t fewest = ~0ull, idle = 0, rounds = 0;
+    while (next < chunks.size() || !d->inFlight.empty() || x2dev_parked_outstanding(d)) {
         x2_completion done;
         while ((r = x2dev_poll(d, &done)) == 1) {
             ++st.launches;
@@ -899,11 +1245,26 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
             }
         }
         if (r < 0) return r;
+        if (next >= chunks.size() && x2dev_ready(d) && x2dev_parked_outstanding(d)) {
+            const uint64_t left = x2dev_parked_outstanding(d) + d->pg.inflightItems;
+            if (left < fewest) {
+                fewest = left;
+                idle = 0;
+            } else {
+                ++idle;
+            }
+            // A pool smaller than what a sample's path reads at once parks it again and again: the frame fails (as
+            // the reference's does, async_render.h:816-818) rather than run on.
+            if (idle > 256 || ++rounds > 4096)
+                return fail(X2_ERR_STATE, "the tile pool cannot hold the parked samples' working set");
+            if ((r = x2dev_launch(d, nullptr, 0, 0)) < 0) return r;
+        }
         while (x2dev_ready(d) && next < chunks.size()) {
             const uint64_t remaining = chunks.size() - next;
             uint64_t want = rate ? std::max<uint64_t>(uint64_t(rate * windowMs / 1000.0 / d->itemsPerChunk), least)
                                  : std::max<uint64_t>(firstChunks, least);
-            want = std::min<uint64_t>({want, cap, remaining});
+            want = std::min<uint64_t>({want, uint64_t(x2dev_chunk_room(d)), remaining});
+            if (!want) break;  // the parked ring is full: the launches in flight make room
             uint64_t samples = 0;
             const uint32_t tilesX = (c->width + 7) / 8;
             for (uint64_t i = 0; i < want; ++i) {
@@ -1082,6 +1443,7 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
     if (a->max_samples == 0 || a->max_samples > 65535 || !(a->target_error > 0.0f))
         return fail(X2_ERR_ARGUMENT, "max_samples must be 1..65535 and target_error positive");
     if (a->cellular && a->device_chain) return fail(X2_ERR_ARGUMENT, "cellular generations run on the host, not the chain");
+    if (a->device_chain && d->pg.on) return fail(X2_ERR_STATE, "the device pass chain reads resident textures");
     x2_config cfg = *c;
     cfg.spp = a->max_samples;
     int r = x2dev_begin_adaptive(d, &cfg, spc);
@@ -1242,6 +1604,7 @@ int x2dev_adjoint_frame(x2dev* d, const x2_config* c, uint32_t spc, const float*
                         float* grad_roughness, x2_frame_stats* stats) {
     using clock = std::chrono::steady_clock;
     const auto start = clock::now();
+    if (d && d->pg.on) return fail(X2_ERR_STATE, "the adjoint reads resident textures (the textures are paged)");
     if (!d || !c || !d_image || !grad_roughness || (!d->texDesc.empty() && !grad_textures))
         return fail(X2_ERR_ARGUMENT, "null device, config, image adjoint or gradient buffer");
     if (!d->prepared) return fail(X2_ERR_STATE, "adjoint before prepare");

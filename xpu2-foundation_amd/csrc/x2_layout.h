@@ -49,9 +49,26 @@ struct alignas(64) LaunchState {
 
 // A texture on the device: its texels (RGB f32, row-major) and where its gradient starts in the adjoint's buffer.
 struct TexDesc {
-    const float* texels;
+    const float* texels;  // resident level 0 (null when paged)
     uint32_t width, height;
     unsigned long long gradOffset;  // in floats
+    // On-demand tiles (texture_core.h:37-60, the GPU's fine-detail cache, async_gpu.h:2911-3200): level 0 as 32x32
+    // tiles of half-float rgb, paged into the device's tile pool by the host; tile (x >> 5, y >> 5) is the scene-wide
+    // tile tileBase + (y >> 5) * tilesX + (x >> 5). coarse: an always-resident smaller level (coarseW x coarseH rgb
+    // floats), read where level 0's tile is missing with partial shading.
+    const float* coarse;
+    uint32_t coarseW, coarseH, tileBase, tilesX;
+    uint32_t paged, pad;
+};
+constexpr uint32_t TexTile = 32, TexTileTexels = TexTile * TexTile;
+constexpr uint32_t NoTileSlot = 0xffffffffu;
+// The paging state the kernels write and the host reads back after each launch: samples parked for a missing tile
+// (a ring the next launches run again), the tiles asked for, provisional samples (partial shading).
+struct PagingState {
+    unsigned long long parkTail;  // parked samples ever written (ring position of the next)
+    uint32_t missCount;           // tiles this launch asked for (entries of the miss list)
+    uint32_t parkOverflow;        // samples that could not be parked (the ring was full): failures
+    uint32_t provisional, pad[3];
 };
 constexpr uint32_t MaxTextures = 16;
 constexpr uint32_t RoughnessSlots = 8;  // materials whose roughness gradient a lane accumulates in registers
@@ -107,6 +124,20 @@ struct RenderParams {
     const unsigned long long* chainMasks;
     const uint32_t* chainBase;
     uint32_t chainSize, chainPad;
+    // On-demand texture tiles: the scene-wide page table (a tile's pool slot, or NoTileSlot), the pool
+    // (TexTileTexels * 3 halves a slot), each tile's asked-for flag and the launch's miss list; the parked ring
+    // (parkCap records) whose [parkedBegin, parkedBegin + parkedCount) this launch runs before its chunks; with
+    // partial shading the provisional film. Null tileSlot: no texture pages.
+    const uint32_t* tileSlot;
+    const uint16_t* tilePool;
+    uint32_t* tileAsked;
+    uint32_t* missList;
+    PagingState* paging;
+    x2_parked* parked;
+    double* provSums;
+    uint32_t* provCounts;
+    unsigned long long parkedBegin;
+    uint32_t parkedCount, parkCap, missCap, partial;
 };
 constexpr uint32_t PassSlots = 4, PassShift = 16, MaxPasses = 160;  // async_integrator.h:3083
 

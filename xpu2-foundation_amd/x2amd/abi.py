@@ -192,6 +192,16 @@ class StagedFrame(ctypes.Structure):
                 ("resumed_from", u32), ("done", u32), ("pad", u32), ("seconds", f64), ("device_seconds", f64)]
 
 
+class Paging(ctypes.Structure):
+    _fields_ = [("pool_tiles", u32), ("coarse_side", u32), ("partial", u32), ("pad", u32)]
+
+
+class PagingStats(ctypes.Structure):
+    _fields_ = [("parked_outstanding", u64), ("parked_total", u64), ("provisional", u64), ("misses", u64),
+                ("uploads", u64), ("evictions", u64), ("overflow", u64), ("tiles", u32), ("pool_tiles", u32),
+                ("resident", u32), ("pad", u32)]
+
+
 # Sizes of the C structures (include/x2dev.h); checked against ctypes here and against the C compiler by a
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
@@ -202,7 +212,8 @@ EXPECTED_SIZES = {
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),
     "x2_grid_options": (GridOptions, 24), "x2_cpu_lane": (CpuLane, 24), "x2_broker_options": (BrokerOptions, 16),
     "x2_grid_frame": (GridFrame, 120), "x2_film_file": (FilmFile, 40), "x2_exr_image": (ExrImage, 56),
-    "x2_staged": (Staged, 24), "x2_staged_frame": (StagedFrame, 48),
+    "x2_staged": (Staged, 24), "x2_staged_frame": (StagedFrame, 48), "x2_paging": (Paging, 16),
+    "x2_paging_stats": (PagingStats, 72),
 }
 for _name, (_cls, _size) in EXPECTED_SIZES.items():
     assert ctypes.sizeof(_cls) == _size, (_name, ctypes.sizeof(_cls), _size)
@@ -297,6 +308,11 @@ SIGNATURES = {
                                          ctypes.POINTER(CpuLane), ctypes.POINTER(BrokerOptions),
                                          ctypes.POINTER(GridFrame)]),
     "x2dev_chunk_room": (u32, [ctypes.c_void_p]),
+    "x2dev_page_textures": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Paging)]),
+    "x2dev_paging_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PagingStats)]),
+    "x2dev_parked_outstanding": (u64, [ctypes.c_void_p]),
+    "x2dev_provisional_film": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(f64)),
+                                              ctypes.POINTER(ctypes.POINTER(u32))]),
     "x2_film_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FilmFile)]),
     "x2_film_load": (ctypes.c_int, [ctypes.c_char_p, u32, u32, u32, ctypes.POINTER(f64), ctypes.POINTER(u32),
                                     ctypes.POINTER(f64), ctypes.POINTER(u32), ctypes.POINTER(u32)]),

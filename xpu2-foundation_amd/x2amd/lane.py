@@ -206,6 +206,30 @@ class GpuLane:
         return sums, counts, out
 
 
+    def page_textures(self, pool_tiles, coarse_side=0, partial=False):
+        """On-demand texture tiles (x2dev_page_textures): level 0 in 32x32 half-float tiles, pool_tiles of them on the
+        device; with partial, a sample meeting a missing tile finishes from the coarse level, provisionally."""
+        abi.check(self.lib.x2dev_page_textures(self.h, ctypes.byref(abi.Paging(pool_tiles, coarse_side,
+                                                                                 1 if partial else 0, 0))),
+                  "x2dev_page_textures")
+
+    def paging_stats(self):
+        st = abi.PagingStats()
+        abi.check(self.lib.x2dev_paging_stats(self.h, ctypes.byref(st)), "x2dev_paging_stats")
+        return st
+
+    def parked_outstanding(self):
+        return int(self.lib.x2dev_parked_outstanding(self.h))
+
+    def provisional_film(self):
+        """(sums [pixels, 4], counts [pixels]) of the provisional samples, as downloaded with the film."""
+        s = ctypes.POINTER(ctypes.c_double)()
+        c = ctypes.POINTER(ctypes.c_uint32)()
+        abi.check(self.lib.x2dev_provisional_film(self.h, ctypes.byref(s), ctypes.byref(c)), "x2dev_provisional_film")
+        n = self.config.width * self.config.height
+        return (np.ctypeslib.as_array(s, shape=(n * 4,)).reshape(n, 4).copy(),
+                np.ctypeslib.as_array(c, shape=(n,)).copy())
+
     def render_staged(self, config, path=None, stage_samples=0, resume=False, max_stages=0, samples_per_chunk=4):
         """A checkpoint render (x2dev_render_staged): the frame in stages, its film saved to path after each, taken
         up from a save at path with resume. Returns (sums, counts, StagedFrame): the film so far."""
