@@ -14,7 +14,9 @@ adjoint_step_over_forward is the adjoint frame's time over the forward frame's. 
 participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp); the batch leg the 64-job batched
 workload, persistent kernel from a hipGraph against one launch per job; the direct leg the prototype estimator on
 the headline scene; the adaptive leg an adaptive frame of the headline scene through the device pass chain and
-through the host's trees; the grid leg the headline frame from the work grid through the broker. roofline.copy_ceiling_gbs is a measured device copy's bandwidth beside the published peak.
+through the host's trees; the grid leg the headline frame from the work grid through the broker; the paging leg the
+textured Cornell box with its texture resident and in tiles on demand; the checkpoint leg the headline frame in
+stages with its film saved after each, and its EXR. roofline.copy_ceiling_gbs is a measured device copy's bandwidth beside the published peak.
 """
 import argparse
 import json
@@ -295,6 +297,78 @@ def grid_leg(reps, windows=(100.0, 10.0)):
     return out
 
 
+def paging_leg(reps):
+    """Texture tiles on demand on the differentiable Cornell box's forward frame (512x512, 32 spp, its back wall a
+    512x512 map): the texture resident (its texels the tiles' half floats), then paged into a pool as large as the
+    texture (256 tiles; the first frame parks every sample that meets a missing tile and runs it again), and paged
+    with partial shading. Frames after the first find their tiles resident."""
+    import numpy as np
+
+    from x2amd import scenes
+
+    scene, config, params = scenes.cornell_textured()
+    half = params["texture"].astype(np.float16).astype(np.float32)
+    resident, _, _ = scenes.cornell_textured(texture=half)
+    out = {"workload": f"cornell textured {config.width}x{config.height}, {config.spp} spp, depth {config.max_depth}, "
+                       "512x512 map in 256 tiles of 32x32 half floats", "reps": reps}
+    for name, sc, paging in [("resident", resident, None), ("paged", scene, (256, False)),
+                             ("paged_partial", scene, (256, True))]:
+        lane = make_lane()
+        lane.prepare(sc)
+        if paging:
+            lane.page_textures(paging[0], partial=paging[1])
+        t0 = time.perf_counter()
+        _, counts, st = lane.render_frame(config)
+        first = time.perf_counter() - t0
+        assert st.errors == 0 and (counts == config.spp).all()
+        g = lane.paging_stats() if paging else None  # the first frame's parked and provisional samples
+        wall = 0.0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, counts, st = lane.render_frame(config)
+            wall += time.perf_counter() - t0
+        entry = {"first_frame_ms": first * 1e3, "ms_per_frame": wall / reps * 1e3,
+                 "mpaths_per_s": config.width * config.height * config.spp * reps / wall / 1e6}
+        if paging:
+            entry.update(first_frame_parked=g.parked_total, first_frame_provisional=g.provisional, uploads=g.uploads)
+        lane.close()
+        out[name] = entry
+    return out
+
+
+def checkpoint_leg(stage_samples=16):
+    """The headline frame as a checkpoint render (x2dev_render_staged): stages of 16 samples a pixel, the film saved
+    whole after each ("X2RS", 75 MB at 1080p, to a temporary file), against the same frame at once; then the finished
+    film's beauty as a half-float ZIP EXR."""
+    import tempfile
+
+    from x2amd import abi, scenes
+    from x2amd import files as F
+
+    scene, config = scenes.icospheres()
+    lane = make_lane()
+    lane.prepare(scene)
+    lane.render_frame(config)
+    t0 = time.perf_counter()
+    lane.render_frame(config)
+    plain = time.perf_counter() - t0
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "headline.exr.xpu2resume")
+        t0 = time.perf_counter()
+        sums, counts, st = lane.render_staged(config, path, stage_samples=stage_samples)
+        staged = time.perf_counter() - t0
+        assert st.errors == 0 and (counts == config.spp).all()
+        t0 = time.perf_counter()
+        F.write_exr(os.path.join(d, "headline.exr"), config.width, config.height, sums, counts, half=True,
+                    compression=abi.X2_EXR_ZIP)
+        exr = time.perf_counter() - t0
+        exr_bytes = os.path.getsize(os.path.join(d, "headline.exr"))
+    lane.close()
+    return {"workload": f"icospheres {config.width}x{config.height}, {config.spp} spp in stages of {stage_samples}",
+            "plain_ms": plain * 1e3, "staged_ms": staged * 1e3, "stages": st.stages, "saves": st.saves,
+            "staged_over_plain": staged / plain, "exr_ms": exr * 1e3, "exr_bytes": exr_bytes}
+
+
 def copy_ceiling(mib=1024):
     """GB/s of a plain device-to-device copy (read + write bytes over the copy's time), the chip's practical
     bandwidth beside the 8 TB/s peak."""
@@ -364,6 +438,8 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--no-grid", action="store_true")
+    ap.add_argument("--no-paging", action="store_true")
+    ap.add_argument("--no-checkpoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB: only run a device copy (for --pmc scaling)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--launch-chunks", type=int, default=LAUNCH_CHUNKS, help="max_chunks_per_launch of the lanes")
@@ -451,6 +527,10 @@ def main():
         out["adaptive"] = adaptive_leg(2)
     if not args.no_grid:
         out["grid"] = grid_leg(2)
+    if not args.no_paging:
+        out["paging"] = paging_leg(2)
+    if not args.no_checkpoint:
+        out["checkpoint"] = checkpoint_leg()
     out["roofline"]["copy_ceiling_gbs"] = copy_ceiling()
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)

@@ -353,7 +353,8 @@ typedef struct x2_paging {
 } x2_paging;
 typedef struct x2_paging_stats {
     uint64_t parked_outstanding; /* parked, not yet run again (GpuLane::parkedOutstanding, async_gpu.h:1427) */
-    uint64_t parked_total, provisional, misses, uploads, evictions, overflow;
+    uint64_t parked_total, provisional;   /* this frame's (since begin) */
+    uint64_t misses, uploads, evictions, overflow; /* since x2dev_page_textures */
     uint32_t tiles, pool_tiles, resident, pad;
 } x2_paging_stats;
 int x2dev_page_textures(x2dev* d, const x2_paging* paging);
