@@ -2020,7 +2020,7 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
     return s;
 }
 
-// Launch shape knobs (measured on MI355X at the headline workload, profiles/renderKernel_icospheres_1080p64.md):
+// Launch shape knobs (measured on MI355X on the 1080p 64-spp icospheres frame):
 // persistent lanes claiming samples one at a time, and an occupancy target of 3 waves per SIMD (168 VGPRs) were the
 // fastest of the variants tried; an LDS traversal stack was slower than private memory at that occupancy.
 #ifndef X2_PERSISTENT
