@@ -439,6 +439,7 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-paging", action="store_true")
+    ap.add_argument("--samples-per-chunk", type=int, default=4, help="sample indices a chunk holds (the headline)")
     ap.add_argument("--no-checkpoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB: only run a device copy (for --pmc scaling)")
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -463,11 +464,11 @@ def main():
     lane.prepare(scene)
     prepare_s = time.perf_counter() - t
     for _ in range(args.warmup):
-        lane.render_frame(config)
+        lane.render_frame(config, args.samples_per_chunk)
     totals = dict(launches=0, kernel_s=0.0, traces=0, shadows=0, nodes=0, tris=0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        _, counts, st = lane.render_frame(config)
+        _, counts, st = lane.render_frame(config, args.samples_per_chunk)
         assert st.audit_failures == 0 and st.errors == 0
         totals["launches"] += st.launches
         totals["kernel_s"] += st.device_seconds
@@ -508,6 +509,7 @@ def main():
         "triangles_per_ray": totals["tris"] / max(1, rays),
         "prepare_s": prepare_s,
         "launch": {"max_chunks_per_launch": LAUNCH_CHUNKS, "launches_per_frame": totals["launches"] / args.steps,
+                   "samples_per_chunk": args.samples_per_chunk,
                    "form": f"wavefront, {args.wave_slots} path slots" if args.wave_slots else "one kernel"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

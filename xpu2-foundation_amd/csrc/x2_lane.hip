@@ -1186,6 +1186,9 @@ namespace {
 int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<uint32_t>* chunkSamples,
               x2_frame_stats& st);
 
+#ifndef X2_CHUNK_ORDER
+#define X2_CHUNK_ORDER 0
+#endif
 int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
     std::vector<x2_chunk> chunks;
     {
@@ -1201,11 +1204,19 @@ int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
         // lanes in flight work a few cells at a time, whose BVH and triangles the Infinity Cache then holds (1.035x
         // over sweeping the whole image range by range on the headline frame).
         for (uint32_t cy = 0; cy < cellsY; ++cy)
-            for (uint32_t cx = 0; cx < cellsX; ++cx)
+            for (uint32_t cx = 0; cx < cellsX; ++cx) {
+#if X2_CHUNK_ORDER == 1  // tile by tile within the cell, each tile's ranges in turn (measured 1.002x: noise)
+                for (uint32_t ty = cy * 8; ty < std::min(tilesY, (cy + 1) * 8); ++ty)
+                    for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
+                        for (const auto& rg : ranges)
+                            chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
+#else
                 for (const auto& rg : ranges)
                     for (uint32_t ty = cy * 8; ty < std::min(tilesY, (cy + 1) * 8); ++ty)
                         for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
                             chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
+#endif
+            }
     }
     return runChunks(d, chunks, nullptr, st);
 }
