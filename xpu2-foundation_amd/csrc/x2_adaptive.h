@@ -65,7 +65,7 @@ public:
     int regionHeight() const { return regionH_; }
 
 private:
-    int width_, height_, perDim_ = 1, regionW_ = 0, regionH_ = 0;
+    int width_, perDim_ = 1, regionW_ = 0, regionH_ = 0;
     std::vector<Tree> trees_;
 };
 

@@ -99,7 +99,7 @@ uint64_t Tree::sampleArea(int tx, int ty) const {
     return fillRectangle(bits);
 }
 
-Regions::Regions(int width, int height, float target) : width_(width), height_(height) {
+Regions::Regions(int width, int height, float target) : width_(width) {
     constexpr int tile = 8, overlap = 8, most = 4;
     auto upTo8 = [](int n) { return (n + 7) / 8 * 8; };
     if (tile + overlap >= width || tile + overlap >= height) {
