@@ -159,3 +159,30 @@ def test_ownership_follows_the_lanes():
     after = own.owners(16)
     print(f"GPU-owned cells by frame: {flipped}; after: {int((after == G.GPU).sum())}")
     assert flipped[0] == 8 and int((after == G.GPU).sum()) > 8
+
+
+@pytest.mark.parametrize("window_ms,first,workers,threads", [(1.0, 16, 4, 3), (20.0, 4096, 64, 6), (0.5, 1, 1, 2)])
+def test_co_rendered_share_with_a_viewport(window_ms, first, workers, threads):
+    """Co-rendering a viewport's share of the samples ([3, 19) of 24) under other broker settings (launch windows, first
+    launches, GPU workers, CPU threads): every pixel in the viewport gets its 16 samples exactly once between the lanes,
+    none outside, and the two films together equal the GPU's own frame of that share."""
+    scene, config = scenes.cornell(width=200, height=144, spp=24)
+    vp = [13, 9, 181, 130]
+    config.viewport[:] = vp
+    cpu = SlowCpuLane(0.0003)
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        alone = G.WorkGrid(config.width, config.height, config.spp, G.options(32, 4, 1, True, 3, 16), viewport=vp)
+        ws, wc, _ = lane.render_grid(config, alone)
+        g = G.WorkGrid(config.width, config.height, config.spp, G.options(32, 4, 2, True, 3, 16), viewport=vp)
+        gs, gc, fr = lane.render_grid(config, g, cpu_render=cpu, cpu_threads=threads, window_ms=window_ms,
+                                      first_chunks=first, gpu_workers=workers)
+        cs, cc = render_chunks(lane, config, cpu.chunks, 4)
+    finally:
+        lane.close()
+    y, x = np.divmod(np.arange(config.width * config.height), config.width)
+    inside = (x >= vp[0]) & (x < vp[2]) & (y >= vp[1]) & (y < vp[3])
+    assert fr.errors == 0 and fr.gpu_chunks + fr.cpu_chunks == g.total_chunks()
+    assert ((gc + cc)[inside] == 16).all() and ((gc + cc)[~inside] == 0).all()
+    assert (wc == gc + cc).all() and close(gs + cs, ws)

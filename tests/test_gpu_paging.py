@@ -113,6 +113,26 @@ def test_paged_frames_through_the_grid_and_stages(frames, tmp_path):
     assert (rc == wc).all() and close(rs, ws) and st.parked_outstanding == 0
 
 
+def test_paged_adaptive_frame_equals_resident():
+    """An adaptive frame (host trees between passes) with paged textures: each pass's parked samples run again before
+    its errors are taken, so every pass samples the resident frame's pixels and the films agree."""
+    scene, resident, config = textured(96, 32, 256)
+    out = []
+    for sc, paged in ((resident, False), (scene, True)):
+        lane = GpuLane(0)
+        try:
+            lane.prepare(sc)
+            if paged:
+                lane.page_textures(64)
+            sums, odd, counts, st, a = lane.render_adaptive(config, 4, 32, 1e-3)
+            out.append((sums, odd, counts, a.passes))
+        finally:
+            lane.close()
+    (rs, ro, rc, rp), (ps, po, pc, pp) = out
+    assert rp == pp and (pc == rc).all() and rc.min() < rc.max()
+    assert close(ps, rs) and close(po, ro)
+
+
 def test_a_pool_too_small_for_a_path_fails_the_frame(frames):
     """A pool far smaller than what the frame's paths read: samples park again and again, and the frame fails with the
     reason rather than running on (the reference's frame fails so too, async_render.h:816-818)."""
