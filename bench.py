@@ -6,8 +6,9 @@ one uniform frame through the broker counterpart (x2dev_render_frame): 132,710,4
 in flight (up to LAUNCH_CHUNKS chunks a launch: fewer, larger launches leave fewer idle lanes at launch ends), the
 film downloaded and every pixel's count audited. The scene is resident in HBM before timing starts.
 
-Prints one JSON line: Mpaths/s (value), Mrays/s, the roofline of the render kernel (algorithmic bytes: 256 B per
-BVH8 node fetched + 48 B per triangle record tested + 72 B of film per path, over the kernel's own HIP-event time)
+Prints one JSON line: Mpaths/s (value), Mrays/s, the roofline of the render kernel (algorithmic bytes: 128 B per
+BVH8 node fetched (one cache line) + 48 B per triangle record tested + 72 B of film per path, over the kernel's own
+HIP-event time)
 and the CPU baseline (the oracle on this host's cores over a bounded sample of the same workload). The adjoint
 leg times one forward frame and one path-replay adjoint frame of the differentiable Cornell box (512x512, 32 spp):
 adjoint_step_over_forward is the adjoint frame's time over the forward frame's. The volume leg renders the
@@ -481,7 +482,9 @@ def main():
 
     paths = config.width * config.height * config.spp * args.steps
     rays = totals["traces"] + totals["shadows"]
-    bytes_total = 256 * totals["nodes"] + 48 * totals["tris"] + 72 * paths
+    from x2amd import abi as _abi
+    bytes_total = _abi.NODE_BYTES * totals["nodes"] + _abi.TRIANGLE_BYTES * totals["tris"] + \
+        _abi.FILM_BYTES_PER_PATH * paths
     per_launch_s = totals["kernel_s"] / max(1, totals["launches"])
     measured = traffic_per_path()
     traffic = None  # HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)

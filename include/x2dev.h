@@ -258,8 +258,13 @@ typedef struct x2_rng_query {
 int x2dev_rng_twin(x2dev* d, const x2_rng_query* queries, uint32_t n, uint32_t draws, uint32_t* words,
                    float* stream_out, uint64_t* branches);
 
-/* Algorithmic byte sizes the roofline counts: one BVH8 node and one triangle record. */
+/* Algorithmic byte sizes the roofline counts: one BVH8 node (one 128-B cache line: 8-bit child bounds above the
+ * node's origin; 256 B of float bounds in a build with X2_NODE128=0) and one triangle record. */
+#if defined(X2_NODE128) && !X2_NODE128
 #define X2_NODE_BYTES 256u
+#else
+#define X2_NODE_BYTES 128u
+#endif
 #define X2_TRIANGLE_BYTES 48u
 #define X2_FILM_BYTES_PER_PATH 72u
 

@@ -213,7 +213,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     j.itemsOneSample = tiles * 64;
     j.paths = j.pixels * c->spp;
     j.triangles = s->triangle_count;
-    const size_t nodeBytes = bvh.nodes.size() * sizeof(Node8), triBytes = std::max<size_t>(1, bvh.tris.size()) * sizeof(TriRecord);
+    const size_t nodeBytes = bvh.deviceNodeBytes(), triBytes = std::max<size_t>(1, bvh.tris.size()) * sizeof(TriRecord);
     auto cleanup = [&](hipError_t e, const char* where) {
         for (void* p : j.buffers) (void)hipFree(p);
         return hipFail(e, where);
@@ -224,7 +224,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
                              size_t(j.pixels) * sizeof(uint32_t)};
     for (int i = 0; i < 7 && e == hipSuccess; ++i)
         if (sizes[i]) e = hipMalloc(&j.buffers[i], sizes[i]);
-    if (e == hipSuccess) e = hipMemcpy(j.buffers[0], bvh.nodes.data(), nodeBytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(j.buffers[0], bvh.deviceNodes(), nodeBytes, hipMemcpyHostToDevice);
     if (e == hipSuccess && !bvh.tris.empty())
         e = hipMemcpy(j.buffers[1], bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -234,7 +234,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     if (e != hipSuccess) return cleanup(e, "x2batch_add_job upload");
     RenderParams& p = j.params;
     p.cfg = *c;
-    p.nodes = static_cast<const Node8*>(j.buffers[0]);
+    p.nodes = static_cast<const DevNode*>(j.buffers[0]);
     p.tris = static_cast<const TriRecord*>(j.buffers[1]);
     p.materials = static_cast<const x2_material*>(j.buffers[2]);
     p.lights = static_cast<const x2_light*>(j.buffers[3]);

@@ -157,6 +157,62 @@ struct Builder {
 }  // namespace
 
 // Builds the BVH8 and the leaf-ordered triangle records. Returns false with err set on bad input.
+// A node's children in one line: per axis the origin is the node's lowest child bound and the step the least power of
+// two whose 255 steps reach its highest; each bound is the nearest step outward, checked (and moved outward until it
+// holds) in the device's own float arithmetic, origin + q * step, with each operation rounded on its own.
+static float decodeStep(float origin, uint32_t q, float step) {
+    const float m = float(q) * step;  // exact: q < 256 and step a power of two
+    return origin + m;
+}
+static void quantizeNodes(const std::vector<Node8>& in, std::vector<Node8Q>& out) {
+    out.assign(in.size(), Node8Q{});
+    for (size_t n = 0; n < in.size(); ++n) {
+        const Node8& w = in[n];
+        Node8Q& o = out[n];
+        const float* lo[3] = {w.lox, w.loy, w.loz};
+        const float* hi[3] = {w.hix, w.hiy, w.hiz};
+        float origin[3];
+        uint8_t ebits[3];
+        float step[3];
+        for (int a = 0; a < 3; ++a) {
+            float l = INFINITY, h = -INFINITY;
+            for (int i = 0; i < 8; ++i)
+                if (w.child[i] != EmptyChild) {
+                    l = std::min(l, lo[a][i]);
+                    h = std::max(h, hi[a][i]);
+                }
+            if (!(l <= h)) l = h = 0.0f;  // no child: any bounds
+            origin[a] = l;
+            int e = -126;  // the least normal step
+            const double extent = double(h) - double(l);
+            if (extent > 0) e = std::max(-126, int(std::ceil(std::log2(extent / 255.0))));
+            while (e < 127 && decodeStep(l, 255u, std::ldexp(1.0f, e)) < h) ++e;
+            step[a] = std::ldexp(1.0f, e);
+            ebits[a] = uint8_t(e + 127);
+        }
+        o.ox = origin[0], o.oy = origin[1], o.oz = origin[2];
+        o.ex = ebits[0], o.ey = ebits[1], o.ez = ebits[2];
+        o.pad0 = 0;
+        for (int i = 0; i < 8; ++i) {
+            o.child[i] = w.child[i];
+            o.pad[i] = 0;
+            for (int a = 0; a < 3; ++a) {
+                uint32_t ql = 0, qh = 255;
+                if (w.child[i] != EmptyChild) {
+                    const double fl = std::floor((double(lo[a][i]) - double(origin[a])) / double(step[a]));
+                    const double fh = std::ceil((double(hi[a][i]) - double(origin[a])) / double(step[a]));
+                    ql = uint32_t(std::min(255.0, std::max(0.0, fl)));
+                    qh = uint32_t(std::min(255.0, std::max(0.0, fh)));
+                    while (ql > 0 && decodeStep(origin[a], ql, step[a]) > lo[a][i]) --ql;
+                    while (qh < 255 && decodeStep(origin[a], qh, step[a]) < hi[a][i]) ++qh;
+                }
+                o.q[2 * a][i] = uint8_t(ql);
+                o.q[2 * a + 1][i] = uint8_t(qh);
+            }
+        }
+    }
+}
+
 bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
     const uint32_t n = s.triangle_count;
     if (n >= (1u << 28)) {
@@ -292,6 +348,7 @@ bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
             }
         }
     }
+    if (X2_NODE128) quantizeNodes(out.nodes, out.qnodes);
     return true;
 }
 

@@ -258,6 +258,38 @@ __device__ __forceinline__ void nodeChildren(const Node8& n, const BoxRay& r, fl
     }
 }
 
+// The same test over a node in one line (Node8Q): six 16-B loads instead of fourteen; each child's bound decoded as
+// origin + q * step (two roundings, as the builder checked it), then the same slab test.
+__device__ __forceinline__ void nodeChildren(const Node8Q& n, const BoxRay& r, float lo, float hi,
+                                             unsigned long long key[8]) {
+    const uint4* q = reinterpret_cast<const uint4*>(&n);
+    const uint4 h = q[0], b0 = q[1], b1 = q[2], b2 = q[3], c0 = q[4], c1 = q[5];
+    const float ox = __uint_as_float(h.x), oy = __uint_as_float(h.y), oz = __uint_as_float(h.z);
+    const float sx = __uint_as_float((h.w & 0xffu) << 23), sy = __uint_as_float(((h.w >> 8) & 0xffu) << 23),
+                sz = __uint_as_float(((h.w >> 16) & 0xffu) << 23);
+    const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    auto at = [](uint32_t a, uint32_t b, int i) {  // byte i of the eight in (a, b)
+        return float(((i < 4 ? a : b) >> (8 * (i & 3))) & 0xffu);
+    };
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float lx = ox + at(b0.x, b0.y, i) * sx, hx = ox + at(b0.z, b0.w, i) * sx;
+        const float ly = oy + at(b1.x, b1.y, i) * sy, hy = oy + at(b1.z, b1.w, i) * sy;
+        const float lz = oz + at(b2.x, b2.y, i) * sz, hz = oz + at(b2.z, b2.w, i) * sz;
+        const float nx = __builtin_fmaf(r.sx ? lx : hx, r.ix, r.ox);
+        const float fx = __builtin_fmaf(r.sx ? hx : lx, r.ix, r.ox);
+        const float ny = __builtin_fmaf(r.sy ? ly : hy, r.iy, r.oy);
+        const float fy = __builtin_fmaf(r.sy ? hy : ly, r.iy, r.oy);
+        const float nz = __builtin_fmaf(r.sz ? lz : hz, r.iz, r.oz);
+        const float fz = __builtin_fmaf(r.sz ? hz : lz, r.iz, r.oz);
+        const float tn = fmaxf(fmaxf(nx, ny), nz);
+        float tf = fminf(fminf(fx, fy), fz);
+        tf = tf * (tf > 0.0f ? FarScale : 1.0f / FarScale);
+        const bool hit = ch[i] != EmptyChild && tn <= tf && tn <= hi && tf >= lo;
+        key[i] = hit ? ((unsigned long long)__float_as_uint(fmaxf(tn, 0.0f)) << 32) | ch[i] : ~0ull;
+    }
+}
+
 __device__ __forceinline__ bool keyLess(float t, uint32_t g, uint32_t p, float bt, uint32_t bg, uint32_t bp) {
     return t < bt || (t == bt && (g < bg || (g == bg && p < bp)));
 }

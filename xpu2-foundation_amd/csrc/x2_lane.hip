@@ -130,7 +130,7 @@ struct x2dev {
     hipStream_t stream = nullptr;
     uint32_t cus = 0, launchThreads = 0;
     // scene
-    Node8* nodes = nullptr;
+    DevNode* nodes = nullptr;
     TriRecord* tris = nullptr;
     x2_material* materials = nullptr;
     x2_light* lights = nullptr;
@@ -392,12 +392,12 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     d->materials = nullptr;
     d->lights = nullptr;
     d->prepared = false;
-    const size_t nb = bvh.nodes.size() * sizeof(Node8), tb = std::max<size_t>(1, bvh.tris.size()) * sizeof(TriRecord);
+    const size_t nb = bvh.deviceNodeBytes(), tb = std::max<size_t>(1, bvh.tris.size()) * sizeof(TriRecord);
     X2_HIP(hipMalloc(&d->nodes, nb));
     X2_HIP(hipMalloc(&d->tris, tb));
     X2_HIP(hipMalloc(&d->materials, std::max<size_t>(1, s->material_count) * sizeof(x2_material)));
     X2_HIP(hipMalloc(&d->lights, std::max<size_t>(1, s->light_count) * sizeof(x2_light)));
-    X2_HIP(hipMemcpy(d->nodes, bvh.nodes.data(), nb, hipMemcpyHostToDevice));
+    X2_HIP(hipMemcpy(d->nodes, bvh.deviceNodes(), nb, hipMemcpyHostToDevice));
     if (!bvh.tris.empty()) X2_HIP(hipMemcpy(d->tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice));
     X2_HIP(hipMemcpy(d->materials, s->materials, s->material_count * sizeof(x2_material), hipMemcpyHostToDevice));
     if (s->light_count) X2_HIP(hipMemcpy(d->lights, s->lights, s->light_count * sizeof(x2_light), hipMemcpyHostToDevice));

@@ -25,7 +25,7 @@ X2_EXR_ZIP = 3
 X2_DEVICE_CPU = 0
 X2_DEVICE_GPU = 1
 X2_NO_CELL = 0xFFFFFFFF
-NODE_BYTES = 256
+NODE_BYTES = 128  # X2_NODE_BYTES: a BVH8 node in one cache line
 TRIANGLE_BYTES = 48
 FILM_BYTES_PER_PATH = 72
 
