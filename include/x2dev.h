@@ -342,7 +342,7 @@ int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* count
 /* ---- On-demand texture tiles and partial shading (GpuLane's fine-detail cache, async_gpu.h:2911-3200, and its
  * parked and provisional samples, async_gpu.cu:274-295,350-357; texture_core.h's tiles). After prepare, every texture's
  * level 0 is kept by the host as 32x32 tiles of half-float rgb (a float texture's levels are held in half floats,
- * texture_core.h:30-31) and the device holds pool_tiles of them at a time (a full pool gives up its oldest load), plus
+ * texture_core.h:30-31) and the device holds pool_tiles of them at a time (a full pool gives up its least recently read tile), plus
  * each texture's coarse level (level 0's halves averaged over boxes, at most coarse_side texels a side; 0: 64),
  * always resident. A sample whose texture lookup finds a tile missing asks for every tile of its footprint and is
  * parked: it ends, and runs again in a later launch (ahead of its chunks, or in a launch of parked samples alone,

@@ -134,13 +134,14 @@ def test_paged_adaptive_frame_equals_resident():
 
 
 def test_a_pool_too_small_for_a_path_fails_the_frame(frames):
-    """A pool far smaller than what the frame's paths read: samples park again and again, and the frame fails with the
-    reason rather than running on (the reference's frame fails so too, async_render.h:816-818)."""
+    """A pool smaller than what a path reads at once (two tiles; a bilinear footprint alone can need four): its samples
+    park again and again, and the frame fails with the reason rather than running on (the reference's frame fails so
+    too, async_render.h:816-818)."""
     scene, config, _, _ = frames
     lane = GpuLane(0)
     try:
         lane.prepare(scene)
-        lane.page_textures(8)
+        lane.page_textures(2)
         with pytest.raises(RuntimeError, match="cannot hold"):
             lane.render_frame(config)
     finally:

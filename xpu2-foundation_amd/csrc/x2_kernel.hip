@@ -762,6 +762,7 @@ __device__ __forceinline__ bool pagedTexel(const RenderParams& P, const TexDesc&
         }
         return false;
     }
+    if (P.tileStamp[tile] != P.stamp) P.tileStamp[tile] = P.stamp;  // read this launch: recently used
     const uint16_t* h = P.tilePool + (size_t(slot) * TexTileTexels + (y & (TexTile - 1)) * TexTile + (x & (TexTile - 1))) * 3;
     out = V3{__half2float(__ushort_as_half(h[0])), __half2float(__ushort_as_half(h[1])),
              __half2float(__ushort_as_half(h[2]))};

@@ -93,12 +93,17 @@ struct Paging {
     std::vector<uint16_t> hostTiles;  // tile t's texels at t * TexTileTexels * 3
     std::vector<uint32_t> slotOf;     // host copy of the page table
     std::vector<uint32_t> tileIn;     // each pool slot's tile (NoTileSlot: free)
-    std::deque<uint32_t> loadOrder;   // pool slots, oldest load first (the next evicted)
+    std::vector<uint64_t> loadSeq;    // each pool slot's load, in order (ties of last use go to the oldest)
+    std::vector<uint32_t> lastUse;    // each tile's last launch to read it, as read back
+    uint32_t used = 0;                // pool slots filled so far
+    uint64_t loads = 0;
     std::vector<uint8_t> queued;      // asked for and not yet sent up
     std::vector<uint32_t> pending;
     std::vector<float*> coarse;       // each texture's coarse level (device)
     uint32_t* tileSlot = nullptr;
     uint32_t* tileAsked = nullptr;
+    uint32_t* tileStamp = nullptr;
+    uint32_t* stampReadback[Slots] = {};  // the tiles' stamps after each launch (pinned)
     uint32_t* missList = nullptr;
     uint16_t* pool = nullptr;
     PagingState* state = nullptr;
@@ -193,6 +198,7 @@ void releasePaging(x2dev* d) {
     Paging& g = d->pg;
     (void)hipFree(g.tileSlot);
     (void)hipFree(g.tileAsked);
+    (void)hipFree(g.tileStamp);
     (void)hipFree(g.missList);
     (void)hipFree(g.pool);
     (void)hipFree(g.state);
@@ -205,6 +211,7 @@ void releasePaging(x2dev* d) {
     if (g.readback) (void)hipHostFree(g.readback);
     for (uint32_t i = 0; i < Slots; ++i) {
         if (g.missReadback[i]) (void)hipHostFree(g.missReadback[i]);
+        if (g.stampReadback[i]) (void)hipHostFree(g.stampReadback[i]);
         if (g.upload[i]) (void)hipHostFree(g.upload[i]);
         if (g.tableStaging[i]) (void)hipHostFree(g.tableStaging[i]);
     }
@@ -637,6 +644,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
         p.parkCap = uint32_t(std::min<uint64_t>(g.parkCap, 0xffffffffu));
         p.missCap = g.missCap;
         p.partial = g.partial ? 1u : 0u;
+        p.tileStamp = g.tileStamp;
     }
     p.vol = d->volDesc;
     p.odd = d->adaptive ? d->odd : nullptr;
@@ -706,20 +714,31 @@ int commitTiles(x2dev* d, uint32_t slot) {
     if (g.pending.empty()) return X2_OK;
     const size_t tileBytes = size_t(TexTileTexels) * 3 * sizeof(uint16_t);
     const size_t n = std::min<size_t>({g.pending.size(), size_t(g.uploadCap), size_t(g.poolTiles)});
+    // The slots the commit fills: free ones first, then those of the least recently read tiles (the oldest load
+    // among equals) — a parked sample's tiles are read again each time it runs, so they stay, and the tiles of
+    // finished samples go first.
+    std::vector<uint32_t> slots;
+    const uint32_t resident = g.used;  // the slots holding a tile before this commit: the only candidates to give up
+    while (slots.size() < n && g.used < g.poolTiles) slots.push_back(g.used++);
+    if (slots.size() < n) {
+        std::vector<uint32_t> order(resident);
+        for (uint32_t i = 0; i < resident; ++i) order[i] = i;
+        const size_t k = n - slots.size();
+        auto older = [&](uint32_t a, uint32_t b) {
+            const uint32_t ua = g.lastUse[g.tileIn[a]], ub = g.lastUse[g.tileIn[b]];
+            return ua != ub ? ua < ub : g.loadSeq[a] < g.loadSeq[b];
+        };
+        std::nth_element(order.begin(), order.begin() + std::ptrdiff_t(k - 1), order.end(), older);
+        slots.insert(slots.end(), order.begin(), order.begin() + std::ptrdiff_t(k));
+    }
     for (size_t i = 0; i < n; ++i) {
         const uint32_t tile = g.pending[i];
-        uint32_t at;
-        if (g.loadOrder.size() < g.poolTiles) {
-            at = uint32_t(g.loadOrder.size());
-        } else {
-            at = g.loadOrder.front();
-            g.loadOrder.pop_front();
-            if (g.tileIn[at] != NoTileSlot) {
-                g.slotOf[g.tileIn[at]] = NoTileSlot;
-                ++g.evictions;
-            }
+        const uint32_t at = slots[i];
+        if (g.tileIn[at] != NoTileSlot) {
+            g.slotOf[g.tileIn[at]] = NoTileSlot;
+            ++g.evictions;
         }
-        g.loadOrder.push_back(at);
+        g.loadSeq[at] = ++g.loads;
         g.tileIn[at] = tile;
         g.slotOf[tile] = at;
         g.queued[tile] = 0;
@@ -803,10 +822,14 @@ int x2dev_page_textures(x2dev* d, const x2_paging* o) {
     }
     g.slotOf.assign(g.tiles, NoTileSlot);
     g.tileIn.assign(g.poolTiles, NoTileSlot);
+    g.loadSeq.assign(g.poolTiles, 0);
+    g.lastUse.assign(g.tiles, 0);
     g.queued.assign(g.tiles, 0);
     const size_t tileBytes = size_t(TexTileTexels) * 3 * sizeof(uint16_t);
     X2_HIP(hipMalloc(&g.tileSlot, size_t(g.tiles) * sizeof(uint32_t)));
     X2_HIP(hipMalloc(&g.tileAsked, size_t(g.tiles) * sizeof(uint32_t)));
+    X2_HIP(hipMalloc(&g.tileStamp, size_t(g.tiles) * sizeof(uint32_t)));
+    X2_HIP(hipMemset(g.tileStamp, 0, size_t(g.tiles) * sizeof(uint32_t)));
     X2_HIP(hipMalloc(&g.missList, size_t(g.missCap) * sizeof(uint32_t)));
     X2_HIP(hipMalloc(&g.pool, size_t(g.poolTiles) * tileBytes));
     X2_HIP(hipMalloc(&g.state, sizeof(PagingState)));
@@ -816,6 +839,7 @@ int x2dev_page_textures(x2dev* d, const x2_paging* o) {
     X2_HIP(hipHostMalloc(&g.readback, Slots * sizeof(PagingState), hipHostMallocDefault));
     for (uint32_t i = 0; i < Slots; ++i) {
         X2_HIP(hipHostMalloc(&g.missReadback[i], size_t(g.missCap) * sizeof(uint32_t), hipHostMallocDefault));
+        X2_HIP(hipHostMalloc(&g.stampReadback[i], size_t(g.tiles) * sizeof(uint32_t), hipHostMallocDefault));
         X2_HIP(hipHostMalloc(&g.upload[i], size_t(g.uploadCap) * tileBytes, hipHostMallocDefault));
         X2_HIP(hipHostMalloc(&g.tableStaging[i], size_t(g.tiles) * sizeof(uint32_t), hipHostMallocDefault));
     }
@@ -905,6 +929,7 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     p.itemEnd = (d->published + count) * d->itemsPerChunk;
     p.parkedBegin = parkedBegin;
     p.parkedCount = uint32_t(parkedCount);
+    p.stamp = uint32_t(d->launches + 1);
     if (wave) {
         const uint32_t perVertex = d->cfg.light_count * std::max(1u, d->cfg.light_samples) +
                                    2u * std::max(1u, d->cfg.bsdf_samples);
@@ -931,6 +956,8 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
         Paging& g = d->pg;
         X2_HIP(hipMemcpyAsync(g.readback + slot, g.state, sizeof(PagingState), hipMemcpyDeviceToHost, d->stream));
         X2_HIP(hipMemcpyAsync(g.missReadback[slot], g.missList, size_t(g.missCap) * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, d->stream));
+        X2_HIP(hipMemcpyAsync(g.stampReadback[slot], g.tileStamp, size_t(g.tiles) * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, d->stream));
         X2_HIP(hipMemsetAsync(reinterpret_cast<char*>(g.state) + offsetof(PagingState, missCount), 0,
                               sizeof(PagingState) - offsetof(PagingState, missCount), d->stream));
@@ -968,6 +995,7 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
         g.provisional += rb.provisional;
         g.overflow += rb.parkOverflow;
         lost = rb.parkOverflow;
+        for (uint32_t t = 0; t < g.tiles; ++t) g.lastUse[t] = std::max(g.lastUse[t], g.stampReadback[f.slot][t]);
         const uint32_t n = std::min(rb.missCount, g.missCap);
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t tile = g.missReadback[f.slot][i];

@@ -163,6 +163,8 @@ struct RenderParams {
     uint32_t* provCounts;
     unsigned long long parkedBegin;
     uint32_t parkedCount, parkCap, missCap, partial;
+    uint32_t* tileStamp;  // each tile's last launch to read it (the pool evicts the least recently used)
+    uint32_t stamp, stampPad;
 };
 constexpr uint32_t PassSlots = 4, PassShift = 16, MaxPasses = 160;  // async_integrator.h:3083
 
