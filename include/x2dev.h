@@ -365,7 +365,9 @@ typedef struct x2_paging_stats {
 int x2dev_page_textures(x2dev* d, const x2_paging* paging);
 int x2dev_paging_stats(const x2dev* d, x2_paging_stats* out);
 uint64_t x2dev_parked_outstanding(const x2dev* d);
-/* The provisional film (partial shading; downloaded with the film): each provisional sample's rgb + alpha and count. */
+/* The provisional film (partial shading; downloaded with the film): the samples parked after a coarse render, each
+ * one's latest rgb + alpha and a count of them. A sample run again in full leaves it (async_gpu.cu:282-295), so after
+ * each launch it counts exactly the parked samples outstanding and a finished frame's is empty. */
 int x2dev_provisional_film(const x2dev* d, const double** sums, const uint32_t** counts);
 
 /* ---- Batched jobs. No counterpart in the reference, which renders one frame at a time and decided against CUDA

@@ -2138,9 +2138,11 @@ __device__ __forceinline__ unsigned long long chainItems(const RenderParams& P) 
     if (!P.chain || P.chain->stop) return 0;
     return (unsigned long long)P.chain->activeTiles * P.chainSize * TilePixels;
 }
+// With Pg, *from receives the parked slot a sample came from (~0u: a chunk's item).
 template <bool Pg = false>
 __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel, uint32_t& index,
-                                          unsigned long long chainEnd = 0) {
+                                          unsigned long long chainEnd = 0, uint32_t* from = nullptr) {
+    if constexpr (Pg) *from = ~0u;
     for (;;) {
         if (P.chain) {  // the device pass chain's render: its active tiles' pixels, chainSize samples each
             const unsigned long long perTile = (unsigned long long)P.chainSize * TilePixels;
@@ -2157,9 +2159,11 @@ __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel
         unsigned long long k = atomicAdd(&P.state->claimed, 1ull);
         if constexpr (Pg) {
             if (k < P.parkedCount) {  // the parked samples this launch runs again, ahead of its chunks
-                const x2_parked r = P.parked[(P.parkedBegin + k) % P.parkCap];
+                const uint32_t at = uint32_t((P.parkedBegin + k) % P.parkCap);
+                const x2_parked r = P.parked[at];
                 pixel = r.pixel;
                 index = r.index;
+                *from = at;
                 return true;
             }
             k -= P.parkedCount;
@@ -2170,14 +2174,20 @@ __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel
     }
 }
 // A parked sample onto the ring (async_gpu.cu:350-357): the host sizes launches so the ring never overwrites a record
-// not yet run again; past that bound a sample is counted as lost (a failure of the frame), never written.
-__device__ __forceinline__ void parkSample(const RenderParams& P, uint32_t pixel, uint32_t index) {
+// not yet run again; past that bound a sample is counted as lost (a failure of the frame), never written. A
+// provisional sample's record keeps its value (v: rgb + alpha) beside it, for the provisional film to give back.
+// The launch's own parked records [parkedBegin, parkedBegin + parkedCount) are below the bound, so none is
+// overwritten while the launch reads it.
+__device__ __forceinline__ void parkSample(const RenderParams& P, uint32_t pixel, uint32_t index,
+                                           const float* v = nullptr) {
     const unsigned long long at = atomicAdd(&P.paging->parkTail, 1ull);
     if (at >= P.parkedBegin + P.parkCap) {
         atomicAdd(&P.paging->parkOverflow, 1u);
         return;
     }
-    P.parked[at % P.parkCap] = x2_parked{pixel, index, 0u, 0u};
+    const uint32_t slot = uint32_t(at % P.parkCap);
+    P.parked[slot] = x2_parked{pixel, index, 0u, 0u};
+    if (v) *reinterpret_cast<float4*>(P.parkedRgba + size_t(slot) * 4) = float4{v[0], v[1], v[2], v[3]};
 }
 template <bool Adj, bool Pg = false>
 struct LaneFeed {
@@ -2186,21 +2196,40 @@ struct LaneFeed {
     const RenderParams& P;
     uint32_t& errors;
     unsigned long long chainEnd = chainItems(P);  // read once: the compaction wrote it before this render
-    __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem<Pg>(P, pixel, index, chainEnd); }
+    uint32_t from = ~0u;  // the parked slot the lane's sample came from (Pg; ~0u: a chunk's item)
+    __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem<Pg>(P, pixel, index, chainEnd, &from); }
+    // A sample that read a coarse level (partial shading) goes into the provisional film, in place of its old value
+    // when it was itself a provisional sample run again, and parks; one run again in full takes its old value out
+    // (async_gpu.cu:282-295), so the provisional film holds exactly the samples still parked.
     __device__ void done(uint32_t pixel, uint32_t index, V3 r, float a, uint32_t flags) {
         if constexpr (!Adj) {
-            if (Pg && flags) {  // a parked or provisional sample (on-demand texture tiles) runs again
+            if constexpr (Pg) {
+                const bool old = P.partial && from != ~0u;
+                float4 was{0.0f, 0.0f, 0.0f, 0.0f};
+                if (old) was = *reinterpret_cast<const float4*>(P.parkedRgba + size_t(from) * 4);
+                double* f = P.provSums + size_t(pixel) * 4;
                 if (flags & SampleProvisional) {
-                    double* f = P.provSums + size_t(pixel) * 4;
-                    atomicAdd(f + 0, double(r.x));
-                    atomicAdd(f + 1, double(r.y));
-                    atomicAdd(f + 2, double(r.z));
-                    atomicAdd(f + 3, double(a));
-                    atomicAdd(P.provCounts + pixel, 1u);
+                    const float v[4] = {r.x, r.y, r.z, a};
+                    atomicAdd(f + 0, double(v[0]) - double(was.x));
+                    atomicAdd(f + 1, double(v[1]) - double(was.y));
+                    atomicAdd(f + 2, double(v[2]) - double(was.z));
+                    atomicAdd(f + 3, double(v[3]) - double(was.w));
+                    if (!old) atomicAdd(P.provCounts + pixel, 1u);
                     atomicAdd(&P.paging->provisional, 1u);
+                    parkSample(P, pixel, index, v);
+                    return;
                 }
-                parkSample(P, pixel, index);
-                return;
+                if (flags) {  // parked on a missing tile: it runs again once the tile is resident
+                    parkSample(P, pixel, index);
+                    return;
+                }
+                if (old) {
+                    atomicAdd(f + 0, -double(was.x));
+                    atomicAdd(f + 1, -double(was.y));
+                    atomicAdd(f + 2, -double(was.z));
+                    atomicAdd(f + 3, -double(was.w));
+                    atomicSub(P.provCounts + pixel, 1u);
+                }
             }
             filmAdd(P, pixel, index, r.x, r.y, r.z, a);
         }

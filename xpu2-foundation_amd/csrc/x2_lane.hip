@@ -108,6 +108,7 @@ struct Paging {
     uint16_t* pool = nullptr;
     PagingState* state = nullptr;
     x2_parked* parked = nullptr;
+    float* parkedRgba = nullptr;  // a provisional parked sample's value (rgb + alpha) by ring slot
     double* provSums = nullptr;
     uint32_t* provCounts = nullptr;
     double* hostProvSums = nullptr;
@@ -203,6 +204,7 @@ void releasePaging(x2dev* d) {
     (void)hipFree(g.pool);
     (void)hipFree(g.state);
     (void)hipFree(g.parked);
+    (void)hipFree(g.parkedRgba);
     (void)hipFree(g.provSums);
     (void)hipFree(g.provCounts);
     for (float* c : g.coarse) (void)hipFree(c);
@@ -510,9 +512,12 @@ int beginPaging(x2dev* d, size_t pixels) {
     const uint64_t cap = std::max<uint64_t>(uint64_t(3) * d->opt.max_chunks_per_launch * d->itemsPerChunk, 1u << 20);
     if (cap > g.parkCap) {
         (void)hipFree(g.parked);
+        (void)hipFree(g.parkedRgba);
         g.parked = nullptr;
+        g.parkedRgba = nullptr;
         g.parkCap = 0;
         X2_HIP(hipMalloc(&g.parked, cap * sizeof(x2_parked)));
+        X2_HIP(hipMalloc(&g.parkedRgba, cap * 4 * sizeof(float)));
         g.parkCap = cap;
     }
     if (pixels != g.provPixels) {
@@ -639,6 +644,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
         p.missList = g.missList;
         p.paging = g.state;
         p.parked = g.parked;
+        p.parkedRgba = g.parkedRgba;
         p.provSums = g.provSums;
         p.provCounts = g.provCounts;
         p.parkCap = uint32_t(std::min<uint64_t>(g.parkCap, 0xffffffffu));

@@ -152,13 +152,16 @@ struct RenderParams {
     // On-demand texture tiles: the scene-wide page table (a tile's pool slot, or NoTileSlot), the pool
     // (TexTileTexels * 3 halves a slot), each tile's asked-for flag and the launch's miss list; the parked ring
     // (parkCap records) whose [parkedBegin, parkedBegin + parkedCount) this launch runs before its chunks; with
-    // partial shading the provisional film. Null tileSlot: no texture pages.
+    // partial shading the provisional film, which holds the samples parked after a coarse render (each parked
+    // record's value, parkedRgba[slot * 4..], leaves it when the sample runs again in full). Null tileSlot: no
+    // texture pages.
     const uint32_t* tileSlot;
     const uint16_t* tilePool;
     uint32_t* tileAsked;
     uint32_t* missList;
     PagingState* paging;
     x2_parked* parked;
+    float* parkedRgba;
     double* provSums;
     uint32_t* provCounts;
     unsigned long long parkedBegin;
