@@ -167,18 +167,35 @@ __device__ __forceinline__ bool woopHit(const WoopRay& w, V3 a, V3 b, V3 c, floa
 constexpr float FarScale = 1.0f + 2.0f * (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
 constexpr float Slack = 1.0f + 0x1p-12f;
 
+#ifndef X2_STACK32
+#define X2_STACK32 1
+#endif
 // A lane's traversal stack: the first LdsDepth entries in LDS, lane-interleaved (entry e of lane l at
 // lds[e * 256 + l], bank-conflict free), deeper ones in private memory. Entries are (child ref, entry distance).
 struct TravStack {
     uint2* lds;
     uint32_t lane;
-    uint2 spill[TraversalStack - LdsDepth > 0 ? TraversalStack - LdsDepth : 1];
+    alignas(8) uint32_t spill[2 * (TraversalStack - LdsDepth > 0 ? TraversalStack - LdsDepth : 1)];  // words
     __device__ __forceinline__ void put(int i, uint2 e) {
-        if (LdsDepth > 0 && i < LdsDepth) lds[i * 256 + lane] = e;
-        else spill[i - LdsDepth] = e;
+        if (LdsDepth > 0 && i < LdsDepth) {
+            lds[i * 256 + lane] = e;
+        } else {
+            spill[2 * (i - LdsDepth)] = e.x;
+            spill[2 * (i - LdsDepth) + 1] = e.y;
+        }
     }
     __device__ __forceinline__ uint2 get(int i) const {
-        return (LdsDepth > 0 && i < LdsDepth) ? lds[i * 256 + lane] : spill[i - LdsDepth];
+        if (LdsDepth > 0 && i < LdsDepth) return lds[i * 256 + lane];
+        return make_uint2(spill[2 * (i - LdsDepth)], spill[2 * (i - LdsDepth) + 1]);
+    }
+    // Entries of one word (a child ref alone, the any-hit walk's), so a push stores 4 B.
+    __device__ __forceinline__ void put1(int i, uint32_t e) {
+        if constexpr (LdsDepth > 0 || !X2_STACK32) put(i, make_uint2(e, 0u));
+        else spill[i] = e;
+    }
+    __device__ __forceinline__ uint32_t get1(int i) const {
+        if constexpr (LdsDepth > 0 || !X2_STACK32) return get(i).x;
+        else return spill[i];
     }
 };
 
@@ -408,14 +425,15 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
     const BoxRay br = boxRay(org, dir);
     const float lo = nearT > 0 ? nearT / Slack : nearT * Slack, hi = farT > 0 ? farT * Slack : farT / Slack;
     int sp = 0;
-    auto pop = [&]() -> uint32_t { return sp > 0 ? st.get(--sp).x : Sentinel; };
-    auto visitNode = [&](uint32_t node) -> uint32_t {  // unsorted: sorting the keys measured slower here
+    auto pop = [&]() -> uint32_t { return sp > 0 ? st.get1(--sp) : Sentinel; };
+    // Unsorted: sorting the keys measured slower here, and so did taking the last child hit without its push and pop.
+    auto visitNode = [&](uint32_t node) -> uint32_t {
         ++k.nodes;
         unsigned long long key[8];
         nodeChildren(P.nodes[node], br, lo, hi, key);
 #pragma unroll
         for (int i = 0; i < 8; ++i)
-            if (key[i] != ~0ull) st.put(sp++, make_uint2(uint32_t(key[i]), 0u));
+            if (key[i] != ~0ull) st.put1(sp++, uint32_t(key[i]));
         return pop();
     };
     auto visitLeaf = [&](uint32_t leafRef) -> bool {
