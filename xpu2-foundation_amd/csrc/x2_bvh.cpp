@@ -49,7 +49,12 @@ struct BinNode {
 #ifndef X2_TRAVERSAL_COST
 #define X2_TRAVERSAL_COST 1.0f
 #endif
-constexpr int Bins = 32;
+#ifndef X2_BINS
+#define X2_BINS 128
+#endif
+// SAH bins a split tries. On 2.6 M triangles 128 renders 1.005x faster than 32 (16: 0.994x; 256 and 1024 no faster;
+// 128 only at nodes of 4,096 triangles or more 1.002x) for 0.8 s more build.
+constexpr int Bins = X2_BINS;
 constexpr uint32_t MinLeaf = X2_MIN_LEAF;  // a leaf at or below this many triangles; the SAH decides up to 8
 constexpr float TraversalCost = X2_TRAVERSAL_COST, IntersectCost = 1.0f;
 
@@ -69,6 +74,9 @@ struct Builder {
             uint32_t node, first, count;
         };
         std::vector<Task> tasks{{0, 0, uint32_t(order.size())}};
+        std::vector<Aabb> bin(Bins);
+        std::vector<uint32_t> cnt(Bins), rightCount(Bins);
+        std::vector<float> rightArea(Bins);
         while (!tasks.empty()) {
             const Task t = tasks.back();
             tasks.pop_back();
@@ -85,23 +93,24 @@ struct Builder {
             }
             int axis = -1, split = -1;
             float bestCost = FLT_MAX;
+            constexpr int nb = Bins;
             for (int k = 0; k < 3; ++k) {
                 const float extent = cb.hi[k] - cb.lo[k];
                 if (!(extent > 0.0f)) continue;
-                Aabb bin[Bins];
-                uint32_t cnt[Bins] = {};
-                const float scale = float(Bins) * (1.0f - 1e-6f) / extent;
+                for (int q = 0; q < nb; ++q) {
+                    bin[q] = Aabb{};
+                    cnt[q] = 0;
+                }
+                const float scale = float(nb) * (1.0f - 1e-6f) / extent;
                 for (uint32_t i = t.first; i < t.first + t.count; ++i) {
                     const uint32_t p = order[i];
-                    const int b = std::min(Bins - 1, int((centroid[size_t(p) * 3 + k] - cb.lo[k]) * scale));
+                    const int b = std::min(nb - 1, int((centroid[size_t(p) * 3 + k] - cb.lo[k]) * scale));
                     ++cnt[b];
                     bin[b].grow(boxes[p]);
                 }
-                float rightArea[Bins];
-                uint32_t rightCount[Bins];
                 Aabb acc;
                 uint32_t n = 0;
-                for (int b = Bins - 1; b > 0; --b) {
+                for (int b = nb - 1; b > 0; --b) {
                     acc.grow(bin[b]);
                     n += cnt[b];
                     rightArea[b] = acc.area();
@@ -109,7 +118,7 @@ struct Builder {
                 }
                 acc = Aabb{};
                 n = 0;
-                for (int b = 0; b < Bins - 1; ++b) {
+                for (int b = 0; b < nb - 1; ++b) {
                     acc.grow(bin[b]);
                     n += cnt[b];
                     if (!n || !rightCount[b + 1]) continue;
@@ -133,11 +142,11 @@ struct Builder {
                 mid = t.first + t.count / 2;  // coincident centroids: halve the list
             } else {
                 const float extent = cb.hi[axis] - cb.lo[axis];
-                const float scale = float(Bins) * (1.0f - 1e-6f) / extent;
+                const float scale = float(nb) * (1.0f - 1e-6f) / extent;
                 const float lo = cb.lo[axis];
                 auto* begin = order.data() + t.first;
                 auto* it = std::partition(begin, begin + t.count, [&](uint32_t p) {
-                    return std::min(Bins - 1, int((centroid[size_t(p) * 3 + axis] - lo) * scale)) < split;
+                    return std::min(nb - 1, int((centroid[size_t(p) * 3 + axis] - lo) * scale)) < split;
                 });
                 mid = t.first + uint32_t(it - begin);
                 if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
