@@ -7,7 +7,7 @@ in flight (up to LAUNCH_CHUNKS chunks a launch: fewer, larger launches leave few
 film downloaded and every pixel's count audited. The scene is resident in HBM before timing starts.
 
 Prints one JSON line: Mpaths/s (value), Mrays/s, the roofline of the render kernel (algorithmic bytes: 128 B per
-BVH8 node fetched (one cache line) + 48 B per triangle record tested + 72 B of film per path, over the kernel's own
+BVH8 node fetched (one cache line) + 36 B per triangle tested (its vertices) + 72 B of film per path, over the kernel's own
 HIP-event time)
 and the CPU baseline (the oracle on this host's cores over a bounded sample of the same workload). The adjoint
 leg times one forward frame and one path-replay adjoint frame of the differentiable Cornell box (512x512, 32 spp):
@@ -483,7 +483,7 @@ def main():
     paths = config.width * config.height * config.spp * args.steps
     rays = totals["traces"] + totals["shadows"]
     from x2amd import abi as _abi
-    bytes_total = _abi.NODE_BYTES * totals["nodes"] + _abi.TRIANGLE_BYTES * totals["tris"] + \
+    bytes_total = _abi.NODE_BYTES * totals["nodes"] + _abi.LEAF_TRIANGLE_BYTES * totals["tris"] + \
         _abi.FILM_BYTES_PER_PATH * paths
     per_launch_s = totals["kernel_s"] / max(1, totals["launches"])
     measured = traffic_per_path()

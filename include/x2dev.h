@@ -259,13 +259,19 @@ int x2dev_rng_twin(x2dev* d, const x2_rng_query* queries, uint32_t n, uint32_t d
                    float* stream_out, uint64_t* branches);
 
 /* Algorithmic byte sizes the roofline counts: one BVH8 node (one 128-B cache line: 8-bit child bounds above the
- * node's origin; 256 B of float bounds in a build with X2_NODE128=0) and one triangle record. */
+ * node's origin; 256 B of float bounds in a build with X2_NODE128=0) and a triangle a leaf test reads (its three
+ * vertices from the packed array, 36 B; the whole 48-B record in a build with X2_TRI36=0). */
 #if defined(X2_NODE128) && !X2_NODE128
 #define X2_NODE_BYTES 256u
 #else
 #define X2_NODE_BYTES 128u
 #endif
 #define X2_TRIANGLE_BYTES 48u
+#if defined(X2_TRI36) && !X2_TRI36
+#define X2_LEAF_TRIANGLE_BYTES 48u
+#else
+#define X2_LEAF_TRIANGLE_BYTES 36u
+#endif
 #define X2_FILM_BYTES_PER_PATH 72u
 
 /* The broker counterpart (async_render.h:318 Renderer::render, GPU lane only): one uniform frame, chunks made as

@@ -213,7 +213,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     j.itemsOneSample = tiles * 64;
     j.paths = j.pixels * c->spp;
     j.triangles = s->triangle_count;
-    const size_t nodeBytes = bvh.deviceNodeBytes(), triBytes = std::max<size_t>(1, bvh.tris.size()) * sizeof(TriRecord);
+    const size_t nodeBytes = bvh.deviceNodeBytes(), triBytes = std::max<size_t>(1, triBufferBytes(bvh.tris.size()));
     auto cleanup = [&](hipError_t e, const char* where) {
         for (void* p : j.buffers) (void)hipFree(p);
         return hipFail(e, where);
@@ -227,6 +227,11 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     if (e == hipSuccess) e = hipMemcpy(j.buffers[0], bvh.deviceNodes(), nodeBytes, hipMemcpyHostToDevice);
     if (e == hipSuccess && !bvh.tris.empty())
         e = hipMemcpy(j.buffers[1], bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice);
+    if (e == hipSuccess && X2_TRI36 && !bvh.tris.empty()) {
+        const std::vector<float> v = triVertices(bvh.tris);
+        e = hipMemcpy(static_cast<char*>(j.buffers[1]) + triVertsOffset(bvh.tris.size()), v.data(), v.size() * sizeof(float),
+                      hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = hipMemcpy(j.buffers[2], s->materials, s->material_count * sizeof(x2_material), hipMemcpyHostToDevice);
     if (e == hipSuccess && s->light_count)
@@ -236,6 +241,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     p.cfg = *c;
     p.nodes = static_cast<const DevNode*>(j.buffers[0]);
     p.tris = static_cast<const TriRecord*>(j.buffers[1]);
+    p.triVerts = reinterpret_cast<const float*>(static_cast<const char*>(j.buffers[1]) + triVertsOffset(bvh.tris.size()));
     p.materials = static_cast<const x2_material*>(j.buffers[2]);
     p.lights = static_cast<const x2_light*>(j.buffers[3]);
     p.sums = static_cast<double*>(j.buffers[5]);

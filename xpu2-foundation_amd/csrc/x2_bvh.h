@@ -16,4 +16,11 @@ struct Bvh8Result {
     uint32_t maxStack = 0, depth = 0;  // traversal stack entries a ray can need; wide-tree depth
 };
 bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err);
+// The records' vertices packed 9 floats a triangle (the leaf tests' array, X2_TRI36).
+inline std::vector<float> triVertices(const std::vector<TriRecord>& tris) {
+    std::vector<float> v;
+    v.reserve(tris.size() * 9);
+    for (const TriRecord& t : tris) v.insert(v.end(), {t.ax, t.ay, t.az, t.bx, t.by, t.bz, t.cx, t.cy, t.cz});
+    return v;
+}
 }  // namespace x2

@@ -337,7 +337,10 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
     const WoopRay w = woopSetup(org, dir, nearT, farT);
     const BoxRay br = boxRay(org, dir);
     const float lo = nearT > 0 ? nearT / Slack : nearT * Slack;
-    uint32_t best = ~0u, bg = ~0u, bp = ~0u;
+    uint32_t best = ~0u;
+#if !X2_TRI36
+    uint32_t bg = ~0u, bp = ~0u;
+#endif
     float bestT = farT;
     int sp = 0;
     // The nearest stacked entry still worth visiting, or Sentinel.
@@ -367,10 +370,21 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
     auto visitLeaf = [&](uint32_t leafRef) {
         const uint32_t first = (leafRef & 0x7fffffffu) >> 3, count = (leafRef & 7u) + 1;
         for (uint32_t i = 0; i < count; ++i) {
-            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
-            const float4 a = t4[0], b = t4[1], c = t4[2];
             ++k.triangles;
             float t;
+#if X2_TRI36
+            const float* v = P.triVerts + size_t(first + i) * 9;
+            if (!woopHit(w, V3{v[0], v[1], v[2]}, V3{v[3], v[4], v[5]}, V3{v[6], v[7], v[8]}, t)) continue;
+            if (best == ~0u || t < bestT) {
+                best = first + i;
+                bestT = t;
+            } else if (t == bestT) {  // a tie in distance: the least (geom, prim) wins, from the records
+                const TriRecord &r = P.tris[first + i], &q = P.tris[best];
+                if (r.geom < q.geom || (r.geom == q.geom && r.prim < q.prim)) best = first + i;
+            }
+#else
+            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
+            const float4 a = t4[0], b = t4[1], c = t4[2];
             if (!woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) continue;
             const uint32_t g = __float_as_uint(a.w), p = __float_as_uint(b.w);
             if (best == ~0u || keyLess(t, g, p, bestT, bg, bp)) {
@@ -379,6 +393,7 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
                 bg = g;
                 bp = p;
             }
+#endif
         }
     };
 #if X2_WHILE_WHILE
@@ -439,11 +454,16 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
     auto visitLeaf = [&](uint32_t leafRef) -> bool {
         const uint32_t first = (leafRef & 0x7fffffffu) >> 3, count = (leafRef & 7u) + 1;
         for (uint32_t i = 0; i < count; ++i) {
-            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
-            const float4 a = t4[0], b = t4[1], c = t4[2];
             ++k.triangles;
             float t;
+#if X2_TRI36
+            const float* v = P.triVerts + size_t(first + i) * 9;
+            if (woopHit(w, V3{v[0], v[1], v[2]}, V3{v[3], v[4], v[5]}, V3{v[6], v[7], v[8]}, t)) return true;
+#else
+            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
+            const float4 a = t4[0], b = t4[1], c = t4[2];
             if (woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) return true;
+#endif
         }
         return false;
     };

@@ -401,13 +401,18 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     d->materials = nullptr;
     d->lights = nullptr;
     d->prepared = false;
-    const size_t nb = bvh.deviceNodeBytes(), tb = std::max<size_t>(1, bvh.tris.size()) * sizeof(TriRecord);
+    const size_t nb = bvh.deviceNodeBytes(), tb = std::max<size_t>(1, triBufferBytes(bvh.tris.size()));
     X2_HIP(hipMalloc(&d->nodes, nb));
     X2_HIP(hipMalloc(&d->tris, tb));
     X2_HIP(hipMalloc(&d->materials, std::max<size_t>(1, s->material_count) * sizeof(x2_material)));
     X2_HIP(hipMalloc(&d->lights, std::max<size_t>(1, s->light_count) * sizeof(x2_light)));
     X2_HIP(hipMemcpy(d->nodes, bvh.deviceNodes(), nb, hipMemcpyHostToDevice));
     if (!bvh.tris.empty()) X2_HIP(hipMemcpy(d->tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice));
+    if (X2_TRI36 && !bvh.tris.empty()) {
+        const std::vector<float> v = triVertices(bvh.tris);
+        X2_HIP(hipMemcpy(reinterpret_cast<char*>(d->tris) + triVertsOffset(bvh.tris.size()), v.data(),
+                         v.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     X2_HIP(hipMemcpy(d->materials, s->materials, s->material_count * sizeof(x2_material), hipMemcpyHostToDevice));
     if (s->light_count) X2_HIP(hipMemcpy(d->lights, s->lights, s->light_count * sizeof(x2_light), hipMemcpyHostToDevice));
     if (!bvh.triUV.empty()) {
@@ -621,6 +626,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
     p.cfg = d->cfg;
     p.nodes = d->nodes;
     p.tris = d->tris;
+    p.triVerts = reinterpret_cast<const float*>(reinterpret_cast<const char*>(d->tris) + triVertsOffset(d->triCount));
     p.triUV = d->triUV;
     p.textures = d->textures;
     p.materials = d->materials;

@@ -6,6 +6,7 @@
 //   triangle      48 B: three float4, the vertices in xyz and (geom, prim, material) as the w words, in leaf order.
 //   film          FP64 sums[pixel][4] (rgb, alpha) + u32 counts[pixel] (async_gpu_abi.h:182-183 sums/counts).
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/x2dev.h"
@@ -64,6 +65,15 @@ struct alignas(X2_TRI64 ? 64 : 16) TriRecord {
     uint32_t material;
 };
 static_assert(sizeof(TriRecord) == (X2_TRI64 ? 64 : X2_TRIANGLE_BYTES), "triangle record is 48 B");
+#ifndef X2_TRI36
+#define X2_TRI36 1
+#endif
+// The leaf tests read a triangle's vertices from a packed array of 36 B a triangle placed after the records in the
+// same allocation (at a 256-B boundary); the ids are read from the record only for an exact tie in distance.
+inline size_t triVertsOffset(size_t triangles) { return (triangles * sizeof(TriRecord) + 255) / 256 * 256; }
+inline size_t triBufferBytes(size_t triangles) {
+    return X2_TRI36 ? triVertsOffset(triangles) + triangles * 9 * sizeof(float) : triangles * sizeof(TriRecord);
+}
 
 // Per-launch state: the claim counter and the totals the kernel adds up (one atomic per wave).
 struct alignas(64) LaunchState {
@@ -120,6 +130,7 @@ struct RenderParams {
     x2_config cfg;
     const DevNode* nodes;
     const TriRecord* tris;
+    const float* triVerts;  // the records' vertices alone, 9 floats a triangle (X2_TRI36), for the leaf tests
     const float* triUV;  // 6 floats per triangle record (leaf order), or null: (0,0), (1,0), (1,1)
     const TexDesc* textures;
     const x2_material* materials;

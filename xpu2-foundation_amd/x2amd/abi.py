@@ -26,7 +26,8 @@ X2_DEVICE_CPU = 0
 X2_DEVICE_GPU = 1
 X2_NO_CELL = 0xFFFFFFFF
 NODE_BYTES = 128  # X2_NODE_BYTES: a BVH8 node in one cache line
-TRIANGLE_BYTES = 48
+TRIANGLE_BYTES = 48  # X2_TRIANGLE_BYTES: a triangle record
+LEAF_TRIANGLE_BYTES = 36  # X2_LEAF_TRIANGLE_BYTES: the vertices a leaf test reads
 FILM_BYTES_PER_PATH = 72
 
 u32 = ctypes.c_uint32
