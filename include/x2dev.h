@@ -164,8 +164,8 @@ typedef struct x2_completion {
     double device_seconds;     /* the launch's own duration, from HIP events around it */
     uint64_t rays_trace;       /* closest-hit rays traced */
     uint64_t rays_shadow;      /* any-hit shadow rays traced */
-    uint64_t nodes_visited;    /* BVH8 nodes fetched (256 B each) */
-    uint64_t triangles_tested; /* triangle records fetched (48 B each) */
+    uint64_t nodes_visited;    /* BVH8 nodes fetched (X2_NODE_BYTES = 128 B each) */
+    uint64_t triangles_tested; /* triangles tested (their packed vertices, X2_LEAF_TRIANGLE_BYTES = 36 B each) */
 } x2_completion;
 
 /* Totals over a frame's launches (x2dev_render_frame, x2dev_adjoint_frame). */
@@ -259,19 +259,11 @@ int x2dev_rng_twin(x2dev* d, const x2_rng_query* queries, uint32_t n, uint32_t d
                    float* stream_out, uint64_t* branches);
 
 /* Algorithmic byte sizes the roofline counts: one BVH8 node (one 128-B cache line: 8-bit child bounds above the
- * node's origin; 256 B of float bounds in a build with X2_NODE128=0) and a triangle a leaf test reads (its three
- * vertices from the packed array, 36 B; the whole 48-B record in a build with X2_TRI36=0). */
-#if defined(X2_NODE128) && !X2_NODE128
-#define X2_NODE_BYTES 256u
-#else
+ * node's origin), a triangle a leaf test reads (its three vertices from the packed array, 36 B) and the whole record
+ * (48 B, read once a hit). */
 #define X2_NODE_BYTES 128u
-#endif
 #define X2_TRIANGLE_BYTES 48u
-#if defined(X2_TRI36) && !X2_TRI36
-#define X2_LEAF_TRIANGLE_BYTES 48u
-#else
 #define X2_LEAF_TRIANGLE_BYTES 36u
-#endif
 #define X2_FILM_BYTES_PER_PATH 72u
 
 /* The broker counterpart (async_render.h:318 Renderer::render, GPU lane only): one uniform frame, chunks made as

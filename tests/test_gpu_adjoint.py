@@ -192,3 +192,25 @@ def test_adjoint_argument_errors(small):
         lane.adjoint_frame(deep, g.data_ptr(), gt.data_ptr(), gr.data_ptr())
     with pytest.raises(RuntimeError):
         lane.set_texture(3, params["texture"])  # no such texture
+
+
+def test_diff_renderer_rejects_wrong_shapes(small):
+    """The texture is copied from the tensor's memory at texture 0's size and the texel gradients are written at the
+    scene's sizes, so a tensor of another shape (or an image gradient of another size) is refused before the device
+    reads or writes it."""
+    from x2amd.diff import DiffRenderer
+    scene, config, params, _ = small
+    r = DiffRenderer(scene, config, params["roughness_materials"])
+    try:
+        tex = torch.tensor(params["texture"], device=DEV)
+        rough = torch.tensor(params["roughness"], device=DEV)
+        with pytest.raises(ValueError):
+            r.forward_image(tex[: tex.shape[0] // 2], rough)
+        with pytest.raises(ValueError):
+            r.forward_image(tex, rough[:-1])
+        with pytest.raises(ValueError):
+            r.adjoint(tex, rough, torch.zeros(config.height // 2, config.width, 3, device=DEV))
+        gt, gr = r.adjoint(tex, rough, torch.zeros(config.height, config.width, 3, device=DEV))
+        assert tuple(gt.shape) == tuple(params["texture"].shape) and gr.shape[0] == len(params["roughness"])
+    finally:
+        r.close()

@@ -3,6 +3,12 @@
 // global illumination"), as the reference restates them in async_adaptive.h / async_adaptive_core.h. The node
 // arithmetic is x2_adaptive_core.h's, which the device's pass chain shares. Built without floating-point
 // contraction: each product and sum rounds on its own, as the reference's mulRn/addRn do.
+//
+// The schedule, the region layout and the tree (Tree, Regions) and cellular (Cellular) rules are adapted from MoonRay
+// rndr/adaptive/AdaptiveRegionTree.{h,cc}, OverlappingRegions.h, ActivePixelMask.h and RenderDriver.cc, by way of the
+// reference's restatement (xpu2/interactive/async_adaptive.h): Apache License 2.0, copyright 2023-2024 DreamWorks
+// Animation LLC (http://www.apache.org/licenses/LICENSE-2.0). Changes: restated in this build's own structure and
+// names; the rules and arithmetic are kept so that the same errors give the same masks.
 #include "x2_adaptive.h"
 
 #include <math.h>

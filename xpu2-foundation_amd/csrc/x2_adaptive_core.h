@@ -2,6 +2,11 @@
 // and the device's pass chain (x2_kernel.hip chainTreesKernel), so both build the same trees, and so the same tile
 // masks, from the same errors (async_adaptive_core.h:29-164, async_integrator.h:3109-3125). Every product and sum
 // rounds on its own (the build never contracts).
+//
+// Adapted from MoonRay rndr/adaptive/AdaptiveRegionTree.{h,cc} and ActivePixelMask.h, by way of the reference's
+// restatement (xpu2/interactive/async_adaptive_core.h): Apache License 2.0, copyright 2023-2024 DreamWorks Animation
+// LLC (http://www.apache.org/licenses/LICENSE-2.0). Changes: restated for HIP host and device code with this
+// build's names; the node arithmetic is unchanged so that the same errors give the same trees.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>

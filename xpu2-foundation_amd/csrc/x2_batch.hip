@@ -227,7 +227,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     if (e == hipSuccess) e = hipMemcpy(j.buffers[0], bvh.deviceNodes(), nodeBytes, hipMemcpyHostToDevice);
     if (e == hipSuccess && !bvh.tris.empty())
         e = hipMemcpy(j.buffers[1], bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice);
-    if (e == hipSuccess && X2_TRI36 && !bvh.tris.empty()) {
+    if (e == hipSuccess && !bvh.tris.empty()) {
         const std::vector<float> v = triVertices(bvh.tris);
         e = hipMemcpy(static_cast<char*>(j.buffers[1]) + triVertsOffset(bvh.tris.size()), v.data(), v.size() * sizeof(float),
                       hipMemcpyHostToDevice);

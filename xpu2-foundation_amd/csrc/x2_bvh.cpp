@@ -357,7 +357,7 @@ bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
             }
         }
     }
-    if (X2_NODE128) quantizeNodes(out.nodes, out.qnodes);
+    quantizeNodes(out.nodes, out.qnodes);
     return true;
 }
 

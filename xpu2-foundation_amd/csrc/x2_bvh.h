@@ -8,15 +8,15 @@
 namespace x2 {
 struct Bvh8Result {
     std::vector<Node8> nodes;
-    std::vector<Node8Q> qnodes;  // the same nodes in one line each (X2_NODE128)
-    const void* deviceNodes() const { return X2_NODE128 ? static_cast<const void*>(qnodes.data()) : nodes.data(); }
+    std::vector<Node8Q> qnodes;  // the same nodes in one line each (the device's)
+    const void* deviceNodes() const { return qnodes.data(); }
     size_t deviceNodeBytes() const { return nodes.size() * sizeof(DevNode); }
     std::vector<TriRecord> tris;
     std::vector<float> triUV;  // 6 floats per triangle record when the scene has uvs
     uint32_t maxStack = 0, depth = 0;  // traversal stack entries a ray can need; wide-tree depth
 };
 bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err);
-// The records' vertices packed 9 floats a triangle (the leaf tests' array, X2_TRI36).
+// The records' vertices packed 9 floats a triangle (the leaf tests' array).
 inline std::vector<float> triVertices(const std::vector<TriRecord>& tris) {
     std::vector<float> v;
     v.reserve(tris.size() * 9);

@@ -167,36 +167,17 @@ __device__ __forceinline__ bool woopHit(const WoopRay& w, V3 a, V3 b, V3 c, floa
 constexpr float FarScale = 1.0f + 2.0f * (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
 constexpr float Slack = 1.0f + 0x1p-12f;
 
-#ifndef X2_STACK32
-#define X2_STACK32 1
-#endif
-// A lane's traversal stack: the first LdsDepth entries in LDS, lane-interleaved (entry e of lane l at
-// lds[e * 256 + l], bank-conflict free), deeper ones in private memory. Entries are (child ref, entry distance).
+// A lane's traversal stack in private memory: the closest-hit walk's entries are (child ref, entry distance) pairs,
+// the any-hit walk's a child ref alone (one word a push).
 struct TravStack {
-    uint2* lds;
-    uint32_t lane;
-    alignas(8) uint32_t spill[2 * (TraversalStack - LdsDepth > 0 ? TraversalStack - LdsDepth : 1)];  // words
+    alignas(8) uint32_t spill[2 * TraversalStack];  // words
     __device__ __forceinline__ void put(int i, uint2 e) {
-        if (LdsDepth > 0 && i < LdsDepth) {
-            lds[i * 256 + lane] = e;
-        } else {
-            spill[2 * (i - LdsDepth)] = e.x;
-            spill[2 * (i - LdsDepth) + 1] = e.y;
-        }
+        spill[2 * i] = e.x;
+        spill[2 * i + 1] = e.y;
     }
-    __device__ __forceinline__ uint2 get(int i) const {
-        if (LdsDepth > 0 && i < LdsDepth) return lds[i * 256 + lane];
-        return make_uint2(spill[2 * (i - LdsDepth)], spill[2 * (i - LdsDepth) + 1]);
-    }
-    // Entries of one word (a child ref alone, the any-hit walk's), so a push stores 4 B.
-    __device__ __forceinline__ void put1(int i, uint32_t e) {
-        if constexpr (LdsDepth > 0 || !X2_STACK32) put(i, make_uint2(e, 0u));
-        else spill[i] = e;
-    }
-    __device__ __forceinline__ uint32_t get1(int i) const {
-        if constexpr (LdsDepth > 0 || !X2_STACK32) return get(i).x;
-        else return spill[i];
-    }
+    __device__ __forceinline__ uint2 get(int i) const { return make_uint2(spill[2 * i], spill[2 * i + 1]); }
+    __device__ __forceinline__ void put1(int i, uint32_t e) { spill[i] = e; }
+    __device__ __forceinline__ uint32_t get1(int i) const { return spill[i]; }
 };
 
 struct Counters {
@@ -229,54 +210,9 @@ __device__ __forceinline__ void cswap(unsigned long long& a, unsigned long long&
     b = hi;
 }
 
-// Tests a node's 8 children; returns the entry distance (>= 0) packed over the child reference, ~0 for a miss.
-__device__ __forceinline__ void nodeChildren(const Node8& n, const BoxRay& r, float lo, float hi,
-                                             unsigned long long key[8]) {
-    const float4* q = reinterpret_cast<const float4*>(&n);
-    float lox[8], hix[8], loy[8], hiy[8], loz[8], hiz[8];
-    uint32_t ch[8];
-    {
-        float4 v;
-#define X2_LOAD8(dst, off)            \
-    v = q[off];                       \
-    dst[0] = v.x;                     \
-    dst[1] = v.y;                     \
-    dst[2] = v.z;                     \
-    dst[3] = v.w;                     \
-    v = q[off + 1];                   \
-    dst[4] = v.x;                     \
-    dst[5] = v.y;                     \
-    dst[6] = v.z;                     \
-    dst[7] = v.w;
-        X2_LOAD8(lox, 0)
-        X2_LOAD8(hix, 2)
-        X2_LOAD8(loy, 4)
-        X2_LOAD8(hiy, 6)
-        X2_LOAD8(loz, 8)
-        X2_LOAD8(hiz, 10)
-#undef X2_LOAD8
-        const uint4* u = reinterpret_cast<const uint4*>(&n.child[0]);
-        const uint4 c0 = u[0], c1 = u[1];
-        ch[0] = c0.x, ch[1] = c0.y, ch[2] = c0.z, ch[3] = c0.w, ch[4] = c1.x, ch[5] = c1.y, ch[6] = c1.z, ch[7] = c1.w;
-    }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const float nx = __builtin_fmaf(r.sx ? lox[i] : hix[i], r.ix, r.ox);
-        const float fx = __builtin_fmaf(r.sx ? hix[i] : lox[i], r.ix, r.ox);
-        const float ny = __builtin_fmaf(r.sy ? loy[i] : hiy[i], r.iy, r.oy);
-        const float fy = __builtin_fmaf(r.sy ? hiy[i] : loy[i], r.iy, r.oy);
-        const float nz = __builtin_fmaf(r.sz ? loz[i] : hiz[i], r.iz, r.oz);
-        const float fz = __builtin_fmaf(r.sz ? hiz[i] : loz[i], r.iz, r.oz);
-        const float tn = fmaxf(fmaxf(nx, ny), nz);
-        float tf = fminf(fminf(fx, fy), fz);
-        tf = tf * (tf > 0.0f ? FarScale : 1.0f / FarScale);
-        const bool hit = ch[i] != EmptyChild && tn <= tf && tn <= hi && tf >= lo;
-        key[i] = hit ? ((unsigned long long)__float_as_uint(fmaxf(tn, 0.0f)) << 32) | ch[i] : ~0ull;
-    }
-}
-
-// The same test over a node in one line (Node8Q): six 16-B loads instead of fourteen; each child's bound decoded as
-// origin + q * step (two roundings, as the builder checked it), then the same slab test.
+// Tests a node's 8 children (a node in one 128-B line, Node8Q: six 16-B loads); returns the entry distance (>= 0)
+// packed over the child reference, ~0 for a miss. Each child's bound is decoded as origin + q * step (two roundings,
+// as the builder checked it), then the slab test.
 __device__ __forceinline__ void nodeChildren(const Node8Q& n, const BoxRay& r, float lo, float hi,
                                              unsigned long long key[8]) {
     const uint4* q = reinterpret_cast<const uint4*>(&n);
@@ -317,9 +253,6 @@ __device__ __forceinline__ bool keyLess(float t, uint32_t g, uint32_t p, float b
 // is the least key whatever the order, so the result is the same as a depth-first walk's.
 constexpr uint32_t Sentinel = 0x7fffffffu;  // an empty stack's pop: neither a node index nor a leaf
 
-#ifndef X2_WHILE_WHILE
-#define X2_WHILE_WHILE 1
-#endif
 
 __device__ __forceinline__ void sort8(unsigned long long key[8]) {  // 19-comparator network: nearest first
     cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[4], key[5]); cswap(key[6], key[7]);
@@ -338,9 +271,6 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
     const BoxRay br = boxRay(org, dir);
     const float lo = nearT > 0 ? nearT / Slack : nearT * Slack;
     uint32_t best = ~0u;
-#if !X2_TRI36
-    uint32_t bg = ~0u, bp = ~0u;
-#endif
     float bestT = farT;
     int sp = 0;
     // The nearest stacked entry still worth visiting, or Sentinel.
@@ -372,7 +302,6 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
         for (uint32_t i = 0; i < count; ++i) {
             ++k.triangles;
             float t;
-#if X2_TRI36
             const float* v = P.triVerts + size_t(first + i) * 9;
             if (!woopHit(w, V3{v[0], v[1], v[2]}, V3{v[3], v[4], v[5]}, V3{v[6], v[7], v[8]}, t)) continue;
             if (best == ~0u || t < bestT) {
@@ -382,21 +311,8 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
                 const TriRecord &r = P.tris[first + i], &q = P.tris[best];
                 if (r.geom < q.geom || (r.geom == q.geom && r.prim < q.prim)) best = first + i;
             }
-#else
-            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
-            const float4 a = t4[0], b = t4[1], c = t4[2];
-            if (!woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) continue;
-            const uint32_t g = __float_as_uint(a.w), p = __float_as_uint(b.w);
-            if (best == ~0u || keyLess(t, g, p, bestT, bg, bp)) {
-                best = first + i;
-                bestT = t;
-                bg = g;
-                bp = p;
-            }
-#endif
         }
     };
-#if X2_WHILE_WHILE
     uint32_t ref = 0, leaf = 0;  // leaf: a postponed leaf reference (0: none)
     for (;;) {
         while (ref < Sentinel) {  // an inner node
@@ -417,18 +333,6 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
         }
         if (ref == Sentinel) break;
     }
-#else
-    uint32_t ref = 0;
-    for (;;) {
-        if (ref & LeafBit) {
-            visitLeaf(ref);
-            ref = pop();
-        } else {
-            ref = visitNode(ref);
-        }
-        if (ref == Sentinel) break;
-    }
-#endif
     tOut = bestT;
     return best;
 }
@@ -456,18 +360,11 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
         for (uint32_t i = 0; i < count; ++i) {
             ++k.triangles;
             float t;
-#if X2_TRI36
             const float* v = P.triVerts + size_t(first + i) * 9;
             if (woopHit(w, V3{v[0], v[1], v[2]}, V3{v[3], v[4], v[5]}, V3{v[6], v[7], v[8]}, t)) return true;
-#else
-            const float4* t4 = reinterpret_cast<const float4*>(P.tris + first + i);
-            const float4 a = t4[0], b = t4[1], c = t4[2];
-            if (woopHit(w, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, t)) return true;
-#endif
         }
         return false;
     };
-#if X2_WHILE_WHILE
     uint32_t ref = 0, leaf = 0;
     for (;;) {
         while (ref < Sentinel) {
@@ -488,18 +385,6 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
         }
         if (ref == Sentinel) return false;
     }
-#else
-    uint32_t ref = 0;
-    for (;;) {
-        if (ref & LeafBit) {
-            if (visitLeaf(ref)) return true;
-            ref = pop();
-        } else {
-            ref = visitNode(ref);
-        }
-        if (ref == Sentinel) return false;
-    }
-#endif
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -991,9 +876,6 @@ __device__ __forceinline__ float voxelAt(const VolumeDesc& v, const uint16_t* po
     if (e == ~0u) return 0.0f;
     return halfValue(pool[size_t(e) * 512 + (uint32_t(k & 7) * 8 + uint32_t(j & 7)) * 8 + uint32_t(i & 7)]);
 }
-#ifndef X2_BRICK_FAST
-#define X2_BRICK_FAST 1
-#endif
 __device__ float sampleGrid(const VolumeDesc& v, const uint16_t* pool, V3 p) {
     float q[3];
     for (int a = 0; a < 3; ++a) {
@@ -1004,7 +886,6 @@ __device__ float sampleGrid(const VolumeDesc& v, const uint16_t* pool, V3 p) {
     const int i = int(fx0), j = int(fy0), k = int(fz0);
     const float fx = q[0] - fx0, fy = q[1] - fy0, fz = q[2] - fz0;
     float t[8];  // the 8 voxels, x fastest
-#if X2_BRICK_FAST
     if (i >= 0 && j >= 0 && k >= 0 && (i & 7) != 7 && (j & 7) != 7 && (k & 7) != 7) {
         // all 8 in one brick (i + 1 < res as i + 1 is inside the brick): one table lookup
         const uint32_t e = v.table[(uint32_t(k >> 3) * v.nb[1] + uint32_t(j >> 3)) * v.nb[0] + uint32_t(i >> 3)];
@@ -1012,9 +893,7 @@ __device__ float sampleGrid(const VolumeDesc& v, const uint16_t* pool, V3 p) {
         const uint16_t* b = pool + size_t(e) * 512 + (uint32_t(k & 7) * 8 + uint32_t(j & 7)) * 8 + uint32_t(i & 7);
         t[0] = halfValue(b[0]), t[1] = halfValue(b[1]), t[2] = halfValue(b[8]), t[3] = halfValue(b[9]);
         t[4] = halfValue(b[64]), t[5] = halfValue(b[65]), t[6] = halfValue(b[72]), t[7] = halfValue(b[73]);
-    } else
-#endif
-    {
+    } else {
         t[0] = voxelAt(v, pool, i, j, k), t[1] = voxelAt(v, pool, i + 1, j, k);
         t[2] = voxelAt(v, pool, i, j + 1, k), t[3] = voxelAt(v, pool, i + 1, j + 1, k);
         t[4] = voxelAt(v, pool, i, j, k + 1), t[5] = voxelAt(v, pool, i + 1, j, k + 1);
@@ -1163,11 +1042,8 @@ struct MarchCk {  // the march as a recorded step begins: its lattice index, tra
     float vt, partial;
     uint32_t count;
 };
-#ifndef X2_MARCH_SLOTS
-#define X2_MARCH_SLOTS 16  // 8, 4, 32 measured slower
-#endif
 struct MarchRecord {
-    static constexpr uint32_t Slots = X2_MARCH_SLOTS;
+    static constexpr uint32_t Slots = 16;  // 4, 8 and 32 measured slower
     MarchCk at[Slots];
     uint32_t used = 0, stride = 8, last = 0;
     __device__ bool wants(uint32_t count) const {
@@ -2094,17 +1970,7 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
 // Launch shape knobs (measured on MI355X on the 1080p 64-spp icospheres frame):
 // persistent lanes claiming samples one at a time, and an occupancy target of 3 waves per SIMD (168 VGPRs) were the
 // fastest of the variants tried; an LDS traversal stack was slower than private memory at that occupancy.
-#ifndef X2_PERSISTENT
-#define X2_PERSISTENT 1
-#endif
-#ifndef X2_WAVES_PER_EU
-#define X2_WAVES_PER_EU 3
-#endif
-#if X2_WAVES_PER_EU
-#define X2_OCCUPANCY __attribute__((amdgpu_waves_per_eu(X2_WAVES_PER_EU)))
-#else
-#define X2_OCCUPANCY
-#endif
+#define X2_OCCUPANCY __attribute__((amdgpu_waves_per_eu(3)))
 
 // One sample per item of the launch (async_gpu.cu:387-473 Feed phase 1): item = chunk * itemsPerChunk + s * 64 +
 // position; the chunk comes from the mailbox ring. The forward render adds the sample to the film; the adjoint
@@ -2167,9 +2033,6 @@ __device__ __forceinline__ void renderItem(const RenderParams& P, unsigned long 
 
 // A persistent lane's samples: it claims the launch's items one at a time (the reference's Feed loop), skipping
 // positions outside the image or the viewport, and adds each finished sample to the film (the adjoint adds none).
-#ifndef X2_REGEN_LANES
-#define X2_REGEN_LANES 32
-#endif
 // The launch's next sample: claims items until one lies in the image and the viewport (false: none left).
 // The device pass chain's render: its items, fixed by the compaction before the render (0: the chain has stopped).
 __device__ __forceinline__ unsigned long long chainItems(const RenderParams& P) {
@@ -2229,7 +2092,7 @@ __device__ __forceinline__ void parkSample(const RenderParams& P, uint32_t pixel
 }
 template <bool Adj, bool Pg = false>
 struct LaneFeed {
-    static constexpr uint32_t Regen = X2_REGEN_LANES;  // finished lanes of a wave that take new samples together
+    static constexpr uint32_t Regen = 32;  // finished lanes of a wave that take new samples together
     static constexpr bool Paged = Pg;                  // texture tiles on demand: parked and provisional samples
     const RenderParams& P;
     uint32_t& errors;
@@ -2276,20 +2139,16 @@ struct LaneFeed {
 };
 
 // A lane per item (the 64 lanes of a wave are one 8x8 tile at one sample index, so camera rays start coherent), or
-// persistent lanes claiming items one at a time (the reference's Feed loop) when built with X2_PERSISTENT: with
+// persistent lanes claiming items one at a time (the reference's Feed loop): with
 // Regen a lane claims its next item when its sample's last vertex is shaded (runSamples over LaneFeed), without it
 // when the whole sample is (renderItem). Every lane reaches the end so the wave's counter sums see all 64 lanes.
 template <bool Adj, bool Vol, bool Direct = false, bool Regen = false, bool Paged = false>
 __device__ __forceinline__ void runLaunch(const RenderParams& P) {
-    __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
     TravStack st;
-    st.lds = lds;
-    st.lane = threadIdx.x;
     Counters k;
     uint32_t errors = 0;
     float gr[Adj ? RoughnessSlots : 1];
     for (uint32_t i = 0; i < (Adj ? RoughnessSlots : 1); ++i) gr[i] = 0.0f;
-#if X2_PERSISTENT
     if constexpr (Regen) {
         LaneFeed<Adj, Paged> feed{P, errors};
         runSamples<Adj, Vol>(P, feed, k, st, gr);
@@ -2300,10 +2159,6 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
             renderItem<Adj, Vol, Direct>(P, item, k, st, errors, gr);
         }
     }
-#else
-    const unsigned long long item = P.itemBegin + (unsigned long long)blockIdx.x * 256u + threadIdx.x;
-    if (item < P.itemEnd) renderItem<Adj, Vol, Direct>(P, item, k, st, errors, gr);
-#endif
     if (errors) atomicAdd(&P.state->errors, errors);
     if constexpr (Adj) {
         for (uint32_t m = 0; m < RoughnessSlots; ++m) {  // one atomic per wave and material
@@ -2329,16 +2184,16 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
 // one is done). Measured against a lane per sample on the same frames: 1.01x on 32 icospheres at 1080p64, 1.55x
 // forward and 1.52x adjoint on the textured Cornell box (RR off, long uneven paths).
 __global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderParams P) {
-    runLaunch<false, false, false, X2_PERSISTENT>(P);
+    runLaunch<false, false, false, true>(P);
 }
 // The same render with texture tiles on demand: samples that meet a missing tile park (or, with partial shading,
 // finish provisionally) and the launch runs the parked samples of earlier launches ahead of its chunks. Its own
 // instantiation, so the render above carries none of it.
 __global__ void __launch_bounds__(256) X2_OCCUPANCY renderPagedKernel(const RenderParams P) {
-    runLaunch<false, false, false, X2_PERSISTENT, true>(P);
+    runLaunch<false, false, false, true, true>(P);
 }
 __global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) {
-    runLaunch<true, false, false, X2_PERSISTENT>(P);
+    runLaunch<true, false, false, true>(P);
 }
 // ---------------------------------------------------------------------------------------------------------------
 // The wavefront form of the surface estimator: the same samples, vertex for vertex, with the traversal taken out of
@@ -2455,10 +2310,7 @@ __device__ __forceinline__ void resolveShadows(const WaveParams& W, uint32_t slo
     }
 }
 
-#ifndef X2_SHADE_WAVES
-#define X2_SHADE_WAVES X2_WAVES_PER_EU
-#endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(X2_SHADE_WAVES)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 waveShadeKernel(const RenderParams P, const WaveParams W, uint32_t p) {
     const uint32_t slot = blockIdx.x * 256u + threadIdx.x;  // slots come in whole workgroups
     uint32_t errors = 0, queuedRay = 0, queuedShadows = 0;
@@ -2478,8 +2330,6 @@ waveShadeKernel(const RenderParams P, const WaveParams W, uint32_t p) {
                 fl &= ~WaveTraced;
                 Counters k;
                 TravStack st;
-                st.lds = nullptr;
-                st.lane = threadIdx.x;
                 WaveQueue queue{W, p, slot, 0};
                 if (!shadeTraced(P, W, slot, pixel, index, S, queue, k, st)) {
                     ++errors;
@@ -2548,15 +2398,9 @@ waveShadeKernel(const RenderParams P, const WaveParams W, uint32_t p) {
         W.shadowList[p][base[1] + waveShadows[wave] + atShadow + j] = slot * W.perVertex + j;
 }
 
-#ifndef X2_TRACE_WAVES
-#define X2_TRACE_WAVES 6
-#endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(X2_TRACE_WAVES)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 waveTraceKernel(const RenderParams P, const WaveParams W, uint32_t p) {
-    __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
     TravStack st;
-    st.lds = lds;
-    st.lane = threadIdx.x;
     Counters k;
     const uint32_t nr = W.counts->rays[p], ns = W.counts->shadows[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // the other parity's lists are free: the next shading pass fills them
@@ -2600,10 +2444,7 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY waveDrainKernel(const Render
                                                                      uint32_t p) {
     const uint32_t slot = blockIdx.x * 256u + threadIdx.x;  // every lane reaches the end (the counter sums)
     const uint32_t fl = slot < W.slots ? W.flags[slot] : 0u;
-    __shared__ uint2 lds[LdsDepth > 0 ? LdsDepth * 256 : 1];
     TravStack st;
-    st.lds = lds;
-    st.lane = threadIdx.x;
     Counters k;
     uint32_t errors = 0;
     if (fl) {
@@ -2864,19 +2705,9 @@ __global__ void waveEndKernel(const RenderParams P, const WaveParams W) {
 }
 
 // A scene with a medium: the same estimator with the medium's light and transmittance on every ray.
-#ifndef X2_VOLUME_WAVES
-#define X2_VOLUME_WAVES 2  // 256 VGPRs, no accumulation registers: 2 waves/SIMD (1 without the bound)
-#endif
-#if X2_VOLUME_WAVES
-#define X2_VOLUME_OCCUPANCY __attribute__((amdgpu_waves_per_eu(X2_VOLUME_WAVES)))
-#else
-#define X2_VOLUME_OCCUPANCY
-#endif
-#ifndef X2_VOLUME_REGEN
-#define X2_VOLUME_REGEN 0  // path regeneration here measured 0.98x on the clouds at 720p32: off
-#endif
-__global__ void __launch_bounds__(256) X2_VOLUME_OCCUPANCY renderVolumeKernel(const RenderParams P) {
-    runLaunch<false, true, false, X2_PERSISTENT && X2_VOLUME_REGEN>(P);
+// 256 VGPRs at 2 waves/SIMD; persistent lanes, one sample at a time (path regeneration measured 0.98x here).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) renderVolumeKernel(const RenderParams P) {
+    runLaunch<false, true, false, false>(P);
 }
 // The prototype estimator's kernel (x2_config.direct_only).
 __global__ void __launch_bounds__(256) directKernel(const RenderParams P) { runLaunch<false, false, true>(P); }
@@ -2891,8 +2722,6 @@ __global__ void __launch_bounds__(256) directKernel(const RenderParams P) { runL
 template <int Waves>
 __device__ __forceinline__ void runBatch(const BatchParams& B) {
     TravStack st;
-    st.lds = nullptr;
-    st.lane = threadIdx.x;
     Counters k;
     uint32_t errors = 0;
     float gr[1] = {0.0f};
@@ -3005,7 +2834,7 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
                         : Kind == 3 ? directKernel
                                     : renderPagedKernel;
     static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
-    if (X2_PERSISTENT && !resident) {
+    if (!resident) {
         int dev = 0, perCu = 0;
         hipDeviceProp_t prop;
         hipError_t e = hipGetDevice(&dev);
@@ -3016,11 +2845,10 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
         resident = std::max(1, perCu) * prop.multiProcessorCount;
     }
     // The device pass chain's render learns its items on the device: a full persistent grid.
-    if (p.chain && !X2_PERSISTENT) return hipErrorInvalidValue;
     // A launch's items: its chunks' and the parked samples it runs again (a launch may hold parked samples alone).
     const unsigned long long items = p.chain ? ~0ull >> 1 : p.itemEnd - p.itemBegin + p.parkedCount;
     unsigned long long blocks = (items + 255) / 256;
-    if (X2_PERSISTENT && blocks > (unsigned long long)resident) blocks = resident;
+    if (blocks > (unsigned long long)resident) blocks = resident;
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL(kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p);
     return hipGetLastError();

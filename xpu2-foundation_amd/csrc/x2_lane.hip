@@ -408,7 +408,7 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     X2_HIP(hipMalloc(&d->lights, std::max<size_t>(1, s->light_count) * sizeof(x2_light)));
     X2_HIP(hipMemcpy(d->nodes, bvh.deviceNodes(), nb, hipMemcpyHostToDevice));
     if (!bvh.tris.empty()) X2_HIP(hipMemcpy(d->tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice));
-    if (X2_TRI36 && !bvh.tris.empty()) {
+    if (!bvh.tris.empty()) {
         const std::vector<float> v = triVertices(bvh.tris);
         X2_HIP(hipMemcpy(reinterpret_cast<char*>(d->tris) + triVertsOffset(bvh.tris.size()), v.data(),
                          v.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -1226,9 +1226,6 @@ namespace {
 int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<uint32_t>* chunkSamples,
               x2_frame_stats& st);
 
-#ifndef X2_CHUNK_ORDER
-#define X2_CHUNK_ORDER 0
-#endif
 int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
     std::vector<x2_chunk> chunks;
     {
@@ -1245,17 +1242,10 @@ int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
         // over sweeping the whole image range by range on the headline frame).
         for (uint32_t cy = 0; cy < cellsY; ++cy)
             for (uint32_t cx = 0; cx < cellsX; ++cx) {
-#if X2_CHUNK_ORDER == 1  // tile by tile within the cell, each tile's ranges in turn (measured 1.002x: noise)
-                for (uint32_t ty = cy * 8; ty < std::min(tilesY, (cy + 1) * 8); ++ty)
-                    for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
-                        for (const auto& rg : ranges)
-                            chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
-#else
                 for (const auto& rg : ranges)
                     for (uint32_t ty = cy * 8; ty < std::min(tilesY, (cy + 1) * 8); ++ty)
                         for (uint32_t tx = cx * 8; tx < std::min(tilesX, (cx + 1) * 8); ++tx)
                             chunks.push_back({ty * tilesX + tx, rg.first, rg.second, cy * cellsX + cx});
-#endif
             }
     }
     return runChunks(d, chunks, nullptr, st);

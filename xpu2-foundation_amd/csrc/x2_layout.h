@@ -1,9 +1,11 @@
 // x2_layout.h — the device's data layout in HBM, shared by the host side (BVH8 builder, lane) and the kernels.
 //
-//   BVH8 node     256 B: the 8 children's AABBs as six float[8] planes (192 B), 8 child references (32 B), 32 B pad.
-//                 A reference is a node index (bit 31 clear), a leaf (bit 31 set: first triangle << 3 | count - 1,
-//                 1..8 triangles from first, first < 2^28), or EmptyChild with an inverted box that no ray enters.
-//   triangle      48 B: three float4, the vertices in xyz and (geom, prim, material) as the w words, in leaf order.
+//   BVH8 node     128 B (Node8Q, one cache line): the node's origin, a power-of-two step per axis, the 8 children's
+//                 bounds as 8-bit steps above the origin, 8 child references, 32 B pad. A reference is a node index
+//                 (bit 31 clear), a leaf (bit 31 set: first triangle << 3 | count - 1, 1..8 triangles from first,
+//                 first < 2^28), or EmptyChild, whose box no ray enters. The builder makes float nodes (Node8) first.
+//   triangle      48 B: three float4, the vertices in xyz and (geom, prim, material) as the w words, in leaf order;
+//                 the same vertices packed at 36 B a triangle after the records (the leaf tests' array).
 //   film          FP64 sums[pixel][4] (rgb, alpha) + u32 counts[pixel] (async_gpu_abi.h:182-183 sums/counts).
 #pragma once
 #include <stddef.h>
@@ -19,11 +21,8 @@ constexpr uint32_t MaxLeafTriangles = 8;
 constexpr uint32_t TilePixels = 64;
 constexpr uint32_t Slots = 4;  // launches whose per-launch counters may be live at once (async_gpu_abi.h:14)
 constexpr int TraversalStack = 128;  // per-lane traversal stack entries; prepare() checks the BVH fits
-#ifndef X2_LDS_DEPTH
-#define X2_LDS_DEPTH 0
-#endif
-constexpr int LdsDepth = X2_LDS_DEPTH;  // of which the first entries live in LDS (8 B x 256 lanes each)
 
+// The builder's node: float bounds, quantized to Node8Q for the device.
 struct alignas(256) Node8 {
     float lox[8], hix[8], loy[8], hiy[8], loz[8], hiz[8];
     uint32_t child[8];
@@ -31,7 +30,7 @@ struct alignas(256) Node8 {
 };
 static_assert(sizeof(Node8) == 256, "BVH8 node is 256 B");
 
-// The same node in one 128-B cache line (X2_NODE128): the children's bounds as 8-bit steps above the node's origin,
+// The device's node in one 128-B cache line: the children's bounds as 8-bit steps above the node's origin,
 // each axis's step a power of two (its float exponent byte), lows rounded down and highs up by the builder, which
 // checks each decoded bound in the device's arithmetic (origin + q * step, two roundings) so that every child's box
 // contains its true box; the traversal then finds the same closest hits (Ylitie, Karras and Laine 2017).
@@ -43,20 +42,10 @@ struct alignas(128) Node8Q {
     uint32_t pad[8];
 };
 static_assert(sizeof(Node8Q) == 128, "a quantized BVH8 node is one 128-B line");
-#ifndef X2_NODE128
-#define X2_NODE128 1  // measured 1.066x on the 1080p icospheres frame (287.5 against 306.5 ms of device time)
-#endif
-#if X2_NODE128
 using DevNode = Node8Q;
-#else
-using DevNode = Node8;
-#endif
 static_assert(sizeof(DevNode) == X2_NODE_BYTES, "the node the device reads");
 
-#ifndef X2_TRI64
-#define X2_TRI64 0  // 1: records padded to 64 B, none across a cache line (measured 0.989x on the 1080p headline)
-#endif
-struct alignas(X2_TRI64 ? 64 : 16) TriRecord {
+struct alignas(16) TriRecord {
     float ax, ay, az;
     uint32_t geom;
     float bx, by, bz;
@@ -64,15 +53,12 @@ struct alignas(X2_TRI64 ? 64 : 16) TriRecord {
     float cx, cy, cz;
     uint32_t material;
 };
-static_assert(sizeof(TriRecord) == (X2_TRI64 ? 64 : X2_TRIANGLE_BYTES), "triangle record is 48 B");
-#ifndef X2_TRI36
-#define X2_TRI36 1
-#endif
+static_assert(sizeof(TriRecord) == X2_TRIANGLE_BYTES, "triangle record is 48 B");
 // The leaf tests read a triangle's vertices from a packed array of 36 B a triangle placed after the records in the
 // same allocation (at a 256-B boundary); the ids are read from the record only for an exact tie in distance.
 inline size_t triVertsOffset(size_t triangles) { return (triangles * sizeof(TriRecord) + 255) / 256 * 256; }
 inline size_t triBufferBytes(size_t triangles) {
-    return X2_TRI36 ? triVertsOffset(triangles) + triangles * 9 * sizeof(float) : triangles * sizeof(TriRecord);
+    return triVertsOffset(triangles) + triangles * 9 * sizeof(float);
 }
 
 // Per-launch state: the claim counter and the totals the kernel adds up (one atomic per wave).
@@ -130,7 +116,7 @@ struct RenderParams {
     x2_config cfg;
     const DevNode* nodes;
     const TriRecord* tris;
-    const float* triVerts;  // the records' vertices alone, 9 floats a triangle (X2_TRI36), for the leaf tests
+    const float* triVerts;  // the records' vertices alone, 9 floats a triangle, for the leaf tests
     const float* triUV;  // 6 floats per triangle record (leaf order), or null: (0,0), (1,0), (1,1)
     const TexDesc* textures;
     const x2_material* materials;

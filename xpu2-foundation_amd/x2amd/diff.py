@@ -22,6 +22,15 @@ class DiffRenderer:
         self.samples_per_chunk = samples_per_chunk
         self.roughness_materials = list(roughness_materials)
         self.materials = [abi.Material.from_buffer_copy(scene.materials[m]) for m in range(scene.material_count)]
+        if not scene.texture_arrays:
+            raise ValueError("DiffRenderer needs a scene with a texture (texture 0 is the parameter)")
+        sizes = [(t.shape[0], t.shape[1]) for t in scene.texture_arrays]
+        self.texture_shape = (sizes[0][0], sizes[0][1], 3)
+        # the adjoint's texel gradients cover every texture of the scene, concatenated in order
+        self.grad_texels = sum(h * w * 3 for h, w in sizes)
+        for m in self.roughness_materials:
+            if not 0 <= m < scene.material_count:
+                raise ValueError(f"roughness material {m} is not a material of the scene")
         self.device = torch.device("cuda", device)
         self.lane = GpuLane(device)
         self.lane.prepare(scene)
@@ -31,9 +40,14 @@ class DiffRenderer:
         self.lane.close()
 
     def _bind(self, texture, roughness):
-        """Uploads the parameters the next frame (or adjoint) reads."""
+        """Uploads the parameters the next frame (or adjoint) reads. The device copies texture 0's texels from the
+        tensor's memory, so its shape must be texture 0's."""
+        if tuple(texture.shape) != self.texture_shape:
+            raise ValueError(f"texture of shape {tuple(texture.shape)}; texture 0 of the scene is {self.texture_shape}")
         tex = texture.detach().to(self.device, torch.float32).contiguous()
-        values = roughness.detach().cpu().tolist() if torch.is_tensor(roughness) else list(roughness)
+        values = roughness.detach().cpu().reshape(-1).tolist() if torch.is_tensor(roughness) else list(roughness)
+        if len(values) != len(self.roughness_materials):
+            raise ValueError(f"{len(values)} roughness values for {len(self.roughness_materials)} materials")
         torch.cuda.synchronize(self.device)
         self.lane.set_texture(0, tex.data_ptr(), on_device=True)
         for k, m in enumerate(self.roughness_materials):
@@ -53,14 +67,19 @@ class DiffRenderer:
 
     def adjoint(self, texture, roughness, grad_image):
         """dLoss/dtexture (H x W x 3) and dLoss/droughness (per listed material) for dLoss/dimage = grad_image."""
-        tex = self._bind(texture, roughness)
+        self._bind(texture, roughness)
+        image_shape = (self.config.height, self.config.width, 3)
+        if tuple(grad_image.shape) != image_shape:
+            raise ValueError(f"image gradient of shape {tuple(grad_image.shape)}; the image is {image_shape}")
         g = grad_image.detach().to(self.device, torch.float32).contiguous()
-        grad_tex = torch.zeros_like(tex)
+        # sized from the scene's textures (the kernel writes every texture's texel gradients), texture 0 returned
+        grad_all = torch.zeros(self.grad_texels, dtype=torch.float32, device=self.device)
         grad_rough = torch.zeros(self.scene.material_count, dtype=torch.float32, device=self.device)
         torch.cuda.synchronize(self.device)
-        self.last_stats = self.lane.adjoint_frame(self.config, g.data_ptr(), grad_tex.data_ptr(), grad_rough.data_ptr(),
+        self.last_stats = self.lane.adjoint_frame(self.config, g.data_ptr(), grad_all.data_ptr(), grad_rough.data_ptr(),
                                                   self.samples_per_chunk)
-        return grad_tex, grad_rough[self.roughness_materials]
+        n0 = self.texture_shape[0] * self.texture_shape[1] * 3
+        return grad_all[:n0].view(self.texture_shape), grad_rough[self.roughness_materials]
 
     def __call__(self, texture, roughness):
         return RenderFunction.apply(texture, roughness, self)
