@@ -46,6 +46,7 @@ def load():
         "oracle_closest": (None, [vp, P(f32), P(f32), f32, f32, c_int, P(u32), P(u32), P(f32)]),
         "oracle_occluded": (c_int, [vp, P(f32), P(f32), f32, f32, c_int]),
         "oracle_sample": (c_int, [vp, vp, u32, u32, P(f64), P(u64)]),
+        "oracle_first_hit_draws": (c_int, [vp, vp, u32, u32, P(f32), u32, P(u32)]),
         "oracle_render": (c_int, [vp, vp, u32, P(f64), P(u32), P(u64)]),
         "oracle_render_chunks": (c_int, [vp, vp, vp, u32, u32, P(f64), P(u32), P(u64)]),
         "oracle_render_pixels": (c_int, [vp, vp, P(u32), P(u32), u32, P(f64), P(f64), P(u32), P(u64)]),
@@ -148,6 +149,16 @@ class OracleScene:
         counters = (u64 * 2)()
         st = self.lib.oracle_sample(self.h, ctypes.addressof(config), pixel, index, rgba, counters)
         return st, tuple(rgba), tuple(counters)
+
+    def first_hit_draws(self, config, pixel, index, capacity=512):
+        """The stratified draws of the sample's first non-mirror vertex: rows (kind 0 light / 1 lobe, which, number,
+        r0, r1) in the order the estimator drew them."""
+        out = (f32 * (5 * capacity))()
+        n = u32()
+        st = self.lib.oracle_first_hit_draws(self.h, ctypes.addressof(config), pixel, index, out, capacity,
+                                             ctypes.byref(n))
+        assert n.value <= capacity
+        return st, np.array(out[: 5 * n.value], np.float64).reshape(-1, 5)
 
     def closest(self, o, d, near=0.0, far=3.4e38, brute=False):
         g, p, t = u32(), u32(), f32()

@@ -1533,6 +1533,13 @@ int integrateDirect(const Scene& S, const x2_config& c, U32 pixel, U32 index, do
     return 0;
 }
 
+// A test hook: while set (on the calling thread), the stratified draws of the sample's first non-mirror vertex are
+// recorded as (kind 0 light / 1 lobe, light or lobe index, sample number, r0, r1), in the order they are drawn.
+struct DrawRecord {
+    float kind, which, number, r0, r1;
+};
+thread_local std::vector<DrawRecord>* g_drawTrace = nullptr;
+
 int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double out[4], Counters& k,
               const Adjoint* adj = nullptr) {
     if (c.direct_only) return adj ? 2 : integrateDirect(S, c, pixel, index, out, k);
@@ -1687,6 +1694,7 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                     const Identity strat{pixel, U64(index) * nl + s, c.seed, lightAddress};
                     r[0] = draw(strat, CameraDimensions + 2, c.sampling);
                     r[1] = draw(strat, CameraDimensions + 3, c.sampling);
+                    if (g_drawTrace) g_drawTrace->push_back({0.0f, float(i), float(s), r[0], r[1]});
                 }
                 V3 wi;
                 LightHit lh;
@@ -1768,6 +1776,7 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                     const Identity strat{pixel, U64(index) * nk + i, c.seed, lobeBranch};
                     r1 = draw(strat, CameraDimensions, c.sampling);
                     r2 = draw(strat, CameraDimensions + 1, c.sampling);
+                    if (g_drawTrace) g_drawTrace->push_back({1.0f, float(q), float(i), r1, r2});
                 }
                 V3 wi;
                 float pdf;
@@ -2188,6 +2197,18 @@ int oracle_sample(const oracle_scene* o, const x2_config* c, uint32_t pixel, uin
     const int st = integrate(o->s, *c, pixel, index, rgba, k);
     counters[0] += k.traces;
     counters[1] += k.shadows;
+    return st;
+}
+int oracle_first_hit_draws(const oracle_scene* o, const x2_config* c, uint32_t pixel, uint32_t index, float* out,
+                           uint32_t capacity, uint32_t* count) {
+    std::vector<DrawRecord> trace;
+    g_drawTrace = &trace;
+    Counters k;
+    double rgba[4];
+    const int st = integrate(o->s, *c, pixel, index, rgba, k);
+    g_drawTrace = nullptr;
+    *count = U32(trace.size());
+    for (U32 i = 0; i < std::min<U32>(capacity, U32(trace.size())); ++i) std::memcpy(out + 5 * i, &trace[i], 5 * sizeof(float));
     return st;
 }
 int oracle_render(const oracle_scene* o, const x2_config* c, uint32_t threads, double* sums, uint32_t* counts,

@@ -63,6 +63,10 @@ int oracle_occluded(const oracle_scene* s, const float o[3], const float d[3], f
 
 /* One sample (async_integrator.h:1986 integratePaths). rgba receives radiance and alpha; counters[0] += traces,
  * counters[1] += shadow rays. Returns the SampleStatus (0 Done, 2 Error, 3 Skip). */
+/* Test hook: the stratified draws of sample (pixel, index)'s first non-mirror vertex, 5 floats a draw (kind: 0 light,
+ * 1 lobe; which; sample number; r0; r1) into out (capacity draws); *count receives how many were drawn. */
+int oracle_first_hit_draws(const oracle_scene* o, const x2_config* c, uint32_t pixel, uint32_t index, float* out,
+                           uint32_t capacity, uint32_t* count);
 int oracle_sample(const oracle_scene* s, const x2_config* c, uint32_t pixel, uint32_t index, double rgba[4],
                   uint64_t counters[2]);
 /* A whole uniform frame: every pixel's samples [0, spp) accumulated in sample-index order into FP64 sums (rgba) and
