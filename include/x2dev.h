@@ -564,6 +564,19 @@ int x2dev_render_staged(x2dev* d, const x2_config* config, uint32_t samples_per_
  * ring's free room). */
 uint32_t x2dev_chunk_room(const x2dev* d);
 
+/* The render that allocates nothing. No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). x2dev_launch's render
+ * of `count` chunks (a host array, samples_per_chunk sample indices a chunk at most) with the prepared scene and
+ * `config`, waited for before it returns, each sample's radiance added to the caller's device film (sums: width *
+ * height * 4 FP64, counts: width * height u32; the caller zeroes them for a new frame). The chunks and the launch's
+ * counters go to a static device array of the library's, 1 MiB + 256 B a GPU, held until the process ends, so at most
+ * X2_STATIC_MAX_CHUNKS chunks a call (more: X2_ERR_ARGUMENT); no hipMalloc, hipFree or allocating library call is
+ * made. Every call on one GPU shares that array: calls that overlap on one GPU race. Needs a prepared lane with no
+ * launch in flight and resident textures; out (may be null) receives the completion. */
+#define X2_STATIC_MAX_CHUNKS 65536u
+int x2dev_render_chunks_static(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, const x2_chunk* chunks,
+                               uint32_t count, double* sums, uint32_t* counts, x2_completion* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -125,6 +126,38 @@ struct Paging {
 
 }  // namespace
 
+namespace {
+// The allocation-free render's device memory (x2dev_render_chunks_static): the launch's counters and claim counter
+// (a LaunchState) at offset 0 and the chunks it renders from StaticChunkOffset, X2_STATIC_MAX_CHUNKS of them
+// (1 MiB + 256 B). Each GPU the library runs on holds its own copy until the process ends. In host code `scratch`
+// names a host-side stand-in, not device memory: host code uses the address hipGetSymbolAddress gives for the
+// current device (staticScratch below), never the array's name.
+constexpr size_t StaticChunkOffset = 256;
+constexpr size_t StaticScratchBytes = StaticChunkOffset + size_t(X2_STATIC_MAX_CHUNKS) * sizeof(x2_chunk);
+static_assert(sizeof(LaunchState) <= StaticChunkOffset, "the launch counters fit before the chunks");
+static __device__ __align__(256) unsigned char scratch[StaticScratchBytes];
+
+constexpr int MaxDevices = 64;
+std::mutex g_scratchLock;
+unsigned char* g_scratchAt[MaxDevices] = {};  // each device's address of `scratch` (null: not looked up yet)
+
+// Looks up `scratch`'s address on `device` once per device, with that device current (called from x2dev_create).
+hipError_t staticScratch(int device, unsigned char** out) {
+    if (device < 0 || device >= MaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> hold(g_scratchLock);
+    if (!g_scratchAt[device]) {
+        hipError_t e = hipSetDevice(device);
+        void* at = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&at, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!at) return hipErrorInvalidSymbol;
+        g_scratchAt[device] = static_cast<unsigned char*>(at);
+    }
+    *out = g_scratchAt[device];
+    return hipSuccess;
+}
+}  // namespace
+
 namespace x2 {  // for the other host files (x2_batch.hip)
 void sobolTable(uint32_t out[17 * 32]) { sobolDirections(out); }
 int reportFailure(int code, const std::string& what) { return fail(code, what); }
@@ -192,6 +225,7 @@ struct x2dev {
     float* gradTextures = nullptr;
     float* gradRoughness = nullptr;
     Paging pg;
+    unsigned char* staticScratch = nullptr;  // this device's address of `scratch` (x2dev_render_chunks_static)
 };
 
 namespace {
@@ -345,6 +379,7 @@ int x2dev_create(const x2dev_options* options, x2dev** out) {
     uint32_t table[17 * 32];
     sobolDirections(table);
     X2_CREATE(uploadSobol(table));
+    X2_CREATE(staticScratch(d->device, &d->staticScratch));
 #undef X2_CREATE
     *out = d;
     return X2_OK;
@@ -983,6 +1018,82 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     d->published += count;
     ++d->launches;
     return 1;
+}
+
+// The allocation-free render (no counterpart in the reference): x2dev_launch's render with the chunks copied into
+// the static `scratch` array instead of the lane's ring, its counters there instead of the lane's launch slots, and
+// the caller's film instead of the lane's. No hipMalloc, hipFree or other allocating call: the kernels take this
+// device's address of `scratch` as their ring and state arguments. Calls that overlap on one GPU share the array and
+// race; each call clears the counters before its kernel runs.
+int x2dev_render_chunks_static(x2dev* d, const x2_config* config, uint32_t samples_per_chunk, const x2_chunk* chunks,
+                               uint32_t count, double* sums, uint32_t* counts, x2_completion* out) {
+    if (!d || !config || (count && !chunks) || !sums || !counts) return fail(X2_ERR_ARGUMENT, "null argument");
+    if (count > X2_STATIC_MAX_CHUNKS)
+        return fail(X2_ERR_ARGUMENT, "x2dev_render_chunks_static takes at most X2_STATIC_MAX_CHUNKS (65536) chunks");
+    if (samples_per_chunk == 0 || samples_per_chunk > 1024) return fail(X2_ERR_ARGUMENT, "samples_per_chunk must be 1..1024");
+    if (!d->prepared) return fail(X2_ERR_STATE, "render before prepare");
+    if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "x2dev_render_chunks_static with launches in flight");
+    if (d->pg.on) return fail(X2_ERR_STATE, "x2dev_render_chunks_static reads resident textures only");
+    if (!d->staticScratch) return fail(X2_ERR_STATE, "the static scratch array was not looked up");
+    const int v = validateConfig(d, *config);
+    if (v) return v;
+    const uint32_t tiles = ((config->width + 7) / 8) * ((config->height + 7) / 8);
+    for (uint32_t i = 0; i < count; ++i)
+        if (chunks[i].count > samples_per_chunk || chunks[i].tile >= tiles || chunks[i].first > 0xffffffffu - chunks[i].count)
+            return fail(X2_ERR_ARGUMENT, "chunk out of range (tile, or count above samples_per_chunk)");
+    if (out) *out = x2_completion{};
+    if (!count) return X2_OK;
+    X2_HIP(hipSetDevice(d->device));
+    LaunchState* state = reinterpret_cast<LaunchState*>(d->staticScratch);
+    x2_chunk* ring = reinterpret_cast<x2_chunk*>(d->staticScratch + StaticChunkOffset);
+    X2_HIP(hipMemcpyAsync(ring, chunks, size_t(count) * sizeof(x2_chunk), hipMemcpyHostToDevice, d->stream));
+    X2_HIP(launchClearState(state, d->stream));  // a fill kernel: the last call's counters and claims are gone
+    RenderParams p = renderParams(d, 0);
+    p.cfg = *config;
+    p.dImage = nullptr;
+    p.gradTextures = nullptr;
+    p.gradRoughness = nullptr;
+    p.ring = ring;
+    p.ringCapacity = X2_STATIC_MAX_CHUNKS;
+    p.sums = sums;
+    p.counts = counts;
+    p.state = state;
+    p.itemsPerChunk = samples_per_chunk * TilePixels;
+    p.itemBegin = 0;
+    p.itemEnd = uint64_t(count) * p.itemsPerChunk;
+    p.adaptive = 0;
+    p.odd = nullptr;
+    for (uint32_t i = 0; i < PassSlots; ++i) {
+        p.passMask[i] = nullptr;
+        p.passBase[i] = nullptr;
+    }
+    // the lane's slot-0 events and readback are free: no launch of the lane is in flight
+    X2_HIP(hipEventRecord(d->begun[0], d->stream));
+    X2_HIP(launchRender(p, d->stream));
+    X2_HIP(hipEventRecord(d->ended[0], d->stream));
+    X2_HIP(hipMemcpyAsync(d->readback, state, sizeof(LaunchState), hipMemcpyDeviceToHost, d->stream));
+    X2_HIP(hipStreamSynchronize(d->stream));
+    if (out) {
+        const LaunchState& s = d->readback[0];
+        float ms = 0.0f;
+        X2_HIP(hipEventElapsedTime(&ms, d->begun[0], d->ended[0]));
+        out->chunks = count;
+        out->items = p.itemEnd;
+        uint64_t samples = 0;
+        for (uint32_t i = 0; i < count; ++i) {  // in-image pixels of the chunk's tile, times its sample indices
+            const uint32_t tilesX = (config->width + 7) / 8, tx = chunks[i].tile % tilesX, ty = chunks[i].tile / tilesX;
+            const uint64_t w = std::min(8u, config->width - tx * 8), h = std::min(8u, config->height - ty * 8);
+            samples += w * h * chunks[i].count;
+        }
+        out->samples = samples;
+        out->errors = s.errors;
+        out->device_seconds = double(ms) * 1e-3;
+        out->rays_trace = s.traces;
+        out->rays_shadow = s.shadows;
+        out->nodes_visited = s.nodes;
+        out->triangles_tested = s.triangles;
+    }
+    return X2_OK;
 }
 
 int x2dev_poll(x2dev* d, x2_completion* out) {

@@ -25,6 +25,7 @@ X2_EXR_ZIP = 3
 X2_DEVICE_CPU = 0
 X2_DEVICE_GPU = 1
 X2_NO_CELL = 0xFFFFFFFF
+X2_STATIC_MAX_CHUNKS = 65536
 NODE_BYTES = 128  # X2_NODE_BYTES: a BVH8 node in one cache line
 TRIANGLE_BYTES = 48  # X2_TRIANGLE_BYTES: a triangle record
 LEAF_TRIANGLE_BYTES = 36  # X2_LEAF_TRIANGLE_BYTES: the vertices a leaf test reads
@@ -309,6 +310,8 @@ SIGNATURES = {
                                          ctypes.POINTER(CpuLane), ctypes.POINTER(BrokerOptions),
                                          ctypes.POINTER(GridFrame)]),
     "x2dev_chunk_room": (u32, [ctypes.c_void_p]),
+    "x2dev_render_chunks_static": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32, ctypes.POINTER(Chunk),
+                                                  u32, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Completion)]),
     "x2dev_page_textures": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Paging)]),
     "x2dev_paging_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PagingStats)]),
     "x2dev_parked_outstanding": (u64, [ctypes.c_void_p]),
