@@ -67,6 +67,15 @@ def test_ragged_image_and_viewport():
     assert_parity(scene, config, "cornell 37x23, viewport [3,2)-(31,21)")
 
 
+def test_icospheres_two_light_samples():
+    """Four lights with two samples each make up to ten shadow rays a vertex: more than the wave's shadow pool holds
+    for a vertex, so the frame runs with every shadow ray traced where it is made (the pooled kernel renders the cases
+    with at most six)."""
+    scene, config = scenes.icospheres(width=64, height=36, spp=1, count=4, level=3)
+    config.light_samples = 2
+    assert_parity(scene, config, "icospheres light_samples 2")
+
+
 def test_several_light_and_bsdf_samples():
     scene, config = scenes.cornell(width=48, height=40, spp=1)
     config.light_samples, config.bsdf_samples = 3, 2

@@ -1429,6 +1429,7 @@ __device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixe
 // How a vertex's shadow rays are answered: InlineTrace traces each one where it is made; the wavefront form queues
 // it with the contribution it would add (WaveQueue) and adds that once the ray is traced.
 struct InlineTrace {
+    __device__ __forceinline__ void vertex(V3, V3) {}
     __device__ __forceinline__ bool shadow(const RenderParams& P, V3 o, V3 d, float nearT, float farT, V3, Counters& k,
                                            TravStack& st) {
         return visible(P, o, d, nearT, farT, k, st);
@@ -1575,6 +1576,7 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
     const uint32_t planned = nk * bsdf.count;
     const float thinning = bounceThinning(top, planned);
     const V3 above = hit.origin, below = hit.below;
+    tracer.vertex(above, below);
 
     // Light samples (:2615-2785).
     const uint64_t lightBranch = childBranch(pv.branch, 0xfffe, 0);
@@ -1775,10 +1777,11 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
     return true;
 }
 
-// Shades the sample's last pending ray: traces it, then shadeHit with shadow rays traced inline.
-template <bool Adj, bool Vol, class St>
-__device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixel, uint32_t index, St& S, Counters& k,
-                                            TravStack& st, float* gr) {
+// Shades the sample's last pending ray: traces it, then shadeHit with Tr answering its shadow rays (InlineTrace:
+// traced where they are made).
+template <bool Adj, bool Vol, class St, class Tr>
+__device__ __forceinline__ bool shadeVertexWith(const RenderParams& P, uint32_t pixel, uint32_t index, St& S, Tr& tracer,
+                                                Counters& k, TravStack& st, float* gr) {
     const Pending pv = S.stack[--S.top];
     if constexpr (Adj) {
         while (S.nrec && S.recs[S.nrec - 1].pos > S.top) {
@@ -1791,8 +1794,13 @@ __device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixe
     const Ray ray{pv.origin, pv.dir, fromCamera ? S.cameraNear : 0.0f, fromCamera ? S.cameraFar : MaxFloat};
     Hit hit;
     if (!traceHit(P, ray, hit, k, st)) return false;
-    InlineTrace tracer;
     return shadeHit<Adj, Vol>(P, pixel, index, S, tracer, pv, ray, hit, k, st, gr);
+}
+template <bool Adj, bool Vol, class St>
+__device__ __forceinline__ bool shadeVertex(const RenderParams& P, uint32_t pixel, uint32_t index, St& S, Counters& k,
+                                            TravStack& st, float* gr) {
+    InlineTrace tracer;
+    return shadeVertexWith<Adj, Vol>(P, pixel, index, S, tracer, k, st, gr);
 }
 
 // Closes a finished sample's open subtrees; with Adj each carries its share of the radiance to its sink.
@@ -1862,6 +1870,128 @@ __device__ void runSamples(const RenderParams& P, Feed& feed, Counters& k, TravS
             S.top = 0;
             live = false;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Shadow rays pooled across the wave. A vertex's shadow rays (at most one per light sample and one per lobe sample)
+// wait in the wave's LDS pool with the radiance each adds when unoccluded; once every lane of the wave has shaded its
+// vertex, the wave traces the pool's rays 64 at a time, each lane taking the next ray whoever made it, and each lane
+// then adds its own rays' radiance in the order it made them. The traces are the same rays as traced where they are
+// made, and the additions the same sums in the same order, so the sample's radiance is bit for bit the same; what
+// changes is that a lane with no ray for one light traces another lane's ray instead of idling (one any-hit loop at
+// full width instead of one per light and lobe, each at the width of the lanes that have a ray for it).
+constexpr uint32_t PoolRays = 6;  // shadow rays a lane's vertex may queue (the host checks the frame's bound)
+struct ShadowPool {  // one wave's, 13,440 B: ray n of lane l at n * 64 + l (bank-conflict free)
+    float dx[64 * PoolRays], dy[64 * PoolRays], dz[64 * PoolRays], far[64 * PoolRays];
+    float vr[64 * PoolRays], vg[64 * PoolRays], vb[64 * PoolRays];  // the radiance each adds when unoccluded
+    float org[6 * 64];                                              // each lane's origins above (0..2), below (3..5)
+    uint16_t list[64 * PoolRays];                                   // the wave's rays, compacted
+    uint8_t flag[64 * PoolRays];                                    // in: origin below; out: 2 unoccluded
+};
+static_assert(sizeof(ShadowPool) == 13440, "a wave's shadow pool");
+__device__ __forceinline__ void waveSync() {  // the wave's LDS writes seen by its own later reads, lane to lane
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+struct PoolQueue {
+    ShadowPool* pool;
+    uint32_t lane, n;
+    bool overflow;
+    V3 above;
+    __device__ __forceinline__ void vertex(V3 a, V3 b) {
+        above = a;
+        pool->org[0 * 64 + lane] = a.x, pool->org[1 * 64 + lane] = a.y, pool->org[2 * 64 + lane] = a.z;
+        pool->org[3 * 64 + lane] = b.x, pool->org[4 * 64 + lane] = b.y, pool->org[5 * 64 + lane] = b.z;
+    }
+    // Every shadow ray of the surface estimator starts at 0 along the ray (nearT) from the vertex's origin above or
+    // below the surface; it is queued, and answered as occluded for now.
+    __device__ __forceinline__ bool shadow(const RenderParams&, V3 o, V3 d, float, float farT, V3 value, Counters&,
+                                           TravStack&) {
+        if (n >= PoolRays) {
+            overflow = true;
+            return false;
+        }
+        const uint32_t e = n++ * 64 + lane;
+        pool->dx[e] = d.x, pool->dy[e] = d.y, pool->dz[e] = d.z, pool->far[e] = farT;
+        pool->vr[e] = value.x, pool->vg[e] = value.y, pool->vb[e] = value.z;
+        const bool isAbove = __float_as_uint(o.x) == __float_as_uint(above.x) &&
+                             __float_as_uint(o.y) == __float_as_uint(above.y) &&
+                             __float_as_uint(o.z) == __float_as_uint(above.z);
+        pool->flag[e] = isAbove ? 0u : 1u;
+        return false;
+    }
+};
+// Traces the wave's queued shadow rays and adds each lane's unoccluded ones, in its order, to its radiance. Every lane
+// of the wave calls it together.
+__device__ __forceinline__ void poolFlush(const RenderParams& P, PoolQueue& q, V3& radiance, Counters& k,
+                                          TravStack& st) {
+    const uint32_t lane = q.lane, n = q.n;
+    const unsigned long long lower = (1ull << lane) - 1ull;
+    uint32_t first = 0, total = 0;  // the lane's first place in the compacted list, the wave's rays
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {  // an exclusive prefix sum of n (< 8) over the lanes, a bit at a time
+        const unsigned long long m = __ballot((n >> b) & 1u);
+        first += uint32_t(__popcll(m & lower)) << b;
+        total += uint32_t(__popcll(m)) << b;
+    }
+    if (total == 0) return;
+    ShadowPool& p = *q.pool;
+    for (uint32_t j = 0; j < n; ++j) p.list[first + j] = uint16_t(j * 64 + lane);
+    waveSync();
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t j = base + lane;
+        if (j < total) {
+            const uint32_t e = p.list[j], owner = e & 63u, side = p.flag[e] ? 3u : 0u;
+            const V3 o{p.org[(side + 0) * 64 + owner], p.org[(side + 1) * 64 + owner], p.org[(side + 2) * 64 + owner]};
+            const V3 d{p.dx[e], p.dy[e], p.dz[e]};
+            p.flag[e] = visible(P, o, d, 0.0f, p.far[e], k, st) ? 2u : 0u;
+        }
+    }
+    waveSync();
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t e = j * 64 + lane;
+        if (p.flag[e] == 2u) radiance = radiance + V3{p.vr[e], p.vg[e], p.vb[e]};
+    }
+    q.n = 0;
+}
+// runSamples with the shadow rays pooled (the forward surface estimator): every lane stays in the loop until the
+// wave's samples are all done, so that each iteration's pool is traced by the whole wave.
+template <class Feed>
+__device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k, TravStack& st, PoolQueue& q) {
+    PathState<false, false> S;
+    S.top = 0;
+    uint32_t pixel = 0, index = 0;
+    bool live = false, done = false;
+    for (;;) {
+        const bool idle = S.top == 0;
+        if (idle && live) {
+            feed.done(pixel, index, S.radiance, S.alpha, 0u);
+            live = false;
+        }
+        const unsigned long long idleLanes = __ballot(idle && !done), openLanes = __ballot(!done);
+        if (!openLanes) break;  // every lane's feed is empty and its last sample done
+        const bool claim = idle && !done && (__popcll(idleLanes) >= Feed::Regen || idleLanes == openLanes);
+        if (claim) {
+            if (!feed.next(pixel, index)) done = true;
+            else if (!startSample<false, false>(P, pixel, index, S)) {
+                feed.failed();
+                S.top = 0;
+            } else {
+                live = true;
+            }
+        }
+        if (S.top) {
+            if (!shadeVertexWith<false, false>(P, pixel, index, S, q, k, st, nullptr) || q.overflow) {
+                feed.failed();  // an invalid value, or more shadow rays than the pool holds (the host prevents it)
+                S.top = 0;
+                live = false;
+                q.n = 0;
+                q.overflow = false;
+            }
+        }
+        poolFlush(P, q, S.radiance, k, st);
     }
 }
 
@@ -2192,6 +2322,28 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderPar
 __global__ void __launch_bounds__(256) X2_OCCUPANCY renderPagedKernel(const RenderParams P) {
     runLaunch<false, false, false, true, true>(P);
 }
+// The forward surface estimator with the wave's shadow rays pooled (the frames whose vertices make at most PoolRays
+// shadow rays: light_count x light_samples + 2 x bsdf_samples <= PoolRays).
+__global__ void __launch_bounds__(256) X2_OCCUPANCY renderPoolKernel(const RenderParams P) {
+    __shared__ ShadowPool pools[4];
+    TravStack st;
+    Counters k;
+    uint32_t errors = 0;
+    LaneFeed<false, false> feed{P, errors};
+    PoolQueue q{&pools[threadIdx.x >> 6], threadIdx.x & 63u, 0u, false, V3{0, 0, 0}};
+    runSamplesPooled(P, feed, k, st, q);
+    if (errors) atomicAdd(&P.state->errors, errors);
+    if (P.counters) {
+        const unsigned long long t = waveSum(k.traces), sh = waveSum(k.shadows), n = waveSum(k.nodes),
+                                 tr = waveSum(k.triangles);
+        if ((threadIdx.x & 63u) == 0) {
+            atomicAdd(&P.state->traces, t);
+            atomicAdd(&P.state->shadows, sh);
+            atomicAdd(&P.state->nodes, n);
+            atomicAdd(&P.state->triangles, tr);
+        }
+    }
+}
 __global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) {
     runLaunch<true, false, false, true>(P);
 }
@@ -2249,6 +2401,7 @@ __device__ __forceinline__ void storeSlot(const WaveParams& W, uint32_t slot, co
 struct WaveQueue {
     const WaveParams& W;
     uint32_t parity, slot, n;
+    __device__ __forceinline__ void vertex(V3, V3) {}
     __device__ __forceinline__ bool shadow(const RenderParams&, V3 o, V3 d, float nearT, float farT, V3 value, Counters&,
                                            TravStack&) {
         if (n >= W.perVertex) return false;  // beyond the bound the queues are sized for: never
@@ -2826,12 +2979,13 @@ __global__ void clearState(LaunchState* s) {
 
 // Host-side launchers the lane calls (x2_lane.hip).
 namespace x2 {
-template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium, 3 the prototype estimator
+template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium, 3 the prototype estimator, 4 paged, 5 pooled
 hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
     const auto kernel = Kind == 0   ? renderKernel
                         : Kind == 1 ? adjointKernel
                         : Kind == 2 ? renderVolumeKernel
                         : Kind == 3 ? directKernel
+                        : Kind == 5 ? renderPoolKernel
                                     : renderPagedKernel;
     static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
     if (!resident) {
@@ -2856,7 +3010,11 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
 hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     if (p.cfg.direct_only) return launchFrame<3>(p, stream);
     if (p.tileSlot) return launchFrame<4>(p, stream);  // texture tiles on demand
-    return p.vol.table ? launchFrame<2>(p, stream) : launchFrame<0>(p, stream);
+    if (p.vol.table) return launchFrame<2>(p, stream);
+    // the pool holds a vertex's shadow rays when every light's samples and every lobe's (two at most) fit
+    const uint32_t nl = p.cfg.light_samples > 1 ? p.cfg.light_samples : 1u, nk = p.cfg.bsdf_samples > 1 ? p.cfg.bsdf_samples : 1u;
+    if (uint64_t(p.cfg.light_count) * nl + 2ull * nk <= PoolRays) return launchFrame<5>(p, stream);
+    return launchFrame<0>(p, stream);
 }
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }
 // One step of the device pass chain: errors and trees (adaptive passes), masks and compaction, the render.
