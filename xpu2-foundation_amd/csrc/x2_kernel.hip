@@ -226,9 +226,11 @@ __device__ __forceinline__ void nodeChildren(const Node8Q& n, const BoxRay& r, f
     };
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const float lx = ox + at(b0.x, b0.y, i) * sx, hx = ox + at(b0.z, b0.w, i) * sx;
-        const float ly = oy + at(b1.x, b1.y, i) * sy, hy = oy + at(b1.z, b1.w, i) * sy;
-        const float lz = oz + at(b2.x, b2.y, i) * sz, hz = oz + at(b2.z, b2.w, i) * sz;
+        // q * step is exact (an 8-bit integer times a power of two), so the fused form rounds once, as the builder's
+        // origin + q * step does: the same bound in one instruction
+        const float lx = __builtin_fmaf(at(b0.x, b0.y, i), sx, ox), hx = __builtin_fmaf(at(b0.z, b0.w, i), sx, ox);
+        const float ly = __builtin_fmaf(at(b1.x, b1.y, i), sy, oy), hy = __builtin_fmaf(at(b1.z, b1.w, i), sy, oy);
+        const float lz = __builtin_fmaf(at(b2.x, b2.y, i), sz, oz), hz = __builtin_fmaf(at(b2.z, b2.w, i), sz, oz);
         const float nx = __builtin_fmaf(r.sx ? lx : hx, r.ix, r.ox);
         const float fx = __builtin_fmaf(r.sx ? hx : lx, r.ix, r.ox);
         const float ny = __builtin_fmaf(r.sy ? ly : hy, r.iy, r.oy);
