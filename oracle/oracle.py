@@ -113,6 +113,18 @@ class OracleScene:
                                           counters)
         return sums, counts, tuple(counters)
 
+    def render_into(self, config, chunks, sums, counts, threads=1):
+        """Adds the chunks' samples (a ctypes array of abi.Chunk) into the caller's film (sums [pixels, 4] float64,
+        counts [pixels] uint32, both C-contiguous); returns the counters (traces, shadows, errors)."""
+        assert sums.dtype == np.float64 and counts.dtype == np.uint32 and sums.flags.c_contiguous
+        assert sums.shape == (config.width * config.height, 4) and counts.shape == (config.width * config.height,)
+        counters = (u64 * 3)()
+        if len(chunks):
+            self.lib.oracle_render_chunks(self.h, ctypes.addressof(config), ctypes.addressof(chunks), len(chunks),
+                                          threads, sums.ctypes.data_as(P(f64)), counts.ctypes.data_as(P(u32)),
+                                          counters)
+        return tuple(counters)
+
     def adjoint(self, config, d_image, texture_floats, material_count, threads=None):
         """FP64 gradients (textures concatenated, roughness per material) for the image adjoint d_image [pixels, 3]."""
         d = np.ascontiguousarray(d_image, dtype=np.float32).reshape(-1)

@@ -87,3 +87,38 @@ def test_batch_argument_errors():
             b.add(textured, tconfig)
     finally:
         b.close()
+
+
+def test_batched_workload_of_64_jobs_through_the_graph():
+    """The batched workload itself (64 seeded jobs of 10k-500k triangles, 64^2-1024^2 pixels, 1-64 spp) from the
+    captured graph: every pixel of every job takes exactly its job's spp, and each job's film equals the oracle's over
+    a 16 x 16 window of the image at the job's full spp (the same samples: sums to the FP64 order)."""
+    from x2amd import abi
+    jobs = scenes.batch_jobs()
+    assert len(jobs) == 64
+    b = Batch(0)
+    try:
+        for scene, config in jobs:
+            b.add(scene, config)
+        st, secs = b.run()
+        assert st.graph == 1 and st.errors == 0 and st.jobs == 64
+        assert st.paths == sum(c.width * c.height * c.spp for _, c in jobs)
+        rng = np.random.default_rng(5)
+        for j, (scene, config) in enumerate(jobs):
+            sums, counts = b.film(j)
+            assert (counts == config.spp).all(), j
+            x0 = int(rng.integers(0, config.width - 16 + 1))
+            y0 = int(rng.integers(0, config.height - 16 + 1))
+            window = abi.Config.from_buffer_copy(config)
+            window.viewport[:] = [x0, y0, x0 + 16, y0 + 16]
+            ref_sums, ref_counts, k = oracle.OracleScene(scene).render(window)
+            inside = ref_counts > 0
+            assert k[2] == 0 and int(inside.sum()) == 256 and (ref_counts[inside] == config.spp).all(), j
+            assert (counts[inside] == ref_counts[inside]).all(), j
+            err = np.abs(sums[inside] - ref_sums[inside]).max() / max(1.0, float(np.abs(ref_sums[inside]).max()))
+            assert err <= 1e-12, (j, err)
+        print(f"64 jobs, {st.paths} paths, {st.device_seconds * 1e3:.1f} ms on the device; job p50 "
+              f"{np.percentile(secs, 50) * 1e3:.1f} ms, p99 {np.percentile(secs, 99) * 1e3:.1f} ms")
+        assert np.all(secs > 0) and np.all(secs <= st.device_seconds * 1.01 + 1e-4)
+    finally:
+        b.close()

@@ -1,16 +1,18 @@
 """Frames from the work grid through the broker (x2dev_render_grid): the GPU alone, and the GPU beside a CPU lane.
 
-The GPU alone renders the film the frame loop renders (counts exact, sums to the FP64 atomics' order), with a
-viewport and a share of the frame's samples too. Beside a CPU lane — here a stand-in that records the chunks it
-claims and takes time over each, so the rights of passage and the end-of-frame split come into play — the two lanes
-claim every chunk exactly once: the GPU's film plus the recorded chunks rendered afterwards equals the whole frame.
-Ownership across frames follows the shares the lanes rendered."""
+The GPU alone renders the oracle's film of the frame (counts exact, sums to the FP64 order) and the frame loop's,
+with a viewport and a share of the frame's samples too. Beside a CPU lane that renders its chunks with the oracle
+into a host film of its own (the CPU device's integrator: async_cpu.h:1584-1673 renderChunk), the two films together
+are the oracle's whole frame: a frame rendered by both devices. Beside a CPU lane stand-in that records the chunks it
+claims and takes time over each, so the rights of passage and the end-of-frame split come into play, the two lanes
+claim every chunk exactly once. Ownership across frames follows the shares the lanes rendered."""
 import threading
 import time
 
 import numpy as np
 import pytest
 
+import oracle
 from helpers import drain
 from x2amd import abi, scenes
 from x2amd import grid as G
@@ -60,6 +62,9 @@ def test_gpu_alone_equals_frame_loop(window_ms):
     assert fr.gpu_samples == config.width * config.height * config.spp
     assert (gc == wc).all() and (gc == config.spp).all()
     assert close(gs, ws)
+    os_, oc, ok = oracle.OracleScene(scene).render(config)
+    assert (gc == oc).all() and close(gs, os_) and ok[2] == 0
+    assert fr.rays_trace == ok[0] and fr.rays_shadow == ok[1]
 
 
 def test_viewport_and_sample_share():
@@ -85,6 +90,57 @@ def test_viewport_and_sample_share():
     assert (gc[inside] == 7).all() and (gc[~inside] == 0).all()
     assert (gc == rc).all() and close(gs, rs)
     assert fr.gpu_samples == int(inside.sum()) * 7
+
+
+class OracleCpuLane:
+    """A CPU lane that renders: each worker adds its chunks' samples, computed by the oracle (the CPU integrator this
+    device is pinned to), into its own host film."""
+
+    def __init__(self, scene, config, threads):
+        self.o = oracle.OracleScene(scene)
+        self.config = config
+        n = config.width * config.height
+        self.sums = [np.zeros((n, 4), np.float64) for _ in range(threads)]
+        self.counts = [np.zeros(n, np.uint32) for _ in range(threads)]
+        self.lock = threading.Lock()
+        self.chunks = 0
+        self.errors = 0
+
+    def __call__(self, worker, chunk):
+        one = (abi.Chunk * 1)(abi.Chunk(chunk.tile, chunk.first, chunk.count, chunk.cell))
+        k = self.o.render_into(self.config, one, self.sums[worker], self.counts[worker], threads=1)
+        with self.lock:
+            self.chunks += 1
+            self.errors += k[2]
+        return 0
+
+    def film(self):
+        return sum(self.sums), sum(self.counts)
+
+
+@pytest.mark.parametrize("size,spp,threads", [(256, 16, 4), (200, 24, 6)])
+def test_frame_rendered_by_gpu_and_oracle_cpu_lane(size, spp, threads):
+    """The GPU and a CPU lane rendering with the oracle share a frame (both devices, each cell's owner first, then
+    stealing): the GPU's film plus the CPU lane's is the oracle's whole frame, counts exact, sums to the FP64 order
+    (the samples are bit-identical on either device; only their order of addition differs)."""
+    scene, config = scenes.cornell(width=size, height=size, spp=spp)
+    cpu = OracleCpuLane(scene, config, threads)
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        g = G.WorkGrid(config.width, config.height, config.spp, G.options(32, 4, 2))
+        gs, gc, fr = lane.render_grid(config, g, cpu_render=cpu, cpu_threads=threads, window_ms=2.0, first_chunks=64)
+    finally:
+        lane.close()
+    cs, cc = cpu.film()
+    ref_s, ref_c, ref_k = oracle.OracleScene(scene).render(config)
+    print(f"GPU {fr.gpu_chunks} chunks, CPU (oracle) {fr.cpu_chunks} chunks, {fr.seconds * 1e3:.1f} ms")
+    assert fr.errors == 0 and cpu.errors == 0 and ref_k[2] == 0
+    assert fr.gpu_chunks > 0 and fr.cpu_chunks > 0 and cpu.chunks == fr.cpu_chunks
+    assert fr.gpu_chunks + fr.cpu_chunks == g.total_chunks()
+    assert (gc + cc == ref_c).all() and (ref_c == config.spp).all()
+    assert close(gs + cs, ref_s)
+    assert 0 < int(cc.sum()) < int(ref_c.sum())
 
 
 class SlowCpuLane:

@@ -1,12 +1,15 @@
 """On-demand texture tiles on the device (x2dev_page_textures; the reference's fine-detail cache and parked samples,
 async_gpu.h:2911-3200, async_gpu.cu:350-357): a frame whose texture lives in a small pool of 32x32 half-float tiles —
-samples parked on missing tiles and run again once their tiles are sent up, the pool evicting as it fills — equals
-the frame with the texture resident (the same half-float texels: counts exact, sums to the FP64 atomics' order), and
-so with partial shading (provisional samples from the coarse level, then run again in full), through the frame loop,
-the work grid's broker and the staged checkpoint render. Calls that read resident textures refuse paged ones."""
+samples parked on missing tiles and run again once their tiles are sent up, the pool evicting as it fills — is the
+oracle's frame of the same half-float texels (counts exact, sums to the FP64 order), and so with partial shading
+(provisional samples from the coarse level, then run again in full), through the frame loop, the work grid's broker
+and the staged checkpoint render. A launch's provisional samples are the oracle's samples with the coarse level as
+the texture. Calls that read resident textures refuse paged ones."""
 import numpy as np
 import pytest
 from helpers import drain
+
+import oracle
 
 from x2amd import abi
 from x2amd import grid as G
@@ -27,15 +30,31 @@ def textured(size=128, spp=8, tex=256):
     return scene, resident, config
 
 
+def coarse_level(texture, side=64):
+    """The always-resident coarse level as x2dev_page_textures makes it: level 0's half floats averaged over boxes
+    (float64 sums, exact for these few halves), at most `side` texels a side."""
+    half = np.asarray(texture, np.float32).astype(np.float16).astype(np.float64)
+    h, w = half.shape[:2]
+    cw, ch = w, h
+    while cw > side:
+        cw = (cw + 1) // 2
+    while ch > side:
+        ch = (ch + 1) // 2
+    out = np.zeros((ch, cw, 3), np.float32)
+    for j in range(ch):
+        y0, y1 = j * h // ch, max(j * h // ch + 1, (j + 1) * h // ch)
+        for i in range(cw):
+            x0, x1 = i * w // cw, max(i * w // cw + 1, (i + 1) * w // cw)
+            out[j, i] = (half[y0:y1, x0:x1].sum(axis=(0, 1)) / ((x1 - x0) * (y1 - y0))).astype(np.float32)
+    return out
+
+
 @pytest.fixture(scope="module")
 def frames():
+    """The paged scene, its config and the oracle's film of the same half-float texels resident."""
     scene, resident, config = textured()
-    lane = GpuLane(0)
-    try:
-        lane.prepare(resident)
-        ws, wc, _ = lane.render_frame(config)
-    finally:
-        lane.close()
+    ws, wc, k = oracle.OracleScene(resident).render(config)
+    assert k[2] == 0 and (wc == config.spp).all()
     return scene, config, ws, wc
 
 
@@ -68,8 +87,15 @@ def test_paged_frame_equals_resident(frames, pool, partial):
 def test_provisional_film_holds_the_parked_samples(frames):
     """Partial shading through the step calls (async_gpu.cu:282-295): after each launch every sample issued is in the
     film or, provisional, in the provisional film, and the provisional film counts exactly the samples still parked;
-    launches of parked samples alone empty it, and the film is then the resident frame's."""
+    launches of parked samples alone empty it, and the film is then the oracle's resident frame. The first launch
+    finds no tile resident, so each of its samples that reads the texture reads the coarse level throughout: the
+    film plus the provisional film is then the oracle's frame with the coarse level as the texture."""
     scene, config, ws, wc = frames
+    _, _, params = scenes.cornell_textured(width=config.width, height=config.height, spp=config.spp, texture_size=256)
+    coarse_scene, _, _ = scenes.cornell_textured(width=config.width, height=config.height, spp=config.spp,
+                                                 texture=coarse_level(params["texture"]))
+    cs, cc, ck = oracle.OracleScene(coarse_scene).render(config)
+    assert ck[2] == 0
     tiles = ((config.width + 7) // 8) * ((config.height + 7) // 8)
     every = [abi.Chunk(t, f, 4, 0) for f in range(0, config.spp, 4) for t in range(tiles)]
     lane = GpuLane(0)
@@ -92,6 +118,8 @@ def test_provisional_film_holds_the_parked_samples(frames):
             trace.append((int(pc.sum()), parked))
             assert (fc + pc == config.spp).all()
             assert int(pc.sum()) == parked
+            if launch == 0:  # every textured sample of the first launch read the coarse level
+                assert int(pc.sum()) > 0 and close(fs + ps, cs)
             assert np.isfinite(ps).all() and np.abs(ps[pc == 0]).max(initial=0.0) <= 1e-9
             if parked == 0:
                 break
@@ -110,14 +138,10 @@ def test_frames_with_other_tiles_evict(tmp_path):
     config.max_depth = 0
     halves = [[0, 0, 64, 128], [64, 0, 128, 128]]
     want = []
-    lane = GpuLane(0)
-    try:
-        lane.prepare(resident)
-        for vp in halves:
-            config.viewport[:] = vp
-            want.append(lane.render_frame(config)[:2])
-    finally:
-        lane.close()
+    o = oracle.OracleScene(resident)
+    for vp in halves:
+        config.viewport[:] = vp
+        want.append(o.render(config)[:2])
     lane = GpuLane(0)
     try:
         lane.prepare(scene)

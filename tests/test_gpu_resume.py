@@ -1,8 +1,11 @@
 """A frame taken up from a saved film (x2dev_seed_film, GpuLane::seedFilm): the first half of a frame's samples on
-one lane, its film saved, the second half on a fresh lane seeded with it — the same film as the whole frame at once
-(counts exact, sums to the FP64 atomics' order); an adaptive frame's errors go on from the seeded film."""
+one lane, its film saved, the second half on a fresh lane seeded with it — the oracle's whole frame and the same film
+as the whole frame at once (counts exact, sums to the FP64 order); an adaptive frame's errors go on from the seeded
+film."""
 import numpy as np
 import pytest
+
+import oracle
 
 from helpers import drain, mean_relative_difference
 from x2amd import abi, scenes
@@ -65,6 +68,11 @@ def test_resumed_frame_equals_whole_frame():
         second.close()
     assert (rc == wc).all() and (rc == 16).all()
     assert mean_relative_difference(rs, rc, ws, wc) <= 1e-12
+    os_, oc, ok = oracle.OracleScene(scene).render(config)
+    assert ok[2] == 0 and (rc == oc).all() and mean_relative_difference(rs, rc, os_, oc) <= 1e-12
+    # the first lane's half film is the oracle's samples 0..7
+    hs_o, hc_o, _ = oracle.OracleScene(scene).render(config, chunks=chunks_for(config, 0, 8))
+    assert (hc == hc_o).all() and mean_relative_difference(hs, hc, hs_o, hc_o) <= 1e-12
 
 
 def test_seeded_adaptive_film_errors():

@@ -6,6 +6,7 @@ EXR holds the film's means."""
 import numpy as np
 import pytest
 
+import oracle
 import oracle_files as O
 from x2amd import abi, scenes
 from x2amd import files as F
@@ -31,6 +32,8 @@ def test_staged_frame_equals_whole_frame(tmp_path):
     print(f"stages {st.stages}, saves {st.saves}, launches {st.launches}, {st.seconds * 1e3:.1f} ms")
     assert st.errors == 0 and st.stages == 4 and st.saves == 4 and st.done == 20 and not st.resumed
     assert (sc == wc).all() and close(ss, ws)
+    os_, oc, ok = oracle.OracleScene(scene).render(config)
+    assert ok[2] == 0 and (sc == oc).all() and close(ss, os_)
     r = O.read_film(path)
     assert r["done"] == 20 and r["spp"] == 20 and (r["counts"] == sc).all() and (r["sums"] == ss).all()
     exr = str(tmp_path / "frame.exr")
@@ -60,6 +63,8 @@ def test_stopped_render_taken_up_from_its_save(tmp_path):
     print(f"taken up from {r.resumed_from}: {r.stages} more stages")
     assert r.resumed and r.resumed_from == 6 and r.stages == 4 and r.done == 16
     assert (rc == wc).all() and close(rs, ws)
+    os_, oc, _ = oracle.OracleScene(scene).render(config)
+    assert (rc == oc).all() and close(rs, os_)
 
 
 def test_save_of_another_render_begins_anew(tmp_path):
