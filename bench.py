@@ -11,7 +11,8 @@ BVH8 node fetched (one cache line) + 36 B per triangle tested (its vertices) + 7
 HIP-event time)
 and the CPU baseline (the oracle on this host's cores over a bounded sample of the same workload). The adjoint
 leg times one forward frame and one path-replay adjoint frame of the differentiable Cornell box (512x512, 32 spp):
-adjoint_step_over_forward is the adjoint frame's time over the forward frame's. The volume leg renders the
+adjoint_step_over_forward is a gradient step's time (the forward frame + the adjoint frame) over the forward frame's.
+The volume leg renders the
 participating-media workload (a 256^3 brick-sparse cloud at 1280x720, 32 spp); the batch leg the 64-job batched
 workload, persistent kernel from a hipGraph against one launch per job; the direct leg the prototype estimator on
 the headline scene; the adaptive leg an adaptive frame of the headline scene through the device pass chain and
@@ -39,25 +40,69 @@ def make_lane(wave_slots=0):
     return GpuLane(0, max_chunks_per_launch=LAUNCH_CHUNKS, ring_chunks=2 * LAUNCH_CHUNKS, wave_slots=wave_slots)
 
 
-def cpu_baseline(scene, width, height, spp, seconds_hint):
-    """The oracle (a CPU restatement of the reference's integrator, -O3, no contraction) over 1920x1080 at `spp`
-    (about 10-20 s on a 16-thread host share), one worker thread per host thread claiming 8x8 tiles."""
+def timed_cpu(scene, config, runs=3):
+    """The oracle (a CPU restatement of the reference's integrator, -O3, no contraction) rendering `config` on every
+    host thread (one worker a thread claiming 8x8 tiles, as the reference's CPU lane): one warm-up, then the median of
+    `runs` (the reference's protocol). Returns (seconds, rays, cores)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    from x2amd import scenes
 
     cores = len(os.sched_getaffinity(0))
     o = oracle.OracleScene(scene)
+    o.render(config, threads=cores)
+    times, rays = [], 0
+    for _ in range(runs):
+        t = time.perf_counter()
+        _, _, k = o.render(config, threads=cores)
+        times.append(time.perf_counter() - t)
+        rays = k[0] + k[1]
+    times.sort()
+    return times[len(times) // 2], rays, cores
+
+
+def cpu_baseline(scene, width, height, spp):
+    """The headline scene on the host's cores at width x height x spp (a bounded sample: the rate is per path)."""
+    from x2amd import scenes
+
     _, cfg = scenes.icospheres(width=width, height=height, spp=spp)
-    t = time.perf_counter()
-    _, counts, k = o.render(cfg, threads=cores)
-    dt = time.perf_counter() - t
+    dt, rays, cores = timed_cpu(scene, cfg)
     paths = width * height * spp
     return {
         "value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": cores, "kind": "port",
-        "sample": f"{width}x{height} at {spp} spp ({paths} paths) of the same scene, {dt:.2f} s",
-        "mrays_per_s": (k[0] + k[1]) / dt / 1e6,
+        "sample": f"{width}x{height} at {spp} spp ({paths} paths) of the same scene, median of 3 after a warm-up, "
+                  f"{dt:.2f} s each",
+        "mrays_per_s": rays / dt / 1e6,
     }
+
+
+def cpu_other_configs():
+    """The CPU oracle on the other configurations' workloads: the Cornell box (256x256, 16 spp: the whole frame) and
+    the differentiable Cornell box's forward frame (512x512, 32 spp, RR off) sampled at 2 spp."""
+    import numpy as np  # noqa: F401
+    from x2amd import scenes
+
+    out = {}
+    scene, cfg = scenes.cornell()
+    dt, rays, cores = timed_cpu(scene, cfg)
+    paths = cfg.width * cfg.height * cfg.spp
+    out["cornell_256_16spp"] = {"mpaths_per_s": paths / dt / 1e6, "mrays_per_s": rays / dt / 1e6, "cores": cores,
+                                "kind": "port", "sample": f"the whole frame ({paths} paths), median of 3, {dt:.3f} s"}
+    scene, cfg, _ = scenes.cornell_textured(spp=2)
+    dt, rays, cores = timed_cpu(scene, cfg)
+    paths = cfg.width * cfg.height * cfg.spp
+    out["cornell_textured_512"] = {"mpaths_per_s": paths / dt / 1e6, "mrays_per_s": rays / dt / 1e6, "cores": cores,
+                                   "kind": "port",
+                                   "sample": f"512x512 at 2 spp ({paths} paths) of the 32-spp forward frame, median of "
+                                             f"3, {dt:.2f} s"}
+    return out
+
+
+def roofline_bytes(stats, paths, per_path):
+    """Algorithmic bytes of a frame's launches: BVH8 nodes fetched (one 128-B line each), triangles tested (their
+    36-B packed vertices), the medium's tap sets (16 B: 8 FP16 voxels), and per_path bytes a path."""
+    from x2amd import abi
+    return (abi.NODE_BYTES * stats.nodes_visited + abi.LEAF_TRIANGLE_BYTES * stats.triangles_tested +
+            abi.VOXEL_TAP_BYTES * stats.voxel_taps + per_path * paths)
 
 
 def adjoint_leg(reps):
@@ -86,17 +131,19 @@ def adjoint_leg(reps):
     d_image = (2 * (sums[:, :3] / counts[:, None] - target)).astype(np.float32)
     dbuf, gt, gr = DeviceBuffer(d_image.nbytes), DeviceBuffer(params["texture"].nbytes), DeviceBuffer(4 * 64)
     dbuf.upload(d_image)
-    fw = dict(wall=0.0, kernel=0.0, traces=0)
+    paths = config.width * config.height * config.spp
+    fw = dict(wall=0.0, kernel=0.0, traces=0, bytes=0)
     t0 = time.perf_counter()
     for _ in range(reps):
         _, _, st = lane.render_frame(config)
         fw["kernel"] += st.device_seconds
         fw["traces"] += st.rays_trace + st.rays_shadow
+        fw["bytes"] += roofline_bytes(st, paths, 72 + 48)  # film + one texel tap (4 x 12 B) a path's camera hit
     fw["wall"] = time.perf_counter() - t0
     gt.zero()
     gr.zero()
     lane.adjoint_frame(config, dbuf.ptr, gt.ptr, gr.ptr)  # warm-up
-    ad = dict(wall=0.0, kernel=0.0, traces=0)
+    ad = dict(wall=0.0, kernel=0.0, traces=0, bytes=0)
     for _ in range(reps):
         gt.zero()
         gr.zero()
@@ -105,12 +152,13 @@ def adjoint_leg(reps):
         ad["wall"] += time.perf_counter() - t0
         ad["kernel"] += st.device_seconds
         ad["traces"] += st.rays_trace + st.rays_shadow
+        ad["bytes"] += roofline_bytes(st, paths, 12 + 48)  # the pixel's image adjoint + one texel tap a path
     grads = gt.download(np.float32, params["texture"].size)
     for b in (dbuf, gt, gr):
         b.free()
     lane.close()
     assert np.isfinite(grads).all() and np.abs(grads).sum() > 0
-    paths = config.width * config.height * config.spp
+    fw_gbs, ad_gbs = fw["bytes"] / fw["kernel"] / 1e9, ad["bytes"] / ad["kernel"] / 1e9
     return {
         "workload": f"differentiable cornell {config.width}x{config.height}, {config.spp} spp, depth "
                     f"{config.max_depth}, {params['texture'].shape[1]}x{params['texture'].shape[0]} albedo map, "
@@ -120,6 +168,14 @@ def adjoint_leg(reps):
         "forward_kernel_ms": fw["kernel"] / reps * 1e3, "adjoint_kernel_ms": ad["kernel"] / reps * 1e3,
         "adjoint_over_forward": ad["wall"] / fw["wall"],
         "step_over_forward": (fw["wall"] + ad["wall"]) / fw["wall"],
+        "step_definition": "a gradient step = the forward frame (the image, for the loss) + the adjoint frame; "
+                           "step_over_forward = (forward + adjoint) / forward, adjoint_over_forward = adjoint / forward",
+        "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "forward_achieved": fw_gbs, "forward_frac": fw_gbs / HBM_PEAK_GBS,
+                     "adjoint_achieved": ad_gbs, "adjoint_frac": ad_gbs / HBM_PEAK_GBS,
+                     "counted": "128 B a BVH8 node, 36 B a tested triangle, a path's film (72 B) or image adjoint "
+                                "(12 B) and one bilinear texel tap (48 B); the adjoint's texel-gradient atomics are "
+                                "not counted"},
         "forward_mpaths_s": paths * reps / fw["wall"] / 1e6,
         "adjoint_mpaths_s": paths * reps / ad["wall"] / 1e6,
         "rays_replayed_equal": fw["traces"] == ad["traces"],
@@ -136,7 +192,8 @@ def volume_leg(reps, cpu):
     lane = make_lane()
     lane.prepare(scene)
     lane.render_frame(config)
-    wall, kernel, rays = 0.0, 0.0, 0
+    wall, kernel, rays, taps, nbytes = 0.0, 0.0, 0, 0, 0
+    frame_paths = config.width * config.height * config.spp
     for _ in range(reps):
         t0 = time.perf_counter()
         _, counts, st = lane.render_frame(config)
@@ -144,25 +201,30 @@ def volume_leg(reps, cpu):
         assert st.audit_failures == 0 and st.errors == 0
         kernel += st.device_seconds
         rays += st.rays_trace + st.rays_shadow
+        taps += st.voxel_taps
+        nbytes += roofline_bytes(st, frame_paths, 72)
     lane.close()
-    paths = config.width * config.height * config.spp * reps
+    paths = frame_paths * reps
+    achieved = nbytes / kernel / 1e9
     out = {
         "workload": f"clouds {scene.volume.c.res[0]}^3 grid ({scene.volume.occupancy():.3f} of bricks occupied), "
                     f"{config.width}x{config.height}, {config.spp} spp, depth {config.max_depth}, volume depth "
                     f"{config.max_volume_depth}, rect + env lights",
         "reps": reps, "ms_per_frame": wall / reps * 1e3, "kernel_ms_per_frame": kernel / reps * 1e3,
         "mpaths_per_s": paths / wall / 1e6, "mrays_per_s": rays / wall / 1e6,
+        "voxel_taps_per_path": taps / paths,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "renderVolumeKernel",
+                     "counted": "16 B a trilinear tap set (8 FP16 voxels of one grid), 128 B a BVH8 node, 36 B a "
+                                "tested triangle, 72 B of film a path"},
     }
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        cores = len(os.sched_getaffinity(0))
-        _, small = scenes.clouds(width=640, height=360, spp=4, volume=scene.volume)
-        t = time.perf_counter()
-        oracle.OracleScene(scene).render(small, threads=cores)
-        dt = time.perf_counter() - t
-        out["cpu"] = {"mpaths_per_s": 640 * 360 * 4 / dt / 1e6, "cores": cores, "kind": "port",
-                      "sample": f"640x360 at 4 spp of the same scene, {dt:.2f} s"}
+        _, cpu_cfg = scenes.clouds(spp=16, volume=scene.volume)
+        dt, cpu_rays, cores = timed_cpu(scene, cpu_cfg)
+        n = cpu_cfg.width * cpu_cfg.height * cpu_cfg.spp
+        out["cpu"] = {"mpaths_per_s": n / dt / 1e6, "mrays_per_s": cpu_rays / dt / 1e6, "cores": cores,
+                      "kind": "port", "sample": f"{cpu_cfg.width}x{cpu_cfg.height} at {cpu_cfg.spp} spp ({n} paths) of "
+                                                f"the same frame, median of 3 after a warm-up, {dt:.2f} s each"}
     return out
 
 
@@ -410,10 +472,13 @@ def copy_calibration(mib):
     print(json.dumps({"copy_mib": mib}))
 
 
+TRAFFIC_FILE = os.path.join("profiles", "renderPoolKernel_icospheres_traffic.json")
+
+
 def traffic_per_path():
     """HBM bytes per path of the render kernel at the headline workload, from the committed rocprofv3 --pmc
     measurement (FETCH_SIZE + WRITE_SIZE, calibrated by a copy), or None."""
-    path = os.path.join(ROOT, "profiles", "renderKernel_icospheres_traffic.json")
+    path = os.path.join(ROOT, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -429,7 +494,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
-    ap.add_argument("--cpu-spp", type=int, default=0, help="CPU baseline spp (0: 64 with >= 12 host threads, else 16)")
+    ap.add_argument("--cpu-spp", type=int, default=0, help="CPU baseline spp (0: 16 with >= 12 host threads, else 4)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-reps", type=int, default=5)
@@ -491,6 +556,9 @@ def main():
     if measured:
         traffic = measured["hbm_bytes_per_path"] * paths / max(1, totals["launches"])
     achieved = bytes_total / max(totals["kernel_s"], 1e-12) / 1e9
+    # the same frame counted with the 256-B node and 48-B triangle records BASELINE.md fixed before the nodes went to
+    # one cache line
+    bytes_256_48 = 256 * totals["nodes"] + 48 * totals["tris"] + _abi.FILM_BYTES_PER_PATH * paths
     out = {
         "metric": "Mpaths/s + Mrays/s at 1080p 64spp (fraction of HBM roofline); adjoint step/forward",
         "value": paths / wall / 1e6,
@@ -516,12 +584,20 @@ def main():
                    "form": f"wavefront, {args.wave_slots} path slots" if args.wave_slots else "one kernel"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "renderKernel", "avg_launch_ms": per_launch_s * 1e3,
-                     "launches": totals["launches"], "algorithmic_bytes": bytes_total},
+                     "traffic_source": f"stored: {TRAFFIC_FILE} (rocprofv3 FETCH_SIZE and WRITE_SIZE of the kernel, "
+                                       "each scaled by its reading for a copy of known size), times this run's paths "
+                                       "a launch" if measured else None,
+                     "kernel": "renderKernel" if args.wave_slots else "renderPoolKernel",
+                     "avg_launch_ms": per_launch_s * 1e3,
+                     "launches": totals["launches"], "algorithmic_bytes": bytes_total,
+                     "bytes_per_node": _abi.NODE_BYTES, "bytes_per_tested_triangle": _abi.LEAF_TRIANGLE_BYTES,
+                     "film_bytes_per_path": _abi.FILM_BYTES_PER_PATH,
+                     "frac_at_256B_nodes_48B_triangles": bytes_256_48 / max(totals["kernel_s"], 1e-12) / 1e9 /
+                     HBM_PEAK_GBS},
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adjoint_reps)
-        out["adjoint_step_over_forward"] = out["adjoint"]["adjoint_over_forward"]
+        out["adjoint_step_over_forward"] = out["adjoint"]["step_over_forward"]  # (forward + adjoint) / forward
     if not args.no_volume:
         out["volume"] = volume_leg(args.volume_reps, not args.no_cpu)
     if not args.no_batch:
@@ -538,9 +614,10 @@ def main():
         out["checkpoint"] = checkpoint_leg()
     out["roofline"]["copy_ceiling_gbs"] = copy_ceiling()
     if not args.no_cpu:
-        cpu_spp = args.cpu_spp or (64 if len(os.sched_getaffinity(0)) >= 12 else 16)
-        out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp, 20)
+        cpu_spp = args.cpu_spp or (16 if len(os.sched_getaffinity(0)) >= 12 else 4)
+        out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_other_configs"] = cpu_other_configs()
     print(json.dumps(out))
 
 

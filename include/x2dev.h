@@ -166,6 +166,7 @@ typedef struct x2_completion {
     uint64_t rays_shadow;      /* any-hit shadow rays traced */
     uint64_t nodes_visited;    /* BVH8 nodes fetched (X2_NODE_BYTES = 128 B each) */
     uint64_t triangles_tested; /* triangles tested (their packed vertices, X2_LEAF_TRIANGLE_BYTES = 36 B each) */
+    uint64_t voxel_taps;       /* the medium's trilinear tap sets (8 FP16 voxels of one grid, X2_VOXEL_TAP_BYTES) */
 } x2_completion;
 
 /* Totals over a frame's launches (x2dev_render_frame, x2dev_adjoint_frame). */
@@ -174,6 +175,7 @@ typedef struct x2_frame_stats {
     uint32_t errors, audit_failures;
     double device_seconds, wall_seconds;
     uint64_t rays_trace, rays_shadow, nodes_visited, triangles_tested;
+    uint64_t voxel_taps;
 } x2_frame_stats;
 
 typedef struct x2dev_options {
@@ -265,6 +267,7 @@ int x2dev_rng_twin(x2dev* d, const x2_rng_query* queries, uint32_t n, uint32_t d
 #define X2_TRIANGLE_BYTES 48u
 #define X2_LEAF_TRIANGLE_BYTES 36u
 #define X2_FILM_BYTES_PER_PATH 72u
+#define X2_VOXEL_TAP_BYTES 16u /* a trilinear tap set: 8 FP16 voxels */
 
 /* The broker counterpart (async_render.h:318 Renderer::render, GPU lane only): one uniform frame, chunks made as
  * the work grid makes them (async_grid.h:129-175, progressive ranges 1,1,2,spc over 64-px cells), launches sized

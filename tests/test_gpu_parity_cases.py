@@ -70,7 +70,7 @@ def test_ragged_image_and_viewport():
 def test_icospheres_two_light_samples():
     """Four lights with two samples each make up to ten shadow rays a vertex: more than the wave's shadow pool holds
     for a vertex, so the frame runs with every shadow ray traced where it is made (the pooled kernel renders the cases
-    with at most six)."""
+    with at most six rays a vertex and two lights or more; the Cornell box's one light is traced where it is made)."""
     scene, config = scenes.icospheres(width=64, height=36, spp=1, count=4, level=3)
     config.light_samples = 2
     assert_parity(scene, config, "icospheres light_samples 2")

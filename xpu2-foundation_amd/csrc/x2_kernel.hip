@@ -913,8 +913,15 @@ __device__ __forceinline__ uint32_t brickAt(const VolumeDesc& v, V3 p) {
     }
     return (uint32_t(b[2]) * v.nb[1] + uint32_t(b[1])) * v.nb[0] + uint32_t(b[0]);
 }
+// The medium's tap sets, counted per lane in LDS (the roofline's voxel bytes: 8 FP16 voxels a set).
+__device__ __forceinline__ uint32_t& tapCount() {
+    __shared__ uint32_t taps[256];
+    return taps[threadIdx.x];
+}
 __device__ __forceinline__ float densityAt(const VolumeDesc& v, V3 p) {
-    return v.majorant[brickAt(v, p)] > 0.0f ? sampleGrid(v, v.density, p) : 0.0f;
+    if (!(v.majorant[brickAt(v, p)] > 0.0f)) return 0.0f;
+    ++tapCount();
+    return sampleGrid(v, v.density, p);
 }
 __device__ V3 rampAt(const VolumeDesc& v, float kelvin) {
     float x = kelvin / v.rampMax * float(v.rampSize - 1);
@@ -936,8 +943,10 @@ __device__ VCoeff coefficientsAt(const VolumeDesc& v, V3 p, bool withEmission) {
     if (d == 0.0f) return k;
     k.sigmaT = d * v.sigmaTScale;
     k.sigmaS = ld3(v.albedo) * k.sigmaT;
-    if (withEmission && v.temperature && v.emissionScale > 0.0f)
+    if (withEmission && v.temperature && v.emissionScale > 0.0f) {
+        ++tapCount();
         k.emission = rampAt(v, sampleGrid(v, v.temperature, p)) * (v.emissionScale * d);
+    }
     return k;
 }
 __device__ bool clipBox(const VolumeDesc& v, V3 o, V3 d, float tnear, float tfar, float& a, float& b) {
@@ -2279,6 +2288,7 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     TravStack st;
     Counters k;
     uint32_t errors = 0;
+    if constexpr (Vol) tapCount() = 0;
     float gr[Adj ? RoughnessSlots : 1];
     for (uint32_t i = 0; i < (Adj ? RoughnessSlots : 1); ++i) gr[i] = 0.0f;
     if constexpr (Regen) {
@@ -2308,6 +2318,10 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
             atomicAdd(&P.state->shadows, sh);
             atomicAdd(&P.state->nodes, n);
             atomicAdd(&P.state->triangles, tr);
+        }
+        if constexpr (Vol) {
+            const unsigned long long taps = waveSum(tapCount());
+            if ((threadIdx.x & 63u) == 0) atomicAdd(&P.state->voxels, taps);
         }
     }
 }
@@ -2346,7 +2360,9 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY renderPoolKernel(const Rende
         }
     }
 }
-__global__ void __launch_bounds__(256) X2_OCCUPANCY adjointKernel(const RenderParams P) {
+// The adjoint at 2 waves a SIMD: its sinks and open-subtree records need 248 registers, which 3 waves could not give
+// without 141 of them spilled (the textured Cornell box's adjoint frame: 478 ms against 658 ms).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) adjointKernel(const RenderParams P) {
     runLaunch<true, false, false, true>(P);
 }
 // ---------------------------------------------------------------------------------------------------------------
@@ -2973,6 +2989,7 @@ __global__ void clearState(LaunchState* s) {
     if (threadIdx.x == 0) {
         s->traces = s->shadows = s->nodes = s->triangles = 0;
         s->claimed = 0;
+        s->voxels = 0;
         s->errors = 0;
     }
 }
@@ -3013,9 +3030,12 @@ hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     if (p.cfg.direct_only) return launchFrame<3>(p, stream);
     if (p.tileSlot) return launchFrame<4>(p, stream);  // texture tiles on demand
     if (p.vol.table) return launchFrame<2>(p, stream);
-    // the pool holds a vertex's shadow rays when every light's samples and every lobe's (two at most) fit
+    // The pool holds a vertex's shadow rays when every light's samples and every lobe's (two at most) fit, and pays
+    // with two lights or more: the rays of different lights then share rounds. With one light (the Cornell box:
+    // 284 ms a frame pooled against 243 ms traced where they are made) a vertex's one light ray fills the any-hit
+    // loop already, and the pool's bookkeeping is the cost.
     const uint32_t nl = p.cfg.light_samples > 1 ? p.cfg.light_samples : 1u, nk = p.cfg.bsdf_samples > 1 ? p.cfg.bsdf_samples : 1u;
-    if (uint64_t(p.cfg.light_count) * nl + 2ull * nk <= PoolRays) return launchFrame<5>(p, stream);
+    if (p.cfg.light_count >= 2 && uint64_t(p.cfg.light_count) * nl + 2ull * nk <= PoolRays) return launchFrame<5>(p, stream);
     return launchFrame<0>(p, stream);
 }
 hipError_t launchAdjoint(const RenderParams& p, hipStream_t stream) { return launchFrame<1>(p, stream); }

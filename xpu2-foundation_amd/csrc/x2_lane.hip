@@ -1092,6 +1092,7 @@ int x2dev_render_chunks_static(x2dev* d, const x2_config* config, uint32_t sampl
         out->rays_shadow = s.shadows;
         out->nodes_visited = s.nodes;
         out->triangles_tested = s.triangles;
+        out->voxel_taps = s.voxels;
     }
     return X2_OK;
 }
@@ -1144,6 +1145,7 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
     out->rays_shadow = s.shadows;
     out->nodes_visited = s.nodes;
     out->triangles_tested = s.triangles;
+    out->voxel_taps = s.voxels;
     return 1;
 }
 
@@ -1391,6 +1393,7 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
             st.rays_shadow += done.rays_shadow;
             st.nodes_visited += done.nodes_visited;
             st.triangles_tested += done.triangles_tested;
+            st.voxel_taps += done.voxel_taps;
             if (done.device_seconds > 0) {
                 const double now = double(done.items) / done.device_seconds;
                 rate = rate ? 0.5 * rate + 0.5 * now : now;
@@ -1575,6 +1578,7 @@ int runChain(x2dev* d, const adaptive::Regions& regions, const std::vector<adapt
     st.rays_shadow += ls.shadows;
     st.nodes_visited += ls.nodes;
     st.triangles_tested += ls.triangles;
+    st.voxel_taps += ls.voxels;
     for (uint32_t v : planned) st.samples += v;
     // The passes rendered, counted as the reference counts a chain's (async_render.h:1069).
     a.passes = uniformEnd + (cs.passes ? cs.passes - (uniformEnd ? 1u : 0u) : 0u);

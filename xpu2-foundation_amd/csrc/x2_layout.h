@@ -65,8 +65,10 @@ inline size_t triBufferBytes(size_t triangles) {
 struct alignas(64) LaunchState {
     unsigned long long traces, shadows, nodes, triangles;
     unsigned long long claimed;  // items claimed by the launch's persistent lanes
-    uint32_t errors, pad[5];
+    unsigned long long voxels;   // the medium's trilinear tap sets (8 FP16 voxels of one grid each)
+    uint32_t errors, pad[3];
 };
+static_assert(sizeof(LaunchState) == 64, "one line of counters a launch");
 
 // A texture on the device: its texels (RGB f32, row-major) and where its gradient starts in the adjoint's buffer.
 struct TexDesc {

@@ -30,6 +30,7 @@ NODE_BYTES = 128  # X2_NODE_BYTES: a BVH8 node in one cache line
 TRIANGLE_BYTES = 48  # X2_TRIANGLE_BYTES: a triangle record
 LEAF_TRIANGLE_BYTES = 36  # X2_LEAF_TRIANGLE_BYTES: the vertices a leaf test reads
 FILM_BYTES_PER_PATH = 72
+VOXEL_TAP_BYTES = 16  # X2_VOXEL_TAP_BYTES: a trilinear tap set of one grid (8 FP16 voxels)
 
 u32 = ctypes.c_uint32
 u64 = ctypes.c_uint64
@@ -110,7 +111,7 @@ class Completion(ctypes.Structure):
     _fields_ = [
         ("chunks", u64), ("items", u64), ("samples", u64), ("errors", u32), ("pad", u32),
         ("device_seconds", f64), ("rays_trace", u64), ("rays_shadow", u64), ("nodes_visited", u64),
-        ("triangles_tested", u64),
+        ("triangles_tested", u64), ("voxel_taps", u64),
     ]
 
 
@@ -123,7 +124,7 @@ class FrameStats(ctypes.Structure):
     _fields_ = [
         ("launches", u64), ("chunks", u64), ("samples", u64), ("errors", u32), ("audit_failures", u32),
         ("device_seconds", f64), ("wall_seconds", f64), ("rays_trace", u64), ("rays_shadow", u64),
-        ("nodes_visited", u64), ("triangles_tested", u64),
+        ("nodes_visited", u64), ("triangles_tested", u64), ("voxel_taps", u64),
     ]
 
 
@@ -210,7 +211,7 @@ EXPECTED_SIZES = {
     "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 292),
-    "x2_completion": (Completion, 72), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 80),
+    "x2_completion": (Completion, 80), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 88),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),
     "x2_grid_options": (GridOptions, 24), "x2_cpu_lane": (CpuLane, 24), "x2_broker_options": (BrokerOptions, 16),
     "x2_grid_frame": (GridFrame, 120), "x2_film_file": (FilmFile, 40), "x2_exr_image": (ExrImage, 56),
