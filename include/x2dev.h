@@ -187,6 +187,9 @@ typedef struct x2dev_options {
                                        default); else the wavefront form with this many path slots (a multiple of
                                        256): shading and traversal in separate kernels. The same samples either way.
                                        No reference counterpart (the reference runs one OptiX kernel). */
+    uint32_t lane;                  /* the lane's number (GpuOptions::lane, async_gpu.h:685): lane k >= 1 fails its
+                                       prepare when XPU2_SIMULATE_CARD_FAIL=k, as a card that fails as it takes a
+                                       scene (multi_device.h:42-46); lane 0 never does */
 } x2dev_options;
 
 typedef struct x2dev x2dev;

@@ -153,6 +153,43 @@ def test_texel_gradients_match_central_differences(full):
     assert np.all(np.abs(adj - fd) <= 1e-2 * np.abs(fd) + 1e-4 * np.abs(fd).max())
 
 
+def test_roughness_gradients_at_full_size_over_a_window(full):
+    """The roughness gradients at the differentiable Cornell box's full size (512x512, 32 spp, depth 8, RR off) for the
+    loss restricted to a 64x64 window (the image adjoint 2 (image - target) there, zero elsewhere): the device's
+    adjoint over the whole frame against the oracle's differences with sampling held fixed (its evalRoughness hook),
+    rendered over the window alone at the same sample identities, h = 1e-3."""
+    scene, config, params, lane, target = full
+    image = image_of(lane, config)
+    vp = [224, 288, 288, 352]
+    y, x = np.divmod(np.arange(config.width * config.height), config.width)
+    inside = (x >= vp[0]) & (x < vp[2]) & (y >= vp[1]) & (y < vp[3])
+    d_image = np.zeros((config.width * config.height, 3), np.float32)
+    d_image[inside] = 2 * (image[inside] - target[inside])
+    _, gr, _ = device_adjoint(lane, scene, config, d_image, params["texture"].size)
+    window = abi.Config.from_buffer_copy(config)
+    window.viewport[:] = vp
+    o = oracle.OracleScene(scene)
+    h = 1e-3
+
+    def loss():
+        sums, counts, k = o.render(window)
+        assert k[2] == 0 and (counts[inside] == config.spp).all()
+        return float(np.sum(sums[inside, :3] / counts[inside, None] * d_image[inside]))
+
+    fds = []
+    for m, r in zip(params["roughness_materials"], params["roughness"]):
+        o.set_eval_roughness(m, float(r) + h)
+        lp = loss()
+        o.set_eval_roughness(m, float(r) - h)
+        lm = loss()
+        o.set_eval_roughness(m, -1.0)
+        fds.append((lp - lm) / (2 * h))
+    scale = max(abs(f) for f in fds)
+    for m, fd in zip(params["roughness_materials"], fds):
+        print(f"material {m}: device adjoint {gr[m]:.6e}, detached difference {fd:.6e}")
+        assert abs(gr[m] - fd) <= 1e-2 * abs(fd) + 1e-3 * scale, (m, gr[m], fd)
+
+
 def test_autograd_function_steps_downhill(small):
     from x2amd.diff import DiffRenderer
     scene, config, params, _ = small

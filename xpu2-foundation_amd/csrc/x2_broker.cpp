@@ -117,6 +117,10 @@ struct Broker {
         int status = 0;
         for (;;) {
             if ((status = collect()) < 0) break;
+            if (out.errors) {  // async_render.h:736: the frame stops feeding and fails
+                status = gridFail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
+                break;
+            }
             if (failed.load()) {
                 status = gridFail(X2_ERR_STATE, "the CPU lane failed a chunk");
                 break;

@@ -1399,6 +1399,7 @@ __device__ __forceinline__ uint32_t sampleFlags(const St& S) {
 template <bool Adj, bool Vol, class St>
 __device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixel, uint32_t index, St& S) {
     const x2_config& c = P.cfg;
+    if (P.failPixel == pixel + 1) return false;  // injected failure, from the host
     const Identity root{pixel, index, c.seed, 0};
     float x, y;
     {
@@ -2014,6 +2015,7 @@ __device__ int integrateDirect(const RenderParams& P, uint32_t pixel, uint32_t i
                                TravStack& st) {
     const x2_config& c = P.cfg;
     if (c.surface_limit == 0 || c.surface_limit > 8) return 2;
+    if (P.failPixel == pixel + 1) return 2;  // injected failure, from the host
     const Identity id{pixel, index, c.seed, 0};
     float x, y;
     {

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -51,6 +52,12 @@ int hipFail(hipError_t e, const char* where) {
         const hipError_t e_ = (call);                  \
         if (e_ != hipSuccess) return hipFail(e_, #call); \
     } while (0)
+
+// A simulation setting from the environment (multi_device.h:47-50): its value as a count, 0 when unset or negative.
+uint32_t simulated(const char* name) {
+    const char* v = std::getenv(name);
+    return v && *v ? uint32_t(std::max(0, std::atoi(v))) : 0u;
+}
 
 // Sobol direction numbers (sample_stream.h:82-100) from the Joe-Kuo primitive polynomials (new-joe-kuo-6.21201,
 // dimensions 2..17: degree s, coefficient bits a, initial m), by the standard recurrence; row 0 is van der Corput.
@@ -182,6 +189,7 @@ struct x2dev {
     VolumeDesc volDesc{};
     void* volBuffers[5] = {};  // table, density, temperature, majorant, ramp
     bool prepared = false;
+    uint32_t failPixel = 0;  // XPU2_SIMULATE_SAMPLE_FAIL as the frame began (RenderParams::failPixel)
     // film
     double* sums = nullptr;
     uint32_t* counts = nullptr;
@@ -334,6 +342,7 @@ int x2dev_create(const x2dev_options* options, x2dev** out) {
     d->opt.counters = options ? options->counters : 1u;
     d->device = d->opt.device;
     d->opt.wave_slots = options ? options->wave_slots : 0u;
+    d->opt.lane = options ? options->lane : 0u;
     if (d->opt.wave_slots % 256 || d->opt.wave_slots > (1u << 24)) {
         delete d;
         return fail(X2_ERR_ARGUMENT, "wave_slots must be a multiple of 256, at most 2^24");
@@ -387,6 +396,8 @@ int x2dev_create(const x2dev_options* options, x2dev** out) {
 
 int x2dev_prepare(x2dev* d, const x2_scene* s) {
     if (!d || !s) return fail(X2_ERR_ARGUMENT, "null device or scene");
+    if (d->opt.lane && simulated("XPU2_SIMULATE_CARD_FAIL") == d->opt.lane)  // async_gpu.h:685-686
+        return fail(X2_ERR_STATE, "failed (simulated, XPU2_SIMULATE_CARD_FAIL)");
     if (!d->inFlight.empty()) return fail(X2_ERR_STATE, "prepare with launches in flight");
     if ((s->triangle_count && (!s->positions || !s->indices)) || !s->geom_first || !s->geom_material ||
         !s->materials || (s->light_count && !s->lights))
@@ -611,6 +622,7 @@ int beginFrame(x2dev* d, const x2_config* c, uint32_t samples_per_chunk, bool ad
     X2_HIP(hipMemsetAsync(d->sums, 0, pixels * 4 * sizeof(double), d->stream));
     X2_HIP(hipMemsetAsync(d->counts, 0, pixels * sizeof(uint32_t), d->stream));
     d->cfg = *c;
+    d->failPixel = simulated("XPU2_SIMULATE_SAMPLE_FAIL");
     d->dImage = nullptr;
     d->gradTextures = nullptr;
     d->gradRoughness = nullptr;
@@ -677,6 +689,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
     p.ringCapacity = d->opt.ring_chunks;
     p.counters = d->opt.counters;
     p.adaptive = d->adaptive ? 1u : 0u;
+    p.failPixel = d->failPixel;
     if (d->pg.on) {
         const Paging& g = d->pg;
         p.tileSlot = g.tileSlot;
@@ -1050,6 +1063,7 @@ int x2dev_render_chunks_static(x2dev* d, const x2_config* config, uint32_t sampl
     X2_HIP(launchClearState(state, d->stream));  // a fill kernel: the last call's counters and claims are gone
     RenderParams p = renderParams(d, 0);
     p.cfg = *config;
+    p.failPixel = simulated("XPU2_SIMULATE_SAMPLE_FAIL");
     p.dImage = nullptr;
     p.gradTextures = nullptr;
     p.gradRoughness = nullptr;
@@ -1365,7 +1379,9 @@ int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
 }
 
 // Launches the chunks in order, at most two launches in flight, sized by the measured rate, and collects every
-// completion (chunkSamples: each chunk's samples, else every in-image pixel of its tile).
+// completion (chunkSamples: each chunk's samples, else every in-image pixel of its tile). A completion that counted
+// an error ends the feed: nothing more is launched, the launches in flight are collected, and the caller fails the
+// frame (async_render.h:679, 736).
 int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<uint32_t>* chunkSamples,
               x2_frame_stats& st) {
     const x2_config* c = &d->cfg;
@@ -1381,7 +1397,7 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
     // none is left; a stretch of launches of parked samples alone that ends none means the pool cannot hold what they
     // need (async_render.h:816-818).
     uint64_t fewest = ~0ull, idle = 0, rounds = 0;
-    while (next < chunks.size() || !d->inFlight.empty() || x2dev_parked_outstanding(d)) {
+    while (!d->inFlight.empty() || (!st.errors && (next < chunks.size() || x2dev_parked_outstanding(d)))) {
         x2_completion done;
         while ((r = x2dev_poll(d, &done)) == 1) {
             ++st.launches;
@@ -1400,6 +1416,13 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
             }
         }
         if (r < 0) return r;
+        if (st.errors) {  // the launches in flight finish; none follows them
+            if (!d->inFlight.empty()) {
+                const hipError_t e = hipEventSynchronize(d->done[d->inFlight.front().slot]);
+                if (e != hipSuccess) return hipFail(e, "hipEventSynchronize");
+            }
+            continue;
+        }
         if (next >= chunks.size() && x2dev_ready(d) && x2dev_parked_outstanding(d)) {
             const uint64_t left = x2dev_parked_outstanding(d) + d->pg.inflightItems;
             if (left < fewest) {
@@ -1456,6 +1479,11 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
     x2_frame_stats st{};
     r = runFrame(d, c, spc, st);
     if (r) return r;
+    if (st.errors) {  // before the audit, as the reference's frame fails (async_render.h:955-958)
+        st.wall_seconds = std::chrono::duration<double>(clock::now() - start).count();
+        if (stats) *stats = st;
+        return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
+    }
     r = x2dev_download_film(d);
     if (r) return r;
     X2_HIP(hipEventSynchronize(d->filmDone));
@@ -1468,7 +1496,6 @@ int x2dev_render_frame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stat
     st.wall_seconds = std::chrono::duration<double>(clock::now() - start).count();
     if (stats) *stats = st;
     if (st.audit_failures) return fail(X2_ERR_STATE, "Sample audit failure (async_render.h:1035)");
-    if (st.errors) return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
     return X2_OK;
 }
 
@@ -1702,6 +1729,12 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
         r = runChunks(d, chunks, &chunkSamples, st);
         if (r) return r;
         ++a->passes;
+        if (st.errors) break;
+    }
+    if (st.errors) {  // before the audit (async_render.h:955-958)
+        st.wall_seconds = std::chrono::duration<double>(clock::now() - start).count();
+        if (stats) *stats = st;
+        return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
     }
     r = x2dev_download_film(d);
     if (r) return r;
@@ -1711,7 +1744,6 @@ int x2dev_render_adaptive(x2dev* d, const x2_config* c, uint32_t spc, x2_adaptiv
     st.wall_seconds = std::chrono::duration<double>(clock::now() - start).count();
     if (stats) *stats = st;
     if (st.audit_failures) return fail(X2_ERR_STATE, "Sample audit failure (async_render.h:1035)");
-    if (st.errors) return fail(X2_ERR_STATE, "GPU sample failure; whole epoch must use fallback (async_render.h:736)");
     return X2_OK;
 }
 
@@ -1773,6 +1805,7 @@ int x2dev_adjoint_frame(x2dev* d, const x2_config* c, uint32_t spc, const float*
     if (c->direct_only) return fail(X2_ERR_ARGUMENT, "the adjoint replays the path tracer, not the prototype estimator");
     X2_HIP(hipSetDevice(d->device));
     d->cfg = *c;
+    d->failPixel = simulated("XPU2_SIMULATE_SAMPLE_FAIL");
     d->itemsPerChunk = spc * TilePixels;
     d->published = 0;
     d->launches = 0;

@@ -166,7 +166,10 @@ struct RenderParams {
     unsigned long long parkedBegin;
     uint32_t parkedCount, parkCap, missCap, partial;
     uint32_t* tileStamp;  // each tile's last launch to read it (the pool evicts the least recently used)
-    uint32_t stamp, stampPad;
+    uint32_t stamp;
+    // Failure injection: nonzero k makes every sample of pixel k - 1 fail as it starts, counted in the launch's
+    // errors like any sample the estimator cannot run (async_gpu.cu:363), so the frame fails as the reference's does.
+    uint32_t failPixel;
 };
 constexpr uint32_t PassSlots = 4, PassShift = 16, MaxPasses = 160;  // async_integrator.h:3083
 

@@ -117,7 +117,7 @@ class Completion(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("device", i32), ("max_chunks_per_launch", u32), ("ring_chunks", u32), ("counters", u32),
-                ("wave_slots", u32)]
+                ("wave_slots", u32), ("lane", u32)]
 
 
 class FrameStats(ctypes.Structure):
@@ -211,7 +211,7 @@ EXPECTED_SIZES = {
     "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 292),
-    "x2_completion": (Completion, 80), "x2dev_options": (Options, 20), "x2_frame_stats": (FrameStats, 88),
+    "x2_completion": (Completion, 80), "x2dev_options": (Options, 24), "x2_frame_stats": (FrameStats, 88),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),
     "x2_grid_options": (GridOptions, 24), "x2_cpu_lane": (CpuLane, 24), "x2_broker_options": (BrokerOptions, 16),
     "x2_grid_frame": (GridFrame, 120), "x2_film_file": (FilmFile, 40), "x2_exr_image": (ExrImage, 56),

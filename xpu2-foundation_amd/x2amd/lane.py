@@ -14,9 +14,10 @@ from . import abi
 
 
 class GpuLane:
-    def __init__(self, device=0, max_chunks_per_launch=16384, ring_chunks=1 << 17, counters=True, wave_slots=0):
+    def __init__(self, device=0, max_chunks_per_launch=16384, ring_chunks=1 << 17, counters=True, wave_slots=0,
+                 lane=0):
         self.lib = abi.load()
-        opts = abi.Options(device, max_chunks_per_launch, ring_chunks, 1 if counters else 0, wave_slots)
+        opts = abi.Options(device, max_chunks_per_launch, ring_chunks, 1 if counters else 0, wave_slots, lane)
         h = ctypes.c_void_p()
         abi.check(self.lib.x2dev_create(ctypes.byref(opts), ctypes.byref(h)), "x2dev_create")
         self.h = h
