@@ -362,22 +362,18 @@ def grid_leg(reps, windows=(100.0, 10.0)):
 
 def paging_leg(reps):
     """Texture tiles on demand on the differentiable Cornell box's forward frame (512x512, 32 spp, its back wall a
-    512x512 map): the texture resident (its texels the tiles' half floats), then paged into a pool as large as the
-    texture (256 tiles; the first frame parks every sample that meets a missing tile and runs it again), and paged
-    with partial shading. Frames after the first find their tiles resident."""
-    import numpy as np
-
+    512x512 map): the texture resident, then its levels wider than 64 paged into a pool as large as them (336 tiles;
+    the first frame parks every sample that meets a missing tile and runs it again), paged with partial shading, and
+    paged into a pool of 96 tiles, under a third of them: the footprints' levels are the working set, not level 0."""
     from x2amd import scenes
 
     scene, config, params = scenes.cornell_textured()
-    half = params["texture"].astype(np.float16).astype(np.float32)
-    resident, _, _ = scenes.cornell_textured(texture=half)
     out = {"workload": f"cornell textured {config.width}x{config.height}, {config.spp} spp, depth {config.max_depth}, "
-                       "512x512 map in 256 tiles of 32x32 half floats", "reps": reps}
-    for name, sc, paging in [("resident", resident, None), ("paged", scene, (256, False)),
-                             ("paged_partial", scene, (256, True))]:
+                       "512x512 map, levels 512..128 in 336 tiles of 32x32 half floats", "reps": reps}
+    for name, paging in [("resident", None), ("paged", (336, False)), ("paged_partial", (336, True)),
+                         ("paged_pool_96", (96, False))]:
         lane = make_lane()
-        lane.prepare(sc)
+        lane.prepare(scene)
         if paging:
             lane.page_textures(paging[0], partial=paging[1])
         t0 = time.perf_counter()
@@ -393,7 +389,9 @@ def paging_leg(reps):
         entry = {"first_frame_ms": first * 1e3, "ms_per_frame": wall / reps * 1e3,
                  "mpaths_per_s": config.width * config.height * config.spp * reps / wall / 1e6}
         if paging:
-            entry.update(first_frame_parked=g.parked_total, first_frame_provisional=g.provisional, uploads=g.uploads)
+            g2 = lane.paging_stats()
+            entry.update(first_frame_parked=g.parked_total, first_frame_provisional=g.provisional, uploads=g2.uploads,
+                         evictions=g2.evictions, tiles=g2.tiles)
         lane.close()
         out[name] = entry
     return out

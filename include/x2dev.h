@@ -76,8 +76,11 @@ typedef struct x2_material {
     uint32_t albedo_map;   /* 0: none; k: the scene's texture k-1 multiplies the albedo */
 } x2_material;
 
-/* An RGB FP32 texture, row-major texels (y * width + x) * 3, read by bilinear filtering at level 0 with periodic
- * addressing and the texture's t flipped (texture_core.h:144-153, :179-181 with unit scale and no offset). */
+/* An RGB FP32 texture, row-major texels (y * width + x) * 3, sides at most 32768. The device makes its mip pyramid
+ * (level k + 1 max(1, w >> 1) x max(1, h >> 1), each texel the mean of the 2 x 2 below it, down to 1 x 1) and reads it
+ * as the path tracer's map lookups do (texture_core.h:161-206; shade_mapnodes.h:344-347, :420-470): anisotropic on
+ * camera paths, trilinear closest texels on glossy ones, one level's closest texel on diffuse ones, at the footprint
+ * the ray's differentials give (shade_material.h:388-558), periodic, t flipped. */
 typedef struct x2_texture {
     uint32_t width, height;
     const float* texels;
@@ -344,14 +347,16 @@ int x2_cellular_step(uint32_t width, uint32_t height, float target_error, const 
 int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors);
 
 /* ---- On-demand texture tiles and partial shading (GpuLane's fine-detail cache, async_gpu.h:2911-3200, and its
- * parked and provisional samples, async_gpu.cu:274-295,350-357; texture_core.h's tiles). After prepare, every texture's
- * level 0 is kept by the host as 32x32 tiles of half-float rgb (a float texture's levels are held in half floats,
- * texture_core.h:30-31) and the device holds pool_tiles of them at a time (a full pool gives up its least recently read tile), plus
- * each texture's coarse level (level 0's halves averaged over boxes, at most coarse_side texels a side; 0: 64),
- * always resident. A sample whose texture lookup finds a tile missing asks for every tile of its footprint and is
- * parked: it ends, and runs again in a later launch (ahead of its chunks, or in a launch of parked samples alone,
- * x2dev_launch with no chunks) once x2dev_launch has sent up the tiles asked for. With partial it finishes from the
- * coarse level instead, into the provisional film, and is parked to run again in full. The frame calls
+ * parked and provisional samples, async_gpu.cu:274-295,350-357; texture_core.h's tiles). After prepare, every level of
+ * every texture's pyramid is held in half floats (a float texture's levels are held so, texture_core.h:28-31): the
+ * levels wider or taller than coarse_side (0: 64) are kept by the host as 32x32 tiles of half-float rgb, of which the
+ * device holds pool_tiles at a time (a full pool gives up its least recently read tile); the coarser levels are
+ * always resident. A sample whose texture lookup finds a tile missing asks for every tile its footprint reads (at
+ * the levels the footprint picks) and is parked: it ends, and runs again in a later launch (ahead of its chunks, or in
+ * a launch of parked samples alone, x2dev_launch with no chunks) once x2dev_launch has sent up the tiles asked for.
+ * With partial, each missing texel is read from its parent on the nearest level up that is resident instead
+ * (async_gpu_device.h:49-58); the sample finishes into the provisional film and is parked to run again in full. The
+ * frame calls
  * (x2dev_render_frame, x2dev_render_adaptive with host trees, x2dev_render_grid, x2dev_render_staged) run parked
  * samples until none is left, so their films are the resident frame's. The surface estimator's forward frames in the
  * one-kernel form only: the wavefront form, the adjoint, the prototype estimator, the device pass chain and media

@@ -52,6 +52,17 @@ def load():
         "oracle_render_pixels": (c_int, [vp, vp, P(u32), P(u32), u32, P(f64), P(f64), P(u32), P(u64)]),
         "oracle_adjoint": (c_int, [vp, vp, u32, P(f32), P(f64), P(f64), P(u64)]),
         "oracle_set_texture": (None, [vp, u32, P(f32)]),
+        "oracle_set_texture_half": (None, [vp, u32, c_int]),
+        "oracle_set_texture_coarse": (None, [vp, u32, u32]),
+        "oracle_map_lookup": (c_int, [vp, c_int, u32, P(f32), P(f32)]),
+        "oracle_map_lookup_swap": (c_int, [vp, c_int, u32, c_int, P(f32), P(f32)]),
+        "oracle_pyramid": (u32, [vp, c_int, P(f32), u64, P(u32)]),
+        "oracle_hit_tex_derivatives": (None, [P(f32)] * 8 + [u32, P(f32)]),
+        "oracle_transfer": (c_int, [P(f32), c_int] + [P(f32)] * 7),
+        "oracle_triangle_partials": (c_int, [P(f32)] * 6),
+        "oracle_hemisphere_differentials": (None, [f32, f32, P(f32)]),
+        "oracle_scatter_differentials": (c_int, [u32, P(f32), f32, P(f32), P(f32), c_int, P(f32), P(f32), P(f32), f32,
+                                                 f32, P(f32), f32, P(f32)]),
         "oracle_set_material": (None, [vp, u32, vp]),
         "oracle_set_eval_roughness": (None, [vp, u32, f32]),
         "oracle_ggx_roughness_log_derivative": (f32, [f32, P(f32), P(f32), P(f32)]),
@@ -77,6 +88,42 @@ def fa(values):
         return None
     a = (f32 * len(values))(*[float(v) for v in values])
     return a
+
+
+class _Texture(ctypes.Structure):  # x2_texture
+    _fields_ = [("width", u32), ("height", u32), ("texels", P(f32))]
+
+
+def _texture(texels):
+    arr = np.ascontiguousarray(texels, np.float32)
+    t = _Texture(arr.shape[1], arr.shape[0], arr.ctypes.data_as(P(f32)))
+    return t, arr
+
+
+def map_lookup(texels, mode, footprint, half=False):
+    """The path tracer's map lookup (mode 0 anisotropic, 1 trilinear closest, 2 closest on one level) over the
+    texture's pyramid at footprint (s, t, ds/dx, dt/dx, ds/dy, dt/dy); None for an invalid value."""
+    t, keep = _texture(texels)
+    out = (f32 * 3)()
+    ok = load().oracle_map_lookup(ctypes.addressof(t), 1 if half else 0, mode, fa(footprint), out)
+    del keep
+    return np.array(out[:], np.float32) if ok else None
+
+
+def pyramid(texels, half=False):
+    """The texture's mip levels, each an H x W x 3 float32 array, finest first."""
+    t, keep = _texture(texels)
+    h, w = keep.shape[:2]
+    cap = h * w * 3 * 2 + 16 * 3
+    out = np.zeros(cap, np.float32)
+    sizes = (u32 * 32)()
+    n = load().oracle_pyramid(ctypes.addressof(t), 1 if half else 0, out.ctypes.data_as(P(f32)), cap, sizes)
+    levels, at = [], 0
+    for k in range(n):
+        lw, lh = sizes[2 * k], sizes[2 * k + 1]
+        levels.append(out[at:at + lw * lh * 3].reshape(lh, lw, 3).copy())
+        at += lw * lh * 3
+    return levels
 
 
 class OracleScene:
@@ -138,6 +185,14 @@ class OracleScene:
     def set_texture(self, index, texels):
         t = np.ascontiguousarray(texels, dtype=np.float32)
         self.lib.oracle_set_texture(self.h, index, t.ctypes.data_as(P(f32)))
+
+    def set_texture_half(self, index, half=True):
+        """Every level of the texture rounded to half floats, as paged tiles hold them."""
+        self.lib.oracle_set_texture_half(self.h, index, 1 if half else 0)
+
+    def set_texture_coarse(self, index, level):
+        """Fetches finer than `level` read their ancestor there (16: off)."""
+        self.lib.oracle_set_texture_coarse(self.h, index, level)
 
     def set_material(self, index, material):
         self.lib.oracle_set_material(self.h, index, ctypes.addressof(material))

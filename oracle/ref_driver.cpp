@@ -2,7 +2,7 @@
 // in place from the reference tree (the Makefile's `ref` target, output oracle/_ref/libref.so). The tests call it to
 // check the oracle's restatement against the reference's code on the same inputs. It wraps, it does not restate:
 // every computation below is the reference's (sample_stream.h, surface_core.h, camera_core.h, triangle_hit.h,
-// resume_core.h, light_core.h, shade_lobes_eval.h, texture_core.h). The shared integrator (async_integrator.h) is not buildable here
+// resume_core.h, light_core.h, shade_lobes_eval.h, shade_material.h, texture_core.h). The shared integrator (async_integrator.h) is not buildable here
 // (it includes NanoVDB, which the image lacks), so it is not wrapped.
 #include <cstdint>
 #include <cstring>
@@ -12,6 +12,7 @@
 #include "resume_core.h"
 #include "sample_stream.h"
 #include "shade_lobes_eval.h"
+#include "shade_material.h"
 #include "surface_core.h"
 #include "texture_core.h"
 #include "triangle_hit.h"
@@ -199,5 +200,133 @@ int ref_bilinear(uint32_t width, uint32_t height, const float* texels, float u, 
     if (status != tx::Ready) return 0;
     put(out, r);
     return 1;
+}
+
+// tex::lookup (texture_core.h:179-206) over the given levels (rgb floats, format 0 pages), with ImageMap's binding
+// (unit scale, no offset or bias, periodic, no gamma) and the mode and swapCross given.
+int ref_map_lookup(uint32_t count, const uint32_t* sizes, const float* texels, uint32_t mode, uint32_t swap,
+                   const float f[6], float out[3]) {
+    namespace tx = ii::tex;
+    std::vector<std::vector<std::vector<V3>>> tiles(count);
+    std::vector<std::vector<const V3*>> pages(count);
+    tx::Image image;
+    image.count = count;
+    const float* at = texels;
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t width = sizes[2 * k], height = sizes[2 * k + 1];
+        const uint32_t nx = (width + tx::Tile - 1) / tx::Tile, ny = (height + tx::Tile - 1) / tx::Tile;
+        tiles[k].resize(size_t(nx) * ny);
+        pages[k].resize(size_t(nx) * ny);
+        for (uint32_t ty = 0; ty < ny; ++ty)
+            for (uint32_t tx0 = 0; tx0 < nx; ++tx0) {
+                const uint32_t stride = std::min<uint32_t>(tx::Tile, width - tx0 * tx::Tile);
+                const uint32_t rows = std::min<uint32_t>(tx::Tile, height - ty * tx::Tile);
+                std::vector<V3>& t = tiles[k][size_t(ty) * nx + tx0];
+                t.resize(size_t(stride) * rows);
+                for (uint32_t y = 0; y < rows; ++y)
+                    for (uint32_t x = 0; x < stride; ++x)
+                        t[size_t(y) * stride + x] = v3(at + (size_t(ty * tx::Tile + y) * width + tx0 * tx::Tile + x) * 3);
+                pages[k][size_t(ty) * nx + tx0] = t.data();
+            }
+        image.levels[k].width = width;
+        image.levels[k].height = height;
+        image.levels[k].pages = pages[k].data();
+        at += size_t(width) * height * 3;
+    }
+    tx::Binding b;
+    b.image = &image;
+    b.sx = b.sy = 1;
+    b.ox = b.oy = 0;
+    b.bias = 0;
+    b.wrap = 1;
+    b.gamma = 0;
+    b.mode = mode;
+    b.swapCross = swap;
+    V3 r;
+    if (tx::lookup(b, tx::Footprint{f[0], f[1], f[2], f[3], f[4], f[5]}, r) != tx::Ready) return 0;
+    put(out, r);
+    return 1;
+}
+// tex::footprint (texture_core.h:208-221), the spatial form: (u + v, v) and its derivatives from the ray's dx, dy.
+int ref_footprint(const float a[3], const float b[3], const float c[3], const float org[3], const float dir[3],
+                  const float dx[3], const float dy[3], float u, float v, uint32_t spp, float out[6]) {
+    ii::surface::Ray ray{};
+    ray.origin = v3(org);
+    ray.direction = v3(dir);
+    ray.dx = v3(dx);
+    ray.dy = v3(dy);
+    ii::tex::Footprint f{};
+    if (!ii::tex::footprint(v3(a), v3(b), v3(c), ray, u, v, spp, f, true)) return 0;
+    out[0] = f.u; out[1] = f.v; out[2] = f.dudx; out[3] = f.dvdx; out[4] = f.dudy; out[5] = f.dvdy;
+    return 1;
+}
+// shade::hitTexDerivatives (shade_material.h:392-432) with the given corners' texture coordinates (uv: 6 floats).
+void ref_hit_tex_derivatives(const float a[3], const float b[3], const float c[3], const float uv[6], const float org[3],
+                             const float dir[3], const float dx[3], const float dy[3], uint32_t spp, float out[4]) {
+    ii::shade::MeshAttributes m{};
+    m.uvs = uv;
+    m.uvRate = ii::shade::RateVertex;
+    ii::surface::Ray ray{};
+    ray.origin = v3(org);
+    ray.direction = v3(dir);
+    ray.dx = v3(dx);
+    ray.dy = v3(dy);
+    ii::shade::State st{};
+    ii::shade::hitTexDerivatives(m, 0, 0, 1, 2, v3(a), v3(b), v3(c), ray, spp, st);
+    out[0] = st.dsdx; out[1] = st.dtdx; out[2] = st.dsdy; out[3] = st.dtdy;
+}
+// shade::transferDifferentials and differentialTexDerivatives (shade_material.h:436-495): diff = dOx, dOy, dDx, dDy
+// (12 floats); out = dPdx, dPdy, dsdx, dtdx, dsdy, dtdy; returns diffValid.
+int ref_transfer(const float diff[12], int valid, const float org[3], const float dir[3], const float Ng[3],
+                 const float P[3], const float dPds[3], const float dPdt[3], float out[10]) {
+    ii::shade::RayDiff d;
+    d.dOx = v3(diff); d.dOy = v3(diff + 3); d.dDx = v3(diff + 6); d.dDy = v3(diff + 9);
+    d.valid = valid ? 1u : 0u;
+    ii::shade::State st{};
+    st.Ng = v3(Ng);
+    st.P = v3(P);
+    st.dPds = v3(dPds);
+    st.dPdt = v3(dPdt);
+    ii::shade::transferDifferentials(d, v3(org), v3(dir), st);
+    ii::shade::differentialTexDerivatives(st);
+    put(out, st.dPdx);
+    put(out + 3, st.dPdy);
+    out[6] = st.dsdx; out[7] = st.dtdx; out[8] = st.dsdy; out[9] = st.dtdy;
+    return st.diffValid ? 1 : 0;
+}
+// shade::trianglePartials (shade_material.h:103-115).
+int ref_triangle_partials(const float x0[3], const float x1[3], const float x2[3], const float uv[6], float dPds[3],
+                          float dPdt[3]) {
+    V3 s, t;
+    if (!ii::shade::trianglePartials(v3(x0), v3(x1), v3(x2), uv, uv + 2, uv + 4, s, t)) return 0;
+    put(dPds, s);
+    put(dPdt, t);
+    return 1;
+}
+// shade::localHemisphereCosineDifferentials (shade_math.h:279-288).
+void ref_hemisphere_differentials(float r1, float r2, float out[6]) {
+    V3 a, b;
+    ii::shade::localHemisphereCosineDifferentials(r1, r2, a, b);
+    put(out, a);
+    put(out + 3, b);
+}
+// shade::scatterDifferentials (shade_material.h:691-705) for a one-lobe Bsdf (makeLobe's); the hit's state: its
+// incoming differentials (12 floats), dPdx and dPdy, diffValid, P and wo. out: dOx, dOy, dDx, dDy; returns valid.
+int ref_scatter_differentials(uint32_t kind, const float N[3], float roughness, const float incoming[12], const float dP[6],
+                              int diffValid, const float P[3], const float wo[3], const float wi[3], float r1, float r2,
+                              const float childOrigin[3], float primaryScale, float out[12]) {
+    const uint32_t rec = makeLobe(kind, N, roughness);
+    ii::shade::State st{};
+    st.incoming.dOx = v3(incoming); st.incoming.dOy = v3(incoming + 3);
+    st.incoming.dDx = v3(incoming + 6); st.incoming.dDy = v3(incoming + 9);
+    st.incoming.valid = 1;
+    st.dPdx = v3(dP);
+    st.dPdy = v3(dP + 3);
+    st.diffValid = diffValid != 0;
+    st.P = v3(P);
+    st.wo = v3(wo);
+    const ii::shade::RayDiff d = ii::shade::scatterDifferentials(g_bsdf, rec, st, v3(wi), r1, r2, v3(childOrigin), primaryScale);
+    put(out, d.dOx); put(out + 3, d.dOy); put(out + 6, d.dDx); put(out + 9, d.dDy);
+    return int(d.valid);
 }
 }

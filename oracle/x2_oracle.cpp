@@ -236,6 +236,11 @@ bool cameraRay(const x2_config& c, float x, float y, Ray& out) {
     out = r;
     return true;
 }
+// The camera ray's auxiliary directions through the next pixel in x and in y (camera_core.h:222-224, :245-246).
+bool cameraDifferentials(const x2_config& c, float x, float y, V3& dx, V3& dy) {
+    const V3 dirX = transformH(c.raster_to_camera, {x + 1, y, -1.0f}), dirY = transformH(c.raster_to_camera, {x, y + 1, -1.0f});
+    return normalizeTo(vectorM(c.camera_to_render, dirX), dx) && normalizeTo(vectorM(c.camera_to_render, dirY), dy);
+}
 
 // ---------------------------------------------------------------------------------------------------------------
 // The watertight ray/triangle test (S. Woop, C. Benthin, I. Wald, "Watertight Ray/Triangle Intersection", JCGT 2013):
@@ -343,11 +348,100 @@ struct BNode {
     float lo[3], hi[3];
     U32 a, n;  // n > 0: leaf of triangles order[a, a+n); n == 0: children a and a + 1
 };
-struct Texture {
-    U32 width, height;
-    std::vector<float> texels;  // RGB, row-major
-    size_t gradOffset;          // in the adjoint's concatenated gradient
+// ---------------------------------------------------------------------------------------------------------------
+// Textures: mip pyramids and the path tracer's map lookups. The filters are texture_core.h:86-206's, which adapt
+// OpenImageIO 3.1.17's texturesys.cpp (Apache-2.0, copyright Contributors to the OpenImageIO project): the
+// anisotropic filter for camera paths, trilinear closest texels for glossy ones, one level's closest texel for
+// diffuse ones (shade_mapnodes.h:344-347), read as MoonRay's ImageMap reads a map (shade_mapnodes.h:420-470: unit
+// scale, no offset or bias, periodic, the t flip, and the footprint's cross terms swapped as XPU's vector shading
+// passes them, async_cpu.h:928). Two deviations, applied alike on the device: theta's sine and cosine come from the
+// shared polynomial, and hypot(A - C, B) is the square root of the FP64 sum of squares.
+constexpr U32 MipMax = 16;
+struct MipLevel {
+    U32 width = 0, height = 0;
+    std::vector<float> texels;  // rgb, row-major
 };
+struct Texture {
+    U32 width = 0, height = 0;   // level 0
+    std::vector<MipLevel> levels;
+    bool half = false;           // every level rounded to half floats (the paged tiles' values)
+    U32 coarseFrom = MipMax;     // a fetch at a finer level reads its ancestor at this level (partial shading
+                                 // with no tile resident: every parent read from the always-resident levels)
+    size_t gradOffset = 0;       // level 0's texels in the adjoint's concatenated gradient
+    size_t levelGrad[MipMax] = {};  // each level's texels in the adjoint's pyramid gradient (floats)
+};
+float roundHalf(float v) {  // to the nearest IEEE binary16 (ties to even) and back
+    U32 x;
+    std::memcpy(&x, &v, 4);
+    const U32 sign = (x >> 16) & 0x8000u, e = (x >> 23) & 0xffu;
+    U32 m = x & 0x7fffffu;
+    uint16_t h;
+    if (e == 0xff) h = uint16_t(sign | 0x7c00u | (m >> 13) | (m && !(m >> 13)));
+    else {
+        const int exp = int(e) - 127 + 15;
+        if (exp >= 31) h = uint16_t(sign | 0x7c00u);
+        else if (exp <= 0) {
+            if (exp < -10) h = uint16_t(sign);
+            else {
+                m |= 0x800000u;
+                const int shift = 14 - exp;
+                U32 q = m >> shift;
+                const U32 rest = m & ((1u << shift) - 1), halfway = 1u << (shift - 1);
+                if (rest > halfway || (rest == halfway && (q & 1u))) ++q;
+                h = uint16_t(sign | q);
+            }
+        } else {
+            U32 q = (U32(exp) << 10) | (m >> 13);
+            const U32 rest = m & 0x1fffu;
+            if (rest > 0x1000u || (rest == 0x1000u && (q & 1u))) ++q;
+            h = uint16_t(sign | q);
+        }
+    }
+    const U32 s = U32(h & 0x8000u) << 16, he = (h >> 10) & 31u, hm = h & 1023u;
+    if (he == 0) {
+        const float f = float(hm) * (1.0f / 16777216.0f);
+        return s ? -f : f;
+    }
+    const U32 u = s | (he == 31 ? 0x7F800000u | (hm << 13) : ((he + 112u) << 23) | (hm << 13));
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+// The pyramid of a texture (the reference reads a file's own levels, down to 1 x 1, host_texture.h:330-380; here
+// they are made): level k + 1 is max(1, w >> 1) x max(1, h >> 1), each texel the mean of the 2 x 2 block below it,
+// ((a + b) + (c + d)) * 0.25 with a, b the upper row, coordinates clamped to the level below. Paged textures then
+// round every level to half floats, as the reference holds a float file's levels (texture_core.h:28-31).
+void buildPyramid(Texture& t) {
+    t.levels.resize(1);
+    while (t.levels.size() < MipMax && (t.levels.back().width > 1 || t.levels.back().height > 1)) {
+        const MipLevel& in = t.levels.back();
+        MipLevel out;
+        out.width = std::max<U32>(1, in.width >> 1);
+        out.height = std::max<U32>(1, in.height >> 1);
+        out.texels.resize(size_t(out.width) * out.height * 3);
+        for (U32 y = 0; y < out.height; ++y)
+            for (U32 x = 0; x < out.width; ++x) {
+                const U32 x0 = std::min(2 * x, in.width - 1), x1 = std::min(2 * x + 1, in.width - 1);
+                const U32 y0 = std::min(2 * y, in.height - 1), y1 = std::min(2 * y + 1, in.height - 1);
+                for (int c = 0; c < 3; ++c) {
+                    const float a = in.texels[(size_t(y0) * in.width + x0) * 3 + c], b = in.texels[(size_t(y0) * in.width + x1) * 3 + c];
+                    const float d0 = in.texels[(size_t(y1) * in.width + x0) * 3 + c], d1 = in.texels[(size_t(y1) * in.width + x1) * 3 + c];
+                    out.texels[(size_t(y) * out.width + x) * 3 + c] = ((a + b) + (d0 + d1)) * 0.25f;
+                }
+            }
+        t.levels.push_back(std::move(out));
+    }
+    if (t.half)
+        for (MipLevel& l : t.levels)
+            for (float& v : l.texels) v = roundHalf(v);
+}
+void setLevel0(Texture& t, const float* texels) {
+    t.levels.assign(1, MipLevel{});
+    t.levels[0].width = t.width;
+    t.levels[0].height = t.height;
+    t.levels[0].texels.assign(texels, texels + size_t(t.width) * t.height * 3);
+    buildPyramid(t);
+}
 // A box-bounded brick grid (the medium; see the volume section below for how it is read).
 struct Volume {
     bool present = false, hasTemperature = false;
@@ -816,35 +910,484 @@ void buildBsdf(const x2_material& m, V3 N, Bsdf& b, V3 albedo, float evalRoughne
     b.selfEmission = load3(m.emission);
 }
 
-// ---------------------------------------------------------------------------------------------------------------
-// Textures (texture_core.h:144-153 bilinear, :179-181 lookup with unit scale and no offset; level 0, periodic).
-struct TexTap {
-    U32 tex = 0, x0 = 0, y0 = 0;
-    float fs = 0, ft = 0;
-    V3 color{0, 0, 0};
-};
 U32 wrapTexel(int x, U32 n) {
     const int m = x % int(n);
     return U32(m < 0 ? m + int(n) : m);
 }
-V3 texelAt(const Texture& t, U32 x, U32 y) { return load3(&t.texels[(size_t(y) * t.width + x) * 3]); }
-bool textureTap(const Texture& t, U32 id, float u, float v, TexTap& tap) {
-    const float su = 1.0f * u + 0.0f, tv = 1 - (1.0f * v + 0.0f);
-    float fs = su * float(t.width) - .5f, ft = tv * float(t.height) - .5f;
-    if (!finite(fs) || !finite(ft) || fabsf(fs) > 1.e8f || fabsf(ft) > 1.e8f) return false;
-    const int x = int(floorf(fs)), y = int(floorf(ft));
-    fs -= float(x);
-    ft -= float(y);
-    const U32 x0 = wrapTexel(x, t.width), x1 = wrapTexel(x + 1, t.width);
-    const U32 y0 = wrapTexel(y, t.height), y1 = wrapTexel(y + 1, t.height);
-    const V3 a = texelAt(t, x0, y0), b = texelAt(t, x1, y0), c = texelAt(t, x0, y1), d = texelAt(t, x1, y1);
-    tap.color = ((a * (1 - fs)) + (b * fs)) * (1 - ft) + ((c * (1 - fs)) + (d * fs)) * ft;
-    tap.tex = id;
-    tap.x0 = x0;
-    tap.y0 = y0;
-    tap.fs = fs;
-    tap.ft = ft;
-    return finite3(tap.color);
+V3 texelAt(const MipLevel& l, U32 x, U32 y) { return load3(&l.texels[(size_t(y) * l.width + x) * 3]); }
+// A texel of level `level`, or with coarseFrom set its ancestor there (the parent walk of async_gpu_device.h:52-58,
+// the coordinates halved a level at a time and clamped to the level read).
+V3 fetchTexel(const Texture& t, U32 level, U32 x, U32 y) {
+    if (level >= t.coarseFrom || t.coarseFrom >= t.levels.size()) return texelAt(t.levels[level], x, y);
+    U32 k = level;
+    while (k < t.coarseFrom) {
+        ++k;
+        x >>= 1;
+        y >>= 1;
+    }
+    const MipLevel& u = t.levels[k];
+    return texelAt(u, x < u.width ? x : u.width - 1, y < u.height ? y : u.height - 1);
+}
+// A map lookup's footprint (tex::Footprint: s, t and their screen derivatives) and mode (0 anisotropic, 1 trilinear
+// closest texels, 2 one level's closest texel).
+struct Footprint {
+    float u, v, dudx, dvdx, dudy, dvdy;
+};
+// texture_core.h:144-153: the four texels about (u, v) of a level, weighted bilinearly; each fetch wraps.
+bool bilinearAt(const Texture& tex, U32 level, float u, float v, V3& out) {
+    const MipLevel& m = tex.levels[level];
+    float s = u * float(m.width) - .5f, t = v * float(m.height) - .5f;
+    if (!finite(s) || !finite(t) || fabsf(s) > 1.e8f || fabsf(t) > 1.e8f) return false;
+    const int x = int(floorf(s)), y = int(floorf(t));
+    s -= float(x);
+    t -= float(y);
+    const U32 x0 = wrapTexel(x, m.width), x1 = wrapTexel(x + 1, m.width);
+    const U32 y0 = wrapTexel(y, m.height), y1 = wrapTexel(y + 1, m.height);
+    const V3 a = fetchTexel(tex, level, x0, y0), b = fetchTexel(tex, level, x1, y0), c = fetchTexel(tex, level, x0, y1),
+             d = fetchTexel(tex, level, x1, y1);
+    out = ((a * (1 - s)) + (b * s)) * (1 - t) + ((c * (1 - s)) + (d * s)) * t;
+    return true;
+}
+// texture_core.h:155-160: the texel nearest (u, v); an exact half-texel tie takes the lower one.
+bool closestAt(const MipLevel& m, float u, float v, U32& x, U32& y) {
+    const float s = u * float(m.width) - .5f, t = v * float(m.height) - .5f;
+    if (!finite(s) || !finite(t) || fabsf(s) > 1.e8f || fabsf(t) > 1.e8f) return false;
+    int ix = int(floorf(s)), iy = int(floorf(t));
+    if (s - float(ix) > .5f) ++ix;
+    if (t - float(iy) > .5f) ++iy;
+    x = wrapTexel(ix, m.width);
+    y = wrapTexel(iy, m.height);
+    return true;
+}
+// OpenImageIO's fast_atan2 and fast_exp (fmath.h), as texture_core.h:94-97 gives them.
+float oiioAtan2(float y, float x) {
+    const float a = fabsf(x), b = fabsf(y);
+    const float k = b == 0 ? 0.f : (a == b ? 1.f : (b > a ? a / b : b / a));
+    const float s = 1.f - (1.f - k), t = s * s;
+    float r = s * (0.430165678f * t + 1.f) / ((0.0579354987f * t + 0.763007998f) * t + 1.f);
+    if (b > a) r = 1.5707963267948966f - r;
+    if (copysignf(1.f, x) < 0) r = Pi - r;
+    return copysignf(r, y);
+}
+float oiioExp(float a) {
+    float x = fminf(fmaxf(a * 1.4426950408889634f, -126.f), 126.f);
+    const int m = int(x);
+    x -= float(m);
+    x = 1.f - (1.f - x);
+    float r = 1.33336498402e-3f;
+    r = x * r + 9.810352697968e-3f;
+    r = x * r + 5.551834031939e-2f;
+    r = x * r + 0.2401793301105f;
+    r = x * r + 0.693144857883f;
+    r = x * r + 1.f;
+    return ldexpf(r, m);
+}
+// The filter's plan (texture_core.h:161-206 up to the reads): the levels and their weights, and for the anisotropic
+// filter the taps along the major axis. Shared by the lookup and the adjoint's gradient scatter.
+struct TexPlan {
+    float s, t;
+    int levels[2];
+    float lw[2];
+    int n;           // taps a level (anisotropic), or 0: one closest texel a level
+    float sm, tm;    // the major axis' step
+    float pos[7], weight[7];
+};
+// swap: the cross terms swapped (the path tracer's maps); the prototype estimator's binding passes them in order.
+bool planLookup(const Texture& tex, U32 mode, const Footprint& f, TexPlan& p, bool swap = true) {
+    const U32 count = U32(tex.levels.size());
+    if (!count || count > MipMax) return false;
+    float s = f.u, t = 1 - f.v;
+    float dx = f.dudx, tx = -f.dvdx, dy = f.dudy, ty = -f.dvdy;
+    if (swap) {  // swapCross (XPU's vector shading)
+        const float c = tx;
+        tx = dy;
+        dy = c;
+    }
+    if (!finite(s) || !finite(t) || !finite(dx) || !finite(tx) || !finite(dy) || !finite(ty)) return false;
+    if (fabsf(s) > 1024) s = s - floorf(s);
+    if (fabsf(t) > 1024) t = t - floorf(t);
+    p.s = s;
+    p.t = t;
+    p.levels[0] = p.levels[1] = -1;
+    p.lw[0] = p.lw[1] = 0;
+    p.n = 0;
+    p.sm = p.tm = 0;
+    if (mode != 0) {  // texture_core.h:162-177
+        const float eps = 1.0e-8f, eps2 = eps * eps;
+        const float dxlen2 = dx * dx + tx * tx, dylen2 = dy * dy + ty * ty;
+        if (dxlen2 < eps2) {
+            if (dylen2 < eps2) {
+                dx = eps;
+                dy = 0;
+                tx = 0;
+                ty = eps;
+            } else {
+                const float sc = eps / sqrtf(dylen2);
+                dx = ty * sc;
+                tx = -dy * sc;
+            }
+        } else if (dylen2 < eps2) {
+            const float sc = eps / sqrtf(dxlen2);
+            dy = -tx * sc;
+            ty = dx * sc;
+        }
+        const float filt = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fmaxf(fabsf(tx), fabsf(ty)));
+        for (U32 m = 0; m < count; ++m) {
+            const MipLevel& l = tex.levels[m];
+            const float w = filt * float(l.width < l.height ? l.width : l.height);
+            if (w <= 1) {
+                p.levels[0] = int(m) - 1;
+                p.levels[1] = int(m);
+                p.lw[1] = fminf(fmaxf(2 * w - 1, 0.f), 1.f);
+                p.lw[0] = 1 - p.lw[1];
+                break;
+            }
+        }
+        if (p.levels[1] < 0) {
+            p.levels[0] = p.levels[1] = int(count) - 1;
+            p.lw[0] = 1;
+            p.lw[1] = 0;
+        } else if (p.levels[0] < 0) {
+            p.levels[0] = p.levels[1] = 0;
+            p.lw[0] = 1;
+            p.lw[1] = 0;
+        } else if (mode == 2) {
+            p.levels[0] = p.levels[1];
+            p.lw[0] = 1;
+            p.lw[1] = 0;
+        }
+        return true;
+    }
+    // The anisotropic filter (texture_core.h:187-206).
+    const double A = double(tx * tx + ty * ty), B = -2.0 * double(dx * tx + dy * ty), C = double(dx * dx + dy * dy);
+    const double root = sqrt((A - C) * (A - C) + B * B);
+    float major = fminf(sqrtf(fmaxf(float((A + C + root) * .5), 0.f)), 1000.f);
+    float minor = fminf(sqrtf(fmaxf(float((A + C - root) * .5), 0.f)), 1000.f);
+    const float theta = oiioAtan2(float(B), float(A - C)) * .5f + Pi * .5f;
+    float aspect = minor > 0 ? fminf(fmaxf(major / minor, 1.f), 1.e6f) : (major > 0 ? 1.e6f : 1.f);
+    if (aspect > 4) {
+        aspect = 4;
+        major = .5f * (major + minor * 4);
+        minor = major / 4;
+    }
+    for (U32 m = 0; m < count; ++m) {
+        const MipLevel& l = tex.levels[m];
+        const float w = minor * float(l.width < l.height ? l.width : l.height);
+        if (w <= 1) {
+            p.levels[0] = int(m) - 1;
+            p.levels[1] = int(m);
+            p.lw[1] = fminf(fmaxf(2 * w - 1, 0.f), 1.f);
+            p.lw[0] = 1 - p.lw[1];
+            break;
+        }
+    }
+    if (p.levels[1] < 0) {
+        p.levels[0] = p.levels[1] = int(count) - 1;
+        p.lw[0] = 1;
+        p.lw[1] = 0;
+    } else if (p.levels[0] < 0) {
+        p.levels[0] = p.levels[1] = 0;
+        p.lw[0] = 1;
+        p.lw[1] = 0;
+        const MipLevel& l = tex.levels[0];
+        const U32 r = l.width > l.height ? l.width : l.height;
+        if (minor * float(r) < .5f) aspect = fminf(fmaxf(major * float(r) * 2, 1.f), 4.f);
+    }
+    int n = int(2 * aspect - 1);
+    if (n < 1) n = 1;
+    if (n > 7) return false;
+    const float LL = 2 * (major - minor);
+    float sn, cs;
+    sincosPoly(theta, sn, cs);
+    p.sm = cs * LL * .5f;
+    p.tm = sn * LL * .5f;
+    p.n = n;
+    if (n == 1) {
+        p.weight[0] = 1;
+        p.pos[0] = 0;
+    } else if (n == 2) {
+        p.weight[0] = p.weight[1] = .5f;
+        p.pos[0] = -.5f;
+        p.pos[1] = .5f;
+    } else {
+        float sum = 0;
+        const float scale = major / LL;
+        for (int i = 0; i < n; ++i) {
+            p.pos[i] = 2 * (float(i) + .5f) * (1.f / float(n)) - 1;
+            const float x = p.pos[i] * scale;
+            p.weight[i] = oiioExp(-2 * x * x);
+            sum += p.weight[i];
+        }
+        for (int i = 0; i < n; ++i) p.weight[i] /= sum;
+    }
+    return true;
+}
+// The map's value (texture_core.h:161-206 after the plan): false, an invalid value (the sample is an Error).
+bool mapLookup(const Texture& tex, U32 mode, const Footprint& f, V3& out, bool swap = true) {
+    TexPlan p;
+    if (!planLookup(tex, mode, f, p, swap)) return false;
+    V3 result{0, 0, 0};
+    for (int k = 0; k < 2; ++k) {
+        if (p.lw[k] == 0) continue;
+        const MipLevel& l = tex.levels[size_t(p.levels[k])];
+        if (p.n == 0) {
+            U32 x, y;
+            if (!closestAt(l, p.s, p.t, x, y)) return false;
+            const V3 value = fetchTexel(tex, U32(p.levels[k]), x, y);
+            if (!finite3(value)) return false;
+            result = result + value * p.lw[k];
+            continue;
+        }
+        V3 acc{0, 0, 0};
+        for (int i = 0; i < p.n; ++i) {
+            V3 value;
+            if (!bilinearAt(tex, U32(p.levels[k]), p.s + p.pos[i] * p.sm, p.t + p.pos[i] * p.tm, value)) return false;
+            acc = acc + value * p.weight[i];
+        }
+        result = result + acc * p.lw[k];
+    }
+    if (!finite3(result)) return false;
+    out = result;
+    return true;
+}
+// Every texel the lookup reads, with its weight in the value (the adjoint's d value / d texel): visit(level, x, y, w).
+template <class Visit>
+void mapWeights(const Texture& tex, const TexPlan& p, Visit&& visit) {
+    for (int k = 0; k < 2; ++k) {
+        if (p.lw[k] == 0) continue;
+        const U32 level = U32(p.levels[k]);
+        const MipLevel& l = tex.levels[level];
+        if (p.n == 0) {
+            U32 x, y;
+            if (closestAt(l, p.s, p.t, x, y)) visit(level, x, y, p.lw[k]);
+            continue;
+        }
+        for (int i = 0; i < p.n; ++i) {
+            float s = (p.s + p.pos[i] * p.sm) * float(l.width) - .5f, t = (p.t + p.pos[i] * p.tm) * float(l.height) - .5f;
+            if (!finite(s) || !finite(t) || fabsf(s) > 1.e8f || fabsf(t) > 1.e8f) continue;
+            const int x = int(floorf(s)), y = int(floorf(t));
+            s -= float(x);
+            t -= float(y);
+            const U32 x0 = wrapTexel(x, l.width), x1 = wrapTexel(x + 1, l.width);
+            const U32 y0 = wrapTexel(y, l.height), y1 = wrapTexel(y + 1, l.height);
+            const float w = p.weight[i] * p.lw[k];
+            visit(level, x0, y0, (1 - s) * (1 - t) * w);
+            visit(level, x1, y0, s * (1 - t) * w);
+            visit(level, x0, y1, (1 - s) * t * w);
+            visit(level, x1, y1, s * t * w);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Ray differentials (shade_state.h:30-33 RayDiff; shade_material.h:388-505 the hit's texture derivatives and the
+// transfer; shade_material.h:691-705 and shade_lobes_eval.h:725-741 the scattered ones), for Lambert and GGX lobes
+// on flat triangles (no vertex normals: dN/dx = dN/dy = 0).
+struct RayDiff {
+    V3 dOx{0, 0, 0}, dOy{0, 0, 0}, dDx{0, 0, 0}, dDy{0, 0, 0};
+    U32 valid = 0;
+};
+float differenceOfProducts(float a, float b, float c, float d) {  // Kahan's, shade_material.h:96-101
+    const float cd = c * d;
+    const float err = fmaf(-c, d, cd);
+    const float dop = fmaf(a, b, -cd);
+    return dop + err;
+}
+// shade_material.h:103-115: dP/ds and dP/dt of a triangle from its corners and texture coordinates.
+bool trianglePartials(V3 x0, V3 x1, V3 x2, const float uv0[2], const float uv1[2], const float uv2[2], V3& dPds, V3& dPdt) {
+    const float A[4] = {uv1[0] - uv0[0], uv1[1] - uv0[1], uv2[0] - uv0[0], uv2[1] - uv0[1]};
+    const V3 b0 = x1 - x0, b1 = x2 - x0;
+    const float det = differenceOfProducts(A[0], A[3], A[1], A[2]);
+    if (!(fabsf(det) >= 1.17549435e-38f) || !(fabsf(det) <= 3.402823466e38f)) return false;
+    const float inv = 1.0f / det;
+    dPds = V3{differenceOfProducts(b0.x, A[3], b1.x, A[1]), differenceOfProducts(b0.y, A[3], b1.y, A[1]),
+              differenceOfProducts(b0.z, A[3], b1.z, A[1])} * inv;
+    dPdt = V3{differenceOfProducts(b1.x, A[0], b0.x, A[2]), differenceOfProducts(b1.y, A[0], b0.y, A[2]),
+              differenceOfProducts(b1.z, A[0], b0.z, A[2])} * inv;
+    return finite3(dPds) && finite3(dPdt);
+}
+V3 normalized(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }  // shade::normalize
+// What a hit knows of its footprint: the texture coordinates' derivatives, dP/dx and dP/dy, whether they hold,
+// and the incoming differentials (the scattered ones start from them).
+struct HitDiff {
+    float dsdx = 0, dtdx = 0, dsdy = 0, dtdy = 0;
+    V3 dPdx{0, 0, 0}, dPdy{0, 0, 0};
+    bool valid = false;
+    RayDiff incoming;
+};
+// shade_material.h:392-432 (hitTexDerivatives): a camera ray's auxiliary rays (directions dx, dy; a pinhole's
+// offsets are zero) meet the triangle's plane; their barycentric offsets map through the texture coordinates.
+void hitTexDerivatives(V3 a, V3 b, V3 c, const float st[6], const Ray& ray, V3 camDx, V3 camDy, U32 spp, HitDiff& h) {
+    h.dsdx = h.dsdy = h.dtdx = h.dtdy = 0.0f;
+    if (isExactlyZero(camDx) || isExactlyZero(camDy) || !spp) return;
+    const V3 e1 = b - a, e2 = c - a, n = cross(e1, e2);
+    U32 axis = fabsf(n.x) > fabsf(n.y) ? 0 : 1;
+    if (fabsf(n.z) > (axis == 0 ? fabsf(n.x) : fabsf(n.y))) axis = 2;
+    auto pa = [&](V3 v) { return axis == 0 ? v.y : axis == 1 ? v.z : v.x; };
+    auto pb = [&](V3 v) { return axis == 0 ? v.z : axis == 1 ? v.x : v.y; };
+    const float det = pa(e1) * pb(e2) - pb(e1) * pa(e2), den = dot(n, ray.direction);
+    if (!(fabsf(det) > 0) || !(fabsf(den) > 0)) return;
+    const float plane = dot(n, a - ray.origin);
+    const V3 primary = ray.direction * (plane / den);
+    const float scale = 1.f / fminf(sqrtf(float(spp)), 2.f);
+    const V3 directions[2] = {camDx, camDy};
+    float ds[2], dt[2];
+    for (int i = 0; i < 2; ++i) {
+        V3 d = directions[i];
+        if (scale < 1) d = normalized(ray.direction + (d - ray.direction) * scale);
+        const float dd = dot(n, d);
+        if (!(fabsf(dd) > 0)) return;
+        const V3 o = V3{0, 0, 0} * scale;
+        const V3 delta = o + d * (dot(n, a - ray.origin - o) / dd) - primary;
+        const float x = (pa(delta) * pb(e2) - pb(delta) * pa(e2)) / det, y = (pa(e1) * pb(delta) - pb(e1) * pa(delta)) / det;
+        ds[i] = (st[2] - st[0]) * x + (st[4] - st[0]) * y;
+        dt[i] = (st[3] - st[1]) * x + (st[5] - st[1]) * y;
+    }
+    if (!(finite(ds[0]) && finite(ds[1]) && finite(dt[0]) && finite(dt[1]))) return;
+    h.dsdx = ds[0];
+    h.dtdx = dt[0];
+    h.dsdy = ds[1];
+    h.dtdy = dt[1];
+}
+// shade_material.h:436-451 (transferDifferentials): the incoming auxiliary rays meet the hit's plane.
+void transferDifferentials(const RayDiff& d, V3 origin, V3 dir, V3 Ng, V3 P, HitDiff& h) {
+    h.dPdx = h.dPdy = V3{0, 0, 0};
+    h.valid = false;
+    if (!d.valid) return;
+    const V3 oX = origin + d.dOx, oY = origin + d.dOy, dX = dir + d.dDx, dY = dir + d.dDy;
+    const float denomX = dot(Ng, dX), denomY = dot(Ng, dY);
+    if (denomX == 0.0f || denomY == 0.0f) return;
+    const float planeD = -dot(Ng, P);
+    const float distX = -(dot(oX, Ng) + planeD) / denomX, distY = -(dot(oY, Ng) + planeD) / denomY;
+    h.dPdx = (oX + dX * distX) - P;
+    h.dPdy = (oY + dY * distY) - P;
+    h.valid = finite3(h.dPdx) && finite3(h.dPdy);
+    if (!h.valid) h.dPdx = h.dPdy = V3{0, 0, 0};
+}
+bool isZeroE(float x, float e) { return fabsf(x) <= e; }
+// shade_material.h:454-495 (differentialTexDerivatives): the texture derivatives from dP/dx and dP/dy over dP/ds and
+// dP/dt; without differentials, the coarsest level (1024).
+void differentialTexDerivatives(V3 dPds, V3 dPdt, V3 Ng, HitDiff& h) {
+    h.dsdx = h.dtdx = h.dsdy = h.dtdy = 0.0f;
+    if (!h.valid) {
+        h.dsdx = 1024.0f;
+        h.dtdy = 1024.0f;
+        return;
+    }
+    const float e = 1e-5f;
+    if (isZeroE(dot(dPds, dPdt), e)) {
+        const float du = dot(dPds, dPds), dv = dot(dPdt, dPdt);
+        if (du > e && dv > e) {
+            const float iu = 1.0f / dot(dPds, dPds), iv = 1.0f / dot(dPdt, dPdt);
+            h.dsdx = dot(h.dPdx, dPds) * iu;
+            h.dtdx = dot(h.dPdx, dPdt) * iv;
+            h.dsdy = dot(h.dPdy, dPds) * iu;
+            h.dtdy = dot(h.dPdy, dPdt) * iv;
+        }
+        return;
+    }
+    const V3 n2 = Ng * Ng;
+    float A[4], bx[2], by[2];
+    if (n2.x > n2.y && n2.x > n2.z) {
+        A[0] = dPds.y; A[1] = dPdt.y; A[2] = dPds.z; A[3] = dPdt.z;
+        bx[0] = h.dPdx.y; bx[1] = h.dPdx.z; by[0] = h.dPdy.y; by[1] = h.dPdy.z;
+    } else if (n2.y > n2.z) {
+        A[0] = dPds.x; A[1] = dPdt.x; A[2] = dPds.z; A[3] = dPdt.z;
+        bx[0] = h.dPdx.x; bx[1] = h.dPdx.z; by[0] = h.dPdy.x; by[1] = h.dPdy.z;
+    } else {
+        A[0] = dPds.x; A[1] = dPdt.x; A[2] = dPds.y; A[3] = dPdt.y;
+        bx[0] = h.dPdx.x; bx[1] = h.dPdx.y; by[0] = h.dPdy.x; by[1] = h.dPdy.y;
+    }
+    const float det = A[0] * A[3] - A[1] * A[2];
+    if (isZeroE(det, e)) return;
+    const float detA = differenceOfProducts(A[0], A[3], A[1], A[2]);
+    if (!(fabsf(detA) >= 1.17549435e-38f) || !(fabsf(detA) <= 3.402823466e38f)) return;
+    const float inv = 1.0f / detA;
+    h.dsdx = differenceOfProducts(bx[0], A[3], bx[1], A[1]) * inv;
+    h.dtdx = differenceOfProducts(bx[1], A[0], bx[0], A[2]) * inv;
+    h.dsdy = differenceOfProducts(by[0], A[3], by[1], A[1]) * inv;
+    h.dtdy = differenceOfProducts(by[1], A[0], by[0], A[2]) * inv;
+}
+// The hit's differentials (shade_material.h:530-558 hitDifferentials for a mesh without subdivision derivatives):
+// a camera ray's texture derivatives by hitTexDerivatives, its auxiliary rays scaled for the samples per pixel as
+// the incoming differentials; any other ray's transferred and turned into texture derivatives.
+void hitDifferentials(V3 a, V3 b, V3 c, const float st[6], V3 N, V3 Ng, V3 P, const Ray& ray, bool camera, V3 camDx,
+                      V3 camDy, const RayDiff& incoming, U32 spp, HitDiff& h) {
+    if (camera && !isExactlyZero(camDx) && !isExactlyZero(camDy) && spp) {
+        hitTexDerivatives(a, b, c, st, ray, camDx, camDy, spp, h);
+        const float scale = 1.f / fminf(sqrtf(float(spp)), 2.f);
+        RayDiff d;
+        d.dOx = V3{0, 0, 0} * scale;
+        d.dOy = V3{0, 0, 0} * scale;
+        d.dDx = normalized(ray.direction + (camDx - ray.direction) * scale) - ray.direction;
+        d.dDy = normalized(ray.direction + (camDy - ray.direction) * scale) - ray.direction;
+        d.valid = 1;
+        h.incoming = d;
+        transferDifferentials(d, ray.origin, ray.direction, Ng, P, h);
+        return;
+    }
+    h.incoming = incoming;
+    transferDifferentials(incoming, ray.origin, ray.direction, Ng, P, h);
+    V3 dPds, dPdt;
+    if (!trianglePartials(a, b, c, st, st + 2, st + 4, dPds, dPdt)) {
+        const Frame f = frameOf(N);
+        dPds = f.x;
+        dPdt = f.y;
+    }
+    differentialTexDerivatives(dPds, dPdt, Ng, h);
+}
+// shade_math.h:279-288: a cosine-sampled direction's differentials in the lobe's frame (r1 clamped to 85 degrees
+// from the normal; the polynomial's sine and cosine of phi).
+void localHemisphereCosineDifferentials(float r1, float r2, V3& dDdr1, V3& dDdr2) {
+    const float c85 = 0.08715580403804779f;  // cosf(85 * (pi / 180)) in binary32
+    const float minR1 = c85 * c85, maxR1 = 1.0f - minR1;
+    r1 = fminf(fmaxf(r1, minR1), maxR1);
+    const float cosTheta = r1 > 1.0f ? 1.0f : sqrtf(r1), sinTheta = sqrtf(1.0f - r1);
+    float sinPhi, cosPhi;
+    sincosPoly(r2 * TwoPi, sinPhi, cosPhi);
+    const float scale = 1.0f / 8.0f, halfScale = 0.5f * scale, twoPiScale = TwoPi * scale;
+    dDdr1 = {-halfScale * cosPhi / sinTheta, -halfScale * sinPhi / sinTheta, halfScale / cosTheta};
+    dDdr2 = {-twoPiScale * sinPhi * sinTheta, twoPiScale * cosPhi * sinTheta, 0.0f};
+}
+// shade_math.h:185-189.
+void squarifyRectangle(V3& e1, V3& e2) {
+    const float l1 = length(e1), l2 = length(e2), l = sqrtf(l1 * l2);
+    e1 = e1 * (l / l1);
+    e2 = e2 * (l / l2);
+}
+// A scattered ray's differentials (shade_material.h:691-705 with shade_lobes_eval.h:725-741): Lambert's from the
+// cosine sample (it ignores the incoming ones, so dP/dx is scaled by the primary scale), GGX's the incoming ones
+// reflected about the half vector (shade_math.h:203-207) and widened by the lobe's dDFactor, 1 + 7 alpha
+// (shade_lobes.h:479, :548). Invalid where the hit's are.
+RayDiff scatterDifferentials(bool lambert, const Frame& frame, float alpha, const HitDiff& h, V3 P, V3 wo, V3 wi,
+                             float r1, float r2, V3 childOrigin, float primaryScale) {
+    RayDiff d;
+    if (!h.valid) return d;
+    V3 dDdx = h.incoming.dDx, dDdy = h.incoming.dDy;
+    if (lambert) {
+        localHemisphereCosineDifferentials(r1, r2, dDdx, dDdy);
+        squarifyRectangle(dDdx, dDdy);
+        dDdx = localToGlobal(frame, dDdx);
+        dDdy = localToGlobal(frame, dDdy);
+    } else {
+        V3 H;
+        if (!halfVector(wo, wi, H)) H = frame.z;
+        const V3 dNdx{0, 0, 0}, dNdy{0, 0, 0};
+        const float Cx = dot(dDdx, H) - dot(wo, dNdx), Cy = dot(dDdy, H) - dot(wo, dNdy), dDotN = -dot(wo, H);
+        dDdx = dDdx - (2.0f * (dDotN * dNdx + Cx * H));
+        dDdy = dDdy - (2.0f * (dDotN * dNdy + Cy * H));
+        const float factor = 1.0f + alpha * 7.0f;
+        dDdx = dDdx * factor;
+        dDdy = dDdy * factor;
+    }
+    const float scale = lambert ? primaryScale : 1.0f;
+    const V3 shift = P - childOrigin;
+    d.dOx = h.dPdx * scale + shift;
+    d.dOy = h.dPdy * scale + shift;
+    d.dDx = normalized(wi + dDdx * scale) - wi;
+    d.dDy = normalized(wi + dDdy * scale) - wi;
+    d.valid = finite3(d.dOx) && finite3(d.dOy) && finite3(d.dDx) && finite3(d.dDy) ? 1u : 0u;
+    return d;
 }
 // d ln f / d r of the GGX lobe for MoonRay's input roughness r (alpha = r^2, A = alpha^2 = r^4; 0 where alpha is
 // clamped): D4 ~ A / (A + tan^2 theta_m)^2, G1 = 2 / (1 + sqrt(1 + A tan^2 theta)) for wo and wi.
@@ -867,10 +1410,13 @@ float ggxRoughnessLogDerivative(float r, V3 N, V3 wo, V3 wi) {
 }
 // The adjoint's gradient sinks and open subtrees (the device's bookkeeping, accumulated here in FP64).
 enum : U32 { SinkNone = 0, SinkTexture = 1, SinkRoughness = 2 };
+// A texture sink keeps its lookup (texture, mode, footprint) and value; the scatter walks the lookup's texels again.
 struct Sink {
     U32 kind = SinkNone, material = 0;
     float g = 0;
-    TexTap tap;
+    U32 tex = 0, mode = 0;
+    Footprint f{0, 0, 0, 0, 0, 0};
+    V3 color{0, 0, 0};
 };
 struct Record {
     U32 pos;
@@ -879,25 +1425,48 @@ struct Record {
 };
 struct Adjoint {
     const float* dImage = nullptr;  // per pixel rgb
-    double* gradTextures = nullptr;
+    double* gradTextures = nullptr;  // level 0 of every texture
+    double* gradLevels = nullptr;    // the levels above (Texture::levelGrad), pushed down to level 0 at the end
     double* gradRoughness = nullptr;
 };
-void scatterTexture(const Scene& S, const Adjoint& A, const TexTap& t, V3 R, V3 delta) {
-    const Texture& d = S.textures[t.tex];
-    const float cs[3] = {t.color.x > 0.0f ? delta.x * R.x / t.color.x : 0.0f,
-                         t.color.y > 0.0f ? delta.y * R.y / t.color.y : 0.0f,
-                         t.color.z > 0.0f ? delta.z * R.z / t.color.z : 0.0f};
+// The contribution R of a lobe whose albedo the map scales has d R / d value = R / value (per channel), and the
+// value is linear in the texels the lookup reads (mapWeights).
+void scatterTexture(const Scene& S, const Adjoint& A, const Sink& s, V3 R, V3 delta) {
+    const Texture& d = S.textures[s.tex];
+    const float cs[3] = {s.color.x > 0.0f ? delta.x * R.x / s.color.x : 0.0f,
+                         s.color.y > 0.0f ? delta.y * R.y / s.color.y : 0.0f,
+                         s.color.z > 0.0f ? delta.z * R.z / s.color.z : 0.0f};
     if (cs[0] == 0.0f && cs[1] == 0.0f && cs[2] == 0.0f) return;
-    const U32 x1 = t.x0 + 1 == d.width ? 0 : t.x0 + 1, y1 = t.y0 + 1 == d.height ? 0 : t.y0 + 1;
-    const U32 xs[4] = {t.x0, x1, t.x0, x1}, ys[4] = {t.y0, t.y0, y1, y1};
-    const float ws[4] = {(1 - t.fs) * (1 - t.ft), t.fs * (1 - t.ft), (1 - t.fs) * t.ft, t.fs * t.ft};
-    for (int k = 0; k < 4; ++k) {
-        double* q = A.gradTextures + d.gradOffset + (size_t(ys[k]) * d.width + xs[k]) * 3;
-        for (int c = 0; c < 3; ++c) q[c] += double(cs[c] * ws[k]);
+    TexPlan p;
+    if (!planLookup(d, s.mode, s.f, p)) return;
+    mapWeights(d, p, [&](U32 level, U32 x, U32 y, float w) {
+        const MipLevel& l = d.levels[level];
+        double* q = (level == 0 ? A.gradTextures + d.gradOffset : A.gradLevels + d.levelGrad[level]) +
+                    (size_t(y) * l.width + x) * 3;
+        for (int c = 0; c < 3; ++c) q[c] += double(cs[c] * w);
+    });
+}
+// The pyramid's transpose: each level's texel gradients go to the 2 x 2 texels below that made it (buildPyramid),
+// a quarter each, from the coarsest level down to level 0.
+void pushDownPyramid(const Texture& t, double* level0, double* levels) {
+    for (size_t k = t.levels.size() - 1; k >= 1; --k) {
+        const MipLevel& up = t.levels[k];
+        const MipLevel& in = t.levels[k - 1];
+        const double* g = levels + t.levelGrad[k];
+        double* below = k == 1 ? level0 : levels + t.levelGrad[k - 1];
+        for (U32 y = 0; y < up.height; ++y)
+            for (U32 x = 0; x < up.width; ++x) {
+                const U32 xs[2] = {std::min(2 * x, in.width - 1), std::min(2 * x + 1, in.width - 1)};
+                const U32 ys[2] = {std::min(2 * y, in.height - 1), std::min(2 * y + 1, in.height - 1)};
+                for (int j = 0; j < 2; ++j)
+                    for (int i = 0; i < 2; ++i)
+                        for (int c = 0; c < 3; ++c)
+                            below[(size_t(ys[j]) * in.width + xs[i]) * 3 + c] += 0.25 * g[(size_t(y) * up.width + x) * 3 + c];
+            }
     }
 }
 void scatterSink(const Scene& S, const Adjoint& A, const Sink& s, V3 R, V3 delta) {
-    if (s.kind == SinkTexture) scatterTexture(S, A, s.tap, R, delta);
+    if (s.kind == SinkTexture) scatterTexture(S, A, s, R, delta);
     else if (s.kind == SinkRoughness)
         A.gradRoughness[s.material] += double((delta.x * R.x + delta.y * R.y + delta.z * R.z) * s.g);
 }
@@ -914,6 +1483,7 @@ struct Pending {
     float heldFar;
     V3 held;
     U32 volumeDepth;  // the volume scattering the path has been through (async_integrator.h:210)
+    RayDiff diff;     // its differentials (async_integrator.h:2985), in scenes with textures
 };
 constexpr U32 MaxPending = 32, PendingReserve = 16;  // async_integrator.h:250-267
 float bounceThinning(U32 top, U32 planned) {
@@ -1473,6 +2043,13 @@ int integrateDirect(const Scene& S, const x2_config& c, U32 pixel, U32 index, do
         return 2;
     Ray ray;
     if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, ray)) return 2;
+    // GpuBackend::material (async_gpu_device.h:356-383): the map read through the default binding (anisotropic,
+    // cross terms in order) at the footprint the hit ray's differentials give; only the camera ray has them, so the
+    // bounces read a zero footprint (level 0).
+    V3 camDx{0, 0, 0}, camDy{0, 0, 0};
+    if (!S.textures.empty() &&
+        !cameraDifferentials(c, float(pixel % c.width) + x, float(pixel / c.width) + y, camDx, camDy))
+        return 2;
     double radiance[3] = {0, 0, 0}, throughput[3] = {1, 1, 1};
     U32 depth = 0, primaryHit = 0;
     const V3 sky = load3(c.sky);
@@ -1496,11 +2073,15 @@ int integrateDirect(const Scene& S, const x2_config& c, U32 pixel, U32 index, do
             if (!S.triUV.empty())
                 for (int q = 0; q < 6; ++q) uv[q] = S.triUV[size_t(tri) * 6 + q];
             const float w = 1.0f - hit.u - hit.v;
-            TexTap tap;
-            if (!textureTap(S.textures[m.albedo_map - 1], m.albedo_map - 1, w * uv[0] + hit.u * uv[2] + hit.v * uv[4],
-                            w * uv[1] + hit.u * uv[3] + hit.v * uv[5], tap))
-                return 2;
-            albedo = albedo * tap.color;
+            HitDiff hd;
+            if (!depth)
+                hitTexDerivatives(S.tri[size_t(tri) * 3], S.tri[size_t(tri) * 3 + 1], S.tri[size_t(tri) * 3 + 2], uv, ray,
+                                  camDx, camDy, c.spp, hd);
+            const Footprint f{w * uv[0] + hit.u * uv[2] + hit.v * uv[4], w * uv[1] + hit.u * uv[3] + hit.v * uv[5],
+                              hd.dsdx, hd.dtdx, hd.dsdy, hd.dtdy};
+            V3 color;
+            if (!mapLookup(S.textures[m.albedo_map - 1], 0, f, color, false)) return 2;
+            albedo = albedo * color;
         }
         const U32 dimension = CameraDimensions + 2 * depth;
         V3 wi, f;
@@ -1552,6 +2133,12 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
         return 2;
     Ray camera;
     if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, camera)) return 2;
+    // Scenes with textures carry ray differentials (the map lookups' footprints): the camera's auxiliary directions,
+    // and MoonRay's primary scale for the samples per pixel (async_integrator.h:2029).
+    const bool textured = !S.textures.empty();
+    V3 camDx{0, 0, 0}, camDy{0, 0, 0};
+    if (textured && !cameraDifferentials(c, float(pixel % c.width) + x, float(pixel / c.width) + y, camDx, camDy)) return 2;
+    const float primaryScale = 1.0f / fminf(sqrtf(float(c.spp ? c.spp : 1)), 2.0f);
     Pending stack[MaxPending];
     U32 top = 0;
     {
@@ -1635,17 +2222,26 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
         const x2_material& mat = S.materials[S.material[tri]];
         V3 albedo = load3(mat.albedo);
         Sink texSink, roughSink;
-        if (mat.albedo_map) {  // the map at the hit's (s, t) (shade_material.h:121-160; cpu_jobs.h:46 apply)
-            float uv[6] = {0, 0, 1, 0, 1, 1};
-            if (!S.triUV.empty())
-                for (int q = 0; q < 6; ++q) uv[q] = S.triUV[size_t(tri) * 6 + q];
+        float uv[6] = {0, 0, 1, 0, 1, 1};
+        if (!S.triUV.empty())
+            for (int q = 0; q < 6; ++q) uv[q] = S.triUV[size_t(tri) * 6 + q];
+        HitDiff hd;
+        if (textured) hitDifferentials(a, b, cc, uv, N, Ng, P, ray, fromCamera, camDx, camDy, pv.diff, c.spp, hd);
+        if (mat.albedo_map) {  // the map at the hit's (s, t) (shade_material.h:121-160, shade_mapnodes.h:420-470)
             const float w = 1.0f - hit.u - hit.v;
             const float ss = w * uv[0] + hit.u * uv[2] + hit.v * uv[4], tt = w * uv[1] + hit.u * uv[3] + hit.v * uv[5];
-            TexTap tap;
-            if (!textureTap(S.textures[mat.albedo_map - 1], mat.albedo_map - 1, ss, tt, tap)) return 2;
-            albedo = albedo * tap.color;
+            // textureMode (shade_mapnodes.h:344-347): camera paths anisotropic, diffuse ones one level's closest
+            // texel, glossy and mirror ones trilinear closest texels.
+            const U32 mode = pv.diffuseDepth ? 2u : (pv.glossyDepth || pv.mirrorDepth) ? 1u : 0u;
+            const Footprint f{ss, tt, hd.dsdx, hd.dtdx, hd.dsdy, hd.dtdy};
+            V3 color;
+            if (!mapLookup(S.textures[mat.albedo_map - 1], mode, f, color)) return 2;
+            albedo = albedo * color;
             texSink.kind = SinkTexture;
-            texSink.tap = tap;
+            texSink.tex = mat.albedo_map - 1;
+            texSink.mode = mode;
+            texSink.f = f;
+            texSink.color = color;
         }
         if (mat.specular_weight > 0.0f && mat.roughness * mat.roughness > 0.001f) {
             roughSink.kind = SinkRoughness;
@@ -1747,7 +2343,7 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                         for (U32 q = 0; q < bsdf.count; ++q) {
                             const V3 share = tkq[q] * rrScale;
                             if (bsdf.lobes[q].kind == KindLambert && texSink.kind == SinkTexture)
-                                scatterTexture(S, *adj, texSink.tap, share, delta);
+                                scatterTexture(S, *adj, texSink, share, delta);
                             if (bsdf.lobes[q].kind == KindGGX && roughSink.kind == SinkRoughness)
                                 adj->gradRoughness[roughSink.material] +=
                                     double((delta.x * share.x + delta.y * share.y + delta.z * share.z) *
@@ -1884,6 +2480,9 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
                 child.nearT = 0.0f;
                 child.heldLight = ~0u;
                 child.volumeDepth = pv.volumeDepth + 1;  // (:3000)
+                if (textured)
+                    child.diff = scatterDifferentials(lobe.kind == KindLambert, lobe.frame, lobe.roughness, hd, P, wo, wi,
+                                                      r1, r2, origin, primaryScale);
                 if (held) {
                     child.heldLight = lightIndex;
                     child.heldFar = shadowFar;
@@ -1986,10 +2585,14 @@ void renderChunks(const Scene& S, const x2_config& c, const std::vector<x2_chunk
 void adjointFrame(const Scene& S, const x2_config& c, U32 threads, const float* dImage, double* gradTex, double* gradR,
                   U64 counters[3]) {
     const U32 tiles = ((c.width + 7) / 8) * ((c.height + 7) / 8);
-    size_t texFloats = 0;
-    for (const Texture& t : S.textures) texFloats += size_t(t.width) * t.height * 3;
+    size_t texFloats = 0, levelFloats = 0;
+    for (const Texture& t : S.textures) {
+        texFloats += size_t(t.width) * t.height * 3;
+        for (size_t k = 1; k < t.levels.size(); ++k) levelFloats += size_t(t.levels[k].width) * t.levels[k].height * 3;
+    }
     threads = std::max<U32>(1, threads);
     std::vector<std::vector<double>> gt(threads, std::vector<double>(texFloats, 0.0)), gr(threads, std::vector<double>(S.materials.size(), 0.0));
+    std::vector<std::vector<double>> gl(threads, std::vector<double>(levelFloats + 1, 0.0));
     std::atomic<U32> next{0};
     std::atomic<U64> traces{0}, shadows{0}, errors{0};
     auto worker = [&](U32 w) {
@@ -1998,6 +2601,7 @@ void adjointFrame(const Scene& S, const x2_config& c, U32 threads, const float* 
         Adjoint A;
         A.dImage = dImage;
         A.gradTextures = gt[w].data();
+        A.gradLevels = gl[w].data();
         A.gradRoughness = gr[w].data();
         for (;;) {
             const U32 tile = next.fetch_add(1);
@@ -2018,10 +2622,14 @@ void adjointFrame(const Scene& S, const x2_config& c, U32 threads, const float* 
     for (U32 t = 1; t < threads; ++t) pool.emplace_back(worker, t);
     worker(0);
     for (auto& t : pool) t.join();
-    for (U32 w = 0; w < threads; ++w) {
-        for (size_t i = 0; i < texFloats; ++i) gradTex[i] += gt[w][i];
-        for (size_t i = 0; i < S.materials.size(); ++i) gradR[i] += gr[w][i];
+    for (U32 w = 1; w < threads; ++w) {
+        for (size_t i = 0; i < texFloats; ++i) gt[0][i] += gt[w][i];
+        for (size_t i = 0; i < levelFloats; ++i) gl[0][i] += gl[w][i];
     }
+    for (const Texture& t : S.textures) pushDownPyramid(t, gt[0].data() + t.gradOffset, gl[0].data());
+    for (size_t i = 0; i < texFloats; ++i) gradTex[i] += gt[0][i];
+    for (U32 w = 0; w < threads; ++w)
+        for (size_t i = 0; i < S.materials.size(); ++i) gradR[i] += gr[w][i];
     counters[0] += traces;
     counters[1] += shadows;
     counters[2] += errors;
@@ -2161,14 +2769,19 @@ oracle_scene* oracle_scene_create(const x2_scene* in) {
             }
     }
     size_t gradOffset = 0;
+    size_t levelOffset = 0;
     for (U32 i = 0; i < in->texture_count; ++i) {
         const x2_texture& t = in->textures[i];
         Texture tex;
         tex.width = t.width;
         tex.height = t.height;
-        tex.texels.assign(t.texels, t.texels + size_t(t.width) * t.height * 3);
+        setLevel0(tex, t.texels);
         tex.gradOffset = gradOffset;
         gradOffset += size_t(t.width) * t.height * 3;
+        for (size_t k = 1; k < tex.levels.size(); ++k) {
+            tex.levelGrad[k] = levelOffset;
+            levelOffset += size_t(tex.levels[k].width) * tex.levels[k].height * 3;
+        }
         s.textures.push_back(std::move(tex));
     }
     s.materials.assign(in->materials, in->materials + in->material_count);
@@ -2224,9 +2837,15 @@ int oracle_adjoint(const oracle_scene* o, const x2_config* c, uint32_t threads, 
     adjointFrame(o->s, *c, threads, d_image, grad_textures, grad_roughness, counters);
     return 0;
 }
-void oracle_set_texture(oracle_scene* o, uint32_t texture, const float* texels) {
+void oracle_set_texture(oracle_scene* o, uint32_t texture, const float* texels) { setLevel0(o->s.textures.at(texture), texels); }
+void oracle_set_texture_coarse(oracle_scene* o, uint32_t texture, uint32_t from) {
+    o->s.textures.at(texture).coarseFrom = from;
+}
+void oracle_set_texture_half(oracle_scene* o, uint32_t texture, int half) {
     Texture& t = o->s.textures.at(texture);
-    t.texels.assign(texels, texels + size_t(t.width) * t.height * 3);
+    const std::vector<float> level0 = t.levels.at(0).texels;
+    t.half = half != 0;
+    setLevel0(t, level0.data());
 }
 void oracle_set_material(oracle_scene* o, uint32_t material, const x2_material* m) { o->s.materials.at(material) = *m; }
 void oracle_set_eval_roughness(oracle_scene* o, uint32_t material, float r) {
@@ -2248,12 +2867,102 @@ int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]) {
     Texture tex;
     tex.width = t->width;
     tex.height = t->height;
-    tex.texels.assign(t->texels, t->texels + size_t(t->width) * t->height * 3);
-    tex.gradOffset = 0;
-    TexTap tap;
-    if (!textureTap(tex, 0, u, v, tap)) return 0;
-    store3(rgb, tap.color);
+    tex.levels.resize(1);
+    tex.levels[0].width = t->width;
+    tex.levels[0].height = t->height;
+    tex.levels[0].texels.assign(t->texels, t->texels + size_t(t->width) * t->height * 3);
+    V3 out;
+    if (!bilinearAt(tex, 0, u, 1 - v, out) || !finite3(out)) return 0;
+    store3(rgb, out);
     return 1;
+}
+int oracle_map_lookup_swap(const x2_texture* t, int half, uint32_t mode, int swap, const float f[6], float rgb[3]) {
+    Texture tex;
+    tex.width = t->width;
+    tex.height = t->height;
+    tex.half = half != 0;
+    setLevel0(tex, t->texels);
+    V3 out;
+    if (!mapLookup(tex, mode, Footprint{f[0], f[1], f[2], f[3], f[4], f[5]}, out, swap != 0)) return 0;
+    store3(rgb, out);
+    return 1;
+}
+int oracle_map_lookup(const x2_texture* t, int half, uint32_t mode, const float f[6], float rgb[3]) {
+    Texture tex;
+    tex.width = t->width;
+    tex.height = t->height;
+    tex.half = half != 0;
+    setLevel0(tex, t->texels);
+    V3 out;
+    if (!mapLookup(tex, mode, Footprint{f[0], f[1], f[2], f[3], f[4], f[5]}, out)) return 0;
+    store3(rgb, out);
+    return 1;
+}
+void oracle_hit_tex_derivatives(const float a[3], const float b[3], const float c[3], const float uv[6],
+                                 const float org[3], const float dir[3], const float dx[3], const float dy[3],
+                                 uint32_t spp, float out[4]) {
+    HitDiff h;
+    const Ray ray{load3(org), load3(dir), 0.0f, 3.402823466e38f};
+    hitTexDerivatives(load3(a), load3(b), load3(c), uv, ray, load3(dx), load3(dy), spp, h);
+    out[0] = h.dsdx; out[1] = h.dtdx; out[2] = h.dsdy; out[3] = h.dtdy;
+}
+int oracle_transfer(const float diff[12], int valid, const float org[3], const float dir[3], const float Ng[3],
+                    const float P[3], const float dPds[3], const float dPdt[3], float out[10]) {
+    RayDiff d;
+    d.dOx = load3(diff); d.dOy = load3(diff + 3); d.dDx = load3(diff + 6); d.dDy = load3(diff + 9);
+    d.valid = valid ? 1u : 0u;
+    HitDiff h;
+    transferDifferentials(d, load3(org), load3(dir), load3(Ng), load3(P), h);
+    differentialTexDerivatives(load3(dPds), load3(dPdt), load3(Ng), h);
+    store3(out, h.dPdx);
+    store3(out + 3, h.dPdy);
+    out[6] = h.dsdx; out[7] = h.dtdx; out[8] = h.dsdy; out[9] = h.dtdy;
+    return h.valid ? 1 : 0;
+}
+int oracle_triangle_partials(const float x0[3], const float x1[3], const float x2[3], const float uv[6], float dPds[3],
+                             float dPdt[3]) {
+    V3 s, t;
+    if (!trianglePartials(load3(x0), load3(x1), load3(x2), uv, uv + 2, uv + 4, s, t)) return 0;
+    store3(dPds, s);
+    store3(dPdt, t);
+    return 1;
+}
+void oracle_hemisphere_differentials(float r1, float r2, float out[6]) {
+    V3 a, b;
+    localHemisphereCosineDifferentials(r1, r2, a, b);
+    store3(out, a);
+    store3(out + 3, b);
+}
+int oracle_scatter_differentials(uint32_t kind, const float N[3], float roughness, const float incoming[12],
+                                 const float dP[6], int diffValid, const float P[3], const float wo[3], const float wi[3],
+                                 float r1, float r2, const float childOrigin[3], float primaryScale, float out[12]) {
+    HitDiff h;
+    h.incoming.dOx = load3(incoming); h.incoming.dOy = load3(incoming + 3);
+    h.incoming.dDx = load3(incoming + 6); h.incoming.dDy = load3(incoming + 9);
+    h.incoming.valid = 1;
+    h.dPdx = load3(dP);
+    h.dPdy = load3(dP + 3);
+    h.valid = diffValid != 0;
+    const float alpha = fmaxf(0.001f, roughness * roughness);
+    const RayDiff d = scatterDifferentials(kind == 0, frameOf(load3(N)), alpha, h, load3(P), load3(wo), load3(wi), r1,
+                                           r2, load3(childOrigin), primaryScale);
+    store3(out, d.dOx); store3(out + 3, d.dOy); store3(out + 6, d.dDx); store3(out + 9, d.dDy);
+    return int(d.valid);
+}
+uint32_t oracle_pyramid(const x2_texture* t, int half, float* out, uint64_t capacity, uint32_t* sizes) {
+    Texture tex;
+    tex.width = t->width;
+    tex.height = t->height;
+    tex.half = half != 0;
+    setLevel0(tex, t->texels);
+    uint64_t at = 0;
+    for (size_t k = 0; k < tex.levels.size(); ++k) {
+        sizes[2 * k] = tex.levels[k].width;
+        sizes[2 * k + 1] = tex.levels[k].height;
+        for (float v : tex.levels[k].texels)
+            if (at < capacity) out[at++] = v;
+    }
+    return uint32_t(tex.levels.size());
 }
 float oracle_volume_density(const oracle_scene* o, const float p[3]) {
     return o->s.volume.present ? densityAt(o->s.volume, load3(p)) : 0.0f;

@@ -86,6 +86,11 @@ int oracle_adjoint(const oracle_scene* s, const x2_config* c, uint32_t threads, 
                    double* grad_textures, double* grad_roughness, uint64_t counters[3]);
 /* Parameter updates (the device's x2dev_set_texture / x2dev_set_material). */
 void oracle_set_texture(oracle_scene* s, uint32_t texture, const float* texels);
+/* Every level of a texture rounded to half floats (as paged tiles hold them), or kept in floats (0). */
+void oracle_set_texture_half(oracle_scene* s, uint32_t texture, int half);
+/* Every fetch at a level finer than `from` reads its ancestor at level `from` (a paged frame's first launch with
+ * partial shading and no tile resident); from >= the level count: off. */
+void oracle_set_texture_coarse(oracle_scene* s, uint32_t texture, uint32_t from);
 void oracle_set_material(oracle_scene* s, uint32_t material, const x2_material* m);
 // Test hook: the GGX lobe's value (not its sampling, pdf or MIS weights) uses roughness r for this material;
 // r < 0 restores the material's own.
@@ -93,6 +98,26 @@ void oracle_set_eval_roughness(oracle_scene* s, uint32_t material, float r);
 /* d ln f / d r of the GGX lobe, and one bilinear texture lookup at (u, v). */
 float oracle_ggx_roughness_log_derivative(float r, const float N[3], const float wo[3], const float wi[3]);
 int oracle_texture_lookup(const x2_texture* t, float u, float v, float rgb[3]);
+/* The path tracer's map lookup over the texture's pyramid (mode 0 anisotropic, 1 trilinear closest texels, 2 one
+ * level's closest texel) at footprint f = (s, t, ds/dx, dt/dx, ds/dy, dt/dy); half: levels rounded to half floats.
+ * 0: an invalid value. The pyramid itself: its levels' texels end to end, sizes[2k], sizes[2k+1] level k's width and
+ * height; returns the level count. */
+int oracle_map_lookup(const x2_texture* t, int half, uint32_t mode, const float f[6], float rgb[3]);
+/* The same with the cross terms swapped (1, the path tracer's maps) or in order (0, the prototype's binding). */
+int oracle_map_lookup_swap(const x2_texture* t, int half, uint32_t mode, int swap, const float f[6], float rgb[3]);
+uint32_t oracle_pyramid(const x2_texture* t, int half, float* out, uint64_t capacity, uint32_t* sizes);
+/* Ray differentials, each as the estimator uses it (the reference's functions are named where they are defined). */
+void oracle_hit_tex_derivatives(const float a[3], const float b[3], const float c[3], const float uv[6],
+                                 const float org[3], const float dir[3], const float dx[3], const float dy[3],
+                                 uint32_t spp, float out[4]);
+int oracle_transfer(const float diff[12], int valid, const float org[3], const float dir[3], const float Ng[3],
+                    const float P[3], const float dPds[3], const float dPdt[3], float out[10]);
+int oracle_triangle_partials(const float x0[3], const float x1[3], const float x2[3], const float uv[6], float dPds[3],
+                             float dPdt[3]);
+void oracle_hemisphere_differentials(float r1, float r2, float out[6]);
+int oracle_scatter_differentials(uint32_t kind, const float N[3], float roughness, const float incoming[12],
+                                 const float dP[6], int diffValid, const float P[3], const float wo[3], const float wi[3],
+                                 float r1, float r2, const float childOrigin[3], float primaryScale, float out[12]);
 /* The prototype estimator's Lambert sample (surface::lambert): 1 with (wi, f, pdf), 0 for an invalid input. */
 int oracle_surface_lambert(const float albedo[3], const float N[3], float u, float v, float wi[3], float f[3],
                            float* pdf);

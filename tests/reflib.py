@@ -35,6 +35,14 @@ def load():
         "ref_lobe_sample": (None, [u32, P(f32), P(f32), P(f32), f32, f32, f32, P(f32), P(f32), P(f32)]),
         "ref_bilinear": (c_int, [u32, u32, P(f32), f32, f32, P(f32)]),
         "ref_surface_lambert": (c_int, [P(f32), P(f32), f32, f32, P(f32), P(f32), P(f32)]),
+        "ref_map_lookup": (c_int, [u32, P(u32), P(f32), u32, u32, P(f32), P(f32)]),
+        "ref_footprint": (c_int, [P(f32)] * 7 + [f32, f32, u32, P(f32)]),
+        "ref_hit_tex_derivatives": (None, [P(f32)] * 8 + [u32, P(f32)]),
+        "ref_transfer": (c_int, [P(f32), c_int] + [P(f32)] * 7),
+        "ref_triangle_partials": (c_int, [P(f32)] * 6),
+        "ref_hemisphere_differentials": (None, [f32, f32, P(f32)]),
+        "ref_scatter_differentials": (c_int, [u32, P(f32), f32, P(f32), P(f32), c_int, P(f32), P(f32), P(f32), f32, f32,
+                                              P(f32), f32, P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

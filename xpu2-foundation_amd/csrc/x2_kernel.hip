@@ -639,104 +639,519 @@ __device__ V3 lobeSample(const Bsdf& b, uint32_t q, V3 wo, float r1, float r2, V
 // ---------------------------------------------------------------------------------------------------------------
 // The estimator (async_integrator.h:1986-3042, the subset above). Pending rays live in the lane's private memory.
 // ---------------------------------------------------------------------------------------------------------------
-// Textures (texture_core.h:144-153 bilinear, :179-181 lookup with unit scale and no offset, level 0, periodic).
-struct TexTap {
-    uint32_t tex, x0, y0;
-    float fs, ft;
-    V3 color;
+// Textures: mip pyramids read by the path tracer's map lookups (texture_core.h:86-206, which adapt OpenImageIO
+// 3.1.17's texturesys.cpp, Apache-2.0, copyright Contributors to the OpenImageIO project): the anisotropic filter
+// on camera paths, trilinear closest texels on glossy ones, one level's closest texel on diffuse ones
+// (shade_mapnodes.h:344-347), as MoonRay's ImageMap binds a map (shade_mapnodes.h:420-470: unit scale, no offset or
+// bias, periodic, the t flip, the footprint's cross terms swapped as XPU's vector shading passes them,
+// async_gpu_device.h:71). theta's sine and cosine come from the shared polynomial and hypot(A - C, B) is the square
+// root of the FP64 sum of squares, on both devices. A texture's levels are FP32 rgb end to end (resident), or, paged,
+// 32 x 32 tiles of half floats in the tile pool for the finer levels and an always-resident array of half floats for
+// the coarse ones.
+struct Footprint {
+    float u, v, dudx, dvdx, dudy, dvdy;
 };
 __device__ __forceinline__ uint32_t wrapTexel(int x, uint32_t n) {
     const int m = x % int(n);
     return uint32_t(m < 0 ? m + int(n) : m);
 }
-__device__ __forceinline__ V3 texel(const TexDesc& t, uint32_t x, uint32_t y) {
-    const float* p = t.texels + (size_t(y) * t.width + x) * 3;
-    return {p[0], p[1], p[2]};
+// OpenImageIO's fast_atan2 and fast_exp (fmath.h), as texture_core.h:94-97 gives them.
+__device__ __forceinline__ float oiioAtan2(float y, float x) {
+    const float a = fabsf(x), b = fabsf(y);
+    const float k = b == 0 ? 0.f : (a == b ? 1.f : (b > a ? a / b : b / a));
+    const float s = 1.f - (1.f - k), t = s * s;
+    float r = s * (0.430165678f * t + 1.f) / ((0.0579354987f * t + 0.763007998f) * t + 1.f);
+    if (b > a) r = 1.5707963267948966f - r;
+    if (copysignf(1.f, x) < 0) r = Pi - r;
+    return copysignf(r, y);
 }
-__device__ bool textureTap(const TexDesc& t, uint32_t id, float u, float v, TexTap& tap) {
-    if (t.paged) return false;  // its level 0 is in tiles: pagedTap reads it
-    const float su = 1.0f * u + 0.0f, tv = 1 - (1.0f * v + 0.0f);
-    float fs = su * float(t.width) - .5f, ft = tv * float(t.height) - .5f;
-    if (!finite1(fs) || !finite1(ft) || fabsf(fs) > 1.e8f || fabsf(ft) > 1.e8f) return false;
-    const int x = int(floorf(fs)), y = int(floorf(ft));
-    fs -= float(x);
-    ft -= float(y);
-    const uint32_t x0 = wrapTexel(x, t.width), x1 = wrapTexel(x + 1, t.width);
-    const uint32_t y0 = wrapTexel(y, t.height), y1 = wrapTexel(y + 1, t.height);
-    const V3 a = texel(t, x0, y0), b = texel(t, x1, y0), c = texel(t, x0, y1), d = texel(t, x1, y1);
-    tap.color = ((a * (1 - fs)) + (b * fs)) * (1 - ft) + ((c * (1 - fs)) + (d * fs)) * ft;
-    tap.tex = id;
-    tap.x0 = x0;
-    tap.y0 = y0;
-    tap.fs = fs;
-    tap.ft = ft;
-    return finite3(tap.color);
+__device__ __forceinline__ float oiioExp(float a) {
+    float x = fminf(fmaxf(a * 1.4426950408889634f, -126.f), 126.f);
+    const int m = int(x);
+    x -= float(m);
+    x = 1.f - (1.f - x);
+    float r = 1.33336498402e-3f;
+    r = x * r + 9.810352697968e-3f;
+    r = x * r + 5.551834031939e-2f;
+    r = x * r + 0.2401793301105f;
+    r = x * r + 0.693144857883f;
+    r = x * r + 1.f;
+    return ldexpf(r, m);
 }
-
-// On-demand tiles (texture_core.h:37-60; the GPU's fine-detail cache, async_gpu.h:2911-3200). A level-0 texel through
-// the page table: half-float rgb in its tile's pool slot. False: the tile is not resident; it is asked for once (its
-// flag set, its id on the launch's miss list; with the list full the flag is cleared, for a later lookup to ask).
-__device__ __forceinline__ bool pagedTexel(const RenderParams& P, const TexDesc& t, uint32_t x, uint32_t y, V3& out) {
-    const uint32_t tile = t.tileBase + (y >> 5) * t.tilesX + (x >> 5);
-    const uint32_t slot = P.tileSlot[tile];
-    if (slot == NoTileSlot) {
+// The filter's plan (texture_core.h:161-206 up to the reads): the levels and their weights, and for the
+// anisotropic filter the taps along the major axis (n 0: one closest texel a level).
+struct TexPlan {
+    float s, t;
+    int levels[2];
+    float lw[2];
+    int n;
+    float sm, tm;
+    float pos[7], weight[7];
+};
+__device__ bool planLookup(const TexDesc& tex, uint32_t mode, const Footprint& f, bool swap, TexPlan& p) {
+    const uint32_t count = tex.count;
+    if (!count || count > MipLevels) return false;
+    float s = f.u, t = 1 - f.v;
+    float dx = f.dudx, tx = -f.dvdx, dy = f.dudy, ty = -f.dvdy;
+    if (swap) {
+        const float c = tx;
+        tx = dy;
+        dy = c;
+    }
+    if (!finite1(s) || !finite1(t) || !finite1(dx) || !finite1(tx) || !finite1(dy) || !finite1(ty)) return false;
+    if (fabsf(s) > 1024) s = s - floorf(s);
+    if (fabsf(t) > 1024) t = t - floorf(t);
+    p.s = s;
+    p.t = t;
+    p.levels[0] = p.levels[1] = -1;
+    p.lw[0] = p.lw[1] = 0;
+    p.n = 0;
+    p.sm = p.tm = 0;
+    float minor, major = 0, aspect = 1;
+    if (mode != 0) {
+        const float eps = 1.0e-8f, eps2 = eps * eps;
+        const float dxlen2 = dx * dx + tx * tx, dylen2 = dy * dy + ty * ty;
+        if (dxlen2 < eps2) {
+            if (dylen2 < eps2) {
+                dx = eps;
+                dy = 0;
+                tx = 0;
+                ty = eps;
+            } else {
+                const float sc = eps / sqrtf(dylen2);
+                dx = ty * sc;
+                tx = -dy * sc;
+            }
+        } else if (dylen2 < eps2) {
+            const float sc = eps / sqrtf(dxlen2);
+            dy = -tx * sc;
+            ty = dx * sc;
+        }
+        minor = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fmaxf(fabsf(tx), fabsf(ty)));  // the filter width
+    } else {
+        const double A = double(tx * tx + ty * ty), B = -2.0 * double(dx * tx + dy * ty), C = double(dx * dx + dy * dy);
+        const double root = sqrt((A - C) * (A - C) + B * B);
+        major = fminf(sqrtf(fmaxf(float((A + C + root) * .5), 0.f)), 1000.f);
+        minor = fminf(sqrtf(fmaxf(float((A + C - root) * .5), 0.f)), 1000.f);
+        const float theta = oiioAtan2(float(B), float(A - C)) * .5f + Pi * .5f;
+        aspect = minor > 0 ? fminf(fmaxf(major / minor, 1.f), 1.e6f) : (major > 0 ? 1.e6f : 1.f);
+        if (aspect > 4) {
+            aspect = 4;
+            major = .5f * (major + minor * 4);
+            minor = major / 4;
+        }
+        float sn, cs;
+        sincosPoly(theta, sn, cs);
+        p.sm = cs;  // scaled by the axis length below
+        p.tm = sn;
+    }
+    for (uint32_t m = 0; m < count; ++m) {
+        const TexLevel& l = tex.levels[m];
+        const float w = minor * float(l.width < l.height ? l.width : l.height);
+        if (w <= 1) {
+            p.levels[0] = int(m) - 1;
+            p.levels[1] = int(m);
+            p.lw[1] = fminf(fmaxf(2 * w - 1, 0.f), 1.f);
+            p.lw[0] = 1 - p.lw[1];
+            break;
+        }
+    }
+    if (p.levels[1] < 0) {
+        p.levels[0] = p.levels[1] = int(count) - 1;
+        p.lw[0] = 1;
+        p.lw[1] = 0;
+    } else if (p.levels[0] < 0) {
+        p.levels[0] = p.levels[1] = 0;
+        p.lw[0] = 1;
+        p.lw[1] = 0;
+        if (mode == 0) {
+            const TexLevel& l = tex.levels[0];
+            const uint32_t r = l.width > l.height ? l.width : l.height;
+            if (minor * float(r) < .5f) aspect = fminf(fmaxf(major * float(r) * 2, 1.f), 4.f);
+        }
+    } else if (mode == 2) {
+        p.levels[0] = p.levels[1];
+        p.lw[0] = 1;
+        p.lw[1] = 0;
+    }
+    if (mode != 0) return true;
+    int n = int(2 * aspect - 1);
+    if (n < 1) n = 1;
+    if (n > 7) return false;
+    const float LL = 2 * (major - minor);
+    p.sm = p.sm * LL * .5f;
+    p.tm = p.tm * LL * .5f;
+    p.n = n;
+    if (n == 1) {
+        p.weight[0] = 1;
+        p.pos[0] = 0;
+    } else if (n == 2) {
+        p.weight[0] = p.weight[1] = .5f;
+        p.pos[0] = -.5f;
+        p.pos[1] = .5f;
+    } else {
+        float sum = 0;
+        const float scale = major / LL;
+        for (int i = 0; i < n; ++i) {
+            p.pos[i] = 2 * (float(i) + .5f) * (1.f / float(n)) - 1;
+            const float x = p.pos[i] * scale;
+            p.weight[i] = oiioExp(-2 * x * x);
+            sum += p.weight[i];
+        }
+        for (int i = 0; i < n; ++i) p.weight[i] /= sum;
+    }
+    return true;
+}
+enum : int { TapOk = 0, TapInvalid = 1, TapMiss = 2, TapCoarse = 3 };
+// A resident texture's texel (level k at levels[k].offset floats).
+struct ResidentTexels {
+    const TexDesc& t;
+    __device__ __forceinline__ int fetch(uint32_t level, uint32_t x, uint32_t y, V3& out) const {
+        const TexLevel& l = t.levels[level];
+        const float* p = t.texels + l.offset + (size_t(y) * l.width + x) * 3;
+        out = V3{p[0], p[1], p[2]};
+        return TapOk;
+    }
+};
+// A paged texture's texel: the coarse levels from the always-resident halves; a finer level's through the page table
+// (half-float rgb in its tile's pool slot). A missing tile is asked for once (its flag set, its id on the launch's
+// miss list; with the list full the flag is cleared, for a later lookup to ask) and the fetch answers TapMiss, or,
+// with partial shading, the texel's parent on the nearest level up that is here, TapCoarse (async_gpu_device.h:49-58:
+// the sample is then provisional). Every fetch of a lookup runs, so a sample asks for all its tiles at once.
+struct PagedTexels {
+    const RenderParams& P;
+    const TexDesc& t;
+    __device__ __forceinline__ bool here(uint32_t level, uint32_t x, uint32_t y, V3& out) const {
+        const TexLevel& l = t.levels[level];
+        if (level >= t.firstResident) {
+            const uint16_t* h = t.coarse + l.offset + (size_t(y) * l.width + x) * 3;
+            out = V3{__half2float(__ushort_as_half(h[0])), __half2float(__ushort_as_half(h[1])),
+                     __half2float(__ushort_as_half(h[2]))};
+            return true;
+        }
+        const uint32_t tile = l.tileBase + (y >> 5) * l.tilesX + (x >> 5);
+        const uint32_t slot = P.tileSlot[tile];
+        if (slot == NoTileSlot) return false;
+        if (P.tileStamp[tile] != P.stamp) P.tileStamp[tile] = P.stamp;  // read this launch: recently used
+        const uint16_t* h = P.tilePool + (size_t(slot) * TexTileTexels + (y & (TexTile - 1)) * TexTile + (x & (TexTile - 1))) * 3;
+        out = V3{__half2float(__ushort_as_half(h[0])), __half2float(__ushort_as_half(h[1])),
+                 __half2float(__ushort_as_half(h[2]))};
+        return true;
+    }
+    __device__ int fetch(uint32_t level, uint32_t x, uint32_t y, V3& out) const {
+        if (here(level, x, y, out)) return TapOk;
+        const TexLevel& l = t.levels[level];
+        const uint32_t tile = l.tileBase + (y >> 5) * l.tilesX + (x >> 5);
         if (atomicExch(P.tileAsked + tile, 1u) == 0u) {
             const uint32_t at = atomicAdd(&P.paging->missCount, 1u);
             if (at < P.missCap) P.missList[at] = tile;
             else atomicExch(P.tileAsked + tile, 0u);
         }
-        return false;
+        if (!P.partial) return TapMiss;
+        for (uint32_t k = level + 1; k < t.count; ++k) {
+            x >>= 1;
+            y >>= 1;
+            const TexLevel& u = t.levels[k];
+            if (here(k, x < u.width ? x : u.width - 1, y < u.height ? y : u.height - 1, out)) return TapCoarse;
+        }
+        return TapMiss;
     }
-    if (P.tileStamp[tile] != P.stamp) P.tileStamp[tile] = P.stamp;  // read this launch: recently used
-    const uint16_t* h = P.tilePool + (size_t(slot) * TexTileTexels + (y & (TexTile - 1)) * TexTile + (x & (TexTile - 1))) * 3;
-    out = V3{__half2float(__ushort_as_half(h[0])), __half2float(__ushort_as_half(h[1])),
-             __half2float(__ushort_as_half(h[2]))};
+};
+// Folds a fetch's status into the lookup's: an invalid value wins, then a miss, then a provisional read.
+__device__ __forceinline__ int worse(int a, int b) {
+    if (a == TapInvalid || b == TapInvalid) return TapInvalid;
+    if (a == TapMiss || b == TapMiss) return TapMiss;
+    return a == TapCoarse || b == TapCoarse ? TapCoarse : TapOk;
+}
+// texture_core.h:144-153 at one level: the four texels about (u, v), each fetch wrapped.
+template <class Texels>
+__device__ int bilinearAt(const Texels& T, const TexLevel& l, uint32_t level, float u, float v, V3& out) {
+    float s = u * float(l.width) - .5f, t = v * float(l.height) - .5f;
+    if (!finite1(s) || !finite1(t) || fabsf(s) > 1.e8f || fabsf(t) > 1.e8f) return TapInvalid;
+    const int x = int(floorf(s)), y = int(floorf(t));
+    s -= float(x);
+    t -= float(y);
+    const uint32_t x0 = wrapTexel(x, l.width), x1 = wrapTexel(x + 1, l.width);
+    const uint32_t y0 = wrapTexel(y, l.height), y1 = wrapTexel(y + 1, l.height);
+    V3 a, b, c, d;
+    int st = T.fetch(level, x0, y0, a);
+    st = worse(st, T.fetch(level, x1, y0, b));
+    st = worse(st, T.fetch(level, x0, y1, c));
+    st = worse(st, T.fetch(level, x1, y1, d));
+    if (st == TapInvalid || st == TapMiss) return st;
+    out = ((a * (1 - s)) + (b * s)) * (1 - t) + ((c * (1 - s)) + (d * s)) * t;
+    return finite3(out) ? st : TapInvalid;
+}
+// texture_core.h:155-160: the texel nearest (u, v); an exact half-texel tie takes the lower one.
+__device__ __forceinline__ bool closestAt(const TexLevel& l, float u, float v, uint32_t& x, uint32_t& y) {
+    const float s = u * float(l.width) - .5f, t = v * float(l.height) - .5f;
+    if (!finite1(s) || !finite1(t) || fabsf(s) > 1.e8f || fabsf(t) > 1.e8f) return false;
+    int ix = int(floorf(s)), iy = int(floorf(t));
+    if (s - float(ix) > .5f) ++ix;
+    if (t - float(iy) > .5f) ++iy;
+    x = wrapTexel(ix, l.width);
+    y = wrapTexel(iy, l.height);
     return true;
 }
-enum : int { TapOk = 0, TapInvalid = 1, TapMiss = 2, TapCoarse = 3 };
-// textureTap over a paged level 0: the same footprint and arithmetic, every tap's tile asked for at once when any is
-// missing (one wait for the sample, not one a tile, texture_core.h:56-58). TapMiss: a tile is missing; with partial
-// shading TapCoarse instead, the footprint read from the coarse level (the sample is then provisional).
-__device__ int pagedTap(const RenderParams& P, const TexDesc& t, uint32_t id, float u, float v, TexTap& tap) {
-    const float su = 1.0f * u + 0.0f, tv = 1 - (1.0f * v + 0.0f);
-    float fs = su * float(t.width) - .5f, ft = tv * float(t.height) - .5f;
-    if (!finite1(fs) || !finite1(ft) || fabsf(fs) > 1.e8f || fabsf(ft) > 1.e8f) return TapInvalid;
-    const int x = int(floorf(fs)), y = int(floorf(ft));
-    fs -= float(x);
-    ft -= float(y);
-    const uint32_t x0 = wrapTexel(x, t.width), x1 = wrapTexel(x + 1, t.width);
-    const uint32_t y0 = wrapTexel(y, t.height), y1 = wrapTexel(y + 1, t.height);
-    V3 a, b, c, d;
-    bool ok = pagedTexel(P, t, x0, y0, a);
-    ok &= pagedTexel(P, t, x1, y0, b);
-    ok &= pagedTexel(P, t, x0, y1, c);
-    ok &= pagedTexel(P, t, x1, y1, d);
-    tap.tex = id;
-    if (ok) {
-        tap.color = ((a * (1 - fs)) + (b * fs)) * (1 - ft) + ((c * (1 - fs)) + (d * fs)) * ft;
-        tap.x0 = x0;
-        tap.y0 = y0;
-        tap.fs = fs;
-        tap.ft = ft;
-        return finite3(tap.color) ? TapOk : TapInvalid;
+// The map's value under the plan (texture_core.h:171-177, :199-206).
+template <class Texels>
+__device__ int mapRead(const Texels& T, const TexDesc& tex, const TexPlan& p, V3& out) {
+    V3 result{0, 0, 0};
+    int status = TapOk;
+    for (int k = 0; k < 2; ++k) {
+        if (p.lw[k] == 0) continue;
+        const uint32_t level = uint32_t(p.levels[k]);
+        const TexLevel& l = tex.levels[level];
+        if (p.n == 0) {
+            uint32_t x, y;
+            if (!closestAt(l, p.s, p.t, x, y)) return TapInvalid;
+            V3 value;
+            const int st = T.fetch(level, x, y, value);
+            if (st == TapMiss) {
+                status = TapMiss;
+                continue;
+            }
+            if (!finite3(value)) return TapInvalid;
+            status = worse(status, st);
+            result = result + value * p.lw[k];
+            continue;
+        }
+        V3 acc{0, 0, 0};
+        for (int i = 0; i < p.n; ++i) {
+            V3 value;
+            const int st = bilinearAt(T, l, level, p.s + p.pos[i] * p.sm, p.t + p.pos[i] * p.tm, value);
+            if (st == TapInvalid) return TapInvalid;
+            status = worse(status, st);
+            if (st == TapMiss) continue;
+            acc = acc + value * p.weight[i];
+        }
+        result = result + acc * p.lw[k];
     }
-    if (!P.partial) return TapMiss;
-    float cs = su * float(t.coarseW) - .5f, ct = tv * float(t.coarseH) - .5f;
-    const int cx = int(floorf(cs)), cy = int(floorf(ct));
-    cs -= float(cx);
-    ct -= float(cy);
-    const uint32_t u0 = wrapTexel(cx, t.coarseW), u1 = wrapTexel(cx + 1, t.coarseW);
-    const uint32_t v0 = wrapTexel(cy, t.coarseH), v1 = wrapTexel(cy + 1, t.coarseH);
-    auto at = [&](uint32_t i, uint32_t j) {
-        const float* q = t.coarse + (size_t(j) * t.coarseW + i) * 3;
-        return V3{q[0], q[1], q[2]};
+    if (status == TapMiss) return TapMiss;
+    if (!finite3(result)) return TapInvalid;
+    out = result;
+    return status;
+}
+// A lookup of a texture, resident or paged.
+__device__ int mapLookup(const RenderParams& P, const TexDesc& tex, uint32_t mode, const Footprint& f, bool swap, V3& out) {
+    TexPlan p;
+    if (!planLookup(tex, mode, f, swap, p)) return TapInvalid;
+    if (tex.paged) return mapRead(PagedTexels{P, tex}, tex, p, out);
+    return mapRead(ResidentTexels{tex}, tex, p, out);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Ray differentials (shade_state.h:30-33; shade_material.h:388-558, :691-705; shade_lobes_eval.h:725-741;
+// shade_math.h:185-207, :279-288), for Lambert and GGX lobes on flat triangles (dN/dx = dN/dy = 0). A pending ray
+// carries its differentials in scenes with textures; the camera's carries its auxiliary directions instead.
+struct RayDiff {
+    V3 dOx, dOy, dDx, dDy;
+    uint32_t valid;
+};
+struct HitDiff {
+    float dsdx, dtdx, dsdy, dtdy;
+    V3 dPdx, dPdy;
+    bool valid;
+    V3 inDx, inDy;  // the incoming direction differentials (the GGX lobe reflects them)
+};
+__device__ __forceinline__ float differenceOfProducts(float a, float b, float c, float d) {  // Kahan's
+    const float cd = c * d;
+    const float err = fmaf(-c, d, cd);
+    const float dop = fmaf(a, b, -cd);
+    return dop + err;
+}
+__device__ __forceinline__ V3 normalized(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ bool trianglePartials(V3 x0, V3 x1, V3 x2, const float st[6], V3& dPds, V3& dPdt) {
+    const float A[4] = {st[2] - st[0], st[3] - st[1], st[4] - st[0], st[5] - st[1]};
+    const V3 b0 = x1 - x0, b1 = x2 - x0;
+    const float det = differenceOfProducts(A[0], A[3], A[1], A[2]);
+    if (!(fabsf(det) >= 1.17549435e-38f) || !(fabsf(det) <= 3.402823466e38f)) return false;
+    const float inv = 1.0f / det;
+    dPds = V3{differenceOfProducts(b0.x, A[3], b1.x, A[1]), differenceOfProducts(b0.y, A[3], b1.y, A[1]),
+              differenceOfProducts(b0.z, A[3], b1.z, A[1])} * inv;
+    dPdt = V3{differenceOfProducts(b1.x, A[0], b0.x, A[2]), differenceOfProducts(b1.y, A[0], b0.y, A[2]),
+              differenceOfProducts(b1.z, A[0], b0.z, A[2])} * inv;
+    return finite3(dPds) && finite3(dPdt);
+}
+// shade_material.h:392-432: a camera ray's auxiliary rays meet the triangle's plane; their barycentric offsets map
+// through the texture coordinates (the pinhole's offsets are zero).
+__device__ void hitTexDerivatives(V3 a, V3 b, V3 c, const float st[6], const Ray& ray, V3 camDx, V3 camDy, uint32_t spp,
+                                  HitDiff& h) {
+    h.dsdx = h.dsdy = h.dtdx = h.dtdy = 0.0f;
+    if (isExactlyZero(camDx) || isExactlyZero(camDy) || !spp) return;
+    const V3 e1 = b - a, e2 = c - a, n = cross(e1, e2);
+    uint32_t axis = fabsf(n.x) > fabsf(n.y) ? 0 : 1;
+    if (fabsf(n.z) > (axis == 0 ? fabsf(n.x) : fabsf(n.y))) axis = 2;
+    auto pa = [&](V3 v) { return axis == 0 ? v.y : axis == 1 ? v.z : v.x; };
+    auto pb = [&](V3 v) { return axis == 0 ? v.z : axis == 1 ? v.x : v.y; };
+    const float det = pa(e1) * pb(e2) - pb(e1) * pa(e2), den = dot(n, ray.direction);
+    if (!(fabsf(det) > 0) || !(fabsf(den) > 0)) return;
+    const float plane = dot(n, a - ray.origin);
+    const V3 primary = ray.direction * (plane / den);
+    const float scale = 1.f / fminf(sqrtf(float(spp)), 2.f);
+    float ds[2], dt[2];
+    for (int i = 0; i < 2; ++i) {
+        V3 d = i ? camDy : camDx;
+        if (scale < 1) d = normalized(ray.direction + (d - ray.direction) * scale);
+        const float dd = dot(n, d);
+        if (!(fabsf(dd) > 0)) return;
+        const V3 o = V3{0, 0, 0} * scale;
+        const V3 delta = o + d * (dot(n, a - ray.origin - o) / dd) - primary;
+        const float x = (pa(delta) * pb(e2) - pb(delta) * pa(e2)) / det, y = (pa(e1) * pb(delta) - pb(e1) * pa(delta)) / det;
+        ds[i] = (st[2] - st[0]) * x + (st[4] - st[0]) * y;
+        dt[i] = (st[3] - st[1]) * x + (st[5] - st[1]) * y;
+    }
+    if (!(finite1(ds[0]) && finite1(ds[1]) && finite1(dt[0]) && finite1(dt[1]))) return;
+    h.dsdx = ds[0];
+    h.dtdx = dt[0];
+    h.dsdy = ds[1];
+    h.dtdy = dt[1];
+}
+// shade_material.h:436-451.
+__device__ __forceinline__ void transferDifferentials(const RayDiff& d, V3 origin, V3 dir, V3 Ng, V3 Pt, HitDiff& h) {
+    h.dPdx = h.dPdy = V3{0, 0, 0};
+    h.valid = false;
+    if (!d.valid) return;
+    const V3 oX = origin + d.dOx, oY = origin + d.dOy, dX = dir + d.dDx, dY = dir + d.dDy;
+    const float denomX = dot(Ng, dX), denomY = dot(Ng, dY);
+    if (denomX == 0.0f || denomY == 0.0f) return;
+    const float planeD = -dot(Ng, Pt);
+    const float distX = -(dot(oX, Ng) + planeD) / denomX, distY = -(dot(oY, Ng) + planeD) / denomY;
+    h.dPdx = (oX + dX * distX) - Pt;
+    h.dPdy = (oY + dY * distY) - Pt;
+    h.valid = finite3(h.dPdx) && finite3(h.dPdy);
+    if (!h.valid) h.dPdx = h.dPdy = V3{0, 0, 0};
+}
+// shade_material.h:454-495; without differentials, the coarsest level (1024).
+__device__ void differentialTexDerivatives(V3 dPds, V3 dPdt, V3 Ng, HitDiff& h) {
+    h.dsdx = h.dtdx = h.dsdy = h.dtdy = 0.0f;
+    if (!h.valid) {
+        h.dsdx = 1024.0f;
+        h.dtdy = 1024.0f;
+        return;
+    }
+    const float e = 1e-5f;
+    if (fabsf(dot(dPds, dPdt)) <= e) {
+        const float du = dot(dPds, dPds), dv = dot(dPdt, dPdt);
+        if (du > e && dv > e) {
+            const float iu = 1.0f / dot(dPds, dPds), iv = 1.0f / dot(dPdt, dPdt);
+            h.dsdx = dot(h.dPdx, dPds) * iu;
+            h.dtdx = dot(h.dPdx, dPdt) * iv;
+            h.dsdy = dot(h.dPdy, dPds) * iu;
+            h.dtdy = dot(h.dPdy, dPdt) * iv;
+        }
+        return;
+    }
+    const V3 n2 = Ng * Ng;
+    float A[4], bx[2], by[2];
+    if (n2.x > n2.y && n2.x > n2.z) {
+        A[0] = dPds.y; A[1] = dPdt.y; A[2] = dPds.z; A[3] = dPdt.z;
+        bx[0] = h.dPdx.y; bx[1] = h.dPdx.z; by[0] = h.dPdy.y; by[1] = h.dPdy.z;
+    } else if (n2.y > n2.z) {
+        A[0] = dPds.x; A[1] = dPdt.x; A[2] = dPds.z; A[3] = dPdt.z;
+        bx[0] = h.dPdx.x; bx[1] = h.dPdx.z; by[0] = h.dPdy.x; by[1] = h.dPdy.z;
+    } else {
+        A[0] = dPds.x; A[1] = dPdt.x; A[2] = dPds.y; A[3] = dPdt.y;
+        bx[0] = h.dPdx.x; bx[1] = h.dPdx.y; by[0] = h.dPdy.x; by[1] = h.dPdy.y;
+    }
+    const float det = A[0] * A[3] - A[1] * A[2];
+    if (fabsf(det) <= e) return;
+    const float detA = differenceOfProducts(A[0], A[3], A[1], A[2]);
+    if (!(fabsf(detA) >= 1.17549435e-38f) || !(fabsf(detA) <= 3.402823466e38f)) return;
+    const float inv = 1.0f / detA;
+    h.dsdx = differenceOfProducts(bx[0], A[3], bx[1], A[1]) * inv;
+    h.dtdx = differenceOfProducts(bx[1], A[0], bx[0], A[2]) * inv;
+    h.dsdy = differenceOfProducts(by[0], A[3], by[1], A[1]) * inv;
+    h.dtdy = differenceOfProducts(by[1], A[0], by[0], A[2]) * inv;
+}
+// shade_material.h:538-558 for a mesh without subdivision derivatives: a camera ray's texture derivatives by
+// hitTexDerivatives and its auxiliary rays, scaled for the samples per pixel, as the incoming differentials; any
+// other ray's transferred and turned into texture derivatives. in: the pending ray's record (the camera's holds its
+// auxiliary directions in dDx and dDy).
+__device__ void hitDifferentials(V3 a, V3 b, V3 c, const float st[6], V3 N, V3 Ng, V3 Pt, const Ray& ray, bool camera,
+                                 const RayDiff& in, uint32_t spp, HitDiff& h) {
+    if (camera && !isExactlyZero(in.dDx) && !isExactlyZero(in.dDy) && spp) {
+        hitTexDerivatives(a, b, c, st, ray, in.dDx, in.dDy, spp, h);
+        const float scale = 1.f / fminf(sqrtf(float(spp)), 2.f);
+        RayDiff d;
+        d.dOx = V3{0, 0, 0} * scale;
+        d.dOy = V3{0, 0, 0} * scale;
+        d.dDx = normalized(ray.direction + (in.dDx - ray.direction) * scale) - ray.direction;
+        d.dDy = normalized(ray.direction + (in.dDy - ray.direction) * scale) - ray.direction;
+        d.valid = 1;
+        h.inDx = d.dDx;
+        h.inDy = d.dDy;
+        transferDifferentials(d, ray.origin, ray.direction, Ng, Pt, h);
+        return;
+    }
+    h.inDx = in.dDx;
+    h.inDy = in.dDy;
+    transferDifferentials(in, ray.origin, ray.direction, Ng, Pt, h);
+    V3 dPds, dPdt;
+    if (!trianglePartials(a, b, c, st, dPds, dPdt)) {
+        const Frame f = frameOf(N);
+        dPds = f.x;
+        dPdt = f.y;
+    }
+    differentialTexDerivatives(dPds, dPdt, Ng, h);
+}
+// A scattered ray's differentials: Lambert's from the cosine sample's (clamped 85 degrees from the normal, scaled by
+// the primary scale as the lobe ignores the incoming ones), GGX's the incoming ones reflected about the half vector
+// and widened by 1 + 7 alpha (shade_lobes.h:479, :548). Invalid where the hit's are.
+__device__ RayDiff scatterDifferentials(bool lambert, const Frame& frame, float alpha, const HitDiff& h, V3 Pt, V3 wo,
+                                        V3 wi, float r1, float r2, V3 childOrigin, float primaryScale) {
+    RayDiff d;
+    d.valid = 0;
+    d.dOx = d.dOy = d.dDx = d.dDy = V3{0, 0, 0};
+    if (!h.valid) return d;
+    V3 dDdx = h.inDx, dDdy = h.inDy;
+    if (lambert) {
+        const float c85 = 0.08715580403804779f;  // cosf(85 * (pi / 180)) in binary32
+        const float minR1 = c85 * c85, maxR1 = 1.0f - minR1;
+        r1 = fminf(fmaxf(r1, minR1), maxR1);
+        const float cosTheta = r1 > 1.0f ? 1.0f : sqrtf(r1), sinTheta = sqrtf(1.0f - r1);
+        float sinPhi, cosPhi;
+        sincosPoly(r2 * TwoPi, sinPhi, cosPhi);
+        const float scale = 1.0f / 8.0f, halfScale = 0.5f * scale, twoPiScale = TwoPi * scale;
+        dDdx = {-halfScale * cosPhi / sinTheta, -halfScale * sinPhi / sinTheta, halfScale / cosTheta};
+        dDdy = {-twoPiScale * sinPhi * sinTheta, twoPiScale * cosPhi * sinTheta, 0.0f};
+        const float l1 = length(dDdx), l2 = length(dDdy), l = sqrtf(l1 * l2);
+        dDdx = dDdx * (l / l1);
+        dDdy = dDdy * (l / l2);
+        dDdx = localToGlobal(frame, dDdx);
+        dDdy = localToGlobal(frame, dDdy);
+    } else {
+        V3 H = wo + wi;
+        const float hl = length(H);
+        if (hl > 0.001f) H = H * (1.0f / hl);
+        else H = frame.z;
+        const V3 dN{0, 0, 0};
+        const float Cx = dot(dDdx, H) - dot(wo, dN), Cy = dot(dDdy, H) - dot(wo, dN), dDotN = -dot(wo, H);
+        dDdx = dDdx - (2.0f * (dDotN * dN + Cx * H));
+        dDdy = dDdy - (2.0f * (dDotN * dN + Cy * H));
+        const float factor = 1.0f + alpha * 7.0f;
+        dDdx = dDdx * factor;
+        dDdy = dDdy * factor;
+    }
+    const float scale = lambert ? primaryScale : 1.0f;
+    const V3 shift = Pt - childOrigin;
+    d.dOx = h.dPdx * scale + shift;
+    d.dOy = h.dPdy * scale + shift;
+    d.dDx = normalized(wi + dDdx * scale) - wi;
+    d.dDy = normalized(wi + dDdy * scale) - wi;
+    d.valid = finite3(d.dOx) && finite3(d.dOy) && finite3(d.dDx) && finite3(d.dDy) ? 1u : 0u;
+    return d;
+}
+// The camera ray's auxiliary directions through the next pixel in x and in y (camera_core.h:222-224, :245-246).
+__device__ bool cameraDifferentials(const x2_config& c, float x, float y, V3& dx, V3& dy) {
+    const float* m = c.raster_to_camera;
+    auto dirAt = [&](V3 p) {
+        const V3 v = vectorM(m, p) + V3{m[12], m[13], m[14]};
+        const float w = ((p.x * m[3] + p.y * m[7]) + p.z * m[11]) + m[15];
+        return w == 0.0f ? V3{0, 0, 0} : V3{v.x / w, v.y / w, v.z / w};
     };
-    tap.color = ((at(u0, v0) * (1 - cs)) + (at(u1, v0) * cs)) * (1 - ct) + ((at(u0, v1) * (1 - cs)) + (at(u1, v1) * cs)) * ct;
-    tap.x0 = u0;
-    tap.y0 = v0;
-    tap.fs = cs;
-    tap.ft = ct;
-    return finite3(tap.color) ? TapCoarse : TapInvalid;
+    const V3 dirX = dirAt(V3{x + 1, y, -1.0f}), dirY = dirAt(V3{x, y + 1, -1.0f});
+    return normalizeTo(vectorM(c.camera_to_render, dirX), dx) && normalizeTo(vectorM(c.camera_to_render, dirY), dy);
 }
 
 // d ln f / d r of the GGX lobe (shade_lobes_eval.h:93-121) at (wo, wi) for MoonRay's input roughness r (the lobe's
@@ -761,12 +1176,15 @@ __device__ float ggxRoughnessLogDerivative(float r, V3 N, V3 wo, V3 wi) {
 }
 
 // The adjoint's bookkeeping. A gradient sink is what a contribution's derivative goes to: the texels of a textured
-// Lambert lobe (its bilinear taps) or a material's roughness (with the GGX log-derivative at the lobe's direction).
+// Lambert lobe (its lookup: texture, mode, footprint, and the value read) or a material's roughness (with the GGX
+// log-derivative at the lobe's direction).
 enum : uint32_t { SinkNone = 0, SinkTexture = 1, SinkRoughness = 2 };
 struct Sink {
     uint32_t kind, material;
     float g;
-    TexTap tap;
+    uint32_t tex, mode;
+    Footprint f;
+    V3 color;
 };
 // An open subtree: the pending ray popped at stack position pos, the running radiance when it was popped, and the
 // sink of the lobe that made it. Its subtree's radiance is what the running radiance gains until the next pop at a
@@ -778,22 +1196,48 @@ struct Record {
 };
 constexpr uint32_t MaxRecords = 34;
 
-__device__ void scatterTexture(const RenderParams& P, const TexTap& t, V3 R, V3 delta) {
-    const TexDesc& d = P.textures[t.tex];
-    const float cs[3] = {t.color.x > 0.0f ? delta.x * R.x / t.color.x : 0.0f,
-                         t.color.y > 0.0f ? delta.y * R.y / t.color.y : 0.0f,
-                         t.color.z > 0.0f ? delta.z * R.z / t.color.z : 0.0f};
+// A contribution R of a lobe whose albedo the map scales has d R / d value = R / value per channel, and the value is
+// linear in the texels its lookup reads: each gets its weight's share, level 0's into the caller's gradient, the
+// levels above into the pyramid gradient (pushed down to level 0 after the frame).
+__device__ void scatterTexture(const RenderParams& P, const Sink& s, V3 R, V3 delta) {
+    const TexDesc& d = P.textures[s.tex];
+    const float cs[3] = {s.color.x > 0.0f ? delta.x * R.x / s.color.x : 0.0f,
+                         s.color.y > 0.0f ? delta.y * R.y / s.color.y : 0.0f,
+                         s.color.z > 0.0f ? delta.z * R.z / s.color.z : 0.0f};
     if (cs[0] == 0.0f && cs[1] == 0.0f && cs[2] == 0.0f) return;
-    const uint32_t x1 = t.x0 + 1 == d.width ? 0 : t.x0 + 1, y1 = t.y0 + 1 == d.height ? 0 : t.y0 + 1;
-    const uint32_t xs[4] = {t.x0, x1, t.x0, x1}, ys[4] = {t.y0, t.y0, y1, y1};
-    const float ws[4] = {(1 - t.fs) * (1 - t.ft), t.fs * (1 - t.ft), (1 - t.fs) * t.ft, t.fs * t.ft};
-    float* g = P.gradTextures + d.gradOffset;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        float* q = g + (size_t(ys[k]) * d.width + xs[k]) * 3;
-        atomicAdd(q + 0, cs[0] * ws[k]);
-        atomicAdd(q + 1, cs[1] * ws[k]);
-        atomicAdd(q + 2, cs[2] * ws[k]);
+    TexPlan p;
+    if (!planLookup(d, s.mode, s.f, true, p)) return;
+    auto add = [&](uint32_t level, uint32_t x, uint32_t y, float w) {
+        const TexLevel& l = d.levels[level];
+        float* q = (level == 0 ? P.gradTextures + d.gradOffset : P.gradPyramid + (d.gradLevels + (long long)l.offset)) +
+                   (size_t(y) * l.width + x) * 3;
+        atomicAdd(q + 0, cs[0] * w);
+        atomicAdd(q + 1, cs[1] * w);
+        atomicAdd(q + 2, cs[2] * w);
+    };
+    for (int k = 0; k < 2; ++k) {
+        if (p.lw[k] == 0) continue;
+        const uint32_t level = uint32_t(p.levels[k]);
+        const TexLevel& l = d.levels[level];
+        if (p.n == 0) {
+            uint32_t x, y;
+            if (closestAt(l, p.s, p.t, x, y)) add(level, x, y, p.lw[k]);
+            continue;
+        }
+        for (int i = 0; i < p.n; ++i) {
+            float u = (p.s + p.pos[i] * p.sm) * float(l.width) - .5f, v = (p.t + p.pos[i] * p.tm) * float(l.height) - .5f;
+            if (!finite1(u) || !finite1(v) || fabsf(u) > 1.e8f || fabsf(v) > 1.e8f) continue;
+            const int x = int(floorf(u)), y = int(floorf(v));
+            u -= float(x);
+            v -= float(y);
+            const uint32_t x0 = wrapTexel(x, l.width), x1 = wrapTexel(x + 1, l.width);
+            const uint32_t y0 = wrapTexel(y, l.height), y1 = wrapTexel(y + 1, l.height);
+            const float w = p.weight[i] * p.lw[k];
+            add(level, x0, y0, (1 - u) * (1 - v) * w);
+            add(level, x1, y0, u * (1 - v) * w);
+            add(level, x0, y1, (1 - u) * v * w);
+            add(level, x1, y1, u * v * w);
+        }
     }
 }
 __device__ __forceinline__ void addRoughness(const RenderParams& P, uint32_t m, float v, float* gr) {
@@ -801,7 +1245,7 @@ __device__ __forceinline__ void addRoughness(const RenderParams& P, uint32_t m, 
     else atomicAdd(P.gradRoughness + m, v);
 }
 __device__ __forceinline__ void scatterSink(const RenderParams& P, const Sink& s, V3 R, V3 delta, float* gr) {
-    if (s.kind == SinkTexture) scatterTexture(P, s.tap, R, delta);
+    if (s.kind == SinkTexture) scatterTexture(P, s, R, delta);
     else if (s.kind == SinkRoughness) addRoughness(P, s.material, (delta.x * R.x + delta.y * R.y + delta.z * R.z) * s.g, gr);
 }
 
@@ -1366,8 +1810,15 @@ struct SampleFlagsNone {};
 struct SampleFlagsKept {
     uint32_t flags;  // SampleParked, SampleProvisional (on-demand texture tiles)
 };
-template <bool Adj, bool Pg = false>
-struct PathState : std::conditional_t<Pg, SampleFlagsKept, SampleFlagsNone> {
+// In scenes with textures every pending ray carries its differentials (diffs, by stack position; cur, the popped
+// ray's), which the map lookups' footprints come from.
+struct DiffsNone {};
+struct DiffsKept {
+    RayDiff diffs[MaxPending];
+    RayDiff cur;
+};
+template <bool Adj, bool Pg = false, bool Tx = false>
+struct PathState : std::conditional_t<Pg, SampleFlagsKept, SampleFlagsNone>, std::conditional_t<Tx, DiffsKept, DiffsNone> {
     Pending stack[MaxPending];
     uint32_t top;
     V3 radiance;
@@ -1385,8 +1836,16 @@ template <class St>
 struct HasSampleFlags {
     static constexpr bool value = false;
 };
-template <bool A>
-struct HasSampleFlags<PathState<A, true>> {
+template <bool A, bool T>
+struct HasSampleFlags<PathState<A, true, T>> {
+    static constexpr bool value = true;
+};
+template <class St>
+struct HasDiffs {
+    static constexpr bool value = false;
+};
+template <bool A, bool G>
+struct HasDiffs<PathState<A, G, true>> {
     static constexpr bool value = true;
 };
 template <class St>
@@ -1413,6 +1872,12 @@ __device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixe
     }
     Ray camera;
     if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, camera)) return false;
+    if constexpr (HasDiffs<St>::value) {  // the camera's record holds its auxiliary directions
+        RayDiff& d = S.diffs[0];
+        d.dOx = d.dOy = V3{0, 0, 0};
+        d.valid = 0;
+        if (!cameraDifferentials(c, float(pixel % c.width) + x, float(pixel / c.width) + y, d.dDx, d.dDy)) return false;
+    }
     S.cameraNear = camera.nearT;
     S.cameraFar = camera.farT;
     Pending& p = S.stack[0];
@@ -1505,6 +1970,15 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
         Ng = -Ng;
     }
     if (!finite3(N) || !finite3(Ng)) return false;
+    // The hit's differentials (shade_material.h:538-558), in scenes with textures; the texture coordinates of the
+    // triangle's corners.
+    float uv[6] = {0, 0, 1, 0, 1, 1};
+    if (P.triUV)
+        for (int q = 0; q < 6; ++q) uv[q] = P.triUV[size_t(hit.tri) * 6 + q];
+    HitDiff hd;
+    hd.dsdx = hd.dtdx = hd.dsdy = hd.dtdy = 0.0f;
+    hd.valid = false;
+    if constexpr (HasDiffs<St>::value) hitDifferentials(a, b, cc, uv, N, Ng, Pt, ray, depth == 0, S.cur, c.spp, hd);
     Bsdf bsdf;
     float rInput = 0.0f;  // the material's input roughness (the adjoint's GGX derivative)
     Sink texSink, roughSink;
@@ -1516,34 +1990,39 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
         bsdf.type = 0;
         bsdf.frame = frameOf(N);
         V3 albedo = ld3(m.albedo);
-        if (m.albedo_map) {  // the map at the hit's (s, t) (shade_material.h:139-141; cpu_jobs.h:46 apply)
-            float uv[6] = {0, 0, 1, 0, 1, 1};
-            if (P.triUV)
-                for (int q = 0; q < 6; ++q) uv[q] = P.triUV[size_t(hit.tri) * 6 + q];
-            const float w = 1.0f - hit.u - hit.v;
-            const float ss = w * uv[0] + hit.u * uv[2] + hit.v * uv[4], tt = w * uv[1] + hit.u * uv[3] + hit.v * uv[5];
-            TexTap tap;
-            const TexDesc& td = P.textures[m.albedo_map - 1];
-            if constexpr (HasSampleFlags<St>::value && !Adj) {  // the paged render (renderPagedKernel)
-                if (td.paged) {
-                    const int r = pagedTap(P, td, m.albedo_map - 1, ss, tt, tap);
-                    if (r == TapInvalid) return false;
+        if (m.albedo_map) {  // the map at the hit's (s, t) (shade_material.h:121-160, shade_mapnodes.h:420-470)
+            if constexpr (!HasDiffs<St>::value) {
+                return false;  // a scene with textures runs the kernels that carry differentials
+            } else {
+                const float w = 1.0f - hit.u - hit.v;
+                const float ss = w * uv[0] + hit.u * uv[2] + hit.v * uv[4], tt = w * uv[1] + hit.u * uv[3] + hit.v * uv[5];
+                // textureMode (shade_mapnodes.h:344-347): camera paths anisotropic, diffuse ones one level's closest
+                // texel, glossy ones trilinear closest texels.
+                const uint32_t mode = diffuseDepth ? 2u : glossyDepth ? 1u : 0u;
+                const Footprint f{ss, tt, hd.dsdx, hd.dtdx, hd.dsdy, hd.dtdy};
+                const TexDesc& td = P.textures[m.albedo_map - 1];
+                if (td.paged && !HasSampleFlags<St>::value) return false;  // paged: the paged render reads it
+                V3 color;
+                const int r = mapLookup(P, td, mode, f, true, color);
+                if (r == TapInvalid) return false;
+                if constexpr (HasSampleFlags<St>::value && !Adj) {  // the paged render (renderPagedKernel)
                     if (r == TapMiss) {  // the sample waits for its tile: it ends here, parked, and runs again
                         S.flags |= SampleParked;
                         S.top = 0;
                         return true;
                     }
                     if (r == TapCoarse) S.flags |= SampleProvisional;
-                } else if (!textureTap(td, m.albedo_map - 1, ss, tt, tap)) {
+                } else if (r != TapOk) {
                     return false;
                 }
-            } else if (!textureTap(td, m.albedo_map - 1, ss, tt, tap)) {  // false for a paged texture
-                return false;
-            }
-            albedo = albedo * tap.color;
-            if constexpr (Adj) {
-                texSink.kind = SinkTexture;
-                texSink.tap = tap;
+                albedo = albedo * color;
+                if constexpr (Adj) {
+                    texSink.kind = SinkTexture;
+                    texSink.tex = m.albedo_map - 1;
+                    texSink.mode = mode;
+                    texSink.f = f;
+                    texSink.color = color;
+                }
             }
         }
         if constexpr (Adj) {
@@ -1655,7 +2134,7 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
                 if constexpr (Adj) {
                     for (uint32_t q = 0; q < bsdf.count; ++q) {
                         const V3 share = tkq[q] * rrScale;
-                        if (bsdf.kind[q] == 0 && texSink.kind == SinkTexture) scatterTexture(P, texSink.tap, share, delta);
+                        if (bsdf.kind[q] == 0 && texSink.kind == SinkTexture) scatterTexture(P, texSink, share, delta);
                         if (bsdf.kind[q] == 1 && roughSink.kind == SinkRoughness)
                             addRoughness(P, roughSink.material, (delta.x * share.x + delta.y * share.y + delta.z * share.z) *
                                              ggxRoughnessLogDerivative(rInput, bsdf.frame.z, wo, wi), gr);
@@ -1773,6 +2252,9 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
                 }
             }
             if constexpr (Adj) sinks[top] = lobeSink;
+            if constexpr (HasDiffs<St>::value)  // async_integrator.h:2985
+                S.diffs[top] = scatterDifferentials(bsdf.kind[q] == 0, bsdf.frame, bsdf.roughness, hd, Pt, wo, wi, r1, r2,
+                                                    origin, 1.0f / fminf(sqrtf(float(c.spp ? c.spp : 1)), 2.0f));
             Pending& child = stack[top++];
             child.origin = origin;
             child.dir = wi;
@@ -1795,6 +2277,7 @@ template <bool Adj, bool Vol, class St, class Tr>
 __device__ __forceinline__ bool shadeVertexWith(const RenderParams& P, uint32_t pixel, uint32_t index, St& S, Tr& tracer,
                                                 Counters& k, TravStack& st, float* gr) {
     const Pending pv = S.stack[--S.top];
+    if constexpr (HasDiffs<St>::value) S.cur = S.diffs[S.top];
     if constexpr (Adj) {
         while (S.nrec && S.recs[S.nrec - 1].pos > S.top) {
             --S.nrec;
@@ -1847,9 +2330,9 @@ __device__ int integrate(const RenderParams& P, uint32_t pixel, uint32_t index, 
 // shading vertices together instead of waiting for the wave's longest path. feed.next gives the next sample (false:
 // none left), feed.done takes a finished sample's radiance and alpha, feed.failed a sample that hit an invalid value
 // (Error; it adds nothing).
-template <bool Adj, bool Vol, class Feed>
+template <bool Adj, bool Vol, bool Tx, class Feed>
 __device__ void runSamples(const RenderParams& P, Feed& feed, Counters& k, TravStack& st, float* gr) {
-    PathState<Adj, Feed::Paged> S;
+    PathState<Adj, Feed::Paged, Tx> S;
     S.top = 0;
     uint32_t pixel = 0, index = 0;
     bool live = false;
@@ -2029,6 +2512,12 @@ __device__ int integrateDirect(const RenderParams& P, uint32_t pixel, uint32_t i
     }
     Ray ray;
     if (!cameraRay(c, float(pixel % c.width) + x, float(pixel / c.width) + y, ray)) return 2;
+    // GpuBackend::material (async_gpu_device.h:356-383): the map through the default binding (anisotropic, cross
+    // terms in order) at the footprint of the hit ray's differentials; only the camera ray has them, so the bounces
+    // read a zero footprint (level 0).
+    V3 camDx{0, 0, 0}, camDy{0, 0, 0};
+    if (P.textures && !cameraDifferentials(c, float(pixel % c.width) + x, float(pixel / c.width) + y, camDx, camDy))
+        return 2;
     double radiance[3] = {0, 0, 0}, throughput[3] = {1, 1, 1};
     uint32_t depth = 0, primaryHit = 0;
     const V3 sky = ld3(c.sky);
@@ -2052,11 +2541,18 @@ __device__ int integrateDirect(const RenderParams& P, uint32_t pixel, uint32_t i
             if (P.triUV)
                 for (int q = 0; q < 6; ++q) uv[q] = P.triUV[size_t(hit.tri) * 6 + q];
             const float w = 1.0f - hit.u - hit.v;
-            TexTap tap;
-            if (!textureTap(P.textures[m.albedo_map - 1], m.albedo_map - 1, w * uv[0] + hit.u * uv[2] + hit.v * uv[4],
-                            w * uv[1] + hit.u * uv[3] + hit.v * uv[5], tap))
-                return 2;
-            albedo = albedo * tap.color;
+            HitDiff hd;
+            hd.dsdx = hd.dtdx = hd.dsdy = hd.dtdy = 0.0f;
+            if (!depth)
+                hitTexDerivatives(V3{tr.ax, tr.ay, tr.az}, V3{tr.bx, tr.by, tr.bz}, V3{tr.cx, tr.cy, tr.cz}, uv, ray, camDx,
+                                  camDy, c.spp, hd);
+            const Footprint f{w * uv[0] + hit.u * uv[2] + hit.v * uv[4], w * uv[1] + hit.u * uv[3] + hit.v * uv[5],
+                              hd.dsdx, hd.dtdx, hd.dsdy, hd.dtdy};
+            const TexDesc& td = P.textures[m.albedo_map - 1];
+            if (td.paged) return 2;  // the prototype estimator reads resident textures
+            V3 color;
+            if (mapLookup(P, td, 0, f, false, color) != TapOk) return 2;
+            albedo = albedo * color;
         }
         const V3 emission = ld3(m.emission), N = hit.normal;
         const uint32_t dimension = CameraDimensions + 2 * depth;
@@ -2285,7 +2781,7 @@ struct LaneFeed {
 // persistent lanes claiming items one at a time (the reference's Feed loop): with
 // Regen a lane claims its next item when its sample's last vertex is shaded (runSamples over LaneFeed), without it
 // when the whole sample is (renderItem). Every lane reaches the end so the wave's counter sums see all 64 lanes.
-template <bool Adj, bool Vol, bool Direct = false, bool Regen = false, bool Paged = false>
+template <bool Adj, bool Vol, bool Direct = false, bool Regen = false, bool Paged = false, bool Tx = false>
 __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     TravStack st;
     Counters k;
@@ -2295,7 +2791,7 @@ __device__ __forceinline__ void runLaunch(const RenderParams& P) {
     for (uint32_t i = 0; i < (Adj ? RoughnessSlots : 1); ++i) gr[i] = 0.0f;
     if constexpr (Regen) {
         LaneFeed<Adj, Paged> feed{P, errors};
-        runSamples<Adj, Vol>(P, feed, k, st, gr);
+        runSamples<Adj, Vol, Tx>(P, feed, k, st, gr);
     } else {
         for (;;) {
             const unsigned long long item = P.itemBegin + atomicAdd(&P.state->claimed, 1ull);
@@ -2338,7 +2834,12 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY renderKernel(const RenderPar
 // finish provisionally) and the launch runs the parked samples of earlier launches ahead of its chunks. Its own
 // instantiation, so the render above carries none of it.
 __global__ void __launch_bounds__(256) X2_OCCUPANCY renderPagedKernel(const RenderParams P) {
-    runLaunch<false, false, false, true, true>(P);
+    runLaunch<false, false, false, true, true, true>(P);
+}
+// The render of a scene with textures: every pending ray carries its differentials, for the maps' footprints. Its
+// own instantiation, so the render of untextured scenes carries none of it.
+__global__ void __launch_bounds__(256) X2_OCCUPANCY renderTexKernel(const RenderParams P) {
+    runLaunch<false, false, false, true, false, true>(P);
 }
 // The forward surface estimator with the wave's shadow rays pooled (the frames whose vertices make at most PoolRays
 // shadow rays: light_count x light_samples + 2 x bsdf_samples <= PoolRays).
@@ -2365,7 +2866,7 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY renderPoolKernel(const Rende
 // The adjoint at 2 waves a SIMD: its sinks and open-subtree records need 248 registers, which 3 waves could not give
 // without 141 of them spilled (the textured Cornell box's adjoint frame: 478 ms against 658 ms).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) adjointKernel(const RenderParams P) {
-    runLaunch<true, false, false, true>(P);
+    runLaunch<true, false, false, true, false, true>(P);
 }
 // ---------------------------------------------------------------------------------------------------------------
 // The wavefront form of the surface estimator: the same samples, vertex for vertex, with the traversal taken out of
@@ -2640,7 +3141,7 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY waveDrainKernel(const Render
     if (slot < W.slots && feedOpen(P)) {
         float gr[1] = {0.0f};
         LaneFeed<false> feed{P, errors};
-        runSamples<false, false>(P, feed, k, st, gr);
+        runSamples<false, false, false>(P, feed, k, st, gr);
     }
     if (errors) atomicAdd(&P.state->errors, errors);
     const unsigned long long drained = waveSum(k.traces);
@@ -2952,6 +3453,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) b
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) batchKernel3(const BatchParams B) {
     runBatch<3>(B);
 }
+// A texture's next mip level: each texel the mean of the 2 x 2 below it, ((a + b) + (c + d)) * 0.25 per channel with a,
+// b the upper row, coordinates clamped to the level below (one thread a texel).
+__global__ void pyramidKernel(const float* in, uint32_t w, uint32_t h, float* out, uint32_t ow, uint32_t oh) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ow * oh) return;
+    const uint32_t x = i % ow, y = i / ow;
+    const uint32_t x0 = min(2 * x, w - 1), x1 = min(2 * x + 1, w - 1), y0 = min(2 * y, h - 1), y1 = min(2 * y + 1, h - 1);
+    const float* a = in + (size_t(y0) * w + x0) * 3;
+    const float* b = in + (size_t(y0) * w + x1) * 3;
+    const float* c = in + (size_t(y1) * w + x0) * 3;
+    const float* d = in + (size_t(y1) * w + x1) * 3;
+    float* o = out + size_t(i) * 3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = ((a[k] + b[k]) + (c[k] + d[k])) * 0.25f;
+}
+// The transpose of one pyramid step: a level's texel gradients go a quarter each to the texels below that made it
+// (one thread a texel below, gathering from its parent; a level one texel wide or high made its parent from that
+// texel twice).
+__global__ void pushDownKernel(const float* g, uint32_t uw, uint32_t uh, float* below, uint32_t w, uint32_t h) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    const float mx = w == 1 ? 2.0f : x < 2 * uw ? 1.0f : 0.0f, my = h == 1 ? 2.0f : y < 2 * uh ? 1.0f : 0.0f;
+    if (mx == 0.0f || my == 0.0f) return;
+    const float* p = g + (size_t(y >> 1) * uw + (x >> 1)) * 3;
+    float* q = below + size_t(i) * 3;
+    const float s = 0.25f * mx * my;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q[k] += s * p[k];
+}
+
 // Zeroes a list of regions (films, counters) with plain stores, so a captured graph can replay it.
 __global__ void fillKernel(const FillRegion* regions, uint32_t n) {
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
@@ -3000,13 +3532,15 @@ __global__ void clearState(LaunchState* s) {
 
 // Host-side launchers the lane calls (x2_lane.hip).
 namespace x2 {
-template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium, 3 the prototype estimator, 4 paged, 5 pooled
+template <int Kind>  // 0 render, 1 adjoint, 2 render with a medium, 3 the prototype estimator, 4 paged, 5 pooled,
+                    // 6 textured
 hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
     const auto kernel = Kind == 0   ? renderKernel
                         : Kind == 1 ? adjointKernel
                         : Kind == 2 ? renderVolumeKernel
                         : Kind == 3 ? directKernel
                         : Kind == 5 ? renderPoolKernel
+                        : Kind == 6 ? renderTexKernel
                                     : renderPagedKernel;
     static int resident = 0;  // persistent lanes: as many workgroups as the chip holds at once
     if (!resident) {
@@ -3032,6 +3566,7 @@ hipError_t launchRender(const RenderParams& p, hipStream_t stream) {
     if (p.cfg.direct_only) return launchFrame<3>(p, stream);
     if (p.tileSlot) return launchFrame<4>(p, stream);  // texture tiles on demand
     if (p.vol.table) return launchFrame<2>(p, stream);
+    if (p.textures) return launchFrame<6>(p, stream);
     // The pool holds a vertex's shadow rays when every light's samples and every lobe's (two at most) fit, and pays
     // with two lights or more: the rays of different lights then share rounds. With one light (the Cornell box:
     // 284 ms a frame pooled against 243 ms traced where they are made) a vertex's one light ray fills the any-hit
@@ -3051,7 +3586,8 @@ hipError_t launchChainStep(const ChainParams& c, const RenderParams& p, bool uni
                        p.chainSize);
     hipLaunchKernelGGL(chainCompactKernel, dim3(1), dim3(1024), 0, stream, c);
     const hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launchFrame<0>(p, stream);
+    if (e != hipSuccess) return e;
+    return p.tileSlot ? launchFrame<4>(p, stream) : p.textures ? launchFrame<6>(p, stream) : launchFrame<0>(p, stream);
 }
 hipError_t launchErrors(const double* sums, const double* odd, const uint32_t* counts, uint32_t pixels, float* errors,
                         hipStream_t stream) {
@@ -3104,6 +3640,15 @@ hipError_t batchResident(int waves, uint32_t threads, int& blocks) {
 hipError_t launchBatch(const BatchParams& b, int waves, uint32_t threads, int blocks, hipStream_t stream) {
     if (waves == 2) hipLaunchKernelGGL(batchKernel2, dim3(unsigned(blocks)), dim3(threads), 0, stream, b);
     else hipLaunchKernelGGL(batchKernel3, dim3(unsigned(blocks)), dim3(threads), 0, stream, b);
+    return hipGetLastError();
+}
+hipError_t launchPyramid(const float* in, uint32_t w, uint32_t h, float* out, hipStream_t stream) {
+    const uint32_t ow = w > 1 ? w >> 1 : 1u, oh = h > 1 ? h >> 1 : 1u;
+    hipLaunchKernelGGL(pyramidKernel, dim3((ow * oh + 255) / 256), dim3(256), 0, stream, in, w, h, out, ow, oh);
+    return hipGetLastError();
+}
+hipError_t launchPushDown(const float* g, uint32_t uw, uint32_t uh, float* below, uint32_t w, uint32_t h, hipStream_t stream) {
+    hipLaunchKernelGGL(pushDownKernel, dim3((w * h + 255) / 256), dim3(256), 0, stream, g, uw, uh, below, w, h);
     return hipGetLastError();
 }
 hipError_t launchFill(const FillRegion* regions, uint32_t n, hipStream_t stream) {

@@ -30,6 +30,8 @@ hipError_t launchErrors(const double* sums, const double* odd, const uint32_t* c
                         hipStream_t stream);
 hipError_t launchResolve(const double* sums, const uint32_t* counts, uint32_t pixels, float* rgb, hipStream_t stream);
 hipError_t launchClearState(LaunchState* s, hipStream_t stream);
+hipError_t launchPyramid(const float* in, uint32_t w, uint32_t h, float* out, hipStream_t stream);
+hipError_t launchPushDown(const float* g, uint32_t uw, uint32_t uh, float* below, uint32_t w, uint32_t h, hipStream_t stream);
 hipError_t uploadSobol(const uint32_t* table);
 hipError_t launchRngTwin(const x2_rng_query* q, uint32_t n, uint32_t draws, uint32_t* words, float* out,
                          uint64_t* branches, hipStream_t stream);
@@ -107,7 +109,7 @@ struct Paging {
     uint64_t loads = 0;
     std::vector<uint8_t> queued;      // asked for and not yet sent up
     std::vector<uint32_t> pending;
-    std::vector<float*> coarse;       // each texture's coarse level (device)
+    std::vector<uint16_t*> coarse;    // each texture's always-resident coarse levels (device, half floats)
     uint32_t* tileSlot = nullptr;
     uint32_t* tileAsked = nullptr;
     uint32_t* tileStamp = nullptr;
@@ -182,7 +184,9 @@ struct x2dev {
     x2_light* lights = nullptr;
     uint32_t nodeCount = 0, triCount = 0, lightCount = 0, materialCount = 0, maxStack = 0;
     float* triUV = nullptr;
-    std::vector<float*> texels;   // one device buffer per texture
+    std::vector<float*> texels;   // one device buffer per texture: its mip pyramid, levels end to end
+    float* gradPyramid = nullptr;  // the adjoint's gradients of every texture's levels 1 and up
+    size_t gradPyramidFloats = 0;
     std::vector<TexDesc> texDesc;  // host copy of the descriptors
     TexDesc* textures = nullptr;
     // the medium's bricks (volDesc.table == null: none)
@@ -249,7 +253,7 @@ void releasePaging(x2dev* d) {
     (void)hipFree(g.parkedRgba);
     (void)hipFree(g.provSums);
     (void)hipFree(g.provCounts);
-    for (float* c : g.coarse) (void)hipFree(c);
+    for (uint16_t* c : g.coarse) (void)hipFree(c);
     if (g.hostProvSums) (void)hipHostFree(g.hostProvSums);
     if (g.hostProvCounts) (void)hipHostFree(g.hostProvCounts);
     if (g.readback) (void)hipHostFree(g.readback);
@@ -274,6 +278,7 @@ void release(x2dev* d) {
     (void)hipFree(d->triUV);
     (void)hipFree(d->textures);
     for (float* t : d->texels) (void)hipFree(t);
+    (void)hipFree(d->gradPyramid);
     for (void* b : d->volBuffers) (void)hipFree(b);
     (void)hipFree(d->sums);
     (void)hipFree(d->counts);
@@ -325,6 +330,37 @@ int validateConfig(const x2dev* d, const x2_config& c) {
         return fail(X2_ERR_ARGUMENT, "volume settings budget (cpu_scene.h:3168): depth <= 16, samples <= 64, "
                                      "a positive quality");
     return X2_OK;
+}
+}  // namespace
+
+namespace {
+// A texture's pyramid layout (sizes and float offsets of its levels, end to end): level k + 1 is
+// max(1, w >> 1) x max(1, h >> 1), down to 1 x 1.
+TexDesc pyramidLayout(uint32_t w, uint32_t h) {
+    TexDesc td{};
+    unsigned long long at = 0;
+    for (;;) {
+        TexLevel& l = td.levels[td.count++];
+        l.width = w;
+        l.height = h;
+        l.offset = at;
+        at += (unsigned long long)w * h * 3;
+        if ((w == 1 && h == 1) || td.count == MipLevels) break;
+        w = w > 1 ? w >> 1 : 1u;
+        h = h > 1 ? h >> 1 : 1u;
+    }
+    td.firstResident = td.count;
+    return td;
+}
+// Texture t's levels 1 and up from its level 0, on the device (queued on the lane's stream).
+hipError_t buildLevels(x2dev* d, uint32_t t) {
+    const TexDesc& td = d->texDesc[t];
+    for (uint32_t k = 1; k < td.count; ++k) {
+        const TexLevel &in = td.levels[k - 1], &out = td.levels[k];
+        const hipError_t e = launchPyramid(d->texels[t] + in.offset, in.width, in.height, d->texels[t] + out.offset, d->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 }  // namespace
 
@@ -414,6 +450,12 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
         if (s->materials[i].albedo_map > s->texture_count) return fail(X2_ERR_ARGUMENT, "albedo_map names no texture");
     if (s->volume_count > 1 || (s->volume_count && !s->volumes))
         return fail(X2_ERR_ARGUMENT, "at most one volume, and its record");
+    if (s->volume_count && s->texture_count) return fail(X2_ERR_ARGUMENT, "a scene with a medium reads no textures");
+    if (d->opt.wave_slots && s->texture_count)
+        return fail(X2_ERR_ARGUMENT, "the wavefront form renders untextured scenes (the one-kernel form carries differentials)");
+    for (uint32_t i = 0; i < s->texture_count; ++i)
+        if (s->textures[i].width > 32768u || s->textures[i].height > 32768u)
+            return fail(X2_ERR_ARGUMENT, "texture sides are at most 32768 (16 mip levels)");
     VolumeHost vol;
     if (s->volume_count) {
         std::string verr;
@@ -432,6 +474,9 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     (void)hipFree(d->triUV);
     (void)hipFree(d->textures);
     for (float* t : d->texels) (void)hipFree(t);
+    (void)hipFree(d->gradPyramid);
+    d->gradPyramid = nullptr;
+    d->gradPyramidFloats = 0;
     releasePaging(d);  // a new scene's textures are resident until paged again
     for (void*& b : d->volBuffers) {
         (void)hipFree(b);
@@ -468,14 +513,22 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     unsigned long long gradOffset = 0;
     for (uint32_t i = 0; i < s->texture_count; ++i) {
         const x2_texture& t = s->textures[i];
-        const size_t bytes = size_t(t.width) * t.height * 3 * sizeof(float);
+        TexDesc td = pyramidLayout(t.width, t.height);
+        const size_t floats = td.levels[td.count - 1].offset + 3;
         float* buf = nullptr;
-        X2_HIP(hipMalloc(&buf, bytes));
+        X2_HIP(hipMalloc(&buf, floats * sizeof(float)));
         d->texels.push_back(buf);
-        X2_HIP(hipMemcpy(buf, t.texels, bytes, hipMemcpyHostToDevice));
-        d->texDesc.push_back(TexDesc{buf, t.width, t.height, gradOffset});
+        X2_HIP(hipMemcpy(buf, t.texels, size_t(t.width) * t.height * 3 * sizeof(float), hipMemcpyHostToDevice));
+        td.texels = buf;
+        td.gradOffset = gradOffset;
         gradOffset += size_t(t.width) * t.height * 3;
+        // its levels 1 and up in the pyramid gradient, at the same spacing as in its pyramid
+        td.gradLevels = td.count > 1 ? (long long)d->gradPyramidFloats - (long long)td.levels[1].offset : 0;
+        d->gradPyramidFloats += floats - size_t(t.width) * t.height * 3;
+        d->texDesc.push_back(td);
+        X2_HIP(buildLevels(d, i));
     }
+    X2_HIP(hipStreamSynchronize(d->stream));
     if (!d->texDesc.empty()) {
         X2_HIP(hipMalloc(&d->textures, d->texDesc.size() * sizeof(TexDesc)));
         X2_HIP(hipMemcpy(d->textures, d->texDesc.data(), d->texDesc.size() * sizeof(TexDesc), hipMemcpyHostToDevice));
@@ -681,6 +734,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
     p.dImage = d->dImage;
     p.gradTextures = d->gradTextures;
     p.gradRoughness = d->gradRoughness;
+    p.gradPyramid = d->gradPyramid;
     p.ring = d->ring;
     p.sums = d->sums;
     p.counts = d->counts;
@@ -835,50 +889,48 @@ int x2dev_page_textures(x2dev* d, const x2_paging* o) {
     g.missCap = 65536;
     g.uploadCap = std::min<uint32_t>(o->pool_tiles, 1024u);
     std::vector<TexDesc> desc = d->texDesc;
-    std::vector<float> texels;
+    std::vector<float> levels;
     for (size_t t = 0; t < desc.size(); ++t) {
         TexDesc& td = desc[t];
-        const uint32_t w = td.width, h = td.height, tx = (w + TexTile - 1) / TexTile, ty = (h + TexTile - 1) / TexTile;
-        texels.resize(size_t(w) * h * 3);
-        X2_HIP(hipMemcpy(texels.data(), d->texels[t], texels.size() * sizeof(float), hipMemcpyDeviceToHost));
-        td.tileBase = g.tiles;
-        td.tilesX = tx;
-        g.tiles += tx * ty;
-        // Level 0 in tiles of half floats (a float file's levels are held in half floats, texture_core.h:30-31),
-        // a partial tile's texels past the image edge zero (never read: taps wrap inside the image).
-        const size_t base = g.hostTiles.size();
-        g.hostTiles.resize(base + size_t(tx) * ty * TexTileTexels * 3, 0);
-        for (uint32_t y = 0; y < h; ++y)
-            for (uint32_t x = 0; x < w; ++x) {
-                const size_t tile = size_t(y / TexTile) * tx + x / TexTile;
-                uint16_t* dst = g.hostTiles.data() + base + (tile * TexTileTexels + (y % TexTile) * TexTile + x % TexTile) * 3;
-                for (int c = 0; c < 3; ++c) dst[c] = x2::toHalf(texels[(size_t(y) * w + x) * 3 + c]);
-            }
-        // The coarse level: level 0 (as its halves read) averaged over boxes, at most coarseSide a side.
-        uint32_t cw = w, ch = h;
-        while (cw > coarseSide) cw = (cw + 1) / 2;
-        while (ch > coarseSide) ch = (ch + 1) / 2;
-        std::vector<float> coarse(size_t(cw) * ch * 3);
-        for (uint32_t j = 0; j < ch; ++j)
-            for (uint32_t i = 0; i < cw; ++i) {
-                const uint32_t x0 = uint32_t(uint64_t(i) * w / cw), x1 = std::max(x0 + 1, uint32_t(uint64_t(i + 1) * w / cw));
-                const uint32_t y0 = uint32_t(uint64_t(j) * h / ch), y1 = std::max(y0 + 1, uint32_t(uint64_t(j + 1) * h / ch));
-                double acc[3] = {0, 0, 0};
-                for (uint32_t y = y0; y < y1; ++y)
-                    for (uint32_t x = x0; x < x1; ++x)
-                        for (int c = 0; c < 3; ++c) acc[c] += x2::fromHalf(x2::toHalf(texels[(size_t(y) * w + x) * 3 + c]));
-                const double area = double(x1 - x0) * double(y1 - y0);
-                for (int c = 0; c < 3; ++c) coarse[(size_t(j) * cw + i) * 3 + c] = float(acc[c] / area);
-            }
-        float* dc = nullptr;
-        X2_HIP(hipMalloc(&dc, coarse.size() * sizeof(float)));
+        levels.resize(td.levels[td.count - 1].offset + 3);
+        X2_HIP(hipMemcpy(levels.data(), d->texels[t], levels.size() * sizeof(float), hipMemcpyDeviceToHost));
+        // Every level in half floats (a float file's levels are held so, texture_core.h:28-31): the levels wider or
+        // taller than coarseSide as 32 x 32 tiles on demand, the coarser ones always resident (async_gpu_device.h:
+        // 49-51). A partial tile's texels past the level's edge are zero (never read: fetches wrap inside it).
+        uint32_t first = 0;
+        while (first + 1 < td.count && std::max(td.levels[first].width, td.levels[first].height) > coarseSide) ++first;
+        td.firstResident = first;
+        for (uint32_t k = 0; k < first; ++k) {
+            TexLevel& l = td.levels[k];
+            const uint32_t w = l.width, h = l.height, tx = (w + TexTile - 1) / TexTile, ty = (h + TexTile - 1) / TexTile;
+            l.tileBase = g.tiles;
+            l.tilesX = tx;
+            g.tiles += tx * ty;
+            const size_t base = g.hostTiles.size();
+            g.hostTiles.resize(base + size_t(tx) * ty * TexTileTexels * 3, 0);
+            const float* src = levels.data() + l.offset;
+            for (uint32_t y = 0; y < h; ++y)
+                for (uint32_t x = 0; x < w; ++x) {
+                    const size_t tile = size_t(y / TexTile) * tx + x / TexTile;
+                    uint16_t* dst = g.hostTiles.data() + base + (tile * TexTileTexels + (y % TexTile) * TexTile + x % TexTile) * 3;
+                    for (int c = 0; c < 3; ++c) dst[c] = x2::toHalf(src[(size_t(y) * w + x) * 3 + c]);
+                }
+        }
+        const unsigned long long from = td.levels[first].offset;
+        std::vector<uint16_t> coarse(levels.size() - from);
+        for (size_t i = 0; i < coarse.size(); ++i) coarse[i] = x2::toHalf(levels[from + i]);
+        for (uint32_t k = first; k < td.count; ++k) td.levels[k].offset -= from;  // now into the coarse halves
+        uint16_t* dc = nullptr;
+        X2_HIP(hipMalloc(&dc, coarse.size() * sizeof(uint16_t)));
         g.coarse.push_back(dc);
-        X2_HIP(hipMemcpy(dc, coarse.data(), coarse.size() * sizeof(float), hipMemcpyHostToDevice));
+        X2_HIP(hipMemcpy(dc, coarse.data(), coarse.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
         td.coarse = dc;
-        td.coarseW = cw;
-        td.coarseH = ch;
         td.paged = 1;
         td.texels = nullptr;
+    }
+    if (!g.tiles) {
+        for (uint16_t* c : g.coarse) (void)hipFree(c);
+        return fail(X2_ERR_ARGUMENT, "coarse_side leaves no level to page (every level is no larger)");
     }
     g.slotOf.assign(g.tiles, NoTileSlot);
     g.tileIn.assign(g.poolTiles, NoTileSlot);
@@ -1288,8 +1340,9 @@ int x2dev_set_texture(x2dev* d, uint32_t texture, const float* texels, int texel
     if (d->pg.on) return fail(X2_ERR_STATE, "a paged texture's texels are the tiles x2dev_page_textures made");
     const TexDesc& t = d->texDesc[texture];
     X2_HIP(hipSetDevice(d->device));
-    X2_HIP(hipMemcpyAsync(d->texels[texture], texels, size_t(t.width) * t.height * 3 * sizeof(float),
+    X2_HIP(hipMemcpyAsync(d->texels[texture], texels, size_t(t.levels[0].width) * t.levels[0].height * 3 * sizeof(float),
                           texels_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, d->stream));
+    X2_HIP(buildLevels(d, texture));  // the pyramid made again from the new level 0
     X2_HIP(hipStreamSynchronize(d->stream));
     return X2_OK;
 }
@@ -1813,8 +1866,27 @@ int x2dev_adjoint_frame(x2dev* d, const x2_config* c, uint32_t spc, const float*
     d->dImage = d_image;
     d->gradTextures = grad_textures;
     d->gradRoughness = grad_roughness;
+    // The levels above 0 gather their texels' gradients apart; after the frame each level's go down to the level
+    // below, a quarter to each texel that made it, ending in the caller's level-0 gradient.
+    if (d->gradPyramidFloats && !d->gradPyramid) X2_HIP(hipMalloc(&d->gradPyramid, d->gradPyramidFloats * sizeof(float)));
+    if (d->gradPyramidFloats)
+        X2_HIP(hipMemsetAsync(d->gradPyramid, 0, d->gradPyramidFloats * sizeof(float), d->stream));
     x2_frame_stats st{};
     r = runFrame(d, c, spc, st);
+    for (size_t t = 0; !r && t < d->texDesc.size(); ++t) {
+        const TexDesc& td = d->texDesc[t];
+        for (uint32_t k = td.count - 1; k >= 1; --k) {
+            const TexLevel &up = td.levels[k], &in = td.levels[k - 1];
+            float* below = k == 1 ? grad_textures + td.gradOffset : d->gradPyramid + (td.gradLevels + (long long)in.offset);
+            const hipError_t e = launchPushDown(d->gradPyramid + (td.gradLevels + (long long)up.offset), up.width,
+                                                up.height, below, in.width, in.height, d->stream);
+            if (e != hipSuccess) r = hipFail(e, "launchPushDown");
+        }
+    }
+    if (!r) {
+        const hipError_t e = hipStreamSynchronize(d->stream);
+        if (e != hipSuccess) r = hipFail(e, "hipStreamSynchronize");
+    }
     d->dImage = nullptr;
     d->gradTextures = nullptr;
     d->gradRoughness = nullptr;

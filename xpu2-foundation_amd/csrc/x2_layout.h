@@ -71,17 +71,25 @@ struct alignas(64) LaunchState {
 static_assert(sizeof(LaunchState) == 64, "one line of counters a launch");
 
 // A texture on the device: its texels (RGB f32, row-major) and where its gradient starts in the adjoint's buffer.
+// A texture's mip pyramid: level k + 1 is max(1, w >> 1) x max(1, h >> 1), each texel the mean of the 2 x 2 below
+// it, down to 1 x 1 (at most MipLevels levels). Resident, every level is rgb floats end to end in `texels` (level k
+// at levels[k].offset floats). Paged (texture_core.h:37-60, the GPU's fine-detail cache, async_gpu.h:2911-3200),
+// every level is half floats: the levels finer than firstResident as 32 x 32 tiles in the device's tile pool (level
+// k's tile (x >> 5, y >> 5) is the scene-wide tile levels[k].tileBase + (y >> 5) * levels[k].tilesX + (x >> 5)), the
+// coarser ones always resident in `coarse` (level k at levels[k].offset halves).
+constexpr uint32_t MipLevels = 16;
+struct TexLevel {
+    uint32_t width, height, tileBase, tilesX;
+    unsigned long long offset;
+};
 struct TexDesc {
-    const float* texels;  // resident level 0 (null when paged)
-    uint32_t width, height;
-    unsigned long long gradOffset;  // in floats
-    // On-demand tiles (texture_core.h:37-60, the GPU's fine-detail cache, async_gpu.h:2911-3200): level 0 as 32x32
-    // tiles of half-float rgb, paged into the device's tile pool by the host; tile (x >> 5, y >> 5) is the scene-wide
-    // tile tileBase + (y >> 5) * tilesX + (x >> 5). coarse: an always-resident smaller level (coarseW x coarseH rgb
-    // floats), read where level 0's tile is missing with partial shading.
-    const float* coarse;
-    uint32_t coarseW, coarseH, tileBase, tilesX;
-    uint32_t paged, pad;
+    const float* texels;
+    const uint16_t* coarse;
+    uint32_t count, paged, firstResident, pad;
+    unsigned long long gradOffset;  // level 0's texels in the caller's texture gradient (floats)
+    long long gradLevels;  // level k's gradient (k >= 1) is at gradLevels + levels[k].offset in the device's pyramid
+                           // gradient (floats)
+    TexLevel levels[MipLevels];
 };
 constexpr uint32_t TexTile = 32, TexTileTexels = TexTile * TexTile;
 constexpr uint32_t NoTileSlot = 0xffffffffu;
@@ -127,6 +135,7 @@ struct RenderParams {
     const float* dImage;
     float* gradTextures;
     float* gradRoughness;
+    float* gradPyramid;  // the texture gradients of levels 1 and up (TexDesc::gradLevels), pushed down after a frame
     const x2_chunk* ring;
     double* sums;
     uint32_t* counts;

@@ -208,8 +208,9 @@ class GpuLane:
 
 
     def page_textures(self, pool_tiles, coarse_side=0, partial=False):
-        """On-demand texture tiles (x2dev_page_textures): level 0 in 32x32 half-float tiles, pool_tiles of them on the
-        device; with partial, a sample meeting a missing tile finishes from the coarse level, provisionally."""
+        """On-demand texture tiles (x2dev_page_textures): every mip level wider or taller than coarse_side (0: 64) in
+        32x32 half-float tiles, pool_tiles of them on the device, the coarser levels resident; with partial, a sample
+        meeting a missing tile reads the texel's parent on the nearest resident level up, provisionally."""
         abi.check(self.lib.x2dev_page_textures(self.h, ctypes.byref(abi.Paging(pool_tiles, coarse_side,
                                                                                  1 if partial else 0, 0))),
                   "x2dev_page_textures")
