@@ -363,15 +363,13 @@ def grid_leg(reps, windows=(100.0, 10.0)):
 def paging_leg(reps):
     """Texture tiles on demand on the differentiable Cornell box's forward frame (512x512, 32 spp, its back wall a
     512x512 map): the texture resident, then its levels wider than 64 paged into a pool as large as them (336 tiles;
-    the first frame parks every sample that meets a missing tile and runs it again), paged with partial shading, and
-    paged into a pool of 96 tiles, under a third of them: the footprints' levels are the working set, not level 0."""
+    the first frame parks every sample that meets a missing tile and runs it again), and paged with partial shading."""
     from x2amd import scenes
 
     scene, config, params = scenes.cornell_textured()
     out = {"workload": f"cornell textured {config.width}x{config.height}, {config.spp} spp, depth {config.max_depth}, "
                        "512x512 map, levels 512..128 in 336 tiles of 32x32 half floats", "reps": reps}
-    for name, paging in [("resident", None), ("paged", (336, False)), ("paged_partial", (336, True)),
-                         ("paged_pool_96", (96, False))]:
+    for name, paging in [("resident", None), ("paged", (336, False)), ("paged_partial", (336, True))]:
         lane = make_lane()
         lane.prepare(scene)
         if paging:

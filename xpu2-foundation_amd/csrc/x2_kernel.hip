@@ -944,10 +944,12 @@ __device__ int mapLookup(const RenderParams& P, const TexDesc& tex, uint32_t mod
 // Ray differentials (shade_state.h:30-33; shade_material.h:388-558, :691-705; shade_lobes_eval.h:725-741;
 // shade_math.h:185-207, :279-288), for Lambert and GGX lobes on flat triangles (dN/dx = dN/dy = 0). A pending ray
 // carries its differentials in scenes with textures; the camera's carries its auxiliary directions instead.
-struct RayDiff {
+// Invalid differentials are marked by a NaN in dOx.x (48 bytes a record, three aligned 16-byte words).
+struct alignas(16) RayDiff {
     V3 dOx, dOy, dDx, dDy;
-    uint32_t valid;
 };
+__device__ __forceinline__ bool diffValid(const RayDiff& d) { return d.dOx.x == d.dOx.x; }
+__device__ __forceinline__ float notANumber() { return __int_as_float(0x7fc00000); }
 struct HitDiff {
     float dsdx, dtdx, dsdy, dtdy;
     V3 dPdx, dPdy;
@@ -1011,7 +1013,7 @@ __device__ void hitTexDerivatives(V3 a, V3 b, V3 c, const float st[6], const Ray
 __device__ __forceinline__ void transferDifferentials(const RayDiff& d, V3 origin, V3 dir, V3 Ng, V3 Pt, HitDiff& h) {
     h.dPdx = h.dPdy = V3{0, 0, 0};
     h.valid = false;
-    if (!d.valid) return;
+    if (!diffValid(d)) return;
     const V3 oX = origin + d.dOx, oY = origin + d.dOy, dX = dir + d.dDx, dY = dir + d.dDy;
     const float denomX = dot(Ng, dX), denomY = dot(Ng, dY);
     if (denomX == 0.0f || denomY == 0.0f) return;
@@ -1068,17 +1070,18 @@ __device__ void differentialTexDerivatives(V3 dPds, V3 dPdt, V3 Ng, HitDiff& h) 
 // hitTexDerivatives and its auxiliary rays, scaled for the samples per pixel, as the incoming differentials; any
 // other ray's transferred and turned into texture derivatives. in: the pending ray's record (the camera's holds its
 // auxiliary directions in dDx and dDy).
+// mapped: the hit's material reads a map (else its texture derivatives are not needed, only dP/dx and dP/dy for the
+// rays it scatters).
 __device__ void hitDifferentials(V3 a, V3 b, V3 c, const float st[6], V3 N, V3 Ng, V3 Pt, const Ray& ray, bool camera,
-                                 const RayDiff& in, uint32_t spp, HitDiff& h) {
+                                 const RayDiff& in, uint32_t spp, bool mapped, HitDiff& h) {
     if (camera && !isExactlyZero(in.dDx) && !isExactlyZero(in.dDy) && spp) {
-        hitTexDerivatives(a, b, c, st, ray, in.dDx, in.dDy, spp, h);
+        if (mapped) hitTexDerivatives(a, b, c, st, ray, in.dDx, in.dDy, spp, h);
         const float scale = 1.f / fminf(sqrtf(float(spp)), 2.f);
         RayDiff d;
         d.dOx = V3{0, 0, 0} * scale;
         d.dOy = V3{0, 0, 0} * scale;
         d.dDx = normalized(ray.direction + (in.dDx - ray.direction) * scale) - ray.direction;
         d.dDy = normalized(ray.direction + (in.dDy - ray.direction) * scale) - ray.direction;
-        d.valid = 1;
         h.inDx = d.dDx;
         h.inDy = d.dDy;
         transferDifferentials(d, ray.origin, ray.direction, Ng, Pt, h);
@@ -1087,6 +1090,7 @@ __device__ void hitDifferentials(V3 a, V3 b, V3 c, const float st[6], V3 N, V3 N
     h.inDx = in.dDx;
     h.inDy = in.dDy;
     transferDifferentials(in, ray.origin, ray.direction, Ng, Pt, h);
+    if (!mapped) return;
     V3 dPds, dPdt;
     if (!trianglePartials(a, b, c, st, dPds, dPdt)) {
         const Frame f = frameOf(N);
@@ -1101,8 +1105,8 @@ __device__ void hitDifferentials(V3 a, V3 b, V3 c, const float st[6], V3 N, V3 N
 __device__ RayDiff scatterDifferentials(bool lambert, const Frame& frame, float alpha, const HitDiff& h, V3 Pt, V3 wo,
                                         V3 wi, float r1, float r2, V3 childOrigin, float primaryScale) {
     RayDiff d;
-    d.valid = 0;
     d.dOx = d.dOy = d.dDx = d.dDy = V3{0, 0, 0};
+    d.dOx.x = notANumber();
     if (!h.valid) return d;
     V3 dDdx = h.inDx, dDdy = h.inDy;
     if (lambert) {
@@ -1139,7 +1143,7 @@ __device__ RayDiff scatterDifferentials(bool lambert, const Frame& frame, float 
     d.dOy = h.dPdy * scale + shift;
     d.dDx = normalized(wi + dDdx * scale) - wi;
     d.dDy = normalized(wi + dDdy * scale) - wi;
-    d.valid = finite3(d.dOx) && finite3(d.dOy) && finite3(d.dDx) && finite3(d.dDy) ? 1u : 0u;
+    if (!(finite3(d.dOx) && finite3(d.dOy) && finite3(d.dDx) && finite3(d.dDy))) d.dOx.x = notANumber();
     return d;
 }
 // The camera ray's auxiliary directions through the next pixel in x and in y (camera_core.h:222-224, :245-246).
@@ -1196,9 +1200,41 @@ struct Record {
 };
 constexpr uint32_t MaxRecords = 34;
 
+// The adjoint workgroup's gradients of the small levels (TexLevel::lds), rgb a texel.
+__device__ __forceinline__ float* adjointLds() {
+    __shared__ float g[AdjLdsTexels * 3];
+    return g;
+}
+__device__ __forceinline__ float* levelGradient(const RenderParams& P, const TexDesc& d, uint32_t level) {
+    return level == 0 ? P.gradTextures + d.gradOffset : P.gradPyramid + (d.gradLevels + (long long)d.levels[level].offset);
+}
+__device__ void adjointLdsClear() {
+    float* g = adjointLds();
+    for (uint32_t i = threadIdx.x; i < AdjLdsTexels * 3; i += blockDim.x) g[i] = 0.0f;
+    __syncthreads();
+}
+// At the workgroup's end: its small levels' gradients into the global ones, one atomic a nonzero value.
+__device__ void adjointLdsFlush(const RenderParams& P) {
+    __syncthreads();
+    const float* g = adjointLds();
+    for (uint32_t t = 0; t < P.textureCount; ++t) {
+        const TexDesc& d = P.textures[t];
+        for (uint32_t k = 0; k < d.count; ++k) {
+            const TexLevel& l = d.levels[k];
+            if (l.lds == NoLds) continue;
+            float* out = levelGradient(P, d, k);
+            const uint32_t n = l.width * l.height * 3;
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const float v = g[l.lds * 3 + i];
+                if (v != 0.0f) atomicAdd(out + i, v);
+            }
+        }
+    }
+}
 // A contribution R of a lobe whose albedo the map scales has d R / d value = R / value per channel, and the value is
 // linear in the texels its lookup reads: each gets its weight's share, level 0's into the caller's gradient, the
-// levels above into the pyramid gradient (pushed down to level 0 after the frame).
+// levels above into the pyramid gradient (pushed down to level 0 after the frame), the small levels by way of the
+// workgroup's LDS.
 __device__ void scatterTexture(const RenderParams& P, const Sink& s, V3 R, V3 delta) {
     const TexDesc& d = P.textures[s.tex];
     const float cs[3] = {s.color.x > 0.0f ? delta.x * R.x / s.color.x : 0.0f,
@@ -1209,8 +1245,14 @@ __device__ void scatterTexture(const RenderParams& P, const Sink& s, V3 R, V3 de
     if (!planLookup(d, s.mode, s.f, true, p)) return;
     auto add = [&](uint32_t level, uint32_t x, uint32_t y, float w) {
         const TexLevel& l = d.levels[level];
-        float* q = (level == 0 ? P.gradTextures + d.gradOffset : P.gradPyramid + (d.gradLevels + (long long)l.offset)) +
-                   (size_t(y) * l.width + x) * 3;
+        if (l.lds != NoLds) {
+            float* q = adjointLds() + (l.lds + y * l.width + x) * 3;
+            atomicAdd(q + 0, cs[0] * w);
+            atomicAdd(q + 1, cs[1] * w);
+            atomicAdd(q + 2, cs[2] * w);
+            return;
+        }
+        float* q = levelGradient(P, d, level) + (size_t(y) * l.width + x) * 3;
         atomicAdd(q + 0, cs[0] * w);
         atomicAdd(q + 1, cs[1] * w);
         atomicAdd(q + 2, cs[2] * w);
@@ -1810,12 +1852,11 @@ struct SampleFlagsNone {};
 struct SampleFlagsKept {
     uint32_t flags;  // SampleParked, SampleProvisional (on-demand texture tiles)
 };
-// In scenes with textures every pending ray carries its differentials (diffs, by stack position; cur, the popped
-// ray's), which the map lookups' footprints come from.
+// In scenes with textures every pending ray carries its differentials (diffs, by stack position: the popped ray's
+// stays at the stack's top until the vertex pushes its first child), which the map lookups' footprints come from.
 struct DiffsNone {};
 struct DiffsKept {
     RayDiff diffs[MaxPending];
-    RayDiff cur;
 };
 template <bool Adj, bool Pg = false, bool Tx = false>
 struct PathState : std::conditional_t<Pg, SampleFlagsKept, SampleFlagsNone>, std::conditional_t<Tx, DiffsKept, DiffsNone> {
@@ -1875,7 +1916,7 @@ __device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixe
     if constexpr (HasDiffs<St>::value) {  // the camera's record holds its auxiliary directions
         RayDiff& d = S.diffs[0];
         d.dOx = d.dOy = V3{0, 0, 0};
-        d.valid = 0;
+        d.dOx.x = notANumber();
         if (!cameraDifferentials(c, float(pixel % c.width) + x, float(pixel / c.width) + y, d.dDx, d.dDy)) return false;
     }
     S.cameraNear = camera.nearT;
@@ -1978,10 +2019,13 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
     HitDiff hd;
     hd.dsdx = hd.dtdx = hd.dsdy = hd.dtdy = 0.0f;
     hd.valid = false;
-    if constexpr (HasDiffs<St>::value) hitDifferentials(a, b, cc, uv, N, Ng, Pt, ray, depth == 0, S.cur, c.spp, hd);
+    if constexpr (HasDiffs<St>::value)
+        hitDifferentials(a, b, cc, uv, N, Ng, Pt, ray, depth == 0, S.diffs[top], c.spp, P.materials[tr.material].albedo_map != 0,
+                         hd);
     Bsdf bsdf;
     float rInput = 0.0f;  // the material's input roughness (the adjoint's GGX derivative)
     Sink texSink, roughSink;
+    V3 texDirect{0, 0, 0};  // the direct light through the textured Lambert lobe, scattered once at the end
     texSink.kind = SinkNone;
     roughSink.kind = SinkNone;
     {
@@ -2134,7 +2178,7 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
                 if constexpr (Adj) {
                     for (uint32_t q = 0; q < bsdf.count; ++q) {
                         const V3 share = tkq[q] * rrScale;
-                        if (bsdf.kind[q] == 0 && texSink.kind == SinkTexture) scatterTexture(P, texSink, share, delta);
+                        if (bsdf.kind[q] == 0 && texSink.kind == SinkTexture) texDirect = texDirect + share;
                         if (bsdf.kind[q] == 1 && roughSink.kind == SinkRoughness)
                             addRoughness(P, roughSink.material, (delta.x * share.x + delta.y * share.y + delta.z * share.z) *
                                              ggxRoughnessLogDerivative(rInput, bsdf.frame.z, wo, wi), gr);
@@ -2233,7 +2277,10 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
                                                     c.volume_transmittance_threshold, vs);
                 }
                 radiance = radiance + direct;
-                if constexpr (Adj) scatterSink(P, lobeSink, direct, delta, gr);
+                if constexpr (Adj) {
+                    if (lobeSink.kind == SinkTexture) texDirect = texDirect + direct;
+                    else scatterSink(P, lobeSink, direct, delta, gr);
+                }
             }
             if (!bounces) continue;
             float keptShare = 1.0f;
@@ -2268,6 +2315,9 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
             child.held = directBefore;
         }
     }
+    // The vertex's direct light through its textured lobe, scattered to the texels once (the scatter is linear in R).
+    if constexpr (Adj)
+        if (texSink.kind == SinkTexture && !isExactlyZero(texDirect)) scatterTexture(P, texSink, texDirect, delta);
     return true;
 }
 
@@ -2277,7 +2327,6 @@ template <bool Adj, bool Vol, class St, class Tr>
 __device__ __forceinline__ bool shadeVertexWith(const RenderParams& P, uint32_t pixel, uint32_t index, St& S, Tr& tracer,
                                                 Counters& k, TravStack& st, float* gr) {
     const Pending pv = S.stack[--S.top];
-    if constexpr (HasDiffs<St>::value) S.cur = S.diffs[S.top];
     if constexpr (Adj) {
         while (S.nrec && S.recs[S.nrec - 1].pos > S.top) {
             --S.nrec;
@@ -2866,7 +2915,9 @@ __global__ void __launch_bounds__(256) X2_OCCUPANCY renderPoolKernel(const Rende
 // The adjoint at 2 waves a SIMD: its sinks and open-subtree records need 248 registers, which 3 waves could not give
 // without 141 of them spilled (the textured Cornell box's adjoint frame: 478 ms against 658 ms).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) adjointKernel(const RenderParams P) {
+    adjointLdsClear();
     runLaunch<true, false, false, true, false, true>(P);
+    adjointLdsFlush(P);
 }
 // ---------------------------------------------------------------------------------------------------------------
 // The wavefront form of the surface estimator: the same samples, vertex for vertex, with the traversal taken out of

@@ -529,6 +529,21 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
         X2_HIP(buildLevels(d, i));
     }
     X2_HIP(hipStreamSynchronize(d->stream));
+    {  // the adjoint's LDS levels: the coarsest of every texture first, while they fit
+        for (TexDesc& td : d->texDesc)
+            for (uint32_t k = 0; k < td.count; ++k) td.levels[k].lds = NoLds;
+        uint32_t used = 0;
+        for (uint32_t k = 0; k < MipLevels; ++k)  // k levels up from each texture's 1 x 1
+            for (TexDesc& td : d->texDesc) {
+                if (k >= td.count) continue;
+                TexLevel& l = td.levels[td.count - 1 - k];
+                const uint32_t n = l.width * l.height;
+                if (n <= AdjLdsTexels - used) {
+                    l.lds = used;
+                    used += n;
+                }
+            }
+    }
     if (!d->texDesc.empty()) {
         X2_HIP(hipMalloc(&d->textures, d->texDesc.size() * sizeof(TexDesc)));
         X2_HIP(hipMemcpy(d->textures, d->texDesc.data(), d->texDesc.size() * sizeof(TexDesc), hipMemcpyHostToDevice));
@@ -735,6 +750,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
     p.gradTextures = d->gradTextures;
     p.gradRoughness = d->gradRoughness;
     p.gradPyramid = d->gradPyramid;
+    p.textureCount = uint32_t(d->texDesc.size());
     p.ring = d->ring;
     p.sums = d->sums;
     p.counts = d->counts;

@@ -78,9 +78,15 @@ static_assert(sizeof(LaunchState) == 64, "one line of counters a launch");
 // k's tile (x >> 5, y >> 5) is the scene-wide tile levels[k].tileBase + (y >> 5) * levels[k].tilesX + (x >> 5)), the
 // coarser ones always resident in `coarse` (level k at levels[k].offset halves).
 constexpr uint32_t MipLevels = 16;
+// The adjoint gathers the gradients of small levels (at most AdjLdsTexels texels over all textures, the coarsest
+// first: the levels diffuse paths read, where every lane's atomics would meet on a few texels) in each workgroup's
+// LDS and adds them to the global gradients once, at the workgroup's end. lds: the level's first texel there, or
+// NoLds.
+constexpr uint32_t AdjLdsTexels = 2048, NoLds = 0xffffffffu;
 struct TexLevel {
     uint32_t width, height, tileBase, tilesX;
     unsigned long long offset;
+    uint32_t lds, ldsPad;
 };
 struct TexDesc {
     const float* texels;
@@ -179,6 +185,7 @@ struct RenderParams {
     // Failure injection: nonzero k makes every sample of pixel k - 1 fail as it starts, counted in the launch's
     // errors like any sample the estimator cannot run (async_gpu.cu:363), so the frame fails as the reference's does.
     uint32_t failPixel;
+    uint32_t textureCount, textureCountPad;
 };
 constexpr uint32_t PassSlots = 4, PassShift = 16, MaxPasses = 160;  // async_integrator.h:3083
 
