@@ -39,6 +39,14 @@ typedef struct x2_parked {
     uint32_t pixel, index, tries, pass;
 } x2_parked;
 
+/* A provisional sample (Refine, async_integrator.h:3065-3073): one the device finished reading a missing texel's
+ * parent a level up (partial shading). Its value stays in the provisional film until the sample is rendered again
+ * in full, which takes it out. tile: the first missing tile's scene-wide id + 1 (0: not known). 32 bytes. */
+typedef struct x2_refine {
+    uint32_t pixel, index, pass, tile;
+    float rgba[4];
+} x2_refine;
+
 /* Light types and sidedness (light_core.h:19-20). Only the environment and rectangle are implemented. */
 #define X2_LIGHT_ENV 0u
 #define X2_LIGHT_RECT 1u
@@ -163,6 +171,8 @@ typedef struct x2_config {
 /* What the host learns when a launch has finished (async_gpu.h:65 Completion), plus traversal counters. */
 typedef struct x2_completion {
     uint64_t chunks, items, samples;
+    uint64_t parked_run;  /* parked samples the launch ran again (Completion::parkedRun) */
+    uint64_t refines_run; /* provisional samples it rendered again (Completion::refinesRun) */
     uint32_t errors, pad;
     double device_seconds;     /* the launch's own duration, from HIP events around it */
     uint64_t rays_trace;       /* closest-hit rays traced */
@@ -210,6 +220,15 @@ int x2dev_ready(const x2dev* d);
 /* Replaces GpuLane::launch(chunks, count, ..., samples) (async_gpu.h:1434-1500): copies the chunks into the ring and
  * queues the render. Returns 1 when queued, 0 when there was nothing to run, negative on failure. */
 int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samples);
+/* Replaces GpuLane::launch(chunks, count, refines, refineCount, samples) (async_gpu.h:1434-1500): the same, with
+ * refine_count provisional samples (at most X2_REFINES_PER_LAUNCH, from x2dev_take_refines) rendered again first. */
+#define X2_REFINES_PER_LAUNCH (1u << 17)
+int x2dev_launch_refines(x2dev* d, const x2_chunk* chunks, uint32_t count, const x2_refine* refines,
+                         uint32_t refine_count, uint64_t samples);
+/* Replaces GpuLane::takeRefines (async_gpu.h:1713-1716): the provisional samples the collected launches made (paged
+ * textures with partial shading and host_refines), at most capacity of them, oldest first. */
+int x2dev_take_refines(x2dev* d, x2_refine* out, uint32_t capacity, uint32_t* taken);
+uint64_t x2dev_refines_waiting(const x2dev* d);
 /* Replaces GpuLane::poll(Completion&) (async_gpu.h:1503-1564): never waits. 1: the oldest launch finished and
  * *out holds it; 0: nothing finished; negative on failure. */
 int x2dev_poll(x2dev* d, x2_completion* out);
@@ -363,7 +382,10 @@ int x2_pixel_errors(const double* sums, const double* odd, const uint32_t* count
  * refuse paged textures; x2dev_set_texture refuses a paged one. No counterpart for the pool size in the reference,
  * which sizes its cache from the card's memory. */
 typedef struct x2_paging {
-    uint32_t pool_tiles, coarse_side, partial, pad;
+    uint32_t pool_tiles, coarse_side, partial;
+    uint32_t host_refines; /* with partial: provisional samples go to the host as x2_refine records (the reference's
+                              flow, async_gpu.cu:274-295), to be rendered again through x2dev_launch_refines,
+                              instead of parking on the device */
 } x2_paging;
 typedef struct x2_paging_stats {
     uint64_t parked_outstanding; /* parked, not yet run again (GpuLane::parkedOutstanding, async_gpu.h:1427) */

@@ -143,6 +143,77 @@ def test_provisional_film_holds_the_parked_samples(frames):
     assert (fc == wc).all() and close(fs, ws)
 
 
+def test_host_refines_round_trip(frames):
+    """The reference's flow for provisional samples (async_gpu.cu:274-295, async_gpu.h:1434-1525, 1713): with host
+    refines a provisional sample leaves the device as a Refine record (its pixel, index and value, the tile it missed)
+    instead of parking; the host takes them and sends them back through a launch, which renders them again in full
+    (refines_run) and takes their values out of the provisional film. The first launch's film plus provisional film is
+    the oracle's frame read from the resident ancestors; after the round trip the film is the oracle's frame."""
+    scene, config, ws, wc = frames
+    cs, cc, ck = oracle_paged(scene, FIRST_RESIDENT).render(config)
+    tiles = ((config.width + 7) // 8) * ((config.height + 7) // 8)
+    every = [abi.Chunk(t, f, 4, 0) for f in range(0, config.spp, 4) for t in range(tiles)]
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        lane.page_textures(96, partial=True, host_refines=True)
+        lane.begin(config)
+        assert lane.launch((abi.Chunk * len(every))(*every))
+        done = drain(lane)
+        assert len(done) == 1 and done[0].refines_run == 0 and done[0].parked_run == 0 and done[0].errors == 0
+        refines = lane.take_refines()
+        lane.download_film()
+        while not lane.film_ready():
+            pass
+        fs, fc = lane.film()
+        ps, pc = lane.provisional_film()
+        n = len(refines)
+        assert n > 0 and int(pc.sum()) == n and lane.parked_outstanding() == 0
+        assert (fc + pc == config.spp).all() and close(fs + ps, cs)
+        mine = np.zeros_like(ps)
+        for r in refines:  # each record's value is its sample's share of the provisional film
+            assert 0 < r.tile <= TILES
+            mine[r.pixel] += np.array(r.rgba[:], np.float64)
+        assert np.abs(mine - ps).max() <= 1e-9 * max(1.0, float(np.abs(ps).max()))
+        rounds = 0
+        while len(refines):  # the tiles they missed go up with the launch that renders them again
+            assert lane.launch((abi.Chunk * 0)(), refines=refines)
+            done = drain(lane)
+            assert sum(c.refines_run for c in done) == len(refines) and all(c.errors == 0 for c in done)
+            refines = lane.take_refines()
+            rounds += 1
+            assert rounds < 8
+        lane.download_film()
+        while not lane.film_ready():
+            pass
+        fs, fc = lane.film()
+        ps, pc = lane.provisional_film()
+        g = lane.paging_stats()
+    finally:
+        lane.close()
+    print(f"host refines: {n} provisional samples, {rounds} round(s) back, provisional renders {g.provisional}")
+    assert int(pc.sum()) == 0 and np.abs(ps).max() <= 1e-9 and g.overflow == 0
+    assert (fc == wc).all() and close(fs, ws)
+
+
+def test_host_refines_through_the_frame_loop_and_the_grid(frames):
+    scene, config, ws, wc = frames
+    lane = GpuLane(0)
+    try:
+        lane.prepare(scene)
+        lane.page_textures(96, partial=True, host_refines=True)
+        fs, fc, st = lane.render_frame(config)
+        stats = lane.paging_stats()  # the first frame's: its tiles were missing
+        g = G.WorkGrid(config.width, config.height, config.spp, G.options(64, 4, 1))
+        gs, gc, fr = lane.render_grid(config, g, window_ms=5.0)
+        prov = lane.provisional_film()
+    finally:
+        lane.close()
+    assert st.errors == 0 and (fc == wc).all() and close(fs, ws)
+    assert fr.errors == 0 and (gc == wc).all() and close(gs, ws)
+    assert int(prov[1].sum()) == 0 and stats.provisional > 0 and stats.overflow == 0
+
+
 def test_frames_with_other_tiles_evict(tmp_path):
     """Camera hits only (depth 0): the left half of the image reads some of the texture's tiles, the right half others.
     In a pool of 48 of the 80 tiles the second frame evicts the first's; each equals the oracle's frame."""

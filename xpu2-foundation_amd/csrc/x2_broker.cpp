@@ -33,6 +33,15 @@ struct Broker {
     std::atomic<bool> stop{false};
     double gpuRate = 0;
     uint64_t queued = 0;
+    std::vector<x2_refine> refines;  // host refines: provisional samples taken, to render again
+
+    // A launch with up to a launch's provisional samples ahead of its chunks; those sent leave the list.
+    int launchWith(const x2_chunk* chunks, uint32_t count, uint64_t samples) {
+        const uint32_t n = uint32_t(std::min<size_t>(refines.size(), X2_REFINES_PER_LAUNCH));
+        const int r = x2dev_launch_refines(d, chunks, count, refines.data(), n, samples);
+        if (r >= 0 && n) refines.erase(refines.begin(), refines.begin() + n);
+        return r;
+    }
 
     double elapsed() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
@@ -98,6 +107,14 @@ struct Broker {
                 gpuRate = gpuRate > 0 ? 0.7 * gpuRate + 0.3 * rate : rate;
             }
             out.errors += c.errors;
+            if (x2dev_refines_waiting(d)) {  // host refines: sent back in the next launches (async_render.h:718-726)
+                const size_t at = refines.size();
+                refines.resize(at + x2dev_refines_waiting(d));
+                uint32_t got = 0;
+                const int e = x2dev_take_refines(d, refines.data() + at, uint32_t(refines.size() - at), &got);
+                refines.resize(at + got);
+                if (e < 0) return e;
+            }
             out.rays_trace += c.rays_trace;
             out.rays_shadow += c.rays_shadow;
             out.nodes_visited += c.nodes_visited;
@@ -134,9 +151,10 @@ struct Broker {
             while (useGpu && x2dev_ready(d)) {
                 const uint32_t want = launchSize(cpuThreads);
                 if (!want) {
-                    // Paged textures: parked samples run again in a launch of their own once no chunk is left.
-                    if (grid.empty() && x2dev_parked_outstanding(d)) {
-                        const int r = x2dev_launch(d, nullptr, 0, 0);
+                    // Paged textures: parked and provisional samples run again in a launch of their own once no
+                    // chunk is left.
+                    if (grid.empty() && (x2dev_parked_outstanding(d) || !refines.empty())) {
+                        const int r = launchWith(nullptr, 0, 0);
                         if (r < 0) {
                             status = r;
                             break;
@@ -172,7 +190,7 @@ struct Broker {
                     grid.rendered(claim[i].cell, Gpu, s);
                     samples += s;
                 }
-                const int r = x2dev_launch(d, claim.data(), got, samples);
+                const int r = launchWith(claim.data(), got, samples);
                 if (r < 0) {
                     status = r;
                     break;
@@ -184,7 +202,9 @@ struct Broker {
                 launched = true;
             }
             if (status < 0) break;
-            if (grid.empty() && !x2dev_in_flight(d) && !x2dev_parked_outstanding(d)) break;
+            if (grid.empty() && !x2dev_in_flight(d) && !x2dev_parked_outstanding(d) && refines.empty() &&
+                !x2dev_refines_waiting(d))
+                break;
             if (!launched) std::this_thread::sleep_for(std::chrono::microseconds(x2dev_in_flight(d) ? 20 : 100));
         }
         if (status < 0) {

@@ -821,6 +821,7 @@ struct ResidentTexels {
 struct PagedTexels {
     const RenderParams& P;
     const TexDesc& t;
+    uint32_t* firstMiss;  // the first tile found missing, + 1 (null: not kept)
     __device__ __forceinline__ bool here(uint32_t level, uint32_t x, uint32_t y, V3& out) const {
         const TexLevel& l = t.levels[level];
         if (level >= t.firstResident) {
@@ -842,6 +843,7 @@ struct PagedTexels {
         if (here(level, x, y, out)) return TapOk;
         const TexLevel& l = t.levels[level];
         const uint32_t tile = l.tileBase + (y >> 5) * l.tilesX + (x >> 5);
+        if (firstMiss && !*firstMiss) *firstMiss = tile + 1;
         if (atomicExch(P.tileAsked + tile, 1u) == 0u) {
             const uint32_t at = atomicAdd(&P.paging->missCount, 1u);
             if (at < P.missCap) P.missList[at] = tile;
@@ -933,10 +935,11 @@ __device__ int mapRead(const Texels& T, const TexDesc& tex, const TexPlan& p, V3
     return status;
 }
 // A lookup of a texture, resident or paged.
-__device__ int mapLookup(const RenderParams& P, const TexDesc& tex, uint32_t mode, const Footprint& f, bool swap, V3& out) {
+__device__ int mapLookup(const RenderParams& P, const TexDesc& tex, uint32_t mode, const Footprint& f, bool swap, V3& out,
+                         uint32_t* firstMiss = nullptr) {
     TexPlan p;
     if (!planLookup(tex, mode, f, swap, p)) return TapInvalid;
-    if (tex.paged) return mapRead(PagedTexels{P, tex}, tex, p, out);
+    if (tex.paged) return mapRead(PagedTexels{P, tex, firstMiss}, tex, p, out);
     return mapRead(ResidentTexels{tex}, tex, p, out);
 }
 
@@ -1850,7 +1853,8 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
 // subtrees (the ancestors of the vertex being shaded). With Pg (the paged render) it keeps the sample's paging flags.
 struct SampleFlagsNone {};
 struct SampleFlagsKept {
-    uint32_t flags;  // SampleParked, SampleProvisional (on-demand texture tiles)
+    uint32_t flags;     // SampleParked, SampleProvisional (on-demand texture tiles)
+    uint32_t missTile;  // the first tile it found missing, + 1 (0: none)
 };
 // In scenes with textures every pending ray carries its differentials (diffs, by stack position: the popped ray's
 // stays at the stack's top until the vertex pushes its first child), which the map lookups' footprints come from.
@@ -1894,6 +1898,11 @@ __device__ __forceinline__ uint32_t sampleFlags(const St& S) {
     if constexpr (HasSampleFlags<St>::value) return S.flags;
     else return 0u;
 }
+template <class St>
+__device__ __forceinline__ uint32_t sampleMissTile(const St& S) {
+    if constexpr (HasSampleFlags<St>::value) return S.missTile;
+    else return 0u;
+}
 
 // Starts sample (pixel, index): its camera ray becomes the one pending ray. False: an invalid value (Error).
 template <bool Adj, bool Vol, class St>
@@ -1933,7 +1942,10 @@ __device__ __forceinline__ bool startSample(const RenderParams& P, uint32_t pixe
     S.top = 1;
     S.radiance = V3{0, 0, 0};
     S.alpha = 0.0f;
-    if constexpr (HasSampleFlags<St>::value) S.flags = 0u;
+    if constexpr (HasSampleFlags<St>::value) {
+        S.flags = 0u;
+        S.missTile = 0u;
+    }
     if constexpr (Adj) {
         const float inv = 1.0f / float(c.spp);
         S.delta = V3{P.dImage[size_t(pixel) * 3] * inv, P.dImage[size_t(pixel) * 3 + 1] * inv,
@@ -2047,7 +2059,9 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
                 const TexDesc& td = P.textures[m.albedo_map - 1];
                 if (td.paged && !HasSampleFlags<St>::value) return false;  // paged: the paged render reads it
                 V3 color;
-                const int r = mapLookup(P, td, mode, f, true, color);
+                uint32_t* firstMiss = nullptr;
+                if constexpr (HasSampleFlags<St>::value) firstMiss = &S.missTile;
+                const int r = mapLookup(P, td, mode, f, true, color, firstMiss);
                 if (r == TapInvalid) return false;
                 if constexpr (HasSampleFlags<St>::value && !Adj) {  // the paged render (renderPagedKernel)
                     if (r == TapMiss) {  // the sample waits for its tile: it ends here, parked, and runs again
@@ -2389,7 +2403,7 @@ __device__ void runSamples(const RenderParams& P, Feed& feed, Counters& k, TravS
         const bool idle = S.top == 0;
         if (idle && live) {
             finishSample<Adj>(P, S, gr);
-            feed.done(pixel, index, S.radiance, S.alpha, sampleFlags(S));
+            feed.done(pixel, index, S.radiance, S.alpha, sampleFlags(S), sampleMissTile(S));
             live = false;
         }
         // A finished lane takes its next sample once Feed::Regen lanes of the wave are finished (or all are), so that
@@ -2728,6 +2742,7 @@ __device__ __forceinline__ unsigned long long chainItems(const RenderParams& P) 
     return (unsigned long long)P.chain->activeTiles * P.chainSize * TilePixels;
 }
 // With Pg, *from receives the parked slot a sample came from (~0u: a chunk's item).
+constexpr uint32_t FromRefine = 0x80000000u;  // a claimed item's origin: a provisional sample rendered again (k)
 template <bool Pg = false>
 __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel, uint32_t& index,
                                           unsigned long long chainEnd = 0, uint32_t* from = nullptr) {
@@ -2747,6 +2762,14 @@ __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel
         }
         unsigned long long k = atomicAdd(&P.state->claimed, 1ull);
         if constexpr (Pg) {
+            if (k < P.refineInCount) {  // provisional samples rendered again, first (async_gpu.cu:394-405)
+                const x2_refine& r = P.refineIn[k];
+                pixel = r.pixel;
+                index = r.index;
+                *from = FromRefine | uint32_t(k);
+                return true;
+            }
+            k -= P.refineInCount;
             if (k < P.parkedCount) {  // the parked samples this launch runs again, ahead of its chunks
                 const uint32_t at = uint32_t((P.parkedBegin + k) % P.parkCap);
                 const x2_parked r = P.parked[at];
@@ -2767,6 +2790,22 @@ __device__ __forceinline__ bool claimItem(const RenderParams& P, uint32_t& pixel
 // provisional sample's record keeps its value (v: rgb + alpha) beside it, for the provisional film to give back.
 // The launch's own parked records [parkedBegin, parkedBegin + parkedCount) are below the bound, so none is
 // overwritten while the launch reads it.
+// A provisional sample's record for the host (host refines; async_gpu.cu:276-279): past the ring's room it is
+// counted, never written (the frame then fails).
+__device__ __forceinline__ void refineRecord(const RenderParams& P, uint32_t pixel, uint32_t index, uint32_t tile,
+                                             const float* v) {
+    const unsigned long long at = atomicAdd(&P.paging->refineTail, 1ull);
+    if (at >= P.refineHead + P.refineCap) {
+        atomicAdd(&P.paging->refineOverflow, 1u);
+        return;
+    }
+    x2_refine& r = P.refineRing[at % P.refineCap];
+    r.pixel = pixel;
+    r.index = index;
+    r.pass = 0u;
+    r.tile = tile;
+    *reinterpret_cast<float4*>(r.rgba) = float4{v[0], v[1], v[2], v[3]};
+}
 __device__ __forceinline__ void parkSample(const RenderParams& P, uint32_t pixel, uint32_t index,
                                            const float* v = nullptr) {
     const unsigned long long at = atomicAdd(&P.paging->parkTail, 1ull);
@@ -2785,17 +2824,20 @@ struct LaneFeed {
     const RenderParams& P;
     uint32_t& errors;
     unsigned long long chainEnd = chainItems(P);  // read once: the compaction wrote it before this render
-    uint32_t from = ~0u;  // the parked slot the lane's sample came from (Pg; ~0u: a chunk's item)
+    uint32_t from = ~0u;  // where the lane's sample came from (Pg): a parked slot, FromRefine | k, or ~0u: a chunk's item
     __device__ bool next(uint32_t& pixel, uint32_t& index) { return claimItem<Pg>(P, pixel, index, chainEnd, &from); }
     // A sample that read a coarse level (partial shading) goes into the provisional film, in place of its old value
-    // when it was itself a provisional sample run again, and parks; one run again in full takes its old value out
-    // (async_gpu.cu:282-295), so the provisional film holds exactly the samples still parked.
-    __device__ void done(uint32_t pixel, uint32_t index, V3 r, float a, uint32_t flags) {
+    // when it was itself a provisional sample run again, and parks, or with host refines goes to the host as a record;
+    // one run again in full takes its old value out (async_gpu.cu:282-295), so the provisional film holds exactly the
+    // samples still parked or held by the host.
+    __device__ void done(uint32_t pixel, uint32_t index, V3 r, float a, uint32_t flags, uint32_t missTile = 0) {
         if constexpr (!Adj) {
             if constexpr (Pg) {
-                const bool old = P.partial && from != ~0u;
+                const bool refined = from != ~0u && (from & FromRefine);
+                const bool old = refined || (P.partial && !P.hostRefines && from != ~0u);
                 float4 was{0.0f, 0.0f, 0.0f, 0.0f};
-                if (old) was = *reinterpret_cast<const float4*>(P.parkedRgba + size_t(from) * 4);
+                if (refined) was = *reinterpret_cast<const float4*>(P.refineIn[from & ~FromRefine].rgba);
+                else if (old) was = *reinterpret_cast<const float4*>(P.parkedRgba + size_t(from) * 4);
                 double* f = P.provSums + size_t(pixel) * 4;
                 if (flags & SampleProvisional) {
                     const float v[4] = {r.x, r.y, r.z, a};
@@ -2805,10 +2847,16 @@ struct LaneFeed {
                     atomicAdd(f + 3, double(v[3]) - double(was.w));
                     if (!old) atomicAdd(P.provCounts + pixel, 1u);
                     atomicAdd(&P.paging->provisional, 1u);
-                    parkSample(P, pixel, index, v);
+                    if (P.hostRefines) refineRecord(P, pixel, index, missTile, v);
+                    else parkSample(P, pixel, index, v);
                     return;
                 }
                 if (flags) {  // parked on a missing tile: it runs again once the tile is resident
+                    if (refined) {  // a provisional sample that read no coarser level either: it stays as it was
+                        const float v[4] = {was.x, was.y, was.z, was.w};
+                        refineRecord(P, pixel, index, missTile, v);
+                        return;
+                    }
                     parkSample(P, pixel, index);
                     return;
                 }
@@ -3606,7 +3654,7 @@ hipError_t launchFrame(const RenderParams& p, hipStream_t stream) {
     }
     // The device pass chain's render learns its items on the device: a full persistent grid.
     // A launch's items: its chunks' and the parked samples it runs again (a launch may hold parked samples alone).
-    const unsigned long long items = p.chain ? ~0ull >> 1 : p.itemEnd - p.itemBegin + p.parkedCount;
+    const unsigned long long items = p.chain ? ~0ull >> 1 : p.itemEnd - p.itemBegin + p.parkedCount + p.refineInCount;
     unsigned long long blocks = (items + 255) / 256;
     if (blocks > (unsigned long long)resident) blocks = resident;
     if (!blocks) return hipSuccess;

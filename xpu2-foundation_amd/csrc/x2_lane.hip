@@ -91,13 +91,22 @@ struct Flight {
     uint32_t slot;
     bool wave;  // run in the wavefront form
     uint64_t parkedBegin = 0, parkedCount = 0, items = 0;  // parked samples it ran again; items it could park
+    uint64_t refines = 0;  // provisional samples it rendered again
 };
 
 // On-demand texture tiles (GpuLane's fine-detail cache, async_gpu.h:2911-3200): every paged texture's level 0 as
 // 32x32 tiles of half-float rgb on the host, a pool of them on the device with the scene-wide page table, the tiles the
 // kernels asked for, the ring of parked samples and the provisional film.
 struct Paging {
-    bool on = false, partial = false;
+    bool on = false, partial = false, hostRefines = false;
+    // host refines: the ring the kernels write provisional records to (refineCap; those below refineHead are taken,
+    // into refinesOut), and each launch slot's records to render again (device, and their pinned staging)
+    x2_refine* refineRing = nullptr;
+    x2_refine* refineReadback = nullptr;
+    x2_refine* refineIn[Slots] = {};
+    x2_refine* refineStaging[Slots] = {};
+    uint64_t refineCap = 0, refineHead = 0;
+    std::deque<x2_refine> refinesOut;
     uint32_t tiles = 0, poolTiles = 0, missCap = 0, uploadCap = 0;
     uint64_t parkCap = 0;
     std::vector<uint16_t> hostTiles;  // tile t's texels at t * TexTileTexels * 3
@@ -254,6 +263,12 @@ void releasePaging(x2dev* d) {
     (void)hipFree(g.provSums);
     (void)hipFree(g.provCounts);
     for (uint16_t* c : g.coarse) (void)hipFree(c);
+    (void)hipFree(g.refineRing);
+    if (g.refineReadback) (void)hipHostFree(g.refineReadback);
+    for (uint32_t i = 0; i < Slots; ++i) {
+        (void)hipFree(g.refineIn[i]);
+        if (g.refineStaging[i]) (void)hipHostFree(g.refineStaging[i]);
+    }
     if (g.hostProvSums) (void)hipHostFree(g.hostProvSums);
     if (g.hostProvCounts) (void)hipHostFree(g.hostProvCounts);
     if (g.readback) (void)hipHostFree(g.readback);
@@ -639,6 +654,18 @@ int beginPaging(x2dev* d, size_t pixels) {
         X2_HIP(hipMalloc(&g.parkedRgba, cap * 4 * sizeof(float)));
         g.parkCap = cap;
     }
+    if (g.hostRefines && cap > g.refineCap) {  // as many provisional records as the parked ring holds samples
+        (void)hipFree(g.refineRing);
+        if (g.refineReadback) (void)hipHostFree(g.refineReadback);
+        g.refineRing = nullptr;
+        g.refineReadback = nullptr;
+        g.refineCap = 0;
+        X2_HIP(hipMalloc(&g.refineRing, cap * sizeof(x2_refine)));
+        X2_HIP(hipHostMalloc(&g.refineReadback, cap * sizeof(x2_refine), hipHostMallocDefault));
+        g.refineCap = cap;
+    }
+    g.refineHead = 0;
+    g.refinesOut.clear();
     if (pixels != g.provPixels) {
         (void)hipFree(g.provSums);
         (void)hipFree(g.provCounts);
@@ -729,6 +756,8 @@ uint32_t x2dev_chunk_room(const x2dev* d) {
         const uint64_t bound = g.parkTail + g.inflightItems + (g.parkTail - g.parkIssued);
         const uint64_t limit = g.parkIssued + g.parkCap;
         room = std::min<uint64_t>(room, bound >= limit ? 0 : (limit - bound) / d->itemsPerChunk);
+        if (g.hostRefines)  // what the launch may write as provisional records
+            room = std::min<uint64_t>(room, g.inflightItems >= g.refineCap ? 0 : (g.refineCap - g.inflightItems) / d->itemsPerChunk);
     }
     return uint32_t(std::min<uint64_t>(room, d->opt.max_chunks_per_launch));
 }
@@ -775,6 +804,9 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
         p.missCap = g.missCap;
         p.partial = g.partial ? 1u : 0u;
         p.tileStamp = g.tileStamp;
+        p.hostRefines = g.hostRefines ? 1u : 0u;
+        p.refineRing = g.refineRing;
+        p.refineCap = uint32_t(std::min<uint64_t>(g.refineCap, 0xffffffffu));
     }
     p.vol = d->volDesc;
     p.odd = d->adaptive ? d->odd : nullptr;
@@ -896,11 +928,13 @@ int x2dev_page_textures(x2dev* d, const x2_paging* o) {
     if (d->volDesc.table) return fail(X2_ERR_STATE, "paged textures: the medium kernel reads resident textures");
     if (d->opt.wave_slots) return fail(X2_ERR_STATE, "paged textures: the wavefront form reads resident textures");
     if (o->pool_tiles == 0 || o->pool_tiles > (1u << 22)) return fail(X2_ERR_ARGUMENT, "pool_tiles must be 1..2^22");
+    if (o->host_refines && !o->partial) return fail(X2_ERR_ARGUMENT, "host_refines needs partial shading");
     const uint32_t coarseSide = o->coarse_side ? o->coarse_side : 64u;
     X2_HIP(hipSetDevice(d->device));
     Paging g;
     g.on = true;
     g.partial = o->partial != 0;
+    g.hostRefines = o->host_refines != 0;
     g.poolTiles = o->pool_tiles;
     g.missCap = 65536;
     g.uploadCap = std::min<uint32_t>(o->pool_tiles, 1024u);
@@ -970,6 +1004,10 @@ int x2dev_page_textures(x2dev* d, const x2_paging* o) {
         X2_HIP(hipHostMalloc(&g.stampReadback[i], size_t(g.tiles) * sizeof(uint32_t), hipHostMallocDefault));
         X2_HIP(hipHostMalloc(&g.upload[i], size_t(g.uploadCap) * tileBytes, hipHostMallocDefault));
         X2_HIP(hipHostMalloc(&g.tableStaging[i], size_t(g.tiles) * sizeof(uint32_t), hipHostMallocDefault));
+        if (g.hostRefines) {
+            X2_HIP(hipMalloc(&g.refineIn[i], size_t(X2_REFINES_PER_LAUNCH) * sizeof(x2_refine)));
+            X2_HIP(hipHostMalloc(&g.refineStaging[i], size_t(X2_REFINES_PER_LAUNCH) * sizeof(x2_refine), hipHostMallocDefault));
+        }
     }
     // The descriptors now read the tiles; level 0's resident copies are let go.
     X2_HIP(hipMemcpy(d->textures, desc.data(), desc.size() * sizeof(TexDesc), hipMemcpyHostToDevice));
@@ -1013,10 +1051,23 @@ int x2dev_provisional_film(const x2dev* d, const double** sums, const uint32_t**
 }
 
 int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samples) {
+    return x2dev_launch_refines(d, chunks, count, nullptr, 0, samples);
+}
+int x2dev_launch_refines(x2dev* d, const x2_chunk* chunks, uint32_t count, const x2_refine* refines,
+                         uint32_t refine_count, uint64_t samples) {
     if (!d) return fail(X2_ERR_ARGUMENT, "null device");
     if (!d->begun_) return fail(X2_ERR_STATE, "launch before begin");
+    if (refine_count) {
+        if (!refines) return fail(X2_ERR_ARGUMENT, "null refines");
+        if (!d->pg.on || !d->pg.hostRefines) return fail(X2_ERR_STATE, "provisional samples to render again, but no host refines");
+        if (refine_count > X2_REFINES_PER_LAUNCH)
+            return fail(X2_ERR_ARGUMENT, "Too many provisional samples for one launch (async_gpu.h:1436)");
+        const uint32_t pixels = d->cfg.width * d->cfg.height;
+        for (uint32_t i = 0; i < refine_count; ++i)
+            if (refines[i].pixel >= pixels) return fail(X2_ERR_ARGUMENT, "a provisional sample's pixel is outside the image");
+    }
     const uint64_t parkedCount = d->pg.on ? d->pg.parkTail - d->pg.parkIssued : 0;
-    if (!count && !parkedCount) return 0;  // nothing to run: no chunks and no parked sample
+    if (!count && !parkedCount && !refine_count) return 0;  // nothing to run: no chunks, parked or provisional samples
     if (count && !chunks) return fail(X2_ERR_ARGUMENT, "null chunks");
     if (d->inFlight.size() >= 2 || count > d->opt.max_chunks_per_launch)
         return fail(X2_ERR_BACKPRESSURE, "GPU launch backpressure (async_gpu.h:1435)");
@@ -1030,13 +1081,15 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     const uint32_t slot = uint32_t(d->launches % Slots);
     const bool wave = waveForm(d);
     uint64_t parkedBegin = 0;
-    const uint64_t items = parkedCount + uint64_t(count) * d->itemsPerChunk;
+    const uint64_t items = refine_count + parkedCount + uint64_t(count) * d->itemsPerChunk;
     if (d->pg.on) {
         Paging& g = d->pg;
         if (wave) return fail(X2_ERR_STATE, "paged textures: the wavefront form reads resident textures");
         if (parkedCount > 0xffffffffu) return fail(X2_ERR_BACKPRESSURE, "too many parked samples for one launch");
         if (g.parkTail + g.inflightItems + items > g.parkIssued + g.parkCap)
             return fail(X2_ERR_BACKPRESSURE, "parked-sample ring would overflow (async_gpu.h:1447)");
+        if (g.hostRefines && g.inflightItems + items > g.refineCap)
+            return fail(X2_ERR_BACKPRESSURE, "Provisional-sample ring would overflow (async_gpu.h:1444)");
         const int r = commitTiles(d, slot);  // the tiles asked for so far, before the samples that wait for them
         if (r) return r;
         parkedBegin = g.parkIssued;
@@ -1051,12 +1104,20 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
                               d->stream));
         done += n;
     }
+    if (refine_count) {
+        std::memcpy(d->pg.refineStaging[slot], refines, size_t(refine_count) * sizeof(x2_refine));
+        X2_HIP(hipMemcpyAsync(d->pg.refineIn[slot], d->pg.refineStaging[slot], size_t(refine_count) * sizeof(x2_refine),
+                              hipMemcpyHostToDevice, d->stream));
+    }
     X2_HIP(launchClearState(d->state + slot, d->stream));
     RenderParams p = renderParams(d, slot);
     p.itemBegin = d->published * d->itemsPerChunk;
     p.itemEnd = (d->published + count) * d->itemsPerChunk;
     p.parkedBegin = parkedBegin;
     p.parkedCount = uint32_t(parkedCount);
+    p.refineIn = d->pg.refineIn[slot];
+    p.refineInCount = refine_count;
+    p.refineHead = d->pg.refineHead;
     p.stamp = uint32_t(d->launches + 1);
     if (wave) {
         const uint32_t perVertex = d->cfg.light_count * std::max(1u, d->cfg.light_samples) +
@@ -1095,6 +1156,7 @@ int x2dev_launch(x2dev* d, const x2_chunk* chunks, uint32_t count, uint64_t samp
     flight.parkedBegin = parkedBegin;
     flight.parkedCount = parkedCount;
     flight.items = items;
+    flight.refines = refine_count;
     d->inFlight.push_back(flight);
     d->published += count;
     ++d->launches;
@@ -1199,8 +1261,18 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
         }
         g.inflightItems -= std::min(g.inflightItems, f.items);
         g.provisional += rb.provisional;
-        g.overflow += rb.parkOverflow;
-        lost = rb.parkOverflow;
+        g.overflow += rb.parkOverflow + rb.refineOverflow;
+        lost = rb.parkOverflow + rb.refineOverflow;
+        if (g.hostRefines) {  // the provisional records it wrote, to the host (async_gpu.h:1520-1525)
+            const uint64_t tail = std::min<uint64_t>(rb.refineTail, g.refineHead + g.refineCap);
+            while (g.refineHead < tail) {
+                const uint64_t at = g.refineHead % g.refineCap, n = std::min<uint64_t>(tail - g.refineHead, g.refineCap - at);
+                X2_HIP(hipMemcpy(g.refineReadback, g.refineRing + at, size_t(n) * sizeof(x2_refine), hipMemcpyDeviceToHost));
+                g.refinesOut.insert(g.refinesOut.end(), g.refineReadback, g.refineReadback + n);
+                g.refineHead += n;
+            }
+            g.refineHead = std::max<uint64_t>(g.refineHead, rb.refineTail);  // records past the ring's room were counted lost
+        }
         for (uint32_t t = 0; t < g.tiles; ++t) g.lastUse[t] = std::max(g.lastUse[t], g.stampReadback[f.slot][t]);
         const uint32_t n = std::min(rb.missCount, g.missCap);
         for (uint32_t i = 0; i < n; ++i) {
@@ -1220,6 +1292,8 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
     out->chunks = f.count;
     out->items = f.count * uint64_t(d->itemsPerChunk);
     out->samples = f.samples;
+    out->parked_run = f.parkedCount;
+    out->refines_run = f.refines;
     out->errors = s.errors + lost;  // a sample the parked ring could not hold is lost: a failure
     out->pad = 0;
     out->device_seconds = ms / 1000.0;
@@ -1230,6 +1304,17 @@ int x2dev_poll(x2dev* d, x2_completion* out) {
     out->voxel_taps = s.voxels;
     return 1;
 }
+
+int x2dev_take_refines(x2dev* d, x2_refine* out, uint32_t capacity, uint32_t* taken) {
+    if (!d || !taken || (capacity && !out)) return fail(X2_ERR_ARGUMENT, "null device, output or count");
+    auto& q = d->pg.refinesOut;
+    const uint32_t n = uint32_t(std::min<size_t>(capacity, q.size()));
+    std::copy(q.begin(), q.begin() + n, out);
+    q.erase(q.begin(), q.begin() + n);
+    *taken = n;
+    return X2_OK;
+}
+uint64_t x2dev_refines_waiting(const x2dev* d) { return d ? d->pg.refinesOut.size() : 0u; }
 
 int x2dev_take_deferred(x2dev* d, x2_parked* out, uint32_t capacity, uint32_t* taken) {
     (void)out;
@@ -1450,7 +1535,7 @@ int runFrame(x2dev* d, const x2_config* c, uint32_t spc, x2_frame_stats& st) {
 // Launches the chunks in order, at most two launches in flight, sized by the measured rate, and collects every
 // completion (chunkSamples: each chunk's samples, else every in-image pixel of its tile). A completion that counted
 // an error ends the feed: nothing more is launched, the launches in flight are collected, and the caller fails the
-// frame (async_render.h:679, 736).
+// frame (async_render.h:679, 736). With host refines the provisional samples the launches make go back in the next.
 int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<uint32_t>* chunkSamples,
               x2_frame_stats& st) {
     const x2_config* c = &d->cfg;
@@ -1462,11 +1547,32 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
     double rate = 0;  // items per device second, from completed launches
     size_t next = 0;
     int r = 0;
+    // Host refines: the provisional samples the collected launches made, sent back in the next launches (the tiles
+    // they found missing are sent up first, async_render.h:718-726).
+    std::vector<x2_refine> refines;
+    auto takeRefines = [&]() -> int {
+        const uint64_t n = x2dev_refines_waiting(d);
+        if (!n) return X2_OK;
+        const size_t at = refines.size();
+        refines.resize(at + n);
+        uint32_t got = 0;
+        const int e = x2dev_take_refines(d, refines.data() + at, uint32_t(n), &got);
+        refines.resize(at + got);
+        return e;
+    };
+    // Launch with up to a launch's refines and `want` chunks from `next`; the refines sent are dropped from the list.
+    auto launchWith = [&](uint64_t want, uint64_t samples) -> int {
+        const uint32_t n = uint32_t(std::min<size_t>(refines.size(), X2_REFINES_PER_LAUNCH));
+        const int e = x2dev_launch_refines(d, want ? chunks.data() + next : nullptr, uint32_t(want), refines.data(), n, samples);
+        if (e >= 0 && n) refines.erase(refines.begin(), refines.begin() + n);
+        return e;
+    };
     // Paged textures: parked samples run again (in the next launches, or in launches of them alone at the end) until
     // none is left; a stretch of launches of parked samples alone that ends none means the pool cannot hold what they
     // need (async_render.h:816-818).
     uint64_t fewest = ~0ull, idle = 0, rounds = 0;
-    while (!d->inFlight.empty() || (!st.errors && (next < chunks.size() || x2dev_parked_outstanding(d)))) {
+    auto outstanding = [&]() { return x2dev_parked_outstanding(d) + refines.size() + x2dev_refines_waiting(d); };
+    while (!d->inFlight.empty() || (!st.errors && (next < chunks.size() || outstanding()))) {
         x2_completion done;
         while ((r = x2dev_poll(d, &done)) == 1) {
             ++st.launches;
@@ -1485,6 +1591,7 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
             }
         }
         if (r < 0) return r;
+        if ((r = takeRefines()) < 0) return r;
         if (st.errors) {  // the launches in flight finish; none follows them
             if (!d->inFlight.empty()) {
                 const hipError_t e = hipEventSynchronize(d->done[d->inFlight.front().slot]);
@@ -1492,8 +1599,8 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
             }
             continue;
         }
-        if (next >= chunks.size() && x2dev_ready(d) && x2dev_parked_outstanding(d)) {
-            const uint64_t left = x2dev_parked_outstanding(d) + d->pg.inflightItems;
+        if (next >= chunks.size() && x2dev_ready(d) && outstanding()) {
+            const uint64_t left = outstanding() + d->pg.inflightItems;
             if (left < fewest) {
                 fewest = left;
                 idle = 0;
@@ -1504,13 +1611,16 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
             // the reference's does, async_render.h:816-818) rather than run on.
             if (idle > 256 || ++rounds > 4096)
                 return fail(X2_ERR_STATE, "the tile pool cannot hold the parked samples' working set");
-            if ((r = x2dev_launch(d, nullptr, 0, 0)) < 0) return r;
+            if ((r = launchWith(0, 0)) < 0) return r;
         }
         while (x2dev_ready(d) && next < chunks.size()) {
             const uint64_t remaining = chunks.size() - next;
             uint64_t want = rate ? std::max<uint64_t>(uint64_t(rate * windowMs / 1000.0 / d->itemsPerChunk), least)
                                  : std::max<uint64_t>(firstChunks, least);
-            want = std::min<uint64_t>({want, uint64_t(x2dev_chunk_room(d)), remaining});
+            const uint64_t refineChunks =
+                (std::min<size_t>(refines.size(), X2_REFINES_PER_LAUNCH) + d->itemsPerChunk - 1) / d->itemsPerChunk;
+            const uint64_t room = x2dev_chunk_room(d);
+            want = std::min<uint64_t>({want, room > refineChunks ? room - refineChunks : 0, remaining});
             if (!want) break;  // the parked ring is full: the launches in flight make room
             uint64_t samples = 0;
             const uint32_t tilesX = (c->width + 7) / 8;
@@ -1525,7 +1635,7 @@ int runChunks(x2dev* d, const std::vector<x2_chunk>& chunks, const std::vector<u
                     if (x < c->width && y < c->height) samples += ch.count;
                 }
             }
-            r = x2dev_launch(d, chunks.data() + next, uint32_t(want), samples);
+            r = launchWith(want, samples);
             if (r < 0) return r;
             next += want;
         }

@@ -102,10 +102,12 @@ constexpr uint32_t NoTileSlot = 0xffffffffu;
 // The paging state the kernels write and the host reads back after each launch: samples parked for a missing tile
 // (a ring the next launches run again), the tiles asked for, provisional samples (partial shading).
 struct PagingState {
-    unsigned long long parkTail;  // parked samples ever written (ring position of the next)
-    uint32_t missCount;           // tiles this launch asked for (entries of the miss list)
-    uint32_t parkOverflow;        // samples that could not be parked (the ring was full): failures
-    uint32_t provisional, pad[3];
+    unsigned long long parkTail;    // parked samples ever written (ring position of the next)
+    unsigned long long refineTail;  // provisional records ever written for the host (host refines)
+    uint32_t missCount;             // tiles this launch asked for (entries of the miss list)
+    uint32_t parkOverflow;          // samples that could not be parked (the ring was full): failures
+    uint32_t provisional;
+    uint32_t refineOverflow;        // provisional records the ring could not hold: failures
 };
 constexpr uint32_t MaxTextures = 16;
 constexpr uint32_t RoughnessSlots = 8;  // materials whose roughness gradient a lane accumulates in registers
@@ -181,6 +183,13 @@ struct RenderParams {
     unsigned long long parkedBegin;
     uint32_t parkedCount, parkCap, missCap, partial;
     uint32_t* tileStamp;  // each tile's last launch to read it (the pool evicts the least recently used)
+    // Host refines: the launch renders refineInCount provisional samples again ahead of its parked samples and
+    // chunks; the provisional samples it makes go to the refine ring (refineCap records; those below refineHead are
+    // taken) instead of parking.
+    x2_refine* refineRing;
+    const x2_refine* refineIn;
+    unsigned long long refineHead;
+    uint32_t refineCap, refineInCount, hostRefines, refinePad;
     uint32_t stamp;
     // Failure injection: nonzero k makes every sample of pixel k - 1 fail as it starts, counted in the launch's
     // errors like any sample the estimator cannot run (async_gpu.cu:363), so the frame fails as the reference's does.

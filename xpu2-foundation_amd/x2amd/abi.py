@@ -47,6 +47,10 @@ class Parked(ctypes.Structure):
     _fields_ = [("pixel", u32), ("index", u32), ("tries", u32), ("pass_", u32)]
 
 
+class Refine(ctypes.Structure):  # x2_refine (Refine, async_integrator.h:3069-3073)
+    _fields_ = [("pixel", u32), ("index", u32), ("pass_", u32), ("tile", u32), ("rgba", f32 * 4)]
+
+
 class Light(ctypes.Structure):
     _fields_ = [
         ("type", u32), ("sidedness", u32), ("hemispherical", u32), ("visibility_mask", u32),
@@ -109,7 +113,8 @@ class Config(ctypes.Structure):
 
 class Completion(ctypes.Structure):
     _fields_ = [
-        ("chunks", u64), ("items", u64), ("samples", u64), ("errors", u32), ("pad", u32),
+        ("chunks", u64), ("items", u64), ("samples", u64), ("parked_run", u64), ("refines_run", u64),
+        ("errors", u32), ("pad", u32),
         ("device_seconds", f64), ("rays_trace", u64), ("rays_shadow", u64), ("nodes_visited", u64),
         ("triangles_tested", u64), ("voxel_taps", u64),
     ]
@@ -196,7 +201,7 @@ class StagedFrame(ctypes.Structure):
 
 
 class Paging(ctypes.Structure):
-    _fields_ = [("pool_tiles", u32), ("coarse_side", u32), ("partial", u32), ("pad", u32)]
+    _fields_ = [("pool_tiles", u32), ("coarse_side", u32), ("partial", u32), ("host_refines", u32)]
 
 
 class PagingStats(ctypes.Structure):
@@ -208,10 +213,10 @@ class PagingStats(ctypes.Structure):
 # Sizes of the C structures (include/x2dev.h); checked against ctypes here and against the C compiler by a
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
-    "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_light": (Light, 184),
+    "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_refine": (Refine, 32), "x2_light": (Light, 184),
     "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
     "x2_config": (Config, 292),
-    "x2_completion": (Completion, 80), "x2dev_options": (Options, 24), "x2_frame_stats": (FrameStats, 88),
+    "x2_completion": (Completion, 96), "x2dev_options": (Options, 24), "x2_frame_stats": (FrameStats, 88),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),
     "x2_grid_options": (GridOptions, 24), "x2_cpu_lane": (CpuLane, 24), "x2_broker_options": (BrokerOptions, 16),
     "x2_grid_frame": (GridFrame, 120), "x2_film_file": (FilmFile, 40), "x2_exr_image": (ExrImage, 56),
@@ -228,6 +233,9 @@ SIGNATURES = {
     "x2dev_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), u32]),
     "x2dev_ready": (ctypes.c_int, [ctypes.c_void_p]),
     "x2dev_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Chunk), u32, u64]),
+    "x2dev_launch_refines": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Chunk), u32, ctypes.POINTER(Refine), u32, u64]),
+    "x2dev_take_refines": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Refine), u32, ctypes.POINTER(u32)]),
+    "x2dev_refines_waiting": (u64, [ctypes.c_void_p]),
     "x2dev_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Completion)]),
     "x2dev_take_deferred": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Parked), u32, ctypes.POINTER(u32)]),
     "x2dev_download_film": (ctypes.c_int, [ctypes.c_void_p]),

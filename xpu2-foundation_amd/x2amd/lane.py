@@ -106,11 +106,24 @@ class GpuLane:
     def max_chunks_per_launch(self):
         return int(self.lib.x2dev_max_chunks_per_launch(self.h))
 
-    def launch(self, chunks, samples=0):
-        """chunks: a ctypes array of abi.Chunk (or a slice copy). Returns True when queued."""
+    def launch(self, chunks, samples=0, refines=None):
+        """chunks: a ctypes array of abi.Chunk (or a slice copy); refines: a ctypes array of abi.Refine to render
+        again first (host refines). Returns True when queued."""
         n = len(chunks)
         ptr = ctypes.cast(chunks, ctypes.POINTER(abi.Chunk)) if n else None
+        if refines is not None and len(refines):
+            rptr = ctypes.cast(refines, ctypes.POINTER(abi.Refine))
+            return abi.check(self.lib.x2dev_launch_refines(self.h, ptr, n, rptr, len(refines), samples),
+                             "x2dev_launch_refines") == 1
         return abi.check(self.lib.x2dev_launch(self.h, ptr, n, samples), "x2dev_launch") == 1
+
+    def take_refines(self):
+        """The provisional samples the collected launches made (GpuLane::takeRefines), a ctypes array of abi.Refine."""
+        n = int(self.lib.x2dev_refines_waiting(self.h))
+        buf = (abi.Refine * max(1, n))()
+        got = ctypes.c_uint32()
+        abi.check(self.lib.x2dev_take_refines(self.h, buf, n, ctypes.byref(got)), "x2dev_take_refines")
+        return (abi.Refine * got.value)(*buf[:got.value])
 
     def poll(self):
         c = abi.Completion()
@@ -207,12 +220,14 @@ class GpuLane:
         return sums, counts, out
 
 
-    def page_textures(self, pool_tiles, coarse_side=0, partial=False):
+    def page_textures(self, pool_tiles, coarse_side=0, partial=False, host_refines=False):
         """On-demand texture tiles (x2dev_page_textures): every mip level wider or taller than coarse_side (0: 64) in
         32x32 half-float tiles, pool_tiles of them on the device, the coarser levels resident; with partial, a sample
-        meeting a missing tile reads the texel's parent on the nearest resident level up, provisionally."""
+        meeting a missing tile reads the texel's parent on the nearest resident level up, provisionally; with
+        host_refines the provisional samples go to the host (take_refines) and come back through launch(refines=)."""
         abi.check(self.lib.x2dev_page_textures(self.h, ctypes.byref(abi.Paging(pool_tiles, coarse_side,
-                                                                                 1 if partial else 0, 0))),
+                                                                                 1 if partial else 0,
+                                                                                 1 if host_refines else 0))),
                   "x2dev_page_textures")
 
     def paging_stats(self):
