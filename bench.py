@@ -481,6 +481,11 @@ def traffic_per_path():
         return json.load(f)
 
 
+def progress(msg):
+    """One line a finished leg, on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     global LAUNCH_CHUNKS
     ap = argparse.ArgumentParser()
@@ -540,6 +545,7 @@ def main():
         totals["tris"] += st.triangles_tested
     wall = time.perf_counter() - t0
     lane.close()
+    progress(f"headline: {args.steps} frames in {wall:.2f} s")
 
     paths = config.width * config.height * config.spp * args.steps
     rays = totals["traces"] + totals["shadows"]
@@ -593,25 +599,35 @@ def main():
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adjoint_reps)
+        progress("adjoint leg done")
         out["adjoint_step_over_forward"] = out["adjoint"]["step_over_forward"]  # (forward + adjoint) / forward
     if not args.no_volume:
         out["volume"] = volume_leg(args.volume_reps, not args.no_cpu)
+        progress("volume leg done")
     if not args.no_batch:
         out["batch"] = batch_leg(args.batch_reps)
+        progress("batch leg done")
     if not args.no_direct:
         out["direct"] = direct_leg(2)
+        progress("direct leg done")
     if not args.no_adaptive:
         out["adaptive"] = adaptive_leg(2)
+        progress("adaptive leg done")
     if not args.no_grid:
         out["grid"] = grid_leg(2)
+        progress("grid leg done")
     if not args.no_paging:
         out["paging"] = paging_leg(2)
+        progress("paging leg done")
     if not args.no_checkpoint:
         out["checkpoint"] = checkpoint_leg()
+        progress("checkpoint leg done")
     out["roofline"]["copy_ceiling_gbs"] = copy_ceiling()
+    progress("copy ceiling done")
     if not args.no_cpu:
         cpu_spp = args.cpu_spp or (16 if len(os.sched_getaffinity(0)) >= 12 else 4)
         out["cpu_baseline"] = cpu_baseline(scene, config.width, config.height, cpu_spp)
+        progress("cpu baseline done")
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         out["cpu_other_configs"] = cpu_other_configs()
     print(json.dumps(out))
