@@ -73,6 +73,11 @@ def load():
         "oracle_phase_eval": (f32, [f32, P(f32), P(f32)]),
         "oracle_phase_sample": (None, [f32, P(f32), f32, f32, P(f32)]),
         "oracle_volume_march": (c_int, [vp, vp, P(f32), P(f32), f32, f32, u32, u64, P(f32)]),
+        "oracle_region_info": (c_int, [vp, P(c_int), P(f32), P(f32)]),
+        "oracle_region_cell_pdf": (f32, [vp, c_int, c_int, c_int]),
+        "oracle_region_point": (None, [vp, P(f32), P(f32)]),
+        "oracle_region_pdf": (f32, [vp, P(f32), P(f32), P(f32)]),
+        "oracle_emissive_integral": (None, [vp, vp, P(f32), P(f32), f32, f32, u32, u64, P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -453,6 +453,14 @@ struct Volume {
     std::vector<float> ramp, majorant;
     std::vector<U32> table;  // per brick: its pool index, ~0u when empty
     std::vector<uint16_t> density, temperature;  // pools of 512 halves a brick, z, y, x within
+    // The glowing medium as a light source (see emissionRegion in the volume section).
+    bool hasRegion = false;
+    int regionRes[3] = {0, 0, 0};
+    U32 regionVisibility = 0;
+    float distToWorld[12] = {}, worldToDist[12] = {};
+    float invUnitVolume = 0;
+    U32 conditionalX = 0, marginalY = 0, marginalZ = 0;
+    std::vector<float> pool;
 };
 struct Scene {
     std::vector<V3> tri;  // 3 vertices per triangle, scene order
@@ -1803,6 +1811,218 @@ V3 volumeSourceTerm(const Scene& S, const x2_config& c, Stream& s, const x2_ligh
     const float ph = phaseEval(g, -rayDir, wi);
     return Li * (tr * (ph / pdfL));
 }
+// The medium as a light source: MoonRay's emissive region over a dense emission distribution (volume_emission.h,
+// cpu_scene.h:1520-1545 for how the reference lays the distribution out). The histogram is the build's own over its
+// brick grid: cells of 4^3 voxels over the medium's box, each cell's value emission scale x the largest density x
+// the brightest ramp entry that a temperature within one voxel of the cell can interpolate (an upper bound of the
+// emission luminance anywhere in the cell). Each (z, y) row gets its x distribution, each z the y distribution of
+// its rows' integrals, and the z distribution those planes' integrals; a distribution's pool record is its integral,
+// n, the n values and the n + 1 cdf entries (accumulated value / n, normalized; uniform when the values sum to 0).
+constexpr int RegionCell = 4;
+U32 poolDistribution(std::vector<float>& pool, const float* func, U32 n) {
+    const U32 at = U32(pool.size());
+    std::vector<float> cdf(size_t(n) + 1, 0.0f);
+    for (U32 i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / float(n);
+    const float integral = cdf[n];
+    for (U32 i = 1; i <= n; ++i) cdf[i] = integral == 0.0f ? float(i) / float(n) : cdf[i] / integral;
+    pool.push_back(integral);
+    pool.push_back(float(n));
+    for (U32 i = 0; i < n; ++i) pool.push_back(func[i]);
+    for (U32 i = 0; i <= n; ++i) pool.push_back(cdf[i]);
+    return at;
+}
+void emissionRegion(const x2_volume& in, Volume& v) {
+    int n[3];
+    for (int a = 0; a < 3; ++a) n[a] = int(in.res[a]) / RegionCell;
+    std::vector<float> value(size_t(n[0]) * n[1] * n[2], 0.0f);
+    auto rampIndex = [&](float kelvin) {
+        float x = kelvin / in.ramp_max_kelvin * float(in.ramp_size - 1);
+        x = fminf(fmaxf(x, 0.0f), float(in.ramp_size - 1));
+        return U32(floorf(x));
+    };
+    bool any = false;
+    const int R[3] = {int(in.res[0]), int(in.res[1]), int(in.res[2])};
+    for (int cz = 0; cz < n[2]; ++cz)
+        for (int cy = 0; cy < n[1]; ++cy)
+            for (int cx = 0; cx < n[0]; ++cx) {
+                float dMax = 0.0f, tMin = FLT_MAX, tMax = -FLT_MAX;
+                for (int z = std::max(cz * RegionCell - 1, 0); z <= std::min(cz * RegionCell + RegionCell, R[2] - 1); ++z)
+                    for (int y = std::max(cy * RegionCell - 1, 0); y <= std::min(cy * RegionCell + RegionCell, R[1] - 1); ++y)
+                        for (int x = std::max(cx * RegionCell - 1, 0); x <= std::min(cx * RegionCell + RegionCell, R[0] - 1); ++x) {
+                            const size_t i = (size_t(z) * R[1] + y) * R[0] + x;
+                            dMax = fmaxf(dMax, halfToFloat(in.density[i]));
+                            const float t = halfToFloat(in.temperature[i]);
+                            tMin = fminf(tMin, t);
+                            tMax = fmaxf(tMax, t);
+                        }
+                if (!(dMax > 0.0f)) continue;
+                const U32 i0 = rampIndex(tMin), i1 = std::min(rampIndex(tMax) + 1, in.ramp_size - 1);
+                float lum = 0.0f;
+                for (U32 i = i0; i <= i1; ++i)
+                    lum = fmaxf(lum, 0.299f * in.ramp[3 * i] + 0.587f * in.ramp[3 * i + 1] + 0.114f * in.ramp[3 * i + 2]);
+                const float e = in.emission_scale * dMax * lum;
+                value[(size_t(cz) * n[1] + cy) * n[0] + cx] = e;
+                if (e > 0.0f) any = true;
+            }
+    if (!any) return;
+    v.hasRegion = true;
+    v.regionVisibility = 0x3ffu;
+    for (int a = 0; a < 3; ++a) {
+        v.regionRes[a] = n[a];
+        const double cell = (double(in.bounds_max[a]) - double(in.bounds_min[a])) / double(n[a]);
+        v.distToWorld[4 * a] = float(cell);
+        v.distToWorld[9 + a] = in.bounds_min[a];
+        v.worldToDist[4 * a] = float(1.0 / cell);
+        v.worldToDist[9 + a] = float(-double(in.bounds_min[a]) / cell);
+    }
+    double volume = 1.0;
+    for (int a = 0; a < 3; ++a) volume *= (double(in.bounds_max[a]) - double(in.bounds_min[a])) / double(n[a]);
+    v.invUnitVolume = float(1.0 / volume);
+    v.pool.clear();
+    std::vector<U32> rows;
+    v.conditionalX = 0;
+    for (int z = 0; z < n[2]; ++z)
+        for (int y = 0; y < n[1]; ++y)
+            rows.push_back(poolDistribution(v.pool, &value[(size_t(z) * n[1] + y) * n[0]], U32(n[0])));
+    v.marginalY = U32(v.pool.size());
+    std::vector<float> rowIntegral(n[1]), planeIntegral(n[2]);
+    for (int z = 0; z < n[2]; ++z) {
+        for (int y = 0; y < n[1]; ++y) rowIntegral[y] = v.pool[rows[size_t(z) * n[1] + y]];
+        planeIntegral[z] = v.pool[poolDistribution(v.pool, rowIntegral.data(), U32(n[1]))];
+    }
+    v.marginalZ = poolDistribution(v.pool, planeIntegral.data(), U32(n[2]));
+}
+// One distribution record: Distribution1D's pdf of entry i and its sampling (findInterval over the cdf, the entry's
+// pdf, where u falls within it).
+float recordPdf(const std::vector<float>& pool, U32 at, int i) {
+    const float integral = pool[at], n = pool[at + 1];
+    return integral > 0.0f ? pool[at + 2 + U32(i)] / (integral * n) : 0.0f;
+}
+int recordSample(const std::vector<float>& pool, U32 at, float u, float& pdf, float& remapped) {
+    const U32 n = U32(pool[at + 1]);
+    const float* cdf = &pool[at + 2 + n];
+    int i = 0;  // the last entry at or below u, within [0, n - 1]
+    for (U32 j = 1; j <= n; ++j)
+        if (cdf[j] <= u) i = int(j);
+    if (i > int(n) - 1) i = int(n) - 1;
+    pdf = recordPdf(pool, at, i);
+    remapped = (u - cdf[i]) / (cdf[i + 1] - cdf[i]);
+    return i;
+}
+U32 recordFloats(int n) { return 2u * U32(n) + 3u; }
+V3 affinePoint(const float* m, V3 p) {
+    return {p.x * m[0] + p.y * m[3] + p.z * m[6] + m[9], p.x * m[1] + p.y * m[4] + p.z * m[7] + m[10],
+            p.x * m[2] + p.y * m[5] + p.z * m[8] + m[11]};
+}
+V3 affineVector(const float* m, V3 w) {
+    return {w.x * m[0] + w.y * m[3] + w.z * m[6], w.x * m[1] + w.y * m[4] + w.z * m[7], w.x * m[2] + w.y * m[5] + w.z * m[8]};
+}
+float regionCellPdf(const Volume& v, int x, int y, int z) {
+    if (x < 0 || x >= v.regionRes[0] || y < 0 || y >= v.regionRes[1] || z < 0 || z >= v.regionRes[2]) return 0.0f;
+    const float pz = recordPdf(v.pool, v.marginalZ, z);
+    const float py = recordPdf(v.pool, v.marginalY + U32(z) * recordFloats(v.regionRes[1]), y);
+    const float px = recordPdf(v.pool, v.conditionalX + U32(z * v.regionRes[1] + y) * recordFloats(v.regionRes[0]), x);
+    return px * py * pz;
+}
+V3 regionPoint(const Volume& v, float u1, float u2, float u3) {
+    float pz, py, px, rz, ry, rx;
+    const int z = recordSample(v.pool, v.marginalZ, u1, pz, rz);
+    const int y = recordSample(v.pool, v.marginalY + U32(z) * recordFloats(v.regionRes[1]), u2, py, ry);
+    const int x = recordSample(v.pool, v.conditionalX + U32(z * v.regionRes[1] + y) * recordFloats(v.regionRes[0]), u3, px, rx);
+    return affinePoint(v.distToWorld, V3{float(x) + rx, float(y) + ry, float(z) + rz});
+}
+// The direction pdf from p (EmissionDistribution::pdf): the 3D pdf along the ray, t^2 dt, cell by cell through a DDA
+// over the region's cells; tEnd where the ray leaves the box.
+float regionPdf(const Volume& v, V3 pWorld, V3 wiWorld, float& tEnd) {
+    const V3 p = affinePoint(v.worldToDist, pWorld);
+    V3 w = affineVector(v.worldToDist, wiWorld);
+    const float lengthScale = 1.0f / sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+    w = w * lengthScale;
+    float tStart = 0.0f;
+    tEnd = 3.402823466e+38f;
+    for (int a = 0; a < 3; ++a) {
+        const float inv = 1.0f / comp(w, a);
+        float t0 = (0.0f - comp(p, a)) * inv, t1 = (float(v.regionRes[a]) - comp(p, a)) * inv;
+        if (t0 > t1) std::swap(t0, t1);
+        tStart = t0 > tStart ? t0 : tStart;
+        tEnd = t1 < tEnd ? t1 : tEnd;
+        if (tStart > tEnd) return 0.0f;
+    }
+    tEnd *= lengthScale;
+    int cell[3], step[3], stop[3];
+    float next[3], delta[3];
+    for (int a = 0; a < 3; ++a) {
+        const float q = comp(p, a) + tStart * comp(w, a);
+        cell[a] = std::min(std::max(int(q), 0), v.regionRes[a] - 1);
+        if (comp(w, a) >= 0) {
+            next[a] = tStart + (float(cell[a]) + 1.0f - q) / comp(w, a);
+            delta[a] = 1.0f / comp(w, a);
+            step[a] = 1;
+            stop[a] = v.regionRes[a];
+        } else {
+            next[a] = tStart + (float(cell[a]) - q) / comp(w, a);
+            delta[a] = -1.0f / comp(w, a);
+            step[a] = -1;
+            stop[a] = -1;
+        }
+    }
+    float t = tStart, sum = 0.0f;
+    for (int i = 0; i < (v.regionRes[0] + v.regionRes[1] + v.regionRes[2]) * 2 + 8; ++i) {
+        const float pd = regionCellPdf(v, cell[0], cell[1], cell[2]);
+        int a = 0;  // the nearest cell wall, the lower axis on a tie
+        if (next[1] < next[a]) a = 1;
+        if (next[2] < next[a]) a = 2;
+        if (pd > 0.0f) {
+            const float t0 = t * lengthScale, t1 = next[a] * lengthScale;
+            sum += pd * v.invUnitVolume * (t1 * t1 * t1 - t0 * t0 * t0) / 3.0f;
+        }
+        t = next[a];
+        cell[a] += step[a];
+        if (cell[a] == stop[a]) break;
+        next[a] += delta[a];
+    }
+    return sum;
+}
+V3 regionSample(const Volume& v, V3 p, float u1, float u2, float u3, float& pdf, float& tEnd) {
+    V3 wi = regionPoint(v, u1, u2, u3) - p;
+    wi = wi * (1.0f / sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z));
+    pdf = regionPdf(v, p, wi, tEnd);
+    return wi;
+}
+// computeEmissiveVolumeIntegral (async_integrator.h:478-566) for the box: the ray ends at tEnd or its first surface;
+// over its stretch of the box (one draw from s, when there is one) a march of the given depth (marchVolume's
+// lattice) adds emission x transmittance x the step's length, the transmittance falling by Beer's law a step at a
+// time, until it is below the threshold.
+V3 emissiveIntegral(const Scene& S, const x2_config& c, Stream& s, V3 o, V3 d, float tnear, float tEnd, U32 depth,
+                    Counters& k) {
+    const Volume& v = S.volume;
+    {
+        ++k.traces;
+        float t;
+        if (traverse(S, o, d, tnear, tEnd, false, false, t) != ~0u && t < tEnd) tEnd = t;
+    }
+    float a, b;
+    if (!clipBox(v, o, d, tnear, tEnd, a, b) || !(b - a > VolEpsilon)) return {0, 0, 0};
+    const float u = s.next();
+    const float stepSize = v.minFeature * c.inv_volume_quality * fmaxf(float(int(depth) - 1), 1.0f);
+    V3 L{0, 0, 0};
+    float tr = 1.0f;
+    for (U32 i = 0; i < MarchSteps; ++i) {
+        const float e = a + stepSize * (u + float(i));
+        const bool last = !(e < b);
+        if (last && !(e - b > VolEpsilon)) break;
+        if (tr < c.volume_transmittance_threshold) break;
+        const float at = last ? b : e, start = i == 0 ? a : a + stepSize * (u + float(i - 1));
+        const VCoeff q = coefficientsAt(v, o + d * at, true);
+        if (q.sigmaT != 0.0f) {
+            const float delta = at - start;
+            L = L + (q.emission * tr) * delta;
+            tr *= expf(-q.sigmaT * delta);
+        }
+        if (last) break;
+    }
+    return L;
+}
 // The dense grids as bricks: a brick is kept when any of its density voxels is nonzero (temperature is kept for the
 // same bricks and reads zero elsewhere); a brick's majorant is the largest density among its own voxels and its 26
 // neighbours' (a point in the brick interpolates voxels at most one away from it).
@@ -1877,6 +2097,7 @@ bool packVolume(const x2_volume& in, Volume& v) {
                         }
                 v.majorant[(size_t(bz) * ny + by) * nx + bx] = m;
             }
+    if (in.temperature && in.emission_scale > 0.0f) emissionRegion(in, v);
     v.present = true;
     return true;
 }
@@ -1969,6 +2190,43 @@ float volumeRadiance(const Scene& S, const x2_config& c, const Identity& root, V
                     }
                 radiance = radiance + direct;
             }
+            // volumeScatterRegions (:999-1057): the glowing medium lighting the steps, toward a place drawn in its
+            // region, at an equi-angular distance.
+            if (v.hasRegion && (v.regionVisibility & 0x100u)) {
+                const float invN = 1.0f / float(samplesPerLight * scatterCount);
+                V3 regionLight{0, 0, 0};
+                for (U32 i = 0; i < samplesPerLight * scatterCount; ++i) {
+                    Identity eid = root;
+                    eid.branch = childBranch(volumeBranch, 0xfe00u, i);
+                    Stream es(eid, pv.depth, 0);
+                    const float u1 = es.next(), u2 = es.next(), u3 = es.next();
+                    const V3 pivot = regionPoint(v, u1, u2, u3);
+                    const float ue = es.next();
+                    const float offset = dot(pivot - o, d);
+                    const V3 foot = o + d * offset;
+                    const float D = length(pivot - foot);
+                    const float thetaA = atan2f(firstStart - offset, D), thetaB = atan2f(rayEnd - offset, D);
+                    const float te = sampleEquiAngular(ue, D, thetaA, thetaB);
+                    const float pdfTe = pdfEquiAngular(te, D, thetaA, thetaB);
+                    if (pdfTe == 0.0f || !finitef(pdfTe)) continue;
+                    const float tePrime = te + offset;
+                    float pdfK;
+                    bool holds;
+                    const VStep& vp = m.steps[stepAt(m, tePrime, pdfK, holds)];
+                    if (!holds || isExactlyZero(vp.sigmaS)) continue;
+                    const float trScatter = vp.tr * expf(-vp.sigmaT * (tePrime - vp.tStart));
+                    const V3 scatterPoint = o + d * tePrime;
+                    V3 wi = pivot - scatterPoint;
+                    wi = wi * (1.0f / length(wi));
+                    float tEnd;
+                    const float pdfWi = regionPdf(v, scatterPoint, wi, tEnd);
+                    if (pdfWi == 0.0f || !finitef(pdfWi)) continue;
+                    const V3 Le = emissiveIntegral(S, c, es, scatterPoint, wi, 0.0f, tEnd, pv.depth + 1, k);
+                    const float ph = phaseEval(v.g, -d, wi);
+                    regionLight = regionLight + (T * vp.sigmaS) * (Le * trScatter) * (invN * ph / (pdfTe * pdfWi));
+                }
+                radiance = radiance + regionLight;
+            }
         }
         ++pv.volumeDepth;
         // approximateVolumeMultipleScattering (:1060-1149)
@@ -2009,6 +2267,17 @@ float volumeRadiance(const Scene& S, const x2_config& c, const Identity& root, V
                         const V3 Ls = volumeSourceTerm(S, c, ms, S.lights[li], scatterPoint, rayDir, g, usl, ai, k);
                         radiance = radiance + throughput * Ls * (invN * bi);
                     }
+                    // the glowing medium lighting this scatter (:1127-1143)
+                    if (v.hasRegion && (v.regionVisibility & 0x100u)) {
+                        const float ue0 = ms.next(), ue1 = ms.next(), ue2 = ms.next();
+                        float pdfWi, tEnd;
+                        const V3 wi = regionSample(v, scatterPoint, ue0, ue1, ue2, pdfWi, tEnd);
+                        if (pdfWi != 0.0f && finitef(pdfWi)) {
+                            const V3 Le = emissiveIntegral(S, c, ms, scatterPoint, wi, 0.0f, tEnd, pv.depth + 1, k);
+                            const float ph = phaseEval(g, -rayDir, wi);
+                            radiance = radiance + (throughput * Le) * (invN * bi * ph / pdfWi);
+                        }
+                    }
                     const float uq0 = ms.next(), uq1 = ms.next();
                     scatterDir = phaseSample(g, -rayDir, uq0, uq1);
                 }
@@ -2017,6 +2286,108 @@ float volumeRadiance(const Scene& S, const x2_config& c, const Identity& root, V
     }
     pv.T = pv.T * vt;
     return vt;
+}
+
+// computeRadianceEmissiveRegionsBundled (async_integrator.h:2481-2613): the glowing medium lighting a surface vertex.
+// The lobes it lights make one sampler (a lobe picked in proportion to its albedo's luminance, baseAlbedo
+// shade_lobes_eval.h:1022-1028); nL light samples (a place drawn in the region) and nB BSDF samples, each weighed
+// against the other strategy by the power heuristic, each carrying the medium's light along its direction from the
+// offset origin on that direction's side.
+void surfaceRegionLight(const Scene& S, const x2_config& c, const Identity& root, U64 branch, U32 depth, bool split,
+                        const Bsdf& bsdf, const Slice& slice, V3 Ng, V3 P, V3 above, V3 below, V3 T, V3& radiance,
+                        Counters& k) {
+    const Volume& v = S.volume;
+    const U32 nL = split ? c.light_samples : 1u, nB = split ? c.bsdf_samples : 1u;
+    U32 recs[2], nRec = 0;
+    float cdf[2], norm = 0.0f;
+    for (U32 q = 0; q < bsdf.count; ++q) {
+        const Lobe& l = bsdf.lobes[q];
+        if (!(lobeRayMask(l.type) & v.regionVisibility)) continue;
+        recs[nRec] = q;
+        cdf[nRec] = fmaxf(fabsf(luminance(l.kind == KindLambert ? l.scale * l.albedo : l.scale)), VolEpsilon);
+        norm += cdf[nRec++];
+    }
+    if (!nRec) return;
+    float acc = 0.0f;
+    for (U32 j = 0; j < nRec; ++j) {
+        cdf[j] = cdf[j] / norm + acc;
+        acc = cdf[j];
+    }
+    V3 fs[2];
+    U32 matched = 0;
+    auto add = [&](V3 contribution) {
+        for (U32 j = 0; j < matched; ++j) radiance = radiance + fs[j] * contribution;
+    };
+    auto emitted = [&](V3 wi, float tEnd, Stream& es) {
+        return emissiveIntegral(S, c, es, dot(wi, Ng) >= 0.0f ? above : below, wi, 0.0f, tEnd, depth + 1, k);
+    };
+    for (U32 ls = 0; ls < nL; ++ls) {
+        Identity eid = root;
+        eid.branch = childBranch(branch, 0xfd00u, ls);
+        Stream es(eid, depth, 0);
+        const float u1 = es.next(), u2 = es.next(), u3 = es.next();
+        float pdfLight, tEnd;
+        const V3 wi = regionSample(v, P, u1, u2, u3, pdfLight, tEnd);
+        if (pdfLight == 0.0f) continue;
+        const U32 flags = surfaceFlags(slice, wi);
+        V3 f{0, 0, 0};
+        float pdfBsdf = 0.0f, prev = 0.0f;
+        matched = 0;
+        for (U32 j = 0; j < nRec; ++j) {
+            float p;
+            const V3 col = lobeEval(bsdf.lobes[recs[j]], slice, wi, p);
+            if (matchesFlags(bsdf.lobes[recs[j]].type, flags)) {
+                f = f + col;
+                fs[matched++] = col;
+            }
+            pdfBsdf += p * (cdf[j] - prev);
+            prev = cdf[j];
+        }
+        if (isExactlyZero(f)) continue;
+        V3 contribution = emitted(wi, tEnd, es);
+        const float mis = nB ? powerHeuristic(float(nL) * pdfLight, float(nB) * pdfBsdf) : 1.0f;
+        contribution = (contribution * T) * (mis / (pdfLight * float(nL)));
+        add(contribution);
+    }
+    for (U32 bs = 0; bs < nB; ++bs) {
+        Identity eid = root;
+        eid.branch = childBranch(branch, 0xfc00u, bs);
+        Stream es(eid, depth, 0);
+        const float uLobe = es.next(), u1 = es.next(), u2 = es.next();
+        U32 pick = 0;
+        while (pick + 1 < nRec && cdf[pick] <= uLobe) ++pick;
+        const float lobePdf = pick == 0 ? cdf[0] : cdf[pick] - cdf[pick - 1];
+        V3 wi;
+        float pdfBsdf;
+        V3 f = lobeSample(bsdf.lobes[recs[pick]], slice, u1, u2, wi, pdfBsdf);
+        if (pdfBsdf == 0.0f) continue;
+        pdfBsdf *= lobePdf;
+        matched = 0;
+        fs[matched++] = f;
+        const U32 flags = surfaceFlags(slice, wi);
+        if (!matchesFlags(bsdf.lobes[recs[pick]].type, flags)) f = V3{0, 0, 0};
+        float prev = 0.0f;
+        for (U32 j = 0; j < nRec; ++j) {
+            if (j != pick) {
+                float p;
+                const V3 col = lobeEval(bsdf.lobes[recs[j]], slice, wi, p);
+                if (matchesFlags(bsdf.lobes[recs[j]].type, flags)) {
+                    f = f + col;
+                    fs[matched++] = col;
+                }
+                pdfBsdf += p * (cdf[j] - prev);
+            }
+            prev = cdf[j];
+        }
+        if (pdfBsdf == 0.0f || isExactlyZero(f)) continue;
+        float tEnd;
+        const float pdfLight = regionPdf(v, P, wi, tEnd);
+        if (pdfLight == 0.0f) continue;
+        V3 contribution = emitted(wi, tEnd, es);
+        const float mis = nL ? powerHeuristic(float(nB) * pdfBsdf, float(nL) * pdfLight) : 1.0f;
+        contribution = (contribution * T) * (mis / (pdfBsdf * float(nB)));
+        add(contribution);
+    }
 }
 
 // The prototype estimator (async_integrator.h:61-126; Config::paths == 0): camera hit, emission, cosine-sampled
@@ -2273,6 +2644,8 @@ int integrate(const Scene& S, const x2_config& c, U32 pixel, U32 index, double o
         }
         const float thinning = bounceThinning(top, planned);
         const V3 above = hit.origin, below = hit.below;
+        if (S.volume.hasRegion)
+            surfaceRegionLight(S, c, root, pv.branch, pv.depth, split, bsdf, slice, Ng, P, above, below, pv.T, radiance, k);
 
         // Light samples (:2615-2785).
         const U64 lightBranch = childBranch(pv.branch, 0xfffe, 0);
@@ -2992,6 +3365,29 @@ int oracle_volume_march(const oracle_scene* o, const x2_config* c, const float o
     out[6] = m.total;
     out[7] = m.reached ? 1.0f : 0.0f;
     return int(n);
+}
+// The emission region (test hooks): its cells and cell volume (0: the medium has no region), a cell's probability,
+// a drawn place, the direction pdf from a point, and the emissive integral along a ray.
+int oracle_region_info(const oracle_scene* o, int res[3], float* invUnitVolume, float distToWorld[12]) {
+    const Volume& v = o->s.volume;
+    if (!v.present || !v.hasRegion) return 0;
+    for (int a = 0; a < 3; ++a) res[a] = v.regionRes[a];
+    *invUnitVolume = v.invUnitVolume;
+    for (int i = 0; i < 12; ++i) distToWorld[i] = v.distToWorld[i];
+    return 1;
+}
+float oracle_region_cell_pdf(const oracle_scene* o, int x, int y, int z) { return regionCellPdf(o->s.volume, x, y, z); }
+void oracle_region_point(const oracle_scene* o, const float u[3], float out[3]) {
+    store3(out, regionPoint(o->s.volume, u[0], u[1], u[2]));
+}
+float oracle_region_pdf(const oracle_scene* o, const float p[3], const float wi[3], float* tEnd) {
+    return regionPdf(o->s.volume, load3(p), load3(wi), *tEnd);
+}
+void oracle_emissive_integral(const oracle_scene* o, const x2_config* c, const float org[3], const float dir[3],
+                              float tnear, float tEnd, uint32_t depth, uint64_t branch, float out[3]) {
+    Counters k;
+    Stream s(Identity{0, 0, c->seed, branch}, depth, 0);
+    store3(out, emissiveIntegral(o->s, *c, s, load3(org), load3(dir), tnear, tEnd, depth, k));
 }
 int oracle_render_chunks(const oracle_scene* o, const x2_config* c, const x2_chunk* chunks, uint32_t n,
                          uint32_t threads, double* sums, uint32_t* counts, uint64_t counters[3]) {

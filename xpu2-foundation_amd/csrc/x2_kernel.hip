@@ -1684,6 +1684,161 @@ __device__ VStep stepAt(const VolumeDesc& v, const x2_config& c, March& m, float
     holds = chosen.tStart <= t && t <= chosen.tStart + chosen.delta;
     return chosen;
 }
+// The glowing medium as a light source: its emission region (volume_emission.h:13-171, MoonRay's EmissiveRegion over
+// a DenseEmissionDistribution; Simon et al., EGSR 2017, for the direction pdf as a line integral of the 3D pdf).
+struct Dist1D {  // one record of the region's pool: its values, cdf, integral and size
+    const float* func;
+    const float* cdf;
+    float integral;
+    uint32_t n;
+};
+__device__ __forceinline__ Dist1D distAt(const float* pool, uint32_t at) {
+    const uint32_t n = uint32_t(pool[at + 1]);
+    return Dist1D{pool + at + 2, pool + at + 2 + n, pool[at], n};
+}
+__device__ __forceinline__ uint32_t distFloats(uint32_t n) { return 2u * n + 3u; }
+// geom::findInterval: the last cdf entry at or below u (entries are nondecreasing), kept within [0, size - 2].
+__device__ __forceinline__ int cdfInterval(const float* cdf, int size, float u) {
+    int lo = 0, count = size;  // entries [lo, lo + count) not yet placed; those before lo are <= u
+    while (count > 0) {
+        const int half = count >> 1;
+        if (cdf[lo + half] <= u) {
+            lo += half + 1;
+            count -= half + 1;
+        } else {
+            count = half;
+        }
+    }
+    return min(max(lo - 1, 0), size - 2);
+}
+__device__ __forceinline__ float distPdf(const Dist1D& d, int i) {
+    return d.integral > 0.0f ? d.func[i] / (d.integral * float(d.n)) : 0.0f;
+}
+__device__ __forceinline__ int distSample(const Dist1D& d, float u, float& pdf, float& remapped) {
+    const int i = cdfInterval(d.cdf, int(d.n) + 1, u);
+    pdf = distPdf(d, i);
+    remapped = (u - d.cdf[i]) / (d.cdf[i + 1] - d.cdf[i]);
+    return i;
+}
+__device__ __forceinline__ V3 affinePoint(const float* m, V3 p) {
+    return {p.x * m[0] + p.y * m[3] + p.z * m[6] + m[9], p.x * m[1] + p.y * m[4] + p.z * m[7] + m[10],
+            p.x * m[2] + p.y * m[5] + p.z * m[8] + m[11]};
+}
+__device__ __forceinline__ V3 affineVector(const float* m, V3 v) {
+    return {v.x * m[0] + v.y * m[3] + v.z * m[6], v.x * m[1] + v.y * m[4] + v.z * m[7], v.x * m[2] + v.y * m[5] + v.z * m[8]};
+}
+// The cell's discrete probability: its z plane's, its row's within the plane, its x within the row.
+__device__ float cellPdf(const EmissionRegion& r, int x, int y, int z) {
+    if (x < 0 || x >= r.res[0] || y < 0 || y >= r.res[1] || z < 0 || z >= r.res[2]) return 0.0f;
+    const float pz = distPdf(distAt(r.pool, r.marginalZ), z);
+    const float py = distPdf(distAt(r.pool, r.marginalY + uint32_t(z) * distFloats(uint32_t(r.res[1]))), y);
+    const float px =
+        distPdf(distAt(r.pool, r.conditionalX + uint32_t(z * r.res[1] + y) * distFloats(uint32_t(r.res[0]))), x);
+    return px * py * pz;
+}
+// A place in the region drawn by its distribution (z, then y given z, then x given both; uniform within the cell).
+__device__ V3 regionPoint(const EmissionRegion& r, float u1, float u2, float u3) {
+    float pz, py, px, rz, ry, rx;
+    const int z = distSample(distAt(r.pool, r.marginalZ), u1, pz, rz);
+    const int y = distSample(distAt(r.pool, r.marginalY + uint32_t(z) * distFloats(uint32_t(r.res[1]))), u2, py, ry);
+    const int x = distSample(
+        distAt(r.pool, r.conditionalX + uint32_t(z * r.res[1] + y) * distFloats(uint32_t(r.res[0]))), u3, px, rx);
+    return affinePoint(r.distToWorld, V3{float(x) + rx, float(y) + ry, float(z) + rz});
+}
+// The solid-angle pdf of direction wi from p: the 3D pdf integrated along the ray with the t^2 Jacobian, cell by
+// cell (a DDA through the cells, each cell contributing pdf * (t1^3 - t0^3) / 3); tEnd: where the ray leaves the
+// region's box (render-space distance).
+__device__ float regionPdf(const EmissionRegion& r, V3 pWorld, V3 wiWorld, float& tEnd) {
+    const V3 p = affinePoint(r.worldToDist, pWorld);
+    V3 w = affineVector(r.worldToDist, wiWorld);
+    const float lengthScale = 1.0f / sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+    w = w * lengthScale;
+    const float pa[3] = {p.x, p.y, p.z}, wa[3] = {w.x, w.y, w.z};
+    float tStart = 0.0f;
+    tEnd = 3.402823466e+38f;
+    for (int ax = 0; ax < 3; ++ax) {
+        const float inv = 1.0f / wa[ax];
+        float t0 = (0.0f - pa[ax]) * inv, t1 = (float(r.res[ax]) - pa[ax]) * inv;
+        if (t0 > t1) {
+            const float sw = t0;
+            t0 = t1;
+            t1 = sw;
+        }
+        tStart = t0 > tStart ? t0 : tStart;
+        tEnd = t1 < tEnd ? t1 : tEnd;
+        if (tStart > tEnd) return 0.0f;
+    }
+    tEnd *= lengthScale;
+    int cell[3], stepDir[3], stop[3];
+    float tNext[3], tDelta[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        const float q = pa[ax] + tStart * wa[ax];
+        const int c0 = int(q);
+        cell[ax] = c0 < 0 ? 0 : c0 > r.res[ax] - 1 ? r.res[ax] - 1 : c0;
+        const bool up = wa[ax] >= 0;
+        tNext[ax] = tStart + ((up ? float(cell[ax]) + 1.0f : float(cell[ax])) - q) / wa[ax];
+        tDelta[ax] = (up ? 1.0f : -1.0f) / wa[ax];
+        stepDir[ax] = up ? 1 : -1;
+        stop[ax] = up ? r.res[ax] : -1;
+    }
+    float t = tStart, sum = 0.0f;
+    const int guard = (r.res[0] + r.res[1] + r.res[2]) * 2 + 8;
+    for (int i = 0; i < guard; ++i) {
+        const float pd = cellPdf(r, cell[0], cell[1], cell[2]);
+        const int ax = (tNext[1] < tNext[0] || tNext[2] < tNext[0]) ? (tNext[2] < tNext[1] ? 2 : 1) : 0;
+        if (pd > 0.0f) {
+            const float t0 = t * lengthScale, t1 = tNext[ax] * lengthScale;
+            sum += pd * r.invUnitVolume * (t1 * t1 * t1 - t0 * t0 * t0) / 3.0f;
+        }
+        t = tNext[ax];
+        cell[ax] += stepDir[ax];
+        if (cell[ax] == stop[ax]) break;
+        tNext[ax] += tDelta[ax];
+    }
+    return sum;
+}
+__device__ V3 regionSample(const EmissionRegion& r, V3 p, float u1, float u2, float u3, float& pdf, float& tEnd) {
+    V3 wi = regionPoint(r, u1, u2, u3) - p;
+    wi = wi * (1.0f / sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z));
+    pdf = regionPdf(r, p, wi, tEnd);
+    return wi;
+}
+// computeEmissiveVolumeIntegral (async_integrator.h:478-566) in the medium's box: the medium's light along the ray
+// from tnear to tEnd or the first surface before it, through the medium's own extinction (Beer's law a step at a
+// time), stepped as a march of the given depth (step ends on the lattice a + stepSize * (u + k), emission and
+// extinction at each step's end) until the transmittance falls below the threshold. One draw from s when the ray
+// has a stretch of the box to march.
+__device__ V3 emissiveIntegral(const RenderParams& P, Stream& s, V3 o, V3 d, float tnear, float tEnd, uint32_t depth,
+                               Counters& k, TravStack& st) {
+    const VolumeDesc& v = P.vol;
+    const x2_config& c = P.cfg;
+    {
+        ++k.traces;
+        float t;
+        if (closestHit(P, o, d, tnear, tEnd, t, k, st) != ~0u && t < tEnd) tEnd = t;
+    }
+    float a, b;
+    if (!clipBox(v, o, d, tnear, tEnd, a, b) || !(b - a > VolEpsilon)) return {0, 0, 0};
+    const float u = s.next();
+    const float stepSize = v.minFeature * c.inv_volume_quality * fmaxf(float(int(depth) - 1), 1.0f);
+    V3 L{0, 0, 0};
+    float tr = 1.0f;
+    for (uint32_t i = 0; i < MarchSteps; ++i) {
+        const float e = a + stepSize * (u + float(i));
+        const bool last = !(e < b);
+        if (last && !(e - b > VolEpsilon)) break;
+        if (tr < c.volume_transmittance_threshold) break;
+        const float at = last ? b : e, start = i == 0 ? a : a + stepSize * (u + float(i - 1));
+        const VCoeff q = coefficientsAt(v, o + d * at, true);
+        if (q.sigmaT != 0.0f) {
+            const float delta = at - start;
+            L = L + (q.emission * tr) * delta;
+            tr *= expf(-q.sigmaT * delta);
+        }
+        if (last) break;
+    }
+    return L;
+}
 // estimateInScatteringSourceTermDirect (:575-591).
 __device__ V3 volumeSourceTerm(const RenderParams& P, Stream& s, const x2_light& L, V3 Pt, V3 rayDir, float g,
                                const float ul[3], float scale, Counters& k, TravStack& st) {
@@ -1795,6 +1950,45 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
                     }
                 radiance = radiance + direct;
             }
+            // The glowing medium lighting the steps (volumeScatterRegions, :999-1057): a place drawn in the region,
+            // an equi-angular distance toward it, the region's direction pdf from there and the medium's light along
+            // that direction.
+            const EmissionRegion& R = v.region;
+            if (R.pool && (R.visibility & 0x100u)) {
+                const float invN = 1.0f / float(samplesPerLight * scatterCount);
+                V3 regionLight{0, 0, 0};
+                for (uint32_t i = 0; i < samplesPerLight * scatterCount; ++i) {
+                    Identity eid = root;
+                    eid.branch = childBranch(volumeBranch, 0xfe00u, i);
+                    Stream es(eid, depth);
+                    const float u1 = es.next(), u2 = es.next(), u3 = es.next();
+                    const V3 pivot = regionPoint(R, u1, u2, u3);
+                    const float ue = es.next();
+                    const float offset = dot(pivot - o, d);
+                    const V3 foot = o + d * offset;
+                    const float D = length(pivot - foot);
+                    const float thetaA = atan2f(firstStart - offset, D), thetaB = atan2f(rayEnd - offset, D);
+                    const float te = D * tanf((1 - ue) * thetaA + ue * thetaB);
+                    const float pdfTe = pdfEquiAngular(te, D, thetaA, thetaB);
+                    if (pdfTe == 0.0f || !finitef(pdfTe)) continue;
+                    const float tePrime = te + offset;
+                    float pdfK;
+                    bool holds;
+                    const VStep vp = stepAt(v, c, m, tePrime, pdfK, holds);
+                    if (!holds || isExactlyZero(vp.sigmaS)) continue;
+                    const float trScatter = vp.tr * expf(-vp.sigmaT * (tePrime - vp.tStart));
+                    const V3 scatterPoint = o + d * tePrime;
+                    V3 wi = pivot - scatterPoint;
+                    wi = wi * (1.0f / length(wi));
+                    float tEnd;
+                    const float pdfWi = regionPdf(R, scatterPoint, wi, tEnd);
+                    if (pdfWi == 0.0f || !finitef(pdfWi)) continue;
+                    const V3 Le = emissiveIntegral(P, es, scatterPoint, wi, 0.0f, tEnd, depth + 1, k, st);
+                    const float ph = phaseEval(v.g, -d, wi);
+                    regionLight = regionLight + (T0 * vp.sigmaS) * (Le * trScatter) * (invN * ph / (pdfTe * pdfWi));
+                }
+                radiance = radiance + regionLight;
+            }
         }
         ++volumeDepth;
         if (volumeDepth < c.max_volume_depth && c.volume_illumination_samples > 0) {
@@ -1836,6 +2030,17 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
                         usl[2] = ms.next();
                         const V3 Ls = volumeSourceTerm(P, ms, P.lights[li], scatterPoint, rayDir, g, usl, ai, k, st);
                         radiance = radiance + throughput * Ls * (invN * bi);
+                    }
+                    // the glowing medium lighting this scatter (:1127-1143)
+                    if (v.region.pool && (v.region.visibility & 0x100u)) {
+                        const float ue0 = ms.next(), ue1 = ms.next(), ue2 = ms.next();
+                        float pdfWi, tEnd;
+                        const V3 wi = regionSample(v.region, scatterPoint, ue0, ue1, ue2, pdfWi, tEnd);
+                        if (pdfWi != 0.0f && finitef(pdfWi)) {
+                            const V3 Le = emissiveIntegral(P, ms, scatterPoint, wi, 0.0f, tEnd, depth + 1, k, st);
+                            const float ph = phaseEval(g, -rayDir, wi);
+                            radiance = radiance + (throughput * Le) * (invN * bi * ph / pdfWi);
+                        }
                     }
                     const float uq0 = ms.next(), uq1 = ms.next();
                     scatterDir = phaseSample(g, -rayDir, uq0, uq1);
@@ -1965,6 +2170,111 @@ struct InlineTrace {
         return visible(P, o, d, nearT, farT, k, st);
     }
 };
+
+// The glowing medium lighting a surface vertex (computeRadianceEmissiveRegionsBundled, async_integrator.h:2481-2613):
+// the lobes the region lights as one sampler (a lobe picked in proportion to its albedo's luminance), light samples
+// (a place drawn in the region) and BSDF samples, weighed against each other by the power heuristic, each carrying
+// the medium's light along its direction (emissiveIntegral from the offset origin on the direction's side).
+__device__ void surfaceRegionLight(const RenderParams& P, const Identity& root, uint64_t branch, uint32_t depth,
+                                   bool split, const Bsdf& bsdf, V3 wo, V3 Ng, V3 Pt, V3 above, V3 below, V3 T,
+                                   V3& radiance, Counters& k, TravStack& st) {
+    const EmissionRegion& R = P.vol.region;
+    const x2_config& c = P.cfg;
+    const uint32_t nL = split ? c.light_samples : 1u, nB = split ? c.bsdf_samples : 1u;
+    uint32_t recs[2], nRec = 0;
+    float cdf[2], norm = 0.0f;
+    for (uint32_t q = 0; q < bsdf.count; ++q) {
+        if (!(lobeRayMask(bsdf.ltype[q]) & R.visibility)) continue;
+        recs[nRec] = q;
+        cdf[nRec] = fmaxf(fabsf(luminance(bsdf.kind[q] == 0 ? bsdf.albedo : bsdf.scale)), VolEpsilon);
+        norm += cdf[nRec++];
+    }
+    if (!nRec) return;
+    {
+        float acc = 0.0f;
+        for (uint32_t j = 0; j < nRec; ++j) {
+            cdf[j] = cdf[j] / norm + acc;
+            acc = cdf[j];
+        }
+    }
+    V3 fs[2];
+    uint32_t matched = 0;
+    auto add = [&](V3 contribution) {
+        for (uint32_t j = 0; j < matched; ++j) radiance = radiance + fs[j] * contribution;
+    };
+    auto emitted = [&](V3 wi, float tEnd, Stream& es) {
+        return emissiveIntegral(P, es, dot(wi, Ng) >= 0.0f ? above : below, wi, 0.0f, tEnd, depth + 1, k, st);
+    };
+    for (uint32_t ls = 0; ls < nL; ++ls) {
+        Identity eid = root;
+        eid.branch = childBranch(branch, 0xfd00u, ls);
+        Stream es(eid, depth);
+        const float u1 = es.next(), u2 = es.next(), u3 = es.next();
+        float pdfLight, tEnd;
+        const V3 wi = regionSample(R, Pt, u1, u2, u3, pdfLight, tEnd);
+        if (pdfLight == 0.0f) continue;
+        // BsdfOneSampler::eval: the matching lobes' sum, the pdf weighted by the lobes' picks
+        const uint32_t flags = surfaceFlags(Ng, wo, wi);
+        V3 f{0, 0, 0};
+        float pdfBsdf = 0.0f, prev = 0.0f;
+        matched = 0;
+        for (uint32_t j = 0; j < nRec; ++j) {
+            float p;
+            const V3 col = lobeEval(bsdf, recs[j], wo, wi, p);
+            if (matchesFlags(bsdf.ltype[recs[j]], flags)) {
+                f = f + col;
+                fs[matched++] = col;
+            }
+            pdfBsdf += p * (cdf[j] - prev);
+            prev = cdf[j];
+        }
+        if (isExactlyZero(f)) continue;
+        V3 contribution = emitted(wi, tEnd, es);
+        const float mis = nB ? powerHeuristic(float(nL) * pdfLight, float(nB) * pdfBsdf) : 1.0f;
+        contribution = (contribution * T) * (mis / (pdfLight * float(nL)));
+        add(contribution);
+    }
+    for (uint32_t bs = 0; bs < nB; ++bs) {
+        Identity eid = root;
+        eid.branch = childBranch(branch, 0xfc00u, bs);
+        Stream es(eid, depth);
+        const float uLobe = es.next(), u1 = es.next(), u2 = es.next();
+        // BsdfOneSampler::sample: a lobe by its pick, sampled, the other matching lobes' values added
+        uint32_t pick = 0;
+        while (pick + 1 < nRec && cdf[pick] <= uLobe) ++pick;
+        const float lobePdf = pick == 0 ? cdf[0] : cdf[pick] - cdf[pick - 1];
+        V3 wi;
+        float pdfBsdf;
+        V3 f = lobeSample(bsdf, recs[pick], wo, u1, u2, wi, pdfBsdf);
+        if (pdfBsdf == 0.0f) continue;
+        pdfBsdf *= lobePdf;
+        matched = 0;
+        fs[matched++] = f;
+        const uint32_t flags = surfaceFlags(Ng, wo, wi);
+        if (!matchesFlags(bsdf.ltype[recs[pick]], flags)) f = V3{0, 0, 0};
+        float prev = 0.0f;
+        for (uint32_t j = 0; j < nRec; ++j) {
+            if (j != pick) {
+                float p;
+                const V3 col = lobeEval(bsdf, recs[j], wo, wi, p);
+                if (matchesFlags(bsdf.ltype[recs[j]], flags)) {
+                    f = f + col;
+                    fs[matched++] = col;
+                }
+                pdfBsdf += p * (cdf[j] - prev);
+            }
+            prev = cdf[j];
+        }
+        if (pdfBsdf == 0.0f || isExactlyZero(f)) continue;
+        float tEnd;
+        const float pdfLight = regionPdf(R, Pt, wi, tEnd);
+        if (pdfLight == 0.0f) continue;
+        V3 contribution = emitted(wi, tEnd, es);
+        const float mis = nL ? powerHeuristic(float(nB) * pdfBsdf, float(nL) * pdfLight) : 1.0f;
+        contribution = (contribution * T) * (mis / (pdfBsdf * float(nB)));
+        add(contribution);
+    }
+}
 
 // Shades a traced vertex (pv, along ray, found hit): adds the light found there and pushes the rays it spawns onto
 // the sample's pending stack. False: an invalid value (Error; the sample adds nothing). With Adj every
@@ -2126,6 +2436,9 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
     const float thinning = bounceThinning(top, planned);
     const V3 above = hit.origin, below = hit.below;
     tracer.vertex(above, below);
+    if constexpr (Vol)
+        if (P.vol.region.pool)
+            surfaceRegionLight(P, root, pv.branch, depth, split, bsdf, wo, Ng, Pt, above, below, T, radiance, k, st);
 
     // Light samples (:2615-2785).
     const uint64_t lightBranch = childBranch(pv.branch, 0xfffe, 0);

@@ -200,7 +200,7 @@ struct x2dev {
     TexDesc* textures = nullptr;
     // the medium's bricks (volDesc.table == null: none)
     VolumeDesc volDesc{};
-    void* volBuffers[5] = {};  // table, density, temperature, majorant, ramp
+    void* volBuffers[6] = {};  // table, density, temperature, majorant, ramp, emission region pool
     bool prepared = false;
     uint32_t failPixel = 0;  // XPU2_SIMULATE_SAMPLE_FAIL as the frame began (RenderParams::failPixel)
     // film
@@ -582,6 +582,11 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
         v.temperature = s->volumes[0].temperature ? static_cast<const uint16_t*>(d->volBuffers[2]) : nullptr;
         v.majorant = static_cast<const float*>(d->volBuffers[3]);
         v.ramp = static_cast<const float*>(d->volBuffers[4]);
+        v.region.pool = nullptr;
+        if (!vol.emissionPool.empty()) {
+            X2_HIP(upload(5, vol.emissionPool.data(), vol.emissionPool.size() * sizeof(float)));
+            v.region.pool = static_cast<const float*>(d->volBuffers[5]);
+        }
         d->volDesc = v;
     }
     d->nodeCount = uint32_t(bvh.nodes.size());

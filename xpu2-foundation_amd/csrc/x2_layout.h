@@ -115,6 +115,23 @@ constexpr uint32_t RoughnessSlots = 8;  // materials whose roughness gradient a 
 // The medium on the device (x2_volume as bricks): the brick table (pool index or ~0u), the FP16 pools (512 halves a
 // brick, z, y, x within), each brick's majorant density, the emission ramp, and the derived sizes (host-computed in
 // float so that every reader rounds alike). table == null: no medium.
+// The glowing medium as a light source (an emissive region over a dense emission distribution: MoonRay's
+// EmissiveRegion and DenseEmissionDistribution, volume_emission.h:43-60): a box of res cells over the medium's box, a
+// cell's value the largest emission luminance within one voxel of it, sampled through the marginals of a 3D
+// histogram (x within each (z, y) row, then y within each z, then z), each 1D distribution a record in the pool:
+// funcInt, n, n values, n + 1 cdf entries. distToWorld maps the cells' coordinates (0..res per axis) to render space
+// and worldToDist back (row-vector affine, 12 floats: the rows x, y, z, then the translation); invUnitVolume is one
+// over a cell's volume. pool == null: the medium does not light the scene.
+struct EmissionRegion {
+    int res[3];
+    uint32_t visibility;  // the ray kinds it lights (lobeRayMask bits; 0x100: volume scattering)
+    float distToWorld[12], worldToDist[12];
+    float invUnitVolume;
+    uint32_t conditionalX, marginalY, marginalZ;  // pool offsets of each set's first record
+    const float* pool;
+};
+constexpr uint32_t EmissionCell = 4;  // voxels a region cell spans per axis
+
 struct VolumeDesc {
     float bmin[3], bmax[3];
     uint32_t res[3], nb[3];
@@ -127,6 +144,7 @@ struct VolumeDesc {
     const uint16_t* temperature;  // null: no emission
     const float* majorant;
     const float* ramp;
+    EmissionRegion region;  // region.pool == null: none
 };
 
 // Everything a render launch reads, by value.

@@ -2,7 +2,8 @@
 // density voxels is nonzero; temperature is kept for the same bricks (it reads zero elsewhere). A brick's majorant
 // is the largest density among its own voxels and its 26 neighbours' (a point in the brick interpolates voxels at
 // most one voxel away), the bound the device's ratio tracking samples free flights against. The derived sizes are
-// computed here once, in float, so that every reader of them rounds alike.
+// computed here once, in float, so that every reader of them rounds alike. A medium that glows (a temperature grid
+// and a positive emission scale) is also made a light source: its emission region (emissionRegion below).
 #include "x2_volume.h"
 
 #include <algorithm>
@@ -22,6 +23,95 @@ float halfToFloat(uint16_t h) {  // IEEE binary16, exactly
     float f;
     std::memcpy(&f, &u, 4);
     return f;
+}
+float rampLuminance(const float* ramp, uint32_t i) {
+    return 0.299f * ramp[3 * i] + 0.587f * ramp[3 * i + 1] + 0.114f * ramp[3 * i + 2];
+}
+// The ramp's entry index below kelvin, as the device's lookup places it (rampAt's x, floored and clamped).
+uint32_t rampIndex(const x2_volume& in, float kelvin) {
+    float x = kelvin / in.ramp_max_kelvin * float(in.ramp_size - 1);
+    x = fminf(fmaxf(x, 0.0f), float(in.ramp_size - 1));
+    return uint32_t(floorf(x));
+}
+// One 1D distribution of the pool (Distribution1D as cpu_scene.h:1520-1525 lays it out): its integral, n, the n
+// values, then the cdf's n + 1 entries, accumulated value by value over n and normalized (uniform when it sums to 0).
+uint32_t appendDistribution(std::vector<float>& pool, const float* func, uint32_t n) {
+    const uint32_t at = uint32_t(pool.size());
+    std::vector<float> cdf(size_t(n) + 1, 0.0f);
+    for (uint32_t i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / float(n);
+    const float integral = cdf[n];
+    for (uint32_t i = 1; i <= n; ++i) cdf[i] = integral == 0.0f ? float(i) / float(n) : cdf[i] / integral;
+    pool.push_back(integral);
+    pool.push_back(float(n));
+    pool.insert(pool.end(), func, func + n);
+    pool.insert(pool.end(), cdf.begin(), cdf.end());
+    return at;
+}
+// The medium's emission region (the reference builds one per emissive volume from a histogram of its emission
+// luminance, cpu_scene.h:1526-1545): cells of EmissionCell^3 voxels over the medium's box, each cell's value an upper
+// bound of the emission luminance anywhere in it: emission scale x the largest density x the brightest ramp entry
+// a temperature within one voxel of the cell can interpolate (a point interpolates voxels at most one voxel away, and
+// the ramp is linear between its entries). The distribution is then the reference's: each (z, y) row's x
+// distribution, each z's y marginal over the rows' integrals, the z marginal over those.
+void emissionRegion(const x2_volume& in, VolumeHost& out) {
+    VolumeDesc& v = out.desc;
+    int n[3];
+    for (int a = 0; a < 3; ++a) n[a] = int(in.res[a] / EmissionCell);
+    const size_t cells = size_t(n[0]) * size_t(n[1]) * size_t(n[2]);
+    std::vector<float> value(cells, 0.0f);
+    bool any = false;
+    const int rx = int(in.res[0]), ry = int(in.res[1]), rz = int(in.res[2]);
+    const int C = int(EmissionCell);
+    for (int cz = 0; cz < n[2]; ++cz)
+        for (int cy = 0; cy < n[1]; ++cy)
+            for (int cx = 0; cx < n[0]; ++cx) {
+                float dMax = 0.0f, tMin = FLT_MAX, tMax = -FLT_MAX;
+                for (int z = std::max(cz * C - 1, 0); z <= std::min(cz * C + C, rz - 1); ++z)
+                    for (int y = std::max(cy * C - 1, 0); y <= std::min(cy * C + C, ry - 1); ++y)
+                        for (int x = std::max(cx * C - 1, 0); x <= std::min(cx * C + C, rx - 1); ++x) {
+                            const size_t i = (size_t(z) * size_t(ry) + size_t(y)) * size_t(rx) + size_t(x);
+                            dMax = fmaxf(dMax, halfToFloat(in.density[i]));
+                            const float t = halfToFloat(in.temperature[i]);
+                            tMin = fminf(tMin, t);
+                            tMax = fmaxf(tMax, t);
+                        }
+                if (!(dMax > 0.0f)) continue;
+                const uint32_t i0 = rampIndex(in, tMin), i1 = std::min(rampIndex(in, tMax) + 1, in.ramp_size - 1);
+                float lum = 0.0f;
+                for (uint32_t i = i0; i <= i1; ++i) lum = fmaxf(lum, rampLuminance(in.ramp, i));
+                const float e = in.emission_scale * dMax * lum;
+                value[(size_t(cz) * size_t(n[1]) + size_t(cy)) * size_t(n[0]) + size_t(cx)] = e;
+                any = any || e > 0.0f;
+            }
+    if (!any) return;
+    EmissionRegion& r = v.region;
+    r = EmissionRegion{};
+    for (int a = 0; a < 3; ++a) r.res[a] = n[a];
+    r.visibility = 0x3ffu;
+    double cell[3];
+    for (int a = 0; a < 3; ++a) cell[a] = (double(in.bounds_max[a]) - double(in.bounds_min[a])) / double(n[a]);
+    for (int a = 0; a < 3; ++a) {
+        r.distToWorld[4 * a] = float(cell[a]);  // the diagonal of the 3x3 part, rows x, y, z
+        r.distToWorld[9 + a] = in.bounds_min[a];
+        r.worldToDist[4 * a] = float(1.0 / cell[a]);
+        r.worldToDist[9 + a] = float(-double(in.bounds_min[a]) / cell[a]);
+    }
+    r.invUnitVolume = float(1.0 / (cell[0] * cell[1] * cell[2]));
+    std::vector<float>& pool = out.emissionPool;
+    pool.clear();
+    std::vector<uint32_t> rows(size_t(n[2]) * size_t(n[1]));
+    r.conditionalX = uint32_t(pool.size());
+    for (int z = 0; z < n[2]; ++z)
+        for (int y = 0; y < n[1]; ++y)
+            rows[size_t(z) * size_t(n[1]) + size_t(y)] = appendDistribution(
+                pool, &value[(size_t(z) * size_t(n[1]) + size_t(y)) * size_t(n[0])], uint32_t(n[0]));
+    r.marginalY = uint32_t(pool.size());
+    std::vector<float> rowIntegrals(static_cast<size_t>(n[1])), planeIntegrals(static_cast<size_t>(n[2]));
+    for (int z = 0; z < n[2]; ++z) {
+        for (int y = 0; y < n[1]; ++y) rowIntegrals[size_t(y)] = pool[rows[size_t(z) * size_t(n[1]) + size_t(y)]];
+        planeIntegrals[size_t(z)] = pool[appendDistribution(pool, rowIntegrals.data(), uint32_t(n[1]))];
+    }
+    r.marginalZ = appendDistribution(pool, planeIntegrals.data(), uint32_t(n[2]));
 }
 }  // namespace
 
@@ -102,6 +192,7 @@ bool packVolume(const x2_volume& in, VolumeHost& out, std::string& err) {
                         }
                 out.majorant[(size_t(bz) * ny + by) * nx + bx] = m;
             }
+    if (in.temperature && in.emission_scale > 0.0f) emissionRegion(in, out);
     return true;
 }
 }  // namespace x2

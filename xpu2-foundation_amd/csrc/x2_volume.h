@@ -11,6 +11,7 @@ struct VolumeHost {
     std::vector<uint32_t> table;
     std::vector<uint16_t> density, temperature;
     std::vector<float> majorant, ramp;
+    std::vector<float> emissionPool;  // the emission region's distributions (desc.region), empty: none
 };
 bool packVolume(const x2_volume& in, VolumeHost& out, std::string& err);
 }  // namespace x2
