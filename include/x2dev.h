@@ -116,7 +116,11 @@ typedef struct x2_volume {
 } x2_volume;
 
 /* A triangle scene. Triangles are ordered mesh by mesh; mesh g owns triangles [geom_first[g], geom_first[g+1]),
- * and a triangle's primitive id is its index less geom_first of its mesh. Meshes are two-sided. A material whose
+ * and a triangle's primitive id is its index less geom_first of its mesh. Meshes are two-sided unless geom_flags
+ * marks them single-sided (X2_GEOM_SINGLE_SIDED, MoonRay's side_type single, cpu_scene.h:110-111,729): then a hit
+ * whose ray runs along the triangle's (b - a) x (c - a), dot >= 0, is no hit, for every ray (the reference's
+ * backFaceCullingFilter, async_cpu.h:122-133, and OPTIX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES,
+ * async_gpu_device.h:333,527). A material whose
  * albedo_map is k > 0 multiplies its albedo by texture k-1 at the hit's (s, t), interpolated from per-vertex uvs
  * (shade_material.h:121-160; a triangle without uvs uses (0,0), (1,0), (1,1)). */
 typedef struct x2_scene {
@@ -132,7 +136,9 @@ typedef struct x2_scene {
     uint32_t texture_count, pad2;
     const x2_volume* volumes;      /* volume_count (0 or 1) */
     uint32_t volume_count, pad3;
+    const uint32_t* geom_flags;    /* geom_count, or null: every mesh two-sided */
 } x2_scene;
+#define X2_GEOM_SINGLE_SIDED 1u
 
 /* The integrator's settings: the used subset of `mixed::Config` (gpu_jobs.h:10-44), plus rr_min_depth (an
  * extension: Russian roulette applies from that non-mirror depth on; 1 is the reference's behaviour). */

@@ -467,6 +467,7 @@ struct Scene {
     std::vector<float> triUV;  // 6 per triangle, or empty: (0,0), (1,0), (1,1)
     std::vector<Texture> textures;
     std::vector<U32> geomOf, primOf, material;
+    std::vector<uint8_t> singleSided;  // per scene triangle, empty when every mesh is two-sided
     std::vector<x2_material> materials;
     // Test hook: per-material roughness used only for the value of the GGX lobe (not for sampling, pdfs or MIS),
     // negative = the material's own. Central differences over it isolate d f / d roughness with the sampling fixed.
@@ -626,7 +627,10 @@ U32 traverse(const Scene& s, V3 org, V3 dir, float nearT, float farT, bool anyHi
     U32 bg = ~0u, bp = ~0u;
     auto test = [&](U32 tri) {
         float t;
-        if (!woopHit(w, s.tri[size_t(tri) * 3], s.tri[size_t(tri) * 3 + 1], s.tri[size_t(tri) * 3 + 2], t)) return false;
+        const V3 a = s.tri[size_t(tri) * 3], b = s.tri[size_t(tri) * 3 + 1], c = s.tri[size_t(tri) * 3 + 2];
+        if (!woopHit(w, a, b, c, t)) return false;
+        // a single-sided mesh's back face is no hit (backFaceCullingFilter, async_cpu.h:122-133)
+        if (!s.singleSided.empty() && s.singleSided[tri] && dot(dir, cross(b - a, c - a)) >= 0.0f) return false;
         if (bestTri == ~0u || keyLess(t, s.geomOf[tri], s.primOf[tri], bestT, bg, bp)) {
             bestTri = tri;
             bestT = t;
@@ -3132,6 +3136,14 @@ oracle_scene* oracle_scene_create(const x2_scene* in) {
         }
     for (U32 t = 0; t < in->triangle_count; ++t)
         for (int v = 0; v < 3; ++v) s.tri[size_t(t) * 3 + v] = load3(in->positions + size_t(in->indices[size_t(t) * 3 + v]) * 3);
+    if (in->geom_flags) {
+        bool any = false;
+        for (U32 g = 0; g < in->geom_count; ++g) any = any || (in->geom_flags[g] & X2_GEOM_SINGLE_SIDED);
+        if (any) {
+            s.singleSided.assign(in->triangle_count, 0);
+            for (U32 t = 0; t < in->triangle_count; ++t) s.singleSided[t] = (in->geom_flags[s.geomOf[t]] & X2_GEOM_SINGLE_SIDED) ? 1 : 0;
+        }
+    }
     if (in->uvs) {
         s.triUV.resize(size_t(in->triangle_count) * 6);
         for (U32 t = 0; t < in->triangle_count; ++t)

@@ -116,3 +116,28 @@ def test_light_kinds_and_materials():
     scene = scenes.SceneData(pos, idx, [0, 2, 4, 5], [0, 1, 2], mats, lights, "lights")
     config = scenes.make_config(64, 48, 1, eye=(0.0, 2.0, 6.0), target=(0.0, 0.8, 0.0), light_count=3, max_depth=6)
     assert_parity(scene, config, "hemispherical env, two-sided and reverse rect lights, GGX-only, emissive")
+
+
+def test_single_sided_meshes():
+    """Back-face culling: the Cornell box with every mesh single-sided (the walls face into the box, the boxes out)
+    and a single-sided screen between the camera and the box facing into it, so that camera rays pass through its
+    back while rays leaving the box meet its front; the walls' outsides are culled too."""
+    base, config = scenes.cornell(width=40, height=32, spp=1)
+    screen = np.array([[-0.6, -0.6, 1.5], [-0.6, 0.6, 1.5], [0.6, 0.6, 1.5], [0.6, -0.6, 1.5]], np.float32)
+    # (b - a) x (c - a) = (0, 1.2, 0) x (1.2, 1.2, 0) points to -z: into the box, away from the camera
+    n = base.positions.shape[0]
+    positions = np.concatenate([base.positions, screen])
+    indices = np.concatenate([base.indices, np.array([[n, n + 1, n + 2], [n, n + 2, n + 3]])])
+    first = np.concatenate([base.geom_first, [base.geom_first[-1] + 2]])
+    mats = np.concatenate([base.geom_material, [0]])
+    geoms = len(first) - 1
+    materials = [base.materials[i] for i in range(base.material_count)]
+    lights = [base.lights[i] for i in range(base.light_count)]
+    scene = scenes.SceneData(positions, indices, first, mats, materials, lights, "cornell single-sided",
+                             single_sided=range(geoms))
+    assert_parity(scene, config, "cornell single-sided with a screen")
+    # the screen is seen through from the camera: the camera's first hits are the box's, as without the screen
+    two = scenes.SceneData(positions, indices, first, mats, materials, lights, "cornell two-sided")
+    a = oracle.OracleScene(scene).render(config)[0]
+    b = oracle.OracleScene(two).render(config)[0]
+    assert not np.array_equal(a, b)

@@ -109,3 +109,65 @@ def test_triangle_membership_matches_fp64_away_from_edges():
             assert abs(tt.value - t) <= 1e-4 * max(1, abs(t)) / min(1.0, abs(det))
         checked += 1
     assert checked > 3000
+
+
+def quad_scene(single):
+    """Two triangles of the square [-1, 1]^2 at z = 0 wound so that (b - a) x (c - a) points to +z."""
+    pos = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    return scenes.SceneData(pos, np.array([[0, 1, 2], [0, 2, 3]]), [0, 2], [0], [scenes.material((0.5, 0.5, 0.5))],
+                            [], "quad", single_sided=[0] if single else ())
+
+
+def test_single_sided_quad():
+    """A single-sided mesh is hit only by rays against its (b - a) x (c - a); a two-sided one from either side."""
+    one, two = oracle.OracleScene(quad_scene(True)), oracle.OracleScene(quad_scene(False))
+    rng = np.random.default_rng(9)
+    for _ in range(400):
+        p = np.array([*rng.uniform(-0.9, 0.9, 2), 0.0])
+        side = 1.0 if rng.random() < 0.5 else -1.0
+        o = p + np.array([*rng.uniform(-0.5, 0.5, 2), side * rng.uniform(0.5, 3)])
+        d = (p - o) / np.linalg.norm(p - o)
+        for brute in (False, True):
+            assert two.closest(o, d, brute=brute)[0] != 0xFFFFFFFF
+            assert two.occluded(o, d, 0.0, 3.4e38, brute=brute)
+            front = side > 0  # from +z the ray runs against the normal
+            assert (one.closest(o, d, brute=brute)[0] != 0xFFFFFFFF) == front
+            assert one.occluded(o, d, 0.0, 3.4e38, brute=brute) == front
+
+
+def test_single_sided_soup_matches_two_sided_where_it_faces():
+    """Over a soup whose second mesh is single-sided: the BVH walk equals brute force; where the two-sided closest
+    hit is a front face or on the two-sided mesh it is also the single-sided scene's; where it is a culled back face,
+    the single-sided scene's hit lies beyond it (or there is none) and is not itself a culled back face."""
+    soup2 = random_soup(3000, 5)
+    soup1 = scenes.SceneData(soup2.positions, soup2.indices, soup2.geom_first, soup2.geom_material,
+                             [scenes.material((0.5, 0.5, 0.5))], [], "soup", single_sided=[1])
+    one, two = oracle.OracleScene(soup1), oracle.OracleScene(soup2)
+    tri = soup2.positions.reshape(-1, 3, 3).astype(np.float64)
+    normal = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    rng = np.random.default_rng(6)
+    culled_seen = 0
+    for _ in range(3000):
+        o = rng.uniform(-7, 7, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        a, b = one.closest(o, d), one.closest(o, d, brute=True)
+        assert a == b
+        assert one.occluded(o, d, 0.0, 3.4e38) == one.occluded(o, d, 0.0, 3.4e38, brute=True) == (a[0] != 0xFFFFFFFF)
+        t2 = two.closest(o, d)
+        if t2[0] == 0xFFFFFFFF:
+            assert a[0] == 0xFFFFFFFF
+            continue
+        g2, p2 = t2[0], t2[1]  # (geom, prim, t)
+        k2 = int(soup2.geom_first[g2]) + p2
+        back = g2 == 1 and float(normal[k2] @ d) > 0  # clear of the dot's rounding
+        front = g2 == 0 or float(normal[k2] @ d) < 0
+        if front:
+            assert a == t2
+        elif back:
+            culled_seen += 1
+            if a[0] != 0xFFFFFFFF:
+                assert a[2] >= t2[2]
+                k1 = int(soup2.geom_first[a[0]]) + a[1]
+                assert a[0] == 0 or float(normal[k1] @ d) < 0
+    assert culled_seen > 100

@@ -229,6 +229,9 @@ bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
         return false;
     }
     std::vector<uint32_t> geomOf(n), materialOf(n), primOf(n);
+    bool anySingle = false;
+    if (s.geom_flags)
+        for (uint32_t g = 0; g < s.geom_count; ++g) anySingle = anySingle || (s.geom_flags[g] & X2_GEOM_SINGLE_SIDED);
     for (uint32_t g = 0; g < s.geom_count; ++g) {
         const uint32_t a = s.geom_first[g], b = s.geom_first[g + 1];
         if (a > b || b > n) {
@@ -271,6 +274,7 @@ bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
     out.nodes.clear();
     out.tris.clear();
     out.triUV.clear();
+    out.triCull.clear();
     out.tris.reserve(n);
     out.maxStack = 0;
     out.depth = 0;
@@ -298,6 +302,7 @@ bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err) {
             r.geom = geomOf[t];
             r.prim = primOf[t];
             r.material = materialOf[t];
+            if (anySingle) out.triCull.push_back((s.geom_flags[geomOf[t]] & X2_GEOM_SINGLE_SIDED) ? 1u : 0u);
             out.tris.push_back(r);
             if (s.uvs)
                 for (int v = 0; v < 3; ++v) {

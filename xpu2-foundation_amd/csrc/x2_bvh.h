@@ -13,6 +13,7 @@ struct Bvh8Result {
     size_t deviceNodeBytes() const { return nodes.size() * sizeof(DevNode); }
     std::vector<TriRecord> tris;
     std::vector<float> triUV;  // 6 floats per triangle record when the scene has uvs
+    std::vector<uint8_t> triCull;  // 1 per triangle record (leaf order) of a single-sided mesh; empty when none is
     uint32_t maxStack = 0, depth = 0;  // traversal stack entries a ray can need; wide-tree depth
 };
 bool buildBvh8(const x2_scene& s, Bvh8Result& out, std::string& err);

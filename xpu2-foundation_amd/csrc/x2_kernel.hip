@@ -266,6 +266,13 @@ __device__ __forceinline__ void sort8(unsigned long long key[8]) {  // 19-compar
     cswap(key[3], key[4]);
 }
 
+// Back-face culling of single-sided meshes (backFaceCullingFilter, async_cpu.h:122-133; the GPU lane's
+// OPTIX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES, async_gpu_device.h:333,527): a hit whose ray runs along the triangle's
+// (b - a) x (c - a) is no hit. triCull is null unless the scene has a single-sided mesh.
+__device__ __forceinline__ bool culled(const RenderParams& P, uint32_t tri, V3 a, V3 b, V3 c, V3 dir) {
+    return P.triCull && P.triCull[tri] && dot(dir, cross(b - a, c - a)) >= 0.0f;
+}
+
 // Closest hit among triangles with t in [near, far]; returns the triangle record index (~0u: none).
 __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float nearT, float farT, float& tOut,
                                Counters& k, TravStack& st) {
@@ -305,13 +312,19 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
             ++k.triangles;
             float t;
             const float* v = P.triVerts + size_t(first + i) * 9;
-            if (!woopHit(w, V3{v[0], v[1], v[2]}, V3{v[3], v[4], v[5]}, V3{v[6], v[7], v[8]}, t)) continue;
+            const V3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
+            if (!woopHit(w, a, b, c, t) || culled(P, first + i, a, b, c, dir)) continue;
             if (best == ~0u || t < bestT) {
                 best = first + i;
                 bestT = t;
             } else if (t == bestT) {  // a tie in distance: the least (geom, prim) wins, from the records
+                // One 64-bit comparison and a select: hipcc (ROCm 7.2) has compiled the two-level comparison's
+                // nested branches with its taken side emptied, keeping the old best on an equal geom and a lesser
+                // prim (renderTexKernel, once culled() was added above).
                 const TriRecord &r = P.tris[first + i], &q = P.tris[best];
-                if (r.geom < q.geom || (r.geom == q.geom && r.prim < q.prim)) best = first + i;
+                const unsigned long long rk = (unsigned long long)r.geom << 32 | r.prim,
+                                         qk = (unsigned long long)q.geom << 32 | q.prim;
+                best = rk < qk ? first + i : best;
             }
         }
     };
@@ -363,7 +376,8 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
             ++k.triangles;
             float t;
             const float* v = P.triVerts + size_t(first + i) * 9;
-            if (woopHit(w, V3{v[0], v[1], v[2]}, V3{v[3], v[4], v[5]}, V3{v[6], v[7], v[8]}, t)) return true;
+            const V3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
+            if (woopHit(w, a, b, c, t) && !culled(P, first + i, a, b, c, dir)) return true;
         }
         return false;
     };

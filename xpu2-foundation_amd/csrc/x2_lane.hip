@@ -193,6 +193,7 @@ struct x2dev {
     x2_light* lights = nullptr;
     uint32_t nodeCount = 0, triCount = 0, lightCount = 0, materialCount = 0, maxStack = 0;
     float* triUV = nullptr;
+    uint8_t* triCull = nullptr;  // the single-sided meshes' triangles (leaf order), or null
     std::vector<float*> texels;   // one device buffer per texture: its mip pyramid, levels end to end
     float* gradPyramid = nullptr;  // the adjoint's gradients of every texture's levels 1 and up
     size_t gradPyramidFloats = 0;
@@ -291,6 +292,7 @@ void release(x2dev* d) {
     (void)hipFree(d->materials);
     (void)hipFree(d->lights);
     (void)hipFree(d->triUV);
+    (void)hipFree(d->triCull);
     (void)hipFree(d->textures);
     for (float* t : d->texels) (void)hipFree(t);
     (void)hipFree(d->gradPyramid);
@@ -487,6 +489,7 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     (void)hipFree(d->materials);
     (void)hipFree(d->lights);
     (void)hipFree(d->triUV);
+    (void)hipFree(d->triCull);
     (void)hipFree(d->textures);
     for (float* t : d->texels) (void)hipFree(t);
     (void)hipFree(d->gradPyramid);
@@ -501,6 +504,7 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     d->texels.clear();
     d->texDesc.clear();
     d->triUV = nullptr;
+    d->triCull = nullptr;
     d->textures = nullptr;
     d->nodes = nullptr;
     d->tris = nullptr;
@@ -524,6 +528,10 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     if (!bvh.triUV.empty()) {
         X2_HIP(hipMalloc(&d->triUV, bvh.triUV.size() * sizeof(float)));
         X2_HIP(hipMemcpy(d->triUV, bvh.triUV.data(), bvh.triUV.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (!bvh.triCull.empty()) {
+        X2_HIP(hipMalloc(&d->triCull, bvh.triCull.size()));
+        X2_HIP(hipMemcpy(d->triCull, bvh.triCull.data(), bvh.triCull.size(), hipMemcpyHostToDevice));
     }
     unsigned long long gradOffset = 0;
     for (uint32_t i = 0; i < s->texture_count; ++i) {
@@ -777,6 +785,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
     p.tris = d->tris;
     p.triVerts = reinterpret_cast<const float*>(reinterpret_cast<const char*>(d->tris) + triVertsOffset(d->triCount));
     p.triUV = d->triUV;
+    p.triCull = d->triCull;
     p.textures = d->textures;
     p.materials = d->materials;
     p.lights = d->lights;

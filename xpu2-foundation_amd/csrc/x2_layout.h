@@ -154,6 +154,7 @@ struct RenderParams {
     const TriRecord* tris;
     const float* triVerts;  // the records' vertices alone, 9 floats a triangle, for the leaf tests
     const float* triUV;  // 6 floats per triangle record (leaf order), or null: (0,0), (1,0), (1,1)
+    const uint8_t* triCull;  // 1 per triangle record (leaf order) of a single-sided mesh, or null: none is
     const TexDesc* textures;
     const x2_material* materials;
     const x2_light* lights;

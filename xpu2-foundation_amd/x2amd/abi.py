@@ -83,6 +83,9 @@ class Volume(ctypes.Structure):
     ]
 
 
+X2_GEOM_SINGLE_SIDED = 1  # x2_scene.geom_flags: the mesh's back faces are culled
+
+
 class Scene(ctypes.Structure):
     _fields_ = [
         ("positions", ctypes.POINTER(f32)), ("indices", ctypes.POINTER(u32)),
@@ -92,6 +95,7 @@ class Scene(ctypes.Structure):
         ("light_count", u32), ("pad", u32),
         ("uvs", ctypes.POINTER(f32)), ("textures", ctypes.POINTER(Texture)), ("texture_count", u32), ("pad2", u32),
         ("volumes", ctypes.POINTER(Volume)), ("volume_count", u32), ("pad3", u32),
+        ("geom_flags", ctypes.POINTER(u32)),
     ]
 
 
@@ -214,7 +218,7 @@ class PagingStats(ctypes.Structure):
 # CPU test that compiles the header.
 EXPECTED_SIZES = {
     "x2_chunk": (Chunk, 16), "x2_parked": (Parked, 16), "x2_refine": (Refine, 32), "x2_light": (Light, 184),
-    "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 112),
+    "x2_material": (Material, 40), "x2_texture": (Texture, 16), "x2_volume": (Volume, 96), "x2_scene": (Scene, 120),
     "x2_config": (Config, 292),
     "x2_completion": (Completion, 96), "x2dev_options": (Options, 24), "x2_frame_stats": (FrameStats, 88),
     "x2_batch_tuning": (BatchTuning, 24), "x2_batch_stats": (BatchStats, 80), "x2_rng_query": (RngQuery, 48), "x2_adaptive": (Adaptive, 40),

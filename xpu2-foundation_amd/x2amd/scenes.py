@@ -35,7 +35,7 @@ class SceneData:
     """A flat triangle scene and the ctypes view of it (kept alive with the arrays)."""
 
     def __init__(self, positions, indices, geom_first, geom_material, materials, lights, name, uvs=None,
-                 textures=(), volume=None):
+                 textures=(), volume=None, single_sided=()):
         self.name = name
         self.positions = np.ascontiguousarray(positions, dtype=np.float32)
         self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
@@ -73,6 +73,11 @@ class SceneData:
         if volume is not None:
             self.c.volumes = ctypes.pointer(volume.c)
             self.c.volume_count = 1
+        self.geom_flags = None  # the meshes whose back faces are culled (single_sided: their indices)
+        if len(single_sided):
+            self.geom_flags = np.zeros(len(self.geom_first) - 1, np.uint32)
+            self.geom_flags[list(single_sided)] = abi.X2_GEOM_SINGLE_SIDED
+            self.c.geom_flags = self.geom_flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
 
     @property
     def triangle_count(self):
