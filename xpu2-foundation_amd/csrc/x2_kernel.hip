@@ -1761,8 +1761,9 @@ __device__ V3 regionPoint(const EmissionRegion& r, float u1, float u2, float u3)
 }
 // The solid-angle pdf of direction wi from p: the 3D pdf integrated along the ray with the t^2 Jacobian, cell by
 // cell (a DDA through the cells, each cell contributing pdf * (t1^3 - t0^3) / 3); tEnd: where the ray leaves the
-// region's box (render-space distance).
-__device__ float regionPdf(const EmissionRegion& r, V3 pWorld, V3 wiWorld, float& tEnd) {
+// region's box (render-space distance); glow (when given): a distance past which the ray meets only cells of zero
+// value up to the box's exit (a little past the last positive cell's exit), 0 when it meets none.
+__device__ float regionPdf(const EmissionRegion& r, V3 pWorld, V3 wiWorld, float& tEnd, float* glow = nullptr) {
     const V3 p = affinePoint(r.worldToDist, pWorld);
     V3 w = affineVector(r.worldToDist, wiWorld);
     const float lengthScale = 1.0f / sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
@@ -1795,7 +1796,7 @@ __device__ float regionPdf(const EmissionRegion& r, V3 pWorld, V3 wiWorld, float
         stepDir[ax] = up ? 1 : -1;
         stop[ax] = up ? r.res[ax] : -1;
     }
-    float t = tStart, sum = 0.0f;
+    float t = tStart, sum = 0.0f, lastLit = 0.0f;
     const int guard = (r.res[0] + r.res[1] + r.res[2]) * 2 + 8;
     for (int i = 0; i < guard; ++i) {
         const float pd = cellPdf(r, cell[0], cell[1], cell[2]);
@@ -1803,27 +1804,33 @@ __device__ float regionPdf(const EmissionRegion& r, V3 pWorld, V3 wiWorld, float
         if (pd > 0.0f) {
             const float t0 = t * lengthScale, t1 = tNext[ax] * lengthScale;
             sum += pd * r.invUnitVolume * (t1 * t1 * t1 - t0 * t0 * t0) / 3.0f;
+            lastLit = t1;
         }
         t = tNext[ax];
         cell[ax] += stepDir[ax];
         if (cell[ax] == stop[ax]) break;
         tNext[ax] += tDelta[ax];
     }
+    // past the last positive cell by 1e-4 of the distance: well inside the next cell (a cell spans 4 voxels), whose
+    // zero value means no emission there
+    if (glow) *glow = lastLit * 1.0001f + 1e-6f;
     return sum;
 }
-__device__ V3 regionSample(const EmissionRegion& r, V3 p, float u1, float u2, float u3, float& pdf, float& tEnd) {
+__device__ V3 regionSample(const EmissionRegion& r, V3 p, float u1, float u2, float u3, float& pdf, float& tEnd,
+                           float* glow = nullptr) {
     V3 wi = regionPoint(r, u1, u2, u3) - p;
     wi = wi * (1.0f / sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z));
-    pdf = regionPdf(r, p, wi, tEnd);
+    pdf = regionPdf(r, p, wi, tEnd, glow);
     return wi;
 }
 // computeEmissiveVolumeIntegral (async_integrator.h:478-566) in the medium's box: the medium's light along the ray
 // from tnear to tEnd or the first surface before it, through the medium's own extinction (Beer's law a step at a
 // time), stepped as a march of the given depth (step ends on the lattice a + stepSize * (u + k), emission and
 // extinction at each step's end) until the transmittance falls below the threshold. One draw from s when the ray
-// has a stretch of the box to march.
-__device__ V3 emissiveIntegral(const RenderParams& P, Stream& s, V3 o, V3 d, float tnear, float tEnd, uint32_t depth,
-                               Counters& k, TravStack& st) {
+// has a stretch of the box to march. glow: a distance past which the ray meets no emission (regionPdf's); the march
+// stops there, which changes nothing it adds (every step beyond adds zero emission) and saves its steps.
+__device__ V3 emissiveIntegral(const RenderParams& P, Stream& s, V3 o, V3 d, float tnear, float tEnd, float glow,
+                               uint32_t depth, Counters& k, TravStack& st) {
     const VolumeDesc& v = P.vol;
     const x2_config& c = P.cfg;
     {
@@ -1837,6 +1844,7 @@ __device__ V3 emissiveIntegral(const RenderParams& P, Stream& s, V3 o, V3 d, flo
     const float stepSize = v.minFeature * c.inv_volume_quality * fmaxf(float(int(depth) - 1), 1.0f);
     V3 L{0, 0, 0};
     float tr = 1.0f;
+    b = fminf(b, glow);
     for (uint32_t i = 0; i < MarchSteps; ++i) {
         const float e = a + stepSize * (u + float(i));
         const bool last = !(e < b);
@@ -1994,10 +2002,10 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
                     const V3 scatterPoint = o + d * tePrime;
                     V3 wi = pivot - scatterPoint;
                     wi = wi * (1.0f / length(wi));
-                    float tEnd;
-                    const float pdfWi = regionPdf(R, scatterPoint, wi, tEnd);
+                    float tEnd, glow;
+                    const float pdfWi = regionPdf(R, scatterPoint, wi, tEnd, &glow);
                     if (pdfWi == 0.0f || !finitef(pdfWi)) continue;
-                    const V3 Le = emissiveIntegral(P, es, scatterPoint, wi, 0.0f, tEnd, depth + 1, k, st);
+                    const V3 Le = emissiveIntegral(P, es, scatterPoint, wi, 0.0f, tEnd, glow, depth + 1, k, st);
                     const float ph = phaseEval(v.g, -d, wi);
                     regionLight = regionLight + (T0 * vp.sigmaS) * (Le * trScatter) * (invN * ph / (pdfTe * pdfWi));
                 }
@@ -2048,10 +2056,10 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
                     // the glowing medium lighting this scatter (:1127-1143)
                     if (v.region.pool && (v.region.visibility & 0x100u)) {
                         const float ue0 = ms.next(), ue1 = ms.next(), ue2 = ms.next();
-                        float pdfWi, tEnd;
-                        const V3 wi = regionSample(v.region, scatterPoint, ue0, ue1, ue2, pdfWi, tEnd);
+                        float pdfWi, tEnd, glow;
+                        const V3 wi = regionSample(v.region, scatterPoint, ue0, ue1, ue2, pdfWi, tEnd, &glow);
                         if (pdfWi != 0.0f && finitef(pdfWi)) {
-                            const V3 Le = emissiveIntegral(P, ms, scatterPoint, wi, 0.0f, tEnd, depth + 1, k, st);
+                            const V3 Le = emissiveIntegral(P, ms, scatterPoint, wi, 0.0f, tEnd, glow, depth + 1, k, st);
                             const float ph = phaseEval(g, -rayDir, wi);
                             radiance = radiance + (throughput * Le) * (invN * bi * ph / pdfWi);
                         }
@@ -2216,16 +2224,16 @@ __device__ void surfaceRegionLight(const RenderParams& P, const Identity& root, 
     auto add = [&](V3 contribution) {
         for (uint32_t j = 0; j < matched; ++j) radiance = radiance + fs[j] * contribution;
     };
-    auto emitted = [&](V3 wi, float tEnd, Stream& es) {
-        return emissiveIntegral(P, es, dot(wi, Ng) >= 0.0f ? above : below, wi, 0.0f, tEnd, depth + 1, k, st);
+    auto emitted = [&](V3 wi, float tEnd, float glow, Stream& es) {
+        return emissiveIntegral(P, es, dot(wi, Ng) >= 0.0f ? above : below, wi, 0.0f, tEnd, glow, depth + 1, k, st);
     };
     for (uint32_t ls = 0; ls < nL; ++ls) {
         Identity eid = root;
         eid.branch = childBranch(branch, 0xfd00u, ls);
         Stream es(eid, depth);
         const float u1 = es.next(), u2 = es.next(), u3 = es.next();
-        float pdfLight, tEnd;
-        const V3 wi = regionSample(R, Pt, u1, u2, u3, pdfLight, tEnd);
+        float pdfLight, tEnd, glow;
+        const V3 wi = regionSample(R, Pt, u1, u2, u3, pdfLight, tEnd, &glow);
         if (pdfLight == 0.0f) continue;
         // BsdfOneSampler::eval: the matching lobes' sum, the pdf weighted by the lobes' picks
         const uint32_t flags = surfaceFlags(Ng, wo, wi);
@@ -2243,7 +2251,7 @@ __device__ void surfaceRegionLight(const RenderParams& P, const Identity& root, 
             prev = cdf[j];
         }
         if (isExactlyZero(f)) continue;
-        V3 contribution = emitted(wi, tEnd, es);
+        V3 contribution = emitted(wi, tEnd, glow, es);
         const float mis = nB ? powerHeuristic(float(nL) * pdfLight, float(nB) * pdfBsdf) : 1.0f;
         contribution = (contribution * T) * (mis / (pdfLight * float(nL)));
         add(contribution);
@@ -2280,10 +2288,10 @@ __device__ void surfaceRegionLight(const RenderParams& P, const Identity& root, 
             prev = cdf[j];
         }
         if (pdfBsdf == 0.0f || isExactlyZero(f)) continue;
-        float tEnd;
-        const float pdfLight = regionPdf(R, Pt, wi, tEnd);
+        float tEnd, glow;
+        const float pdfLight = regionPdf(R, Pt, wi, tEnd, &glow);
         if (pdfLight == 0.0f) continue;
-        V3 contribution = emitted(wi, tEnd, es);
+        V3 contribution = emitted(wi, tEnd, glow, es);
         const float mis = nL ? powerHeuristic(float(nB) * pdfBsdf, float(nL) * pdfLight) : 1.0f;
         contribution = (contribution * T) * (mis / (pdfBsdf * float(nB)));
         add(contribution);
