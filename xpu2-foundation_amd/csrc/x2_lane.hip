@@ -915,6 +915,9 @@ int commitTiles(x2dev* d, uint32_t slot) {
             ++g.evictions;
         }
         g.loadSeq[at] = ++g.loads;
+        // the launch this commit precedes reads it (its parked samples asked for it): a later commit, before that
+        // launch's stamps are read back, must not take it for a stale one
+        g.lastUse[tile] = std::max(g.lastUse[tile], uint32_t(d->launches + 1));
         g.tileIn[at] = tile;
         g.slotOf[tile] = at;
         g.queued[tile] = 0;
