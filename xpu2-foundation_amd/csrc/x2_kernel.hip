@@ -165,6 +165,7 @@ __device__ __forceinline__ bool woopHit(const WoopRay& w, V3 a, V3 b, V3 c, floa
 // depend on the order nodes are visited in. Boxes are culled conservatively: the far distance is scaled by
 // 1 + 2 gamma(3) (Ize 2013) and compared against the best distance with a relative slack of 2^-12.
 constexpr float FarScale = 1.0f + 2.0f * (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
+static_assert(FarScale - 1.0f == 3.0f * 0x1p-23f, "the far scale is 1 + 3 ulp of 1");
 constexpr float Slack = 1.0f + 0x1p-12f;
 
 // A lane's traversal stack in private memory: the closest-hit walk's entries are (child ref, entry distance) pairs,
@@ -224,22 +225,26 @@ __device__ __forceinline__ void nodeChildren(const Node8Q& n, const BoxRay& r, f
     auto at = [](uint32_t a, uint32_t b, int i) {  // byte i of the eight in (a, b)
         return float(((i < 4 ? a : b) >> (8 * (i & 3))) & 0xffu);
     };
+    // the near and far planes' bytes per axis, chosen once for the node by the ray's direction (lows near when the
+    // direction is positive)
+    const uint32_t nx0 = r.sx ? b0.x : b0.z, nx1 = r.sx ? b0.y : b0.w, fx0 = r.sx ? b0.z : b0.x, fx1 = r.sx ? b0.w : b0.y;
+    const uint32_t ny0 = r.sy ? b1.x : b1.z, ny1 = r.sy ? b1.y : b1.w, fy0 = r.sy ? b1.z : b1.x, fy1 = r.sy ? b1.w : b1.y;
+    const uint32_t nz0 = r.sz ? b2.x : b2.z, nz1 = r.sz ? b2.y : b2.w, fz0 = r.sz ? b2.z : b2.x, fz1 = r.sz ? b2.w : b2.y;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         // q * step is exact (an 8-bit integer times a power of two), so the fused form rounds once, as the builder's
         // origin + q * step does: the same bound in one instruction
-        const float lx = __builtin_fmaf(at(b0.x, b0.y, i), sx, ox), hx = __builtin_fmaf(at(b0.z, b0.w, i), sx, ox);
-        const float ly = __builtin_fmaf(at(b1.x, b1.y, i), sy, oy), hy = __builtin_fmaf(at(b1.z, b1.w, i), sy, oy);
-        const float lz = __builtin_fmaf(at(b2.x, b2.y, i), sz, oz), hz = __builtin_fmaf(at(b2.z, b2.w, i), sz, oz);
-        const float nx = __builtin_fmaf(r.sx ? lx : hx, r.ix, r.ox);
-        const float fx = __builtin_fmaf(r.sx ? hx : lx, r.ix, r.ox);
-        const float ny = __builtin_fmaf(r.sy ? ly : hy, r.iy, r.oy);
-        const float fy = __builtin_fmaf(r.sy ? hy : ly, r.iy, r.oy);
-        const float nz = __builtin_fmaf(r.sz ? lz : hz, r.iz, r.oz);
-        const float fz = __builtin_fmaf(r.sz ? hz : lz, r.iz, r.oz);
+        const float nx = __builtin_fmaf(__builtin_fmaf(at(nx0, nx1, i), sx, ox), r.ix, r.ox);
+        const float fx = __builtin_fmaf(__builtin_fmaf(at(fx0, fx1, i), sx, ox), r.ix, r.ox);
+        const float ny = __builtin_fmaf(__builtin_fmaf(at(ny0, ny1, i), sy, oy), r.iy, r.oy);
+        const float fy = __builtin_fmaf(__builtin_fmaf(at(fy0, fy1, i), sy, oy), r.iy, r.oy);
+        const float nz = __builtin_fmaf(__builtin_fmaf(at(nz0, nz1, i), sz, oz), r.iz, r.oz);
+        const float fz = __builtin_fmaf(__builtin_fmaf(at(fz0, fz1, i), sz, oz), r.iz, r.oz);
         const float tn = fmaxf(fmaxf(nx, ny), nz);
         float tf = fminf(fminf(fx, fy), fz);
-        tf = tf * (tf > 0.0f ? FarScale : 1.0f / FarScale);
+        // tf (1 + 2 gamma(3)) for tf > 0 in one rounding, as tf * FarScale (FarScale - 1 = 3 * 2^-23 exactly);
+        // towards zero otherwise
+        tf = __builtin_fmaf(fabsf(tf), FarScale - 1.0f, tf);
         const bool hit = ch[i] != EmptyChild && tn <= tf && tn <= hi && tf >= lo;
         key[i] = hit ? ((unsigned long long)__float_as_uint(fmaxf(tn, 0.0f)) << 32) | ch[i] : ~0ull;
     }
