@@ -142,15 +142,16 @@ class GpuLane:
     def film_ready(self):
         return abi.check(self.lib.x2dev_film_ready(self.h), "x2dev_film_ready") == 1
 
-    def film(self):
-        """(sums [pixels, 4] float64, counts [pixels] uint32), copies of the downloaded film."""
+    def film(self, copy=True):
+        """(sums [pixels, 4] float64, counts [pixels] uint32) of the downloaded film: copies, or with copy=False views
+        of the lane's pinned host film, valid until its next download (GpuLane::sums/counts)."""
         s = ctypes.POINTER(ctypes.c_double)()
         c = ctypes.POINTER(ctypes.c_uint32)()
         abi.check(self.lib.x2dev_film(self.h, ctypes.byref(s), ctypes.byref(c)), "x2dev_film")
         n = self.config.width * self.config.height
-        sums = np.ctypeslib.as_array(s, shape=(n * 4,)).reshape(n, 4).copy()
-        counts = np.ctypeslib.as_array(c, shape=(n,)).copy()
-        return sums, counts
+        sums = np.ctypeslib.as_array(s, shape=(n * 4,)).reshape(n, 4)
+        counts = np.ctypeslib.as_array(c, shape=(n,))
+        return (sums.copy(), counts.copy()) if copy else (sums, counts)
 
     def set_texture(self, index, texels, on_device=False):
         """texels: a float32 array (host) or a device pointer (int) when on_device."""
@@ -178,15 +179,16 @@ class GpuLane:
             raise RuntimeError(f"x2dev_adjoint_frame failed ({r}): {err.decode() if err else ''}")
         return st
 
-    def render_frame(self, config, samples_per_chunk=4):
-        """One uniform frame through the broker counterpart; returns (sums, counts, FrameStats)."""
+    def render_frame(self, config, samples_per_chunk=4, copy=True):
+        """One uniform frame through the broker counterpart; returns (sums, counts, FrameStats) (copy=False: views of
+        the pinned host film, valid until the lane's next download)."""
         st = abi.FrameStats()
         r = self.lib.x2dev_render_frame(self.h, ctypes.byref(config), samples_per_chunk, ctypes.byref(st))
         self.config = config
         if r < 0:
             err = self.lib.x2dev_last_error()
             raise RuntimeError(f"x2dev_render_frame failed ({r}): {err.decode() if err else ''}")
-        sums, counts = self.film()
+        sums, counts = self.film(copy)
         return sums, counts, st
 
     def chunk_room(self):
