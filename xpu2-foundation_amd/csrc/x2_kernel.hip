@@ -2855,7 +2855,8 @@ __device__ __forceinline__ void poolFlush(const RenderParams& P, PoolQueue& q, V
     q.n = 0;
 }
 // runSamples with the shadow rays pooled (the forward surface estimator): every lane stays in the loop until the
-// wave's samples are all done, so that each iteration's pool is traced by the whole wave.
+// wave's samples are all done, so that each iteration's pool is traced by the whole wave. Only a kernel whose one
+// argument is the RenderParams calls it: the loop reads them at the start of the kernel's argument segment.
 template <class Feed>
 __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k, TravStack& st, PoolQueue& q) {
     PathState<false, false> S;
@@ -2863,6 +2864,13 @@ __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k,
     uint32_t pixel = 0, index = 0;
     bool live = false, done = false;
     for (;;) {
+        // The parameters read through a pointer made opaque each iteration: the compiler then loads each field
+        // where it is used (scalar loads from the kernel's argument segment) instead of holding every field it
+        // can hoist in scalar registers across the loop, which spilled 200 of them to vector lanes (every reload
+        // a VALU read-lane; 55 spilled this way).
+        auto kp = (const __attribute__((address_space(4))) RenderParams*)__builtin_amdgcn_kernarg_segment_ptr();
+        asm volatile("" : "+s"(kp));
+        const RenderParams& PL = *(const RenderParams*)kp;
         const bool idle = S.top == 0;
         if (idle && live) {
             feed.done(pixel, index, S.radiance, S.alpha, 0u);
@@ -2873,7 +2881,7 @@ __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k,
         const bool claim = idle && !done && (__popcll(idleLanes) >= Feed::Regen || idleLanes == openLanes);
         if (claim) {
             if (!feed.next(pixel, index)) done = true;
-            else if (!startSample<false, false>(P, pixel, index, S)) {
+            else if (!startSample<false, false>(PL, pixel, index, S)) {
                 feed.failed();
                 S.top = 0;
             } else {
@@ -2881,7 +2889,7 @@ __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k,
             }
         }
         if (S.top) {
-            if (!shadeVertexWith<false, false>(P, pixel, index, S, q, k, st, nullptr) || q.overflow) {
+            if (!shadeVertexWith<false, false>(PL, pixel, index, S, q, k, st, nullptr) || q.overflow) {
                 feed.failed();  // an invalid value, or more shadow rays than the pool holds (the host prevents it)
                 S.top = 0;
                 live = false;
@@ -2889,7 +2897,7 @@ __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k,
                 q.overflow = false;
             }
         }
-        poolFlush(P, q, S.radiance, k, st);
+        poolFlush(PL, q, S.radiance, k, st);
     }
 }
 
