@@ -261,16 +261,6 @@ __device__ __forceinline__ bool keyLess(float t, uint32_t g, uint32_t p, float b
 constexpr uint32_t Sentinel = 0x7fffffffu;  // an empty stack's pop: neither a node index nor a leaf
 
 
-__device__ __forceinline__ void sort8(unsigned long long key[8]) {  // 19-comparator network: nearest first
-    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[4], key[5]); cswap(key[6], key[7]);
-    cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[4], key[6]); cswap(key[5], key[7]);
-    cswap(key[1], key[2]); cswap(key[5], key[6]); cswap(key[0], key[4]); cswap(key[3], key[7]);
-    cswap(key[1], key[5]); cswap(key[2], key[6]);
-    cswap(key[1], key[4]); cswap(key[3], key[6]);
-    cswap(key[2], key[4]); cswap(key[3], key[5]);
-    cswap(key[3], key[4]);
-}
-
 // Back-face culling of single-sided meshes (backFaceCullingFilter, async_cpu.h:122-133; the GPU lane's
 // OPTIX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES, async_gpu_device.h:333,527): a hit whose ray runs along the triangle's
 // (b - a) x (c - a) is no hit. triCull is null unless the scene has a single-sided mesh.
@@ -302,14 +292,18 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
         const float hi = best == ~0u ? (farT > 0 ? farT * Slack : farT / Slack) : (bestT > 0 ? bestT * Slack : bestT / Slack);
         unsigned long long key[8];
         nodeChildren(P.nodes[node], br, lo, hi, key);
-        sort8(key);
+        // The nearest child next, the others stacked in child order: sorting all eight (a 19-comparator network on
+        // the 64-bit keys) culled no more than it cost (583 against 578 Mpaths/s on the 1080p icospheres frame).
+        unsigned long long m = key[0];
 #pragma unroll
-        for (int i = 7; i >= 1; --i)
-            if (key[i] != ~0ull) {
+        for (int i = 1; i < 8; ++i) m = key[i] < m ? key[i] : m;
+#pragma unroll
+        for (int i = 7; i >= 0; --i)
+            if (key[i] != ~0ull && key[i] != m) {
                 st.put(sp, make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32)));
                 ++sp;
             }
-        return key[0] != ~0ull ? uint32_t(key[0]) : pop();
+        return m != ~0ull ? uint32_t(m) : pop();
     };
     auto visitLeaf = [&](uint32_t leafRef) {
         const uint32_t first = (leafRef & 0x7fffffffu) >> 3, count = (leafRef & 7u) + 1;
