@@ -2471,15 +2471,20 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
         lid.branch = lightAddress;
         Stream ls(lid, depth);
         for (uint32_t s = 0; s < nl; ++s) {
+            // the sample's stream block and, at a split vertex, its stratified pair's block, computed together
+            const Identity strat{pixel, uint64_t(index) * nl + s, c.seed, lightAddress};
+            Identity src = lid;
+            uint32_t srcGroup = 0;
+            if (split) pairSource(strat, CameraDimensions + 2, c.sampling, src, srcGroup);
+            Words streamBlock, pairBlock;
+            block2(lid, ls.nextGroup(), src, srcGroup, streamBlock, pairBlock);
+            ls.prime(streamBlock);
             float r[3];
             r[0] = ls.next();
             r[1] = ls.next();
             r[2] = ls.next();
             const float u = ls.next();
-            if (split) {
-                const Identity strat{pixel, uint64_t(index) * nl + s, c.seed, lightAddress};
-                drawPair(strat, CameraDimensions + 2, c.sampling, r[0], r[1]);
-            }
+            if (split) pairFrom(strat, CameraDimensions + 2, c.sampling, pairBlock, r[0], r[1]);
             V3 wi;
             LightHit lh;
             if (!lightSample(L, Pt, cull, cullN, r, wi, lh)) continue;
@@ -2547,12 +2552,17 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
             Identity bid = root;
             bid.branch = sampleBranch;
             Stream bs(bid, depth);
+            // the sample's stream block and, at a split vertex, its stratified pair's block, computed together
+            const Identity strat{pixel, uint64_t(index) * nk + i, c.seed, lobeBranch};
+            Identity src = bid;
+            uint32_t srcGroup = 0;
+            if (split) pairSource(strat, CameraDimensions, c.sampling, src, srcGroup);
+            Words streamBlock, pairBlock;
+            block2(bid, bs.nextGroup(), src, srcGroup, streamBlock, pairBlock);
+            bs.prime(streamBlock);
             float r1 = bs.next(), r2 = bs.next();
             const float choice = bs.next(), u = bs.next();
-            if (split) {
-                const Identity strat{pixel, uint64_t(index) * nk + i, c.seed, lobeBranch};
-                drawPair(strat, CameraDimensions, c.sampling, r1, r2);
-            }
+            if (split) pairFrom(strat, CameraDimensions, c.sampling, pairBlock, r1, r2);
             V3 wi;
             float pdf;
             const V3 f = lobeSample(bsdf, q, wo, r1, r2, wi, pdf);

@@ -42,6 +42,35 @@ __device__ __forceinline__ Words block(const Identity& id, uint32_t group) {
     const uint64_t k = id.stream ^ (id.branch * 0x9e3779b97f4a7c15ull);
     return philox(id.pixel, uint32_t(id.index), uint32_t(id.index >> 32), group, uint32_t(k), uint32_t(k >> 32));
 }
+// Two blocks computed together, their rounds interleaved: four independent multiplies a round instead of two, so a
+// round's latency is spent once for both (the same words as two block() calls).
+__device__ __forceinline__ void block2(const Identity& a, uint32_t ga, const Identity& b, uint32_t gb, Words& wa,
+                                       Words& wb) {
+    const uint64_t ka = a.stream ^ (a.branch * 0x9e3779b97f4a7c15ull), kb = b.stream ^ (b.branch * 0x9e3779b97f4a7c15ull);
+    uint32_t a0 = a.pixel, a1 = uint32_t(a.index), a2 = uint32_t(a.index >> 32), a3 = ga, aka = uint32_t(ka),
+             akb = uint32_t(ka >> 32);
+    uint32_t b0 = b.pixel, b1 = uint32_t(b.index), b2 = uint32_t(b.index >> 32), b3 = gb, bka = uint32_t(kb),
+             bkb = uint32_t(kb >> 32);
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t pa = uint64_t(a0) * 0xd2511f53u, pb = uint64_t(a2) * 0xcd9e8d57u;
+        const uint64_t qa = uint64_t(b0) * 0xd2511f53u, qb = uint64_t(b2) * 0xcd9e8d57u;
+        a0 = uint32_t(pb >> 32) ^ a1 ^ aka;
+        a1 = uint32_t(pb);
+        a2 = uint32_t(pa >> 32) ^ a3 ^ akb;
+        a3 = uint32_t(pa);
+        b0 = uint32_t(qb >> 32) ^ b1 ^ bka;
+        b1 = uint32_t(qb);
+        b2 = uint32_t(qa >> 32) ^ b3 ^ bkb;
+        b3 = uint32_t(qa);
+        aka += 0x9e3779b9u;
+        akb += 0xbb67ae85u;
+        bka += 0x9e3779b9u;
+        bkb += 0xbb67ae85u;
+    }
+    wa = Words{{a0, a1, a2, a3}};
+    wb = Words{{b0, b1, b2, b3}};
+}
 __device__ __forceinline__ uint32_t pick(const Words& w, uint32_t i) {
     return i == 0 ? w.v[0] : i == 1 ? w.v[1] : i == 2 ? w.v[2] : w.v[3];
 }
@@ -92,6 +121,34 @@ __device__ __forceinline__ uint32_t samplingWord(const Identity& id, uint32_t di
 __device__ __forceinline__ float draw(const Identity& id, uint32_t dimension, uint32_t policy) {
     return unit(samplingWord(id, dimension, policy));
 }
+// drawPair in two steps, for a caller that computes the pair's block together with another: pairSource names the
+// block (identity and group) the pair reads, pairFrom makes the two draws from it.
+__device__ __forceinline__ void pairSource(const Identity& id, uint32_t dimension, uint32_t policy, Identity& src,
+                                           uint32_t& group) {
+    const bool camera = dimension < 5, diffuse = dimension == 16, light = dimension == 18;
+    const bool sobolPair = (camera && (policy & 1)) || (diffuse && (policy & 2)) || (light && (policy & 4));
+    src = id;
+    if (!sobolPair) {
+        group = dimension >> 2;
+        return;
+    }
+    const uint32_t d = camera ? dimension : dimension - 14;
+    src.index = id.index >> 32;
+    group = 0x534f424fu + (d >> 2);
+}
+__device__ __forceinline__ void pairFrom(const Identity& id, uint32_t dimension, uint32_t policy, const Words& w,
+                                         float& u0, float& u1) {
+    const bool camera = dimension < 5, diffuse = dimension == 16, light = dimension == 18;
+    const bool sobolPair = (camera && (policy & 1)) || (diffuse && (policy & 2)) || (light && (policy & 4));
+    if (!sobolPair) {
+        u0 = unit(pick(w, dimension & 3));
+        u1 = unit(pick(w, (dimension + 1) & 3));
+        return;
+    }
+    const uint32_t d = camera ? dimension : dimension - 14;
+    u0 = unit(owenScramble(sobol(uint32_t(id.index), d), pick(w, d & 3)));
+    u1 = unit(owenScramble(sobol(uint32_t(id.index), d + 1), pick(w, (d + 1) & 3)));
+}
 // Two draws of an even dimension and the next (0/1 camera, 16/17 first-hit BSDF, 18/19 first-hit light), the Philox
 // block they share computed once: Philox words of one group, or the Sobol pair's one seed block. The same words as
 // two draw() calls.
@@ -122,6 +179,12 @@ struct Stream {
     __device__ __forceinline__ Stream() : id{0, 0, 0, 0}, dimension(0), group(~0u), w{{0, 0, 0, 0}} {}
     __device__ __forceinline__ Stream(const Identity& i, uint32_t bounce)
         : id(i), dimension(CameraDimensions + bounce * DimensionsPerBounce), group(~0u), w{{0, 0, 0, 0}} {}
+    // The group whose block the next draw needs, and that block computed elsewhere (block2) handed in.
+    __device__ __forceinline__ uint32_t nextGroup() const { return dimension >> 2; }
+    __device__ __forceinline__ void prime(const Words& b) {
+        group = dimension >> 2;
+        w = b;
+    }
     __device__ __forceinline__ float next() {
         if ((dimension >> 2) != group) {
             group = dimension >> 2;
