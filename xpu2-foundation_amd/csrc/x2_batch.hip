@@ -220,7 +220,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     };
     hipError_t e = hipSuccess;
     const size_t sizes[7] = {nodeBytes, triBytes, std::max<size_t>(1, s->material_count) * sizeof(x2_material),
-                             std::max<size_t>(1, s->light_count) * sizeof(x2_light), bvh.triCull.size(),
+                             lightBufferBytes(s->light_count), bvh.triCull.size(),
                              size_t(j.pixels) * 4 * sizeof(double),
                              size_t(j.pixels) * sizeof(uint32_t)};
     for (int i = 0; i < 7 && e == hipSuccess; ++i)
@@ -235,8 +235,10 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     }
     if (e == hipSuccess)
         e = hipMemcpy(j.buffers[2], s->materials, s->material_count * sizeof(x2_material), hipMemcpyHostToDevice);
-    if (e == hipSuccess && s->light_count)
-        e = hipMemcpy(j.buffers[3], s->lights, s->light_count * sizeof(x2_light), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const std::vector<char> lb = lightBuffer(s->lights, s->light_count);
+        e = hipMemcpy(j.buffers[3], lb.data(), lb.size(), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess && !bvh.triCull.empty())
         e = hipMemcpy(j.buffers[4], bvh.triCull.data(), bvh.triCull.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(e, "x2batch_add_job upload");
@@ -247,6 +249,7 @@ int x2batch_add_job(x2batch* b, const x2_scene* s, const x2_config* c, uint32_t*
     p.triVerts = reinterpret_cast<const float*>(static_cast<const char*>(j.buffers[1]) + triVertsOffset(bvh.tris.size()));
     p.materials = static_cast<const x2_material*>(j.buffers[2]);
     p.lights = static_cast<const x2_light*>(j.buffers[3]);
+    p.lightCorners = reinterpret_cast<const float*>(static_cast<const char*>(j.buffers[3]) + s->light_count * sizeof(x2_light));
     p.triCull = static_cast<const uint8_t*>(j.buffers[4]);
     p.sums = static_cast<double*>(j.buffers[5]);
     p.counts = static_cast<uint32_t*>(j.buffers[6]);

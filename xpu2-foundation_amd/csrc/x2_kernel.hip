@@ -532,16 +532,16 @@ __device__ bool lightIntersect(const x2_light& l, V3 p, V3 wi, float maxDistance
     h.distance = renderDistance;
     return true;
 }
-__device__ bool canIlluminate(const x2_light& l, V3 p, bool cull, V3 n) {
+// corners: the light's four corners, rows(local_to_render, (+-half_width, +-half_height, 0)) + position, made on the
+// host with the same float operations (lightCorners, beside the lights).
+__device__ bool canIlluminate(const x2_light& l, const float* corners, V3 p, bool cull, V3 n) {
     const float threshold = Epsilon - 0.0f;
     if (l.type == X2_LIGHT_ENV) return true;
     if (l.sidedness != X2_SIDE_TWO && planeDistance(l, p) < threshold) return false;
     if (!cull) return true;
-    const float hx = l.half_width, hy = l.half_height;
-    const float cx[4] = {hx, hx, -hx, -hx}, cy[4] = {hy, -hy, hy, -hy};
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (dot(n, (rows(l.local_to_render, V3{cx[i], cy[i], 0.0f}) + ld3(l.position)) - p) > threshold) return true;
+        if (dot(n, ld3(corners + 3 * i) - p) > threshold) return true;
     return false;
 }
 __device__ __forceinline__ float powerHeuristic(float f, float g) { return (f * f) / (f * f + g * g); }
@@ -2465,7 +2465,7 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
     const uint64_t lightBranch = childBranch(pv.branch, 0xfffe, 0);
     for (uint32_t i = 0; i < c.light_count; ++i) {
         const x2_light& L = P.lights[i];
-        if (!canIlluminate(L, Pt, cull, cullN)) continue;
+        if (!canIlluminate(L, P.lightCorners + 12 * i, Pt, cull, cullN)) continue;
         const uint64_t lightAddress = childBranch(lightBranch, i, 0);
         Identity lid = root;
         lid.branch = lightAddress;
@@ -3039,9 +3039,11 @@ __device__ __forceinline__ unsigned long long waveSum(uint32_t v) {
 // pixel's own base, so a pixel's indices stay contiguous (async_gpu.cu:440-446).
 __device__ __forceinline__ bool decodeItem(const RenderParams& P, unsigned long long item, uint32_t& pixel,
                                            uint32_t& index) {
-    const unsigned long long chunkIndex = item / P.itemsPerChunk;
-    const uint32_t offset = uint32_t(item - chunkIndex * P.itemsPerChunk);
-    const x2_chunk chunk = P.ring[chunkIndex % P.ringCapacity];
+    // a power-of-two chunk size and ring (the usual launch shape) by shift and mask, not a 64-bit division
+    const unsigned long long ipc = P.itemsPerChunk, cap = P.ringCapacity;
+    const unsigned long long chunkIndex = ipc && (ipc & (ipc - 1)) == 0 ? item >> __builtin_ctzll(ipc) : item / ipc;
+    const uint32_t offset = uint32_t(item - chunkIndex * ipc);
+    const x2_chunk chunk = P.ring[cap && (cap & (cap - 1)) == 0 ? chunkIndex & (cap - 1) : chunkIndex % cap];
     const uint32_t s = offset / TilePixels, position = offset % TilePixels;
     if (s >= chunk.count) return false;
     pixel = tilePixel(chunk.tile, position, P.cfg);

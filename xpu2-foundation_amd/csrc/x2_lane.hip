@@ -178,6 +178,32 @@ hipError_t staticScratch(int device, unsigned char** out) {
 
 namespace x2 {  // for the other host files (x2_batch.hip)
 void sobolTable(uint32_t out[17 * 32]) { sobolDirections(out); }
+
+// A rectangle light's corners (+-half width, +-half height) through local_to_render, plus its position, in the order
+// and with the float operations of the device's canIlluminate (rows(): (m0 x + m3 y) + m6 z, z = 0): made once here
+// instead of at every vertex, the same floats. Zeros for an environment light.
+static void lightCorners(const x2_light& l, float out[12]) {
+    for (int i = 0; i < 12; ++i) out[i] = 0.0f;
+    if (l.type != X2_LIGHT_RECT) return;
+    const float hx = l.half_width, hy = l.half_height;
+    const float cx[4] = {hx, hx, -hx, -hx}, cy[4] = {hy, -hy, hy, -hy};
+    const float* m = l.local_to_render;
+    for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const float v = ((m[a] * cx[i]) + (m[3 + a] * cy[i])) + (m[6 + a] * 0.0f);
+            out[3 * i + a] = v + l.position[a];
+        }
+}
+size_t lightBufferBytes(uint32_t count) {
+    return std::max<size_t>(1, count) * (sizeof(x2_light) + 12 * sizeof(float));
+}
+std::vector<char> lightBuffer(const x2_light* lights, uint32_t count) {
+    std::vector<char> b(lightBufferBytes(count), 0);
+    if (count) std::memcpy(b.data(), lights, count * sizeof(x2_light));
+    for (uint32_t i = 0; i < count; ++i)
+        lightCorners(lights[i], reinterpret_cast<float*>(b.data() + count * sizeof(x2_light)) + 12 * i);
+    return b;
+}
 int reportFailure(int code, const std::string& what) { return fail(code, what); }
 }  // namespace x2
 
@@ -515,7 +541,7 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
     X2_HIP(hipMalloc(&d->nodes, nb));
     X2_HIP(hipMalloc(&d->tris, tb));
     X2_HIP(hipMalloc(&d->materials, std::max<size_t>(1, s->material_count) * sizeof(x2_material)));
-    X2_HIP(hipMalloc(&d->lights, std::max<size_t>(1, s->light_count) * sizeof(x2_light)));
+    X2_HIP(hipMalloc(&d->lights, lightBufferBytes(s->light_count)));
     X2_HIP(hipMemcpy(d->nodes, bvh.deviceNodes(), nb, hipMemcpyHostToDevice));
     if (!bvh.tris.empty()) X2_HIP(hipMemcpy(d->tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice));
     if (!bvh.tris.empty()) {
@@ -524,7 +550,10 @@ int x2dev_prepare(x2dev* d, const x2_scene* s) {
                          v.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     X2_HIP(hipMemcpy(d->materials, s->materials, s->material_count * sizeof(x2_material), hipMemcpyHostToDevice));
-    if (s->light_count) X2_HIP(hipMemcpy(d->lights, s->lights, s->light_count * sizeof(x2_light), hipMemcpyHostToDevice));
+    {
+        const std::vector<char> lb = lightBuffer(s->lights, s->light_count);
+        X2_HIP(hipMemcpy(d->lights, lb.data(), lb.size(), hipMemcpyHostToDevice));
+    }
     if (!bvh.triUV.empty()) {
         X2_HIP(hipMalloc(&d->triUV, bvh.triUV.size() * sizeof(float)));
         X2_HIP(hipMemcpy(d->triUV, bvh.triUV.data(), bvh.triUV.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -789,6 +818,7 @@ RenderParams renderParams(const x2dev* d, uint32_t slot) {
     p.textures = d->textures;
     p.materials = d->materials;
     p.lights = d->lights;
+    p.lightCorners = reinterpret_cast<const float*>(reinterpret_cast<const char*>(d->lights) + d->lightCount * sizeof(x2_light));
     p.dImage = d->dImage;
     p.gradTextures = d->gradTextures;
     p.gradRoughness = d->gradRoughness;

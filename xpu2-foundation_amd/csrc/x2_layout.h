@@ -158,6 +158,7 @@ struct RenderParams {
     const TexDesc* textures;
     const x2_material* materials;
     const x2_light* lights;
+    const float* lightCorners;  // 12 floats a light, beside the lights: a rectangle's corners (lightCorners, host)
     // the adjoint's image adjoint (dLoss / d mean rgb per pixel) and gradient outputs (adjoint launches only)
     const float* dImage;
     float* gradTextures;
