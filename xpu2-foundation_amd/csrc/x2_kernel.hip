@@ -545,6 +545,13 @@ __device__ bool canIlluminate(const x2_light& l, const float* corners, V3 p, boo
     return false;
 }
 __device__ __forceinline__ float powerHeuristic(float f, float g) { return (f * f) / (f * f + g * g); }
+// Light i of a loop over the lights, at which every active lane of the wave is: one index for the wave, and the
+// lights (never written by a kernel) read as constant memory, so that its fields are scalar loads. As a plain global
+// read every field was a vector load, each waited on in turn.
+__device__ __forceinline__ const x2_light& waveLight(const x2_light* lights, uint32_t i) {
+    const auto c = (const __attribute__((address_space(4))) x2_light*)lights;
+    return *(const x2_light*)(c + __builtin_amdgcn_readfirstlane(i));
+}
 
 // ---------------------------------------------------------------------------------------------------------------
 // Lobes: Lambert and GGX (shade_lobes.h:481-556; shade_lobes_eval.h:11, 93-146, 609-668; shade_math.h:135-262).
@@ -1908,7 +1915,7 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
         if (scatterCount > 0) {
             const float invN = 1.0f / float(samplesPerLight * scatterCount);
             for (uint32_t li = 0; li < c.light_count; ++li) {
-                const x2_light& L = P.lights[li];
+                const x2_light& L = waveLight(P.lights, li);
                 const bool equiAngular = L.type == X2_LIGHT_RECT;
                 V3 direct{0, 0, 0};
                 for (uint32_t ls = 0; ls < samplesPerLight; ++ls)
@@ -2049,7 +2056,7 @@ __device__ float volumeRadiance(const RenderParams& P, const Identity& root, V3&
                         usl[0] = ms.next();
                         usl[1] = ms.next();
                         usl[2] = ms.next();
-                        const V3 Ls = volumeSourceTerm(P, ms, P.lights[li], scatterPoint, rayDir, g, usl, ai, k, st);
+                        const V3 Ls = volumeSourceTerm(P, ms, waveLight(P.lights, li), scatterPoint, rayDir, g, usl, ai, k, st);
                         radiance = radiance + throughput * Ls * (invN * bi);
                     }
                     // the glowing medium lighting this scatter (:1127-1143)
@@ -2464,7 +2471,7 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
     // Light samples (:2615-2785).
     const uint64_t lightBranch = childBranch(pv.branch, 0xfffe, 0);
     for (uint32_t i = 0; i < c.light_count; ++i) {
-        const x2_light& L = P.lights[i];
+        const x2_light& L = waveLight(P.lights, i);
         if (!canIlluminate(L, P.lightCorners + 12 * i, Pt, cull, cullN)) continue;
         const uint64_t lightAddress = childBranch(lightBranch, i, 0);
         Identity lid = root;
@@ -2572,7 +2579,8 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
             uint32_t hits = 0;
             for (uint32_t j = 0; j < c.light_count; ++j) {
                 LightHit lh;
-                if ((lobeMask & P.lights[j].visibility_mask) && lightIntersect(P.lights[j], Pt, wi, MaxFloat, lh)) ++hits;
+                const x2_light& Lj = waveLight(P.lights, j);
+                if ((lobeMask & Lj.visibility_mask) && lightIntersect(Lj, Pt, wi, MaxFloat, lh)) ++hits;
             }
             V3 direct{0, 0, 0};
             float lightDistance = 0;
@@ -2582,11 +2590,12 @@ __device__ __forceinline__ bool shadeHit(const RenderParams& P, uint32_t pixel, 
                 const uint32_t chosen = uint32_t(choice * float(hits)) < hits ? uint32_t(choice * float(hits)) : hits - 1;
                 for (uint32_t j = 0, seen = 0; j < c.light_count; ++j) {
                     LightHit lh;
-                    if (!(lobeMask & P.lights[j].visibility_mask) || !lightIntersect(P.lights[j], Pt, wi, MaxFloat, lh))
+                    const x2_light& Lj = waveLight(P.lights, j);
+                    if (!(lobeMask & Lj.visibility_mask) || !lightIntersect(Lj, Pt, wi, MaxFloat, lh))
                         continue;
                     if (seen++ != chosen) continue;
                     float pdfL;
-                    const V3 Li = lightEval(P.lights[j], wi, lh, pdfL) * float(hits);
+                    const V3 Li = lightEval(Lj, wi, lh, pdfL) * float(hits);
                     if (!(pdfL == 0.0f || isExactlyZero(Li)) && fabsf(pdfL) < 3.4e38f && finite3(Li)) {
                         direct = Li * pt * powerHeuristic(float(nk) * pdf, float(nl) * pdfL);
                         lightDistance = lh.distance;
