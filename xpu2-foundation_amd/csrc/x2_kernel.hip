@@ -1351,16 +1351,20 @@ __device__ __forceinline__ uint32_t surfaceFlags(V3 Ng, V3 wo, V3 wi) {
     return dot(Ng, wo) * dot(Ng, wi) > 0.0f ? (AllLobeTypes & ~Transmission) : (AllLobeTypes & ~Reflection);
 }
 
-__device__ bool traceHit(const RenderParams& P, const Ray& ray, Hit& h, Counters& k, TravStack& st) {
-    ++k.traces;
-    float t;
-    const uint32_t tri = closestHit(P, ray.origin, ray.direction, ray.nearT, ray.farT, t, k, st);
+// The hit record of a closest-hit answer (tri: the triangle record, ~0u: none). False: an invalid value.
+__device__ __forceinline__ bool hitRecord(const RenderParams& P, const Ray& ray, uint32_t tri, Hit& h) {
     h.geom = ~0u;
     if (tri == ~0u) return true;
     const TriRecord& r = P.tris[tri];
     h.geom = r.geom;
     h.tri = tri;
     return geometryHit(h, ray, V3{r.ax, r.ay, r.az}, V3{r.bx, r.by, r.bz}, V3{r.cx, r.cy, r.cz});
+}
+__device__ bool traceHit(const RenderParams& P, const Ray& ray, Hit& h, Counters& k, TravStack& st) {
+    ++k.traces;
+    float t;
+    const uint32_t tri = closestHit(P, ray.origin, ray.direction, ray.nearT, ray.farT, t, k, st);
+    return hitRecord(P, ray, tri, h);
 }
 __device__ __forceinline__ bool visible(const RenderParams& P, V3 o, V3 d, float nearT, float farT, Counters& k,
                                         TravStack& st) {
@@ -2845,58 +2849,187 @@ struct PoolQueue {
         return false;
     }
 };
-// Traces the wave's queued shadow rays and adds each lane's unoccluded ones, in its order, to its radiance. Every lane
-// of the wave calls it together.
-__device__ __forceinline__ void poolFlush(const RenderParams& P, PoolQueue& q, V3& radiance, Counters& k,
-                                          TravStack& st) {
+// One trace phase of the wave: each lane's own next ray (closest hit) and the rays the wave's last vertices queued
+// in the pool (any hit), in one loop. A lane starts on its own ray, or on the pool's next ray when it has none, and
+// when its ray ends takes the pool's next one, until the pool is empty: the wave waits once for its longest ray
+// instead of once for the longest closest-hit ray and again for the longest shadow ray, and a lane whose ray ended
+// early traces another instead of idling. Each ray's answer is the one closestHit or anyHit gives it (the least
+// key, whatever the order; occluded or not): a pool ray is walked as a closest-hit ray that stops at its first hit,
+// with its far distance as the bound. The pool's next ray is counted in a register for the wave: the lanes that
+// want a ray take consecutive ones by their rank among them. Returns the lane's own ray's triangle record (~0u:
+// none). Every lane of the wave calls it together.
+__device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, const Ray& ray, Counters& k,
+                              TravStack& st) {
+    ShadowPool& p = *q.pool;
     const uint32_t lane = q.lane, n = q.n;
     const unsigned long long lower = (1ull << lane) - 1ull;
-    uint32_t first = 0, total = 0;  // the lane's first place in the compacted list, the wave's rays
+    uint32_t first = 0, total = 0;  // the lane's first place in the compacted list, the wave's pool rays
 #pragma unroll
     for (int b = 0; b < 3; ++b) {  // an exclusive prefix sum of n (< 8) over the lanes, a bit at a time
         const unsigned long long m = __ballot((n >> b) & 1u);
         first += uint32_t(__popcll(m & lower)) << b;
         total += uint32_t(__popcll(m)) << b;
     }
-    if (total == 0) return;
-    ShadowPool& p = *q.pool;
     for (uint32_t j = 0; j < n; ++j) p.list[first + j] = uint16_t(j * 64 + lane);
     waveSync();
-    for (uint32_t base = 0; base < total; base += 64) {
-        const uint32_t j = base + lane;
-        if (j < total) {
-            const uint32_t e = p.list[j], owner = e & 63u, side = p.flag[e] ? 3u : 0u;
-            const V3 o{p.org[(side + 0) * 64 + owner], p.org[(side + 1) * 64 + owner], p.org[(side + 2) * 64 + owner]};
-            const V3 d{p.dx[e], p.dy[e], p.dz[e]};
-            p.flag[e] = visible(P, o, d, 0.0f, p.far[e], k, st) ? 2u : 0u;
+    uint32_t ownBest = ~0u, next = 0;
+    bool active = false, pooled = false, occluded = false;
+    uint32_t e = 0;  // the current pool ray's slot
+    WoopRay w;
+    BoxRay br;
+    V3 dir{0, 0, 0};
+    float lo = 0.0f, farT = 0.0f, bestT = 0.0f;
+    uint32_t best = ~0u, ref = Sentinel, leaf = 0;
+    int sp = 0;
+    auto start = [&](V3 o, V3 d, float nearT, float f) {
+        w = woopSetup(o, d, nearT, f);
+        br = boxRay(o, d);
+        dir = d;
+        lo = nearT > 0 ? nearT / Slack : nearT * Slack;
+        farT = f;
+        best = ~0u;
+        bestT = f;
+        sp = 0;
+        ref = 0;
+        leaf = 0;
+        occluded = false;
+        active = true;
+    };
+    auto take = [&]() {  // the lanes without a ray take the pool's next ones, in lane order
+        const unsigned long long want = __ballot(!active);
+        if (next >= total) return;
+        if (!active) {
+            const uint32_t j = next + uint32_t(__popcll(want & lower));
+            if (j < total) {
+                e = p.list[j];
+                const uint32_t owner = e & 63u, side = p.flag[e] ? 3u : 0u;
+                pooled = true;
+                ++k.shadows;
+                start(V3{p.org[(side + 0) * 64 + owner], p.org[(side + 1) * 64 + owner],
+                         p.org[(side + 2) * 64 + owner]},
+                      V3{p.dx[e], p.dy[e], p.dz[e]}, 0.0f, p.far[e]);
+            }
         }
+        next += uint32_t(__popcll(want));
+    };
+    auto pop = [&]() -> uint32_t {  // the nearest stacked entry still worth visiting, or Sentinel
+        while (sp > 0) {
+            --sp;
+            const float bound = best == ~0u ? farT : bestT;
+            const uint2 s = st.get(sp);
+            if (__uint_as_float(s.y) <= (bound > 0 ? bound * Slack : bound / Slack)) return s.x;
+        }
+        return Sentinel;
+    };
+    auto visitNode = [&](uint32_t node) -> uint32_t {  // stacks the far children, returns the nearest (or a pop)
+        ++k.nodes;
+        const float hi = best == ~0u ? (farT > 0 ? farT * Slack : farT / Slack) : (bestT > 0 ? bestT * Slack : bestT / Slack);
+        unsigned long long key[8];
+        nodeChildren(P.nodes[node], br, lo, hi, key);
+        unsigned long long m = key[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i) m = key[i] < m ? key[i] : m;
+#pragma unroll
+        for (int i = 7; i >= 0; --i)
+            if (key[i] != ~0ull && key[i] != m) {
+                st.put(sp, make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32)));
+                ++sp;
+            }
+        return m != ~0ull ? uint32_t(m) : pop();
+    };
+    auto visitLeaf = [&](uint32_t leafRef) {  // a pool ray stops at its first hit
+        const uint32_t firstTri = (leafRef & 0x7fffffffu) >> 3, count = (leafRef & 7u) + 1;
+        for (uint32_t i = 0; i < count; ++i) {
+            ++k.triangles;
+            float t;
+            const float* v = P.triVerts + size_t(firstTri + i) * 9;
+            const V3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
+            if (!woopHit(w, a, b, c, t) || culled(P, firstTri + i, a, b, c, dir)) continue;
+            if (pooled) {
+                occluded = true;
+                return;
+            }
+            if (best == ~0u || t < bestT) {
+                best = firstTri + i;
+                bestT = t;
+            } else if (t == bestT) {  // a tie in distance: the least (geom, prim) wins (closestHit's 64-bit select)
+                const TriRecord &r = P.tris[firstTri + i], &qr = P.tris[best];
+                const unsigned long long rk = (unsigned long long)r.geom << 32 | r.prim,
+                                         qk = (unsigned long long)qr.geom << 32 | qr.prim;
+                best = rk < qk ? firstTri + i : best;
+            }
+        }
+    };
+    if (own) {
+        ++k.traces;
+        start(ray.origin, ray.direction, ray.nearT, ray.farT);
+    }
+    take();
+    for (;;) {
+        if (active) {
+            while (ref < Sentinel) {  // an inner node
+                ref = visitNode(ref);
+                if ((ref & LeafBit) && leaf == 0) {
+                    leaf = ref;
+                    ref = pop();
+                }
+                if (!__any(leaf == 0)) break;
+            }
+            while (leaf || (ref & LeafBit)) {
+                if (!leaf) {
+                    leaf = ref;
+                    ref = pop();
+                }
+                visitLeaf(leaf);
+                leaf = 0;
+                if (occluded) ref = Sentinel;
+            }
+            if (ref == Sentinel) {  // the ray is done: its answer
+                if (pooled) p.flag[e] = occluded ? 0u : 2u;
+                else ownBest = best;
+                active = false;
+            }
+        }
+        take();
+        if (!__ballot(active)) break;
     }
     waveSync();
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t e = j * 64 + lane;
-        if (p.flag[e] == 2u) radiance = radiance + V3{p.vr[e], p.vg[e], p.vb[e]};
-    }
-    q.n = 0;
+    return ownBest;
 }
-// runSamples with the shadow rays pooled (the forward surface estimator): every lane stays in the loop until the
-// wave's samples are all done, so that each iteration's pool is traced by the whole wave. Only a kernel whose one
-// argument is the RenderParams calls it: the loop reads them at the start of the kernel's argument segment.
+
+// The forward surface estimator with the shadow rays pooled, software-pipelined: an iteration traces each lane's
+// next ray together with the shadow rays the lanes' last vertices queued (traceWave), adds those rays' light to the
+// samples that queued them, each lane's in the order its vertex made them, then shades the new hits, which queue
+// the next iteration's shadow rays. A sample whose last vertex queued shadow rays is held until they are traced and
+// its light added; its lane meanwhile starts its next sample. Every addition to a sample's radiance comes in the
+// order the one-vertex-at-a-time loop makes it, so the film is bit for bit the same. Every lane stays in the loop
+// until the wave's samples are all done. Only a kernel whose one argument is the RenderParams calls it: the loop
+// reads them at the start of the kernel's argument segment.
 template <class Feed>
 __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k, TravStack& st, PoolQueue& q) {
     PathState<false, false> S;
     S.top = 0;
-    uint32_t pixel = 0, index = 0;
-    bool live = false, done = false;
+    uint32_t pixel = 0, index = 0, heldPixel = 0, heldIndex = 0;
+    V3 heldRadiance{0, 0, 0};
+    float heldAlpha = 0.0f;
+    bool live = false, done = false, held = false;
     for (;;) {
-        const RenderParams& PL = argParams();  // 200 scalar registers spilled with P held across the loop, 55 so
-        const bool idle = S.top == 0;
-        if (idle && live) {
-            feed.done(pixel, index, S.radiance, S.alpha, 0u);
+        const RenderParams& PL = argParams();
+        if (live && S.top == 0) {  // the sample's last vertex is shaded: it ends, or waits for its shadow rays
+            if (q.n) {
+                held = true;
+                heldPixel = pixel;
+                heldIndex = index;
+                heldRadiance = S.radiance;
+                heldAlpha = S.alpha;
+            } else {
+                feed.done(pixel, index, S.radiance, S.alpha, 0u);
+            }
             live = false;
         }
-        const unsigned long long idleLanes = __ballot(idle && !done), openLanes = __ballot(!done);
-        if (!openLanes) break;  // every lane's feed is empty and its last sample done
-        const bool claim = idle && !done && (__popcll(idleLanes) >= Feed::Regen || idleLanes == openLanes);
+        const unsigned long long idleLanes = __ballot(!live && !done), openLanes = __ballot(!done);
+        if (!__ballot(!done || held)) break;  // every lane's feed is empty and its last sample done
+        const bool claim = !live && !done && (__popcll(idleLanes) >= Feed::Regen || idleLanes == openLanes);
         if (claim) {
             if (!feed.next(pixel, index)) done = true;
             else if (!startSample<false, false>(PL, pixel, index, S)) {
@@ -2906,8 +3039,34 @@ __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k,
                 live = true;
             }
         }
-        if (S.top) {
-            if (!shadeVertexWith<false, false>(PL, pixel, index, S, q, k, st, nullptr) || q.overflow) {
+        // the lane's next ray: its sample's last pending one (popped once traced, so that only the ray is held
+        // through the trace)
+        auto rayOf = [&](const Pending& r) {
+            const bool fromCamera = (r.depths & 0xffu) == 0;
+            return Ray{r.origin, r.dir, fromCamera ? S.cameraNear : 0.0f, fromCamera ? S.cameraFar : MaxFloat};
+        };
+        const uint32_t tri = traceWave(PL, q, live, live ? rayOf(S.stack[S.top - 1]) : Ray{}, k, st);
+        {  // the pool rays' light, into the sample that queued them, in the order its vertex queued them
+            V3 r = held ? heldRadiance : S.radiance;
+            const ShadowPool& pool = *q.pool;
+            for (uint32_t j = 0; j < q.n; ++j) {
+                const uint32_t e = j * 64 + q.lane;
+                if (pool.flag[e] == 2u) r = r + V3{pool.vr[e], pool.vg[e], pool.vb[e]};
+            }
+            q.n = 0;
+            if (held) {
+                feed.done(heldPixel, heldIndex, r, heldAlpha, 0u);
+                held = false;
+            } else {
+                S.radiance = r;
+            }
+        }
+        if (live) {
+            const Pending pv = S.stack[--S.top];
+            const Ray ray = rayOf(pv);
+            Hit hit;
+            if (!hitRecord(PL, ray, tri, hit) || !shadeHit<false, false>(PL, pixel, index, S, q, pv, ray, hit, k, st, nullptr) ||
+                q.overflow) {
                 feed.failed();  // an invalid value, or more shadow rays than the pool holds (the host prevents it)
                 S.top = 0;
                 live = false;
@@ -2915,7 +3074,6 @@ __device__ void runSamplesPooled(const RenderParams& P, Feed& feed, Counters& k,
                 q.overflow = false;
             }
         }
-        poolFlush(PL, q, S.radiance, k, st);
     }
 }
 
