@@ -14,6 +14,11 @@ extern __constant__ uint32_t kSobolDirections[17 * 32];
 
 struct Words { uint32_t v[4]; };
 
+// a ^ b ^ c in one instruction (gfx950's three-input bitwise op, truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Philox4x32-10 (sample_stream.h:31-38): 10 rounds of two 32x32->64 multiplies and a Weyl key schedule. Each
 // product is one 64-bit multiply (v_mad_u64_u32), its halves the round's high and low words.
 __device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t ka, uint32_t kb) {
@@ -21,9 +26,9 @@ __device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, u
     for (int round = 0; round < 10; ++round) {
         const uint64_t pa = uint64_t(c0) * 0xd2511f53u, pb = uint64_t(c2) * 0xcd9e8d57u;
         const uint32_t alo = uint32_t(pa), ahi = uint32_t(pa >> 32), blo = uint32_t(pb), bhi = uint32_t(pb >> 32);
-        c0 = bhi ^ c1 ^ ka;
+        c0 = xor3(bhi, c1, ka);
         c1 = blo;
-        c2 = ahi ^ c3 ^ kb;
+        c2 = xor3(ahi, c3, kb);
         c3 = alo;
         ka += 0x9e3779b9u;
         kb += 0xbb67ae85u;
@@ -55,13 +60,13 @@ __device__ __forceinline__ void block2(const Identity& a, uint32_t ga, const Ide
     for (int round = 0; round < 10; ++round) {
         const uint64_t pa = uint64_t(a0) * 0xd2511f53u, pb = uint64_t(a2) * 0xcd9e8d57u;
         const uint64_t qa = uint64_t(b0) * 0xd2511f53u, qb = uint64_t(b2) * 0xcd9e8d57u;
-        a0 = uint32_t(pb >> 32) ^ a1 ^ aka;
+        a0 = xor3(uint32_t(pb >> 32), a1, aka);
         a1 = uint32_t(pb);
-        a2 = uint32_t(pa >> 32) ^ a3 ^ akb;
+        a2 = xor3(uint32_t(pa >> 32), a3, akb);
         a3 = uint32_t(pa);
-        b0 = uint32_t(qb >> 32) ^ b1 ^ bka;
+        b0 = xor3(uint32_t(qb >> 32), b1, bka);
         b1 = uint32_t(qb);
-        b2 = uint32_t(qa >> 32) ^ b3 ^ bkb;
+        b2 = xor3(uint32_t(qa >> 32), b3, bkb);
         b3 = uint32_t(qa);
         aka += 0x9e3779b9u;
         akb += 0xbb67ae85u;
