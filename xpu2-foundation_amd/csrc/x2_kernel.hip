@@ -277,12 +277,15 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
     uint32_t best = ~0u;
     float bestT = farT;
     int sp = 0;
+    bool cached = false;  // the stack's top entry held in registers (traceWave's form)
+    uint2 top = make_uint2(0u, 0u);
     // The nearest stacked entry still worth visiting, or Sentinel.
     auto pop = [&]() -> uint32_t {
         while (sp > 0) {
             --sp;
             const float bound = best == ~0u ? farT : bestT;
-            const uint2 e = st.get(sp);
+            const uint2 e = cached ? top : st.get(sp);
+            cached = false;
             if (__uint_as_float(e.y) <= (bound > 0 ? bound * Slack : bound / Slack)) return e.x;
         }
         return Sentinel;
@@ -300,7 +303,9 @@ __device__ uint32_t closestHit(const RenderParams& P, V3 org, V3 dir, float near
 #pragma unroll
         for (int i = 7; i >= 0; --i)
             if (key[i] != ~0ull && key[i] != m) {
-                st.put(sp, make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32)));
+                if (cached) st.put(sp - 1, top);
+                top = make_uint2(uint32_t(key[i]), uint32_t(key[i] >> 32));
+                cached = true;
                 ++sp;
             }
         return m != ~0ull ? uint32_t(m) : pop();
@@ -358,15 +363,28 @@ __device__ bool anyHit(const RenderParams& P, V3 org, V3 dir, float nearT, float
     const BoxRay br = boxRay(org, dir);
     const float lo = nearT > 0 ? nearT / Slack : nearT * Slack, hi = farT > 0 ? farT * Slack : farT / Slack;
     int sp = 0;
-    auto pop = [&]() -> uint32_t { return sp > 0 ? st.get1(--sp) : Sentinel; };
-    // Unsorted: sorting the keys measured slower here, and so did taking the last child hit without its push and pop.
+    bool cached = false;  // the stack's top entry held in a register (traceWave's form)
+    uint32_t top = 0;
+    auto pop = [&]() -> uint32_t {
+        if (sp == 0) return Sentinel;
+        --sp;
+        const uint32_t e = cached ? top : st.get1(sp);
+        cached = false;
+        return e;
+    };
+    // Unsorted: sorting the keys measured slower here.
     auto visitNode = [&](uint32_t node) -> uint32_t {
         ++k.nodes;
         unsigned long long key[8];
         nodeChildren(P.nodes[node], br, lo, hi, key);
 #pragma unroll
         for (int i = 0; i < 8; ++i)
-            if (key[i] != ~0ull) st.put1(sp++, uint32_t(key[i]));
+            if (key[i] != ~0ull) {
+                if (cached) st.put1(sp - 1, top);
+                top = uint32_t(key[i]);
+                cached = true;
+                ++sp;
+            }
         return pop();
     };
     auto visitLeaf = [&](uint32_t leafRef) -> bool {
