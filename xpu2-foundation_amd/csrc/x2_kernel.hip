@@ -2972,10 +2972,30 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
     };
     auto visitLeaf = [&](uint32_t leafRef) {  // a pool ray stops at its first hit
         const uint32_t firstTri = (leafRef & 0x7fffffffu) >> 3, count = (leafRef & 7u) + 1;
+        // A leaf's first three triangles (the builder's leaves hold at most three unless the SAH prefers more) read
+        // before any is tested, so their loads wait once, not one after another.
+        float pre[3][9];
+        const float* v0 = P.triVerts + size_t(firstTri) * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) pre[0][q] = v0[q];
+        if (count > 1)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) pre[1][q] = v0[9 + q];
+        if (count > 2)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) pre[2][q] = v0[18 + q];
         for (uint32_t i = 0; i < count; ++i) {
             ++k.triangles;
             float t;
-            const float* v = P.triVerts + size_t(firstTri + i) * 9;
+            float v[9];
+            if (i < 3) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] = i == 0 ? pre[0][q] : i == 1 ? pre[1][q] : pre[2][q];
+            } else {
+                const float* g = P.triVerts + size_t(firstTri + i) * 9;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] = g[q];
+            }
             const V3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
             if (!woopHit(w, a, b, c, t) || culled(P, firstTri + i, a, b, c, dir)) continue;
             if (pooled) {
