@@ -2896,7 +2896,8 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
     WoopRay w;
     BoxRay br;
     V3 dir{0, 0, 0};
-    float lo = 0.0f, farT = 0.0f, bestT = 0.0f;
+    float lo = 0.0f, farT = 0.0f, bestT = 0.0f, hi = 0.0f;  // hi: the bound on entry distances, bestT with slack
+    auto slackAbove = [](float b) { return b > 0 ? b * Slack : b / Slack; };
     uint32_t best = ~0u, ref = Sentinel, leaf = 0;
     int sp = 0;
     // The stack's top entry held in registers (topRef, topDist) instead of st[sp - 1]: the pop that most often
@@ -2912,6 +2913,7 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
         farT = f;
         best = ~0u;
         bestT = f;
+        hi = slackAbove(f);
         sp = 0;
         cached = false;
         ref = 0;
@@ -2939,7 +2941,6 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
     auto pop = [&]() -> uint32_t {  // the nearest stacked entry still worth visiting, or Sentinel
         while (sp > 0) {
             --sp;
-            const float bound = best == ~0u ? farT : bestT;
             uint2 s;
             if (cached) {
                 s = make_uint2(topRef, topDist);
@@ -2947,13 +2948,12 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
             } else {
                 s = st.get(sp);
             }
-            if (__uint_as_float(s.y) <= (bound > 0 ? bound * Slack : bound / Slack)) return s.x;
+            if (__uint_as_float(s.y) <= hi) return s.x;
         }
         return Sentinel;
     };
     auto visitNode = [&](uint32_t node) -> uint32_t {  // stacks the far children, returns the nearest (or a pop)
         ++k.nodes;
-        const float hi = best == ~0u ? (farT > 0 ? farT * Slack : farT / Slack) : (bestT > 0 ? bestT * Slack : bestT / Slack);
         unsigned long long key[8];
         nodeChildren(P.nodes[node], br, lo, hi, key);
         unsigned long long m = key[0];
@@ -3005,6 +3005,7 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
             if (best == ~0u || t < bestT) {
                 best = firstTri + i;
                 bestT = t;
+                hi = slackAbove(t);
             } else if (t == bestT) {  // a tie in distance: the least (geom, prim) wins (closestHit's 64-bit select)
                 const TriRecord &r = P.tris[firstTri + i], &qr = P.tris[best];
                 const unsigned long long rk = (unsigned long long)r.geom << 32 | r.prim,
