@@ -259,6 +259,10 @@ __device__ __forceinline__ bool keyLess(float t, uint32_t g, uint32_t p, float b
 // together. Node and leaf work stay apart, so a wave is not split between the two on every step. The closest hit
 // is the least key whatever the order, so the result is the same as a depth-first walk's.
 constexpr uint32_t Sentinel = 0x7fffffffu;  // an empty stack's pop: neither a node index nor a leaf
+// traceWave's wave leaves its inner nodes for the postponed leaves once at most this many of its searching lanes
+// still hold none (rather than none): the wave then does not wait on a few lanes' long descents, and the leaves'
+// closer hits cull sooner (the 1080p icospheres frame: 820 Mpaths/s at 4 or 8, 814 at 16, 803 at 1, 780 at 0).
+constexpr int LeafWait = 8;
 
 
 // Back-face culling of single-sided meshes (backFaceCullingFilter, async_cpu.h:122-133; the GPU lane's
@@ -3027,7 +3031,7 @@ __device__ uint32_t traceWave(const RenderParams& P, PoolQueue& q, bool own, con
                     leaf = ref;
                     ref = pop();
                 }
-                if (!__any(leaf == 0)) break;
+                if (__popcll(__ballot(leaf == 0)) <= LeafWait) break;
             }
             while (leaf || (ref & LeafBit)) {
                 if (!leaf) {
