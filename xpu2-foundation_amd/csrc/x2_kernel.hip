@@ -2823,12 +2823,11 @@ __device__ void runSamples(const RenderParams& P0, Feed& feed, Counters& k, Trav
 
 // ---------------------------------------------------------------------------------------------------------------
 // Shadow rays pooled across the wave. A vertex's shadow rays (at most one per light sample and one per lobe sample)
-// wait in the wave's LDS pool with the radiance each adds when unoccluded; once every lane of the wave has shaded its
-// vertex, the wave traces the pool's rays 64 at a time, each lane taking the next ray whoever made it, and each lane
-// then adds its own rays' radiance in the order it made them. The traces are the same rays as traced where they are
-// made, and the additions the same sums in the same order, so the sample's radiance is bit for bit the same; what
-// changes is that a lane with no ray for one light traces another lane's ray instead of idling (one any-hit loop at
-// full width instead of one per light and lobe, each at the width of the lanes that have a ray for it).
+// wait in the wave's LDS pool with the radiance each adds when unoccluded; the wave's next trace phase (traceWave)
+// traces them together with the lanes' next rays, each lane taking the pool's next ray whoever made it once its own
+// is done, and each lane then adds its own rays' radiance in the order it made them. The traces are the same rays as
+// traced where they are made, and the additions the same sums in the same order, so the sample's radiance is bit for
+// bit the same; what changes is that a lane with no ray traces another lane's instead of idling.
 constexpr uint32_t PoolRays = 6;  // shadow rays a lane's vertex may queue (the host checks the frame's bound)
 struct ShadowPool {  // one wave's, 13,440 B: ray n of lane l at n * 64 + l (bank-conflict free)
     float dx[64 * PoolRays], dy[64 * PoolRays], dz[64 * PoolRays], far[64 * PoolRays];
